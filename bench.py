@@ -1,0 +1,240 @@
+"""Benchmark: Rendered-mode path tracing on one MI355X, with the LBVH beside it.
+
+One step = one 64-spp frame of the 1.31M-triangle spheres scene at 1920x1080
+(max 4 diffuse bounces, Russian roulette after bounce 3, next-event
+estimation, FP32), traced by the HIP path kernel over the device-built LBVH.
+`value` is rays per second (closest-hit + shadow rays, counted on the device).
+The same line carries the LBVH build and refit rates (Mtri/s) on that scene and
+on the 65,536-triangle set, the 256x256 closest-hit raycast rate, the path
+kernel's roofline figures, and the CPU twin (the oracle) timed on the host's
+cores on a bounded sample of the same frame.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--spp 64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import phiray_amd as P  # noqa: E402
+from phiray_amd import scenes as S  # noqa: E402
+
+METRIC = "Mrays/s + LBVH Mtri/s; s/frame to relMSE≤0.01: learned vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of the walk (one ray): the ray in (32), the hit out (8),
+# the root node (32), both children at every inner step (2 x 32), one
+# triangle record per test (48: three corners in 16-byte rows).
+RAY_BYTES, HIT_BYTES, ROOT_BYTES, STEP_BYTES, TEST_BYTES = 32, 8, 32, 64, 48
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def lbvh_rates(gpu: P.Gpu, boxes: np.ndarray, reps: int) -> dict:
+    prims = P.GpuArray.from_(gpu, boxes)
+    b = P.GpuBvh.build_on(gpu, prims, 4)
+    clock = P.Clock(gpu)
+    for _ in range(3):
+        b.rebuild_async()
+        b.refit_async()
+    gpu.wait()
+    clock.start()
+    for _ in range(reps):
+        b.rebuild_async()
+    t_build, _ = clock.stop()
+    clock.start()
+    for _ in range(reps):
+        b.refit_async()
+    t_refit, _ = clock.stop()
+    n = len(boxes)
+    tb, tr = t_build / reps, t_refit / reps
+    nodes = b.download()
+    reach = len(nodes.reachable())
+    # Algorithmic bytes: bounds read 32, Morton read 32 + write 4, sort 4 passes
+    # x (4+4 read, 4+4 write) + histogram read 4, Karras read 4, node topology
+    # (2n-1) x 32, refit per primitive (order 4 + box 32) and per reachable
+    # node (parent 4 + counter 8 + child 32 + node 32).
+    refit_bytes = n * 36 + reach * (4 + 8 + 32 + 32)
+    build_bytes = n * (32 + 36 + 68 + 4) + (2 * n - 1) * 32 + refit_bytes
+    return {
+        "tris": n,
+        "depth": b.depth,
+        "build_ms": tb * 1e3,
+        "build_mtri_s": n / tb / 1e6,
+        "build_gbs": build_bytes / tb / 1e9,
+        "build_hbm_frac": build_bytes / tb / 1e9 / HBM_PEAK_GBS,
+        "refit_ms": tr * 1e3,
+        "refit_mtri_s": n / tr / 1e6,
+        "refit_gbs": refit_bytes / tr / 1e9,
+        "bytes_per_tri_build": build_bytes / n,
+        "bytes_per_tri_refit": refit_bytes / n,
+    }
+
+
+def raycast_rate(gpu: P.Gpu, reps: int) -> dict:
+    corners, _ = S.lbvh_scene(65536, 0)
+    tr = P.Tracer.new(gpu, S.tris_from_corners(corners), 4)
+    rays = S.pinhole_rays(256, 256)
+    drays = P.GpuArray.from_(gpu, rays)
+    hits = P.GpuArray.zeroed(gpu, P.HIT, len(rays))
+    for _ in range(3):
+        tr.raycast_on(drays, hits)
+    gpu.wait()
+    c0 = tr.counters()
+    clock = P.Clock(gpu)
+    clock.start()
+    for _ in range(reps):
+        tr.raycast_on(drays, hits)
+    t, _ = clock.stop()
+    c1 = tr.counters()
+    steps, tests = (int(c1[4] - c0[4]) // reps), (int(c1[5] - c0[5]) // reps)
+    per = t / reps
+    nb = len(rays) * (RAY_BYTES + 20 + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES
+    return {"rays": len(rays), "ms": per * 1e3, "mrays_s": len(rays) / per / 1e6, "gbs": nb / per / 1e9,
+            "steps_per_ray": steps / len(rays), "tests_per_ray": tests / len(rays)}
+
+
+def cpu_baseline(tris: np.ndarray, cam: np.ndarray, w: int, h: int, budget_s: float) -> dict:
+    """The oracle (the CPU twin) on the host's cores, on a band of rows of the
+    same frame, sample after sample until `budget_s` seconds have passed."""
+    from oracle import oracle as O
+
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    sc = O.Scene(tris, 4, P.NODE, cores)
+    t_scene = time.perf_counter() - t0
+    rows = 16
+    first = (h // 2 - rows // 2) * w
+    count = rows * w
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    rays, f = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        c, s = sc.sample(cam, w, h, px, f, cores, first, count)
+        rays += c + s
+        f += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{rows} rows x {w} px of the same 1920x1080 frame (rows {h // 2 - rows // 2}-{h // 2 + rows // 2 - 1}), "
+                      f"{f} samples, {rays} rays in {dt:.1f} s; CPU LBVH build of the scene took {t_scene:.2f} s",
+            "s_per_frame_est": dt / (f * count) * (w * h) * 64}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["all", "path", "lbvh"], default="all")
+    ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is design-only)"}))
+        return 2
+
+    gpu = P.Gpu.new(0)
+    if gpu is None:
+        print(json.dumps({"error": "no HIP device"}))
+        return 1
+    log(f"device {gpu.arch()} ({gpu.cu_count()} CUs)")
+    out: dict = {}
+
+    if args.copy_mb:
+        n = args.copy_mb * (1 << 20) // 16
+        src = P.GpuArray.zeroed(gpu, np.dtype((np.uint32, 4)), n)
+        dst = P.GpuArray.zeroed(gpu, np.dtype((np.uint32, 4)), n)
+        clock = P.Clock(gpu)
+        P.copy_kernel(dst, src)
+        clock.start()
+        P.copy_kernel(dst, src)
+        t, _ = clock.stop()
+        out["copy"] = {"bytes": 2 * 16 * n, "ms": t * 1e3, "gbs": 2 * 16 * n / t / 1e9}
+        log(f"copy kernel {out['copy']}")
+
+    log("generating the spheres scene")
+    tris = S.spheres_scene(spheres=64, level=5, seed=1)
+    w, h, spp = args.width, args.height, args.spp
+    cam = S.path_camera(w, h, rr_from=3)
+
+    if args.only in ("all", "lbvh"):
+        log("LBVH rates")
+        out["lbvh_scene"] = lbvh_rates(gpu, S.of_tri(tris["a"], tris["b"], tris["c"]), 20)
+        corners, _ = S.lbvh_scene(65536, 0)
+        out["lbvh_65536"] = lbvh_rates(gpu, S.corner_boxes(corners), 50)
+        out["raycast_65536"] = raycast_rate(gpu, 50)
+        log(f"lbvh {out['lbvh_scene']['build_mtri_s']:.1f} Mtri/s (scene), {out['lbvh_65536']['build_mtri_s']:.1f} Mtri/s (65536)")
+
+    line = {"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"path trace, 64 icospheres (level 5) + ground + 1 m^2 emissive quad = {len(tris)} triangles, "
+                                   f"{w}x{h}, {spp} spp, max 4 diffuse bounces, Russian roulette after bounce 3, NEE",
+                       "tris": int(len(tris)), "width": w, "height": h, "spp": spp, "bounces": 4}}
+
+    if args.only in ("all", "path"):
+        log("tracer setup")
+        t0 = time.perf_counter()
+        tracer = P.Tracer.new(gpu, tris, 4)
+        log(f"Tracer::new {time.perf_counter() - t0:.2f} s, depth {tracer.depth}, {tracer.glow} emitters")
+        for i in range(args.warmup):
+            tracer.sample(cam, w, h, spp)
+            log(f"warmup frame {i}: {tracer.time()[1]:.3f} s")
+        c0 = tracer.counters()
+        gpu_s = []
+        clock = P.Clock(gpu)
+        clock.start()
+        for i in range(args.steps):
+            tracer.sample(cam, w, h, spp)
+            gpu_s.append(tracer.time()[1])
+            log(f"frame {i}: {gpu_s[-1]:.3f} s")
+        total, _ = clock.stop()
+        c1 = tracer.counters()
+        closest, shadow, steps, tests = (int(c1[k] - c0[k]) for k in range(4))
+        rays = closest + shadow
+        per = total / args.steps
+        line["value"] = rays / total / 1e6
+        line["ms_per_step"] = per * 1e3
+        kernel_s = float(np.mean(gpu_s))
+        nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
+        achieved = nbytes / kernel_s / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path",
+                            "per_ray_bytes": "32 ray + 8 hit + 32 root + 64 x inner steps + 48 x triangle tests",
+                            "launch_ms": kernel_s * 1e3}
+        line["s_per_frame"] = per
+        line["rays_per_frame"] = rays / args.steps
+        line["closest_per_frame"] = closest / args.steps
+        line["shadow_per_frame"] = shadow / args.steps
+        line["steps_per_ray"] = steps / max(rays, 1)
+        line["tests_per_ray"] = tests / max(rays, 1)
+        line["learned_denoiser"] = None
+        if not args.no_cpu:
+            log("CPU twin baseline")
+            line["cpu_baseline"] = cpu_baseline(tris, cam, w, h, args.cpu_seconds)
+    line.update(out)
+    if out.get("lbvh_scene"):
+        line["lbvh_mtri_s"] = out["lbvh_scene"]["build_mtri_s"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,159 @@
+/* phiray_hip.h — the C ABI of the gfx950 LBVH + path-tracer library.
+ *
+ * These are the entry points a thin Rust FFI crate binds to re-back phiray's
+ * `Gpu`, `GpuArray<T>`, `GpuBvh`, `GpuSort`-sorted build, `Tracer` and
+ * `Clock` with hand-written HIP kernels. Each declaration names the reference
+ * interface it replaces (paths relative to the phiray repository).
+ *
+ * Conventions:
+ *  - every function returning `int` returns 0 (PHR_OK) or a negative code;
+ *    phr_last_error() describes the last failure on the calling thread;
+ *  - plain pointers and sizes only; POD records cross by bytes in phiray's
+ *    own layouts: Aabb 32 B, Node 32 B, Link 16 B, Hit 20 B, ray 32 B
+ *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 176 B;
+ *  - one HIP stream per phr_gpu; calls on one phr_gpu are serialised by the
+ *    caller (phiray's `Tracer` is `&mut` from one render thread).
+ */
+#ifndef PHIRAY_HIP_H
+#define PHIRAY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHR_OK 0
+#define PHR_ERR_ARG -1
+#define PHR_ERR_HIP -2
+#define PHR_ERR_UNSUPPORTED -3
+#define PHR_ERR_SIZE -4
+
+typedef struct phr_gpu phr_gpu;
+typedef struct phr_array phr_array;
+typedef struct phr_bvh phr_bvh;
+typedef struct phr_tracer phr_tracer;
+typedef struct phr_clock phr_clock;
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* phr_last_error(void);
+uint32_t phr_abi_version(void);
+
+/* ---- device: Gpu (src/gpu/mod.rs:84-331) ----------------------------------- */
+/* Gpu::new (src/gpu/mod.rs:108); PHR_ERR_UNSUPPORTED = None (no device). */
+int phr_gpu_open(int device, phr_gpu** out);
+void phr_gpu_close(phr_gpu* g);
+/* Gpu::wait (src/gpu/mod.rs:272). */
+int phr_gpu_wait(phr_gpu* g);
+/* Gpu::wait_for (src/gpu/mod.rs:292): *done = 1 when drained within limit. */
+int phr_gpu_wait_for(phr_gpu* g, uint64_t limit_ns, int* done);
+/* Gpu::max_binding (src/gpu/mod.rs:253). */
+uint64_t phr_gpu_max_binding(phr_gpu* g);
+int phr_gpu_cu_count(phr_gpu* g);
+const char* phr_gpu_arch(phr_gpu* g);
+void* phr_gpu_stream(phr_gpu* g);
+
+/* ---- arrays: GpuArray<T> (src/gpu/array.rs:7-188) --------------------------- */
+/* GpuArray::new (src/gpu/array.rs:20-23): capacity, no items. */
+int phr_array_new(phr_gpu* g, uint32_t elem, uint64_t cap, phr_array** out);
+/* GpuArray::zeroed (src/gpu/array.rs:27-29). */
+int phr_array_zeroed(phr_gpu* g, uint32_t elem, uint64_t len, phr_array** out);
+/* GpuArray::from (src/gpu/array.rs:31-35). */
+int phr_array_from(phr_gpu* g, uint32_t elem, const void* items, uint64_t len, phr_array** out);
+/* GpuArray::write (src/gpu/array.rs:47-56): *grown = reallocated. */
+int phr_array_write(phr_array* a, const void* items, uint64_t len, int* grown);
+/* GpuArray::write_at (src/gpu/array.rs:88-107). */
+int phr_array_write_at(phr_array* a, uint64_t at, const void* items, uint64_t len, int* grown);
+/* GpuArray::read_range (src/gpu/array.rs:136-187), blocking. */
+int phr_array_read_range(phr_array* a, uint64_t from, uint64_t count, void* out);
+uint64_t phr_array_len(const phr_array* a);
+uint64_t phr_array_cap(const phr_array* a);
+uint32_t phr_array_elem(const phr_array* a);
+void* phr_array_device_ptr(phr_array* a);
+void phr_array_free(phr_array* a);
+/* A plain streaming copy kernel (the HBM-counter calibration reference; no
+ * counterpart in phiray). */
+int phr_array_copy_kernel(phr_array* dst, phr_array* src);
+
+/* ---- timing: Clock (src/gpu/clock.rs:17-87) --------------------------------- */
+/* Clock::new (src/gpu/clock.rs:41). */
+int phr_clock_new(phr_gpu* g, phr_clock** out);
+/* Clock::run split in two (src/gpu/clock.rs:54-72): stamp, enqueue work, then
+ * stop = stamp, wait, and report the GPU span and the wall time. */
+int phr_clock_start(phr_clock* c);
+int phr_clock_stop(phr_clock* c, double* gpu_s, double* wall_s);
+void phr_clock_free(phr_clock* c);
+
+/* ---- BVH: GpuBvh (src/bvh/gpu.rs:11-104) ------------------------------------ */
+/* GpuBvh::build (src/bvh/gpu.rs:23-75): prims are n Aabb (32 B) on the host.
+ * The whole build (bounds, Morton, sort, Karras, packing, refit) runs on the
+ * device; only the depth comes back. */
+int phr_bvh_build(phr_gpu* g, const void* prims, uint32_t n, uint32_t leaf, phr_bvh** out);
+/* The same over an Aabb array already on the device (borrowed, not owned). */
+int phr_bvh_build_on(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out);
+/* Build again from the current prims (same count), enqueued / then synced. */
+int phr_bvh_rebuild_async(phr_bvh* b);
+int phr_bvh_rebuild(phr_bvh* b);
+/* GpuBvh::refit (src/bvh/gpu.rs:88-90): bounds again, topology kept. */
+int phr_bvh_refit_async(phr_bvh* b);
+int phr_bvh_refit(phr_bvh* b);
+/* GpuBvh::update (src/bvh/gpu.rs:93-99): *refitted = 0 when n changed (rebuild). */
+int phr_bvh_update(phr_bvh* b, const void* prims, uint32_t n, int* refitted);
+/* GpuBvh::download (src/bvh/gpu.rs:102-104): nodes[2n-1], order[n], depth. */
+int phr_bvh_download(phr_bvh* b, void* nodes, uint32_t* order, uint32_t* depth);
+/* GpuSort::read's codes half (src/bvh/sort.rs:109-111): sorted Morton codes. */
+int phr_bvh_codes(phr_bvh* b, uint32_t* sorted_codes);
+uint32_t phr_bvh_len(const phr_bvh* b);
+uint32_t phr_bvh_depth(const phr_bvh* b);
+uint32_t phr_bvh_leaf(const phr_bvh* b);
+/* GpuBvh's public fields prims / order / nodes (src/bvh/gpu.rs:12-15). */
+phr_array* phr_bvh_nodes(phr_bvh* b);
+phr_array* phr_bvh_order(phr_bvh* b);
+phr_array* phr_bvh_prims(phr_bvh* b);
+void phr_bvh_free(phr_bvh* b);
+
+/* ---- tracer: Tracer (src/trace/mod.rs:50-245) -------------------------------- */
+/* Tracer::new (src/trace/build.rs:11-13) over n triangle records (128 B each,
+ * Tri's rows a..nc, color, emit); the tree is the device-built LBVH with
+ * `leaf` primitives a leaf. PHR_ERR_ARG for an empty scene (None). */
+int phr_tracer_new(phr_gpu* g, const void* tris, uint32_t n, uint32_t leaf, phr_tracer** out);
+void phr_tracer_free(phr_tracer* t);
+uint32_t phr_tracer_len(const phr_tracer* t);
+uint32_t phr_tracer_depth(const phr_tracer* t);
+float phr_tracer_extent(const phr_tracer* t);
+uint32_t phr_tracer_glow(const phr_tracer* t);
+phr_bvh* phr_tracer_bvh(phr_tracer* t);
+/* Slot -> input triangle (Scene::reorder, src/trace/scene.rs:188-190). */
+int phr_tracer_order(phr_tracer* t, uint32_t* out);
+/* The emitter table (src/trace/lamp.rs:51-97): [count, faces, insts, cdf]. */
+int phr_tracer_lights(phr_tracer* t, uint32_t* out, uint32_t cap, uint32_t* len);
+/* trace() over host rays (src/trace/wgsl/bvh.wgsl:194-232): hits are 20 B
+ * {t, slot, u, v, inst}; slot = 0xffffffff for a miss. */
+int phr_tracer_raycast(phr_tracer* t, const void* rays, uint32_t nrays, int any_hit, void* hits);
+/* The same over device arrays (rays: 32 B records; hits: 20 B), enqueued. */
+int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array* hits);
+/* Tracer::set_samples (src/trace/mod.rs:242). */
+void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
+/* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
+ * cam is the 176-byte camera record. */
+int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done);
+/* Tracer::time (src/trace/sample.rs:253-255). */
+int phr_tracer_time(phr_tracer* t, double* submit_s, double* gpu_s);
+/* Rays traced so far: closest-hit and shadow (any-hit). */
+int phr_tracer_rays(phr_tracer* t, uint64_t* closest, uint64_t* shadow);
+/* Walk counters: [closest rays, shadow rays, inner-node steps, triangle tests]
+ * of the path kernel, then [steps, tests] of the raycasts. */
+int phr_tracer_counters(phr_tracer* t, uint64_t* out6);
+/* The image buffer (src/trace/read.rs:37-56), raw words. */
+int phr_tracer_px(phr_tracer* t, uint64_t from, uint64_t count, uint32_t* out);
+phr_array* phr_tracer_image(phr_tracer* t);
+/* Tracer::denoise (keep = 0) / denoise_shown (keep = 1), src/trace/read.rs:73-119:
+ * the four a-trous passes of src/kernel/denoise.wgsl over the image buffer. */
+int phr_tracer_denoise(phr_tracer* t, int keep);
+/* Tracer::radiance (src/trace/read.rs:241-251): w*h rgb floats. */
+int phr_tracer_radiance(phr_tracer* t, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,197 @@
+"""ctypes loader for the CPU oracle (liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, the smoke check and the benchmark's CPU-baseline leg as the
+checker; never by the product library. `build()` compiles it with the
+Makefile beside this file.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I = C.c_int
+_SIGS = {
+    "orc_code": (_U32, [_P]),
+    "orc_morton": (C.c_uint64, [_P, _U32]),
+    "orc_of_tri": (None, [_P, _P, _P, _P]),
+    "orc_scene": (None, [_P, _U32, _P]),
+    "orc_codes": (None, [_P, _U32, _P, _I]),
+    "orc_sort_codes": (None, [_P, _U32, _P, _P, _I]),
+    "orc_radix_model": (None, [_P, _U32, _P, _P]),
+    "orc_delta": (C.c_int32, [_P, _U32, C.c_int64, C.c_int64]),
+    "orc_links": (None, [_P, _U32, _P, _I]),
+    "orc_assemble": (_U32, [_P, _U32, _U32, _P]),
+    "orc_refit": (None, [_P, _U32, _P, _P]),
+    "orc_build": (_U32, [_P, _U32, _U32, _P, _P, _I]),
+    "orc_build_compact": (_U32, [_P, _U32, _U32, _P, _P, _P]),
+    "orc_compact": (_U32, [_P, _U32]),
+    "orc_reachable": (_U32, [_P, _U32, _P]),
+    "orc_ray_space": (None, [_P, _P]),
+    "orc_tri_t": (_I, [_P, _P, _P, _P, _P, _P]),
+    "orc_moller_trumbore": (_I, [_P, _P, _P, _P, _P, _P]),
+    "orc_safe_inv": (None, [_P, _P]),
+    "orc_box_t": (C.c_float, [_P, _P, _P, C.c_float]),
+    "orc_raycast": (None, [_P, _P, _P, _U32, _I, _P, _P, _I]),
+    "orc_raycast_mat": (None, [_P, _P, _P, _P, _P, _U32, _I, _P, _P, _I]),
+    "orc_sin": (C.c_float, [C.c_float]),
+    "orc_cos": (C.c_float, [C.c_float]),
+    "orc_exp2": (C.c_float, [C.c_float]),
+    "orc_hash": (_U32, [_U32]),
+    "orc_rnd": (C.c_float, [C.POINTER(_U32)]),
+    "orc_scene_build": (_U32, [_P, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]),
+    "orc_stamp_camera": (None, [_P, _U32, _U32, C.c_float, _U32, _U32]),
+    "orc_sample": (None, [_P, _P, _U32, _U32, _P, _P, _I]),
+    "orc_hardware_threads": (_I, []),
+    "orc_log2": (C.c_float, [C.c_float]),
+    "orc_denoise": (None, [_P, _U32, _U32, _U32]),
+    "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P]),
+}
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        h = C.CDLL(str(LIB))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def threads() -> int:
+    return int(lib().orc_hardware_threads())
+
+
+# ---- LBVH -------------------------------------------------------------------
+def codes(prims: np.ndarray, nthreads: int = 1) -> np.ndarray:
+    prims = np.ascontiguousarray(prims)
+    out = np.zeros(len(prims), dtype=np.uint32)
+    lib().orc_codes(ptr(prims), len(prims), ptr(out), nthreads)
+    return out
+
+
+def sort_codes(c: np.ndarray, nthreads: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    c = np.ascontiguousarray(c, dtype=np.uint32)
+    order = np.zeros(len(c), dtype=np.uint32)
+    srt = np.zeros(len(c), dtype=np.uint32)
+    lib().orc_sort_codes(ptr(c), len(c), ptr(order), ptr(srt), nthreads)
+    return order, srt
+
+
+def radix_model(c: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    c = np.ascontiguousarray(c, dtype=np.uint32)
+    order = np.zeros(len(c), dtype=np.uint32)
+    srt = np.zeros(len(c), dtype=np.uint32)
+    lib().orc_radix_model(ptr(c), len(c), ptr(order), ptr(srt))
+    return order, srt
+
+
+def links(sorted_codes: np.ndarray, link_dtype, nthreads: int = 1) -> np.ndarray:
+    s = np.ascontiguousarray(sorted_codes, dtype=np.uint32)
+    out = np.zeros(max(len(s) - 1, 0), dtype=link_dtype)
+    lib().orc_links(ptr(s), len(s), ptr(out), nthreads)
+    return out
+
+
+def build_bvh(prims: np.ndarray, leaf: int, node_dtype, nthreads: int = 1):
+    prims = np.ascontiguousarray(prims)
+    n = len(prims)
+    nodes = np.zeros(max(2 * n - 1, 0), dtype=node_dtype)
+    order = np.zeros(n, dtype=np.uint32)
+    depth = lib().orc_build(ptr(prims), n, leaf, ptr(nodes), ptr(order), nthreads)
+    return nodes, order, int(depth)
+
+
+def refit(nodes: np.ndarray, order: np.ndarray, prims: np.ndarray) -> np.ndarray:
+    nodes = nodes.copy()
+    prims = np.ascontiguousarray(prims)
+    lib().orc_refit(ptr(nodes), len(nodes), ptr(np.ascontiguousarray(order, dtype=np.uint32)), ptr(prims))
+    return nodes
+
+
+def reachable(nodes: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(nodes), dtype=np.uint32)
+    m = lib().orc_reachable(ptr(nodes), len(nodes), ptr(out))
+    return out[:m]
+
+
+# ---- walk ---------------------------------------------------------------------
+def raycast(nodes: np.ndarray, tris9: np.ndarray, rays: np.ndarray, hit_dtype, any_hit: bool = False,
+            nthreads: int = 1, stats: bool = False, faces: np.ndarray | None = None, mats: np.ndarray | None = None):
+    rays = np.ascontiguousarray(rays)
+    hits = np.zeros(len(rays), dtype=hit_dtype)
+    st = np.zeros((len(rays), 2), dtype=np.uint32) if stats else None
+    lib().orc_raycast_mat(ptr(nodes), ptr(np.ascontiguousarray(tris9, dtype=np.float32)),
+                          ptr(faces) if faces is not None else None, ptr(mats) if mats is not None else None,
+                          ptr(rays), len(rays), 1 if any_hit else 0, ptr(hits), ptr(st) if stats else None, nthreads)
+    return (hits, st) if stats else hits
+
+
+# ---- tracer -------------------------------------------------------------------
+class OrcScene(C.Structure):
+    _fields_ = [("nodes", _P), ("node_count", _U32), ("tris", _P), ("faces", _P), ("verts", _P),
+                ("mats", _P), ("lights", _P)]
+
+
+class Scene:
+    """The oracle's tracer scene (all arrays kept alive here)."""
+
+    def __init__(self, tris: np.ndarray, leaf: int, node_dtype, nthreads: int = 1):
+        tris = np.ascontiguousarray(tris)
+        n = len(tris)
+        self.n = n
+        self.nodes = np.zeros(max(2 * n - 1, 1), dtype=node_dtype)
+        self.order = np.zeros(n, dtype=np.uint32)
+        self.tri9 = np.zeros(9 * n, dtype=np.float32)
+        self.faces = np.zeros(4 * n, dtype=np.uint32)
+        self.verts = np.zeros(8 * 3 * n, dtype=np.float32)
+        self.mats = np.zeros(16 * n, dtype=np.float32)
+        self.lights = np.zeros(1 + 3 * n, dtype=np.uint32)
+        nv, nm, depth = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        ext = C.c_float(0)
+        lib().orc_scene_build(ptr(tris), n, leaf, ptr(self.nodes), ptr(self.order), ptr(self.tri9), ptr(self.faces),
+                              ptr(self.verts), C.byref(nv), ptr(self.mats), C.byref(nm), ptr(self.lights),
+                              C.byref(depth), C.byref(ext), nthreads)
+        self.nverts, self.nmats, self.depth, self.extent = nv.value, nm.value, depth.value, ext.value
+        m = int(self.lights[0])
+        self.lights = self.lights[: 1 + 3 * m].copy()
+        self.glow = m
+        self._c = OrcScene(ptr(self.nodes), len(self.nodes), ptr(self.tri9), ptr(self.faces), ptr(self.verts),
+                           ptr(self.mats), ptr(self.lights))
+
+    def stamp(self, cam: np.ndarray, w: int, h: int, frame: int) -> np.ndarray:
+        c = np.ascontiguousarray(np.array(cam).reshape(1).copy())
+        lib().orc_stamp_camera(ptr(c), w, h, C.c_float(self.extent), self.glow, frame)
+        return c
+
+    def sample(self, cam: np.ndarray, w: int, h: int, px: np.ndarray, frame: int, nthreads: int = 1,
+               first: int = 0, count: int | None = None) -> tuple[int, int]:
+        c = self.stamp(cam, w, h, frame)
+        rays = np.zeros(2, dtype=np.uint64)
+        n = w * h
+        count = n - first if count is None else count
+        lib().orc_sample(C.byref(self._c), ptr(c), first, count, ptr(px), ptr(rays), nthreads)
+        return int(rays[0]), int(rays[1])

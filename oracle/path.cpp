@@ -1,0 +1,589 @@
+// CPU oracle: the Rendered-mode integrator and the scene it reads.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h). phiray has no CPU twin of its
+// path tracer; this restates the WGSL for the feature subset the benchmark
+// scenes use, operation for operation, with the same random-number draws:
+//   src/trace/wgsl/util.wgsl:1-10        hash, rnd (PCG-RXS-M-XS)
+//   src/trace/wgsl/util.wgsl:17-34       guide
+//   src/trace/wgsl/util.wgsl:101-145     emit (linear sums, counts; the display
+//                                        transform is not restated)
+//   src/trace/wgsl/util.wgsl:162-179     pixel, frame_size, frame_point
+//   src/trace/wgsl/shade.wgsl:41-373     main (camera ray, the camera path)
+//   src/trace/wgsl/integrate.wgsl:6-23   cosine_dir, glow_mis
+//   src/trace/wgsl/integrate.wgsl:66-85  direct
+//   src/trace/wgsl/integrate.wgsl:111-230 indirect (+ Russian roulette start knob)
+//   src/trace/wgsl/light.wgsl:1-5,65-67,87-106,118-167(lamp pick only),174-198,264-312
+//   src/trace/wgsl/brdf.wgsl:9-12,61-63,73-82,109-155,196-214,239-269
+//   src/trace/wgsl/procedural.wgsl:73-81 fresnel_dielectric_cos
+//   src/trace/wgsl/layout.wgsl:79-101,114-158  unoct, world_ng, tri_of
+//   src/trace/lamp.rs:51-97              the emitter table
+//   src/trace/scene.rs:22-38, scene/add.rs:9-69  vertex packing and sharing
+// Restated subset: opaque surfaces (trans = 0), alpha cut-outs, no textures,
+// procedurals, mix closures, subsurface, sheen or coat (all zero weight), one
+// identity instance, a flat world colour (no environment image), lamps only as
+// unlit placeholders (`Camera::count` with zero colour), the glossy lobe off
+// (Max Bounces glossy 0), Pixel Filter 0 (phiray's own one-pixel box), no
+// adaptive sampling, no clamping, no grid overlay, a black sky.
+// `pow(x, 5)` is x^5 by multiplication and 0 for x < 0 (WGSL's pow is
+// undefined there); sin/cos/exp2 are the deterministic ones in walk.cpp.
+#include "oracle.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <vector>
+#include <array>
+#include <atomic>
+
+namespace {
+
+const float PI = 3.14159265f;   // layout.wgsl:28
+const float INF = 3.0e38f;      // layout.wgsl:39
+
+struct V3 { float x, y, z; };
+inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalize(V3 a) { return a / length(a); }
+inline float fmin_(float a, float b) { return b < a ? b : a; }
+inline float fmax_(float a, float b) { return b > a ? b : a; }
+inline float clampf(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
+inline V3 vmax(V3 a, float s) { return {fmax_(a.x, s), fmax_(a.y, s), fmax_(a.z, s)}; }
+inline float maxc(V3 a) { return fmax_(a.x, fmax_(a.y, a.z)); }
+inline float pw5(float x) { if (!(x > 0.f)) return 0.f; float x2 = x * x; return (x2 * x2) * x; }
+inline float wgsl_sign(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+inline V3 mixv(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
+inline V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+inline float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline uint32_t f_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+struct Hit { float t; uint32_t tri; float u, v; uint32_t inst; };
+
+struct Tri {  // layout.wgsl:2-22, the fields the restated path reads
+    V3 a, b, c, na, nb, nc;
+    float color[4];
+    float rough, metal, trans, ior;
+    V3 emit;
+};
+
+struct Surf {  // brdf.wgsl:46-58
+    V3 pos, n, v, albedo, f0;
+    float rough;
+    float bleed;
+    float eta, dielectric;
+};
+
+struct Ctx {
+    const orc_scene_t* s;
+    const orc_camera* cam;
+    uint64_t closest = 0, shadow = 0;
+};
+
+// ---- walk (bvh.wgsl:194-232 over one identity instance) ----
+Hit trace(Ctx& c, V3 o, V3 d, float tmax, bool first) {
+    orc_ray r;
+    r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z; r.tmax = tmax;
+    r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r._pad = 0.f;
+    orc_hit h;
+    orc_raycast_mat(c.s->nodes, c.s->tris, c.s->faces, c.s->mats, &r, 1, first ? 1 : 0, &h, nullptr, 1);
+    if (first) c.shadow++; else c.closest++;
+    return Hit{h.t, h.tri, h.u, h.v, h.inst};
+}
+
+float visible(Ctx& c, V3 org, V3 l, float dist) {  // light.wgsl:65-67
+    return trace(c, org, l, dist, true).tri != ORC_NONE ? 0.0f : 1.0f;
+}
+
+// ---- random numbers (util.wgsl:1-10) ----
+inline uint32_t hash_u(uint32_t x) {
+    uint32_t h = x * 747796405u + 2891336453u;
+    h = ((h >> ((h >> 28u) + 4u)) ^ h) * 277803737u;
+    return (h >> 22u) ^ h;
+}
+inline float rnd(uint32_t* seed) {
+    *seed = hash_u(*seed);
+    return (float)(*seed) / 4294967295.0f;
+}
+
+// ---- scene reads ----
+V3 unoct(float ex, float ey) {  // layout.wgsl:79-86
+    if (ex > 1.5f) return v3(0, 0, 0);
+    V3 n = v3(ex, ey, 1.0f - std::fabs(ex) - std::fabs(ey));
+    if (n.z < 0.f) n = v3((1.0f - std::fabs(ey)) * wgsl_sign(ex), (1.0f - std::fabs(ex)) * wgsl_sign(ey), n.z);
+    return normalize(n);
+}
+
+Tri tri_of(const orc_scene_t* s, uint32_t f, uint32_t m) {  // layout.wgsl:114-158, identity instance
+    const uint32_t* fc = s->faces + 4 * (size_t)f;
+    const float* x = s->verts + 8 * (size_t)fc[0];
+    const float* y = s->verts + 8 * (size_t)fc[1];
+    const float* z = s->verts + 8 * (size_t)fc[2];
+    const float* mt = s->mats + ORC_MAT_WORDS * (size_t)m;
+    Tri t;
+    t.a = ld3(x); t.b = ld3(y); t.c = ld3(z);
+    t.na = unoct(x[4], x[5]); t.nb = unoct(y[4], y[5]); t.nc = unoct(z[4], z[5]);
+    for (int k = 0; k < 4; ++k) t.color[k] = mt[k];
+    t.rough = mt[4]; t.metal = mt[5]; t.trans = mt[6]; t.ior = mt[7];
+    t.emit = ld3(mt + 8);
+    return t;
+}
+inline uint32_t mat_at(const orc_scene_t* s, uint32_t f) { return s->faces[4 * (size_t)f + 3]; }
+
+// ---- BRDF (brdf.wgsl) ----
+float ggx_lambda(float alpha2, float cos_n) {  // brdf.wgsl:9-12
+    float t = alpha2 * fmax_(1.0f / fmax_(cos_n * cos_n, 1e-8f) - 1.0f, 0.0f);
+    return 0.5f * (std::sqrt(1.0f + t) - 1.0f);
+}
+
+float fresnel_dielectric_cos(float cosi, float eta) {  // procedural.wgsl:73-81
+    float c = std::fabs(cosi);
+    float g = eta * eta - 1.0f + c * c;
+    if (g <= 0.f) return 1.0f;
+    g = std::sqrt(g);
+    float a = (g - c) / (g + c);
+    float b = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+    return 0.5f * a * a * (1.0f + b * b);
+}
+
+V3 fresnel_of(const Surf& s, float cs) {  // brdf.wgsl:73-82
+    float schlick = pw5(1.0f - cs);
+    V3 f = s.f0 + (v3(1, 1, 1) - s.f0) * schlick;
+    if (s.eta > 1.0f && s.dielectric > 0.f) {
+        float r = (s.eta - 1.0f) / (s.eta + 1.0f);
+        float face_on = r * r;
+        float add = s.dielectric * (fresnel_dielectric_cos(cs, s.eta) - face_on - (1.0f - face_on) * schlick);
+        f = f + v3(add, add, add);
+    }
+    return vmax(f, 0.f);
+}
+
+// brdf.wgsl:109-116 — returns the (x, y) = ab pair; env_brdf(f0) = f0 * ab.x + ab.y.
+void env_ab(float rough, float nv, float* abx, float* aby) {
+    const float c0[4] = {-1.0f, -0.0275f, -0.572f, 0.022f};
+    const float c1[4] = {1.0f, 0.0425f, 1.04f, -0.04f};
+    float r[4];
+    for (int k = 0; k < 4; ++k) r[k] = rough * c0[k] + c1[k];
+    float a004 = fmin_(r[0] * r[0], orc_exp2(-9.28f * nv)) * r[0] + r[1];
+    *abx = -1.04f * a004 + r[2];
+    *aby = 1.04f * a004 + r[3];
+}
+inline V3 env_brdf(V3 f0, float rough, float nv) {
+    float ax, ay;
+    env_ab(rough, nv, &ax, &ay);
+    return f0 * ax + v3(ay, ay, ay);
+}
+
+inline V3 dielectric_f0(float ior) {  // brdf.wgsl:123-126
+    float r = (ior - 1.0f) / (ior + 1.0f);
+    return v3(r * r, r * r, r * r);
+}
+
+float under_layer(V3 f0, float rough, float nv) {  // brdf.wgsl:135-138
+    V3 e = env_brdf(f0, rough, nv);
+    return clampf(1.0f - fmax_(e.x, fmax_(e.y, e.z)), 0.0f, 1.0f);
+}
+
+float under_layer_real(float eta, float rough, float nv) {  // brdf.wgsl:146-155
+    float r = (eta - 1.0f) / (eta + 1.0f);
+    float face_on = r * r;
+    float climb = env_brdf(v3(0, 0, 0), rough, nv).x;
+    float flat = env_brdf(v3(1, 1, 1), rough, nv).x - climb;
+    float schlick = (1.0f - face_on) * pw5(1.0f - nv);
+    float real = fresnel_dielectric_cos(nv, eta) - face_on;
+    float scale = clampf(real / fmax_(schlick, 1e-5f), 0.0f, 1.0f);
+    return clampf(1.0f - (face_on * flat + climb * scale), 0.0f, 1.0f);
+}
+
+float glossy_floor(const orc_camera* p, float min_pdf) {  // brdf.wgsl:202-208
+    if (p->integrator[0] < 0.5f) return 0.3f;
+    if (p->integrator[2] <= 0.0f) return 0.0f;
+    float blur_pdf = min_pdf / p->integrator[2];
+    if (blur_pdf >= 1.0f) return 0.0f;
+    return std::sqrt(std::sqrt(1.0f - blur_pdf) * 0.5f);
+}
+inline bool caustic_reflective(const orc_camera* p) {  // brdf.wgsl:213
+    return p->integrator[0] < 0.5f || (((uint32_t)(p->integrator[3] + 0.5f)) & 1u) != 0u;
+}
+
+// brdf.wgsl:239-269 with bleed, sheen and coat at zero weight, where the
+// terms they add are exact zeros and the factors exact ones.
+V3 brdf_w(const Surf& s, V3 l, V3 e, float wd, float ws) {
+    float raw = dot(s.n, l);
+    float nl = fmax_(raw, 0.0f);
+    if (nl <= 0.f) return v3(0, 0, 0);
+    V3 hv = normalize(l + s.v);
+    float nh = fmax_(dot(s.n, hv), 0.0f);
+    float hvv = fmax_(dot(hv, s.v), 0.0f);
+    float a2 = fmax_(s.rough * s.rough * s.rough * s.rough, 1e-4f);
+    float dd = nh * nh * (a2 - 1.0f) + 1.0f;
+    float dist = a2 / (PI * dd * dd);
+    V3 fres = fresnel_of(s, hvv);
+    float nv = fmax_(dot(s.n, s.v), 1e-4f);
+    float g2 = 1.0f / (1.0f + ggx_lambda(a2, nv) + ggx_lambda(a2, fmax_(raw, 1e-4f)));
+    V3 spec = fres * dist * g2 * 0.25f / nv;
+    V3 diffuse = s.albedo / PI;
+    return (diffuse * wd + spec * ws) * e * nl;
+}
+inline V3 brdf(const Surf& s, V3 l, V3 e) { return brdf_w(s, l, e, 1.0f, 1.0f); }
+
+Surf plain_surf(V3 pos, V3 n, V3 v, V3 albedo, V3 f0, float rough) {  // brdf.wgsl:61-63
+    return Surf{pos, n, v, albedo, f0, rough, 0.0f, 0.0f, 0.0f};
+}
+
+// ---- lights (light.wgsl) ----
+V3 onb(V3 n, float a, float b) {  // light.wgsl:1-5
+    V3 tx = normalize(std::fabs(n.z) > 0.9f ? cross(n, v3(1, 0, 0)) : cross(n, v3(0, 0, 1)));
+    V3 ty = cross(n, tx);
+    return tx * a + ty * b;
+}
+V3 cone_dir(V3 axis, float ang, uint32_t* seed) {  // light.wgsl:34-39
+    if (ang <= 0.f) return axis;
+    float r = ang * std::sqrt(rnd(seed));
+    float ph = 2.0f * PI * rnd(seed);
+    return normalize(axis + onb(axis, r * orc_cos(ph), r * orc_sin(ph)));
+}
+inline V3 env_at(const orc_camera* p, V3) { return v3(p->env[0], p->env[1], p->env[2]); }  // light.wgsl:48-51
+
+struct Shadowed { V3 c, dir; float dist; };
+inline Shadowed unshadowed() { return Shadowed{v3(0, 0, 0), v3(0, 0, 1), 0.0f}; }
+
+inline bool world_lit(const orc_camera* p) {  // light.wgsl:87-90, no environment image
+    V3 e = v3(p->env[0], p->env[1], p->env[2]);
+    return dot(e, e) > 0.f;
+}
+inline bool uses_rig(const orc_camera* p) { return p->count == 0u && !world_lit(p); }
+
+Shadowed rig_query(const orc_camera* p, uint32_t k, const Surf& s, uint32_t* seed) {  // light.wgsl:94-101
+    if (k == 0u) {
+        V3 key = cone_dir(ld3(p->sun), 0.06f, seed);
+        return Shadowed{brdf(s, key, v3(2, 2, 2)), key, INF};
+    }
+    V3 rim = normalize(ld3(p->up) * 0.6f - ld3(p->back) * 0.8f + ld3(p->right) * 0.2f);
+    return Shadowed{brdf(s, rim, v3(0.9f, 0.9f, 0.9f)), rim, INF};
+}
+V3 rig_fill(const orc_camera* p, const Surf& s) {  // light.wgsl:103-106
+    V3 fill = normalize(-ld3(p->right) * 0.8f + ld3(p->up) * 0.2f + ld3(p->back) * 0.6f);
+    return brdf(s, fill, v3(0.45f, 0.45f, 0.45f));
+}
+
+// light.wgsl:118-140 with every lamp unlit: the pick draws its one number
+// and finds no light.
+Shadowed lamp_query(uint32_t* seed) {
+    (void)rnd(seed);
+    return unshadowed();
+}
+
+float lamp_chance(const orc_scene_t* s, const Hit& h) {  // light.wgsl:264-278
+    const uint32_t* d = s->lights;
+    uint32_t count = d[0];
+    if (count == 0u) return 0.0f;
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2u;
+        uint32_t f = d[1 + mid], i = d[1 + count + mid];
+        if (f < h.tri || (f == h.tri && i < h.inst)) lo = mid + 1; else hi = mid;
+    }
+    if (d[1 + lo] != h.tri || d[1 + count + lo] != h.inst) return 0.0f;
+    float prev = lo > 0 ? bits_f(d[2 * count + lo]) : 0.0f;
+    return bits_f(d[1 + 2 * count + lo]) - prev;
+}
+
+Shadowed glow_query(const orc_scene_t* sc, const Surf& s, uint32_t* seed) {  // light.wgsl:281-312
+    const uint32_t* d = sc->lights;
+    uint32_t count = d[0];
+    if (count == 0u) return unshadowed();
+    float u = rnd(seed);
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2u;
+        if (bits_f(d[1 + 2 * count + mid]) < u) lo = mid + 1; else hi = mid;
+    }
+    float prev = lo > 0 ? bits_f(d[2 * count + lo]) : 0.0f;
+    float chance = fmax_(bits_f(d[1 + 2 * count + lo]) - prev, 1e-8f);
+    uint32_t f = d[1 + lo];
+    Tri t = tri_of(sc, f, mat_at(sc, f));
+    float a = rnd(seed);
+    float b = rnd(seed);
+    if (a + b > 1.0f) { a = 1.0f - a; b = 1.0f - b; }
+    V3 q = t.a + (t.b - t.a) * a + (t.c - t.a) * b;
+    V3 cr = cross(t.b - t.a, t.c - t.a);
+    float area = 0.5f * length(cr);
+    V3 to = q - s.pos;
+    float d2 = fmax_(dot(to, to), 1e-6f);
+    V3 l = to / std::sqrt(d2);
+    float cos_l = std::fabs(dot(normalize(cr), l));
+    V3 e = t.emit * cos_l * area / (chance * d2);
+    float pl = chance * d2 / fmax_(area * cos_l, 1e-8f);
+    float pb = fmax_(dot(s.n, l), 0.0f) / PI;
+    float w = pl * pl / fmax_(pl * pl + pb * pb, 1e-12f);
+    return Shadowed{brdf(s, l, e) * w, l, std::sqrt(d2) * 0.999f};
+}
+
+// integrate.wgsl:66-85 (no environment image: the world query is empty).
+V3 direct(Ctx& c, const Surf& s, uint32_t* seed) {
+    const orc_camera* p = c.cam;
+    bool rig = uses_rig(p);
+    V3 sum = rig ? rig_fill(p, s) : v3(0, 0, 0);
+    for (uint32_t k = 0; k < 4u; ++k) {
+        Shadowed q = unshadowed();
+        if (k == 0u) q = rig ? rig_query(p, 0u, s, seed) : lamp_query(seed);
+        else if (k == 1u) { if (rig) q = rig_query(p, 1u, s, seed); }
+        else if (k == 2u) q = glow_query(c.s, s, seed);
+        if (maxc(q.c) > 0.f) sum = sum + q.c * visible(c, s.pos, q.dir, q.dist);
+    }
+    return sum;
+}
+
+// ---- integrate.wgsl ----
+V3 cosine_dir(V3 n, uint32_t* seed) {  // integrate.wgsl:6-11
+    float r1 = rnd(seed);
+    float rr = std::sqrt(r1);
+    float ph = 2.0f * PI * rnd(seed);
+    return normalize(onb(n, rr * orc_cos(ph), rr * orc_sin(ph)) + n * std::sqrt(fmax_(0.0f, 1.0f - r1)));
+}
+
+V3 glow_mis(const orc_scene_t* s, const Tri& t, const Hit& h, V3 from_n, V3 dir) {  // integrate.wgsl:15-23
+    V3 e = t.emit;
+    float chance = lamp_chance(s, h);
+    if (chance <= 0.f) return e;
+    V3 cr = cross(t.b - t.a, t.c - t.a);
+    float pl = chance * h.t * h.t / fmax_(0.5f * length(cr) * std::fabs(dot(normalize(cr), dir)), 1e-8f);
+    float pb = fmax_(dot(from_n, dir), 0.0f) / PI;
+    return e * pb * pb / fmax_(pb * pb + pl * pl, 1e-12f);
+}
+
+V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // integrate.wgsl:111-230
+    const orc_camera* p = c.cam;
+    bool lit = world_lit(p);
+    if (p->glow == 0u && !lit) {
+        V3 dir = cosine_dir(n, seed);
+        return env_at(p, dir) * mixf(0.25f, 1.0f, visible(c, pos, dir, p->ao));
+    }
+    uint32_t depth = bounce ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
+    V3 acc = v3(0, 0, 0), thr = v3(1, 1, 1), org = pos, nrm = n;
+    float min_pdf = 3.0e38f;
+    V3 dir = v3(0, 0, 1);
+    float pb = 1.0f;
+    for (uint32_t b = 0; b < depth; ++b) {
+        dir = cosine_dir(nrm, seed);
+        pb = fmax_(dot(nrm, dir), 0.0f) / PI;
+        min_pdf = fmin_(min_pdf, pb);
+        Hit h = trace(c, org, dir, INF, false);
+        if (h.tri == ORC_NONE) {
+            acc = acc + thr * env_at(p, dir) * 1.0f;
+            break;
+        }
+        if (!bounce) break;
+        Tri t = tri_of(c.s, h.tri, mat_at(c.s, h.tri));
+        V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        V3 ng = dot(ng0, dir) < 0.f ? ng0 : -ng0;
+        V3 hit = org + dir * h.t + ng * (p->eps + h.t * 1e-5f);
+        V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+        float rough = fmax_(t.rough, glossy_floor(p, min_pdf));
+        bool glossy = caustic_reflective(p);
+        V3 fd = glossy ? dielectric_f0(t.ior) : v3(0, 0, 0);
+        float metal = glossy ? t.metal : 0.0f;
+        float tw = t.trans * (1.0f - t.metal);
+        V3 albedo = base * (1.0f - t.metal) * (1.0f - tw);
+        if (maxc(albedo) > 0.f) albedo = albedo * under_layer(fd, rough, fmax_(dot(ng, -dir), 1e-4f));
+        acc = acc + thr * glow_mis(c.s, t, h, nrm, dir);
+        float near = (!lit && h.t <= p->ao) ? 0.25f : 1.0f;
+        Surf s = plain_surf(hit, ng, -dir, albedo, mixv(fd, base, metal), rough);
+        acc = acc + thr * direct(c, s, seed) * near;
+        thr = thr * albedo;
+        if (b + 1u < depth && b + 1u >= p->rr_from) {
+            float keep = clampf(maxc(thr), 0.0f, 0.95f);
+            if (keep < 1e-4f || rnd(seed) > keep) break;
+            thr = thr / keep;
+        }
+        org = hit;
+        nrm = ng;
+    }
+    return acc;
+}
+
+// ---- the image buffer (util.wgsl:17-34, 101-145) ----
+inline float pxf(const uint32_t* px, size_t w) { return bits_f(px[w]); }
+inline void pxs(uint32_t* px, size_t w, float v) { px[w] = f_bits(v); }
+
+void guide(uint32_t* px, size_t n, size_t i, bool keep, V3 albedo, V3 nn, float dist) {
+    size_t a = 7 * n + 3 * i, b = 10 * n + 3 * i, c = 13 * n + i;
+    V3 old_a = keep ? v3(pxf(px, a), pxf(px, a + 1), pxf(px, a + 2)) : v3(0, 0, 0);
+    V3 old_n = keep ? v3(pxf(px, b), pxf(px, b + 1), pxf(px, b + 2)) : v3(0, 0, 0);
+    float old_d = keep ? pxf(px, c) : 0.0f;
+    V3 sa = old_a + albedo, sn = old_n + nn;
+    pxs(px, a, sa.x); pxs(px, a + 1, sa.y); pxs(px, a + 2, sa.z);
+    pxs(px, b, sn.x); pxs(px, b + 1, sn.y); pxs(px, b + 2, sn.z);
+    pxs(px, c, old_d + dist);
+}
+
+void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[4], float cover, float missed) {
+    size_t a = n + 3 * i, g = 20 * n + 3 * i, gc = 23 * n + i, fc = 24 * n + i, mc = 33 * n + i, cn = 31 * n + i;
+    float gone = (keep ? pxf(px, mc) : 0.0f) + missed;
+    pxs(px, mc, gone);
+    uint32_t taken = (keep ? px[cn] : 0u) + 1u;
+    px[cn] = taken;
+    float film = (keep ? pxf(px, fc) : 0.0f) + cover;
+    pxs(px, fc, film);
+    V3 l = (keep ? v3(pxf(px, a), pxf(px, a + 1), pxf(px, a + 2)) : v3(0, 0, 0)) + lin;
+    V3 gs = (keep ? v3(pxf(px, g), pxf(px, g + 1), pxf(px, g + 2)) : v3(0, 0, 0)) + v3(over[0], over[1], over[2]);
+    float cov = (keep ? pxf(px, gc) : 0.0f) + over[3];
+    pxs(px, a, l.x); pxs(px, a + 1, l.y); pxs(px, a + 2, l.z);
+    pxs(px, g, gs.x); pxs(px, g + 1, gs.y); pxs(px, g + 2, gs.z);
+    pxs(px, gc, cov);
+}
+
+// shade.wgsl:41-373 for one pixel and the camera's frame.
+void shade(Ctx& c, uint32_t i, uint32_t* px) {
+    const orc_camera* p = c.cam;
+    const orc_scene_t* sc = c.s;
+    size_t n = (size_t)p->w * p->h;
+    bool keep = p->frame > 0u;
+    if (p->frame == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
+    float fw = (float)p->w, fh = (float)p->h;
+    float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
+    float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
+    uint32_t seed = hash_u(i * 9781u + p->frame * 6271u + 1u);
+    float ju = rnd(&seed);
+    float jv = rnd(&seed);
+    float jx = (ju - 0.5f) / fw;
+    float jy = (jv - 0.5f) / fh;
+    float vx = (su + jx - p->pan[0]) / p->scale;
+    float vy = -((sv + jy) / p->aspect - p->pan[1]) / p->scale;
+    V3 o, d;
+    if (p->persp != 0u) {
+        o = ld3(p->pivot) + ld3(p->back) * (p->dist / p->units);
+        d = normalize(ld3(p->right) * vx + ld3(p->up) * vy - ld3(p->back) * p->dist);
+    } else {
+        o = ld3(p->pivot) + (ld3(p->right) * vx + ld3(p->up) * vy + ld3(p->back) * 1.0e4f) / p->units;
+        d = -ld3(p->back);
+    }
+    px[25 * n + i] = 0u;  // outline_store(i, 0)
+    V3 col = v3(0, 0, 0), thr = v3(1, 1, 1);
+    bool depth_set = p->depth[0] != 0u;
+    uint32_t total = depth_set ? p->depth[0] - 1u : 8u;
+    uint32_t see_through = depth_set ? p->depth[1] - 1u : 8u;
+    uint32_t through = depth_set ? std::min(p->depth[2] - 1u, total) : 8u;
+    bool diffuse_on = !depth_set || (p->bounces > 0u && total > 0u);
+    uint32_t n_see = 0u;
+    uint32_t passes = see_through + through + 1u;
+    static const float none[4] = {0, 0, 0, 0};
+    for (uint32_t bounce = 0; bounce < passes; ++bounce) {
+        Hit h = trace(c, o, d, INF, false);
+        if (h.tri == ORC_NONE) {
+            if (bounce == 0u) {
+                // The theme's sky (black here) rides the overlay term, not the light.
+                static const float sky_over[4] = {0, 0, 0, 1};
+                guide(px, n, i, keep, v3(0, 0, 0), -d, -1.0f);
+                emit(px, n, i, keep, v3(0, 0, 0), sky_over, 1.0f, 1.0f);
+                return;
+            }
+            col = col + thr * env_at(p, d);
+            break;
+        }
+        if (bounce == 0u) px[25 * n + i] = h.inst + 1u;
+        uint32_t m = mat_at(sc, h.tri);
+        Tri t = tri_of(sc, h.tri, m);
+        float w = 1.0f - h.u - h.v;
+        V3 hit = o + d * h.t;
+        V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        bool front = dot(ng0, d) < 0.f;
+        V3 ng = front ? ng0 : -ng0;
+        V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
+        sn = dot(sn, sn) > 1e-12f ? normalize(sn) * (front ? 1.0f : -1.0f) : ng;
+        if (dot(sn, -d) < 0.f) sn = ng;
+        // hit_lift / through_lift (light.wgsl:174-198)
+        float start = 1.0e4f / p->units;
+        float lift = (p->persp == 0u && h.t > start * 0.5f)
+                         ? p->eps + std::fabs(h.t - start) * 1e-5f + start * 2.4e-7f
+                         : p->eps + h.t * 1e-5f;
+        V3 off = ng * lift;
+        V3 pos = hit + off;
+        V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+        float alpha = t.color[3];
+        float rough = t.rough, metal = t.metal;
+        V3 emit_c = t.emit;
+        if (bounce == 0u) guide(px, n, i, keep, base, sn, h.t);
+        if (rnd(&seed) >= alpha) {
+            n_see++;
+            if (n_see > see_through) break;
+            float reach = fmax_(fmax_(std::fabs(hit.x), std::fabs(hit.y)), fmax_(std::fabs(hit.z), 0.5f));
+            float tl = lift - p->eps + reach * (256.0f / 8388608.0f);
+            o = hit - ng * tl;
+            continue;
+        }
+        col = col + thr * emit_c;
+        V3 fd = dielectric_f0(t.ior);
+        V3 diffuse = base * (1.0f - metal);
+        if (maxc(diffuse) > 0.f) diffuse = diffuse * under_layer_real(t.ior, fmax_(rough, 0.04f), fmax_(dot(sn, -d), 1e-4f));
+        Surf s{pos, sn, -d, diffuse, mixv(fd, base, metal), fmax_(rough, 0.04f), 0.0f, t.ior, 1.0f - metal};
+        V3 cc = direct(c, s, &seed) * 1.0f;
+        V3 bounced = v3(0, 0, 0);
+        const V3 lw = v3(0.2126f, 0.7152f, 0.0722f);
+        float pd = dot(vmax(s.albedo, 0.0f), lw);
+        float pg = 0.0f;  // the glossy lobe is off
+        float p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
+        bool take_diffuse = rnd(&seed) < p_diffuse;
+        if (take_diffuse && pd > 0.f) bounced = bounced + s.albedo * indirect(c, pos, sn, &seed, diffuse_on) / p_diffuse;
+        (void)rnd(&seed);  // ggx_sample_vndf's two numbers (shade.wgsl:317)
+        (void)rnd(&seed);
+        cc = cc + bounced;
+        col = col + thr * cc;
+        break;  // opaque: tw == 0 ends the camera path (shade.wgsl:345-346)
+    }
+    emit(px, n, i, keep, col, none, 1.0f, 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t orc_hash(uint32_t x) { return hash_u(x); }
+float orc_rnd(uint32_t* seed) { return rnd(seed); }
+
+void orc_sample(const orc_scene_t* s, const orc_camera* cam, uint32_t first, uint32_t count,
+                uint32_t* px, uint64_t* rays, int threads) {
+    std::atomic<uint64_t> closest{0}, shadow{0};
+    auto run = [&](uint32_t a, uint32_t b) {
+        Ctx c;
+        c.s = s;
+        c.cam = cam;
+        for (uint32_t i = a; i < b; ++i) shade(c, i, px);
+        closest += c.closest;
+        shadow += c.shadow;
+    };
+    if (threads <= 1 || count < 256) run(first, first + count);
+    else {
+        std::vector<std::thread> pool;
+        // Interleaved rows of 64 pixels spread the work of a frame evenly.
+        std::atomic<uint32_t> next{first};
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back([&] {
+                Ctx c;
+                c.s = s;
+                c.cam = cam;
+                for (;;) {
+                    uint32_t a = next.fetch_add(64);
+                    if (a >= first + count) break;
+                    uint32_t b = std::min(first + count, a + 64);
+                    for (uint32_t i = a; i < b; ++i) shade(c, i, px);
+                }
+                closest += c.closest;
+                shadow += c.shadow;
+            });
+        for (auto& th : pool) th.join();
+    }
+    if (rays) { rays[0] += closest; rays[1] += shadow; }
+}
+
+}  // extern "C"

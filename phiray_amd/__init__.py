@@ -1,0 +1,522 @@
+"""Host-side mirror of phiray's GPU API over the gfx950 HIP library.
+
+`Gpu`, `GpuArray`, `GpuBvh` and `Tracer` follow phiray's Rust types of the same
+names (src/gpu/mod.rs, src/gpu/array.rs, src/bvh/gpu.rs, src/trace/mod.rs):
+the same constructors, argument meanings and error behaviour (`None` for no
+device or an empty scene, `False` from `update` when the count changed). Every
+call goes through the C ABI declared in include/phiray_hip.h; there is no
+CPU path. Importing this package loads libphiray_hip.so and fails loudly when
+it has not been built (``python -m phiray_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libphiray_hip.so"
+
+# ---- record layouts (phiray's own, src/bvh/node.rs, src/trace/...) ----------
+AABB = np.dtype([("lo", "<f4", 3), ("_a", "<f4"), ("hi", "<f4", 3), ("_b", "<f4")])
+NODE = np.dtype([("lo", "<f4", 3), ("li", "<u4"), ("hi", "<f4", 3), ("ri", "<u4")])
+LINK = np.dtype([("left", "<u4"), ("right", "<u4"), ("first", "<u4"), ("last", "<u4")])
+HIT = np.dtype([("t", "<f4"), ("tri", "<u4"), ("u", "<f4"), ("v", "<f4"), ("inst", "<u4")])
+RAY = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("_pad", "<f4")])
+TRI = np.dtype([
+    ("a", "<f4", 3), ("rough", "<f4"), ("na", "<f4", 3), ("trans", "<f4"),
+    ("b", "<f4", 3), ("metal", "<f4"), ("nb", "<f4", 3), ("ior", "<f4"),
+    ("c", "<f4", 3), ("_c", "<f4"), ("nc", "<f4", 3), ("_nc", "<f4"),
+    ("color", "<f4", 4), ("emit", "<f4", 3), ("bump", "<f4"),
+])
+CAMERA = np.dtype([
+    ("right", "<f4", 3), ("persp", "<u4"), ("up", "<f4", 3), ("scale", "<f4"),
+    ("back", "<f4", 3), ("units", "<f4"), ("pivot", "<f4", 3), ("aspect", "<f4"),
+    ("sun", "<f4", 3), ("dist", "<f4"), ("pan", "<f4", 2), ("ao", "<f4"), ("eps", "<f4"),
+    ("env", "<f4", 4), ("frame", "<u4"), ("count", "<u4"), ("glow", "<u4"), ("bounces", "<u4"),
+    ("depth", "<u4", 4), ("integrator", "<f4", 4), ("w", "<u4"), ("h", "<u4"), ("rr_from", "<u4"), ("_pad", "<u4"),
+])
+assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
+assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 176
+
+LEAF = 0x80000000
+NONE = 0xFFFFFFFF
+STACK = 32
+PX_LEN = 34  # words a pixel in the image buffer (src/trace/read.rs:56)
+
+# C ABI entry points and their signatures (include/phiray_hip.h).
+_P = C.c_void_p
+_I = C.c_int
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_SIGS = {
+    "phr_last_error": (C.c_char_p, []),
+    "phr_abi_version": (_U32, []),
+    "phr_gpu_open": (_I, [_I, C.POINTER(_P)]),
+    "phr_gpu_close": (None, [_P]),
+    "phr_gpu_wait": (_I, [_P]),
+    "phr_gpu_wait_for": (_I, [_P, _U64, C.POINTER(_I)]),
+    "phr_gpu_max_binding": (_U64, [_P]),
+    "phr_gpu_cu_count": (_I, [_P]),
+    "phr_gpu_arch": (C.c_char_p, [_P]),
+    "phr_gpu_stream": (_P, [_P]),
+    "phr_array_new": (_I, [_P, _U32, _U64, C.POINTER(_P)]),
+    "phr_array_zeroed": (_I, [_P, _U32, _U64, C.POINTER(_P)]),
+    "phr_array_from": (_I, [_P, _U32, _P, _U64, C.POINTER(_P)]),
+    "phr_array_write": (_I, [_P, _P, _U64, C.POINTER(_I)]),
+    "phr_array_write_at": (_I, [_P, _U64, _P, _U64, C.POINTER(_I)]),
+    "phr_array_read_range": (_I, [_P, _U64, _U64, _P]),
+    "phr_array_len": (_U64, [_P]),
+    "phr_array_cap": (_U64, [_P]),
+    "phr_array_elem": (_U32, [_P]),
+    "phr_array_device_ptr": (_P, [_P]),
+    "phr_array_free": (None, [_P]),
+    "phr_array_copy_kernel": (_I, [_P, _P]),
+    "phr_clock_new": (_I, [_P, C.POINTER(_P)]),
+    "phr_clock_start": (_I, [_P]),
+    "phr_clock_stop": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "phr_clock_free": (None, [_P]),
+    "phr_bvh_build": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "phr_bvh_build_on": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "phr_bvh_rebuild_async": (_I, [_P]),
+    "phr_bvh_rebuild": (_I, [_P]),
+    "phr_bvh_refit_async": (_I, [_P]),
+    "phr_bvh_refit": (_I, [_P]),
+    "phr_bvh_update": (_I, [_P, _P, _U32, C.POINTER(_I)]),
+    "phr_bvh_download": (_I, [_P, _P, _P, C.POINTER(_U32)]),
+    "phr_bvh_codes": (_I, [_P, _P]),
+    "phr_bvh_len": (_U32, [_P]),
+    "phr_bvh_depth": (_U32, [_P]),
+    "phr_bvh_leaf": (_U32, [_P]),
+    "phr_bvh_nodes": (_P, [_P]),
+    "phr_bvh_order": (_P, [_P]),
+    "phr_bvh_prims": (_P, [_P]),
+    "phr_bvh_free": (None, [_P]),
+    "phr_tracer_new": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "phr_tracer_free": (None, [_P]),
+    "phr_tracer_len": (_U32, [_P]),
+    "phr_tracer_depth": (_U32, [_P]),
+    "phr_tracer_extent": (C.c_float, [_P]),
+    "phr_tracer_glow": (_U32, [_P]),
+    "phr_tracer_bvh": (_P, [_P]),
+    "phr_tracer_order": (_I, [_P, _P]),
+    "phr_tracer_lights": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
+    "phr_tracer_raycast": (_I, [_P, _P, _U32, _I, _P]),
+    "phr_tracer_raycast_on": (_I, [_P, _P, _I, _P]),
+    "phr_tracer_set_samples": (None, [_P, _U32]),
+    "phr_tracer_sample": (_I, [_P, _P, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(_I)]),
+    "phr_tracer_time": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "phr_tracer_rays": (_I, [_P, C.POINTER(_U64), C.POINTER(_U64)]),
+    "phr_tracer_counters": (_I, [_P, _P]),
+    "phr_tracer_px": (_I, [_P, _U64, _U64, _P]),
+    "phr_tracer_image": (_P, [_P]),
+    "phr_tracer_radiance": (_I, [_P, _P]),
+    "phr_tracer_denoise": (_I, [_P, _I]),
+}
+
+_lib = None
+
+
+class PhrError(RuntimeError):
+    """A negative status from the C ABI, with the library's message."""
+
+
+def lib() -> C.CDLL:
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m phiray_amd.build` (hipcc, gfx950)")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().phr_last_error().decode(errors="replace")
+        raise PhrError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class Gpu:
+    """phiray's `Gpu` (src/gpu/mod.rs:84-331): one device and its queue."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def new(cls, device: int = 0) -> "Gpu | None":
+        """`Gpu::new`: None when there is no device."""
+        out = C.c_void_p()
+        rc = lib().phr_gpu_open(device, C.byref(out))
+        if rc != 0:
+            return None
+        return cls(out.value)
+
+    def wait(self) -> None:
+        _check(lib().phr_gpu_wait(self._h), "Gpu::wait")
+
+    def wait_for(self, seconds: float) -> bool:
+        done = C.c_int(0)
+        _check(lib().phr_gpu_wait_for(self._h, int(seconds * 1e9), C.byref(done)), "Gpu::wait_for")
+        return bool(done.value)
+
+    def max_binding(self) -> int:
+        return int(lib().phr_gpu_max_binding(self._h))
+
+    def cu_count(self) -> int:
+        return int(lib().phr_gpu_cu_count(self._h))
+
+    def arch(self) -> str:
+        return lib().phr_gpu_arch(self._h).decode()
+
+    def close(self) -> None:
+        if self._h:
+            lib().phr_gpu_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuArray:
+    """phiray's `GpuArray<T>` (src/gpu/array.rs:7-188) over a numpy dtype."""
+
+    def __init__(self, gpu: Gpu, handle: int, dtype: np.dtype):
+        self.gpu = gpu
+        self._h = C.c_void_p(handle)
+        self.dtype = np.dtype(dtype)
+        self._owned = True
+
+    @classmethod
+    def _borrow(cls, gpu: Gpu, handle: int, dtype) -> "GpuArray":
+        a = cls(gpu, handle, dtype)
+        a._owned = False
+        return a
+
+    @classmethod
+    def zeroed(cls, gpu: Gpu, dtype, n: int) -> "GpuArray":
+        out = C.c_void_p()
+        _check(lib().phr_array_zeroed(gpu._h, np.dtype(dtype).itemsize, n, C.byref(out)), "GpuArray::zeroed")
+        return cls(gpu, out.value, dtype)
+
+    @classmethod
+    def from_(cls, gpu: Gpu, items: np.ndarray) -> "GpuArray":
+        items = np.ascontiguousarray(items)
+        out = C.c_void_p()
+        _check(lib().phr_array_from(gpu._h, items.dtype.itemsize, _ptr(items), len(items), C.byref(out)), "GpuArray::from")
+        return cls(gpu, out.value, items.dtype)
+
+    def write(self, items: np.ndarray) -> bool:
+        items = np.ascontiguousarray(items, dtype=self.dtype)
+        grown = C.c_int(0)
+        _check(lib().phr_array_write(self._h, _ptr(items), len(items), C.byref(grown)), "GpuArray::write")
+        return bool(grown.value)
+
+    def write_at(self, at: int, items: np.ndarray) -> bool:
+        items = np.ascontiguousarray(items, dtype=self.dtype)
+        grown = C.c_int(0)
+        _check(lib().phr_array_write_at(self._h, at, _ptr(items), len(items), C.byref(grown)), "GpuArray::write_at")
+        return bool(grown.value)
+
+    def __len__(self) -> int:
+        return int(lib().phr_array_len(self._h))
+
+    def read_range(self, start: int, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=self.dtype)
+        _check(lib().phr_array_read_range(self._h, start, count, _ptr(out)), "GpuArray::read_range")
+        return out
+
+    def read(self) -> np.ndarray:
+        return self.read_range(0, len(self))
+
+    def free(self) -> None:
+        if self._owned and self._h:
+            lib().phr_array_free(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Clock:
+    """phiray's `Clock` (src/gpu/clock.rs:17-87): HIP events on the device's stream."""
+
+    def __init__(self, gpu: Gpu):
+        self.gpu = gpu
+        out = C.c_void_p()
+        _check(lib().phr_clock_new(gpu._h, C.byref(out)), "Clock::new")
+        self._h = out
+
+    def start(self) -> None:
+        _check(lib().phr_clock_start(self._h), "Clock start")
+
+    def stop(self) -> tuple[float, float]:
+        """(GPU seconds, wall seconds) since start, after the device drains."""
+        g, w = C.c_double(0), C.c_double(0)
+        _check(lib().phr_clock_stop(self._h, C.byref(g), C.byref(w)), "Clock stop")
+        return g.value, w.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().phr_clock_free(self._h)
+        except Exception:
+            pass
+
+
+def copy_kernel(dst: GpuArray, src: GpuArray) -> None:
+    _check(lib().phr_array_copy_kernel(dst._h, src._h), "copy kernel")
+
+
+class Bvh:
+    """A downloaded tree: phiray's `Bvh {nodes, order, depth}` (src/bvh/build.rs:17-24)."""
+
+    def __init__(self, nodes: np.ndarray, order: np.ndarray, depth: int):
+        self.nodes, self.order, self.depth = nodes, order, depth
+
+    def reachable(self) -> list[int]:
+        """src/bvh/shape.rs:39-51: indices from the root, depth first."""
+        out, stack = [], ([0] if len(self.nodes) else [])
+        li, ri = self.nodes["li"], self.nodes["ri"]
+        while stack:
+            i = stack.pop()
+            out.append(i)
+            if not (li[i] & LEAF):
+                stack.extend([int(li[i]), int(ri[i])])
+        return out
+
+
+class GpuBvh:
+    """phiray's `GpuBvh` (src/bvh/gpu.rs:11-104), built entirely on the device."""
+
+    def __init__(self, gpu: Gpu, handle: int):
+        self.gpu = gpu
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def build(cls, gpu: Gpu, prims: np.ndarray, leaf: int = 4) -> "GpuBvh":
+        prims = np.ascontiguousarray(prims, dtype=AABB)
+        out = C.c_void_p()
+        _check(lib().phr_bvh_build(gpu._h, _ptr(prims), len(prims), leaf, C.byref(out)), "GpuBvh::build")
+        return cls(gpu, out.value)
+
+    @classmethod
+    def build_on(cls, gpu: Gpu, prims: GpuArray, leaf: int = 4) -> "GpuBvh":
+        out = C.c_void_p()
+        _check(lib().phr_bvh_build_on(gpu._h, prims._h, leaf, C.byref(out)), "GpuBvh::build_on")
+        b = cls(gpu, out.value)
+        b._prims = prims  # keep the borrowed array alive
+        return b
+
+    @property
+    def depth(self) -> int:
+        return int(lib().phr_bvh_depth(self._h))
+
+    @property
+    def leaf(self) -> int:
+        return int(lib().phr_bvh_leaf(self._h))
+
+    def __len__(self) -> int:
+        return int(lib().phr_bvh_len(self._h))
+
+    def rebuild(self) -> None:
+        _check(lib().phr_bvh_rebuild(self._h), "GpuBvh::rebuild")
+
+    def rebuild_async(self) -> None:
+        _check(lib().phr_bvh_rebuild_async(self._h), "GpuBvh::rebuild_async")
+
+    def refit(self) -> None:
+        _check(lib().phr_bvh_refit(self._h), "GpuBvh::refit")
+
+    def refit_async(self) -> None:
+        _check(lib().phr_bvh_refit_async(self._h), "GpuBvh::refit_async")
+
+    def update(self, prims: np.ndarray) -> bool:
+        """`GpuBvh::update`: False when the count changed (rebuild instead)."""
+        prims = np.ascontiguousarray(prims, dtype=AABB)
+        ok = C.c_int(0)
+        _check(lib().phr_bvh_update(self._h, _ptr(prims), len(prims), C.byref(ok)), "GpuBvh::update")
+        return bool(ok.value)
+
+    def download(self) -> Bvh:
+        n = len(self)
+        nodes = np.zeros(max(2 * n - 1, 0), dtype=NODE)
+        order = np.zeros(n, dtype=np.uint32)
+        depth = C.c_uint32(0)
+        _check(lib().phr_bvh_download(self._h, _ptr(nodes), _ptr(order), C.byref(depth)), "GpuBvh::download")
+        return Bvh(nodes, order, int(depth.value))
+
+    def sorted_codes(self) -> np.ndarray:
+        out = np.zeros(len(self), dtype=np.uint32)
+        _check(lib().phr_bvh_codes(self._h, _ptr(out)), "GpuBvh codes")
+        return out
+
+    def free(self) -> None:
+        if self._h:
+            lib().phr_bvh_free(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Tracer:
+    """phiray's `Tracer` (src/trace/mod.rs:50-245) for the Rendered path."""
+
+    SAMPLE_COUNT = 32  # src/trace/mod.rs:129
+
+    def __init__(self, gpu: Gpu, handle: int):
+        self.gpu = gpu
+        self._h = C.c_void_p(handle)
+        self.size = (0, 0)
+
+    @classmethod
+    def new(cls, gpu: Gpu, tris: np.ndarray, leaf: int = 4) -> "Tracer | None":
+        """`Tracer::new`: None for an empty scene."""
+        tris = np.ascontiguousarray(tris, dtype=TRI)
+        if len(tris) == 0:
+            return None
+        out = C.c_void_p()
+        _check(lib().phr_tracer_new(gpu._h, _ptr(tris), len(tris), leaf, C.byref(out)), "Tracer::new")
+        return cls(gpu, out.value)
+
+    def __len__(self) -> int:
+        return int(lib().phr_tracer_len(self._h))
+
+    @property
+    def depth(self) -> int:
+        return int(lib().phr_tracer_depth(self._h))
+
+    @property
+    def extent(self) -> float:
+        return float(lib().phr_tracer_extent(self._h))
+
+    @property
+    def glow(self) -> int:
+        return int(lib().phr_tracer_glow(self._h))
+
+    def bvh(self) -> Bvh:
+        b = GpuBvh(self.gpu, lib().phr_tracer_bvh(self._h))
+        try:
+            return b.download()
+        finally:
+            b._h = C.c_void_p()  # borrowed: the tracer owns it
+
+    def order(self) -> np.ndarray:
+        out = np.zeros(len(self), dtype=np.uint32)
+        _check(lib().phr_tracer_order(self._h, _ptr(out)), "Tracer order")
+        return out
+
+    def lights(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        _check(lib().phr_tracer_lights(self._h, None, 0, C.byref(n)), "Tracer lights")
+        out = np.zeros(n.value, dtype=np.uint32)
+        _check(lib().phr_tracer_lights(self._h, _ptr(out), n.value, C.byref(n)), "Tracer lights")
+        return out
+
+    def raycast(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), dtype=HIT)
+        _check(lib().phr_tracer_raycast(self._h, _ptr(rays), len(rays), 1 if any_hit else 0, _ptr(hits)), "Tracer raycast")
+        return hits
+
+    def raycast_on(self, rays: GpuArray, hits: GpuArray, any_hit: bool = False) -> None:
+        _check(lib().phr_tracer_raycast_on(self._h, rays._h, 1 if any_hit else 0, hits._h), "Tracer raycast_on")
+
+    def set_samples(self, n: int) -> None:
+        lib().phr_tracer_set_samples(self._h, n)
+
+    def sample(self, cam: np.ndarray, w: int, h: int, spp: int = 1) -> tuple[int, bool]:
+        """`Tracer::sample`: (index of the last sample, done)."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        frame = C.c_uint32(0)
+        done = C.c_int(0)
+        _check(lib().phr_tracer_sample(self._h, _ptr(cam), w, h, spp, C.byref(frame), C.byref(done)), "Tracer::sample")
+        self.size = (max(w, 1), max(h, 1))
+        return int(frame.value), bool(done.value)
+
+    def time(self) -> tuple[float, float]:
+        """`Tracer::time`: (submit seconds, GPU seconds) of the last sample call."""
+        a, b = C.c_double(0), C.c_double(0)
+        _check(lib().phr_tracer_time(self._h, C.byref(a), C.byref(b)), "Tracer::time")
+        return a.value, b.value
+
+    def rays(self) -> tuple[int, int]:
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().phr_tracer_rays(self._h, C.byref(a), C.byref(b)), "Tracer rays")
+        return int(a.value), int(b.value)
+
+    def counters(self) -> np.ndarray:
+        """[closest, shadow, steps, tests] of the path kernel, [steps, tests] of raycasts."""
+        out = np.zeros(6, dtype=np.uint64)
+        _check(lib().phr_tracer_counters(self._h, _ptr(out)), "Tracer counters")
+        return out
+
+    def px(self) -> np.ndarray:
+        w, h = self.size
+        n = PX_LEN * w * h + 1
+        out = np.zeros(n, dtype=np.uint32)
+        _check(lib().phr_tracer_px(self._h, 0, n, _ptr(out)), "Tracer px")
+        return out
+
+    def denoise(self, keep: bool = False) -> None:
+        """`Tracer::denoise` (keep=False) / `denoise_shown` (keep=True)."""
+        _check(lib().phr_tracer_denoise(self._h, 1 if keep else 0), "Tracer::denoise")
+
+    def radiance(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros((w * h, 3), dtype=np.float32)
+        _check(lib().phr_tracer_radiance(self._h, _ptr(out)), "Tracer::radiance")
+        return out
+
+    def free(self) -> None:
+        if self._h:
+            lib().phr_tracer_free(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def planes(px: np.ndarray, w: int, h: int) -> dict[str, np.ndarray]:
+    """The image buffer's planes (src/trace/read.rs:37-56) as arrays."""
+    n = w * h
+    f = px.view(np.float32)
+    return {
+        "radiance": f[n:4 * n].reshape(n, 3),
+        "albedo": f[7 * n:10 * n].reshape(n, 3),
+        "normal": f[10 * n:13 * n].reshape(n, 3),
+        "depth": f[13 * n:14 * n],
+        "grid": f[20 * n:23 * n].reshape(n, 3),
+        "cov": f[23 * n:24 * n],
+        "film": f[24 * n:25 * n],
+        "outline": px[25 * n:26 * n],
+        "count": px[31 * n:32 * n],
+        "missed": f[33 * n:34 * n],
+    }

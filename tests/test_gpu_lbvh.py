@@ -1,0 +1,112 @@
+"""GPU LBVH == CPU LBVH, bit for bit (tests/bvh.rs, ported and widened).
+
+Through the C ABI: `GpuBvh::build` on the device against the oracle's
+`Bvh::build` on the same boxes — order, depth, the links of every reachable
+node and their bounds (the reference's own comparison, tests/bvh.rs:21-29) —
+then `update` after every box moves against the oracle's refit on the old
+topology (tests/bvh.rs:47-52). Bit-exact: integer links and exact min/max.
+"""
+import numpy as np
+import pytest
+
+import phiray_amd as P
+from phiray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def same(got: P.Bvh, nodes, order, depth, what=""):
+    assert (got.order == order).all(), f"{what}: order"
+    assert got.depth == depth, f"{what}: depth {got.depth} vs {depth}"
+    want = P.Bvh(nodes, order, depth)
+    r = np.array(want.reachable(), dtype=np.int64)
+    g, c = got.nodes[r], nodes[r]
+    bad = np.nonzero((g["li"] != c["li"]) | (g["ri"] != c["ri"]))[0]
+    assert len(bad) == 0, f"{what}: links differ at nodes {r[bad[:8]]}"
+    gb = g.view(np.uint32).reshape(-1, 8)
+    cb = c.view(np.uint32).reshape(-1, 8)
+    bad = np.nonzero((gb != cb).any(1))[0]
+    assert len(bad) == 0, f"{what}: bounds differ at nodes {r[bad[:8]]}"
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 257, 100_003])
+def test_gpu_build_and_refit_match_cpu(gpu, orc, n):
+    print(f"[lbvh] tests/bvh.rs n={n}", flush=True)
+    p = S.xorshift_prims_gpu_test(n, 0.0)
+    nodes, order, depth = orc.build_bvh(p, 4, P.NODE, 8)
+    g = P.GpuBvh.build(gpu, p, 4)
+    same(g.download(), nodes, order, depth, f"build n={n}")
+    moved = S.xorshift_prims_gpu_test(n, 2.5)
+    assert g.update(moved)
+    same(g.download(), orc.refit(nodes, order, moved), order, depth, f"refit n={n}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 100, 5000])
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_unit_sets(gpu, orc, n, leaf):
+    print(f"[lbvh] src/bvh/test.rs n={n} leaf={leaf}", flush=True)
+    p = S.xorshift_prims_unit(n)
+    nodes, order, depth = orc.build_bvh(p, leaf, P.NODE)
+    same(P.GpuBvh.build(gpu, p, leaf).download(), nodes, order, depth)
+
+
+def test_all_equal_boxes(gpu, orc):
+    print("[lbvh] 1000 equal boxes", flush=True)
+    p = np.zeros(1000, dtype=P.AABB)
+    p["hi"] = 1.0
+    nodes, order, depth = orc.build_bvh(p, 1, P.NODE)
+    same(P.GpuBvh.build(gpu, p, 1).download(), nodes, order, depth)
+
+
+def test_depth_cap(gpu, orc):
+    print("[lbvh] depth cap chain", flush=True)
+    from tests.test_oracle_lbvh import deep_chain_prims
+
+    p = deep_chain_prims()
+    nodes, order, depth = orc.build_bvh(p, 1, P.NODE)
+    assert depth == 31
+    same(P.GpuBvh.build(gpu, p, 1).download(), nodes, order, depth)
+
+
+def test_sorted_codes_and_count_change(gpu, orc):
+    print("[lbvh] sorted codes", flush=True)
+    p = S.xorshift_prims_gpu_test(40_000)
+    g = P.GpuBvh.build(gpu, p, 4)
+    order, srt = orc.sort_codes(orc.codes(p))
+    assert (g.sorted_codes() == srt).all()
+    assert g.update(p[:-1]) is False  # a new count asks for a rebuild
+
+
+def test_lbvh_workload_and_refit(gpu, orc):
+    print("[lbvh] 65536 triangles seed 0", flush=True)
+    corners, moved = S.lbvh_scene(65536, 0)
+    p, pm = S.corner_boxes(corners), S.corner_boxes(moved)
+    nodes, order, depth = orc.build_bvh(p, 4, P.NODE, 8)
+    g = P.GpuBvh.build(gpu, p, 4)
+    same(g.download(), nodes, order, depth, "build 65536")
+    assert g.update(pm)
+    same(g.download(), orc.refit(nodes, order, pm), order, depth, "refit 65536")
+
+
+@pytest.mark.parametrize("n", [1_310_724, 3_000_001])
+def test_large_builds(gpu, orc, n):
+    print(f"[lbvh] large n={n}", flush=True)
+    corners, _ = S.lbvh_scene(n, 5)
+    p = S.corner_boxes(corners)
+    nodes, order, depth = orc.build_bvh(p, 4, P.NODE, 8)
+    same(P.GpuBvh.build(gpu, p, 4).download(), nodes, order, depth, f"n={n}")
+
+
+def test_rebuild_is_deterministic(gpu):
+    print("[lbvh] rebuild twice", flush=True)
+    corners, _ = S.lbvh_scene(200_000, 9)
+    p = S.corner_boxes(corners)
+    prims = P.GpuArray.from_(gpu, p)
+    g = P.GpuBvh.build_on(gpu, prims, 4)
+    a = g.download()
+    for _ in range(3):
+        g.rebuild()
+        b = g.download()
+        assert (a.order == b.order).all() and a.depth == b.depth
+        r = np.array(a.reachable())
+        assert (a.nodes[r].view(np.uint32) == b.nodes[r].view(np.uint32)).all()
