@@ -34,6 +34,7 @@ typedef struct phr_array phr_array;
 typedef struct phr_bvh phr_bvh;
 typedef struct phr_tracer phr_tracer;
 typedef struct phr_clock phr_clock;
+typedef struct phr_pass phr_pass;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* phr_last_error(void);
@@ -83,6 +84,29 @@ int phr_clock_new(phr_gpu* g, phr_clock** out);
 int phr_clock_start(phr_clock* c);
 int phr_clock_stop(phr_clock* c, double* gpu_s, double* wall_s);
 void phr_clock_free(phr_clock* c);
+
+/* ---- kernels and recorded passes: Kernel / Pass (src/kernel/mod.rs:51-133,
+ *      src/gpu/pass.rs:18-155) ----------------------------------------------- */
+/* Kernel lookup by phiray NAME ("bvh.morton", "bvh.karras", "bvh.refit",
+ * "radix.seed|clear|count|offset|scatter", "radix.spread", "denoise"): its
+ * Param size in bytes and binding count (BIND_LIST length). */
+int phr_kernel_known(const char* name, uint32_t* param_size, uint32_t* binds);
+int phr_pass_new(phr_gpu* g, phr_pass** out);
+void phr_pass_free(phr_pass* p);
+/* Pass::add / add_over (src/gpu/pass.rs:64-101): params' first two words
+ * (n, groups_x) are filled in; a binding-count mismatch is PHR_ERR_ARG. */
+int phr_pass_add(phr_pass* p, const char* name, uint32_t n, uint32_t items, const void* params, uint32_t param_len,
+                 phr_array* const* arrays, uint32_t count, uint32_t* step);
+/* Pass::set (src/gpu/pass.rs:104-106). */
+int phr_pass_set(phr_pass* p, uint32_t step, uint32_t offset, const void* data, uint32_t len);
+/* Pass::run (src/gpu/pass.rs:109-113): all steps in order, replayed as one
+ * hipGraph (captured again after add/set/clear). Enqueued, not waited. */
+int phr_pass_run(phr_pass* p);
+/* Pass::run_step (src/gpu/pass.rs:126-137). */
+int phr_pass_run_step(phr_pass* p, uint32_t step);
+/* Pass::clear / len (src/gpu/pass.rs:140-148). */
+void phr_pass_clear(phr_pass* p);
+uint32_t phr_pass_len(const phr_pass* p);
 
 /* ---- BVH: GpuBvh (src/bvh/gpu.rs:11-104) ------------------------------------ */
 /* GpuBvh::build (src/bvh/gpu.rs:23-75): prims are n Aabb (32 B) on the host.

@@ -77,6 +77,15 @@ _SIGS = {
     "phr_clock_start": (_I, [_P]),
     "phr_clock_stop": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "phr_clock_free": (None, [_P]),
+    "phr_kernel_known": (_I, [C.c_char_p, C.POINTER(_U32), C.POINTER(_U32)]),
+    "phr_pass_new": (_I, [_P, C.POINTER(_P)]),
+    "phr_pass_free": (None, [_P]),
+    "phr_pass_add": (_I, [_P, C.c_char_p, _U32, _U32, _P, _U32, C.POINTER(_P), _U32, C.POINTER(_U32)]),
+    "phr_pass_set": (_I, [_P, _U32, _U32, _P, _U32]),
+    "phr_pass_run": (_I, [_P]),
+    "phr_pass_run_step": (_I, [_P, _U32]),
+    "phr_pass_clear": (None, [_P]),
+    "phr_pass_len": (_U32, [_P]),
     "phr_bvh_build": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
     "phr_bvh_build_on": (_I, [_P, _P, _U32, C.POINTER(_P)]),
     "phr_bvh_rebuild_async": (_I, [_P]),
@@ -286,6 +295,57 @@ class Clock:
 
 def copy_kernel(dst: GpuArray, src: GpuArray) -> None:
     _check(lib().phr_array_copy_kernel(dst._h, src._h), "copy kernel")
+
+
+class Pass:
+    """phiray's `Pass` (src/gpu/pass.rs:18-155) over kernels named as phiray names them."""
+
+    def __init__(self, gpu: Gpu):
+        self.gpu = gpu
+        out = C.c_void_p()
+        _check(lib().phr_pass_new(gpu._h, C.byref(out)), "Pass::new")
+        self._h = out
+        self._keep: list = []
+
+    @staticmethod
+    def kernel(name: str) -> tuple[int, int]:
+        """(Param bytes, bindings) of a registered kernel."""
+        size, binds = C.c_uint32(0), C.c_uint32(0)
+        _check(lib().phr_kernel_known(name.encode(), C.byref(size), C.byref(binds)), f"kernel {name}")
+        return size.value, binds.value
+
+    def add(self, name: str, n: int, params: bytes, arrays: list, items: int | None = None) -> int:
+        buf = C.create_string_buffer(bytes(params), len(params))
+        arr = (C.c_void_p * len(arrays))(*[a._h.value for a in arrays])
+        step = C.c_uint32(0)
+        _check(lib().phr_pass_add(self._h, name.encode(), n, n if items is None else items, buf, len(params), arr,
+                                  len(arrays), C.byref(step)), f"Pass::add {name}")
+        self._keep.append(arrays)
+        return step.value
+
+    def set(self, step: int, offset: int, data: bytes) -> None:
+        buf = C.create_string_buffer(bytes(data), len(data))
+        _check(lib().phr_pass_set(self._h, step, offset, buf, len(data)), "Pass::set")
+
+    def run(self) -> None:
+        _check(lib().phr_pass_run(self._h), "Pass::run")
+
+    def run_step(self, step: int) -> None:
+        _check(lib().phr_pass_run_step(self._h, step), "Pass::run_step")
+
+    def clear(self) -> None:
+        lib().phr_pass_clear(self._h)
+        self._keep.clear()
+
+    def __len__(self) -> int:
+        return int(lib().phr_pass_len(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().phr_pass_free(self._h)
+        except Exception:
+            pass
 
 
 class Bvh:

@@ -108,6 +108,17 @@ __global__ __launch_bounds__(256) void k_denoise(uint32_t* __restrict__ px, uint
 
 }  // namespace
 
+// One pass (the `denoise` kernel of a recorded Pass).
+int phr_denoise_pass_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, uint32_t step, uint32_t round, uint32_t last,
+                             uint32_t keep, const float s[4]) {
+    const size_t n = (size_t)w * h;
+    if (n == 0) return PHR_OK;
+    hipLaunchKernelGGL(k_denoise, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, px, w, h, step, round, last,
+                       keep, s[0], s[1], s[2], s[3]);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
+
 // Enqueue the four passes of Tracer::denoise_with over an image buffer of w x h.
 int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep) {
     const float s[4] = {64.0f, 8.0f, 40.0f, 7.0f};
