@@ -16,6 +16,7 @@
 // steps as one hipGraph, captured on first run and after any add/set/clear.
 #include "phr_runtime.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -146,12 +147,87 @@ __global__ __launch_bounds__(256) void kp_offset(RadixParam p, uint32_t* __restr
         tally[k] = 0u;
     }
 }
+// radix.scatter, stable. phiray's kernel takes each key's slot from an atomic
+// cursor (src/bvh/radix.rs:193-199), so keys of one digit land in whatever
+// order the atomics resolve in; its own model (src/bvh/sort.rs:127-150) and
+// the order it promises (Bvh::sort_codes) need every pass stable. Here a tile
+// of 2048 keys ranks its digits in index order with wave ballots, publishes
+// its per-digit counts, and looks back over earlier tiles (ids from a ticket,
+// so an earlier tile is always running or done) for its offsets within the
+// digit's run that starts at the cursor `tally[256 + d]`. The result is the
+// model's, whatever the timing.
+constexpr uint32_t SC_ITEMS = 8, SC_TILE = 256 * SC_ITEMS;
+constexpr uint32_t SC_AGG = 0x40000000u, SC_PRE = 0x80000000u, SC_VAL = 0x3fffffffu;
+
+__global__ __launch_bounds__(256) void kp_scatter_clear(uint32_t* __restrict__ scratch, uint32_t words) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < words; i += gridDim.x * 256) scratch[i] = 0u;
+}
+
 __global__ __launch_bounds__(256) void kp_scatter(RadixParam p, const uint2* __restrict__ src, uint2* __restrict__ dst,
-                                                  uint32_t* __restrict__ tally) {
-    uint32_t i = item(p.groups_x);
-    if (i >= p.n) return;
-    uint2 k = src[i];
-    dst[atomicAdd(&tally[256u + rdigit(k, p.shift)], 1u)] = k;
+                                                  const uint32_t* __restrict__ tally, uint32_t* __restrict__ scratch) {
+    __shared__ uint32_t s_whist[4][256];
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t n = p.n;
+    if (tid == 0) s_tile = atomicAdd(&scratch[0], 1u);
+    for (uint32_t i = tid; i < 4 * 256; i += 256) (&s_whist[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t wbase = tile * SC_TILE + wave * (SC_ITEMS * 64);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint2 key[SC_ITEMS];
+    uint32_t rank[SC_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < SC_ITEMS; ++r) {
+        uint32_t idx = wbase + r * 64 + lane;
+        bool ok = idx < n;
+        key[r] = ok ? src[idx] : make_uint2(0u, 0u);
+        uint32_t d = rdigit(key[r], p.shift);
+        uint64_t mask = __ballot(ok);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            mask &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t pre = 0;
+        if (ok) pre = s_whist[wave][d];
+        uint32_t below = (uint32_t)__popcll(mask & lt);
+        if (ok && below == 0) s_whist[wave][d] = pre + (uint32_t)__popcll(mask);
+        rank[r] = pre + below;
+    }
+    __syncthreads();
+    const uint32_t d = tid;
+    uint32_t c0 = s_whist[0][d], c1 = s_whist[1][d], c2 = s_whist[2][d], c3 = s_whist[3][d];
+    uint32_t total = c0 + c1 + c2 + c3;
+    uint32_t* status = scratch + 1;
+    uint32_t* my = status + (size_t)tile * 256 + d;
+    __hip_atomic_store(my, (tile == 0 ? SC_PRE : SC_AGG) | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int32_t j = (int32_t)tile - 1;
+        for (;;) {
+            uint32_t s = __hip_atomic_load(status + (size_t)j * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (s & SC_PRE) { excl += s & SC_VAL; break; }
+            if (s & SC_AGG) { excl += s & SC_VAL; --j; continue; }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __hip_atomic_store(my, SC_PRE | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_whist[0][d] = 0;
+    s_whist[1][d] = c0;
+    s_whist[2][d] = c0 + c1;
+    s_whist[3][d] = c0 + c1 + c2;
+    s_base[d] = tally[256u + d] + excl;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < SC_ITEMS; ++r) {
+        uint32_t idx = wbase + r * 64 + lane;
+        if (idx < n) {
+            uint32_t dd = rdigit(key[r], p.shift);
+            dst[s_base[dd] + s_whist[wave][dd] + rank[r]] = key[r];
+        }
+    }
 }
 __global__ __launch_bounds__(256) void kp_spread(RadixParam p, const uint2* __restrict__ key, uint32_t* __restrict__ codes) {
     uint32_t i = item(p.groups_x);
@@ -183,6 +259,8 @@ struct phr_pass {
         std::vector<phr_array*> arrays;
         uint32_t items;
         dim3 grid;
+        uint32_t* scratch = nullptr;  // radix.scatter's ticket and look-back words
+        uint32_t scratch_words = 0;
     };
     phr_gpu* gpu = nullptr;
     std::vector<Step> steps;
@@ -192,6 +270,12 @@ struct phr_pass {
 };
 
 namespace {
+
+void free_steps(phr_pass* p) {
+    for (auto& s : p->steps)
+        if (s.scratch) (void)hipFree(s.scratch);
+    p->steps.clear();
+}
 
 void drop_graph(phr_pass* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -219,8 +303,16 @@ void launch(phr_pass* ps, const phr_pass::Step& s, hipStream_t st) {
         case 4: hipLaunchKernelGGL(kp_clear, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (uint32_t*)P(0)); break;
         case 5: hipLaunchKernelGGL(kp_count, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (const uint2*)P(0), (uint32_t*)P(1)); break;
         case 6: hipLaunchKernelGGL(kp_offset, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (uint32_t*)P(0)); break;
-        case 7: hipLaunchKernelGGL(kp_scatter, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (const uint2*)P(0), (uint2*)P(1),
-                                   (uint32_t*)P(2)); break;
+        case 7: {
+            RadixParam rp = param_of<RadixParam>(s);
+            uint32_t tiles = (rp.n + SC_TILE - 1) / SC_TILE;
+            hipLaunchKernelGGL(kp_scatter_clear, dim3(std::min<uint32_t>((s.scratch_words + 255) / 256, 1024)), dim3(256), 0, st,
+                               s.scratch, s.scratch_words);
+            if (tiles)
+                hipLaunchKernelGGL(kp_scatter, dim3(tiles), dim3(256), 0, st, rp, (const uint2*)P(0), (uint2*)P(1),
+                                   (const uint32_t*)P(2), s.scratch);
+            break;
+        }
         case 8: hipLaunchKernelGGL(kp_spread, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (const uint2*)P(0), (uint32_t*)P(1)); break;
         case 9: {
             DenoiseParam d = param_of<DenoiseParam>(s);
@@ -246,6 +338,8 @@ int phr_pass_new(phr_gpu* g, phr_pass** out) {
 void phr_pass_free(phr_pass* p) {
     if (!p) return;
     drop_graph(p);
+    (void)hipStreamSynchronize(p->gpu->stream);
+    free_steps(p);
     delete p;
 }
 
@@ -285,6 +379,13 @@ int phr_pass_add(phr_pass* p, const char* name, uint32_t n, uint32_t items, cons
     s.grid = dim3(x, (blocks + x - 1) / x, 1);
     uint32_t head[2] = {n, x};
     std::memcpy(s.params.data(), head, 8);
+    if (kind == 7) {  // radix.scatter keeps look-back words sized for its n
+        s.scratch_words = 1 + ((n + SC_TILE - 1) / SC_TILE) * 256;
+        if (hipMalloc(&s.scratch, 4ull * s.scratch_words) != hipSuccess) {
+            phr_set_error_msg("radix.scatter: scratch allocation failed");
+            return PHR_ERR_HIP;
+        }
+    }
     p->steps.push_back(std::move(s));
     drop_graph(p);
     if (step) *step = (uint32_t)p->steps.size() - 1;
@@ -328,8 +429,9 @@ int phr_pass_run_step(phr_pass* p, uint32_t step) {
 
 void phr_pass_clear(phr_pass* p) {
     if (!p) return;
-    p->steps.clear();
     drop_graph(p);
+    (void)hipStreamSynchronize(p->gpu->stream);
+    free_steps(p);
 }
 uint32_t phr_pass_len(const phr_pass* p) { return p ? (uint32_t)p->steps.size() : 0; }
 
