@@ -17,6 +17,7 @@ import argparse
 import json
 import os
 import pathlib
+import re
 import sys
 import time
 
@@ -35,6 +36,19 @@ HBM_PEAK_GBS = 8000.0
 # the root node (32), both children at every inner step (2 x 32), one
 # triangle record per test (48: three corners in 16-byte rows).
 RAY_BYTES, HIT_BYTES, ROOT_BYTES, STEP_BYTES, TEST_BYTES = 32, 8, 32, 64, 48
+
+
+def profiled_traffic(name: str, spp: int, w: int, h: int) -> tuple[float | None, str | None]:
+    """HBM bytes of one k_path launch from the committed counter summary, when
+    it was measured on this same workload (FETCH_SIZE + WRITE_SIZE, each
+    scaled by what the counter reads for a copy of known size)."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", f.read_text())
+    if not m or (spp, w, h) != (64, 1920, 1080):
+        return None, None
+    return float(m.group(1)), f"profiles/{name}"
 
 
 def log(msg: str) -> None:
@@ -80,6 +94,36 @@ def lbvh_rates(gpu: P.Gpu, boxes: np.ndarray, reps: int) -> dict:
         "bytes_per_tri_build": build_bytes / n,
         "bytes_per_tri_refit": refit_bytes / n,
     }
+
+
+def batch_rates(gpu: P.Gpu, reps: int) -> dict:
+    """The 256-scene batch: one build makes every scene's tree (segments)."""
+    boxes, seg_lo = S.batch_scenes(256)
+    prims = P.GpuArray.from_(gpu, boxes)
+    b = P.GpuBvh.build_batch_on(gpu, prims, seg_lo, 4)
+    clock = P.Clock(gpu)
+    for _ in range(2):
+        b.rebuild_async()
+        b.refit_async()
+    gpu.wait()
+    clock.start()
+    for _ in range(reps):
+        b.rebuild_async()
+    t_build, _ = clock.stop()
+    clock.start()
+    for _ in range(reps):
+        b.refit_async()
+    t_refit, _ = clock.stop()
+    n = len(boxes)
+    tb, tr = t_build / reps, t_refit / reps
+    # Algorithmic bytes as for one tree, plus two sort passes over the segment id
+    # (8 read + 8 write + 4 segment id each) and the segment id written once (4).
+    reach = n  # about one reachable node per box at leaf 4
+    refit_bytes = n * 36 + reach * (4 + 8 + 32 + 32)
+    build_bytes = n * (32 + 36 + 68 + 4 + 4 + 2 * 20) + (2 * n - len(seg_lo) + 1) * 32 + refit_bytes
+    return {"scenes": len(seg_lo) - 1, "tris": n, "build_ms": tb * 1e3, "build_mtri_s": n / tb / 1e6,
+            "build_gbs": build_bytes / tb / 1e9, "refit_ms": tr * 1e3, "refit_mtri_s": n / tr / 1e6,
+            "max_depth": b.depth}
 
 
 def raycast_rate(gpu: P.Gpu, reps: int) -> dict:
@@ -180,6 +224,8 @@ def main() -> int:
         corners, _ = S.lbvh_scene(65536, 0)
         out["lbvh_65536"] = lbvh_rates(gpu, S.corner_boxes(corners), 50)
         out["raycast_65536"] = raycast_rate(gpu, 50)
+        log("batch of 256 scenes")
+        out["lbvh_batch_256"] = batch_rates(gpu, 10)
         log(f"lbvh {out['lbvh_scene']['build_mtri_s']:.1f} Mtri/s (scene), {out['lbvh_65536']['build_mtri_s']:.1f} Mtri/s (65536)")
 
     line = {"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -215,8 +261,10 @@ def main() -> int:
         kernel_s = float(np.mean(gpu_s))
         nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
         achieved = nbytes / kernel_s / 1e9
+        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
+                            "algorithmic_bytes": nbytes, "kernel": "k_path",
                             "per_ray_bytes": "32 ray + 8 hit + 32 root + 64 x inner steps + 48 x triangle tests",
                             "launch_ms": kernel_s * 1e3}
         line["s_per_frame"] = per

@@ -127,7 +127,24 @@ int phr_bvh_update(phr_bvh* b, const void* prims, uint32_t n, int* refitted);
 int phr_bvh_download(phr_bvh* b, void* nodes, uint32_t* order, uint32_t* depth);
 /* GpuSort::read's codes half (src/bvh/sort.rs:109-111): sorted Morton codes. */
 int phr_bvh_codes(phr_bvh* b, uint32_t* sorted_codes);
+/* Many trees in one build: seg_lo holds segs + 1 non-decreasing starts from 0
+ * to n, and segment s is boxes [seg_lo[s], seg_lo[s+1]). Each segment gets the
+ * tree Bvh::build (src/bvh/build.rs:94-100) gives its boxes alone, with its
+ * own scene box: one tree per mesh as Forest::build makes them
+ * (src/trace/forest.rs:44-66), or a batch of independent scenes. The node
+ * array holds the trees back to back; a tree's children and leaf ranges are
+ * indices into the whole array and the whole box list, as Forest::nodes lays
+ * the mesh trees out (src/trace/forest.rs:130-160). At most 65,536 segments. */
+int phr_bvh_build_batch(phr_gpu* g, const void* prims, uint32_t n, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf,
+                        phr_bvh** out);
+int phr_bvh_build_batch_on(phr_gpu* g, phr_array* prims, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf, phr_bvh** out);
+/* One segment's tree relative to itself, as GpuBvh::download would give it
+ * for that segment's boxes alone: nodes[2 n_s - 1], order[n_s], depth. */
+int phr_bvh_download_segment(phr_bvh* b, uint32_t seg, void* nodes, uint32_t* order, uint32_t* depth);
+uint32_t phr_bvh_segments(const phr_bvh* b);
+uint32_t phr_bvh_segment_depth(const phr_bvh* b, uint32_t seg);
 uint32_t phr_bvh_len(const phr_bvh* b);
+/* The deepest segment's depth (Bvh::depth for a single tree). */
 uint32_t phr_bvh_depth(const phr_bvh* b);
 uint32_t phr_bvh_leaf(const phr_bvh* b);
 /* GpuBvh's public fields prims / order / nodes (src/bvh/gpu.rs:12-15). */

@@ -127,7 +127,66 @@ typedef struct {
     const orc_node* nodes; uint32_t node_count;
     const float* tris; const uint32_t* faces; const float* verts;
     const float* mats; const uint32_t* lights;
+    /* A forest (forest != 0): the top tree at node 0 and instance words as
+     * Forest::inst_words lays them out (ORC_INSTANCE_LEN words a record, then
+     * the top-tree slot order, then the draw order). Otherwise one tree at
+     * node 0 under one identity instance, and insts is unused. */
+    const uint32_t* insts; uint32_t ninsts; uint32_t forest;
 } orc_scene_t;
+
+/* ---- instances, meshes and the forest (src/trace/instance.rs, mesh.rs, forest.rs) ---- */
+#define ORC_INSTANCE_LEN 40u
+#define ORC_INST 0x40000000u
+/* src/trace/instance.rs:23-44: world is m[col][row] (translation in m[3]). */
+typedef struct {
+    float world[16];
+    uint32_t mesh;
+    uint32_t casts_shadow;
+    float color[3];
+    uint32_t hash;
+} orc_instance;
+/* src/trace/mesh.rs:12-22: face and vertex ranges; bounds as Mesh::fit leaves them. */
+typedef struct { uint32_t face_lo, face_hi, vert_lo, vert_hi; float lo[3], hi[3]; } orc_mesh;
+/* A scene of meshes and instances (src/trace/scene.rs:40-67): vertices 8
+ * floats (object-space position xyz, w, packed normal xy, uv), faces 4 words
+ * (3 vertex ids + material), materials ORC_MAT_WORDS floats. The meshes tile
+ * the faces in order. */
+typedef struct {
+    const float* verts; uint32_t nverts;
+    const uint32_t* faces; uint32_t nfaces;
+    const float* mats; uint32_t nmats;
+    const orc_mesh* meshes; uint32_t nmeshes;
+    const orc_instance* insts; uint32_t ninsts;
+} orc_scene_in;
+
+void orc_inst_inverse(const float world[16], float out[12]);           /* instance.rs:75-93 */
+uint32_t orc_inst_mirrors(const float world[16]);                        /* instance.rs:111-113 */
+void orc_inst_to_world(const float world[16], const float p[3], float out[3]); /* instance.rs:68-71 */
+void orc_inst_world_box(const float world[16], const orc_aabb* b, orc_aabb* out); /* instance.rs:116-124 */
+void orc_mesh_texspace(const orc_mesh* m, float lo[3], float size[3]);  /* mesh.rs:43-54 */
+void orc_mesh_fit(orc_mesh* m, const float* verts);                     /* mesh.rs:56-72, 8-float vertices */
+/* Instance::words (instance.rs:126-160). */
+void orc_inst_words(const orc_instance* inst, uint32_t blas_root, const orc_mesh* mesh, uint32_t out[40]);
+/* Sizes the forest build writes: node count (Forest::tlas_slots + each mesh
+ * tree), instance words, emitter table words. */
+uint32_t orc_forest_node_count(const orc_scene_in* in);
+uint32_t orc_forest_inst_len(const orc_scene_in* in);
+uint32_t orc_forest_lights_len(const orc_scene_in* in);
+/* Forest::build (forest.rs:44-66, with Bvh::build for every tree) + nodes +
+ * inst_words, the faces put in leaf order (Scene::reorder), the triangle
+ * cache (9 floats a slot, object space), the emitter table
+ * (lamp.rs:51-97) and the extent (forest.rs:189-199). order[slot] = input
+ * face. depths: one per mesh, then the top tree's. Returns the node count. */
+uint32_t orc_forest_build(const orc_scene_in* in, uint32_t leaf, orc_node* nodes, uint32_t* order, float* tri_cache,
+                          uint32_t* faces, uint32_t* insts, uint32_t* lights, uint32_t* depths, float* extent, int threads);
+/* Forest::retlas after new matrices (pose.rs:107-124): the top tree, the
+ * instance words and the extent again; the mesh trees stay. `nodes`,
+ * `insts`, `lights` as orc_forest_build wrote them; faces in leaf order. */
+void orc_forest_reinstance(const orc_scene_in* in, orc_node* nodes, uint32_t* insts, uint32_t* lights,
+                           const uint32_t* faces_slot, uint32_t* tlas_depth, float* extent);
+/* trace (bvh.wgsl:120-232) through a scene: forest or single tree. */
+void orc_trace_scene(const orc_scene_t* s, const orc_ray* rays, uint32_t nrays, int any_hit, orc_hit* hits,
+                     uint32_t* stats, int threads);
 
 /* Tracer::new over `tris` (src/trace/build.rs:11-13, src/trace/scene/add.rs,
  * src/trace/scene.rs:188-190, src/trace/lamp.rs:51-97) with the LBVH as its

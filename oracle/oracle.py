@@ -53,6 +53,19 @@ _SIGS = {
     "orc_log2": (C.c_float, [C.c_float]),
     "orc_denoise": (None, [_P, _U32, _U32, _U32]),
     "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P]),
+    "orc_inst_inverse": (None, [_P, _P]),
+    "orc_inst_mirrors": (_U32, [_P]),
+    "orc_inst_to_world": (None, [_P, _P, _P]),
+    "orc_inst_world_box": (None, [_P, _P, _P]),
+    "orc_mesh_texspace": (None, [_P, _P, _P]),
+    "orc_mesh_fit": (None, [_P, _P]),
+    "orc_inst_words": (None, [_P, _U32, _P, _P]),
+    "orc_forest_node_count": (_U32, [_P]),
+    "orc_forest_inst_len": (_U32, [_P]),
+    "orc_forest_lights_len": (_U32, [_P]),
+    "orc_forest_build": (_U32, [_P, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _I]),
+    "orc_forest_reinstance": (None, [_P, _P, _P, _P, _P, _P, _P]),
+    "orc_trace_scene": (None, [_P, _P, _U32, _I, _P, _P, _I]),
 }
 
 _lib = None
@@ -153,7 +166,106 @@ def raycast(nodes: np.ndarray, tris9: np.ndarray, rays: np.ndarray, hit_dtype, a
 # ---- tracer -------------------------------------------------------------------
 class OrcScene(C.Structure):
     _fields_ = [("nodes", _P), ("node_count", _U32), ("tris", _P), ("faces", _P), ("verts", _P),
-                ("mats", _P), ("lights", _P)]
+                ("mats", _P), ("lights", _P), ("insts", _P), ("ninsts", _U32), ("forest", _U32)]
+
+
+INSTANCE_LEN = 40
+INST = 0x40000000
+# src/trace/instance.rs:23-44 and src/trace/mesh.rs:12-22 as the oracle takes them.
+INSTANCE = np.dtype([("world", "<f4", (16,)), ("mesh", "<u4"), ("casts_shadow", "<u4"), ("color", "<f4", (3,)),
+                     ("hash", "<u4")])
+MESH = np.dtype([("face_lo", "<u4"), ("face_hi", "<u4"), ("vert_lo", "<u4"), ("vert_hi", "<u4"), ("lo", "<f4", (3,)),
+                 ("hi", "<f4", (3,))])
+
+
+class OrcSceneIn(C.Structure):
+    _fields_ = [("verts", _P), ("nverts", _U32), ("faces", _P), ("nfaces", _U32), ("mats", _P), ("nmats", _U32),
+                ("meshes", _P), ("nmeshes", _U32), ("insts", _P), ("ninsts", _U32)]
+
+
+def inst_words(inst: np.ndarray, blas_root: int, mesh: np.ndarray) -> np.ndarray:
+    out = np.zeros(INSTANCE_LEN, dtype=np.uint32)
+    lib().orc_inst_words(ptr(np.ascontiguousarray(inst)), blas_root, ptr(np.ascontiguousarray(mesh)), ptr(out))
+    return out
+
+
+def inst_world_box(world: np.ndarray, box: np.ndarray) -> np.ndarray:
+    out = np.zeros(1, dtype=box.dtype)
+    lib().orc_inst_world_box(ptr(np.ascontiguousarray(world, dtype=np.float32)), ptr(np.ascontiguousarray(box)), ptr(out))
+    return out[0]
+
+
+def inst_inverse(world: np.ndarray) -> np.ndarray:
+    out = np.zeros(12, dtype=np.float32)
+    lib().orc_inst_inverse(ptr(np.ascontiguousarray(world, dtype=np.float32)), ptr(out))
+    return out.reshape(3, 4)
+
+
+class Forest:
+    """The oracle's scene of meshes under instances (Forest::build with LBVH trees)."""
+
+    def __init__(self, verts, faces, mats, meshes, insts, leaf: int, node_dtype, nthreads: int = 1):
+        words = lambda a, dt: np.ascontiguousarray(a).view(dt).reshape(-1).copy()  # noqa: E731
+        self.verts = words(verts, np.float32)
+        self.in_faces = words(faces, np.uint32)
+        self.mats = words(mats, np.float32)
+        self.meshes = np.ascontiguousarray(meshes, dtype=MESH)
+        self.insts = np.ascontiguousarray(insts, dtype=INSTANCE)
+        self.node_dtype = node_dtype
+        self.n = len(self.in_faces) // 4
+        self._in = OrcSceneIn(ptr(self.verts), len(self.verts) // 8, ptr(self.in_faces), self.n, ptr(self.mats),
+                              len(self.mats) // 16, ptr(self.meshes), len(self.meshes), ptr(self.insts), len(self.insts))
+        L = lib()
+        self.nodes = np.zeros(L.orc_forest_node_count(C.byref(self._in)), dtype=node_dtype)
+        self.order = np.zeros(self.n, dtype=np.uint32)
+        self.tri9 = np.zeros(9 * self.n, dtype=np.float32)
+        self.faces = np.zeros(4 * self.n, dtype=np.uint32)
+        self.inst_words = np.zeros(L.orc_forest_inst_len(C.byref(self._in)), dtype=np.uint32)
+        self.lights = np.zeros(L.orc_forest_lights_len(C.byref(self._in)), dtype=np.uint32)
+        self.depths = np.zeros(len(self.meshes) + 1, dtype=np.uint32)
+        ext = C.c_float(0)
+        L.orc_forest_build(C.byref(self._in), leaf, ptr(self.nodes), ptr(self.order), ptr(self.tri9), ptr(self.faces),
+                           ptr(self.inst_words), ptr(self.lights), ptr(self.depths), C.byref(ext), nthreads)
+        self.extent = ext.value
+        self._finish()
+
+    def _finish(self):
+        m = int(self.lights[0])
+        self.lights_used = self.lights[: 1 + 3 * m].copy()
+        self.glow = m
+        self._c = OrcScene(ptr(self.nodes), len(self.nodes), ptr(self.tri9), ptr(self.faces), ptr(self.verts),
+                           ptr(self.mats), ptr(self.lights_used), ptr(self.inst_words), len(self.insts), 1)
+
+    def reinstance(self, worlds: np.ndarray) -> None:
+        """New matrices (pose.rs:107-124): the top tree, instance and light tables again."""
+        self.insts["world"] = np.asarray(worlds, dtype=np.float32).reshape(len(self.insts), 16)
+        d, ext = C.c_uint32(0), C.c_float(0)
+        lib().orc_forest_reinstance(C.byref(self._in), ptr(self.nodes), ptr(self.inst_words), ptr(self.lights),
+                                    ptr(self.faces), C.byref(d), C.byref(ext))
+        self.depths[-1] = d.value
+        self.extent = ext.value
+        self._finish()
+
+    def raycast(self, rays: np.ndarray, hit_dtype, any_hit: bool = False, nthreads: int = 1, stats: bool = False):
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), dtype=hit_dtype)
+        st = np.zeros(2 * len(rays), dtype=np.uint32)
+        lib().orc_trace_scene(C.byref(self._c), ptr(rays), len(rays), 1 if any_hit else 0, ptr(hits), ptr(st), nthreads)
+        return (hits, st) if stats else hits
+
+    def stamp(self, cam: np.ndarray, w: int, h: int, frame: int) -> np.ndarray:
+        c = np.ascontiguousarray(np.array(cam).reshape(1).copy())
+        lib().orc_stamp_camera(ptr(c), w, h, C.c_float(self.extent), self.glow, frame)
+        return c
+
+    def sample(self, cam: np.ndarray, w: int, h: int, px: np.ndarray, frame: int, nthreads: int = 1,
+               first: int = 0, count: int | None = None) -> tuple[int, int]:
+        c = self.stamp(cam, w, h, frame)
+        rays = np.zeros(2, dtype=np.uint64)
+        n = w * h
+        count = n - first if count is None else count
+        lib().orc_sample(C.byref(self._c), ptr(c), first, count, ptr(px), ptr(rays), nthreads)
+        return int(rays[0]), int(rays[1])
 
 
 class Scene:
@@ -180,7 +292,7 @@ class Scene:
         self.lights = self.lights[: 1 + 3 * m].copy()
         self.glow = m
         self._c = OrcScene(ptr(self.nodes), len(self.nodes), ptr(self.tri9), ptr(self.faces), ptr(self.verts),
-                           ptr(self.mats), ptr(self.lights))
+                           ptr(self.mats), ptr(self.lights), None, 0, 0)
 
     def stamp(self, cam: np.ndarray, w: int, h: int, frame: int) -> np.ndarray:
         c = np.ascontiguousarray(np.array(cam).reshape(1).copy())

@@ -15,12 +15,13 @@
 //   src/trace/wgsl/light.wgsl:1-5,65-67,87-106,118-167(lamp pick only),174-198,264-312
 //   src/trace/wgsl/brdf.wgsl:9-12,61-63,73-82,109-155,196-214,239-269
 //   src/trace/wgsl/procedural.wgsl:73-81 fresnel_dielectric_cos
-//   src/trace/wgsl/layout.wgsl:79-101,114-158  unoct, world_ng, tri_of
+//   src/trace/wgsl/layout.wgsl:62-158    inst_at, place_n, unoct, world_ng, tri_of
 //   src/trace/lamp.rs:51-97              the emitter table
 //   src/trace/scene.rs:22-38, scene/add.rs:9-69  vertex packing and sharing
 // Restated subset: opaque surfaces (trans = 0), alpha cut-outs, no textures,
-// procedurals, mix closures, subsurface, sheen or coat (all zero weight), one
-// identity instance, a flat world colour (no environment image), lamps only as
+// procedurals, mix closures, subsurface, sheen or coat (all zero weight),
+// instances placed by any matrix (a forest) or one identity instance, a flat
+// world colour (no environment image), lamps only as
 // unlit placeholders (`Camera::count` with zero colour), the glossy lobe off
 // (Max Bounces glossy 0), Pixel Filter 0 (phiray's own one-pixel box), no
 // adaptive sampling, no clamping, no grid overlay, a black sky.
@@ -89,13 +90,13 @@ struct Ctx {
     uint64_t closest = 0, shadow = 0;
 };
 
-// ---- walk (bvh.wgsl:194-232 over one identity instance) ----
+// ---- walk (bvh.wgsl:120-232: the top tree and the instances' trees, or one tree) ----
 Hit trace(Ctx& c, V3 o, V3 d, float tmax, bool first) {
     orc_ray r;
     r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z; r.tmax = tmax;
     r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r._pad = 0.f;
     orc_hit h;
-    orc_raycast_mat(c.s->nodes, c.s->tris, c.s->faces, c.s->mats, &r, 1, first ? 1 : 0, &h, nullptr, 1);
+    orc_trace_scene(c.s, &r, 1, first ? 1 : 0, &h, nullptr, 1);
     if (first) c.shadow++; else c.closest++;
     return Hit{h.t, h.tri, h.u, h.v, h.inst};
 }
@@ -123,7 +124,24 @@ V3 unoct(float ex, float ey) {  // layout.wgsl:79-86
     return normalize(n);
 }
 
-Tri tri_of(const orc_scene_t* s, uint32_t f, uint32_t m) {  // layout.wgsl:114-158, identity instance
+// Instance i's record, or none for a single-tree scene (one identity instance).
+inline const uint32_t* inst_at(const orc_scene_t* s, uint32_t i) {
+    return s->forest ? s->insts + ORC_INSTANCE_LEN * (size_t)i : nullptr;
+}
+
+// A vec4 dot as the kernel evaluates it: ((x + y) + z) + w.
+inline float dot4(const float* r, const float* p) { return ((r[0] * p[0] + r[1] * p[1]) + r[2] * p[2]) + r[3] * p[3]; }
+
+V3 place_n(const float* i0, const float* i1, const float* i2, V3 n) {  // layout.wgsl:70-77
+    V3 m = (ld3(i0) * n.x + ld3(i1) * n.y) + ld3(i2) * n.z;
+    float l = length(m);
+    if (l < 1e-12f) return v3(0, 0, 0);
+    return m / l;
+}
+
+// Face f under instance i with material m (layout.wgsl:114-158): its corners
+// and normals placed in the world unless the instance sits at the identity.
+Tri tri_of(const orc_scene_t* s, uint32_t f, uint32_t i, uint32_t m) {
     const uint32_t* fc = s->faces + 4 * (size_t)f;
     const float* x = s->verts + 8 * (size_t)fc[0];
     const float* y = s->verts + 8 * (size_t)fc[1];
@@ -132,6 +150,17 @@ Tri tri_of(const orc_scene_t* s, uint32_t f, uint32_t m) {  // layout.wgsl:114-1
     Tri t;
     t.a = ld3(x); t.b = ld3(y); t.c = ld3(z);
     t.na = unoct(x[4], x[5]); t.nb = unoct(y[4], y[5]); t.nc = unoct(z[4], z[5]);
+    const uint32_t* w = inst_at(s, i);
+    if (w && (w[27] & 1u) == 0u) {
+        const float* r = (const float*)w;
+        t.a = v3(dot4(r, x), dot4(r + 4, x), dot4(r + 8, x));
+        t.b = v3(dot4(r, y), dot4(r + 4, y), dot4(r + 8, y));
+        t.c = v3(dot4(r, z), dot4(r + 4, z), dot4(r + 8, z));
+        const float* iv = (const float*)(w + 12);
+        t.na = place_n(iv, iv + 4, iv + 8, t.na);
+        t.nb = place_n(iv, iv + 4, iv + 8, t.nb);
+        t.nc = place_n(iv, iv + 4, iv + 8, t.nc);
+    }
     for (int k = 0; k < 4; ++k) t.color[k] = mt[k];
     t.rough = mt[4]; t.metal = mt[5]; t.trans = mt[6]; t.ior = mt[7];
     t.emit = ld3(mt + 8);
@@ -311,7 +340,7 @@ Shadowed glow_query(const orc_scene_t* sc, const Surf& s, uint32_t* seed) {  // 
     float prev = lo > 0 ? bits_f(d[2 * count + lo]) : 0.0f;
     float chance = fmax_(bits_f(d[1 + 2 * count + lo]) - prev, 1e-8f);
     uint32_t f = d[1 + lo];
-    Tri t = tri_of(sc, f, mat_at(sc, f));
+    Tri t = tri_of(sc, f, d[1 + count + lo], mat_at(sc, f));
     float a = rnd(seed);
     float b = rnd(seed);
     if (a + b > 1.0f) { a = 1.0f - a; b = 1.0f - b; }
@@ -384,7 +413,7 @@ V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // integrate.w
             break;
         }
         if (!bounce) break;
-        Tri t = tri_of(c.s, h.tri, mat_at(c.s, h.tri));
+        Tri t = tri_of(c.s, h.tri, h.inst, mat_at(c.s, h.tri));
         V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
         V3 ng = dot(ng0, dir) < 0.f ? ng0 : -ng0;
         V3 hit = org + dir * h.t + ng * (p->eps + h.t * 1e-5f);
@@ -493,10 +522,13 @@ void shade(Ctx& c, uint32_t i, uint32_t* px) {
         }
         if (bounce == 0u) px[25 * n + i] = h.inst + 1u;
         uint32_t m = mat_at(sc, h.tri);
-        Tri t = tri_of(sc, h.tri, m);
+        Tri t = tri_of(sc, h.tri, h.inst, m);
         float w = 1.0f - h.u - h.v;
         V3 hit = o + d * h.t;
+        // world_ng (layout.wgsl:96-100): turned back under a mirroring instance.
         V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        const uint32_t* wi = inst_at(sc, h.inst);
+        if (wi && (wi[27] & 4u)) ng0 = -ng0;
         bool front = dot(ng0, d) < 0.f;
         V3 ng = front ? ng0 : -ng0;
         V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;

@@ -9,6 +9,7 @@
 // plus the deterministic sin/cos/exp2 the integrator uses (see below).
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -127,48 +128,50 @@ float orc_box_t(const orc_node* n, const float o[3], const float inv[3], float b
     return ORC_INF;
 }
 
-static void walk_one(const orc_node* nodes, const float* tris, const uint32_t* faces, const float* mats,
-                     const orc_ray& ray, bool any, orc_hit* best, uint32_t* st) {
-    // src/trace/wgsl/bvh.wgsl:120-190, one identity instance (bit 0 of word 27),
-    // root at node 0. An any-hit (shadow) ray passes glass and cut-outs: a
-    // triangle whose material has transmission > 0.5 or alpha < 0.5
-    // (bvh.wgsl:158-160); with no material table every triangle is opaque.
-    if (st) { st[0] = 0; st[1] = 0; }
-    best->t = ray.tmax; best->tri = ORC_NONE; best->u = 0.f; best->v = 0.f; best->inst = 0;
+// The stack walk of one tree from `root` (bvh.wgsl:142-189) in the space the
+// ray is given in, keeping the nearest hit in `best` (carried in, so a later
+// tree only takes strictly nearer hits) and naming instance `inst` in it.
+// An any-hit (shadow) ray passes glass and cut-outs: a triangle whose material
+// has transmission > 0.5 or alpha < 0.5 (bvh.wgsl:158-160); with no material
+// table every triangle is opaque. Returns true when an any-hit ray stopped.
+static bool walk_tree(const orc_node* nodes, uint32_t root, const float* tris, const uint32_t* faces, const float* mats,
+                      const float o[3], const float d[3], bool any, uint32_t inst, orc_hit* best, uint32_t* steps_io,
+                      uint32_t* tests_io) {
     float inv[3];
-    orc_safe_inv(ray.d, inv);
-    orc_node nd = nodes[0];
-    if (orc_box_t(&nd, ray.o, inv, best->t) >= ORC_INF) return;
+    orc_safe_inv(d, inv);
+    orc_node nd = nodes[root];
+    if (orc_box_t(&nd, o, inv, best->t) >= ORC_INF) return false;
     orc_rayspace rs;
-    orc_ray_space(ray.d, &rs);
+    orc_ray_space(d, &rs);
     uint32_t stack[ORC_STACK];
     uint32_t sp = 0;
     uint32_t steps = 0, tests = 0;
+    bool stopped = false;
     for (;;) {
         if (nd.lo.i & ORC_LEAF) {
             for (uint32_t id = nd.lo.i & ~ORC_LEAF; id < nd.hi.i; ++id) {
                 const float* t = tris + 9 * (size_t)id;
                 float h[3];
                 ++tests;
-                if (!orc_tri_t(t, t + 3, t + 6, ray.o, &rs, h)) continue;
+                if (!orc_tri_t(t, t + 3, t + 6, o, &rs, h)) continue;
                 if (h[0] < best->t) {
                     if (any) {
                         if (faces && mats) {
                             const float* m = mats + ORC_MAT_WORDS * (size_t)faces[4 * (size_t)id + 3];
                             if (m[6] > 0.5f || m[3] < 0.5f) continue;
                         }
-                        best->t = h[0]; best->tri = id; best->u = h[1]; best->v = h[2]; best->inst = 0;
-                        if (st) { st[0] = steps; st[1] = tests; }
-                        return;
+                        best->t = h[0]; best->tri = id; best->u = h[1]; best->v = h[2]; best->inst = inst;
+                        stopped = true;
+                        goto done;
                     }
-                    best->t = h[0]; best->tri = id; best->u = h[1]; best->v = h[2]; best->inst = 0;
+                    best->t = h[0]; best->tri = id; best->u = h[1]; best->v = h[2]; best->inst = inst;
                 }
             }
         } else {
             ++steps;
             orc_node nl = nodes[nd.lo.i], nr = nodes[nd.hi.i];
-            float tl = orc_box_t(&nl, ray.o, inv, best->t);
-            float tr = orc_box_t(&nr, ray.o, inv, best->t);
+            float tl = orc_box_t(&nl, o, inv, best->t);
+            float tr = orc_box_t(&nr, o, inv, best->t);
             if (tl < ORC_INF && tr < ORC_INF) {
                 if (tl <= tr) { stack[sp] = nd.hi.i; nd = nl; }
                 else { stack[sp] = nd.lo.i; nd = nr; }
@@ -182,7 +185,105 @@ static void walk_one(const orc_node* nodes, const float* tris, const uint32_t* f
         --sp;
         nd = nodes[stack[sp]];
     }
+done:
+    if (steps_io) *steps_io += steps;
+    if (tests_io) *tests_io += tests;
+    return stopped;
+}
+
+static void walk_one(const orc_node* nodes, const float* tris, const uint32_t* faces, const float* mats,
+                     const orc_ray& ray, bool any, orc_hit* best, uint32_t* st) {
+    // src/trace/wgsl/bvh.wgsl:120-190, one identity instance (bit 0 of word 27),
+    // root at node 0.
+    best->t = ray.tmax; best->tri = ORC_NONE; best->u = 0.f; best->v = 0.f; best->inst = 0;
+    uint32_t steps = 0, tests = 0;
+    walk_tree(nodes, 0, tris, faces, mats, ray.o, ray.d, any, 0, best, &steps, &tests);
     if (st) { st[0] = steps; st[1] = tests; }
+}
+
+// A vec4 dot as the kernel evaluates it: ((x + y) + z) + w.
+static inline float dot4(const float* r, float x, float y, float z, float w) { return ((r[0] * x + r[1] * y) + r[2] * z) + r[3] * w; }
+
+// trace_blas (bvh.wgsl:120-190): instance i's tree in its own space. The ray
+// goes there by the inverse (direction left unnormalised, so t is the
+// world's), unless the instance sits at the identity (word 27 bit 0); a
+// shadow ray passes an instance that casts none (bit 1).
+static bool trace_blas(const orc_scene_t* s, uint32_t i, const float ow[3], const float dw[3], bool any, orc_hit* best,
+                       uint32_t* steps, uint32_t* tests) {
+    const uint32_t* w = s->insts + ORC_INSTANCE_LEN * (size_t)i;
+    const uint32_t root = w[24];
+    if (root == ORC_NONE) return false;
+    if (any && (w[27] & 2u)) return false;
+    float o[3] = {ow[0], ow[1], ow[2]}, d[3] = {dw[0], dw[1], dw[2]};
+    if ((w[27] & 1u) == 0u) {
+        const float* r = (const float*)(w + 12);
+        for (int k = 0; k < 3; ++k) {
+            o[k] = dot4(r + 4 * k, ow[0], ow[1], ow[2], 1.0f);
+            d[k] = (r[4 * k] * dw[0] + r[4 * k + 1] * dw[1]) + r[4 * k + 2] * dw[2];
+        }
+    }
+    return walk_tree(s->nodes, root, s->tris, s->faces, s->mats, o, d, any, i, best, steps, tests);
+}
+
+// trace (bvh.wgsl:194-232): the top tree over the instances' world boxes; a
+// leaf's slots name instances (inst_order) whose own trees the ray walks.
+static void trace_forest(const orc_scene_t* s, const orc_ray& ray, bool any, orc_hit* best, uint32_t* st) {
+    best->t = ray.tmax; best->tri = ORC_NONE; best->u = 0.f; best->v = 0.f; best->inst = 0;
+    uint32_t steps = 0, tests = 0;
+    const uint32_t* inst_order = s->insts + ORC_INSTANCE_LEN * (size_t)s->ninsts;
+    float inv[3];
+    orc_safe_inv(ray.d, inv);
+    orc_node nd = s->nodes[0];
+    if (orc_box_t(&nd, ray.o, inv, best->t) < ORC_INF) {
+        uint32_t stack[ORC_STACK];
+        uint32_t sp = 0;
+        for (;;) {
+            if (nd.lo.i & ORC_LEAF) {
+                bool stop = false;
+                for (uint32_t k = nd.lo.i & ~(ORC_LEAF | ORC_INST); k < nd.hi.i; ++k) {
+                    trace_blas(s, inst_order[k], ray.o, ray.d, any, best, &steps, &tests);
+                    if (any && best->tri != ORC_NONE) { stop = true; break; }
+                }
+                if (stop) break;
+            } else {
+                ++steps;
+                orc_node nl = s->nodes[nd.lo.i], nr = s->nodes[nd.hi.i];
+                float tl = orc_box_t(&nl, ray.o, inv, best->t);
+                float tr = orc_box_t(&nr, ray.o, inv, best->t);
+                if (tl < ORC_INF && tr < ORC_INF) {
+                    if (tl <= tr) { stack[sp] = nd.hi.i; nd = nl; }
+                    else { stack[sp] = nd.lo.i; nd = nr; }
+                    ++sp;
+                    continue;
+                }
+                if (tl < ORC_INF) { nd = nl; continue; }
+                if (tr < ORC_INF) { nd = nr; continue; }
+            }
+            if (sp == 0) break;
+            --sp;
+            nd = s->nodes[stack[sp]];
+        }
+    }
+    if (st) { st[0] = steps; st[1] = tests; }
+}
+
+void orc_trace_scene(const orc_scene_t* s, const orc_ray* rays, uint32_t nrays, int any_hit, orc_hit* hits,
+                     uint32_t* stats, int threads) {
+    auto run = [&](uint32_t a, uint32_t b) {
+        for (uint32_t i = a; i < b; ++i) {
+            uint32_t* st = stats ? stats + 2 * (size_t)i : nullptr;
+            if (s->forest) trace_forest(s, rays[i], any_hit != 0, &hits[i], st);
+            else walk_one(s->nodes, s->tris, s->faces, s->mats, rays[i], any_hit != 0, &hits[i], st);
+        }
+    };
+    if (threads <= 1 || nrays < 1024) { run(0, nrays); return; }
+    std::vector<std::thread> pool;
+    uint32_t per = (nrays + threads - 1) / threads;
+    for (int k = 0; k < threads; ++k) {
+        uint32_t a = k * per, b = std::min<uint32_t>(nrays, a + per);
+        if (a < b) pool.emplace_back(run, a, b);
+    }
+    for (auto& th : pool) th.join();
 }
 
 void orc_raycast(const orc_node* nodes, const float* tris, const orc_ray* rays, uint32_t nrays,

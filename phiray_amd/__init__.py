@@ -40,6 +40,21 @@ CAMERA = np.dtype([
 assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
 assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 176
 
+# A scene of meshes under instances (src/trace/scene.rs:40-67, mesh.rs:12-22,
+# instance.rs:23-44): a vertex is its object-space position (w 1) then its
+# packed normal and uv; a face three vertex ids and its material; a material
+# 16 floats (color rgba, rough, metal, trans, ior, emit rgb, bump, 0 x 4); an
+# instance's world matrix is m[col][row], translation in m[3].
+VERT = np.dtype([("p", "<f4", 4), ("a", "<f4", 4)])
+FACE = np.dtype([("v", "<u4", 3), ("mat", "<u4")])
+MAT = np.dtype([("color", "<f4", 4), ("rough", "<f4"), ("metal", "<f4"), ("trans", "<f4"), ("ior", "<f4"),
+                ("emit", "<f4", 3), ("bump", "<f4"), ("_pad", "<f4", 4)])
+MESH = np.dtype([("face_lo", "<u4"), ("face_hi", "<u4"), ("vert_lo", "<u4"), ("vert_hi", "<u4"), ("lo", "<f4", 3),
+                 ("hi", "<f4", 3)])
+INSTANCE = np.dtype([("world", "<f4", 16), ("mesh", "<u4"), ("casts_shadow", "<u4"), ("color", "<f4", 3),
+                     ("hash", "<u4")])
+INSTANCE_LEN = 40  # words of an instance record (src/trace/instance.rs:18)
+INST = 0x40000000  # a top-tree leaf (src/trace/forest.rs:17)
 LEAF = 0x80000000
 NONE = 0xFFFFFFFF
 STACK = 32
@@ -95,6 +110,11 @@ _SIGS = {
     "phr_bvh_update": (_I, [_P, _P, _U32, C.POINTER(_I)]),
     "phr_bvh_download": (_I, [_P, _P, _P, C.POINTER(_U32)]),
     "phr_bvh_codes": (_I, [_P, _P]),
+    "phr_bvh_build_batch": (_I, [_P, _P, _U32, _P, _U32, _U32, C.POINTER(_P)]),
+    "phr_bvh_build_batch_on": (_I, [_P, _P, _P, _U32, _U32, C.POINTER(_P)]),
+    "phr_bvh_download_segment": (_I, [_P, _U32, _P, _P, C.POINTER(_U32)]),
+    "phr_bvh_segments": (_U32, [_P]),
+    "phr_bvh_segment_depth": (_U32, [_P, _U32]),
     "phr_bvh_len": (_U32, [_P]),
     "phr_bvh_depth": (_U32, [_P]),
     "phr_bvh_leaf": (_U32, [_P]),
@@ -387,6 +407,48 @@ class GpuBvh:
         b = cls(gpu, out.value)
         b._prims = prims  # keep the borrowed array alive
         return b
+
+    @classmethod
+    def build_batch(cls, gpu: Gpu, prims: np.ndarray, seg_lo, leaf: int = 4) -> "GpuBvh":
+        """Many trees in one build: segment s is prims[seg_lo[s]:seg_lo[s+1]]."""
+        prims = np.ascontiguousarray(prims, dtype=AABB)
+        lo = np.ascontiguousarray(seg_lo, dtype=np.uint32)
+        out = C.c_void_p()
+        _check(lib().phr_bvh_build_batch(gpu._h, _ptr(prims), len(prims), _ptr(lo), len(lo) - 1, leaf, C.byref(out)),
+               "GpuBvh::build_batch")
+        b = cls(gpu, out.value)
+        b._seg_lo = lo
+        return b
+
+    @classmethod
+    def build_batch_on(cls, gpu: Gpu, prims: GpuArray, seg_lo, leaf: int = 4) -> "GpuBvh":
+        lo = np.ascontiguousarray(seg_lo, dtype=np.uint32)
+        out = C.c_void_p()
+        _check(lib().phr_bvh_build_batch_on(gpu._h, prims._h, _ptr(lo), len(lo) - 1, leaf, C.byref(out)), "GpuBvh::build_batch_on")
+        b = cls(gpu, out.value)
+        b._prims, b._seg_lo = prims, lo
+        return b
+
+    @property
+    def segments(self) -> int:
+        return int(lib().phr_bvh_segments(self._h))
+
+    def segment_depth(self, s: int) -> int:
+        return int(lib().phr_bvh_segment_depth(self._h, s))
+
+    def download_segment(self, s: int) -> Bvh:
+        """Segment s's tree relative to itself (what GpuBvh::download gives for its boxes alone)."""
+        lo = getattr(self, "_seg_lo", np.array([0, len(self)], dtype=np.uint32))
+        m = int(lo[s + 1]) - int(lo[s])
+        nodes = np.zeros(max(2 * m - 1, 0), dtype=NODE)
+        order = np.zeros(m, dtype=np.uint32)
+        depth = C.c_uint32(0)
+        _check(lib().phr_bvh_download_segment(self._h, s, _ptr(nodes), _ptr(order), C.byref(depth)), "GpuBvh::download_segment")
+        return Bvh(nodes, order, int(depth.value))
+
+    def nodes_raw(self) -> np.ndarray:
+        """The whole node array as the device holds it."""
+        return GpuArray._borrow(self.gpu, lib().phr_bvh_nodes(self._h), NODE).read()
 
     @property
     def depth(self) -> int:

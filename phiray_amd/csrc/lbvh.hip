@@ -1,36 +1,81 @@
 // GPU LBVH for gfx950: scene bounds, 30-bit Morton keys, a wave64 onesweep
-// radix sort of (code, slot), Karras links, leaf packing and an atomic-counter
+// radix sort of (code, slot), Karras links, leaf packing and a bottom-up
 // refit into phiray's 32-byte Node layout.
 //
 // The result is the tree phiray's CPU builder makes (src/bvh/build.rs:94-100,
-// Bvh::build): the same order, links, depth and reachable-node bounds. The
-// stages and why each looks the way it does:
+// Bvh::build): the same order, links, depth and reachable-node bounds. One
+// build may hold many trees ("segments"): the boxes come concatenated, a
+// table of segment starts splits them, and every segment gets the tree
+// Bvh::build would give its boxes alone, with its own scene box. That is one
+// tree per mesh of a Forest (src/trace/forest.rs:44-66) or a batch of
+// independent scenes, in the same launches as a single tree.
 //
-//  1. bounds   one pass over the boxes; blocks reduce in registers and LDS, the
-//              last block to finish folds the partials and writes the scene
+// Ids. Slots are positions in the sorted order, global over the segments.
+// The inner node of a segment's Karras tree numbered i (src/bvh/build.rs:
+// 28-61, one end of its range is slot i) has id `lo + i`, where lo is the
+// segment's first slot; its leaf k has id `n + lo + k`. So an inner node's id
+// is a slot inside its own range, whatever segment it lies in. The node array
+// written out keeps phiray's layout per tree: segment s's tree starts at a
+// caller-given `node_base[s]`, inner node i at node_base + i, leaf k at
+// node_base + n_s - 1 + k, children as node indices and leaf ranges as
+// global slots (what Forest::nodes makes of the mesh trees).
+//
+// The stages and why each looks the way it does:
+//
+//  1. bounds   each wave takes a contiguous run of boxes and keeps a running
+//              box while the segment stays the same; at a segment change it
+//              folds the box into that segment's six words with atomic
+//              min/max on an order-preserving integer image of the floats.
+//              No block waits on another and no cache-wide fence is needed
+//              (min and max are exact, so the fold order does not matter).
+//              The Morton launch reads the words back and forms each scene
 //              box and its per-axis reciprocal (src/bvh/build.rs:70-80). The
 //              same launch clears the sort's counters.
 //  2. morton   the code of every box centre (src/bvh/node.rs:96-105) and, in
-//              the same pass, the histograms of all four 8-bit digits.
-//  3. sort x4  onesweep LSD passes (bits 0-7, 8-15, 16-23, 24-29): a tile of
-//              2048 keys ranks its digits with wave64 ballots into per-wave LDS
-//              counters, publishes its per-digit counts and looks back over
-//              earlier tiles for its offsets (tiles take ids from a ticket so
-//              an earlier tile is always resident), then scatters through LDS so
-//              each digit's run leaves in one contiguous write. Every pass is
-//              stable and pass 0 starts from the slot order, so the result is
-//              the sort by (code, slot) that src/bvh/build.rs:87-92 asks for.
-//  4. karras   one thread per internal node: direction, exponential and binary
-//              search, split by clz of XOR'd keys (src/bvh/build.rs:28-61), and
-//              the parent of each child.
-//  5. pack     one thread per node climbs its parents (at most 32 steps) for its
-//              depth; a subtree of fewer than `leaf + 1` slots or at depth 31
-//              becomes a leaf (src/bvh/build.rs:139-168), whatever lies under a
-//              packed node is unreachable, and the reachable leaves are listed.
-//  6. refit    one thread per listed leaf: its box from its primitives, then up
-//              the parents; at each inner node the second of its two children
-//              to arrive (an atomic counter) unions both and carries on, and
-//              resets the counter for the next refit (src/bvh/build.rs:180-194).
+//              the same pass, the histograms of every digit the sort uses.
+//  3. sort     onesweep LSD passes over the code (bits 0-7, 8-15, 16-23,
+//              24-29), then over the segment id when there is more than one:
+//              a tile of 2048 keys ranks its digits with wave64 ballots into
+//              per-wave LDS counters, publishes its per-digit counts and looks
+//              back over earlier tiles for its offsets (tiles take ids from a
+//              ticket so an earlier tile is always resident), then scatters
+//              through LDS so each digit's run leaves in one contiguous write.
+//              Every pass is stable and the first starts from the input order,
+//              so the result is the sort by (segment, code, index) — within a
+//              segment the sort by (code, slot) of src/bvh/build.rs:87-92.
+//  4. karras   one thread per slot: direction, exponential and binary search,
+//              split by clz of XOR'd keys within the slot's segment
+//              (src/bvh/build.rs:28-61), and the parent of each child.
+//  5. pack     every node's depth without walking the tree: the ranges of a
+//              tree's inner nodes nest, so the inner ranges covering slot k
+//              are exactly leaf k's ancestors, and those covering the edge
+//              between slots g and g + 1 are exactly the inner node split at g
+//              and its ancestors. Every inner range adds +1 where it starts
+//              and -1 where it ends; one prefix sum of those (a chained scan)
+//              counts the ranges covering each edge (a segment's terms sum to
+//              zero, so one scan serves all of them). An inner node's depth is
+//              its split edge's count less one; a leaf's is the larger count
+//              of the two edges beside it (its parent is the deeper of the two
+//              nodes split there). One end of a Karras node's range is its own
+//              slot, so the Karras launch stores that term and adds only the
+//              far end's with an atomic. Then one thread per node: a subtree of
+//              fewer than `leaf + 1` slots or at depth 31 becomes a leaf
+//              (src/bvh/build.rs:139-168), whatever lies under a packed node is
+//              unreachable, and each reachable leaf is entered in a slot table
+//              under the first slot of its range.
+//  6. refit    bottom-up (src/bvh/build.rs:180-194) in levels over ranges of
+//              slots. A level-0 block owns 2048 slots: one thread per leaf
+//              starting there computes the leaf box from its primitives and
+//              climbs; at an inner node the second of its two children to
+//              arrive (a workgroup-scope counter in LDS) unions both and goes
+//              on. A climb stops at a node whose parent's range leaves the
+//              block; that node is handed to the next level, whose blocks own
+//              64 times as many slots and carry those climbs on the same way,
+//              until one block owns every slot. Every inner node's range is
+//              inside the block that finishes it, so all synchronisation stays
+//              inside one workgroup: no agent-scope acquire or release, which
+//              on gfx950 writes back and invalidates the XCD's L2 at every
+//              step of every climb.
 #include "phr_runtime.h"
 
 #include <algorithm>
@@ -43,19 +88,31 @@ constexpr uint32_t OS_THREADS = 256;
 constexpr uint32_t OS_ITEMS = 8;
 constexpr uint32_t OS_TILE = OS_THREADS * OS_ITEMS;  // 2048 keys a tile
 constexpr uint32_t OS_WAVES = OS_THREADS / 64;
-constexpr uint32_t OS_PASSES = 4;
+constexpr uint32_t CODE_PASSES = 4;
+constexpr uint32_t MAX_PASSES = 6;  // four over the code, up to two over the segment id
+constexpr uint32_t MAX_SEGMENTS = 1u << 16;
 constexpr uint32_t FLAG_AGG = 0x40000000u;
 constexpr uint32_t FLAG_PRE = 0x80000000u;
 constexpr uint32_t VAL_MASK = 0x3fffffffu;
 constexpr uint32_t BOUNDS_BLOCKS_MAX = 512;
+constexpr uint32_t KEY_MAX = 0xff7fffffu;  // fkey(f32::MAX)
+constexpr uint32_t KEY_MIN = 0x00800000u;  // fkey(f32::MIN)
 
-// Words at the front of the cleared block: 4 x 256 digit counts, then the
-// pass tickets, the leaf count and the depth; the look-back status follows.
+// Words at the front of the cleared block: MAX_PASSES x 256 digit counts, the
+// pass tickets, the scan's ticket and one depth per segment; the look-back
+// status follows.
 constexpr uint32_t Z_HIST = 0;
-constexpr uint32_t Z_TICKET = 1024;
-constexpr uint32_t Z_LEAVES = 1028;
-constexpr uint32_t Z_DEPTH = 1029;
-constexpr uint32_t Z_STATUS = 1088;
+constexpr uint32_t Z_TICKET = MAX_PASSES * 256;
+constexpr uint32_t Z_SCAN_TICKET = Z_TICKET + 8;
+constexpr uint32_t Z_DEPTH = Z_SCAN_TICKET + 8;
+
+// Refit levels: slots owned by a level-0 block, and blocks of one level per
+// block of the next. A level's inner-node counters live in LDS (2 bits each)
+// while its block owns at most RF_LDS_SLOTS slots, in global memory above.
+constexpr uint32_t RF_THREADS = 256;
+constexpr uint32_t RF_SLOTS = 2048;
+constexpr uint32_t RF_GROUP = 64;
+constexpr uint32_t RF_LDS_SLOTS = RF_SLOTS * RF_GROUP;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = __lane_id();
@@ -91,60 +148,105 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return add + x - v;
 }
 
-// ---- 1. scene bounds ------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ prims, uint32_t n, float* __restrict__ partials,
-                                                float* __restrict__ scene, uint32_t* __restrict__ arrive,
-                                                uint32_t* __restrict__ zero, uint32_t zero_words) {
-    __shared__ float s_red[6][4];
-    __shared__ uint32_t s_last;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t gtid = blockIdx.x * blockDim.x + tid, gsize = gridDim.x * blockDim.x;
-    for (uint32_t i = gtid; i < zero_words; i += gsize) zero[i] = 0u;
-    // Aabb::default() is (f32::MAX, f32::MIN), src/bvh/node.rs:67-71.
-    float lo0 = 3.40282347e38f, lo1 = 3.40282347e38f, lo2 = 3.40282347e38f;
-    float hi0 = -3.40282347e38f, hi1 = -3.40282347e38f, hi2 = -3.40282347e38f;
-    for (uint32_t i = gtid; i < n; i += gsize) {
-        float4 a = prims[2 * i], b = prims[2 * i + 1];
-        lo0 = fminf(lo0, a.x); lo1 = fminf(lo1, a.y); lo2 = fminf(lo2, a.z);
-        hi0 = fmaxf(hi0, b.x); hi1 = fmaxf(hi1, b.y); hi2 = fmaxf(hi2, b.z);
+// The segment holding slot (or input index) p: the last s with seg_lo[s] <= p.
+// An empty segment shares its start with the next, so the search lands on the
+// segment that holds p.
+__device__ __forceinline__ uint32_t seg_of_slot(const uint32_t* __restrict__ seg_lo, uint32_t segs, uint32_t p) {
+    uint32_t lo = 0, hi = segs;  // seg_lo[lo] <= p < seg_lo[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (seg_lo[mid] <= p) lo = mid; else hi = mid;
     }
-    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
-    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
-    if (lane == 0) {
-        s_red[0][wave] = lo0; s_red[1][wave] = lo1; s_red[2][wave] = lo2;
-        s_red[3][wave] = hi0; s_red[4][wave] = hi1; s_red[5][wave] = hi2;
-    }
-    __syncthreads();
-    if (tid < 6) {
-        float v = s_red[tid][0];
-        for (int w = 1; w < 4; ++w) v = tid < 3 ? fminf(v, s_red[tid][w]) : fmaxf(v, s_red[tid][w]);
-        partials[8 * blockIdx.x + tid] = v;
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(arrive, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (tid < 6) {
-        float v = tid < 3 ? 3.40282347e38f : -3.40282347e38f;
-        for (uint32_t b = 0; b < gridDim.x; ++b) {
-            float p = partials[8 * b + tid];
-            v = tid < 3 ? fminf(v, p) : fmaxf(v, p);
-        }
-        s_red[tid][0] = v;
-    }
-    __syncthreads();
-    if (tid < 3) {
-        float lo = s_red[tid][0], hi = s_red[tid + 3][0];
-        float e = hi - lo;
-        scene[tid] = lo;
-        scene[4 + tid] = e > 0.f ? 1.0f / e : 0.f;  // src/bvh/build.rs:76-80
-    }
-    if (tid == 0) *arrive = 0u;
+    return lo;
 }
 
-// ---- 2. Morton codes + all four digit histograms ---------------------------
+// A segment as the per-node kernels need it.
+struct Seg {
+    uint32_t lo, n, base;
+};
+__device__ __forceinline__ Seg seg_at(const uint32_t* __restrict__ seg_lo, const uint32_t* __restrict__ node_base, uint32_t s) {
+    return Seg{seg_lo[s], seg_lo[s + 1] - seg_lo[s], node_base[s]};
+}
+// Where node `id` (inner: lo + i; leaf: n + lo + k) lands in the node array.
+__device__ __forceinline__ uint32_t out_of(uint32_t id, uint32_t n, const Seg& g) {
+    return id < n ? g.base + (id - g.lo) : g.base + (g.n - 1) + (id - n - g.lo);
+}
+
+// ---- 1. scene bounds ------------------------------------------------------
+// Order-preserving integer image of a float: for non-NaN a, b,
+// a < b exactly when fkey(a) < fkey(b) as unsigned integers.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unfkey(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+
+struct Box {
+    float lo0, lo1, lo2, hi0, hi1, hi2;
+};
+__device__ __forceinline__ Box empty_box() {
+    // Aabb::default() is (f32::MAX, f32::MIN), src/bvh/node.rs:67-71.
+    return Box{3.40282347e38f, 3.40282347e38f, 3.40282347e38f, -3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+}
+__device__ __forceinline__ void grow(Box& b, float4 lo, float4 hi) {
+    b.lo0 = fminf(b.lo0, lo.x); b.lo1 = fminf(b.lo1, lo.y); b.lo2 = fminf(b.lo2, lo.z);
+    b.hi0 = fmaxf(b.hi0, hi.x); b.hi1 = fmaxf(b.hi1, hi.y); b.hi2 = fmaxf(b.hi2, hi.z);
+}
+
+// Fold the wave's running box into segment s's words (all lanes call it).
+__device__ __forceinline__ void fold_box(Box b, uint32_t* __restrict__ acc, uint32_t s) {
+    b.lo0 = wave_min(b.lo0); b.lo1 = wave_min(b.lo1); b.lo2 = wave_min(b.lo2);
+    b.hi0 = wave_max(b.hi0); b.hi1 = wave_max(b.hi1); b.hi2 = wave_max(b.hi2);
+    if (__lane_id() == 0) {
+        uint32_t* a = acc + 6 * s;
+        atomicMin(&a[0], fkey(b.lo0)); atomicMin(&a[1], fkey(b.lo1)); atomicMin(&a[2], fkey(b.lo2));
+        atomicMax(&a[3], fkey(b.hi0)); atomicMax(&a[4], fkey(b.hi1)); atomicMax(&a[5], fkey(b.hi2));
+    }
+}
+
+// acc[6s..6s+2] = fkey of segment s's low corner, acc[6s+3..6s+5] of its high
+// corner. They hold fkey(f32::MAX) and fkey(f32::MIN) between builds; the
+// packing launch restores them after the Morton launch has read them.
+__global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ prims, uint32_t n, const uint32_t* __restrict__ seg_lo,
+                                                uint32_t segs, uint32_t* __restrict__ acc, uint32_t* __restrict__ zero,
+                                                uint32_t zero_words) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t gtid = blockIdx.x * blockDim.x + tid, gsize = gridDim.x * blockDim.x;
+    for (uint32_t i = gtid; i < zero_words; i += gsize) zero[i] = 0u;
+    const uint32_t waves = gsize / 64, gw = gtid / 64;
+    const uint32_t chunk = ((n + waves - 1) / waves + 63) & ~63u;
+    const uint32_t c0 = gw * chunk, c1 = min(c0 + chunk, n);
+    if (c0 >= n) return;  // whole waves only
+    uint32_t cur = seg_of_slot(seg_lo, segs, c0), cur_end = seg_lo[cur + 1];
+    Box b = empty_box();
+    for (uint32_t base = c0; base < c1; base += 64) {
+        const uint32_t p = base + lane;
+        const bool active = p < c1;
+        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+        if (active) { lo = prims[2 * p]; hi = prims[2 * p + 1]; }
+        if (min(base + 64, c1) <= cur_end) {
+            if (active) grow(b, lo, hi);
+            continue;
+        }
+        // A segment ends inside this run of 64: fold as each one closes.
+        bool done = !active;
+        for (;;) {
+            const bool mine = !done && p < cur_end;
+            if (mine) grow(b, lo, hi);
+            done = done || mine;
+            const uint64_t left = __ballot(!done);
+            if (left == 0) break;
+            fold_box(b, acc, cur);
+            b = empty_box();
+            const uint32_t first = base + (uint32_t)__ffsll((unsigned long long)left) - 1;
+            cur = seg_of_slot(seg_lo, segs, first);
+            cur_end = seg_lo[cur + 1];
+        }
+    }
+    fold_box(b, acc, cur);
+}
+
+// ---- 2. Morton codes + the digit histograms --------------------------------
 __device__ __forceinline__ uint32_t spread10(float v) {
     // src/bvh/node.rs:97-103: clamp to [0, 1], times 1023, truncated; NaN -> 0.
     float c = v;
@@ -157,47 +259,79 @@ __device__ __forceinline__ uint32_t spread10(float v) {
     return (x | (x << 2)) & 0x09249249u;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ prims, const float* __restrict__ scene, uint32_t n,
-                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_h[OS_PASSES * 256];
+__device__ __forceinline__ float recip_extent(float lo, float hi) {
+    float e = hi - lo;
+    return e > 0.f ? 1.0f / e : 0.f;  // src/bvh/build.rs:76-80
+}
+
+// seg_of (input index -> segment) is written only when the sort has segment
+// passes (seg_passes > 0).
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ prims, const uint32_t* __restrict__ acc,
+                                                const uint32_t* __restrict__ seg_lo, uint32_t segs, uint32_t seg_passes, uint32_t n,
+                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ hist,
+                                                uint32_t* __restrict__ far_ends, uint32_t* __restrict__ seg_of) {
+    __shared__ uint32_t s_h[MAX_PASSES * 256];
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < OS_PASSES * 256; i += 256) s_h[i] = 0u;
+    for (uint32_t i = tid; i < MAX_PASSES * 256; i += 256) s_h[i] = 0u;
     __syncthreads();
-    const float slo0 = scene[0], slo1 = scene[1], slo2 = scene[2];
-    const float inv0 = scene[4], inv1 = scene[5], inv2 = scene[6];
     const uint32_t base = blockIdx.x * OS_TILE;
+    uint32_t s = base < n ? seg_of_slot(seg_lo, segs, base) : 0u;
+    uint32_t s_end = seg_lo[s + 1];
+    float slo0 = 0.f, slo1 = 0.f, slo2 = 0.f, inv0 = 0.f, inv1 = 0.f, inv2 = 0.f;
+    uint32_t have = PHR_NONE;
 #pragma unroll
     for (uint32_t r = 0; r < OS_ITEMS; ++r) {
         uint32_t i = base + r * 256 + tid;
         if (i < n) {
-            float4 a = prims[2 * i], b = prims[2 * i + 1];
+            while (i >= s_end) {  // runs of slots only move forward
+                ++s;
+                s_end = seg_lo[s + 1];
+            }
+            if (have != s) {
+                const uint32_t* a = acc + 6 * s;
+                slo0 = unfkey(a[0]); slo1 = unfkey(a[1]); slo2 = unfkey(a[2]);
+                inv0 = recip_extent(slo0, unfkey(a[3]));
+                inv1 = recip_extent(slo1, unfkey(a[4]));
+                inv2 = recip_extent(slo2, unfkey(a[5]));
+                have = s;
+            }
+            float4 lo = prims[2 * i], hi = prims[2 * i + 1];
             // Aabb::center then (c - s.lo) * inv, src/bvh/build.rs:81-84
-            float c0 = 0.5f * (a.x + b.x), c1 = 0.5f * (a.y + b.y), c2 = 0.5f * (a.z + b.z);
+            float c0 = 0.5f * (lo.x + hi.x), c1 = 0.5f * (lo.y + hi.y), c2 = 0.5f * (lo.z + hi.z);
             float u0 = (c0 - slo0) * inv0, u1 = (c1 - slo1) * inv1, u2 = (c2 - slo2) * inv2;
             uint32_t code = (spread10(u0) << 2) | (spread10(u1) << 1) | spread10(u2);
             codes[i] = code;
+            far_ends[i] = 0u;
             atomicAdd(&s_h[code & 0xFFu], 1u);
             atomicAdd(&s_h[256 + ((code >> 8) & 0xFFu)], 1u);
             atomicAdd(&s_h[512 + ((code >> 16) & 0xFFu)], 1u);
             atomicAdd(&s_h[768 + (code >> 24)], 1u);
+            if (seg_passes) {
+                seg_of[i] = s;
+                atomicAdd(&s_h[1024 + (s & 0xFFu)], 1u);
+                if (seg_passes > 1) atomicAdd(&s_h[1280 + ((s >> 8) & 0xFFu)], 1u);
+            }
         }
     }
     __syncthreads();
-    for (uint32_t i = tid; i < OS_PASSES * 256; i += 256)
+    for (uint32_t i = tid; i < MAX_PASSES * 256; i += 256)
         if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
 // ---- 3. onesweep pass ------------------------------------------------------
-template <bool FIRST>
+// FIRST: the values are the input indices. SEGD: the digit is a byte of the
+// value's segment id instead of a byte of the key (the key rides along).
+template <bool FIRST, bool SEGD>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                  const uint32_t* __restrict__ hist, uint32_t* status,
-                                                  uint32_t* ticket, uint32_t n, uint32_t shift) {
+                                                  const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* ticket,
+                                                  uint32_t n, uint32_t shift, const uint32_t* __restrict__ seg_of) {
     __shared__ uint32_t s_whist[OS_WAVES][256];
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
     __shared__ uint32_t s_keys[OS_TILE];
     __shared__ uint32_t s_vals[OS_TILE];
+    __shared__ uint8_t s_dig[OS_TILE];
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_tile;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -206,20 +340,22 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t wbase = tile * OS_TILE + wave * (OS_ITEMS * 64);
-    uint32_t key[OS_ITEMS], val[OS_ITEMS], rank[OS_ITEMS];
+    uint32_t key[OS_ITEMS], val[OS_ITEMS], rank[OS_ITEMS], dig[OS_ITEMS];
 #pragma unroll
     for (uint32_t r = 0; r < OS_ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         bool ok = idx < n;
         key[r] = ok ? kin[idx] : 0u;
         val[r] = FIRST ? idx : (ok ? vin[idx] : 0u);
+        if constexpr (SEGD) dig[r] = ok ? (seg_of[val[r]] >> shift) & 0xFFu : 0u;
+        else dig[r] = (key[r] >> shift) & 0xFFu;
     }
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (uint32_t r = 0; r < OS_ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         bool ok = idx < n;
-        uint32_t d = (key[r] >> shift) & 0xFFu;
+        uint32_t d = dig[r];
         uint64_t mask = __ballot(ok);
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b) {
@@ -263,10 +399,11 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     for (uint32_t r = 0; r < OS_ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         if (idx < n) {
-            uint32_t dd = (key[r] >> shift) & 0xFFu;
+            uint32_t dd = dig[r];
             uint32_t lp = s_local[dd] + s_whist[wave][dd] + rank[r];
             s_keys[lp] = key[r];
             s_vals[lp] = val[r];
+            s_dig[lp] = (uint8_t)dd;
         }
     }
     __syncthreads();
@@ -275,178 +412,409 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     for (uint32_t r = 0; r < OS_ITEMS; ++r) {
         uint32_t idx = r * 256 + tid;
         if (idx < count) {
-            uint32_t k = s_keys[idx];
-            uint32_t dd = (k >> shift) & 0xFFu;
+            uint32_t dd = s_dig[idx];
             uint32_t dest = s_global[dd] + (idx - s_local[dd]);
-            kout[dest] = k;
+            kout[dest] = s_keys[idx];
             vout[dest] = s_vals[idx];
         }
     }
 }
 
 // ---- 4. Karras links --------------------------------------------------------
+// delta over the segment's own slots (src/bvh/mod.rs:33-43): i, j local.
 __device__ __forceinline__ int32_t kdelta(const uint32_t* __restrict__ codes, int64_t n, int64_t i, int64_t j) {
-    // src/bvh/mod.rs:33-43
     if (j < 0 || j >= n) return -1;
     uint32_t a = codes[i], b = codes[j];
     if (a != b) return (int32_t)__clz(a ^ b);
     return 32 + (int32_t)__clz((uint32_t)i ^ (uint32_t)j);
 }
 
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t n32, PhrLink* __restrict__ links,
-                                                uint32_t* __restrict__ parent) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k + 1 >= n32) return;
-    const int64_t n = n32, i = k;
-    int64_t d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1) > 0) ? 1 : -1;
-    int32_t dmin = kdelta(codes, n, i, i - d);
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t n, const uint32_t* __restrict__ seg_lo,
+                                                uint32_t segs, PhrLink* __restrict__ links, uint32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ leaf_at, uint32_t* __restrict__ own_end,
+                                                uint32_t* __restrict__ far_ends) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    leaf_at[p] = PHR_NONE;
+    const uint32_t sg = seg_of_slot(seg_lo, segs, p);
+    const uint32_t lo = seg_lo[sg], ns = seg_lo[sg + 1] - lo, k = p - lo;
+    if (k == 0) parent[ns == 1 ? n + p : p] = PHR_NONE;  // the root
+    if (k + 1 >= ns) {
+        own_end[p] = 0u;  // the last slot of a segment starts no inner node
+        return;
+    }
+    const uint32_t* c = codes + lo;
+    const int64_t m = ns, i = k;
+    int64_t d = (kdelta(c, m, i, i + 1) - kdelta(c, m, i, i - 1) > 0) ? 1 : -1;
+    int32_t dmin = kdelta(c, m, i, i - d);
     int64_t lmax = 2;
-    while (kdelta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    while (kdelta(c, m, i, i + lmax * d) > dmin) lmax *= 2;
     int64_t l = 0, t = lmax / 2;
     while (t >= 1) {
-        if (kdelta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+        if (kdelta(c, m, i, i + (l + t) * d) > dmin) l += t;
         t /= 2;
     }
     int64_t j = i + l * d;
-    int32_t dnode = kdelta(codes, n, i, j);
+    int32_t dnode = kdelta(c, m, i, j);
     int64_t s = 0;
     t = l;
     for (;;) {
         t = (t + 1) >> 1;
-        if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+        if (kdelta(c, m, i, i + (s + t) * d) > dnode) s += t;
         if (t <= 1) break;
     }
     int64_t g = i + s * d + (d < 0 ? d : 0);
     int64_t first = i < j ? i : j, last = i < j ? j : i;
-    uint32_t left = (uint32_t)((first == g) ? n - 1 + g : g);
-    uint32_t right = (uint32_t)((last == g + 1) ? n + g : g + 1);
-    links[k] = PhrLink{left, right, (uint32_t)first, (uint32_t)last};
-    parent[left] = k;
-    parent[right] = k;
-    if (k == 0) parent[0] = PHR_NONE;
+    uint32_t left = (uint32_t)((first == g) ? n + lo + g : lo + g);
+    uint32_t right = (uint32_t)((last == g + 1) ? n + lo + g + 1 : lo + g + 1);
+    links[p] = PhrLink{left, right, lo + (uint32_t)first, lo + (uint32_t)last};
+    parent[left] = p;
+    parent[right] = p;
+    // +1 where the range starts, -1 where it ends (two's complement words).
+    if (first == i) {
+        own_end[p] = 1u;
+        atomicAdd(&far_ends[lo + last], 0xffffffffu);
+    } else {
+        own_end[p] = 0xffffffffu;
+        atomicAdd(&far_ends[lo + first], 1u);
+    }
 }
 
 // ---- 5. leaf packing --------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
-                                              uint32_t n, uint32_t leaf, uint32_t* __restrict__ leaf_list,
-                                              uint32_t* __restrict__ leaf_count, uint32_t* __restrict__ depth_out) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t total = 2 * n - 1;
-    bool listed = false;
-    uint32_t depth = 0;
-    if (k < total) {
-        // Depth by climbing; deeper than 31 means an ancestor at depth 31 was
-        // packed (d + 1 >= STACK), so this node is unreachable.
-        uint32_t x = k, d = 0;
-        bool deep = false;
-        while (x != 0) {
-            if (d == PHR_STACK - 1) { deep = true; break; }
-            x = parent[x];
-            ++d;
+// Chained scan over the slots: stab[e] = sum over j <= e of the +1/-1 terms
+// (own_end[j] + far_ends[j]), the number of inner ranges covering the edge
+// between slots e and e + 1.
+// Tiles take ids from a ticket, publish their total and look back over
+// earlier tiles (64-bit status words: flags in bits 62-63, a signed total in
+// bits 0-61, since a tile may hold more ends than starts).
+constexpr uint32_t SC_ITEMS = 8;
+constexpr uint32_t SC_TILE = 256 * SC_ITEMS;
+constexpr uint64_t SC_AGG = 1ull << 62;
+constexpr uint64_t SC_PRE = 1ull << 63;
+constexpr uint64_t SC_MASK = SC_AGG - 1;
+
+__device__ __forceinline__ int64_t sc_value(uint64_t s) { return (int64_t)(s << 2) >> 2; }
+
+__global__ __launch_bounds__(256) void k_stab(const uint32_t* __restrict__ own_end, const uint32_t* __restrict__ far_ends, uint32_t n,
+                                              uint32_t* __restrict__ stab, uint64_t* status, uint32_t* ticket) {
+    __shared__ uint32_t s_tile;
+    __shared__ int64_t s_wave[4];
+    __shared__ int64_t s_excl;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * SC_TILE + tid * SC_ITEMS;
+    int32_t v[SC_ITEMS];
+    int64_t mine = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < SC_ITEMS; ++r) {
+        uint32_t i = base + r;
+        v[r] = i < n ? (int32_t)(own_end[i] + far_ends[i]) : 0;
+        mine += v[r];
+    }
+    int64_t x = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    int64_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+        if (w < wave) before += s_wave[w];
+        total += s_wave[w];
+    }
+    if (tid == 0) {
+        uint64_t* my = status + tile;
+        int64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(my, SC_PRE | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, SC_AGG | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int32_t j = (int32_t)tile - 1;
+            for (;;) {
+                uint64_t st = __hip_atomic_load(status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (st & SC_PRE) { excl += sc_value(st); break; }
+                if (st & SC_AGG) { excl += sc_value(st); --j; continue; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __hip_atomic_store(my, SC_PRE | ((uint64_t)(excl + total) & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (!deep) {
-            // A packed parent mutes its subtree. Ranges shrink down the tree, so
-            // only the parent's own size test matters; its depth (d - 1 <= 30) is
-            // below the cap.
+        s_excl = excl;
+    }
+    __syncthreads();
+    int64_t run = s_excl + before + x - mine;
+#pragma unroll
+    for (uint32_t r = 0; r < SC_ITEMS; ++r) {
+        run += v[r];
+        uint32_t i = base + r;
+        if (i < n) stab[i] = (uint32_t)run;
+    }
+}
+
+// One thread per node id (inner [0, n), leaf [n, 2n)). Depth of an inner node
+// = ranges covering its split edge less one (itself); of leaf slot k = the
+// larger of the coverings of edges k - 1 and k (an edge across a segment's end
+// is covered by nothing). Deeper than 31 means an ancestor at depth 31 was
+// packed (d + 1 >= STACK), so the node is unreachable.
+__global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+                                              const uint32_t* __restrict__ stab, uint32_t n, const uint32_t* __restrict__ seg_lo,
+                                              uint32_t segs, uint32_t leaf, uint32_t* __restrict__ leaf_at,
+                                              uint32_t* __restrict__ depth_out, uint32_t* __restrict__ acc) {
+    __shared__ uint32_t s_seg[4], s_depth[4];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t j = t; j < 6 * segs; j += gridDim.x * blockDim.x) acc[j] = (j % 6) < 3 ? KEY_MAX : KEY_MIN;
+    bool listed = false;
+    uint32_t depth = 0, sg = 0;
+    if (t < 2 * n) {
+        const bool inner = t < n;
+        const uint32_t slot = inner ? t : t - n;
+        sg = seg_of_slot(seg_lo, segs, slot);
+        bool node = true;
+        if (inner) node = slot + 1 < seg_lo[sg + 1];  // the last slot of a segment starts no inner node
+        if (node) {
+            PhrLink l{};
+            uint32_t d, first;
+            if (inner) {
+                l = links[t];
+                d = stab[l.left < n ? l.left : l.left - n] - 1u;
+                first = l.first;
+            } else {
+                first = slot;
+                d = max(slot > 0 ? stab[slot - 1] : 0u, stab[slot]);
+            }
+            // A packed parent mutes its subtree. Ranges shrink down the tree,
+            // so only the parent's own size test matters; its depth (d - 1 <=
+            // 30) is below the cap.
             bool muted = false;
-            if (k != 0) {
-                PhrLink pl = links[parent[k]];
+            const uint32_t up = parent[t];
+            if (up != PHR_NONE && d < PHR_STACK) {
+                PhrLink pl = links[up];
                 muted = pl.last - pl.first < leaf;
             }
-            if (!muted) {
-                if (k + 1 < n) {
-                    PhrLink l = links[k];
-                    listed = (l.last - l.first < leaf) || (d + 1 >= PHR_STACK);
-                } else {
-                    listed = true;
-                }
+            if (d < PHR_STACK && !muted) {
+                listed = inner ? ((l.last - l.first < leaf) || (d + 1 >= PHR_STACK)) : true;
+                if (listed) leaf_at[first] = t;
                 depth = d;
             }
         }
     }
     // Every reachable node has a listed leaf below it at least as deep, so the
-    // deepest listed leaf is the tree's depth.
-    uint64_t m = __ballot(listed);
+    // deepest listed leaf is its tree's depth. Waves whose listed nodes share
+    // one segment fold through LDS to one word a block, and a fold only takes
+    // the atomic when it would raise the depth already there.
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(listed);
+    bool uniform = true;
+    uint32_t s0 = 0, dm = 0;
     if (m) {
-        uint32_t lane = __lane_id();
-        uint32_t first_lane = (uint32_t)__ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == first_lane) base = atomicAdd(leaf_count, (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, (int)first_lane);
-        if (listed) leaf_list[base + (uint32_t)__popcll(m & lanemask_lt())] = k;
-        uint32_t dm = wave_max_u(listed ? depth : 0u);
-        if (lane == first_lane) atomicMax(depth_out, dm);
+        const uint32_t first_lane = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        s0 = (uint32_t)__shfl((int)sg, (int)first_lane);
+        uniform = __ballot(listed && sg != s0) == 0;
+        dm = wave_max_u(listed ? depth : 0u);
+        if (!uniform && listed && depth > __hip_atomic_load(&depth_out[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&depth_out[sg], depth);
+    }
+    if (lane == 0) {
+        s_seg[wave] = (m && uniform) ? s0 : PHR_NONE;
+        s_depth[wave] = dm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const uint32_t w = threadIdx.x, sw = s_seg[w];
+        bool lead = sw != PHR_NONE;
+        uint32_t dw = s_depth[w];
+        for (uint32_t v = 0; v < 4 && lead; ++v) {
+            if (v == w || s_seg[v] != sw) continue;
+            if (v < w) lead = false;  // the first wave of this segment folds for all
+            else dw = max(dw, s_depth[v]);
+        }
+        if (lead && dw > __hip_atomic_load(&depth_out[sw], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&depth_out[sw], dw);
     }
 }
 
 // ---- 6. refit ------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ prims, const uint32_t* __restrict__ order,
-                                               const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
-                                               const uint32_t* __restrict__ leaf_list, const uint32_t* __restrict__ leaf_count,
-                                               uint32_t n, float4* nodes, uint32_t* cnt) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= *leaf_count) return;
-    const uint32_t k = leaf_list[j];
-    uint32_t first, end;
-    if (k + 1 < n) {
-        PhrLink l = links[k];
-        first = l.first;
-        end = l.last + 1;
-    } else {
-        first = k - (n - 1);
-        end = first + 1;
+__device__ __forceinline__ void store_node(float4* nodes, uint32_t k, const Box& b, uint32_t w0, uint32_t w1) {
+    nodes[2 * k] = make_float4(b.lo0, b.lo1, b.lo2, __uint_as_float(w0));
+    nodes[2 * k + 1] = make_float4(b.hi0, b.hi1, b.hi2, __uint_as_float(w1));
+}
+
+// Arrival counters of one block's inner nodes: 2 bits per node in LDS (a node
+// sees at most two arrivals), indexed by the node's id less the block's first
+// slot (an inner node's id is a slot of its own range, so it lies in the
+// block's slots); or, for blocks owning more slots than LDS holds, one word
+// per node in global memory, put back to zero by the second arrival.
+template <uint32_t CW>
+struct Arrivals {
+    uint32_t* s;
+    uint32_t* g;
+    uint64_t s0;
+    __device__ __forceinline__ bool second(uint32_t p) {
+        if constexpr (CW > 0) {
+            uint32_t i = (uint32_t)(p - s0), sh = 2 * (i & 15u);
+            uint32_t old = __hip_atomic_fetch_add(&s[i >> 4], 1u << sh, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return ((old >> sh) & 3u) != 0u;
+        } else {
+            uint32_t old = __hip_atomic_fetch_add(&g[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) return false;
+            __hip_atomic_store(&g[p], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return true;
+        }
     }
-    float lo0 = 3.40282347e38f, lo1 = 3.40282347e38f, lo2 = 3.40282347e38f;
-    float hi0 = -3.40282347e38f, hi1 = -3.40282347e38f, hi2 = -3.40282347e38f;
-    for (uint32_t s = first; s < end; ++s) {
-        uint32_t p = order[s];
-        float4 a = prims[2 * p], b = prims[2 * p + 1];
-        lo0 = fminf(lo0, a.x); lo1 = fminf(lo1, a.y); lo2 = fminf(lo2, a.z);
-        hi0 = fmaxf(hi0, b.x); hi1 = fmaxf(hi1, b.y); hi2 = fmaxf(hi2, b.z);
-    }
-    nodes[2 * k] = make_float4(lo0, lo1, lo2, __uint_as_float(first | PHR_LEAF));
-    nodes[2 * k + 1] = make_float4(hi0, hi1, hi2, __uint_as_float(end));
-    uint32_t x = k;
-    while (x != 0) {
-        uint32_t p = parent[x];
-        uint32_t old = __hip_atomic_fetch_add(&cnt[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;  // the sibling's thread finishes p
-        __hip_atomic_store(&cnt[p], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        PhrLink l = links[p];
-        uint32_t sib = (l.left == x) ? l.right : l.left;
-        float4 sa = nodes[2 * sib], sb = nodes[2 * sib + 1];
-        lo0 = fminf(lo0, sa.x); lo1 = fminf(lo1, sa.y); lo2 = fminf(lo2, sa.z);
-        hi0 = fmaxf(hi0, sb.x); hi1 = fmaxf(hi1, sb.y); hi2 = fmaxf(hi2, sb.z);
-        nodes[2 * p] = make_float4(lo0, lo1, lo2, __uint_as_float(l.left));
-        nodes[2 * p + 1] = make_float4(hi0, hi1, hi2, __uint_as_float(l.right));
+};
+
+// Carry a finished node x of segment g up while its parent's range lies in
+// [s0, s1); a node whose parent's range leaves it is handed to the next level.
+template <uint32_t CW>
+__device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t n, uint64_t s0, uint64_t s1,
+                                      const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent, float4* nodes,
+                                      Arrivals<CW>& arr, uint32_t* s_fcount, uint32_t* front_out) {
+    for (;;) {
+        const uint32_t p = parent[x];
+        if (p == PHR_NONE) return;  // the root is done
+        const PhrLink l = links[p];
+        if (l.first < s0 || l.last >= s1) {
+            uint32_t at = atomicAdd(s_fcount, 1u);
+            front_out[at] = x;
+            return;
+        }
+        if (!arr.second(p)) return;  // the sibling's climb finishes p
+        const uint32_t sib = (l.left == x) ? l.right : l.left;
+        const uint32_t so = out_of(sib, n, g);
+        const float4 sa = nodes[2 * so], sb = nodes[2 * so + 1];
+        grow(b, sa, sb);
+        store_node(nodes, out_of(p, n, g), b, out_of(l.left, n, g), out_of(l.right, n, g));
         x = p;
     }
+}
+
+// Level 0: one block per RF_SLOTS slots; a thread per leaf starting there.
+__global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __restrict__ prims, const uint32_t* __restrict__ order,
+                                                             const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+                                                             const uint32_t* __restrict__ leaf_at, uint32_t n,
+                                                             const uint32_t* __restrict__ seg_lo, const uint32_t* __restrict__ node_base,
+                                                             uint32_t segs, uint32_t leaf_flag, float4* nodes,
+                                                             uint32_t* __restrict__ front_out, uint32_t* __restrict__ fcount_out) {
+    constexpr uint32_t CW = RF_SLOTS / 16;
+    __shared__ uint32_t s_cnt[CW];
+    __shared__ uint32_t s_fcount;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < CW; i += RF_THREADS) s_cnt[i] = 0u;
+    if (tid == 0) s_fcount = 0u;
+    __syncthreads();
+    const uint64_t s0 = (uint64_t)blockIdx.x * RF_SLOTS, s1 = s0 + RF_SLOTS;
+    Arrivals<CW> arr{s_cnt, nullptr, s0};
+    uint32_t* out = front_out + s0;
+    for (uint32_t r = 0; r < RF_SLOTS / RF_THREADS; ++r) {
+        const uint64_t slot = s0 + r * RF_THREADS + tid;
+        if (slot >= n) break;
+        const uint32_t k = leaf_at[slot];
+        if (k == PHR_NONE) continue;
+        uint32_t first, end;
+        if (k < n) {
+            PhrLink l = links[k];
+            first = l.first;
+            end = l.last + 1;
+        } else {
+            first = k - n;
+            end = first + 1;
+        }
+        const Seg g = seg_at(seg_lo, node_base, seg_of_slot(seg_lo, segs, first));
+        Box b = empty_box();
+        for (uint32_t q = first; q < end; ++q) {
+            uint32_t pi = order[q];
+            grow(b, prims[2 * pi], prims[2 * pi + 1]);
+        }
+        store_node(nodes, out_of(k, n, g), b, first | leaf_flag, end);
+        climb<CW>(k, b, g, n, s0, s1, links, parent, nodes, arr, &s_fcount, out);
+    }
+    __syncthreads();
+    if (tid == 0) fcount_out[blockIdx.x] = s_fcount;
+}
+
+// Level L >= 1: block b owns the slots of blocks [RF_GROUP b, RF_GROUP b + RF_GROUP)
+// of level L - 1 (span slots each) and carries on the climbs they handed over.
+template <uint32_t CW>
+__global__ __launch_bounds__(RF_THREADS) void k_refit_up(const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+                                                         uint32_t n, const uint32_t* __restrict__ seg_lo,
+                                                         const uint32_t* __restrict__ node_base, uint32_t segs, float4* nodes,
+                                                         uint32_t* __restrict__ gcnt, uint32_t below_blocks, uint64_t span,
+                                                         const uint32_t* __restrict__ front_in, const uint32_t* __restrict__ fcount_in,
+                                                         uint32_t* __restrict__ front_out, uint32_t* __restrict__ fcount_out) {
+    __shared__ uint32_t s_cnt[CW > 0 ? CW : 1];
+    __shared__ uint32_t s_off[RF_GROUP + 1];
+    __shared__ uint32_t s_fcount;
+    const uint32_t tid = threadIdx.x;
+    if constexpr (CW > 0)
+        for (uint32_t i = tid; i < CW; i += RF_THREADS) s_cnt[i] = 0u;
+    if (tid == 0) s_fcount = 0u;
+    const uint32_t j0 = blockIdx.x * RF_GROUP;
+    if (tid < 64) {
+        // Inclusive scan of the handed-over counts over one wave.
+        const uint32_t j = j0 + tid;
+        uint32_t c = j < below_blocks ? fcount_in[j] : 0u, x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = (uint32_t)__shfl_up((int)x, o);
+            if (tid >= (uint32_t)o) x += y;
+        }
+        s_off[tid + 1] = x;
+        if (tid == 0) s_off[0] = 0u;
+    }
+    __syncthreads();
+    const uint64_t s0 = (uint64_t)blockIdx.x * RF_GROUP * span, s1 = s0 + RF_GROUP * span;
+    Arrivals<CW> arr{s_cnt, gcnt, s0};
+    uint32_t* out = front_out + s0;
+    const uint32_t total = s_off[RF_GROUP];
+    for (uint32_t e = tid; e < total; e += RF_THREADS) {
+        uint32_t lo = 0, hi = RF_GROUP;  // the list holding entry e: s_off[lo] <= e < s_off[lo + 1]
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] <= e) lo = mid; else hi = mid;
+        }
+        const uint32_t x = front_in[(uint64_t)(j0 + lo) * span + (e - s_off[lo])];
+        const Seg g = seg_at(seg_lo, node_base, seg_of_slot(seg_lo, segs, x < n ? x : x - n));
+        const uint32_t xo = out_of(x, n, g);
+        const float4 a = nodes[2 * xo], c = nodes[2 * xo + 1];
+        Box b{a.x, a.y, a.z, c.x, c.y, c.z};
+        climb<CW>(x, b, g, n, s0, s1, links, parent, nodes, arr, &s_fcount, out);
+    }
+    __syncthreads();
+    if (tid == 0) fcount_out[blockIdx.x] = s_fcount;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// GpuBvh (src/bvh/gpu.rs:11-104)
+// GpuBvh (src/bvh/gpu.rs:11-104), one tree or a batch of segments
 // ---------------------------------------------------------------------------
 struct phr_bvh {
     phr_gpu* gpu = nullptr;
-    uint32_t n = 0, leaf = 4, depth = 0, tiles = 0, bounds_blocks = 0;
+    uint32_t n = 0, leaf = 4, segs = 1, nonempty = 1, tiles = 0, bounds_blocks = 0, seg_passes = 0, leaf_flag = PHR_LEAF;
+    std::vector<uint32_t> seg_lo_h, node_base_h, depth_h;
     phr_array* prims = nullptr;
     bool own_prims = false;
-    phr_array* order = nullptr;  // n u32: sorted slot -> primitive
-    phr_array* nodes = nullptr;  // 2n-1 nodes
-    uint32_t* codes = nullptr;   // n: unsorted codes, then sorted codes after the last pass
+    phr_array* order = nullptr;  // n u32: sorted slot -> input index
+    phr_array* nodes = nullptr;  // the node array written (owned unless own_nodes is false)
+    bool own_nodes = true;
+    uint32_t* seg_lo = nullptr;     // segs + 1
+    uint32_t* node_base = nullptr;  // segs
+    uint32_t* seg_of = nullptr;     // n, only with segment passes
+    uint32_t* codes = nullptr;      // n: sorted codes after the last pass
     uint32_t* keys_b = nullptr;
     uint32_t* vals_b = nullptr;
-    uint32_t* zero = nullptr;  // histograms, tickets, counters, look-back status
-    uint32_t zero_words = 0;
-    uint32_t* arrive = nullptr;
-    float* scene = nullptr;
-    float* partials = nullptr;
-    PhrLink* links = nullptr;
-    uint32_t* parent = nullptr;
-    uint32_t* cnt = nullptr;
-    uint32_t* leaf_list = nullptr;
+    uint32_t* zero = nullptr;  // histograms, tickets, depths, look-back status
+    uint32_t zero_words = 0, status_at = 0, scan_status = 0;
+    uint32_t* acc = nullptr;  // each segment's scene box as six order-preserving keys
+    PhrLink* links = nullptr;     // n, by inner id
+    uint32_t* parent = nullptr;   // 2n, by node id
+    uint32_t* cnt = nullptr;      // n: arrival counters of the levels above LDS size
+    uint32_t* leaf_at = nullptr;  // n: the reachable leaf whose range starts at a slot, or NONE
+    uint32_t* own_end = nullptr;   // n: +1/-1 for the inner range of which a slot is one end
+    uint32_t* far_ends = nullptr;  // n: the +1/-1 terms other ranges' far ends put on a slot
+    uint32_t* front[2] = {nullptr, nullptr};   // n each: nodes handed from one refit level to the next
+    uint32_t* fcount[2] = {nullptr, nullptr};  // per block of a level: how many it handed over
+    uint32_t level0_blocks = 0;
 };
 
 namespace {
@@ -455,46 +823,104 @@ void bvh_release(phr_bvh* b) {
     if (!b) return;
     if (b->own_prims) phr_array_release(b->prims);
     phr_array_release(b->order);
-    phr_array_release(b->nodes);
-    for (void* p : {(void*)b->codes, (void*)b->keys_b, (void*)b->vals_b, (void*)b->zero, (void*)b->arrive, (void*)b->scene,
-                    (void*)b->partials, (void*)b->links, (void*)b->parent, (void*)b->cnt, (void*)b->leaf_list})
+    if (b->own_nodes) phr_array_release(b->nodes);
+    for (void* p : {(void*)b->seg_lo, (void*)b->node_base, (void*)b->seg_of, (void*)b->codes, (void*)b->keys_b, (void*)b->vals_b,
+                    (void*)b->zero, (void*)b->acc, (void*)b->links, (void*)b->parent, (void*)b->cnt, (void*)b->leaf_at,
+                    (void*)b->own_end, (void*)b->far_ends, (void*)b->front[0], (void*)b->front[1], (void*)b->fcount[0],
+                    (void*)b->fcount[1]})
         if (p) (void)hipFree(p);
     delete b;
 }
 
-int bvh_alloc(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, phr_bvh** out) {
-    uint32_t n = (uint32_t)prims->len;
+// seg_lo: segs + 1 non-decreasing starts from 0 to n. nodes_ext/node_base:
+// an existing node array and where each segment's tree goes in it, or null
+// for a new array with the trees back to back.
+int bvh_alloc(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, const uint32_t* seg_lo, uint32_t segs, phr_array* nodes_ext,
+              const uint32_t* node_base, uint32_t leaf_flag, phr_bvh** out) {
+    const uint32_t n = (uint32_t)prims->len;
     PHR_CHECK(n < (1u << 30), PHR_ERR_SIZE, "GpuBvh: at most 2^30 - 1 primitives");
     PHR_CHECK(leaf >= 1, PHR_ERR_ARG, "GpuBvh: leaf size must be at least 1");
+    PHR_CHECK(segs >= 1 && segs <= MAX_SEGMENTS, PHR_ERR_SIZE, "GpuBvh: between 1 and 65536 segments");
+    std::vector<uint32_t> lo(segs + 1);
+    if (seg_lo) {
+        std::memcpy(lo.data(), seg_lo, 4ull * (segs + 1));
+    } else {
+        PHR_CHECK(segs == 1, PHR_ERR_ARG, "GpuBvh: segment starts missing");
+        lo[0] = 0;
+        lo[1] = n;
+    }
+    PHR_CHECK(lo[0] == 0 && lo[segs] == n, PHR_ERR_ARG, "GpuBvh: segment starts must run from 0 to the box count");
+    uint32_t nonempty = 0;
+    uint64_t total_nodes = 0;
+    std::vector<uint32_t> base(segs);
+    for (uint32_t s = 0; s < segs; ++s) {
+        PHR_CHECK(lo[s] <= lo[s + 1], PHR_ERR_ARG, "GpuBvh: segment starts must not decrease");
+        const uint32_t ns = lo[s + 1] - lo[s];
+        base[s] = node_base ? node_base[s] : (uint32_t)total_nodes;
+        if (ns) {
+            ++nonempty;
+            total_nodes += 2ull * ns - 1;
+            if (nodes_ext)
+                PHR_CHECK((uint64_t)base[s] + 2ull * ns - 1 <= nodes_ext->len, PHR_ERR_SIZE, "GpuBvh: a tree does not fit the node array");
+        }
+    }
     phr_bvh* b = new phr_bvh();
     b->gpu = g;
     b->n = n;
     b->leaf = leaf;
+    b->segs = segs;
+    b->nonempty = nonempty;
+    b->leaf_flag = leaf_flag;
+    b->seg_lo_h = lo;
+    b->node_base_h = base;
+    b->depth_h.assign(segs, 0);
     b->prims = prims;
-    b->own_prims = own;
+    b->own_prims = false;  // taken over only on success: the caller frees them on failure
+    b->seg_passes = segs <= 1 ? 0 : (segs <= 256 ? 1 : 2);
     b->tiles = (n + OS_TILE - 1) / OS_TILE;
     b->bounds_blocks = std::min<uint32_t>(BOUNDS_BLOCKS_MAX, std::max<uint32_t>(1, (n + 2047) / 2048));
-    b->zero_words = Z_STATUS + OS_PASSES * b->tiles * 256;
+    b->status_at = (Z_DEPTH + segs + 63) & ~63u;
+    b->scan_status = (b->status_at + (CODE_PASSES + b->seg_passes) * b->tiles * 256 + 1) & ~1u;  // 8-byte aligned
+    b->zero_words = b->scan_status + 2 * ((n + SC_TILE - 1) / SC_TILE) + 2;
     int rc = PHR_OK;
     auto fail = [&](int code) { bvh_release(b); return code; };
     if ((rc = phr_array_alloc(g, 4, n, &b->order)) != PHR_OK) return fail(rc);
-    if ((rc = phr_array_alloc(g, 32, n ? 2ull * n - 1 : 0, &b->nodes)) != PHR_OK) return fail(rc);
+    if (nodes_ext) {
+        b->nodes = nodes_ext;
+        b->own_nodes = false;
+    } else if ((rc = phr_array_alloc(g, 32, total_nodes, &b->nodes)) != PHR_OK) {
+        return fail(rc);
+    }
     uint64_t nn = std::max<uint64_t>(n, 1);
     auto dev = [&](void** p, uint64_t bytes) -> bool {
         if (hipMalloc(p, std::max<uint64_t>(bytes, 16)) != hipSuccess) return false;
         return hipMemsetAsync(*p, 0, std::max<uint64_t>(bytes, 16), g->stream) == hipSuccess;
     };
-    bool ok = dev((void**)&b->codes, 4 * nn) && dev((void**)&b->keys_b, 4 * nn) && dev((void**)&b->vals_b, 4 * nn) &&
-              dev((void**)&b->zero, 4ull * b->zero_words) && dev((void**)&b->arrive, 64) && dev((void**)&b->scene, 32) &&
-              dev((void**)&b->partials, 32ull * b->bounds_blocks) && dev((void**)&b->links, 16 * nn) &&
-              dev((void**)&b->parent, 8 * nn) && dev((void**)&b->cnt, 4 * nn) && dev((void**)&b->leaf_list, 4 * nn);
+    b->level0_blocks = (uint32_t)((nn + RF_SLOTS - 1) / RF_SLOTS);
+    bool ok = dev((void**)&b->seg_lo, 4ull * (segs + 1)) && dev((void**)&b->node_base, 4ull * segs) &&
+              (b->seg_passes == 0 || dev((void**)&b->seg_of, 4 * nn)) && dev((void**)&b->codes, 4 * nn) &&
+              dev((void**)&b->keys_b, 4 * nn) && dev((void**)&b->vals_b, 4 * nn) && dev((void**)&b->zero, 4ull * b->zero_words) &&
+              dev((void**)&b->acc, 24ull * segs) && dev((void**)&b->links, 16 * nn) && dev((void**)&b->parent, 8 * nn) &&
+              dev((void**)&b->cnt, 4 * nn) && dev((void**)&b->leaf_at, 4 * nn) && dev((void**)&b->own_end, 4 * nn) &&
+              dev((void**)&b->far_ends, 4 * nn) && dev((void**)&b->front[0], 4 * nn) && dev((void**)&b->front[1], 4 * nn) &&
+              dev((void**)&b->fcount[0], 4ull * b->level0_blocks) && dev((void**)&b->fcount[1], 4ull * b->level0_blocks);
+    // Aabb::default() as keys for every segment: fkey(f32::MAX) x 3, fkey(f32::MIN) x 3.
+    std::vector<uint32_t> acc0(6ull * segs);
+    for (size_t j = 0; j < acc0.size(); ++j) acc0[j] = (j % 6) < 3 ? KEY_MAX : KEY_MIN;
+    ok = ok && hipMemcpyAsync(b->acc, acc0.data(), 4 * acc0.size(), hipMemcpyHostToDevice, g->stream) == hipSuccess &&
+         hipMemcpyAsync(b->seg_lo, lo.data(), 4ull * (segs + 1), hipMemcpyHostToDevice, g->stream) == hipSuccess &&
+         hipMemcpyAsync(b->node_base, base.data(), 4ull * segs, hipMemcpyHostToDevice, g->stream) == hipSuccess &&
+         hipStreamSynchronize(g->stream) == hipSuccess;
     if (!ok) {
         phr_set_error_msg("GpuBvh: device allocation failed");
         return fail(PHR_ERR_HIP);
     }
+    b->own_prims = own;
     *out = b;
     return PHR_OK;
 }
+
+int bvh_enqueue_refit(phr_bvh* b);
 
 // Enqueue the whole build on the device's stream (no host synchronisation).
 int bvh_enqueue_build(phr_bvh* b) {
@@ -503,55 +929,118 @@ int bvh_enqueue_build(phr_bvh* b) {
     hipStream_t s = b->gpu->stream;
     const float4* prims = (const float4*)b->prims->ptr;
     uint32_t* order = (uint32_t*)b->order->ptr;
-    float4* nodes = (float4*)b->nodes->ptr;
     uint32_t* hist = b->zero + Z_HIST;
     uint32_t* ticket = b->zero + Z_TICKET;
-    uint32_t* status = b->zero + Z_STATUS;
-    hipLaunchKernelGGL(k_bounds, dim3(b->bounds_blocks), dim3(256), 0, s, prims, n, b->partials, b->scene, b->arrive, b->zero,
+    uint32_t* status = b->zero + b->status_at;
+    const uint32_t passes = CODE_PASSES + b->seg_passes;
+    hipLaunchKernelGGL(k_bounds, dim3(b->bounds_blocks), dim3(256), 0, s, prims, n, b->seg_lo, b->segs, b->acc, b->zero,
                        b->zero_words);
-    hipLaunchKernelGGL(k_morton, dim3(b->tiles), dim3(256), 0, s, prims, (const float*)b->scene, n, b->codes, hist);
-    // Pass 0: codes (slot order) -> keys_b/vals_b; 1: -> codes/order; 2: -> b; 3: -> codes/order.
-    uint32_t* kin = b->codes;
+    // The passes alternate between (codes, order) and (keys_b, vals_b) and
+    // must end in (codes, order): with an odd count the codes start in keys_b.
+    uint32_t* unsorted = (passes % 2) ? b->keys_b : b->codes;
+    hipLaunchKernelGGL(k_morton, dim3(b->tiles), dim3(256), 0, s, prims, (const uint32_t*)b->acc, b->seg_lo, b->segs, b->seg_passes,
+                       n, unsorted, hist, b->far_ends, b->seg_of);
+    uint32_t* kin = unsorted;
     uint32_t* vin = nullptr;
-    for (uint32_t p = 0; p < OS_PASSES; ++p) {
-        bool to_b = (p % 2) == 0;
-        uint32_t* kout = to_b ? b->keys_b : b->codes;
-        uint32_t* vout = to_b ? b->vals_b : order;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const bool to_codes = ((passes - p) % 2) == 1;
+        uint32_t* kout = to_codes ? b->codes : b->keys_b;
+        uint32_t* vout = to_codes ? order : b->vals_b;
+        const bool seg = p >= CODE_PASSES;
+        const uint32_t shift = seg ? 8 * (p - CODE_PASSES) : 8 * p;
+        uint32_t* st = status + (size_t)p * b->tiles * 256;
         if (p == 0)
-            hipLaunchKernelGGL(k_onesweep<true>, dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist + 256 * p,
-                               status + (size_t)p * b->tiles * 256, ticket + p, n, 8 * p);
+            hipLaunchKernelGGL((k_onesweep<true, false>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist + 256 * p,
+                               st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
+        else if (!seg)
+            hipLaunchKernelGGL((k_onesweep<false, false>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout,
+                               hist + 256 * p, st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
         else
-            hipLaunchKernelGGL(k_onesweep<false>, dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist + 256 * p,
-                               status + (size_t)p * b->tiles * 256, ticket + p, n, 8 * p);
+            hipLaunchKernelGGL((k_onesweep<false, true>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout,
+                               hist + 256 * p, st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
         kin = kout;
         vin = vout;
     }
-    if (n > 1) hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, s, b->codes, n, b->links, b->parent);
-    uint32_t total = 2 * n - 1;
-    hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, s, b->links, b->parent, n, b->leaf, b->leaf_list,
-                       b->zero + Z_LEAVES, b->zero + Z_DEPTH);
-    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, s, prims, order, b->links, b->parent, b->leaf_list,
-                       b->zero + Z_LEAVES, n, nodes, b->cnt);
+    hipLaunchKernelGGL(k_karras, dim3((n + 255) / 256), dim3(256), 0, s, b->codes, n, b->seg_lo, b->segs, b->links, b->parent,
+                       b->leaf_at, b->own_end, b->far_ends);
+    // The slot scan writes its coverings over the sort's spare key buffer.
+    uint32_t* stab = b->keys_b;
+    hipLaunchKernelGGL(k_stab, dim3((n + SC_TILE - 1) / SC_TILE), dim3(256), 0, s, b->own_end, b->far_ends, n, stab,
+                       (uint64_t*)(b->zero + b->scan_status), b->zero + Z_SCAN_TICKET);
+    hipLaunchKernelGGL(k_pack, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->parent, stab, n, b->seg_lo, b->segs,
+                       b->leaf, b->leaf_at, b->zero + Z_DEPTH, b->acc);
     PHR_HIP(hipGetLastError());
-    return PHR_OK;
+    return bvh_enqueue_refit(b);
 }
 
+// The refit levels: level 0 over RF_SLOTS-slot blocks, then RF_GROUP times
+// wider blocks until one block owns every slot.
 int bvh_enqueue_refit(phr_bvh* b) {
     const uint32_t n = b->n;
     if (n == 0) return PHR_OK;
-    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, b->gpu->stream, (const float4*)b->prims->ptr,
-                       (const uint32_t*)b->order->ptr, b->links, b->parent, b->leaf_list, b->zero + Z_LEAVES, n,
-                       (float4*)b->nodes->ptr, b->cnt);
+    hipStream_t s = b->gpu->stream;
+    float4* nodes = (float4*)b->nodes->ptr;
+    uint32_t blocks = b->level0_blocks;
+    hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(RF_THREADS), 0, s, (const float4*)b->prims->ptr,
+                       (const uint32_t*)b->order->ptr, b->links, b->parent, b->leaf_at, n, b->seg_lo, b->node_base, b->segs,
+                       b->leaf_flag, nodes, b->front[0], b->fcount[0]);
+    uint64_t span = RF_SLOTS;
+    int cur = 0;
+    while (blocks > 1) {
+        const uint32_t up = (blocks + RF_GROUP - 1) / RF_GROUP;
+        if (span * RF_GROUP <= RF_LDS_SLOTS)
+            hipLaunchKernelGGL(k_refit_up<RF_LDS_SLOTS / 16>, dim3(up), dim3(RF_THREADS), 0, s, b->links, b->parent, n, b->seg_lo,
+                               b->node_base, b->segs, nodes, b->cnt, blocks, span, b->front[cur], b->fcount[cur], b->front[1 - cur],
+                               b->fcount[1 - cur]);
+        else
+            hipLaunchKernelGGL(k_refit_up<0>, dim3(up), dim3(RF_THREADS), 0, s, b->links, b->parent, n, b->seg_lo, b->node_base,
+                               b->segs, nodes, b->cnt, blocks, span, b->front[cur], b->fcount[cur], b->front[1 - cur],
+                               b->fcount[1 - cur]);
+        blocks = up;
+        span *= RF_GROUP;
+        cur = 1 - cur;
+    }
     PHR_HIP(hipGetLastError());
     return PHR_OK;
 }
 
 int bvh_read_depth(phr_bvh* b) {
-    if (b->n == 0) { b->depth = 0; return PHR_OK; }
-    uint32_t d = 0;
-    PHR_HIP(hipMemcpyAsync(&d, b->zero + Z_DEPTH, 4, hipMemcpyDeviceToHost, b->gpu->stream));
+    if (b->n == 0) {
+        std::fill(b->depth_h.begin(), b->depth_h.end(), 0u);
+        return PHR_OK;
+    }
+    PHR_HIP(hipMemcpyAsync(b->depth_h.data(), b->zero + Z_DEPTH, 4ull * b->segs, hipMemcpyDeviceToHost, b->gpu->stream));
     PHR_HIP(hipStreamSynchronize(b->gpu->stream));
-    b->depth = d;
+    return PHR_OK;
+}
+
+int bvh_new(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, const uint32_t* seg_lo, uint32_t segs, phr_array* nodes_ext,
+            const uint32_t* node_base, uint32_t leaf_flag, phr_bvh** out) {
+    phr_bvh* b = nullptr;
+    int rc = bvh_alloc(g, prims, own, leaf, seg_lo, segs, nodes_ext, node_base, leaf_flag, &b);
+    if (rc != PHR_OK) return rc;
+    if ((rc = bvh_enqueue_build(b)) != PHR_OK || (rc = bvh_read_depth(b)) != PHR_OK) {
+        if (own) b->own_prims = false;  // the caller still owns them on failure
+        bvh_release(b);
+        return rc;
+    }
+    *out = b;
+    return PHR_OK;
+}
+
+int upload_prims(phr_gpu* g, const void* prims, uint32_t n, phr_array** out) {
+    phr_array* a = nullptr;
+    int rc = phr_array_alloc(g, 32, n, &a);
+    if (rc != PHR_OK) return rc;
+    if (n) {
+        hipError_t e = hipMemcpyAsync(a->ptr, prims, 32ull * n, hipMemcpyHostToDevice, g->stream);
+        if (e != hipSuccess) {
+            phr_array_release(a);
+            phr_set_error("hipMemcpyAsync", e);
+            return PHR_ERR_HIP;
+        }
+    }
+    *out = a;
     return PHR_OK;
 }
 
@@ -559,45 +1048,51 @@ int bvh_read_depth(phr_bvh* b) {
 
 // Internal entry points for the tracer (trace.hip).
 int phr_bvh_internal_build(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out) {
-    phr_bvh* b = nullptr;
-    int rc = bvh_alloc(g, prims, false, leaf, &b);
-    if (rc != PHR_OK) return rc;
-    if ((rc = bvh_enqueue_build(b)) != PHR_OK || (rc = bvh_read_depth(b)) != PHR_OK) {
-        bvh_release(b);
-        return rc;
-    }
-    *out = b;
-    return PHR_OK;
+    return bvh_new(g, prims, false, leaf, nullptr, 1, nullptr, nullptr, PHR_LEAF, out);
 }
+// Trees over segments of `prims` written into `nodes` at `node_base` (host
+// arrays of segs + 1 and segs words), leaves flagged with `leaf_flag`.
+int phr_bvh_internal_build_into(phr_gpu* g, phr_array* prims, uint32_t leaf, const uint32_t* seg_lo, uint32_t segs,
+                                phr_array* nodes, const uint32_t* node_base, uint32_t leaf_flag, phr_bvh** out) {
+    return bvh_new(g, prims, false, leaf, seg_lo, segs, nodes, node_base, leaf_flag, out);
+}
+int phr_bvh_internal_rebuild_async(phr_bvh* b) { return bvh_enqueue_build(b); }
+int phr_bvh_internal_refit_async(phr_bvh* b) { return bvh_enqueue_refit(b); }
+int phr_bvh_internal_read_depth(phr_bvh* b) { return bvh_read_depth(b); }
 phr_array* phr_bvh_internal_nodes(phr_bvh* b) { return b->nodes; }
 phr_array* phr_bvh_internal_order(phr_bvh* b) { return b->order; }
+uint32_t phr_bvh_internal_segment_depth(const phr_bvh* b, uint32_t s) { return s < b->segs ? b->depth_h[s] : 0; }
 
 extern "C" {
 
 int phr_bvh_build(phr_gpu* g, const void* prims, uint32_t n, uint32_t leaf, phr_bvh** out) {
     PHR_CHECK(g && out && (prims || n == 0), PHR_ERR_ARG, "phr_bvh_build: null argument");
     phr_array* a = nullptr;
-    int rc = phr_array_alloc(g, 32, n, &a);
+    int rc = upload_prims(g, prims, n, &a);
     if (rc != PHR_OK) return rc;
-    if (n) {
-        hipError_t e = hipMemcpyAsync(a->ptr, prims, 32ull * n, hipMemcpyHostToDevice, g->stream);
-        if (e != hipSuccess) { phr_array_release(a); phr_set_error("hipMemcpyAsync", e); return PHR_ERR_HIP; }
-    }
-    phr_bvh* b = nullptr;
-    if ((rc = bvh_alloc(g, a, true, leaf, &b)) != PHR_OK) { phr_array_release(a); return rc; }
-    if ((rc = bvh_enqueue_build(b)) != PHR_OK || (rc = bvh_read_depth(b)) != PHR_OK) { bvh_release(b); return rc; }
-    *out = b;
-    return PHR_OK;
+    if ((rc = bvh_new(g, a, true, leaf, nullptr, 1, nullptr, nullptr, PHR_LEAF, out)) != PHR_OK) phr_array_release(a);
+    return rc;
 }
 
 int phr_bvh_build_on(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out) {
     PHR_CHECK(g && prims && out && prims->elem == 32, PHR_ERR_ARG, "phr_bvh_build_on: prims must be an array of 32-byte boxes");
-    phr_bvh* b = nullptr;
-    int rc = bvh_alloc(g, prims, false, leaf, &b);
+    return bvh_new(g, prims, false, leaf, nullptr, 1, nullptr, nullptr, PHR_LEAF, out);
+}
+
+int phr_bvh_build_batch(phr_gpu* g, const void* prims, uint32_t n, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf,
+                        phr_bvh** out) {
+    PHR_CHECK(g && out && seg_lo && (prims || n == 0), PHR_ERR_ARG, "phr_bvh_build_batch: null argument");
+    phr_array* a = nullptr;
+    int rc = upload_prims(g, prims, n, &a);
     if (rc != PHR_OK) return rc;
-    if ((rc = bvh_enqueue_build(b)) != PHR_OK || (rc = bvh_read_depth(b)) != PHR_OK) { bvh_release(b); return rc; }
-    *out = b;
-    return PHR_OK;
+    if ((rc = bvh_new(g, a, true, leaf, seg_lo, segs, nullptr, nullptr, PHR_LEAF, out)) != PHR_OK) phr_array_release(a);
+    return rc;
+}
+
+int phr_bvh_build_batch_on(phr_gpu* g, phr_array* prims, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf, phr_bvh** out) {
+    PHR_CHECK(g && prims && out && seg_lo && prims->elem == 32, PHR_ERR_ARG,
+              "phr_bvh_build_batch_on: prims must be an array of 32-byte boxes");
+    return bvh_new(g, prims, false, leaf, seg_lo, segs, nullptr, nullptr, PHR_LEAF, out);
 }
 
 int phr_bvh_rebuild_async(phr_bvh* b) {
@@ -639,13 +1134,39 @@ int phr_bvh_update(phr_bvh* b, const void* prims, uint32_t n, int* refitted) {
 
 int phr_bvh_download(phr_bvh* b, void* nodes, uint32_t* order, uint32_t* depth) {
     PHR_CHECK(b, PHR_ERR_ARG, "phr_bvh_download: null");
+    PHR_CHECK(b->segs == 1, PHR_ERR_ARG, "phr_bvh_download: a batch downloads one segment at a time");
+    return phr_bvh_download_segment(b, 0, nodes, order, depth);
+}
+
+// One segment's tree as Bvh::build of its boxes alone would hold it: node
+// indices, leaf ranges and the order all relative to the segment.
+int phr_bvh_download_segment(phr_bvh* b, uint32_t seg, void* nodes, uint32_t* order, uint32_t* depth) {
+    PHR_CHECK(b && seg < b->segs, PHR_ERR_ARG, "phr_bvh_download_segment: no such segment");
     hipStream_t s = b->gpu->stream;
-    if (b->n) {
-        if (nodes) PHR_HIP(hipMemcpyAsync(nodes, b->nodes->ptr, 32ull * (2ull * b->n - 1), hipMemcpyDeviceToHost, s));
-        if (order) PHR_HIP(hipMemcpyAsync(order, b->order->ptr, 4ull * b->n, hipMemcpyDeviceToHost, s));
+    const uint32_t lo = b->seg_lo_h[seg], ns = b->seg_lo_h[seg + 1] - lo, base = b->node_base_h[seg];
+    if (ns) {
+        const uint64_t cnt = 2ull * ns - 1;
+        if (nodes) PHR_HIP(hipMemcpyAsync(nodes, (char*)b->nodes->ptr + 32ull * base, 32ull * cnt, hipMemcpyDeviceToHost, s));
+        if (order) PHR_HIP(hipMemcpyAsync(order, (uint32_t*)b->order->ptr + lo, 4ull * ns, hipMemcpyDeviceToHost, s));
+        PHR_HIP(hipStreamSynchronize(s));
+        if (nodes) {
+            uint32_t* w = (uint32_t*)nodes;
+            for (uint64_t k = 0; k < cnt; ++k) {
+                uint32_t& a = w[8 * k + 3];
+                uint32_t& c = w[8 * k + 7];
+                if (a & PHR_LEAF) {
+                    a = (((a & ~b->leaf_flag) - lo) | PHR_LEAF);
+                    c -= lo;
+                } else {
+                    a -= base;
+                    c -= base;
+                }
+            }
+        }
+        if (order)
+            for (uint32_t k = 0; k < ns; ++k) order[k] -= lo;
     }
-    PHR_HIP(hipStreamSynchronize(s));
-    if (depth) *depth = b->depth;
+    if (depth) *depth = b->depth_h[seg];
     return PHR_OK;
 }
 
@@ -657,7 +1178,14 @@ int phr_bvh_codes(phr_bvh* b, uint32_t* sorted_codes) {
 }
 
 uint32_t phr_bvh_len(const phr_bvh* b) { return b ? b->n : 0; }
-uint32_t phr_bvh_depth(const phr_bvh* b) { return b ? b->depth : 0; }
+uint32_t phr_bvh_depth(const phr_bvh* b) {
+    if (!b) return 0;
+    uint32_t d = 0;
+    for (uint32_t x : b->depth_h) d = std::max(d, x);
+    return d;
+}
+uint32_t phr_bvh_segments(const phr_bvh* b) { return b ? b->segs : 0; }
+uint32_t phr_bvh_segment_depth(const phr_bvh* b, uint32_t seg) { return b && seg < b->segs ? b->depth_h[seg] : 0; }
 uint32_t phr_bvh_leaf(const phr_bvh* b) { return b ? b->leaf : 0; }
 phr_array* phr_bvh_nodes(phr_bvh* b) { return b ? b->nodes : nullptr; }
 phr_array* phr_bvh_order(phr_bvh* b) { return b ? b->order : nullptr; }

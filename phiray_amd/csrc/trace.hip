@@ -729,419 +729,6 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
 }
 
 
-// ---- the persistent path kernel --------------------------------------------
-//
-// The integrator above restated as a state machine so that every iteration of
-// a lane's loop is exactly one ray query through one copy of the walk:
-// camera, bounce and shadow rays of all lanes of a wave go through the same
-// instructions together instead of through separately inlined walks. A lane
-// takes a pixel, runs its samples in order (so the pixel's sums add in the
-// same order as the CPU twin's) and, when the pixel is done, takes the next
-// from a global counter with one atomic per wave. Random numbers are drawn at
-// the same points of the path as shade.wgsl draws them: NEE's draws all come
-// before its shadow rays (direct() traces after drawing), the lobe pick after
-// NEE, the GGX sample's two draws after the diffuse path returns.
-__device__ __forceinline__ PhrHit walk_rt(const SceneDev& s, V3 o, V3 d, float tmax, bool any, uint32_t* stk,
-                                          uint32_t& steps, uint32_t& tests) {
-    PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
-    V3 inv = safe_inv(d);
-    float4 lo = s.nodes[0], hi = s.nodes[1];
-    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
-    RaySpace rs = ray_space(d);
-    uint32_t sp = 0;
-    for (;;) {
-        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
-        if (li & PHR_LEAF) {
-            bool done = false;
-            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
-                ++tests;
-                float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
-                float u, v;
-                float t = tri_t(t0, t1, t2, o, rs, u, v);
-                if (t < best.t) {
-                    if (any) {
-                        const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
-                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
-                        best = PhrHit{t, id, u, v, 0u};
-                        done = true;
-                        break;
-                    }
-                    best = PhrHit{t, id, u, v, 0u};
-                }
-            }
-            if (done) return best;
-        } else {
-            ++steps;
-            float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
-            float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
-            float tl = box_t(llo, lhi, o, inv, best.t);
-            float tr = box_t(rlo, rhi, o, inv, best.t);
-            if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) { stk[sp * TRACE_THREADS] = ri; lo = llo; hi = lhi; }
-                else { stk[sp * TRACE_THREADS] = li; lo = rlo; hi = rhi; }
-                ++sp;
-                continue;
-            }
-            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
-            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
-        }
-        if (sp == 0) break;
-        --sp;
-        uint32_t x = stk[sp * TRACE_THREADS];
-        lo = s.nodes[2 * x];
-        hi = s.nodes[2 * x + 1];
-    }
-    return best;
-}
-
-enum : uint32_t { S_CAM, S_MAIN, S_SHADOW, S_AO, S_IND, S_FINISH };
-
-// Next-event estimation at a shading point (integrate.wgsl:66-85), its draws
-// made now: the studio rig's fill goes straight into `dsum`, the shadowed
-// queries (rig key and rim when RIG, the emitter's always) are queued in
-// order. Returns how many shadow rays follow.
-template <bool RIG>
-__device__ __forceinline__ uint32_t nee_setup(const SceneDev& sc, const PhrCamera* p, const Surf& su, uint32_t* seed, V3& dsum,
-                                              V3& q0c, V3& q0d, float& q0t, V3& q1c, V3& q1d, float& q1t, V3& q2c, V3& q2d,
-                                              float& q2t) {
-    uint32_t qn = 0;
-    dsum = v3(0, 0, 0);
-    if (RIG) {
-        V3 fill = normalize(-ld3(p->right) * 0.8f + ld3(p->up) * 0.2f + ld3(p->back) * 0.6f);
-        dsum = brdf_w(su, fill, v3(0.45f, 0.45f, 0.45f), 1.0f, 1.0f);
-        V3 key = cone_dir(ld3(p->sun), 0.06f, seed);
-        V3 c0 = brdf_w(su, key, v3(2, 2, 2), 1.0f, 1.0f);
-        if (maxc(c0) > 0.f) { q0c = c0; q0d = key; q0t = PHR_INF; qn = 1; }
-        V3 rim = normalize(ld3(p->up) * 0.6f - ld3(p->back) * 0.8f + ld3(p->right) * 0.2f);
-        V3 c1 = brdf_w(su, rim, v3(0.9f, 0.9f, 0.9f), 1.0f, 1.0f);
-        if (maxc(c1) > 0.f) {
-            if (qn == 0) { q0c = c1; q0d = rim; q0t = PHR_INF; } else { q1c = c1; q1d = rim; q1t = PHR_INF; }
-            qn++;
-        }
-    } else {
-        (void)rnd(seed);  // lamp_query's pick over unlit lamps (light.wgsl:126)
-    }
-    Shadowed g = glow_query(sc, su, seed);
-    if (maxc(g.c) > 0.f) {
-        if (!RIG || qn == 0) { q0c = g.c; q0d = g.dir; q0t = g.dist; }
-        else if (qn == 1) { q1c = g.c; q1d = g.dir; q1t = g.dist; }
-        else { q2c = g.c; q2d = g.dir; q2t = g.dist; }
-        qn++;
-    }
-    return qn;
-}
-
-// The persistent path kernel. State kept small so the walk's registers fit
-// beside it: the guides and the count-like sums go into the pixel's sums the
-// moment they are known (each is a separate running sum, so only its own order
-// matters); sums of 1.0 per sample are kept as one integer (exact in f32 below
-// 2^24); the ray in flight is derived from the stage.
-template <bool RIG>
-__global__ __launch_bounds__(TRACE_THREADS) void k_path_sm(SceneDev sc, const PhrCamera* __restrict__ p, uint32_t first,
-                                                           uint32_t count, uint32_t frame0, uint32_t spp,
-                                                           uint32_t* __restrict__ px, unsigned long long* __restrict__ counters,
-                                                           uint32_t* __restrict__ next_pixel) {
-    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
-    uint32_t* stk = s_stack + threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const size_t n = (size_t)p->w * p->h;
-    const bool keep = frame0 > 0u;
-    const bool depth_set = p->depth[0] != 0u;
-    const uint32_t total = depth_set ? p->depth[0] - 1u : 8u;
-    const uint32_t see_through = depth_set ? p->depth[1] - 1u : 8u;
-    const uint32_t through = depth_set ? min(p->depth[2] - 1u, total) : 8u;
-    const bool diffuse_on = !depth_set || (p->bounces > 0u && total > 0u);
-    const uint32_t passes = see_through + through + 1u;
-    const bool lit = world_lit(p);
-    const uint32_t idepth = diffuse_on ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
-
-    bool active = true, have = false, ind = false, shaded = false, took = false;
-    uint32_t pix = 0, smp = 0, stage = S_FINISH, seed = 0, bounce = 0, n_see = 0, ib = 0, qn = 0, qi = 0;
-    // the pixel's sums: linear light, albedo, normal, depth, and the misses
-    V3 a_l = v3(0, 0, 0), a_ga = v3(0, 0, 0), a_gn = v3(0, 0, 0);
-    float a_gd = 0.f;
-    uint32_t a_miss = 0, a_outline = 0;
-    // o / d of the camera path; after its opaque hit, the shading point and normal
-    V3 P0 = v3(0, 0, 0), P1 = v3(0, 0, 1), col = v3(0, 0, 0);
-    V3 dsum = v3(0, 0, 0), cc = v3(0, 0, 0), m_alb = v3(0, 0, 0);
-    float p_diffuse = 1.f;
-    V3 q0c = v3(0, 0, 0), q0d = v3(0, 0, 1), q1c = v3(0, 0, 0), q1d = v3(0, 0, 1), q2c = v3(0, 0, 0), q2d = v3(0, 0, 1);
-    float q0t = 0.f, q1t = 0.f, q2t = 0.f;
-    // the diffuse path: throughput, sum, origin (the last hit), its normal, direction
-    V3 iacc = v3(0, 0, 0), ithr = v3(1, 1, 1), iorg = v3(0, 0, 0), inrm = v3(0, 0, 1), idir = v3(0, 0, 1), ialb = v3(0, 0, 0);
-    float imin = 3.0e38f, inear = 1.f;
-    PhrHit h{PHR_INF, PHR_NONE, 0.f, 0.f, 0u};
-    uint32_t cnt_closest = 0, cnt_shadow = 0, cnt_steps = 0, cnt_tests = 0;
-
-    for (;;) {
-        bool need = active && !have;
-        uint64_t m = __ballot(need);
-        if (m) {
-            uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(next_pixel, (uint32_t)__popcll(m));
-            base = (uint32_t)__shfl((int)base, (int)leader);
-            if (need) {
-                uint32_t k = base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-                if (k < count) {
-                    have = true;
-                    pix = first + k;
-                    smp = 0;
-                    stage = S_CAM;
-                    const uint32_t i = pix;
-                    a_l = keep ? v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2)) : v3(0, 0, 0);
-                    a_ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
-                    a_gn = keep ? v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2)) : v3(0, 0, 0);
-                    a_gd = keep ? ldf(px, 13 * n + i) : 0.f;
-                    a_miss = 0;
-                    a_outline = 0;
-                } else {
-                    active = false;
-                }
-            }
-        }
-        if (__ballot(active) == 0ull) break;
-        bool want = false;
-        while (have && !want) {
-            if (stage == S_CAM) {  // shade.wgsl:65-95
-                const uint32_t i = pix, frame = frame0 + smp;
-                const float fw = (float)p->w, fh = (float)p->h;
-                float su_ = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
-                float sv_ = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
-                seed = pcg_hash(i * 9781u + frame * 6271u + 1u);
-                float ju = rnd(&seed);
-                float jv = rnd(&seed);
-                float jx = (ju - 0.5f) / fw, jy = (jv - 0.5f) / fh;
-                float vx = (su_ + jx - p->pan[0]) / p->scale;
-                float vy = -((sv_ + jy) / p->aspect - p->pan[1]) / p->scale;
-                if (p->persp != 0u) {
-                    P0 = ld3(p->pivot) + ld3(p->back) * (p->dist / p->units);
-                    P1 = normalize(ld3(p->right) * vx + ld3(p->up) * vy - ld3(p->back) * p->dist);
-                } else {
-                    P0 = ld3(p->pivot) + (ld3(p->right) * vx + ld3(p->up) * vy + ld3(p->back) * 1.0e4f) / p->units;
-                    P1 = -ld3(p->back);
-                }
-                col = v3(0, 0, 0);
-                bounce = 0;
-                n_see = 0;
-                shaded = false;
-                took = false;
-                stage = S_MAIN;
-                want = true;
-            } else if (stage == S_MAIN) {  // shade.wgsl:125-303, the camera ray's hit
-                const V3 o = P0, d = P1;
-                if (h.tri == PHR_NONE) {
-                    if (bounce == 0u) {  // guide(sky) and emit(missed): nothing met
-                        a_gn = a_gn + -d;
-                        a_gd = a_gd + -1.0f;
-                        a_miss++;
-                        a_outline = 0;
-                    } else {
-                        col = col + env_rgb(p);
-                    }
-                    stage = S_FINISH;
-                    continue;
-                }
-                Tri t = tri_of(sc, h.tri, mat_at(sc, h.tri));
-                float w = 1.0f - h.u - h.v;
-                V3 hit = o + d * h.t;
-                V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
-                bool front = dot(ng0, d) < 0.f;
-                V3 ng = front ? ng0 : -ng0;
-                V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
-                sn = dot(sn, sn) > 1e-12f ? normalize(sn) * (front ? 1.0f : -1.0f) : ng;
-                if (dot(sn, -d) < 0.f) sn = ng;
-                float start = 1.0e4f / p->units;
-                float lift = (p->persp == 0u && h.t > start * 0.5f) ? p->eps + fabsf(h.t - start) * 1e-5f + start * 2.4e-7f
-                                                                    : p->eps + h.t * 1e-5f;
-                V3 pos = hit + ng * lift;
-                V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
-                if (bounce == 0u) {  // guide(base, n, t)
-                    a_outline = h.inst + 1u;
-                    a_ga = a_ga + base;
-                    a_gn = a_gn + sn;
-                    a_gd = a_gd + h.t;
-                }
-                if (rnd(&seed) >= t.color[3]) {  // a cut-out: the camera ray goes on through
-                    n_see++;
-                    bounce++;
-                    if (n_see > see_through || bounce >= passes) { stage = S_FINISH; continue; }
-                    float reach = fmax_(fmax_(fabsf(hit.x), fabsf(hit.y)), fmax_(fabsf(hit.z), 0.5f));
-                    float tl = lift - p->eps + reach * (256.0f / 8388608.0f);
-                    P0 = hit - ng * tl;
-                    want = true;
-                    continue;
-                }
-                col = col + t.emit;
-                V3 fd = dielectric_f0(t.ior);
-                V3 diffuse = base * (1.0f - t.metal);
-                if (maxc(diffuse) > 0.f) diffuse = diffuse * under_layer_real(t.ior, fmax_(t.rough, 0.04f), fmax_(dot(sn, -d), 1e-4f));
-                Surf su{pos, sn, -d, diffuse, mixv(fd, base, t.metal), fmax_(t.rough, 0.04f), t.ior, 1.0f - t.metal};
-                qn = nee_setup<RIG>(sc, p, su, &seed, dsum, q0c, q0d, q0t, q1c, q1d, q1t, q2c, q2d, q2t);
-                m_alb = diffuse;
-                shaded = true;
-                P0 = pos;  // from here P0 / P1 are the shading point and its normal
-                P1 = sn;
-                ind = false;
-                qi = 0;
-                stage = S_SHADOW;
-                want = qn > 0;
-            } else if (stage == S_SHADOW) {
-                if (qi < qn) {  // a shadow ray came back
-                    float vis = h.tri != PHR_NONE ? 0.0f : 1.0f;
-                    V3 qc = qi == 0 ? q0c : (qi == 1 ? q1c : q2c);
-                    dsum = dsum + qc * vis;
-                    qi++;
-                    if (qi < qn) { want = true; continue; }
-                }
-                if (!ind) {  // shade.wgsl:256-303 up to the diffuse path
-                    cc = dsum * 1.0f;
-                    float pd = dot(vmax(m_alb, 0.0f), v3(0.2126f, 0.7152f, 0.0722f));
-                    float pg = 0.0f;
-                    p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
-                    bool take = rnd(&seed) < p_diffuse;
-                    took = take && pd > 0.f;
-                    if (took) {
-                        if (p->glow == 0u && !lit) {  // integrate.wgsl:118-121
-                            idir = cosine_dir(P1, &seed);
-                            stage = S_AO;
-                            want = true;
-                            continue;
-                        }
-                        iacc = v3(0, 0, 0);
-                        ithr = v3(1, 1, 1);
-                        iorg = P0;
-                        inrm = P1;
-                        imin = 3.0e38f;
-                        ib = 0;
-                        idir = cosine_dir(inrm, &seed);
-                        imin = fmin_(imin, fmax_(dot(inrm, idir), 0.0f) / PHR_PI);
-                        stage = S_IND;
-                        want = true;
-                        continue;
-                    }
-                    stage = S_FINISH;
-                    continue;
-                }
-                // integrate.wgsl:178-217: the bounce's direct light, then roulette
-                iacc = iacc + ithr * dsum * inear;
-                ithr = ithr * ialb;
-                bool stop = false;
-                if (ib + 1u < idepth && ib + 1u >= p->rr_from) {
-                    float kp = clampf(maxc(ithr), 0.0f, 0.95f);
-                    if (kp < 1e-4f || rnd(&seed) > kp) stop = true;
-                    else ithr = ithr / kp;
-                }
-                ib++;
-                if (stop || ib >= idepth) { stage = S_FINISH; continue; }
-                idir = cosine_dir(inrm, &seed);
-                imin = fmin_(imin, fmax_(dot(inrm, idir), 0.0f) / PHR_PI);
-                stage = S_IND;
-                want = true;
-            } else if (stage == S_AO) {
-                float vis = h.tri != PHR_NONE ? 0.0f : 1.0f;
-                iacc = env_rgb(p) * mixf(0.25f, 1.0f, vis);
-                stage = S_FINISH;
-            } else if (stage == S_IND) {  // integrate.wgsl:143-177, a diffuse bounce's hit
-                if (h.tri == PHR_NONE) {
-                    iacc = iacc + ithr * env_rgb(p) * 1.0f;
-                    stage = S_FINISH;
-                    continue;
-                }
-                if (!diffuse_on) { stage = S_FINISH; continue; }
-                Tri t = tri_of(sc, h.tri, mat_at(sc, h.tri));
-                V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
-                V3 ng = dot(ng0, idir) < 0.f ? ng0 : -ng0;
-                V3 hit = iorg + idir * h.t + ng * (p->eps + h.t * 1e-5f);
-                V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
-                float rough = fmax_(t.rough, glossy_floor(p, imin));
-                bool glossy = p->integrator[0] < 0.5f || (((uint32_t)(p->integrator[3] + 0.5f)) & 1u) != 0u;
-                V3 fd = glossy ? dielectric_f0(t.ior) : v3(0, 0, 0);
-                float metal = glossy ? t.metal : 0.0f;
-                float tw = t.trans * (1.0f - t.metal);
-                V3 albedo = base * (1.0f - t.metal) * (1.0f - tw);
-                if (maxc(albedo) > 0.f) albedo = albedo * under_layer(fd, rough, fmax_(dot(ng, -idir), 1e-4f));
-                iacc = iacc + ithr * glow_mis(sc, t, h, inrm, idir);
-                inear = (!lit && h.t <= p->ao) ? 0.25f : 1.0f;
-                Surf su{hit, ng, -idir, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
-                qn = nee_setup<RIG>(sc, p, su, &seed, dsum, q0c, q0d, q0t, q1c, q1d, q1t, q2c, q2d, q2t);
-                ialb = albedo;
-                iorg = hit;  // the next bounce leaves from here, along this normal
-                inrm = ng;
-                ind = true;
-                qi = 0;
-                stage = S_SHADOW;
-                want = qn > 0;
-            } else {  // S_FINISH: the sample's light into the pixel's sums, in order
-                if (shaded) {  // shade.wgsl:301-342: direct + what the diffuse path brought
-                    V3 bounced = v3(0, 0, 0);
-                    if (took) bounced = bounced + m_alb * iacc / p_diffuse;
-                    // (the GGX sample's two draws come here in shade.wgsl; nothing
-                    // after them reads the stream, so they are not drawn)
-                    cc = cc + bounced;
-                    col = col + cc;
-                }
-                a_l = a_l + col;
-                smp++;
-                if (smp < spp) { stage = S_CAM; continue; }
-                const uint32_t i = pix;
-                const float fs = (float)spp;
-                float cov = (keep ? ldf(px, 23 * n + i) : 0.f) + (float)a_miss;
-                float film = (keep ? ldf(px, 24 * n + i) : 0.f) + fs;
-                float gone = (keep ? ldf(px, 33 * n + i) : 0.f) + (float)a_miss;
-                uint32_t taken = (keep ? px[31 * n + i] : 0u) + spp;
-                if (frame0 == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
-                stf(px, n + 3 * i, a_l.x); stf(px, n + 3 * i + 1, a_l.y); stf(px, n + 3 * i + 2, a_l.z);
-                stf(px, 7 * n + 3 * i, a_ga.x); stf(px, 7 * n + 3 * i + 1, a_ga.y); stf(px, 7 * n + 3 * i + 2, a_ga.z);
-                stf(px, 10 * n + 3 * i, a_gn.x); stf(px, 10 * n + 3 * i + 1, a_gn.y); stf(px, 10 * n + 3 * i + 2, a_gn.z);
-                stf(px, 13 * n + i, a_gd);
-                if (!keep) { px[20 * n + 3 * i] = 0u; px[20 * n + 3 * i + 1] = 0u; px[20 * n + 3 * i + 2] = 0u; }
-                stf(px, 23 * n + i, cov);
-                stf(px, 24 * n + i, film);
-                px[25 * n + i] = a_outline;
-                px[31 * n + i] = taken;
-                stf(px, 33 * n + i, gone);
-                have = false;
-            }
-        }
-        h = PhrHit{PHR_INF, PHR_NONE, 0.f, 0.f, 0u};
-        if (want) {
-            V3 ro, rd;
-            float rt = PHR_INF;
-            bool any = false;
-            if (stage == S_MAIN) { ro = P0; rd = P1; }
-            else if (stage == S_IND) { ro = iorg; rd = idir; }
-            else if (stage == S_AO) { ro = P0; rd = idir; rt = p->ao; any = true; }
-            else {  // S_SHADOW
-                ro = ind ? iorg : P0;
-                rd = qi == 0 ? q0d : (qi == 1 ? q1d : q2d);
-                rt = qi == 0 ? q0t : (qi == 1 ? q1t : q2t);
-                any = true;
-            }
-            h.t = rt;
-            if (any) cnt_shadow++; else cnt_closest++;
-            h = walk_rt(sc, ro, rd, rt, any, stk, cnt_steps, cnt_tests);
-        }
-    }
-    unsigned long long rc = cnt_closest, rs = cnt_shadow, st = cnt_steps, tt = cnt_tests;
-    for (int k = 32; k > 0; k >>= 1) {
-        rc += __shfl_xor(rc, k);
-        rs += __shfl_xor(rs, k);
-        st += __shfl_xor(st, k);
-        tt += __shfl_xor(tt, k);
-    }
-    if (lane == 0 && (rc | rs)) {
-        atomicAdd(&counters[0], rc);
-        atomicAdd(&counters[1], rs);
-        atomicAdd(&counters[2], st);
-        atomicAdd(&counters[3], tt);
-    }
-}
-
-__global__ void k_zero_u32(uint32_t* p) {
-    if (threadIdx.x == 0) *p = 0u;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1177,8 +764,6 @@ struct phr_tracer {
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
-    uint32_t* next_pixel = nullptr;  // the persistent kernel's work counter
-    uint32_t path_blocks = 0;
     double last_gpu_s = 0.0, last_submit_s = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1214,7 +799,6 @@ void tracer_release(phr_tracer* t) {
     if (t->bvh) phr_bvh_free(t->bvh);
     if (t->rays) (void)hipFree(t->rays);
     if (t->cam_dev) (void)hipFree(t->cam_dev);
-    if (t->next_pixel) (void)hipFree(t->next_pixel);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
     delete t;
@@ -1356,7 +940,7 @@ int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, ph
     if ((rc = phr_array_alloc(g, 16, n, &t->faces)) != PHR_OK) return fail(rc);
     if ((rc = phr_array_alloc(g, 48, n, &t->tris)) != PHR_OK) return fail(rc);
     if (hipMalloc(&t->rays, 48) != hipSuccess || hipMemsetAsync(t->rays, 0, 48, s) != hipSuccess ||
-        hipMalloc(&t->cam_dev, sizeof(PhrCamera)) != hipSuccess || hipMalloc(&t->next_pixel, 64) != hipSuccess) {
+        hipMalloc(&t->cam_dev, sizeof(PhrCamera)) != hipSuccess) {
         phr_set_error_msg("phr_tracer_new: allocation failed");
         return fail(PHR_ERR_HIP);
     }
@@ -1383,10 +967,6 @@ int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, ph
         phr_set_error_msg("phr_tracer_new: event creation failed");
         return fail(PHR_ERR_HIP);
     }
-    // The persistent path kernel fills the device: as many blocks as fit at once.
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_path_sm<false>, TRACE_THREADS, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    t->path_blocks = (uint32_t)(per_cu * std::max(g->cu_count, 1));
     *out = t;
     return PHR_OK;
 }
@@ -1494,21 +1074,8 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     auto t0 = std::chrono::steady_clock::now();
     PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->last, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
     PHR_HIP(hipEventRecord(t->ev0, s));
-    static const bool mega = getenv("PHIRAY_PATH_KERNEL") && std::strcmp(getenv("PHIRAY_PATH_KERNEL"), "mega") == 0;
-    if (mega) {
-        hipLaunchKernelGGL(k_path, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
-                           (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
-    } else {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, t->next_pixel);
-        uint32_t blocks = (uint32_t)std::min<uint64_t>(t->path_blocks, (n + TRACE_THREADS - 1) / TRACE_THREADS);
-        const bool rig = cam.count == 0u && !(cam.env[0] * cam.env[0] + cam.env[1] * cam.env[1] + cam.env[2] * cam.env[2] > 0.f);
-        if (rig)
-            hipLaunchKernelGGL(k_path_sm<true>, dim3(blocks), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, 0u,
-                               (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays, t->next_pixel);
-        else
-            hipLaunchKernelGGL(k_path_sm<false>, dim3(blocks), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, 0u,
-                               (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays, t->next_pixel);
-    }
+    hipLaunchKernelGGL(k_path, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
+                       (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(t->ev1, s));
     t->last_submit_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

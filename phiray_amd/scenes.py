@@ -11,6 +11,10 @@ IEEE round to nearest), so the same inputs are produced wherever this runs.
   u = (x >> 40) * 2^-24; centroid ~ U[0,1]^3, vertices at centroid +
   U[-0.01, 0.01]^3; the refit perturbation takes the stream's next draws,
   U[-0.001, 0.001]^3 per vertex.
+* `batch_scenes(count, lo, hi, seed)` — the batch of independent scenes:
+  scene i has round(exp(U[ln lo, ln hi])) triangles (U from SplitMix64 at
+  `seed`, in float64) made by `lbvh_scene` at seed i; returns the
+  concatenated boxes and the segment starts.
 * `pinhole_rays(w, h)` — primary rays from (0.5, 0.5, -1.5) along +z, 45°
   horizontal field of view, through pixel centres.
 * `spheres_scene(...)` — icospheres on a two-triangle ground plane under one
@@ -127,6 +131,20 @@ def tris_from_corners(corners: np.ndarray, color=(0.5, 0.5, 0.5, 1.0)) -> np.nda
 
 def corner_boxes(corners: np.ndarray) -> np.ndarray:
     return of_tri(corners[:, 0], corners[:, 1], corners[:, 2])
+
+
+def batch_counts(count: int, lo: float = 1e3, hi: float = 2e5, seed: int = 0x5C3) -> np.ndarray:
+    """Triangle counts of the batch: round(exp(U[ln lo, ln hi])) each."""
+    u = (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def batch_scenes(count: int = 256, lo: float = 1e3, hi: float = 2e5, seed: int = 0x5C3) -> tuple[np.ndarray, np.ndarray]:
+    """Boxes of `count` independent scenes back to back, and the segment starts."""
+    counts = batch_counts(count, lo, hi, seed)
+    boxes = [corner_boxes(lbvh_scene(int(m), i)[0]) for i, m in enumerate(counts)]
+    seg_lo = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    return np.concatenate(boxes), seg_lo
 
 
 def pinhole_rays(w: int = 256, h: int = 256, origin=(0.5, 0.5, -1.5), fov_deg: float = 45.0) -> np.ndarray:
@@ -247,3 +265,151 @@ def path_camera(w: int, h: int, eye=(0.0, -6.0, 2.5), target=(0.0, 0.0, 0.3), fo
     c["rr_from"] = rr_from
     c["ao"] = ao
     return cam
+
+
+# ---- scenes of meshes under instances ----------------------------------------
+def _signum(x: np.ndarray) -> np.ndarray:
+    """Rust f32::signum: the sign bit decides (so -0.0 gives -1)."""
+    return np.where(np.signbit(x), F32(-1.0), F32(1.0)).astype(F32)
+
+
+def oct_pack(n: np.ndarray) -> np.ndarray:
+    """src/trace/scene.rs:22-38 in float32: octahedral normal, (2, 2) for a zero normal."""
+    n = np.asarray(n, dtype=F32).reshape(-1, 3)
+    ax, ay, az = np.abs(n[:, 0]), np.abs(n[:, 1]), np.abs(n[:, 2])
+    l = ((ax + ay) + az).astype(F32)
+    safe = np.where(l < F32(1e-12), F32(1.0), l).astype(F32)
+    x, y, z = (n[:, 0] / safe).astype(F32), (n[:, 1] / safe).astype(F32), (n[:, 2] / safe).astype(F32)
+    up = z >= F32(0.0)
+    ox = np.where(up, x, ((F32(1.0) - np.abs(y)) * _signum(x)).astype(F32))
+    oy = np.where(up, y, ((F32(1.0) - np.abs(x)) * _signum(y)).astype(F32))
+    out = np.stack([ox, oy], axis=1).astype(F32)
+    out[l < F32(1e-12)] = 0.0
+    out[(n == 0).all(axis=1)] = 2.0
+    return out
+
+
+def world_of(m: np.ndarray) -> np.ndarray:
+    """A 4 x 4 matrix M (p' = M [p, 1]) as an instance's m[col][row] words."""
+    return np.ascontiguousarray(np.asarray(m, dtype=F32).T.reshape(16))
+
+
+def rotation(axis, angle: float) -> np.ndarray:
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    c, s = np.cos(angle), np.sin(angle)
+    x, y, z = a
+    return np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s],
+                     [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s],
+                     [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c)]])
+
+
+def affine(rot=None, scale=1.0, move=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """M = translate(move) @ rot @ scale, as a float32 4 x 4."""
+    m = np.eye(4)
+    r = np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64)
+    m[:3, :3] = r @ np.diag(np.broadcast_to(np.asarray(scale, dtype=np.float64), (3,)))
+    m[:3, 3] = move
+    return m.astype(F32)
+
+
+class SceneBuilder:
+    """A scene of meshes (own space, own tree) placed by instances, as
+    src/trace/scene.rs builds one with `instance`, `add` and `place`."""
+
+    def __init__(self):
+        self._verts, self._faces, self._mats, self._meshes, self._insts = [], [], [], [], []
+        self._nv = self._nf = 0
+
+    def material(self, color=(0.5, 0.5, 0.5, 1.0), rough=1.0, metal=0.0, trans=0.0, ior=1.0, emit=(0.0, 0.0, 0.0),
+                 bump=0.0) -> int:
+        from . import MAT
+        m = np.zeros(1, dtype=MAT)
+        m["color"], m["rough"], m["metal"], m["trans"], m["ior"], m["emit"], m["bump"] = color, rough, metal, trans, ior, emit, bump
+        self._mats.append(m)
+        return len(self._mats) - 1
+
+    def mesh(self, pos, tris, mat, normals=None) -> int:
+        """A mesh over `pos` [v, 3] and faces `tris` [f, 3] (ids into pos);
+        `mat` one id or one per face; `normals` [v, 3] or flat."""
+        from . import FACE, MESH, VERT
+        pos = np.asarray(pos, dtype=F32).reshape(-1, 3)
+        tris = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+        v = np.zeros(len(pos), dtype=VERT)
+        v["p"][:, :3] = pos
+        v["p"][:, 3] = 1.0
+        nrm = np.zeros_like(pos) if normals is None else np.asarray(normals, dtype=F32).reshape(-1, 3)
+        v["a"][:, :2] = oct_pack(nrm)
+        f = np.zeros(len(tris), dtype=FACE)
+        f["v"] = tris + self._nv
+        f["mat"] = mat
+        m = np.zeros(1, dtype=MESH)
+        m["face_lo"], m["face_hi"] = self._nf, self._nf + len(tris)
+        m["vert_lo"], m["vert_hi"] = self._nv, self._nv + len(pos)
+        if len(pos):  # Mesh::fit (src/trace/mesh.rs:56-72)
+            m["lo"], m["hi"] = pos.min(0), pos.max(0)
+        else:
+            m["lo"], m["hi"] = 0.0, 1.0
+        self._verts.append(v)
+        self._faces.append(f)
+        self._meshes.append(m)
+        self._nv += len(pos)
+        self._nf += len(tris)
+        return len(self._meshes) - 1
+
+    def instance(self, mesh: int, m=None, casts_shadow: bool = True, color=(1.0, 1.0, 1.0), hash: int = 0) -> int:
+        from . import INSTANCE
+        i = np.zeros(1, dtype=INSTANCE)
+        i["world"] = world_of(np.eye(4) if m is None else m)
+        i["mesh"], i["casts_shadow"], i["color"], i["hash"] = mesh, 1 if casts_shadow else 0, color, hash
+        self._insts.append(i)
+        return len(self._insts) - 1
+
+    def arrays(self):
+        from . import FACE, INSTANCE, MAT, MESH, VERT
+        cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dtype=dt)  # noqa: E731
+        return (cat(self._verts, VERT), cat(self._faces, FACE), cat(self._mats, MAT), cat(self._meshes, MESH),
+                cat(self._insts, INSTANCE))
+
+
+def box_mesh() -> tuple[np.ndarray, np.ndarray]:
+    """The unit cube [-1, 1]^3: 8 corners, 12 outward-wound triangles."""
+    p = np.array([[x, y, z] for z in (-1, 1) for y in (-1, 1) for x in (-1, 1)], dtype=np.float64)
+    q = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    t = []
+    for a, b, c, d in q:
+        t += [(a, b, c), (a, c, d)]
+    return p, np.array(t, dtype=np.int64)
+
+
+def bodies_scene(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, light: float = 30.0):
+    """The rigid-body workload: `bodies` shapes, each its own mesh under one
+    instance (a box or an icosphere of subdivision 0..5, picked per body),
+    radius U[0.2, 0.6], turned about a random axis, placed uniformly over a
+    spread x spread x 4 box above a ground plane, under one emissive quad.
+    Returns (builder, world matrices [bodies, 4, 4])."""
+    b = SceneBuilder()
+    u = uniform(seed, bodies * 9).reshape(bodies, 9).astype(np.float64)
+    shapes = [box_mesh()] + [icosphere(k) for k in range(6)]
+    mats = [b.material(color=(0.2 + 0.6 * u[i, 6], 0.2 + 0.6 * u[i, 7], 0.2 + 0.6 * u[i, 8], 1.0)) for i in range(8)]
+    worlds = []
+    for i in range(bodies):
+        k = min(int(u[i, 0] * 7), 6)
+        pos, tris = shapes[k]
+        normals = None if k == 0 else pos
+        mesh = b.mesh(pos, tris, mats[i % 8], normals)
+        r = 0.2 + 0.4 * u[i, 1]
+        ang = 2 * np.pi * u[i, 2]
+        axis = (u[i, 3] - 0.5, u[i, 4] - 0.5, 0.5)
+        move = ((u[i, 5] - 0.5) * spread, (u[i, 6] - 0.5) * spread, 0.6 + 4.0 * u[i, 7])
+        m = affine(rotation(axis, ang), r, move)
+        worlds.append(m)
+        b.instance(mesh, m)
+    h = spread
+    ground = b.mesh([[-h, -h, 0], [h, -h, 0], [h, h, 0], [-h, h, 0]], [[0, 1, 2], [0, 2, 3]], b.material())
+    b.instance(ground)
+    s = 1.5
+    lamp = b.mesh([[-s, -s, 0], [-s, s, 0], [s, s, 0], [s, -s, 0]], [[0, 1, 2], [0, 2, 3]],
+                  b.material(color=(0, 0, 0, 1), emit=(light, light, light)))
+    b.instance(lamp, affine(move=(0.0, 0.0, 8.0)))
+    return b, np.array(worlds)
