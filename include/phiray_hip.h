@@ -193,6 +193,65 @@ phr_array* phr_tracer_image(phr_tracer* t);
 int phr_tracer_denoise(phr_tracer* t, int keep);
 /* Tracer::radiance (src/trace/read.rs:241-251): w*h rgb floats. */
 int phr_tracer_radiance(phr_tracer* t, float* out);
+/* Tracer::albedo / normal / depth (src/trace/read.rs:253-307): the guide
+ * planes' sums over the per-pixel count; w*h rgb, rgb and one float. */
+int phr_tracer_albedo(phr_tracer* t, float* out);
+int phr_tracer_normal(phr_tracer* t, float* out);
+int phr_tracer_depth_plane(phr_tracer* t, float* out);
+
+/* ---- a scene of meshes under instances: the forest -------------------------- */
+/* A mesh (src/trace/mesh.rs:12-22): its face and vertex ranges and the box of
+ * its vertices (Mesh::fit). 40 bytes. */
+typedef struct {
+    uint32_t face_lo, face_hi, vert_lo, vert_hi;
+    float lo[3], hi[3];
+} phr_mesh;
+/* An instance (src/trace/instance.rs:23-44): world is m[col][row] with the
+ * translation in m[3]; casts_shadow 0 lets shadow rays pass it. 88 bytes. */
+typedef struct {
+    float world[16];
+    uint32_t mesh;
+    uint32_t casts_shadow;
+    float color[3];
+    uint32_t hash;
+} phr_instance;
+/* A scene (src/trace/scene.rs:40-67): vertices 32 B each (object-space
+ * position, w = 1, packed normal xy, uv), faces 16 B (3 vertex ids and a
+ * material), materials 64 B (color rgba, rough, metal, trans, ior, emit rgb,
+ * bump, 4 zero words), meshes tiling the faces in order, instances. */
+typedef struct {
+    const void* verts; uint32_t nverts;
+    const void* faces; uint32_t nfaces;
+    const void* mats; uint32_t nmats;
+    const phr_mesh* meshes; uint32_t nmeshes;
+    const phr_instance* insts; uint32_t ninsts;
+} phr_scene_desc;
+/* Tracer::from_scene (src/trace/build.rs:15-25, src/trace/forest.rs:44-66):
+ * one tree per mesh over its faces in its own space and a top tree over the
+ * instances' world boxes, all built on the device in two LBVH launches
+ * (the mesh trees as one batch), laid out as Forest::nodes lays them out.
+ * PHR_ERR_ARG for an empty scene or one whose instances place nothing (None),
+ * PHR_ERR_UNSUPPORTED for a material the path does not carry (transmission). */
+int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* scene, uint32_t leaf, phr_tracer** out);
+/* Tracer::reinstance (src/trace/pose.rs:102-125): new matrices (16 floats each,
+ * m[col][row]); the top tree, instance table and light table follow, the
+ * image starts over. *ok = 0 when the count differs or the tracer was not
+ * built from a scene (build it again). */
+int phr_tracer_reinstance(phr_tracer* t, const float* worlds, uint32_t n, int* ok);
+/* Tracer::repose (src/trace/pose.rs:70-100): the same faces over vertices
+ * moved (32 B each, object space): every mesh tree refitted (one batched
+ * launch sequence), the top tree rebuilt, the image started over. *ok = 0
+ * when the vertex count differs. The _on form takes a device array. */
+int phr_tracer_repose(phr_tracer* t, const void* verts, uint32_t nverts, int* ok);
+int phr_tracer_repose_on(phr_tracer* t, phr_array* verts, int* ok);
+/* Tracer::retree (src/trace/pose.rs:130-155): every mesh tree built again over
+ * the pose held, faces put in their new order; the image keeps its samples. */
+int phr_tracer_retree(phr_tracer* t, int* ok);
+/* The forest as the kernels read it: the node array (top tree first) and the
+ * instance words (Forest::inst_words, src/trace/forest.rs:168-181). */
+phr_array* phr_tracer_nodes(phr_tracer* t);
+phr_array* phr_tracer_instances(phr_tracer* t);
+uint32_t phr_tracer_instance_count(const phr_tracer* t);
 
 #ifdef __cplusplus
 }

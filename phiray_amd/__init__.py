@@ -142,6 +142,17 @@ _SIGS = {
     "phr_tracer_image": (_P, [_P]),
     "phr_tracer_radiance": (_I, [_P, _P]),
     "phr_tracer_denoise": (_I, [_P, _I]),
+    "phr_tracer_albedo": (_I, [_P, _P]),
+    "phr_tracer_normal": (_I, [_P, _P]),
+    "phr_tracer_depth_plane": (_I, [_P, _P]),
+    "phr_tracer_from_scene": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "phr_tracer_reinstance": (_I, [_P, _P, _U32, C.POINTER(_I)]),
+    "phr_tracer_repose": (_I, [_P, _P, _U32, C.POINTER(_I)]),
+    "phr_tracer_repose_on": (_I, [_P, _P, C.POINTER(_I)]),
+    "phr_tracer_retree": (_I, [_P, C.POINTER(_I)]),
+    "phr_tracer_nodes": (_P, [_P]),
+    "phr_tracer_instances": (_P, [_P]),
+    "phr_tracer_instance_count": (_U32, [_P]),
 }
 
 _lib = None
@@ -505,6 +516,11 @@ class GpuBvh:
             pass
 
 
+class _SceneDesc(C.Structure):
+    _fields_ = [("verts", _P), ("nverts", _U32), ("faces", _P), ("nfaces", _U32), ("mats", _P), ("nmats", _U32),
+                ("meshes", _P), ("nmeshes", _U32), ("insts", _P), ("ninsts", _U32)]
+
+
 class Tracer:
     """phiray's `Tracer` (src/trace/mod.rs:50-245) for the Rendered path."""
 
@@ -524,6 +540,57 @@ class Tracer:
         out = C.c_void_p()
         _check(lib().phr_tracer_new(gpu._h, _ptr(tris), len(tris), leaf, C.byref(out)), "Tracer::new")
         return cls(gpu, out.value)
+
+    @classmethod
+    def from_scene(cls, gpu: Gpu, verts: np.ndarray, faces: np.ndarray, mats: np.ndarray, meshes: np.ndarray,
+                   insts: np.ndarray, leaf: int = 4) -> "Tracer | None":
+        """`Tracer::from_scene` over meshes under instances (the forest); None when empty."""
+        arrs = [np.ascontiguousarray(verts, dtype=VERT), np.ascontiguousarray(faces, dtype=FACE),
+                np.ascontiguousarray(mats, dtype=MAT), np.ascontiguousarray(meshes, dtype=MESH),
+                np.ascontiguousarray(insts, dtype=INSTANCE)]
+        if len(arrs[1]) == 0 or len(arrs[4]) == 0:
+            return None
+        desc = _SceneDesc(*[x for a in arrs for x in (_ptr(a), len(a))])
+        out = C.c_void_p()
+        _check(lib().phr_tracer_from_scene(gpu._h, C.byref(desc), leaf, C.byref(out)), "Tracer::from_scene")
+        t = cls(gpu, out.value)
+        t._scene = arrs
+        return t
+
+    def reinstance(self, worlds: np.ndarray) -> bool:
+        """`Tracer::reinstance`: worlds [n, 16] as m[col][row]; False = build again."""
+        w = np.ascontiguousarray(worlds, dtype=np.float32).reshape(-1, 16)
+        ok = C.c_int(0)
+        _check(lib().phr_tracer_reinstance(self._h, _ptr(w), len(w), C.byref(ok)), "Tracer::reinstance")
+        return bool(ok.value)
+
+    def repose(self, verts: np.ndarray) -> bool:
+        """`Tracer::repose`: the same faces over moved vertices; False = build again."""
+        v = np.ascontiguousarray(verts, dtype=VERT)
+        ok = C.c_int(0)
+        _check(lib().phr_tracer_repose(self._h, _ptr(v), len(v), C.byref(ok)), "Tracer::repose")
+        return bool(ok.value)
+
+    def repose_on(self, verts: GpuArray) -> bool:
+        ok = C.c_int(0)
+        _check(lib().phr_tracer_repose_on(self._h, verts._h, C.byref(ok)), "Tracer::repose_on")
+        return bool(ok.value)
+
+    def retree(self) -> bool:
+        """`Tracer::retree`: every mesh tree built again over the pose held."""
+        ok = C.c_int(0)
+        _check(lib().phr_tracer_retree(self._h, C.byref(ok)), "Tracer::retree")
+        return bool(ok.value)
+
+    def nodes(self) -> np.ndarray:
+        return GpuArray._borrow(self.gpu, lib().phr_tracer_nodes(self._h), NODE).read()
+
+    def instances(self) -> np.ndarray:
+        """Forest::inst_words as the kernels read them (records, slot order, draw order)."""
+        h = lib().phr_tracer_instances(self._h)
+        if not h:
+            return np.zeros(0, dtype=np.uint32)
+        return GpuArray._borrow(self.gpu, h, np.uint32).read()
 
     def __len__(self) -> int:
         return int(lib().phr_tracer_len(self._h))
@@ -612,6 +679,24 @@ class Tracer:
         w, h = self.size
         out = np.zeros((w * h, 3), dtype=np.float32)
         _check(lib().phr_tracer_radiance(self._h, _ptr(out)), "Tracer::radiance")
+        return out
+
+    def albedo(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros((w * h, 3), dtype=np.float32)
+        _check(lib().phr_tracer_albedo(self._h, _ptr(out)), "Tracer::albedo")
+        return out
+
+    def normal(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros((w * h, 3), dtype=np.float32)
+        _check(lib().phr_tracer_normal(self._h, _ptr(out)), "Tracer::normal")
+        return out
+
+    def depth_plane(self) -> np.ndarray:
+        w, h = self.size
+        out = np.zeros(w * h, dtype=np.float32)
+        _check(lib().phr_tracer_depth_plane(self._h, _ptr(out)), "Tracer::depth")
         return out
 
     def free(self) -> None:

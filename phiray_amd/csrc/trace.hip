@@ -59,6 +59,9 @@ static_assert(sizeof(PhrTri) == 128, "tri layout");
 constexpr uint32_t MAT_WORDS = 16;
 constexpr uint32_t PX_LEN = 34;  // src/trace/read.rs:56
 
+constexpr uint32_t INSTANCE_LEN = 40;  // words of an instance record (src/trace/instance.rs:18)
+constexpr uint32_t PHR_INST = 1u << 30;  // a top-tree leaf's range is instance slots (src/trace/forest.rs:17)
+
 struct SceneDev {
     const float4* nodes;
     const float4* tris;    // 3 float4 per slot: (a.xyz, b.x) (b.yz, c.xy) (c.z, -, -, -)
@@ -66,6 +69,10 @@ struct SceneDev {
     const float4* verts;   // 2 float4 per vertex: (pos, 1) (oct.xy, u, v)
     const float* mats;     // MAT_WORDS per material
     const uint32_t* lights;
+    // A forest: instance records (world rows 0-11, inverse rows 12-23, tree
+    // root 24, face range 25-26, flags 27), then the top-tree slot order.
+    const uint32_t* insts;
+    uint32_t ninsts;
 };
 
 namespace {
@@ -190,13 +197,132 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const f
     return best;
 }
 
-constexpr uint32_t TRACE_THREADS = 256;
+// trace_blas (bvh.wgsl:120-190) for instance i: the ray taken into the
+// mesh's space by the instance's inverse rows (direction left unnormalised,
+// so t stays the world's) unless the instance sits at the identity (word 27
+// bit 0); a shadow ray passes an instance that casts none (bit 1). The
+// nearest hit so far rides in `best`, so this tree only takes strictly
+// nearer hits. Returns true when an any-hit ray stopped.
+__device__ __forceinline__ float dot4(float4 r, float x, float y, float z, float w) {
+    return ((r.x * x + r.y * y) + r.z * z) + r.w * w;
+}
 
-__global__ __launch_bounds__(TRACE_THREADS) void k_raycast(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                                           const uint4* __restrict__ faces, const float* __restrict__ mats,
-                                                           const PhrRay* __restrict__ rays, uint32_t nrays, uint32_t any,
+template <bool ANY>
+__device__ __forceinline__ bool walk_blas(const SceneDev& s, uint32_t i, V3 ow, V3 dw, PhrHit& best, uint32_t* stk,
+                                          uint32_t stride, uint32_t& steps, uint32_t& tests) {
+    const uint32_t* w = s.insts + INSTANCE_LEN * (size_t)i;
+    const uint4 info = *(const uint4*)(w + 24);
+    const uint32_t root = info.x, flags = info.w;
+    if (root == PHR_NONE) return false;
+    if (ANY && (flags & 2u)) return false;
+    V3 o = ow, d = dw;
+    if ((flags & 1u) == 0u) {
+        const float4 r0 = *(const float4*)(w + 12), r1 = *(const float4*)(w + 16), r2 = *(const float4*)(w + 20);
+        o = v3(dot4(r0, ow.x, ow.y, ow.z, 1.0f), dot4(r1, ow.x, ow.y, ow.z, 1.0f), dot4(r2, ow.x, ow.y, ow.z, 1.0f));
+        d = v3((r0.x * dw.x + r0.y * dw.y) + r0.z * dw.z, (r1.x * dw.x + r1.y * dw.y) + r1.z * dw.z,
+               (r2.x * dw.x + r2.y * dw.y) + r2.z * dw.z);
+    }
+    const V3 inv = safe_inv(d);
+    float4 lo = s.nodes[2 * root], hi = s.nodes[2 * root + 1];
+    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return false;
+    const RaySpace rs = ray_space(d);
+    uint32_t sp = 0;
+    for (;;) {
+        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
+        if (li & PHR_LEAF) {
+            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
+                ++tests;
+                float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
+                float u, v;
+                float t = tri_t(t0, t1, t2, o, rs, u, v);
+                if (t < best.t) {
+                    if (ANY) {
+                        const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
+                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
+                        best = PhrHit{t, id, u, v, i};
+                        return true;
+                    }
+                    best = PhrHit{t, id, u, v, i};
+                }
+            }
+        } else {
+            ++steps;
+            float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
+            float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
+            float tl = box_t(llo, lhi, o, inv, best.t);
+            float tr = box_t(rlo, rhi, o, inv, best.t);
+            if (tl < PHR_INF && tr < PHR_INF) {
+                if (tl <= tr) { stk[sp * stride] = ri; lo = llo; hi = lhi; }
+                else { stk[sp * stride] = li; lo = rlo; hi = rhi; }
+                ++sp;
+                continue;
+            }
+            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
+            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
+        }
+        if (sp == 0) break;
+        --sp;
+        uint32_t x = stk[sp * stride];
+        lo = s.nodes[2 * x];
+        hi = s.nodes[2 * x + 1];
+    }
+    return false;
+}
+
+// trace (bvh.wgsl:194-232): the top tree over the instances' world boxes; a
+// leaf's slots name the instances (inst_order) whose trees the ray walks.
+// The top tree's stack is the column's first PHR_STACK words, a mesh tree's
+// the next PHR_STACK.
+template <bool ANY>
+__device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint32_t* stk, uint32_t stride, uint32_t& steps,
+                              uint32_t& tests) {
+    PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
+    const uint32_t* inst_order = s.insts + INSTANCE_LEN * (size_t)s.ninsts;
+    const V3 inv = safe_inv(d);
+    float4 lo = s.nodes[0], hi = s.nodes[1];
+    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
+    uint32_t* bstk = stk + PHR_STACK * stride;
+    uint32_t sp = 0;
+    for (;;) {
+        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
+        if (li & PHR_LEAF) {
+            for (uint32_t k = li & ~(PHR_LEAF | PHR_INST); k < ri; ++k) {
+                walk_blas<ANY>(s, inst_order[k], o, d, best, bstk, stride, steps, tests);
+                if (ANY && best.tri != PHR_NONE) return best;
+            }
+        } else {
+            ++steps;
+            float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
+            float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
+            float tl = box_t(llo, lhi, o, inv, best.t);
+            float tr = box_t(rlo, rhi, o, inv, best.t);
+            if (tl < PHR_INF && tr < PHR_INF) {
+                if (tl <= tr) { stk[sp * stride] = ri; lo = llo; hi = lhi; }
+                else { stk[sp * stride] = li; lo = rlo; hi = rhi; }
+                ++sp;
+                continue;
+            }
+            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
+            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
+        }
+        if (sp == 0) break;
+        --sp;
+        uint32_t x = stk[sp * stride];
+        lo = s.nodes[2 * x];
+        hi = s.nodes[2 * x + 1];
+    }
+    return best;
+}
+
+constexpr uint32_t TRACE_THREADS = 256;
+// LDS stack words a thread: one tree, or the top tree and a mesh tree.
+template <bool F>
+constexpr uint32_t stack_words() { return F ? 2 * PHR_STACK : PHR_STACK; }
+
+template <bool F>
+__global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const PhrRay* __restrict__ rays, uint32_t nrays, uint32_t any,
                                                            PhrHit* __restrict__ hits, unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
     const uint32_t i = blockIdx.x * TRACE_THREADS + threadIdx.x;
     uint32_t steps = 0, tests = 0;
     if (i < nrays) {
@@ -204,8 +330,13 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(const float4* __restr
         float4 a = r[0], b = r[1];
         V3 o = v3(a.x, a.y, a.z), d = v3(b.x, b.y, b.z);
         uint32_t* stk = s_stack + threadIdx.x;
-        PhrHit h = any ? walk<true>(nodes, tris, faces, mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
-                       : walk<false>(nodes, tris, faces, mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+        PhrHit h;
+        if constexpr (F)
+            h = any ? walk_forest<true>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests)
+                    : walk_forest<false>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+        else
+            h = any ? walk<true>(sc.nodes, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
+                    : walk<false>(sc.nodes, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
         hits[i] = h;
     }
     if (counters) {
@@ -264,16 +395,22 @@ struct Ctx {
     uint32_t closest, shadow, steps, tests;
 };
 
+// F: the scene is a forest (top tree + instances' trees); else one tree under
+// one identity instance.
+template <bool F>
 __device__ __forceinline__ PhrHit trace(Ctx& c, V3 o, V3 d, float tmax, bool first) {
     if (first) {
         c.shadow++;
+        if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
         return walk<true>(c.s.nodes, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
     }
     c.closest++;
+    if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
     return walk<false>(c.s.nodes, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
 }
+template <bool F>
 __device__ __forceinline__ float visible(Ctx& c, V3 org, V3 l, float dist) {
-    return trace(c, org, l, dist, true).tri != PHR_NONE ? 0.0f : 1.0f;
+    return trace<F>(c, org, l, dist, true).tri != PHR_NONE ? 0.0f : 1.0f;
 }
 
 __device__ __forceinline__ V3 unoct(float ex, float ey) {  // layout.wgsl:79-86
@@ -285,7 +422,18 @@ __device__ __forceinline__ V3 unoct(float ex, float ey) {  // layout.wgsl:79-86
 
 __device__ __forceinline__ uint32_t mat_at(const SceneDev& s, uint32_t f) { return s.faces[f].w; }
 
-__device__ Tri tri_of(const SceneDev& s, uint32_t f, uint32_t m) {  // layout.wgsl:114-158
+__device__ __forceinline__ V3 place_n(float4 i0, float4 i1, float4 i2, V3 n) {  // layout.wgsl:70-77
+    V3 m = (v3(i0.x, i0.y, i0.z) * n.x + v3(i1.x, i1.y, i1.z) * n.y) + v3(i2.x, i2.y, i2.z) * n.z;
+    float l = length(m);
+    if (l < 1e-12f) return v3(0, 0, 0);
+    return m / l;
+}
+
+// Face f under instance i with material m (layout.wgsl:114-158): in a forest
+// its corners and normals are placed in the world unless the instance sits
+// at the identity.
+template <bool F>
+__device__ Tri tri_of(const SceneDev& s, uint32_t f, uint32_t i, uint32_t m) {
     uint4 fc = s.faces[f];
     float4 xp = s.verts[2 * fc.x], xa = s.verts[2 * fc.x + 1];
     float4 yp = s.verts[2 * fc.y], ya = s.verts[2 * fc.y + 1];
@@ -295,6 +443,19 @@ __device__ Tri tri_of(const SceneDev& s, uint32_t f, uint32_t m) {  // layout.wg
     Tri t;
     t.a = v3(xp.x, xp.y, xp.z); t.b = v3(yp.x, yp.y, yp.z); t.c = v3(zp.x, zp.y, zp.z);
     t.na = unoct(xa.x, xa.y); t.nb = unoct(ya.x, ya.y); t.nc = unoct(za.x, za.y);
+    if constexpr (F) {
+        const uint32_t* w = s.insts + INSTANCE_LEN * (size_t)i;
+        if ((w[27] & 1u) == 0u) {
+            const float4* r = (const float4*)w;
+            const float4 r0 = r[0], r1 = r[1], r2 = r[2], i0 = r[3], i1 = r[4], i2 = r[5];
+            t.a = v3(dot4(r0, xp.x, xp.y, xp.z, xp.w), dot4(r1, xp.x, xp.y, xp.z, xp.w), dot4(r2, xp.x, xp.y, xp.z, xp.w));
+            t.b = v3(dot4(r0, yp.x, yp.y, yp.z, yp.w), dot4(r1, yp.x, yp.y, yp.z, yp.w), dot4(r2, yp.x, yp.y, yp.z, yp.w));
+            t.c = v3(dot4(r0, zp.x, zp.y, zp.z, zp.w), dot4(r1, zp.x, zp.y, zp.z, zp.w), dot4(r2, zp.x, zp.y, zp.z, zp.w));
+            t.na = place_n(i0, i1, i2, t.na);
+            t.nb = place_n(i0, i1, i2, t.nb);
+            t.nc = place_n(i0, i1, i2, t.nc);
+        }
+    }
     t.color[0] = m0.x; t.color[1] = m0.y; t.color[2] = m0.z; t.color[3] = m0.w;
     t.rough = m1.x; t.metal = m1.y; t.trans = m1.z; t.ior = m1.w;
     t.emit = v3(m2.x, m2.y, m2.z);
@@ -421,6 +582,7 @@ __device__ float lamp_chance(const SceneDev& s, const PhrHit& h) {  // light.wgs
     return __uint_as_float(d[1 + 2 * count + lo]) - prev;
 }
 
+template <bool F>
 __device__ Shadowed glow_query(const SceneDev& sc, const Surf& s, uint32_t* seed) {  // light.wgsl:281-312
     const uint32_t* d = sc.lights;
     uint32_t count = d[0];
@@ -434,7 +596,7 @@ __device__ Shadowed glow_query(const SceneDev& sc, const Surf& s, uint32_t* seed
     float prev = lo > 0 ? __uint_as_float(d[2 * count + lo]) : 0.0f;
     float chance = fmax_(__uint_as_float(d[1 + 2 * count + lo]) - prev, 1e-8f);
     uint32_t f = d[1 + lo];
-    Tri t = tri_of(sc, f, mat_at(sc, f));
+    Tri t = tri_of<F>(sc, f, d[1 + count + lo], mat_at(sc, f));
     float a = rnd(seed);
     float b = rnd(seed);
     if (a + b > 1.0f) { a = 1.0f - a; b = 1.0f - b; }
@@ -452,6 +614,7 @@ __device__ Shadowed glow_query(const SceneDev& sc, const Surf& s, uint32_t* seed
     return Shadowed{brdf_w(s, l, e, 1.0f, 1.0f) * w, l, sqrtf(d2) * 0.999f};
 }
 
+template <bool F>
 __device__ V3 direct(Ctx& c, const Surf& s, uint32_t* seed) {  // integrate.wgsl:66-85
     const PhrCamera* p = c.p;
     bool rig = p->count == 0u && !world_lit(p);
@@ -475,9 +638,9 @@ __device__ V3 direct(Ctx& c, const Surf& s, uint32_t* seed) {  // integrate.wgsl
                 q = Shadowed{brdf_w(s, rim, v3(0.9f, 0.9f, 0.9f), 1.0f, 1.0f), rim, PHR_INF};
             }
         } else {
-            q = glow_query(c.s, s, seed);
+            q = glow_query<F>(c.s, s, seed);
         }
-        if (maxc(q.c) > 0.f) sum = sum + q.c * visible(c, s.pos, q.dir, q.dist);
+        if (maxc(q.c) > 0.f) sum = sum + q.c * visible<F>(c, s.pos, q.dir, q.dist);
     }
     return sum;
 }
@@ -499,12 +662,13 @@ __device__ V3 glow_mis(const SceneDev& s, const Tri& t, const PhrHit& h, V3 from
     return e * pb * pb / fmax_(pb * pb + pl * pl, 1e-12f);
 }
 
+template <bool F>
 __device__ V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // integrate.wgsl:111-230
     const PhrCamera* p = c.p;
     bool lit = world_lit(p);
     if (p->glow == 0u && !lit) {
         V3 dir = cosine_dir(n, seed);
-        return env_rgb(p) * mixf(0.25f, 1.0f, visible(c, pos, dir, p->ao));
+        return env_rgb(p) * mixf(0.25f, 1.0f, visible<F>(c, pos, dir, p->ao));
     }
     uint32_t depth = bounce ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
     V3 acc = v3(0, 0, 0), thr = v3(1, 1, 1), org = pos, nrm = n;
@@ -513,13 +677,13 @@ __device__ V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // 
         V3 dir = cosine_dir(nrm, seed);
         float pb = fmax_(dot(nrm, dir), 0.0f) / PHR_PI;
         min_pdf = fmin_(min_pdf, pb);
-        PhrHit h = trace(c, org, dir, PHR_INF, false);
+        PhrHit h = trace<F>(c, org, dir, PHR_INF, false);
         if (h.tri == PHR_NONE) {
             acc = acc + thr * env_rgb(p) * 1.0f;
             break;
         }
         if (!bounce) break;
-        Tri t = tri_of(c.s, h.tri, mat_at(c.s, h.tri));
+        Tri t = tri_of<F>(c.s, h.tri, h.inst, mat_at(c.s, h.tri));
         V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
         V3 ng = dot(ng0, dir) < 0.f ? ng0 : -ng0;
         V3 hit = org + dir * h.t + ng * (p->eps + h.t * 1e-5f);
@@ -534,7 +698,7 @@ __device__ V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // 
         acc = acc + thr * glow_mis(c.s, t, h, nrm, dir);
         float near = (!lit && h.t <= p->ao) ? 0.25f : 1.0f;
         Surf s{hit, ng, -dir, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
-        acc = acc + thr * direct(c, s, seed) * near;
+        acc = acc + thr * direct<F>(c, s, seed) * near;
         thr = thr * albedo;
         if (b + 1u < depth && b + 1u >= p->rr_from) {
             float keep = clampf(maxc(thr), 0.0f, 0.95f);
@@ -555,6 +719,7 @@ struct Sample {
     uint32_t outline;
 };
 
+template <bool F>
 __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
     const PhrCamera* p = c.p;
     Sample out;
@@ -588,7 +753,7 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
     uint32_t n_see = 0u;
     uint32_t passes = see_through + through + 1u;
     for (uint32_t bounce = 0; bounce < passes; ++bounce) {
-        PhrHit h = trace(c, o, d, PHR_INF, false);
+        PhrHit h = trace<F>(c, o, d, PHR_INF, false);
         if (h.tri == PHR_NONE) {
             if (bounce == 0u) {
                 out.albedo = v3(0, 0, 0);
@@ -604,10 +769,13 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
         }
         if (bounce == 0u) out.outline = h.inst + 1u;
         uint32_t m = mat_at(c.s, h.tri);
-        Tri t = tri_of(c.s, h.tri, m);
+        Tri t = tri_of<F>(c.s, h.tri, h.inst, m);
         float w = 1.0f - h.u - h.v;
         V3 hit = o + d * h.t;
+        // world_ng (layout.wgsl:96-100): turned back under a mirroring instance.
         V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        if constexpr (F)
+            if (c.s.insts[INSTANCE_LEN * (size_t)h.inst + 27] & 4u) ng0 = -ng0;
         bool front = dot(ng0, d) < 0.f;
         V3 ng = front ? ng0 : -ng0;
         V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
@@ -637,13 +805,13 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
         V3 diffuse = base * (1.0f - t.metal);
         if (maxc(diffuse) > 0.f) diffuse = diffuse * under_layer_real(t.ior, fmax_(t.rough, 0.04f), fmax_(dot(sn, -d), 1e-4f));
         Surf s{pos, sn, -d, diffuse, mixv(fd, base, t.metal), fmax_(t.rough, 0.04f), t.ior, 1.0f - t.metal};
-        V3 cc = direct(c, s, &seed) * 1.0f;
+        V3 cc = direct<F>(c, s, &seed) * 1.0f;
         V3 bounced = v3(0, 0, 0);
         float pd = dot(vmax(s.albedo, 0.0f), v3(0.2126f, 0.7152f, 0.0722f));
         float pg = 0.0f;
         float p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
         bool take_diffuse = rnd(&seed) < p_diffuse;
-        if (take_diffuse && pd > 0.f) bounced = bounced + s.albedo * indirect(c, pos, sn, &seed, diffuse_on) / p_diffuse;
+        if (take_diffuse && pd > 0.f) bounced = bounced + s.albedo * indirect<F>(c, pos, sn, &seed, diffuse_on) / p_diffuse;
         (void)rnd(&seed);  // ggx_sample_vndf's two numbers (shade.wgsl:317)
         (void)rnd(&seed);
         cc = cc + bounced;
@@ -659,10 +827,11 @@ __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = _
 
 // Samples [frame0, frame0 + spp) of pixels [first, first + count), summed
 // into px in sample order (util.wgsl:17-34, 101-145: sums restart at frame 0).
+template <bool F>
 __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t first,
                                                         uint32_t count, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
                                                         unsigned long long* __restrict__ rays) {
-    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
     const uint32_t k = blockIdx.x * TRACE_THREADS + threadIdx.x;
     const PhrCamera& cam = *camp;
     Ctx c;
@@ -688,7 +857,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
         float gone = keep ? ldf(px, 33 * n + i) : 0.f;
         uint32_t outline = 0;
         for (uint32_t s = 0; s < spp; ++s) {
-            Sample sm = shade(c, i, frame0 + s);
+            Sample sm = shade<F>(c, i, frame0 + s);
             ga = ga + sm.albedo;
             gn = gn + sm.normal;
             gd = gd + sm.depth;
@@ -729,6 +898,23 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
 }
 
 
+// Each mesh tree's root box (object space), for the instances' world boxes.
+__global__ __launch_bounds__(256) void k_roots(const float4* __restrict__ nodes, const uint32_t* __restrict__ bases, uint32_t n,
+                                               float4* __restrict__ out) {
+    uint32_t m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= n) return;
+    uint32_t b = bases[m];
+    out[2 * m] = b == PHR_NONE ? make_float4(0.f, 0.f, 0.f, 0.f) : nodes[2 * b];
+    out[2 * m + 1] = b == PHR_NONE ? make_float4(0.f, 0.f, 0.f, 0.f) : nodes[2 * b + 1];
+}
+
+// Top-tree slot -> instance id (Forest::retlas, src/trace/forest.rs:124-126).
+__global__ __launch_bounds__(256) void k_inst_order(const uint32_t* __restrict__ placed, const uint32_t* __restrict__ tlas_order,
+                                                    uint32_t p, uint32_t* __restrict__ out) {
+    uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < p) out[k] = placed[tlas_order[k]];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -736,6 +922,10 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
 // ---------------------------------------------------------------------------
 struct phr_bvh;
 int phr_bvh_internal_build(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out);
+int phr_bvh_internal_build_into(phr_gpu* g, phr_array* prims, uint32_t leaf, const uint32_t* seg_lo, uint32_t segs,
+                                phr_array* nodes, const uint32_t* node_base, uint32_t leaf_flag, phr_bvh** out);
+int phr_bvh_internal_rebuild_async(phr_bvh* b);
+int phr_bvh_internal_refit_async(phr_bvh* b);
 phr_array* phr_bvh_internal_nodes(phr_bvh* b);
 phr_array* phr_bvh_internal_order(phr_bvh* b);
 extern "C" void phr_bvh_free(phr_bvh* b);
@@ -766,6 +956,26 @@ struct phr_tracer {
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
     double last_gpu_s = 0.0, last_submit_s = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // A forest (src/trace/forest.rs): the top tree over the instances' world
+    // boxes at node 0, each mesh's tree behind it; what a new pose keeps
+    // (src/trace/pose.rs:28-39) on the host.
+    bool forest = false;
+    uint32_t nmeshes = 0, ninsts = 0, tlas_slots = 0;
+    phr_array* nodes = nullptr;       // the whole node array
+    phr_array* wboxes = nullptr;      // the placed instances' world boxes
+    phr_array* insts = nullptr;       // instance words, top-tree slot order, draw order
+    phr_array* placed_dev = nullptr;  // placed instance ids (a top-tree box's owner)
+    phr_bvh* tlas = nullptr;
+    std::vector<float> pos;            // 8 floats a vertex, object space
+    std::vector<uint32_t> faces_in;    // 4 words a face, the scene's order
+    std::vector<uint32_t> faces_slot;  // 4 words a face, leaf order
+    std::vector<float> mats_h;
+    std::vector<phr_mesh> meshes;
+    std::vector<phr_instance> inst_in;
+    std::vector<uint32_t> glow_meshes;  // meshes holding an emissive face
+    uint32_t* bases_dev = nullptr;      // each mesh tree's root node, or NONE
+    std::vector<uint32_t> placed, node_base;
+    std::vector<PhrAabb> roots;  // each mesh tree's root box, object space
 };
 
 namespace {
@@ -795,8 +1005,12 @@ struct WordsHash {  // FNV-1a over the words
 
 void tracer_release(phr_tracer* t) {
     if (!t) return;
-    for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->lights, t->px}) phr_array_release(a);
     if (t->bvh) phr_bvh_free(t->bvh);
+    if (t->tlas) phr_bvh_free(t->tlas);
+    for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->lights, t->px, t->nodes, t->wboxes, t->insts,
+                         t->placed_dev})
+        phr_array_release(a);
+    if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->rays) (void)hipFree(t->rays);
     if (t->cam_dev) (void)hipFree(t->cam_dev);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
@@ -850,13 +1064,254 @@ int build_lights(phr_tracer* t, const std::vector<uint32_t>& in_faces, const std
 
 SceneDev scene_dev(phr_tracer* t) {
     SceneDev s;
-    s.nodes = (const float4*)phr_bvh_internal_nodes(t->bvh)->ptr;
+    s.nodes = (const float4*)(t->forest ? t->nodes : phr_bvh_internal_nodes(t->bvh))->ptr;
+    s.insts = t->forest ? (const uint32_t*)t->insts->ptr : nullptr;
+    s.ninsts = t->ninsts;
     s.tris = (const float4*)t->tris->ptr;
     s.faces = (const uint4*)t->faces->ptr;
     s.verts = (const float4*)t->verts->ptr;
     s.mats = (const float*)t->mats->ptr;
     s.lights = (const uint32_t*)t->lights->ptr;
     return s;
+}
+
+void launch_raycast(phr_tracer* t, const PhrRay* rays, uint32_t nrays, int any_hit, PhrHit* hits) {
+    SceneDev sc = scene_dev(t);
+    dim3 grid((nrays + TRACE_THREADS - 1) / TRACE_THREADS), block(TRACE_THREADS);
+    if (t->forest)
+        hipLaunchKernelGGL(k_raycast<true>, grid, block, 0, t->gpu->stream, sc, rays, nrays, any_hit ? 1u : 0u, hits, t->rays + 4);
+    else
+        hipLaunchKernelGGL(k_raycast<false>, grid, block, 0, t->gpu->stream, sc, rays, nrays, any_hit ? 1u : 0u, hits, t->rays + 4);
+}
+
+// ---- instances (src/trace/instance.rs:60-160) and meshes (mesh.rs:43-54) ----
+// a(r, c) = m[c][r], the rows of the upper 3 x 3.
+inline float wa(const float w[16], int r, int c) { return w[4 * c + r]; }
+
+float inst_det(const float w[16]) {  // instance.rs:96-100
+    return wa(w, 0, 0) * (wa(w, 1, 1) * wa(w, 2, 2) - wa(w, 1, 2) * wa(w, 2, 1)) -
+           wa(w, 0, 1) * (wa(w, 1, 0) * wa(w, 2, 2) - wa(w, 1, 2) * wa(w, 2, 0)) +
+           wa(w, 0, 2) * (wa(w, 1, 0) * wa(w, 2, 1) - wa(w, 1, 1) * wa(w, 2, 0));
+}
+
+void inst_to_world(const float w[16], const float p[3], float out[3]) {  // instance.rs:68-71
+    for (int r = 0; r < 3; ++r) out[r] = ((w[r] * p[0] + w[4 + r] * p[1]) + w[8 + r] * p[2]) + w[12 + r];
+}
+
+void inst_inverse(const float w[16], float out[12]) {  // instance.rs:75-93; singular -> zero
+    const float det = inst_det(w);
+    if (std::fabs(det) < 1e-20f) {
+        for (int k = 0; k < 12; ++k) out[k] = 0.f;
+        return;
+    }
+    const float d = 1.0f / det;
+    const float inv[3][3] = {
+        {(wa(w, 1, 1) * wa(w, 2, 2) - wa(w, 1, 2) * wa(w, 2, 1)) * d, (wa(w, 0, 2) * wa(w, 2, 1) - wa(w, 0, 1) * wa(w, 2, 2)) * d,
+         (wa(w, 0, 1) * wa(w, 1, 2) - wa(w, 0, 2) * wa(w, 1, 1)) * d},
+        {(wa(w, 1, 2) * wa(w, 2, 0) - wa(w, 1, 0) * wa(w, 2, 2)) * d, (wa(w, 0, 0) * wa(w, 2, 2) - wa(w, 0, 2) * wa(w, 2, 0)) * d,
+         (wa(w, 0, 2) * wa(w, 1, 0) - wa(w, 0, 0) * wa(w, 1, 2)) * d},
+        {(wa(w, 1, 0) * wa(w, 2, 1) - wa(w, 1, 1) * wa(w, 2, 0)) * d, (wa(w, 0, 1) * wa(w, 2, 0) - wa(w, 0, 0) * wa(w, 2, 1)) * d,
+         (wa(w, 0, 0) * wa(w, 1, 1) - wa(w, 0, 1) * wa(w, 1, 0)) * d},
+    };
+    const float tx = w[12], ty = w[13], tz = w[14];
+    for (int r = 0; r < 3; ++r) {
+        out[4 * r] = inv[r][0];
+        out[4 * r + 1] = inv[r][1];
+        out[4 * r + 2] = inv[r][2];
+        out[4 * r + 3] = -((inv[r][0] * tx + inv[r][1] * ty) + inv[r][2] * tz);
+    }
+}
+
+bool inst_identity(const float w[16]) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r)
+            if (!(w[4 * c + r] == (r == c ? 1.0f : 0.0f))) return false;
+    return true;
+}
+
+// The box's eight corners placed (instance.rs:116-124).
+PhrAabb inst_world_box(const float w[16], const PhrAabb& b) {
+    PhrAabb out;
+    for (int j = 0; j < 3; ++j) { out.lo[j] = 3.40282347e38f; out.hi[j] = -3.40282347e38f; }
+    for (int k = 0; k < 8; ++k) {
+        const float c[3] = {(k & 1) ? b.hi[0] : b.lo[0], (k & 2) ? b.hi[1] : b.lo[1], (k & 4) ? b.hi[2] : b.lo[2]};
+        float q[3];
+        inst_to_world(w, c, q);
+        for (int j = 0; j < 3; ++j) {
+            out.lo[j] = fmin_(out.lo[j], q[j]);
+            out.hi[j] = fmax_(out.hi[j], q[j]);
+        }
+    }
+    out._a = 0.f;
+    out._b = 0.f;
+    return out;
+}
+
+// Instance::words (instance.rs:126-160).
+void inst_words(const phr_instance& in, uint32_t root, const phr_mesh& mesh, uint32_t* out) {
+    for (uint32_t k = 0; k < INSTANCE_LEN; ++k) out[k] = 0u;
+    const float* m = in.world;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) out[r * 4 + c] = fbits(m[4 * c + r]);
+    float inv[12];
+    inst_inverse(m, inv);
+    for (int k = 0; k < 12; ++k) out[12 + k] = fbits(inv[k]);
+    out[24] = root;
+    out[25] = mesh.face_lo;
+    out[26] = mesh.face_hi;
+    out[27] = (inst_identity(m) ? 1u : 0u) | ((in.casts_shadow ? 0u : 1u) << 1) | ((inst_det(m) < 0.0f ? 1u : 0u) << 2);
+    for (int k = 0; k < 3; ++k) out[28 + k] = fbits(in.color[k]);
+    out[31] = in.hash;
+    for (int k = 0; k < 3; ++k) {  // Mesh::texspace (mesh.rs:43-54)
+        float half = 0.5f * (mesh.hi[k] - mesh.lo[k]);
+        const float centre = 0.5f * (mesh.hi[k] + mesh.lo[k]);
+        half = half == 0.0f ? 1.0f : std::max(half, 0.00001f);
+        out[32 + k] = fbits(centre - half);
+        out[35 + k] = fbits(2.0f * half);
+    }
+    out[38] = in.mesh;
+}
+
+// lamp.rs:51-97 over the instances: each emissive face under each instance of
+// its mesh, by placed area times luminance, sorted by face then instance.
+int forest_lights(phr_tracer* t) {
+    std::vector<std::vector<uint32_t>> by_mesh(t->nmeshes);
+    for (uint32_t i = 0; i < t->ninsts; ++i)
+        if (t->inst_in[i].mesh < t->nmeshes) by_mesh[t->inst_in[i].mesh].push_back(i);
+    std::vector<uint32_t> ids, owners;
+    std::vector<float> pw;
+    for (uint32_t m = 0; m < t->nmeshes; ++m) {
+        for (uint32_t f = t->meshes[m].face_lo; f < t->meshes[m].face_hi; ++f) {
+            const uint32_t* fc = &t->faces_slot[4 * (size_t)f];
+            const float* e = &t->mats_h[MAT_WORDS * (size_t)fc[3] + 8];
+            const float glow = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+            if (glow <= 0.f) continue;
+            for (uint32_t i : by_mesh[m]) {
+                float a[3], b[3], c[3];
+                inst_to_world(t->inst_in[i].world, &t->pos[8 * (size_t)fc[0]], a);
+                inst_to_world(t->inst_in[i].world, &t->pos[8 * (size_t)fc[1]], b);
+                inst_to_world(t->inst_in[i].world, &t->pos[8 * (size_t)fc[2]], c);
+                float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+                float sq = -0.0f;
+                for (int k = 0; k < 3; ++k) sq = sq + cr[k] * cr[k];
+                const float p = (0.5f * std::sqrt(sq)) * glow;
+                if (p > 0.f) {
+                    ids.push_back(f);
+                    owners.push_back(i);
+                    pw.push_back(p);
+                }
+            }
+        }
+    }
+    float total = -0.0f;
+    for (float p : pw) total = total + p;
+    const uint32_t cnt = (uint32_t)ids.size();
+    std::vector<uint32_t> words(1 + 3 * (size_t)cnt);
+    words[0] = cnt;
+    float run = 0.0f;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        words[1 + k] = ids[k];
+        words[1 + cnt + k] = owners[k];
+        run = run + pw[k];
+        words[1 + 2 * cnt + k] = fbits(run / total);
+    }
+    t->glow = cnt;
+    if (!t->lights || t->lights->cap < words.size()) {
+        phr_array_release(t->lights);
+        t->lights = nullptr;
+        int rc = phr_array_alloc(t->gpu, 4, std::max<size_t>(words.size(), 4), &t->lights);
+        if (rc != PHR_OK) return rc;
+    }
+    t->lights->len = words.size();
+    PHR_HIP(hipMemcpyAsync(t->lights->ptr, words.data(), 4 * words.size(), hipMemcpyHostToDevice, t->gpu->stream));
+    PHR_HIP(hipStreamSynchronize(t->gpu->stream));
+    return PHR_OK;
+}
+
+// Each mesh tree's root box, read back once per new set of trees or bounds.
+int forest_roots(phr_tracer* t) {
+    float4* out = nullptr;
+    PHR_HIP(hipMalloc(&out, 32ull * std::max<uint32_t>(t->nmeshes, 1)));
+    hipLaunchKernelGGL(k_roots, dim3((t->nmeshes + 255) / 256), dim3(256), 0, t->gpu->stream, (const float4*)t->nodes->ptr,
+                       t->bases_dev, t->nmeshes, out);
+    t->roots.assign(t->nmeshes, PhrAabb{});
+    hipError_t e = hipMemcpyAsync(t->roots.data(), out, 32ull * t->nmeshes, hipMemcpyDeviceToHost, t->gpu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->gpu->stream);
+    (void)hipFree(out);
+    if (e != hipSuccess) { phr_set_error("forest roots", e); return PHR_ERR_HIP; }
+    return PHR_OK;
+}
+
+// Forest::retlas + inst_words (src/trace/forest.rs:111-127, 168-181): the
+// placed instances' world boxes on the host, the top tree over them on the
+// device (into node slots [0, 2p - 1), leaves flagged INST), the instance
+// table, and the extent (the top tree's root box's diagonal, forest.rs:189-199).
+int forest_top(phr_tracer* t, bool fresh) {
+    const uint32_t p = (uint32_t)t->placed.size();
+    std::vector<PhrAabb> wb(p);
+    float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f}, hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (uint32_t j = 0; j < p; ++j) {
+        const phr_instance& in = t->inst_in[t->placed[j]];
+        wb[j] = inst_world_box(in.world, t->roots[in.mesh]);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin_(lo[k], wb[j].lo[k]);
+            hi[k] = fmax_(hi[k], wb[j].hi[k]);
+        }
+    }
+    float sq = -0.0f;
+    for (int k = 0; k < 3; ++k) sq = sq + (hi[k] - lo[k]) * (hi[k] - lo[k]);
+    t->extent = std::max(std::sqrt(sq), 1e-3f);
+    hipStream_t s = t->gpu->stream;
+    PHR_HIP(hipMemcpyAsync(t->wboxes->ptr, wb.data(), 32ull * p, hipMemcpyHostToDevice, s));
+    int rc;
+    if (fresh) {
+        const uint32_t seg[2] = {0, p}, base[1] = {0};
+        if ((rc = phr_bvh_internal_build_into(t->gpu, t->wboxes, 1, seg, 1, t->nodes, base, PHR_LEAF | PHR_INST, &t->tlas)) != PHR_OK)
+            return rc;
+    } else if ((rc = phr_bvh_internal_rebuild_async(t->tlas)) != PHR_OK) {
+        return rc;
+    }
+    const uint32_t n = t->ninsts;
+    std::vector<uint32_t> words((size_t)(INSTANCE_LEN + 2) * n, 0u);
+    phr_mesh unit{0, 0, 0, 0, {0.f, 0.f, 0.f}, {1.f, 1.f, 1.f}};
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t m = t->inst_in[i].mesh;
+        const bool ok = m < t->nmeshes;
+        const uint32_t root = ok && t->meshes[m].face_hi > t->meshes[m].face_lo ? t->node_base[m] : PHR_NONE;
+        inst_words(t->inst_in[i], root, ok ? t->meshes[m] : unit, &words[INSTANCE_LEN * (size_t)i]);
+    }
+    uint32_t* draw = &words[(INSTANCE_LEN + 1) * (size_t)n];  // Forest::regroup (forest.rs:93-109)
+    uint32_t at = 0;
+    std::vector<std::vector<uint32_t>> by_mesh(t->nmeshes);
+    for (uint32_t i = 0; i < n; ++i)
+        if (t->inst_in[i].mesh < t->nmeshes) by_mesh[t->inst_in[i].mesh].push_back(i);
+    for (uint32_t m = 0; m < t->nmeshes; ++m)
+        for (uint32_t i : by_mesh[m]) draw[at++] = i;
+    PHR_HIP(hipMemcpyAsync(t->insts->ptr, words.data(), 4 * words.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_inst_order, dim3((p + 255) / 256), dim3(256), 0, s, (const uint32_t*)t->placed_dev->ptr,
+                       (const uint32_t*)phr_bvh_internal_order(t->tlas)->ptr, p, (uint32_t*)t->insts->ptr + INSTANCE_LEN * (size_t)n);
+    PHR_HIP(hipGetLastError());
+    PHR_HIP(hipStreamSynchronize(s));
+    return PHR_OK;
+}
+
+// The faces into their trees' order: slot-ordered faces and the triangle
+// cache on the device, and the slot order on the host.
+int forest_gather(phr_tracer* t) {
+    hipStream_t s = t->gpu->stream;
+    const uint32_t n = t->n;
+    phr_array* order = phr_bvh_internal_order(t->bvh);
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr, (const float4*)t->verts->ptr,
+                       (const uint32_t*)order->ptr, n, (uint4*)t->faces->ptr, (float4*)t->tris->ptr);
+    t->order.resize(n);
+    PHR_HIP(hipMemcpyAsync(t->order.data(), order->ptr, 4ull * n, hipMemcpyDeviceToHost, s));
+    PHR_HIP(hipStreamSynchronize(s));
+    t->faces_slot.resize(4 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k)
+        std::memcpy(&t->faces_slot[4 * (size_t)k], &t->faces_in[4 * (size_t)t->order[k]], 16);
+    return PHR_OK;
 }
 
 }  // namespace
@@ -1005,9 +1460,7 @@ int phr_tracer_raycast(phr_tracer* t, const void* rays, uint32_t nrays, int any_
     hipError_t e = hipMalloc(&dh, 20ull * nrays);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, 32ull * nrays, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        SceneDev sc = scene_dev(t);
-        hipLaunchKernelGGL(k_raycast, dim3((nrays + TRACE_THREADS - 1) / TRACE_THREADS), dim3(TRACE_THREADS), 0, s, sc.nodes, sc.tris, sc.faces, sc.mats,
-                           (const PhrRay*)dr, nrays, any_hit ? 1u : 0u, (PhrHit*)dh, t->rays + 4);
+        launch_raycast(t, (const PhrRay*)dr, nrays, any_hit, (PhrHit*)dh);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, 20ull * nrays, hipMemcpyDeviceToHost, s);
@@ -1024,10 +1477,7 @@ int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array
               "phr_tracer_raycast_on: rays must be 32-byte records and hits 20-byte records, as many");
     uint32_t nrays = (uint32_t)rays->len;
     if (nrays == 0) return PHR_OK;
-    SceneDev sc = scene_dev(t);
-    hipLaunchKernelGGL(k_raycast, dim3((nrays + TRACE_THREADS - 1) / TRACE_THREADS), dim3(TRACE_THREADS), 0, t->gpu->stream,
-                       sc.nodes, sc.tris, sc.faces, sc.mats, (const PhrRay*)rays->ptr, nrays, any_hit ? 1u : 0u, (PhrHit*)hits->ptr,
-                       t->rays + 4);
+    launch_raycast(t, (const PhrRay*)rays->ptr, nrays, any_hit, (PhrHit*)hits->ptr);
     PHR_HIP(hipGetLastError());
     return PHR_OK;
 }
@@ -1074,8 +1524,12 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     auto t0 = std::chrono::steady_clock::now();
     PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->last, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
     PHR_HIP(hipEventRecord(t->ev0, s));
-    hipLaunchKernelGGL(k_path, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
-                       (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
+    if (t->forest)
+        hipLaunchKernelGGL(k_path<true>, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
+                           (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
+    else
+        hipLaunchKernelGGL(k_path<false>, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
+                           (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(t->ev1, s));
     t->last_submit_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1156,5 +1610,217 @@ int phr_tracer_radiance(phr_tracer* t, float* out) {
     }
     return PHR_OK;
 }
+
+// The guide planes over the per-pixel count (src/trace/read.rs:253-307).
+static int guide_plane(phr_tracer* t, uint64_t at, int comps, float* out, const char* what) {
+    PHR_CHECK(t && t->px && out, PHR_ERR_ARG, what);
+    uint64_t n = (uint64_t)t->w * t->h;
+    std::vector<uint32_t> sums(comps * n), cnt(n);
+    int rc = phr_array_read_range(t->px, at * n, comps * n, sums.data());
+    if (rc == PHR_OK) rc = phr_array_read_range(t->px, 31 * n, n, cnt.data());
+    if (rc != PHR_OK) return rc;
+    for (uint64_t p = 0; p < n; ++p) {
+        float k = (float)std::max<uint32_t>(cnt[p], 1u);
+        for (int c = 0; c < comps; ++c) {
+            float v;
+            std::memcpy(&v, &sums[comps * p + c], 4);
+            out[comps * p + c] = v / k;
+        }
+    }
+    return PHR_OK;
+}
+int phr_tracer_albedo(phr_tracer* t, float* out) { return guide_plane(t, 7, 3, out, "phr_tracer_albedo: nothing traced yet"); }
+int phr_tracer_normal(phr_tracer* t, float* out) { return guide_plane(t, 10, 3, out, "phr_tracer_normal: nothing traced yet"); }
+int phr_tracer_depth_plane(phr_tracer* t, float* out) { return guide_plane(t, 13, 1, out, "phr_tracer_depth_plane: nothing traced yet"); }
+
+// Tracer::from_scene (src/trace/build.rs:15-25) with the forest of
+// src/trace/forest.rs:44-66: the faces' boxes in their meshes' spaces, one
+// batched LBVH build for every mesh tree (written behind the top tree's
+// slots), the faces into leaf order, each tree's root box back on the host,
+// then the top tree over the placed instances' world boxes, the instance
+// table and the light table.
+int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, phr_tracer** out) {
+    PHR_CHECK(g && d && out, PHR_ERR_ARG, "phr_tracer_from_scene: null argument");
+    PHR_CHECK(d->nfaces > 0 && d->nmeshes > 0 && d->ninsts > 0 && d->verts && d->faces && d->mats && d->meshes && d->insts,
+              PHR_ERR_ARG, "phr_tracer_from_scene: empty scene");
+    PHR_CHECK(d->nmeshes <= 65536, PHR_ERR_SIZE, "phr_tracer_from_scene: at most 65,536 meshes");
+    const uint32_t* fcs = (const uint32_t*)d->faces;
+    const float* mts = (const float*)d->mats;
+    for (uint32_t f = 0; f < d->nfaces; ++f) {
+        const uint32_t* fc = fcs + 4 * (size_t)f;
+        PHR_CHECK(fc[0] < d->nverts && fc[1] < d->nverts && fc[2] < d->nverts && fc[3] < d->nmats, PHR_ERR_ARG,
+                  "phr_tracer_from_scene: a face names a vertex or material that is not there");
+    }
+    for (uint32_t m = 0; m < d->nmats; ++m)
+        PHR_CHECK(mts[MAT_WORDS * (size_t)m + 6] == 0.f, PHR_ERR_UNSUPPORTED, "phr_tracer_from_scene: transmission (glass) is not supported");
+    for (uint32_t m = 0; m < d->nmeshes; ++m) {
+        const phr_mesh& me = d->meshes[m];
+        const uint32_t want = m == 0 ? 0u : d->meshes[m - 1].face_hi;
+        PHR_CHECK(me.face_lo == want && me.face_lo <= me.face_hi && me.face_hi <= d->nfaces, PHR_ERR_ARG,
+                  "phr_tracer_from_scene: the meshes must tile the faces in order");
+    }
+    PHR_CHECK(d->meshes[d->nmeshes - 1].face_hi == d->nfaces, PHR_ERR_ARG, "phr_tracer_from_scene: the meshes must cover every face");
+    bool any_placed = false;
+    for (uint32_t i = 0; i < d->ninsts && !any_placed; ++i) {
+        const uint32_t m = d->insts[i].mesh;
+        any_placed = m < d->nmeshes && d->meshes[m].face_hi > d->meshes[m].face_lo;
+    }
+    PHR_CHECK(any_placed, PHR_ERR_ARG, "phr_tracer_from_scene: no instance places a mesh with faces");
+    phr_tracer* t = new phr_tracer();
+    t->gpu = g;
+    t->forest = true;
+    t->n = d->nfaces;
+    t->nverts = d->nverts;
+    t->nmats = d->nmats;
+    t->nmeshes = d->nmeshes;
+    t->ninsts = d->ninsts;
+    t->leaf = leaf;
+    t->tlas_slots = std::max<uint32_t>(2 * d->ninsts - 1, 1);
+    auto fail = [&](int rc) { tracer_release(t); return rc; };
+    t->pos.assign((const float*)d->verts, (const float*)d->verts + 8 * (size_t)d->nverts);
+    t->faces_in.assign(fcs, fcs + 4 * (size_t)d->nfaces);
+    t->mats_h.assign(mts, mts + MAT_WORDS * (size_t)d->nmats);
+    t->meshes.assign(d->meshes, d->meshes + d->nmeshes);
+    t->inst_in.assign(d->insts, d->insts + d->ninsts);
+    for (uint32_t i = 0; i < d->ninsts; ++i) {
+        const uint32_t m = d->insts[i].mesh;
+        if (m < d->nmeshes && d->meshes[m].face_hi > d->meshes[m].face_lo) t->placed.push_back(i);
+    }
+    std::vector<uint32_t> seg_lo(d->nmeshes + 1), bases(d->nmeshes, PHR_NONE);
+    t->node_base.assign(d->nmeshes, 0u);
+    uint64_t nodes = t->tlas_slots;
+    for (uint32_t m = 0; m < d->nmeshes; ++m) {
+        const uint32_t cnt = d->meshes[m].face_hi - d->meshes[m].face_lo;
+        seg_lo[m] = d->meshes[m].face_lo;
+        t->node_base[m] = (uint32_t)nodes;
+        if (cnt) {
+            bases[m] = (uint32_t)nodes;
+            nodes += 2ull * cnt - 1;
+        }
+        bool glows = false;
+        for (uint32_t f = d->meshes[m].face_lo; f < d->meshes[m].face_hi && !glows; ++f) {
+            const float* e = mts + MAT_WORDS * (size_t)fcs[4 * (size_t)f + 3] + 8;
+            glows = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2] > 0.f;
+        }
+        if (glows) t->glow_meshes.push_back(m);
+    }
+    seg_lo[d->nmeshes] = d->nfaces;
+    PHR_CHECK(nodes < (1ull << 31), PHR_ERR_SIZE, "phr_tracer_from_scene: too many nodes");
+    const uint32_t n = t->n, p = (uint32_t)t->placed.size();
+    hipStream_t s = g->stream;
+    int rc;
+    if ((rc = phr_array_alloc(g, 16, n, &t->in_faces)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, t->nverts, &t->verts)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4 * MAT_WORDS, t->nmats, &t->mats)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, n, &t->prims)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 16, n, &t->faces)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 48, n, &t->tris)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, nodes, &t->nodes)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, p, &t->wboxes)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4, (size_t)(INSTANCE_LEN + 2) * t->ninsts, &t->insts)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4, p, &t->placed_dev)) != PHR_OK) return fail(rc);
+    if (hipMalloc(&t->rays, 48) != hipSuccess || hipMemsetAsync(t->rays, 0, 48, s) != hipSuccess ||
+        hipMalloc(&t->cam_dev, sizeof(PhrCamera)) != hipSuccess || hipMalloc(&t->bases_dev, 4ull * t->nmeshes) != hipSuccess) {
+        phr_set_error_msg("phr_tracer_from_scene: allocation failed");
+        return fail(PHR_ERR_HIP);
+    }
+    if (hipMemcpyAsync(t->in_faces->ptr, fcs, 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(t->verts->ptr, d->verts, 32ull * t->nverts, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(t->mats->ptr, mts, 64ull * t->nmats, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(t->bases_dev, bases.data(), 4ull * t->nmeshes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(t->placed_dev->ptr, t->placed.data(), 4ull * p, hipMemcpyHostToDevice, s) != hipSuccess) {
+        phr_set_error_msg("phr_tracer_from_scene: upload failed");
+        return fail(PHR_ERR_HIP);
+    }
+    hipLaunchKernelGGL(k_tri_boxes, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
+                       (const float4*)t->verts->ptr, n, (float4*)t->prims->ptr);
+    if ((rc = phr_bvh_internal_build_into(g, t->prims, leaf, seg_lo.data(), t->nmeshes, t->nodes, t->node_base.data(), PHR_LEAF,
+                                          &t->bvh)) != PHR_OK)
+        return fail(rc);
+    if ((rc = forest_gather(t)) != PHR_OK || (rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, true)) != PHR_OK ||
+        (rc = forest_lights(t)) != PHR_OK)
+        return fail(rc);
+    if (hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess) {
+        phr_set_error_msg("phr_tracer_from_scene: event creation failed");
+        return fail(PHR_ERR_HIP);
+    }
+    *out = t;
+    return PHR_OK;
+}
+
+int phr_tracer_reinstance(phr_tracer* t, const float* worlds, uint32_t n, int* ok) {
+    PHR_CHECK(t && ok && (worlds || n == 0), PHR_ERR_ARG, "phr_tracer_reinstance: null argument");
+    *ok = 0;
+    if (!t->forest || n != t->ninsts) return PHR_OK;
+    for (uint32_t i = 0; i < n; ++i) std::memcpy(t->inst_in[i].world, worlds + 16 * (size_t)i, 64);
+    int rc;
+    if ((rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK) return rc;
+    t->have_last = false;  // the image starts again (pose.rs:117-119)
+    t->frame = 0;
+    *ok = 1;
+    return PHR_OK;
+}
+
+// The moved vertices are in t->verts: boxes, refit, triangle cache, roots,
+// top tree, lights (the emissive meshes' vertices come back to the host).
+static int repose_device(phr_tracer* t, bool host_pos_current) {
+    hipStream_t s = t->gpu->stream;
+    const uint32_t n = t->n;
+    hipLaunchKernelGGL(k_tri_boxes, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
+                       (const float4*)t->verts->ptr, n, (float4*)t->prims->ptr);
+    int rc = phr_bvh_internal_refit_async(t->bvh);
+    if (rc != PHR_OK) return rc;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr, (const float4*)t->verts->ptr,
+                       (const uint32_t*)phr_bvh_internal_order(t->bvh)->ptr, n, (uint4*)t->faces->ptr, (float4*)t->tris->ptr);
+    PHR_HIP(hipGetLastError());
+    if (!host_pos_current)
+        for (uint32_t m : t->glow_meshes) {
+            const phr_mesh& me = t->meshes[m];
+            if (me.vert_hi > me.vert_lo && me.vert_hi <= t->nverts)
+                PHR_HIP(hipMemcpyAsync(&t->pos[8 * (size_t)me.vert_lo], (const float*)t->verts->ptr + 8 * (size_t)me.vert_lo,
+                                       32ull * (me.vert_hi - me.vert_lo), hipMemcpyDeviceToHost, s));
+        }
+    if ((rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK) return rc;
+    t->have_last = false;  // pose.rs:92-94
+    t->frame = 0;
+    return PHR_OK;
+}
+
+int phr_tracer_repose(phr_tracer* t, const void* verts, uint32_t nverts, int* ok) {
+    PHR_CHECK(t && ok && (verts || nverts == 0), PHR_ERR_ARG, "phr_tracer_repose: null argument");
+    *ok = 0;
+    if (!t->forest || nverts != t->nverts) return PHR_OK;
+    std::memcpy(t->pos.data(), verts, 32ull * nverts);
+    PHR_HIP(hipMemcpyAsync(t->verts->ptr, verts, 32ull * nverts, hipMemcpyHostToDevice, t->gpu->stream));
+    int rc = repose_device(t, true);
+    if (rc == PHR_OK) *ok = 1;
+    return rc;
+}
+
+int phr_tracer_repose_on(phr_tracer* t, phr_array* verts, int* ok) {
+    PHR_CHECK(t && ok && verts && verts->elem == 32, PHR_ERR_ARG, "phr_tracer_repose_on: vertices are 32-byte records");
+    *ok = 0;
+    if (!t->forest || verts->len != t->nverts) return PHR_OK;
+    PHR_HIP(hipMemcpyAsync(t->verts->ptr, verts->ptr, 32ull * t->nverts, hipMemcpyDeviceToDevice, t->gpu->stream));
+    int rc = repose_device(t, false);
+    if (rc == PHR_OK) *ok = 1;
+    return rc;
+}
+
+int phr_tracer_retree(phr_tracer* t, int* ok) {
+    PHR_CHECK(t && ok, PHR_ERR_ARG, "phr_tracer_retree: null argument");
+    *ok = 0;
+    if (!t->forest) return PHR_OK;
+    int rc;
+    if ((rc = phr_bvh_internal_rebuild_async(t->bvh)) != PHR_OK || (rc = forest_gather(t)) != PHR_OK ||
+        (rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK)
+        return rc;
+    *ok = 1;  // the image keeps its samples (pose.rs:14-16)
+    return PHR_OK;
+}
+
+phr_array* phr_tracer_nodes(phr_tracer* t) { return t ? (t->forest ? t->nodes : phr_bvh_internal_nodes(t->bvh)) : nullptr; }
+phr_array* phr_tracer_instances(phr_tracer* t) { return t && t->forest ? t->insts : nullptr; }
+uint32_t phr_tracer_instance_count(const phr_tracer* t) { return t ? (t->forest ? t->ninsts : 1u) : 0u; }
 
 }  // extern "C"
