@@ -126,6 +126,51 @@ def batch_rates(gpu: P.Gpu, reps: int) -> dict:
             "max_depth": b.depth}
 
 
+def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
+    """The rigid-body workload: 1,024 bodies, each its own mesh under one
+    instance (box or icosphere s0..s5), a ground plane and a lamp. A substep
+    refits every mesh tree over moved vertices (two resident poses in turn)
+    and rebuilds the top tree over new matrices; then one frame is traced."""
+    b, worlds = S.bodies_scene(1024)
+    v, f, m, me, ins = b.arrays()
+    t0 = time.perf_counter()
+    tr = P.Tracer.from_scene(gpu, v, f, m, me, ins, 4)
+    build_s = time.perf_counter() - t0
+    v2 = v.copy()
+    v2["p"][:, :3] *= np.float32(1.02)  # every body swells by 2% in its own space
+    poses = [P.GpuArray.from_(gpu, v2), P.GpuArray.from_(gpu, v)]
+    base = ins["world"].copy()
+    turn = [S.world_of(S.affine(S.rotation((0.3, 0.2, 1.0), 0.01 * (k + 1)))) for k in range(substeps)]
+    mats = []
+    for k in range(substeps):
+        wk = base.copy()
+        r = turn[k].reshape(4, 4).T  # back to a math matrix
+        for i in range(len(worlds)):
+            wk[i] = S.world_of(r @ worlds[i])
+        mats.append(wk)
+    tr.repose_on(poses[0])
+    tr.reinstance(mats[0])
+    gpu.wait()
+    clock = P.Clock(gpu)
+    clock.start()
+    for k in range(substeps):
+        assert tr.repose_on(poses[k % 2]) and tr.reinstance(mats[k])
+    g_s, wall_s = clock.stop()
+    cam = S.path_camera(w, h, eye=(0.0, -16.0, 9.0), target=(0.0, 0.0, 1.5), rr_from=3)
+    tr.set_samples(1 << 20)
+    tr.sample(cam, w, h, spp)
+    c0 = tr.counters()
+    clock.start()
+    tr.sample(cam, w, h, spp)
+    t_frame, _ = clock.stop()
+    c1 = tr.counters()
+    rays = int(c1[0] - c0[0] + c1[1] - c0[1])
+    return {"bodies": len(worlds), "meshes": len(me), "tris": len(f), "build_s": build_s,
+            "substep_ms": wall_s / substeps * 1e3, "substep_gpu_span_ms": g_s / substeps * 1e3,
+            "frame": f"{w}x{h}, {spp} spp", "frame_ms": t_frame * 1e3, "mrays_s": rays / t_frame / 1e6,
+            "rays_per_frame": rays}
+
+
 def raycast_rate(gpu: P.Gpu, reps: int) -> dict:
     corners, _ = S.lbvh_scene(65536, 0)
     tr = P.Tracer.new(gpu, S.tris_from_corners(corners), 4)
@@ -187,7 +232,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "path", "lbvh"], default="all")
+    ap.add_argument("--only", choices=["all", "path", "lbvh", "forest"], default="all")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -226,6 +271,9 @@ def main() -> int:
         out["raycast_65536"] = raycast_rate(gpu, 50)
         log("batch of 256 scenes")
         out["lbvh_batch_256"] = batch_rates(gpu, 10)
+    if args.only in ("all", "forest"):
+        log("forest of 1,024 bodies")
+        out["forest_1024"] = forest_rates(gpu)
         log(f"lbvh {out['lbvh_scene']['build_mtri_s']:.1f} Mtri/s (scene), {out['lbvh_65536']['build_mtri_s']:.1f} Mtri/s (65536)")
 
     line = {"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

@@ -23,9 +23,12 @@
 // The stages and why each looks the way it does:
 //
 //  1. bounds   each wave takes a contiguous run of boxes and keeps a running
-//              box while the segment stays the same; at a segment change it
-//              folds the box into that segment's six words with atomic
-//              min/max on an order-preserving integer image of the floats.
+//              box while the segment stays the same; a segment that ends in
+//              the run is folded into its six words with atomic min/max on an
+//              order-preserving integer image of the floats, and the box of
+//              the last segment a block reaches goes to that block's own slot,
+//              folded by a one-block launch (so one tree's boxes meet no
+//              contended atomic).
 //              No block waits on another and no cache-wide fence is needed
 //              (min and max are exact, so the fold order does not matter).
 //              The Morton launch reads the words back and forms each scene
@@ -113,6 +116,7 @@ constexpr uint32_t RF_THREADS = 256;
 constexpr uint32_t RF_SLOTS = 2048;
 constexpr uint32_t RF_GROUP = 64;
 constexpr uint32_t RF_LDS_SLOTS = RF_SLOTS * RF_GROUP;
+constexpr uint32_t UP_LEFT = 0x80000000u;  // in an up record's sibling word: the node is the left child
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = __lane_id();
@@ -193,57 +197,123 @@ __device__ __forceinline__ void grow(Box& b, float4 lo, float4 hi) {
     b.hi0 = fmaxf(b.hi0, hi.x); b.hi1 = fmaxf(b.hi1, hi.y); b.hi2 = fmaxf(b.hi2, hi.z);
 }
 
-// Fold the wave's running box into segment s's words (all lanes call it).
-__device__ __forceinline__ void fold_box(Box b, uint32_t* __restrict__ acc, uint32_t s) {
+__device__ __forceinline__ void fold_keys(uint32_t* __restrict__ acc, uint32_t s, const Box& b) {
+    uint32_t* a = acc + 6 * s;
+    atomicMin(&a[0], fkey(b.lo0)); atomicMin(&a[1], fkey(b.lo1)); atomicMin(&a[2], fkey(b.lo2));
+    atomicMax(&a[3], fkey(b.hi0)); atomicMax(&a[4], fkey(b.hi1)); atomicMax(&a[5], fkey(b.hi2));
+}
+__device__ __forceinline__ Box wave_box(Box b) {
     b.lo0 = wave_min(b.lo0); b.lo1 = wave_min(b.lo1); b.lo2 = wave_min(b.lo2);
     b.hi0 = wave_max(b.hi0); b.hi1 = wave_max(b.hi1); b.hi2 = wave_max(b.hi2);
-    if (__lane_id() == 0) {
-        uint32_t* a = acc + 6 * s;
-        atomicMin(&a[0], fkey(b.lo0)); atomicMin(&a[1], fkey(b.lo1)); atomicMin(&a[2], fkey(b.lo2));
-        atomicMax(&a[3], fkey(b.hi0)); atomicMax(&a[4], fkey(b.hi1)); atomicMax(&a[5], fkey(b.hi2));
-    }
+    return b;
 }
+__device__ __forceinline__ Box join(const Box& a, const Box& b) {
+    return Box{fminf(a.lo0, b.lo0), fminf(a.lo1, b.lo1), fminf(a.lo2, b.lo2),
+               fmaxf(a.hi0, b.hi0), fmaxf(a.hi1, b.hi1), fmaxf(a.hi2, b.hi2)};
+}
+
+// A block's box of the last segment its boxes reach, for the fold launch.
+struct Part {
+    uint32_t seg;
+    float lo0, lo1, lo2, hi0, hi1, hi2, pad;
+};
 
 // acc[6s..6s+2] = fkey of segment s's low corner, acc[6s+3..6s+5] of its high
 // corner. They hold fkey(f32::MAX) and fkey(f32::MIN) between builds; the
 // packing launch restores them after the Morton launch has read them.
+//
+// Each block takes a contiguous run of boxes, a quarter to each wave. A
+// segment that ends inside a wave's run is folded into its words at once
+// (few blocks touch it); the box of the last segment the block reaches goes
+// to the block's own Part slot, so the long runs of one segment - one tree's
+// boxes - meet no contended atomic here. k_bounds_fold folds the Parts.
 __global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ prims, uint32_t n, const uint32_t* __restrict__ seg_lo,
-                                                uint32_t segs, uint32_t* __restrict__ acc, uint32_t* __restrict__ zero,
-                                                uint32_t zero_words) {
-    const uint32_t tid = threadIdx.x, lane = tid & 63;
+                                                uint32_t segs, uint32_t* __restrict__ acc, Part* __restrict__ parts,
+                                                uint32_t* __restrict__ zero, uint32_t zero_words) {
+    __shared__ uint32_t s_seg[4];
+    __shared__ Box s_box[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t gtid = blockIdx.x * blockDim.x + tid, gsize = gridDim.x * blockDim.x;
     for (uint32_t i = gtid; i < zero_words; i += gsize) zero[i] = 0u;
-    const uint32_t waves = gsize / 64, gw = gtid / 64;
-    const uint32_t chunk = ((n + waves - 1) / waves + 63) & ~63u;
-    const uint32_t c0 = gw * chunk, c1 = min(c0 + chunk, n);
-    if (c0 >= n) return;  // whole waves only
-    uint32_t cur = seg_of_slot(seg_lo, segs, c0), cur_end = seg_lo[cur + 1];
+    const uint32_t bchunk = ((n + gridDim.x - 1) / gridDim.x + 255) & ~255u, wchunk = bchunk / 4;
+    const uint32_t b0 = min(blockIdx.x * bchunk, n), b1 = min(b0 + bchunk, n);
+    const uint32_t c0 = min(b0 + wave * wchunk, b1), c1 = min(c0 + wchunk, b1);
+    uint32_t cur = PHR_NONE;
     Box b = empty_box();
-    for (uint32_t base = c0; base < c1; base += 64) {
-        const uint32_t p = base + lane;
-        const bool active = p < c1;
-        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
-        if (active) { lo = prims[2 * p]; hi = prims[2 * p + 1]; }
-        if (min(base + 64, c1) <= cur_end) {
-            if (active) grow(b, lo, hi);
-            continue;
+    if (c0 < c1) {
+        cur = seg_of_slot(seg_lo, segs, c0);
+        uint32_t cur_end = seg_lo[cur + 1];
+        for (uint32_t base = c0; base < c1; base += 64) {
+            const uint32_t p = base + lane;
+            const bool active = p < c1;
+            float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+            if (active) { lo = prims[2 * p]; hi = prims[2 * p + 1]; }
+            if (min(base + 64, c1) <= cur_end) {
+                if (active) grow(b, lo, hi);
+                continue;
+            }
+            // A segment ends inside this run of 64: fold each as it closes.
+            bool done = !active;
+            for (;;) {
+                const bool mine = !done && p < cur_end;
+                if (mine) grow(b, lo, hi);
+                done = done || mine;
+                const uint64_t left = __ballot(!done);
+                if (left == 0) break;
+                const Box w = wave_box(b);
+                if (lane == 0) fold_keys(acc, cur, w);
+                b = empty_box();
+                const uint32_t first = base + (uint32_t)__ffsll((unsigned long long)left) - 1;
+                cur = seg_of_slot(seg_lo, segs, first);
+                cur_end = seg_lo[cur + 1];
+            }
         }
-        // A segment ends inside this run of 64: fold as each one closes.
-        bool done = !active;
-        for (;;) {
-            const bool mine = !done && p < cur_end;
-            if (mine) grow(b, lo, hi);
-            done = done || mine;
-            const uint64_t left = __ballot(!done);
-            if (left == 0) break;
-            fold_box(b, acc, cur);
-            b = empty_box();
-            const uint32_t first = base + (uint32_t)__ffsll((unsigned long long)left) - 1;
-            cur = seg_of_slot(seg_lo, segs, first);
-            cur_end = seg_lo[cur + 1];
-        }
+        b = wave_box(b);
     }
-    fold_box(b, acc, cur);
+    if (lane == 0) {
+        s_seg[wave] = cur;
+        s_box[wave] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t last = PHR_NONE;
+        for (int w = 3; w >= 0 && last == PHR_NONE; --w) last = s_seg[w];
+        Box part = empty_box();
+        for (int w = 0; w < 4; ++w) {
+            if (s_seg[w] == PHR_NONE) continue;
+            if (s_seg[w] == last) part = join(part, s_box[w]);
+            else fold_keys(acc, s_seg[w], s_box[w]);  // a segment that ended inside this block
+        }
+        parts[blockIdx.x] = Part{last, part.lo0, part.lo1, part.lo2, part.hi0, part.hi1, part.hi2, 0.f};
+    }
+}
+
+// One block folds the blocks' Parts into the segments' words. Parts are in
+// slot order, so a thread's strided Parts have non-decreasing segments: it
+// keeps a running box and folds when the segment changes; a wave whose
+// threads all end on one segment folds once.
+__global__ __launch_bounds__(256) void k_bounds_fold(const Part* __restrict__ parts, uint32_t nparts, uint32_t* __restrict__ acc) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    uint32_t cur = PHR_NONE;
+    Box b = empty_box();
+    for (uint32_t k = tid; k < nparts; k += blockDim.x) {
+        const Part pt = parts[k];
+        if (pt.seg == PHR_NONE) continue;
+        if (pt.seg != cur) {
+            if (cur != PHR_NONE) fold_keys(acc, cur, b);
+            cur = pt.seg;
+            b = empty_box();
+        }
+        b = join(b, Box{pt.lo0, pt.lo1, pt.lo2, pt.hi0, pt.hi1, pt.hi2});
+    }
+    const uint32_t first = (uint32_t)__shfl((int)cur, 0);
+    const bool same = __ballot(cur != first && cur != PHR_NONE) == 0;
+    if (same) {
+        const Box w = wave_box(b);
+        if (lane == 0 && first != PHR_NONE) fold_keys(acc, first, w);
+    } else if (cur != PHR_NONE) {
+        fold_keys(acc, cur, b);
+    }
 }
 
 // ---- 2. Morton codes + the digit histograms --------------------------------
@@ -430,7 +500,7 @@ __device__ __forceinline__ int32_t kdelta(const uint32_t* __restrict__ codes, in
 }
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t n, const uint32_t* __restrict__ seg_lo,
-                                                uint32_t segs, PhrLink* __restrict__ links, uint32_t* __restrict__ parent,
+                                                uint32_t segs, PhrLink* __restrict__ links, uint4* __restrict__ up,
                                                 uint32_t* __restrict__ leaf_at, uint32_t* __restrict__ own_end,
                                                 uint32_t* __restrict__ far_ends) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -438,7 +508,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     leaf_at[p] = PHR_NONE;
     const uint32_t sg = seg_of_slot(seg_lo, segs, p);
     const uint32_t lo = seg_lo[sg], ns = seg_lo[sg + 1] - lo, k = p - lo;
-    if (k == 0) parent[ns == 1 ? n + p : p] = PHR_NONE;  // the root
+    if (k == 0) up[ns == 1 ? n + p : p] = make_uint4(PHR_NONE, 0u, 0u, 0u);  // the root
     if (k + 1 >= ns) {
         own_end[p] = 0u;  // the last slot of a segment starts no inner node
         return;
@@ -468,8 +538,10 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     uint32_t left = (uint32_t)((first == g) ? n + lo + g : lo + g);
     uint32_t right = (uint32_t)((last == g + 1) ? n + lo + g + 1 : lo + g + 1);
     links[p] = PhrLink{left, right, lo + (uint32_t)first, lo + (uint32_t)last};
-    parent[left] = p;
-    parent[right] = p;
+    // Each child's way up: its parent, the parent's range, its sibling (bit 31
+    // set when the child is the left one).
+    up[left] = make_uint4(p, lo + (uint32_t)first, lo + (uint32_t)last, right | UP_LEFT);
+    up[right] = make_uint4(p, lo + (uint32_t)first, lo + (uint32_t)last, left);
     // +1 where the range starts, -1 where it ends (two's complement words).
     if (first == i) {
         own_end[p] = 1u;
@@ -558,7 +630,7 @@ __global__ __launch_bounds__(256) void k_stab(const uint32_t* __restrict__ own_e
 // larger of the coverings of edges k - 1 and k (an edge across a segment's end
 // is covered by nothing). Deeper than 31 means an ancestor at depth 31 was
 // packed (d + 1 >= STACK), so the node is unreachable.
-__global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+__global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links, const uint4* __restrict__ up,
                                               const uint32_t* __restrict__ stab, uint32_t n, const uint32_t* __restrict__ seg_lo,
                                               uint32_t segs, uint32_t leaf, uint32_t* __restrict__ leaf_at,
                                               uint32_t* __restrict__ depth_out, uint32_t* __restrict__ acc) {
@@ -588,10 +660,9 @@ __global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links,
             // so only the parent's own size test matters; its depth (d - 1 <=
             // 30) is below the cap.
             bool muted = false;
-            const uint32_t up = parent[t];
-            if (up != PHR_NONE && d < PHR_STACK) {
-                PhrLink pl = links[up];
-                muted = pl.last - pl.first < leaf;
+            if (d < PHR_STACK) {
+                const uint4 u = up[t];
+                muted = u.x != PHR_NONE && u.z - u.y < leaf;
             }
             if (d < PHR_STACK && !muted) {
                 listed = inner ? ((l.last - l.first < leaf) || (d + 1 >= PHR_STACK)) : true;
@@ -666,32 +737,34 @@ struct Arrivals {
 
 // Carry a finished node x of segment g up while its parent's range lies in
 // [s0, s1); a node whose parent's range leaves it is handed to the next level.
+// One record a step: the parent, its range and the sibling come together.
 template <uint32_t CW>
 __device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t n, uint64_t s0, uint64_t s1,
-                                      const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent, float4* nodes,
-                                      Arrivals<CW>& arr, uint32_t* s_fcount, uint32_t* front_out) {
+                                      const uint4* __restrict__ up, float4* nodes, Arrivals<CW>& arr, uint32_t* s_fcount,
+                                      uint32_t* front_out) {
     for (;;) {
-        const uint32_t p = parent[x];
+        const uint4 u = up[x];
+        const uint32_t p = u.x;
         if (p == PHR_NONE) return;  // the root is done
-        const PhrLink l = links[p];
-        if (l.first < s0 || l.last >= s1) {
+        if (u.y < s0 || u.z >= s1) {
             uint32_t at = atomicAdd(s_fcount, 1u);
             front_out[at] = x;
             return;
         }
         if (!arr.second(p)) return;  // the sibling's climb finishes p
-        const uint32_t sib = (l.left == x) ? l.right : l.left;
-        const uint32_t so = out_of(sib, n, g);
+        const uint32_t sib = u.w & ~UP_LEFT;
+        const uint32_t so = out_of(sib, n, g), xo = out_of(x, n, g);
         const float4 sa = nodes[2 * so], sb = nodes[2 * so + 1];
         grow(b, sa, sb);
-        store_node(nodes, out_of(p, n, g), b, out_of(l.left, n, g), out_of(l.right, n, g));
+        const bool x_left = (u.w & UP_LEFT) != 0u;
+        store_node(nodes, out_of(p, n, g), b, x_left ? xo : so, x_left ? so : xo);
         x = p;
     }
 }
 
 // Level 0: one block per RF_SLOTS slots; a thread per leaf starting there.
 __global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __restrict__ prims, const uint32_t* __restrict__ order,
-                                                             const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+                                                             const PhrLink* __restrict__ links, const uint4* __restrict__ up,
                                                              const uint32_t* __restrict__ leaf_at, uint32_t n,
                                                              const uint32_t* __restrict__ seg_lo, const uint32_t* __restrict__ node_base,
                                                              uint32_t segs, uint32_t leaf_flag, float4* nodes,
@@ -727,7 +800,7 @@ __global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __res
             grow(b, prims[2 * pi], prims[2 * pi + 1]);
         }
         store_node(nodes, out_of(k, n, g), b, first | leaf_flag, end);
-        climb<CW>(k, b, g, n, s0, s1, links, parent, nodes, arr, &s_fcount, out);
+        climb<CW>(k, b, g, n, s0, s1, up, nodes, arr, &s_fcount, out);
     }
     __syncthreads();
     if (tid == 0) fcount_out[blockIdx.x] = s_fcount;
@@ -736,7 +809,7 @@ __global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __res
 // Level L >= 1: block b owns the slots of blocks [RF_GROUP b, RF_GROUP b + RF_GROUP)
 // of level L - 1 (span slots each) and carries on the climbs they handed over.
 template <uint32_t CW>
-__global__ __launch_bounds__(RF_THREADS) void k_refit_up(const PhrLink* __restrict__ links, const uint32_t* __restrict__ parent,
+__global__ __launch_bounds__(RF_THREADS) void k_refit_up(const uint4* __restrict__ up,
                                                          uint32_t n, const uint32_t* __restrict__ seg_lo,
                                                          const uint32_t* __restrict__ node_base, uint32_t segs, float4* nodes,
                                                          uint32_t* __restrict__ gcnt, uint32_t below_blocks, uint64_t span,
@@ -777,7 +850,7 @@ __global__ __launch_bounds__(RF_THREADS) void k_refit_up(const PhrLink* __restri
         const uint32_t xo = out_of(x, n, g);
         const float4 a = nodes[2 * xo], c = nodes[2 * xo + 1];
         Box b{a.x, a.y, a.z, c.x, c.y, c.z};
-        climb<CW>(x, b, g, n, s0, s1, links, parent, nodes, arr, &s_fcount, out);
+        climb<CW>(x, b, g, n, s0, s1, up, nodes, arr, &s_fcount, out);
     }
     __syncthreads();
     if (tid == 0) fcount_out[blockIdx.x] = s_fcount;
@@ -806,8 +879,9 @@ struct phr_bvh {
     uint32_t* zero = nullptr;  // histograms, tickets, depths, look-back status
     uint32_t zero_words = 0, status_at = 0, scan_status = 0;
     uint32_t* acc = nullptr;  // each segment's scene box as six order-preserving keys
+    void* parts = nullptr;    // one Part per bounds block
     PhrLink* links = nullptr;     // n, by inner id
-    uint32_t* parent = nullptr;   // 2n, by node id
+    uint4* up = nullptr;          // 2n, by node id: parent, parent's range, sibling
     uint32_t* cnt = nullptr;      // n: arrival counters of the levels above LDS size
     uint32_t* leaf_at = nullptr;  // n: the reachable leaf whose range starts at a slot, or NONE
     uint32_t* own_end = nullptr;   // n: +1/-1 for the inner range of which a slot is one end
@@ -825,7 +899,7 @@ void bvh_release(phr_bvh* b) {
     phr_array_release(b->order);
     if (b->own_nodes) phr_array_release(b->nodes);
     for (void* p : {(void*)b->seg_lo, (void*)b->node_base, (void*)b->seg_of, (void*)b->codes, (void*)b->keys_b, (void*)b->vals_b,
-                    (void*)b->zero, (void*)b->acc, (void*)b->links, (void*)b->parent, (void*)b->cnt, (void*)b->leaf_at,
+                    (void*)b->zero, (void*)b->acc, b->parts, (void*)b->links, (void*)b->up, (void*)b->cnt, (void*)b->leaf_at,
                     (void*)b->own_end, (void*)b->far_ends, (void*)b->front[0], (void*)b->front[1], (void*)b->fcount[0],
                     (void*)b->fcount[1]})
         if (p) (void)hipFree(p);
@@ -900,7 +974,8 @@ int bvh_alloc(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, const uint3
     bool ok = dev((void**)&b->seg_lo, 4ull * (segs + 1)) && dev((void**)&b->node_base, 4ull * segs) &&
               (b->seg_passes == 0 || dev((void**)&b->seg_of, 4 * nn)) && dev((void**)&b->codes, 4 * nn) &&
               dev((void**)&b->keys_b, 4 * nn) && dev((void**)&b->vals_b, 4 * nn) && dev((void**)&b->zero, 4ull * b->zero_words) &&
-              dev((void**)&b->acc, 24ull * segs) && dev((void**)&b->links, 16 * nn) && dev((void**)&b->parent, 8 * nn) &&
+              dev((void**)&b->acc, 24ull * segs) && dev(&b->parts, 32ull * b->bounds_blocks) &&
+              dev((void**)&b->links, 16 * nn) && dev((void**)&b->up, 32 * nn) &&
               dev((void**)&b->cnt, 4 * nn) && dev((void**)&b->leaf_at, 4 * nn) && dev((void**)&b->own_end, 4 * nn) &&
               dev((void**)&b->far_ends, 4 * nn) && dev((void**)&b->front[0], 4 * nn) && dev((void**)&b->front[1], 4 * nn) &&
               dev((void**)&b->fcount[0], 4ull * b->level0_blocks) && dev((void**)&b->fcount[1], 4ull * b->level0_blocks);
@@ -933,8 +1008,9 @@ int bvh_enqueue_build(phr_bvh* b) {
     uint32_t* ticket = b->zero + Z_TICKET;
     uint32_t* status = b->zero + b->status_at;
     const uint32_t passes = CODE_PASSES + b->seg_passes;
-    hipLaunchKernelGGL(k_bounds, dim3(b->bounds_blocks), dim3(256), 0, s, prims, n, b->seg_lo, b->segs, b->acc, b->zero,
-                       b->zero_words);
+    hipLaunchKernelGGL(k_bounds, dim3(b->bounds_blocks), dim3(256), 0, s, prims, n, b->seg_lo, b->segs, b->acc, (Part*)b->parts,
+                       b->zero, b->zero_words);
+    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, s, (const Part*)b->parts, b->bounds_blocks, b->acc);
     // The passes alternate between (codes, order) and (keys_b, vals_b) and
     // must end in (codes, order): with an odd count the codes start in keys_b.
     uint32_t* unsorted = (passes % 2) ? b->keys_b : b->codes;
@@ -961,13 +1037,13 @@ int bvh_enqueue_build(phr_bvh* b) {
         kin = kout;
         vin = vout;
     }
-    hipLaunchKernelGGL(k_karras, dim3((n + 255) / 256), dim3(256), 0, s, b->codes, n, b->seg_lo, b->segs, b->links, b->parent,
+    hipLaunchKernelGGL(k_karras, dim3((n + 255) / 256), dim3(256), 0, s, b->codes, n, b->seg_lo, b->segs, b->links, b->up,
                        b->leaf_at, b->own_end, b->far_ends);
     // The slot scan writes its coverings over the sort's spare key buffer.
     uint32_t* stab = b->keys_b;
     hipLaunchKernelGGL(k_stab, dim3((n + SC_TILE - 1) / SC_TILE), dim3(256), 0, s, b->own_end, b->far_ends, n, stab,
                        (uint64_t*)(b->zero + b->scan_status), b->zero + Z_SCAN_TICKET);
-    hipLaunchKernelGGL(k_pack, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->parent, stab, n, b->seg_lo, b->segs,
+    hipLaunchKernelGGL(k_pack, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->up, stab, n, b->seg_lo, b->segs,
                        b->leaf, b->leaf_at, b->zero + Z_DEPTH, b->acc);
     PHR_HIP(hipGetLastError());
     return bvh_enqueue_refit(b);
@@ -982,18 +1058,18 @@ int bvh_enqueue_refit(phr_bvh* b) {
     float4* nodes = (float4*)b->nodes->ptr;
     uint32_t blocks = b->level0_blocks;
     hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(RF_THREADS), 0, s, (const float4*)b->prims->ptr,
-                       (const uint32_t*)b->order->ptr, b->links, b->parent, b->leaf_at, n, b->seg_lo, b->node_base, b->segs,
+                       (const uint32_t*)b->order->ptr, b->links, b->up, b->leaf_at, n, b->seg_lo, b->node_base, b->segs,
                        b->leaf_flag, nodes, b->front[0], b->fcount[0]);
     uint64_t span = RF_SLOTS;
     int cur = 0;
     while (blocks > 1) {
         const uint32_t up = (blocks + RF_GROUP - 1) / RF_GROUP;
         if (span * RF_GROUP <= RF_LDS_SLOTS)
-            hipLaunchKernelGGL(k_refit_up<RF_LDS_SLOTS / 16>, dim3(up), dim3(RF_THREADS), 0, s, b->links, b->parent, n, b->seg_lo,
+            hipLaunchKernelGGL(k_refit_up<RF_LDS_SLOTS / 16>, dim3(up), dim3(RF_THREADS), 0, s, b->up, n, b->seg_lo,
                                b->node_base, b->segs, nodes, b->cnt, blocks, span, b->front[cur], b->fcount[cur], b->front[1 - cur],
                                b->fcount[1 - cur]);
         else
-            hipLaunchKernelGGL(k_refit_up<0>, dim3(up), dim3(RF_THREADS), 0, s, b->links, b->parent, n, b->seg_lo, b->node_base,
+            hipLaunchKernelGGL(k_refit_up<0>, dim3(up), dim3(RF_THREADS), 0, s, b->up, n, b->seg_lo, b->node_base,
                                b->segs, nodes, b->cnt, blocks, span, b->front[cur], b->fcount[cur], b->front[1 - cur],
                                b->fcount[1 - cur]);
         blocks = up;
