@@ -171,6 +171,46 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
             "rays_per_frame": rays}
 
 
+def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
+    """mean((x - y)^2 / (y^2 + 0.01)) over pixels and channels."""
+    x, y = x.astype(np.float64), y.astype(np.float64)
+    return float(np.mean((x - y) ** 2 / (y * y + 0.01)))
+
+
+def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, ref_spp: int = 2048) -> dict:
+    """Seconds a frame needs to reach relMSE <= 0.01 against a ref_spp render
+    (its own sample streams): raw, and after the a-trous denoiser (plain)."""
+    tr = P.Tracer.new(gpu, tris, 4)
+    tr.set_samples(1 << 30)
+    ref_cam = cam.copy()
+    ref_cam["stream"] = 1 << 24  # samples the trial renders never draw
+    for _ in range(ref_spp // 64):
+        tr.sample(ref_cam, w, h, 64)
+    ref = tr.radiance()
+    clock = P.Clock(gpu)
+    trials, best = [], {"raw": None, "plain": None}
+    for spp in (1, 2, 4, 8, 16, 32, 64, 128, 256):
+        fresh = cam.copy()
+        fresh["stream"] = spp  # a new camera record: the image starts again
+        tr.sample(fresh, w, h, spp)
+        render_s = tr.time()[1]
+        raw = rel_mse(tr.radiance(), ref)
+        clock.start()
+        tr.denoise(keep=False)
+        den_s, _ = clock.stop()
+        plain = rel_mse(tr.radiance(), ref)
+        trials.append({"spp": spp, "render_s": render_s, "denoise_s": den_s, "relmse_raw": raw, "relmse_plain": plain})
+        if best["raw"] is None and raw <= 0.01:
+            best["raw"] = {"spp": spp, "s_per_frame": render_s}
+        if best["plain"] is None and plain <= 0.01:
+            best["plain"] = {"spp": spp, "s_per_frame": render_s + den_s}
+        log(f"convergence spp {spp}: relMSE raw {raw:.4f}, denoised {plain:.4f}")
+        if best["raw"] is not None and best["plain"] is not None:
+            break
+    return {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
+            "plain": best["plain"], "learned": None, "trials": trials}
+
+
 def raycast_rate(gpu: P.Gpu, reps: int) -> dict:
     corners, _ = S.lbvh_scene(65536, 0)
     tr = P.Tracer.new(gpu, S.tris_from_corners(corners), 4)
@@ -232,7 +272,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "path", "lbvh", "forest"], default="all")
+    ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge"], default="all")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -271,10 +311,15 @@ def main() -> int:
         out["raycast_65536"] = raycast_rate(gpu, 50)
         log("batch of 256 scenes")
         out["lbvh_batch_256"] = batch_rates(gpu, 10)
+        log(f"lbvh {out['lbvh_scene']['build_mtri_s']:.1f} Mtri/s (scene), {out['lbvh_65536']['build_mtri_s']:.1f} Mtri/s (65536)")
+    if args.only in ("all", "converge"):
+        log("spp to relMSE <= 0.01")
+        out["s_per_frame_to_relmse_0.01"] = convergence(gpu, tris, S.path_camera(args.width, args.height, rr_from=3),
+                                                         args.width, args.height)
     if args.only in ("all", "forest"):
         log("forest of 1,024 bodies")
         out["forest_1024"] = forest_rates(gpu)
-        log(f"lbvh {out['lbvh_scene']['build_mtri_s']:.1f} Mtri/s (scene), {out['lbvh_65536']['build_mtri_s']:.1f} Mtri/s (65536)")
+        log(f"forest: substep {out['forest_1024']['substep_ms']:.3f} ms, {out['forest_1024']['mrays_s']:.1f} Mrays/s")
 
     line = {"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",

@@ -35,6 +35,7 @@ typedef struct phr_bvh phr_bvh;
 typedef struct phr_tracer phr_tracer;
 typedef struct phr_clock phr_clock;
 typedef struct phr_pass phr_pass;
+typedef struct phr_sort phr_sort;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* phr_last_error(void);
@@ -152,6 +153,26 @@ phr_array* phr_bvh_nodes(phr_bvh* b);
 phr_array* phr_bvh_order(phr_bvh* b);
 phr_array* phr_bvh_prims(phr_bvh* b);
 void phr_bvh_free(phr_bvh* b);
+
+/* ---- sort: GpuSort (src/bvh/sort.rs:13-111) ---------------------------------- */
+/* GpuSort::record (src/bvh/sort.rs:30-51): a sort of n u32 codes held in a
+ * device array (borrowed; new codes may be written before each run). */
+int phr_sort_record(phr_gpu* g, phr_array* codes, uint32_t n, phr_sort** out);
+/* GpuSort::run (src/bvh/sort.rs:61-63): enqueued. Stable: the result is the
+ * sort by (code, slot) of the reference's 64-bit LSD passes. */
+int phr_sort_run(phr_sort* s);
+/* GpuSort::keys / code (src/bvh/sort.rs:66-73): (original slot, code) pairs
+ * and the sorted codes alone, on the device. */
+phr_array* phr_sort_keys(phr_sort* s);
+phr_array* phr_sort_code(phr_sort* s);
+/* GpuSort::len / steps / bytes (src/bvh/sort.rs:76-95): steps is the
+ * dispatch count (6 here: histograms, 4 passes, keys). */
+uint32_t phr_sort_len(const phr_sort* s);
+uint32_t phr_sort_steps(const phr_sort* s);
+uint64_t phr_sort_bytes(const phr_sort* s);
+/* GpuSort::order / read (src/bvh/sort.rs:99-111), blocking; either may be null. */
+int phr_sort_read(phr_sort* s, uint32_t* order, uint32_t* sorted);
+void phr_sort_free(phr_sort* s);
 
 /* ---- tracer: Tracer (src/trace/mod.rs:50-245) -------------------------------- */
 /* Tracer::new (src/trace/build.rs:11-13) over n triangle records (128 B each,

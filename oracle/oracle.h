@@ -103,7 +103,7 @@ typedef struct {
     uint32_t frame, count, glow, bounces;
     uint32_t depth[4];
     float integrator[4];
-    uint32_t w, h, rr_from, _pad;
+    uint32_t w, h, rr_from, stream;  /* added to the sample index in the seed */
 } orc_camera;
 
 /* Input triangle: src/trace/kernel.rs:11-40 (Tri) rows a..nc, color, emit —

@@ -482,7 +482,7 @@ void shade(Ctx& c, uint32_t i, uint32_t* px) {
     float fw = (float)p->w, fh = (float)p->h;
     float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
     float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
-    uint32_t seed = hash_u(i * 9781u + p->frame * 6271u + 1u);
+    uint32_t seed = hash_u(i * 9781u + (p->frame + p->stream) * 6271u + 1u);
     float ju = rnd(&seed);
     float jv = rnd(&seed);
     float jx = (ju - 0.5f) / fw;

@@ -35,7 +35,7 @@ CAMERA = np.dtype([
     ("back", "<f4", 3), ("units", "<f4"), ("pivot", "<f4", 3), ("aspect", "<f4"),
     ("sun", "<f4", 3), ("dist", "<f4"), ("pan", "<f4", 2), ("ao", "<f4"), ("eps", "<f4"),
     ("env", "<f4", 4), ("frame", "<u4"), ("count", "<u4"), ("glow", "<u4"), ("bounces", "<u4"),
-    ("depth", "<u4", 4), ("integrator", "<f4", 4), ("w", "<u4"), ("h", "<u4"), ("rr_from", "<u4"), ("_pad", "<u4"),
+    ("depth", "<u4", 4), ("integrator", "<f4", 4), ("w", "<u4"), ("h", "<u4"), ("rr_from", "<u4"), ("stream", "<u4"),
 ])
 assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
 assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 176
@@ -122,6 +122,15 @@ _SIGS = {
     "phr_bvh_order": (_P, [_P]),
     "phr_bvh_prims": (_P, [_P]),
     "phr_bvh_free": (None, [_P]),
+    "phr_sort_record": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "phr_sort_run": (_I, [_P]),
+    "phr_sort_keys": (_P, [_P]),
+    "phr_sort_code": (_P, [_P]),
+    "phr_sort_len": (_U32, [_P]),
+    "phr_sort_steps": (_U32, [_P]),
+    "phr_sort_bytes": (_U64, [_P]),
+    "phr_sort_read": (_I, [_P, _P, _P]),
+    "phr_sort_free": (None, [_P]),
     "phr_tracer_new": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
     "phr_tracer_free": (None, [_P]),
     "phr_tracer_len": (_U32, [_P]),
@@ -375,6 +384,44 @@ class Pass:
         try:
             if self._h:
                 lib().phr_pass_free(self._h)
+        except Exception:
+            pass
+
+
+class GpuSort:
+    """phiray's `GpuSort` (src/bvh/sort.rs:13-111): recorded over a device array of codes, replayed."""
+
+    def __init__(self, gpu: Gpu, codes: GpuArray, n: int | None = None):
+        self.gpu, self._codes = gpu, codes
+        out = C.c_void_p()
+        _check(lib().phr_sort_record(gpu._h, codes._h, len(codes) if n is None else n, C.byref(out)), "GpuSort::record")
+        self._h = out
+
+    def run(self) -> None:
+        _check(lib().phr_sort_run(self._h), "GpuSort::run")
+
+    def __len__(self) -> int:
+        return int(lib().phr_sort_len(self._h))
+
+    def steps(self) -> int:
+        return int(lib().phr_sort_steps(self._h))
+
+    def bytes(self) -> int:
+        return int(lib().phr_sort_bytes(self._h))
+
+    def keys(self) -> np.ndarray:
+        """(original slot, code) pairs, sorted."""
+        return GpuArray._borrow(self.gpu, lib().phr_sort_keys(self._h), np.dtype((np.uint32, 2))).read()
+
+    def read(self) -> tuple[np.ndarray, np.ndarray]:
+        order, srt = np.zeros(len(self), np.uint32), np.zeros(len(self), np.uint32)
+        _check(lib().phr_sort_read(self._h, _ptr(order), _ptr(srt)), "GpuSort::read")
+        return order, srt
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().phr_sort_free(self._h)
         except Exception:
             pass
 

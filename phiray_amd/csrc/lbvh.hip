@@ -856,6 +856,40 @@ __global__ __launch_bounds__(RF_THREADS) void k_refit_up(const uint4* __restrict
     if (tid == 0) fcount_out[blockIdx.x] = s_fcount;
 }
 
+// ---- GpuSort (src/bvh/sort.rs:13-111) ---------------------------------------
+// The four digit histograms of arbitrary 32-bit codes.
+__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t* __restrict__ codes, uint32_t n, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[4 * 256];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < 4 * 256; i += 256) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * OS_TILE;
+#pragma unroll
+    for (uint32_t r = 0; r < OS_ITEMS; ++r) {
+        const uint32_t i = base + r * 256 + tid;
+        if (i < n) {
+            const uint32_t c = codes[i];
+            atomicAdd(&s_h[c & 0xFFu], 1u);
+            atomicAdd(&s_h[256 + ((c >> 8) & 0xFFu)], 1u);
+            atomicAdd(&s_h[512 + ((c >> 16) & 0xFFu)], 1u);
+            atomicAdd(&s_h[768 + (c >> 24)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < 4 * 256; i += 256)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// The sorted keys as GpuSort::keys holds them, (original slot, code), and the
+// counters cleared for the next replay.
+__global__ __launch_bounds__(256) void k_sort_keys(const uint32_t* __restrict__ order, const uint32_t* __restrict__ sorted,
+                                                   uint32_t n, uint2* __restrict__ keys, uint32_t* __restrict__ zero,
+                                                   uint32_t zero_words) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    for (uint32_t i = gtid; i < n; i += gsize) keys[i] = make_uint2(order[i], sorted[i]);
+    for (uint32_t i = gtid; i < zero_words; i += gsize) zero[i] = 0u;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1267,5 +1301,112 @@ phr_array* phr_bvh_nodes(phr_bvh* b) { return b ? b->nodes : nullptr; }
 phr_array* phr_bvh_order(phr_bvh* b) { return b ? b->order : nullptr; }
 phr_array* phr_bvh_prims(phr_bvh* b) { return b ? b->prims : nullptr; }
 void phr_bvh_free(phr_bvh* b) { bvh_release(b); }
+
+// GpuSort (src/bvh/sort.rs:13-111): a stable sort of n 32-bit codes with
+// their slots, recorded once over a device array of codes and replayed. The
+// keys come out as (original slot, code) in (code, slot) order - the order of
+// the reference's 64-bit LSD sort - and the codes alone beside them.
+struct phr_sort {
+    phr_gpu* gpu = nullptr;
+    phr_array* codes = nullptr;  // borrowed
+    phr_array* keys = nullptr;   // n x (slot, code)
+    phr_array* code = nullptr;   // n sorted codes
+    uint32_t n = 0, tiles = 0, zero_words = 0;
+    uint32_t *order = nullptr, *kb = nullptr, *vb = nullptr, *zero = nullptr;
+};
+
+static void sort_release(phr_sort* s) {
+    if (!s) return;
+    phr_array_release(s->keys);
+    phr_array_release(s->code);
+    for (void* p : {(void*)s->order, (void*)s->kb, (void*)s->vb, (void*)s->zero})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+int phr_sort_record(phr_gpu* g, phr_array* codes, uint32_t n, phr_sort** out) {
+    PHR_CHECK(g && codes && out && codes->elem == 4 && codes->len >= n, PHR_ERR_ARG,
+              "phr_sort_record: codes must be a u32 array of at least n items");
+    phr_sort* s = new phr_sort();
+    s->gpu = g;
+    s->codes = codes;
+    s->n = n;
+    s->tiles = (n + OS_TILE - 1) / OS_TILE;
+    s->zero_words = Z_TICKET + 8 + CODE_PASSES * s->tiles * 256;
+    const uint64_t nn = std::max<uint32_t>(n, 1);
+    int rc;
+    if ((rc = phr_array_alloc(g, 8, n, &s->keys)) != PHR_OK || (rc = phr_array_alloc(g, 4, n, &s->code)) != PHR_OK) {
+        sort_release(s);
+        return rc;
+    }
+    bool ok = hipMalloc((void**)&s->order, 4 * nn) == hipSuccess && hipMalloc((void**)&s->kb, 4 * nn) == hipSuccess &&
+              hipMalloc((void**)&s->vb, 4 * nn) == hipSuccess && hipMalloc((void**)&s->zero, 4ull * s->zero_words) == hipSuccess &&
+              hipMemsetAsync(s->zero, 0, 4ull * s->zero_words, g->stream) == hipSuccess &&
+              hipStreamSynchronize(g->stream) == hipSuccess;
+    if (!ok) {
+        sort_release(s);
+        phr_set_error_msg("phr_sort_record: allocation failed");
+        return PHR_ERR_HIP;
+    }
+    *out = s;
+    return PHR_OK;
+}
+
+// GpuSort::run (src/bvh/sort.rs:61-63): histograms, four stable onesweep
+// passes (8 bits each), then the keys; enqueued.
+int phr_sort_run(phr_sort* s) {
+    PHR_CHECK(s, PHR_ERR_ARG, "phr_sort_run: null");
+    const uint32_t n = s->n;
+    if (n == 0) return PHR_OK;
+    hipStream_t st = s->gpu->stream;
+    uint32_t* hist = s->zero + Z_HIST;
+    uint32_t* ticket = s->zero + Z_TICKET;
+    uint32_t* status = s->zero + Z_TICKET + 8;
+    hipLaunchKernelGGL(k_sort_hist, dim3(s->tiles), dim3(256), 0, st, (const uint32_t*)s->codes->ptr, n, hist);
+    const uint32_t* kin = (const uint32_t*)s->codes->ptr;
+    const uint32_t* vin = nullptr;
+    uint32_t* sorted = (uint32_t*)s->code->ptr;
+    for (uint32_t p = 0; p < CODE_PASSES; ++p) {
+        const bool to_final = (p % 2) == 1;  // passes 1 and 3 land in (code, order)
+        uint32_t* kout = to_final ? sorted : s->kb;
+        uint32_t* vout = to_final ? s->order : s->vb;
+        uint32_t* stp = status + (size_t)p * s->tiles * 256;
+        if (p == 0)
+            hipLaunchKernelGGL((k_onesweep<true, false>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout, hist, stp,
+                               ticket, n, 0u, (const uint32_t*)nullptr);
+        else
+            hipLaunchKernelGGL((k_onesweep<false, false>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout,
+                               hist + 256 * p, stp, ticket + p, n, 8 * p, (const uint32_t*)nullptr);
+        kin = kout;
+        vin = vout;
+    }
+    const uint32_t blocks = std::min<uint32_t>(1024, std::max<uint32_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, (const uint32_t*)s->order, (const uint32_t*)sorted, n,
+                       (uint2*)s->keys->ptr, s->zero, s->zero_words);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
+
+phr_array* phr_sort_keys(phr_sort* s) { return s ? s->keys : nullptr; }
+phr_array* phr_sort_code(phr_sort* s) { return s ? s->code : nullptr; }
+uint32_t phr_sort_len(const phr_sort* s) { return s ? s->n : 0; }
+uint32_t phr_sort_steps(const phr_sort* s) { return s ? 2 + CODE_PASSES : 0; }
+uint64_t phr_sort_bytes(const phr_sort* s) {
+    return s ? 8ull * s->n + 4ull * s->n * 4 + 4ull * s->zero_words : 0;
+}
+
+// GpuSort::order / read (src/bvh/sort.rs:99-111), blocking.
+int phr_sort_read(phr_sort* s, uint32_t* order, uint32_t* sorted) {
+    PHR_CHECK(s, PHR_ERR_ARG, "phr_sort_read: null");
+    hipStream_t st = s->gpu->stream;
+    if (s->n) {
+        if (order) PHR_HIP(hipMemcpyAsync(order, s->order, 4ull * s->n, hipMemcpyDeviceToHost, st));
+        if (sorted) PHR_HIP(hipMemcpyAsync(sorted, s->code->ptr, 4ull * s->n, hipMemcpyDeviceToHost, st));
+    }
+    PHR_HIP(hipStreamSynchronize(st));
+    return PHR_OK;
+}
+
+void phr_sort_free(phr_sort* s) { sort_release(s); }
 
 }  // extern "C"

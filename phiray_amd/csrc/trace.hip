@@ -39,7 +39,7 @@ struct PhrCamera {
     uint32_t frame, count, glow, bounces;
     uint32_t depth[4];
     float integrator[4];
-    uint32_t w, h, rr_from, _pad;
+    uint32_t w, h, rr_from, stream;  // stream: added to the sample index in the seed (0: the reference's seeds)
 };
 static_assert(sizeof(PhrCamera) == 176, "camera layout");
 
@@ -729,7 +729,7 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
     float fw = (float)p->w, fh = (float)p->h;
     float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
     float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
-    uint32_t seed = pcg_hash(i * 9781u + frame * 6271u + 1u);
+    uint32_t seed = pcg_hash(i * 9781u + (frame + p->stream) * 6271u + 1u);
     float ju = rnd(&seed);
     float jv = rnd(&seed);
     float jx = (ju - 0.5f) / fw;
@@ -825,15 +825,19 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
 __device__ __forceinline__ float ldf(const uint32_t* px, size_t w) { return __uint_as_float(px[w]); }
 __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = __float_as_uint(v); }
 
-// Samples [frame0, frame0 + spp) of pixels [first, first + count), summed
-// into px in sample order (util.wgsl:17-34, 101-145: sums restart at frame 0).
+// Samples [frame0, frame0 + spp) of every pixel, summed into px in sample
+// order (util.wgsl:17-34, 101-145: sums restart at frame 0).
 template <bool F>
-__global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t first,
-                                                        uint32_t count, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+__global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x,
+                                                        uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
                                                         unsigned long long* __restrict__ rays) {
     __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
-    const uint32_t k = blockIdx.x * TRACE_THREADS + threadIdx.x;
     const PhrCamera& cam = *camp;
+    // A block traces a 16 x 16 tile, a wave 16 x 4 pixels of it: neighbouring
+    // rays walk neighbouring nodes.
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
+    const bool inside = pxl < cam.w && pyl < cam.h;
     Ctx c;
     c.s = sc;
     c.p = camp;
@@ -842,8 +846,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
     c.shadow = 0;
     c.steps = 0;
     c.tests = 0;
-    if (k < count) {
-        const uint32_t i = first + k;
+    if (inside) {
+        const uint32_t i = pyl * cam.w + pxl;
         const size_t n = (size_t)cam.w * cam.h;
         bool keep = frame0 > 0u;
         V3 l = keep ? v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2)) : v3(0, 0, 0);
@@ -1524,12 +1528,13 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     auto t0 = std::chrono::steady_clock::now();
     PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->last, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
     PHR_HIP(hipEventRecord(t->ev0, s));
+    const uint32_t tiles_x = (w + 15) / 16, tiles = tiles_x * ((h + 15) / 16);
     if (t->forest)
-        hipLaunchKernelGGL(k_path<true>, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
-                           (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
+        hipLaunchKernelGGL(k_path<true>, dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, tiles_x, frame0,
+                           spp, (uint32_t*)t->px->ptr, t->rays);
     else
-        hipLaunchKernelGGL(k_path<false>, dim3((uint32_t)((n + TRACE_THREADS - 1) / TRACE_THREADS)), dim3(TRACE_THREADS), 0, s, sc,
-                           (const PhrCamera*)t->cam_dev, 0u, (uint32_t)n, frame0, spp, (uint32_t*)t->px->ptr, t->rays);
+        hipLaunchKernelGGL(k_path<false>, dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, tiles_x, frame0,
+                           spp, (uint32_t*)t->px->ptr, t->rays);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(t->ev1, s));
     t->last_submit_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
