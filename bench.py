@@ -177,38 +177,73 @@ def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
     return float(np.mean((x - y) ** 2 / (y * y + 0.01)))
 
 
-def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, ref_spp: int = 2048) -> dict:
+def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, ref_spp: int = 4096,
+                net: "P.Kpcn | None" = None) -> dict:
     """Seconds a frame needs to reach relMSE <= 0.01 against a ref_spp render
-    (its own sample streams): raw, and after the a-trous denoiser (plain)."""
+    (its own sample streams): raw, after the a-trous denoiser (plain), and
+    after the learned denoiser when weights are given. Each trial renders
+    spp samples afresh; its time is the render's GPU time plus the
+    denoiser's."""
     tr = P.Tracer.new(gpu, tris, 4)
     tr.set_samples(1 << 30)
     ref_cam = cam.copy()
     ref_cam["stream"] = 1 << 24  # samples the trial renders never draw
+    t0 = time.perf_counter()
     for _ in range(ref_spp // 64):
         tr.sample(ref_cam, w, h, 64)
     ref = tr.radiance()
+    log(f"convergence: {ref_spp}-spp reference in {time.perf_counter() - t0:.1f} s")
     clock = P.Clock(gpu)
-    trials, best = [], {"raw": None, "plain": None}
+    out = P.GpuArray.zeroed(gpu, np.float32, w * h * 3) if net is not None else None
+    if net is not None:  # warm the network's workspace at this size
+        tr.sample(cam, w, h, 1)
+        tr.denoise_learned(net, out)
+    trials, best = [], {"raw": None, "plain": None, "learned": None}
     for spp in (1, 2, 4, 8, 16, 32, 64, 128, 256):
         fresh = cam.copy()
         fresh["stream"] = spp  # a new camera record: the image starts again
         tr.sample(fresh, w, h, spp)
         render_s = tr.time()[1]
         raw = rel_mse(tr.radiance(), ref)
+        trial = {"spp": spp, "render_s": render_s, "relmse_raw": raw}
+        if net is not None:
+            clock.start()
+            tr.denoise_learned(net, out)
+            net_s, _ = clock.stop()
+            learned = rel_mse(out.read().reshape(-1, 3), ref)
+            trial.update({"learned_s": net_s, "relmse_learned": learned})
+            if best["learned"] is None and learned <= 0.01:
+                best["learned"] = {"spp": spp, "s_per_frame": render_s + net_s, "denoise_s": net_s}
         clock.start()
         tr.denoise(keep=False)
         den_s, _ = clock.stop()
         plain = rel_mse(tr.radiance(), ref)
-        trials.append({"spp": spp, "render_s": render_s, "denoise_s": den_s, "relmse_raw": raw, "relmse_plain": plain})
+        trial.update({"denoise_s": den_s, "relmse_plain": plain})
+        trials.append(trial)
         if best["raw"] is None and raw <= 0.01:
             best["raw"] = {"spp": spp, "s_per_frame": render_s}
         if best["plain"] is None and plain <= 0.01:
-            best["plain"] = {"spp": spp, "s_per_frame": render_s + den_s}
-        log(f"convergence spp {spp}: relMSE raw {raw:.4f}, denoised {plain:.4f}")
-        if best["raw"] is not None and best["plain"] is not None:
+            best["plain"] = {"spp": spp, "s_per_frame": render_s + den_s, "denoise_s": den_s}
+        log(f"convergence spp {spp}: relMSE raw {raw:.4f}, a-trous {plain:.4f}"
+            + (f", learned {trial['relmse_learned']:.4f}" if net is not None else ""))
+        if all(v is not None for k, v in best.items() if net is not None or k != "learned"):
             break
-    return {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
-            "plain": best["plain"], "learned": None, "trials": trials}
+    res = {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
+           "plain": best["plain"], "learned": best["learned"], "trials": trials}
+    if best["learned"] and best["raw"]:
+        res["raw_over_learned"] = best["raw"]["s_per_frame"] / best["learned"]["s_per_frame"]
+    if best["learned"] and best["plain"]:
+        res["plain_over_learned"] = best["plain"]["s_per_frame"] / best["learned"]["s_per_frame"]
+    return res
+
+
+def load_net(gpu: P.Gpu) -> "P.Kpcn | None":
+    from phiray_amd import kpcn as K
+
+    if not K.WEIGHTS.exists():
+        log(f"no trained weights at {K.WEIGHTS}: the learned leg is skipped")
+        return None
+    return P.Kpcn(gpu, K.load(K.WEIGHTS))
 
 
 def raycast_rate(gpu: P.Gpu, reps: int) -> dict:
@@ -315,7 +350,7 @@ def main() -> int:
     if args.only in ("all", "converge"):
         log("spp to relMSE <= 0.01")
         out["s_per_frame_to_relmse_0.01"] = convergence(gpu, tris, S.path_camera(args.width, args.height, rr_from=3),
-                                                         args.width, args.height)
+                                                         args.width, args.height, net=load_net(gpu))
     if args.only in ("all", "forest"):
         log("forest of 1,024 bodies")
         out["forest_1024"] = forest_rates(gpu)
@@ -366,7 +401,6 @@ def main() -> int:
         line["shadow_per_frame"] = shadow / args.steps
         line["steps_per_ray"] = steps / max(rays, 1)
         line["tests_per_ray"] = tests / max(rays, 1)
-        line["learned_denoiser"] = None
         if not args.no_cpu:
             log("CPU twin baseline")
             line["cpu_baseline"] = cpu_baseline(tris, cam, w, h, args.cpu_seconds)

@@ -36,6 +36,7 @@ typedef struct phr_tracer phr_tracer;
 typedef struct phr_clock phr_clock;
 typedef struct phr_pass phr_pass;
 typedef struct phr_sort phr_sort;
+typedef struct phr_kpcn phr_kpcn;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* phr_last_error(void);
@@ -273,6 +274,54 @@ int phr_tracer_retree(phr_tracer* t, int* ok);
 phr_array* phr_tracer_nodes(phr_tracer* t);
 phr_array* phr_tracer_instances(phr_tracer* t);
 uint32_t phr_tracer_instance_count(const phr_tracer* t);
+
+/* ---- the learned denoiser (no counterpart in phiray: it stands beside
+ * kernel::Denoise, src/kernel/denoise.rs:26-64) ------------------------------
+ * A kernel-predicting convolutional network: a 4-level encoder/decoder of 3x3
+ * convs (32/48/64/96 channels) predicting a softmax-normalised 5x5 filter per
+ * pixel over the albedo-demodulated radiance. BF16 MFMA with FP32
+ * accumulation; FP32 master weights; Adam (beta 0.9/0.999, eps 1e-8).
+ * Frames are 10 floats a pixel (radiance 3, albedo 3, normal 3, depth 1: the
+ * tracer's readbacks, src/trace/read.rs:241-307), images row-major, a batch
+ * image after image; radiance arrays 3 floats a pixel. The loss is
+ * mean((y - t)^2 / (t^2 + 0.01)) over pixels and channels. */
+/* Layer table: 4 words a layer {cin, cout, level, relu}; returns the count. */
+uint32_t phr_kpcn_layers(uint32_t* out);
+/* Weights: each layer's W[9][cout][cin] (tap = 3 (dy + 1) + dx + 1) then
+ * b[cout], layer after layer, FP32. */
+uint64_t phr_kpcn_param_count(void);
+int phr_kpcn_new(phr_gpu* g, phr_kpcn** out);
+void phr_kpcn_free(phr_kpcn* k);
+/* Sets the master weights and clears the optimiser state. */
+int phr_kpcn_set_params(phr_kpcn* k, const float* host, uint64_t count);
+int phr_kpcn_get_params(phr_kpcn* k, float* host, uint64_t count);
+/* The gradient left by the last phr_kpcn_backward / phr_kpcn_train_step. */
+int phr_kpcn_get_grads(phr_kpcn* k, float* host, uint64_t count);
+/* Denoised radiance of b frames of w x h. */
+int phr_kpcn_infer(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, phr_array* out);
+/* The loss against targets (and the denoised radiance into `out` if given). */
+int phr_kpcn_loss(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b, uint32_t w, uint32_t h,
+                  phr_array* out, float* loss);
+/* Forward, loss and backward: the gradient w.r.t. every weight. `loss` may be
+ * null (then nothing waits for the device). */
+int phr_kpcn_backward(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b, uint32_t w, uint32_t h,
+                      float* loss);
+/* One Adam step at learning rate lr over the last gradient. */
+int phr_kpcn_adam(phr_kpcn* k, float lr);
+/* phr_kpcn_backward then phr_kpcn_adam. */
+int phr_kpcn_train_step(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b, uint32_t w, uint32_t h,
+                        float lr, float* loss);
+/* One training step over b images of w x h gathered from resident data sets:
+ * frame image fidx[j] of `frames` against target image tidx[j] of `targets`.
+ * PHR_ERR_ARG for an index past either set. */
+int phr_kpcn_train_indexed(phr_kpcn* k, phr_array* frames, const uint32_t* fidx, phr_array* targets,
+                           const uint32_t* tidx, uint32_t b, uint32_t w, uint32_t h, float lr, float* loss);
+/* The tracer's frame (10 floats a pixel) at float offset `at` of `out`. */
+int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at);
+/* The learned denoiser over what has been traced: into `out` (w * h * 3
+ * floats) when given, else in place of the linear sums (Tracer::denoise's
+ * keep = false, src/trace/read.rs:85-119). */
+int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out);
 
 #ifdef __cplusplus
 }

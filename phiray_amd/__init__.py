@@ -162,6 +162,21 @@ _SIGS = {
     "phr_tracer_nodes": (_P, [_P]),
     "phr_tracer_instances": (_P, [_P]),
     "phr_tracer_instance_count": (_U32, [_P]),
+    "phr_kpcn_layers": (_U32, [_P]),
+    "phr_kpcn_param_count": (_U64, []),
+    "phr_kpcn_new": (_I, [_P, C.POINTER(_P)]),
+    "phr_kpcn_free": (None, [_P]),
+    "phr_kpcn_set_params": (_I, [_P, _P, _U64]),
+    "phr_kpcn_get_params": (_I, [_P, _P, _U64]),
+    "phr_kpcn_get_grads": (_I, [_P, _P, _U64]),
+    "phr_kpcn_infer": (_I, [_P, _P, _U32, _U32, _U32, _P]),
+    "phr_kpcn_loss": (_I, [_P, _P, _P, _U32, _U32, _U32, _P, C.POINTER(C.c_float)]),
+    "phr_kpcn_backward": (_I, [_P, _P, _P, _U32, _U32, _U32, C.POINTER(C.c_float)]),
+    "phr_kpcn_adam": (_I, [_P, C.c_float]),
+    "phr_kpcn_train_step": (_I, [_P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
+    "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
+    "phr_tracer_frame": (_I, [_P, _P, _U64]),
+    "phr_tracer_denoise_learned": (_I, [_P, _P, _P]),
 }
 
 _lib = None
@@ -722,6 +737,17 @@ class Tracer:
         """`Tracer::denoise` (keep=False) / `denoise_shown` (keep=True)."""
         _check(lib().phr_tracer_denoise(self._h, 1 if keep else 0), "Tracer::denoise")
 
+    def denoise_learned(self, kpcn: "Kpcn", out: GpuArray | None = None) -> None:
+        """The learned denoiser over what has been traced: into `out` (w*h*3
+        floats) when given, else in place of the linear sums, as
+        `denoise(keep=False)` leaves them."""
+        _check(lib().phr_tracer_denoise_learned(self._h, kpcn._h, out._h if out is not None else None),
+               "Tracer::denoise_learned")
+
+    def frame_into(self, out: GpuArray, at: int = 0) -> None:
+        """The denoiser's input frame (10 floats a pixel) at float `at` of `out`."""
+        _check(lib().phr_tracer_frame(self._h, out._h, at), "Tracer frame")
+
     def radiance(self) -> np.ndarray:
         w, h = self.size
         out = np.zeros((w * h, 3), dtype=np.float32)
@@ -749,6 +775,96 @@ class Tracer:
     def free(self) -> None:
         if self._h:
             lib().phr_tracer_free(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Kpcn:
+    """The learned denoiser (a kernel-predicting CNN beside `kernel::Denoise`),
+    run and trained by the library's own HIP kernels. Frames are GpuArrays of
+    float32, 10 per pixel; radiance arrays 3 per pixel; batches image after
+    image."""
+
+    FRAME = 10
+
+    def __init__(self, gpu: Gpu, params: np.ndarray | None = None):
+        self.gpu = gpu
+        out = C.c_void_p()
+        _check(lib().phr_kpcn_new(gpu._h, C.byref(out)), "Kpcn::new")
+        self._h = out
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def layers() -> np.ndarray:
+        """{cin, cout, level, relu} per layer."""
+        n = lib().phr_kpcn_layers(None)
+        out = np.zeros((n, 4), dtype=np.uint32)
+        lib().phr_kpcn_layers(_ptr(out))
+        return out
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().phr_kpcn_param_count())
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().phr_kpcn_set_params(self._h, _ptr(p), p.size), "Kpcn::set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), dtype=np.float32)
+        _check(lib().phr_kpcn_get_params(self._h, _ptr(out), out.size), "Kpcn::params")
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), dtype=np.float32)
+        _check(lib().phr_kpcn_get_grads(self._h, _ptr(out), out.size), "Kpcn::grads")
+        return out
+
+    def infer(self, frames: GpuArray, b: int, w: int, h: int, out: GpuArray) -> None:
+        _check(lib().phr_kpcn_infer(self._h, frames._h, b, w, h, out._h), "Kpcn::infer")
+
+    def loss(self, frames: GpuArray, targets: GpuArray, b: int, w: int, h: int, out: GpuArray | None = None) -> float:
+        v = C.c_float(0.0)
+        _check(lib().phr_kpcn_loss(self._h, frames._h, targets._h, b, w, h, out._h if out is not None else None,
+                                   C.byref(v)), "Kpcn::loss")
+        return float(v.value)
+
+    def backward(self, frames: GpuArray, targets: GpuArray, b: int, w: int, h: int) -> float:
+        v = C.c_float(0.0)
+        _check(lib().phr_kpcn_backward(self._h, frames._h, targets._h, b, w, h, C.byref(v)), "Kpcn::backward")
+        return float(v.value)
+
+    def adam(self, lr: float) -> None:
+        _check(lib().phr_kpcn_adam(self._h, lr), "Kpcn::adam")
+
+    def train_step(self, frames: GpuArray, targets: GpuArray, b: int, w: int, h: int, lr: float,
+                   want_loss: bool = True) -> float | None:
+        v = C.c_float(0.0)
+        _check(lib().phr_kpcn_train_step(self._h, frames._h, targets._h, b, w, h, lr,
+                                         C.byref(v) if want_loss else None), "Kpcn::train_step")
+        return float(v.value) if want_loss else None
+
+    def train_indexed(self, frames: GpuArray, fidx, targets: GpuArray, tidx, w: int, h: int, lr: float,
+                      want_loss: bool = True) -> float | None:
+        """One step over images fidx of `frames` against images tidx of `targets`."""
+        fi = np.ascontiguousarray(fidx, dtype=np.uint32)
+        ti = np.ascontiguousarray(tidx, dtype=np.uint32)
+        if fi.size != ti.size:
+            raise ValueError("Kpcn::train_indexed: fidx and tidx differ in length")
+        v = C.c_float(0.0)
+        _check(lib().phr_kpcn_train_indexed(self._h, frames._h, _ptr(fi), targets._h, _ptr(ti), fi.size, w, h, lr,
+                                            C.byref(v) if want_loss else None), "Kpcn::train_indexed")
+        return float(v.value) if want_loss else None
+
+    def free(self) -> None:
+        if self._h:
+            lib().phr_kpcn_free(self._h)
         self._h = C.c_void_p()
 
     def __del__(self):

@@ -113,6 +113,7 @@ constexpr uint32_t Z_DEPTH = Z_SCAN_TICKET + 8;
 // block of the next. A level's inner-node counters live in LDS (2 bits each)
 // while its block owns at most RF_LDS_SLOTS slots, in global memory above.
 constexpr uint32_t RF_THREADS = 256;
+constexpr uint32_t RF_LEAF_THREADS = 1024;  // level 0 is latency-bound: more waves in flight
 constexpr uint32_t RF_SLOTS = 2048;
 constexpr uint32_t RF_GROUP = 64;
 constexpr uint32_t RF_LDS_SLOTS = RF_SLOTS * RF_GROUP;
@@ -763,7 +764,7 @@ __device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t 
 }
 
 // Level 0: one block per RF_SLOTS slots; a thread per leaf starting there.
-__global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __restrict__ prims, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(RF_LEAF_THREADS) void k_refit_leaves(const float4* __restrict__ prims, const uint32_t* __restrict__ order,
                                                              const PhrLink* __restrict__ links, const uint4* __restrict__ up,
                                                              const uint32_t* __restrict__ leaf_at, uint32_t n,
                                                              const uint32_t* __restrict__ seg_lo, const uint32_t* __restrict__ node_base,
@@ -773,14 +774,14 @@ __global__ __launch_bounds__(RF_THREADS) void k_refit_leaves(const float4* __res
     __shared__ uint32_t s_cnt[CW];
     __shared__ uint32_t s_fcount;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < CW; i += RF_THREADS) s_cnt[i] = 0u;
+    for (uint32_t i = tid; i < CW; i += RF_LEAF_THREADS) s_cnt[i] = 0u;
     if (tid == 0) s_fcount = 0u;
     __syncthreads();
     const uint64_t s0 = (uint64_t)blockIdx.x * RF_SLOTS, s1 = s0 + RF_SLOTS;
     Arrivals<CW> arr{s_cnt, nullptr, s0};
     uint32_t* out = front_out + s0;
-    for (uint32_t r = 0; r < RF_SLOTS / RF_THREADS; ++r) {
-        const uint64_t slot = s0 + r * RF_THREADS + tid;
+    for (uint32_t r = 0; r < RF_SLOTS / RF_LEAF_THREADS; ++r) {
+        const uint64_t slot = s0 + r * RF_LEAF_THREADS + tid;
         if (slot >= n) break;
         const uint32_t k = leaf_at[slot];
         if (k == PHR_NONE) continue;
@@ -1091,7 +1092,7 @@ int bvh_enqueue_refit(phr_bvh* b) {
     hipStream_t s = b->gpu->stream;
     float4* nodes = (float4*)b->nodes->ptr;
     uint32_t blocks = b->level0_blocks;
-    hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(RF_THREADS), 0, s, (const float4*)b->prims->ptr,
+    hipLaunchKernelGGL(k_refit_leaves, dim3(blocks), dim3(RF_LEAF_THREADS), 0, s, (const float4*)b->prims->ptr,
                        (const uint32_t*)b->order->ptr, b->links, b->up, b->leaf_at, n, b->seg_lo, b->node_base, b->segs,
                        b->leaf_flag, nodes, b->front[0], b->fcount[0]);
     uint64_t span = RF_SLOTS;

@@ -980,6 +980,8 @@ struct phr_tracer {
     uint32_t* bases_dev = nullptr;      // each mesh tree's root node, or NONE
     std::vector<uint32_t> placed, node_base;
     std::vector<PhrAabb> roots;  // each mesh tree's root box, object space
+    float* frame_dev = nullptr;  // the learned denoiser's input and output, sized for w x h
+    uint64_t frame_px = 0;
 };
 
 namespace {
@@ -1015,6 +1017,7 @@ void tracer_release(phr_tracer* t) {
                          t->placed_dev})
         phr_array_release(a);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
+    if (t->frame_dev) (void)hipFree(t->frame_dev);
     if (t->rays) (void)hipFree(t->rays);
     if (t->cam_dev) (void)hipFree(t->cam_dev);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
@@ -1584,6 +1587,87 @@ int phr_tracer_px(phr_tracer* t, uint64_t from, uint64_t count, uint32_t* out) {
 }
 
 phr_array* phr_tracer_image(phr_tracer* t) { return t ? t->px : nullptr; }
+
+namespace {
+
+// The frame the learned denoiser reads: per pixel the radiance, albedo,
+// normal and depth sums over the pixel's count (src/trace/read.rs:241-307),
+// ten floats in that order.
+__global__ __launch_bounds__(256) void k_frame(const uint32_t* __restrict__ px, uint64_t n, float* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = px[31 * n + i];
+    const float k = (float)(c > 1u ? c : 1u);
+    float* o = out + 10 * i;
+    for (int j = 0; j < 3; ++j) {
+        o[j] = __uint_as_float(px[1 * n + 3 * i + j]) / k;
+        o[3 + j] = __uint_as_float(px[7 * n + 3 * i + j]) / k;
+        o[6 + j] = __uint_as_float(px[10 * n + 3 * i + j]) / k;
+    }
+    o[9] = __uint_as_float(px[13 * n + i]) / k;
+}
+
+// The denoised radiance back into the linear sums: sum = value * count.
+__global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, uint64_t n, const float* __restrict__ rad) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = px[31 * n + i];
+    const float k = (float)(c > 1u ? c : 1u);
+    for (int j = 0; j < 3; ++j) px[1 * n + 3 * i + j] = __float_as_uint(rad[3 * i + j] * k);
+}
+
+int ensure_frame(phr_tracer* t) {
+    const uint64_t n = (uint64_t)t->w * t->h;
+    if (t->frame_dev && t->frame_px == n) return PHR_OK;
+    if (t->frame_dev) (void)hipFree(t->frame_dev);
+    t->frame_dev = nullptr;
+    t->frame_px = 0;
+    PHR_HIP(hipMalloc((void**)&t->frame_dev, n * 13 * 4));
+    t->frame_px = n;
+    return PHR_OK;
+}
+
+}  // namespace
+
+int phr_kpcn_infer_dev(phr_kpcn* k, const float* frames, uint32_t b, uint32_t w, uint32_t h, float* out);
+
+// The denoiser's input frame of what has been traced, 10 floats a pixel,
+// written at item `at` (in floats) of `out`, so a batch fills image by image.
+int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at) {
+    PHR_CHECK(t && t->px && out, PHR_ERR_ARG, "phr_tracer_frame: nothing traced yet");
+    const uint64_t n = (uint64_t)t->w * t->h;
+    PHR_CHECK((at + 10 * n) * 4 <= out->len * out->elem, PHR_ERR_SIZE, "phr_tracer_frame: output too small");
+    hipLaunchKernelGGL(k_frame, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, t->gpu->stream,
+                       (const uint32_t*)t->px->ptr, n, (float*)out->ptr + at);
+    PHR_HIP(hipGetLastError());
+    PHR_HIP(hipStreamSynchronize(t->gpu->stream));
+    return PHR_OK;
+}
+
+// The learned denoiser over what has been traced (no counterpart in phiray; it
+// stands beside Tracer::denoise). With `out`, the denoised radiance goes there
+// (w * h * 3 floats) and the image is left as it is; without, it replaces the
+// linear sums, as Tracer::denoise with keep = false does.
+int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
+    PHR_CHECK(t && k, PHR_ERR_ARG, "phr_tracer_denoise_learned: null");
+    if (!t->px) return PHR_OK;
+    const uint64_t n = (uint64_t)t->w * t->h;
+    if (out) PHR_CHECK(out->len * out->elem >= n * 12, PHR_ERR_SIZE, "phr_tracer_denoise_learned: output too small");
+    int rc = ensure_frame(t);
+    if (rc != PHR_OK) return rc;
+    const uint32_t g = (uint32_t)((n + 255) / 256);
+    float* rad = out ? (float*)out->ptr : t->frame_dev + 10 * n;
+    hipLaunchKernelGGL(k_frame, dim3(g), dim3(256), 0, t->gpu->stream, (const uint32_t*)t->px->ptr, n, t->frame_dev);
+    PHR_HIP(hipGetLastError());
+    rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
+    if (rc != PHR_OK) return rc;
+    if (!out) {
+        hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, t->gpu->stream, (uint32_t*)t->px->ptr, n, rad);
+        PHR_HIP(hipGetLastError());
+    }
+    PHR_HIP(hipStreamSynchronize(t->gpu->stream));
+    return PHR_OK;
+}
 
 // Tracer::denoise / denoise_shown (src/trace/read.rs:73-119): four a-trous
 // passes over what has been traced; `keep` leaves the linear sums as they are.

@@ -1,0 +1,199 @@
+"""Host side of the learned denoiser: weight initialisation and files, the
+training set the path tracer renders, and the training loop.
+
+Everything that computes runs in the library's HIP kernels (`Kpcn`,
+`Tracer`); this module only decides what to render and which images a
+batch holds. The network stands beside phiray's `kernel::Denoise`
+(src/kernel/denoise.rs:26-64), which it does not replace.
+
+Training data: seeded spheres scenes (the Rendered-mode generator) seen by
+the 1920x1080 camera, cropped to `crop` x `crop` windows. Inputs are
+`spp_in`-sample frames, several independent renders per scene. Targets are
+`spp_ref`-sample renders of the same window on sample streams the inputs
+never draw.
+"""
+from __future__ import annotations
+
+import pathlib
+import time
+
+import numpy as np
+
+from . import Gpu, GpuArray, Kpcn, Tracer
+from . import scenes as S
+
+
+def _log(msg: str) -> None:
+    print(msg, flush=True)
+
+HERE = pathlib.Path(__file__).resolve().parent
+WEIGHTS = HERE / "kpcn_weights.npy"
+REF_STREAM = 1 << 24  # sample streams of the targets; inputs use small ones
+
+
+def layer_slices(layers: np.ndarray | None = None) -> list[tuple[slice, slice, tuple[int, int]]]:
+    """(weights, bias, (cout, cin)) slices of the flat parameter vector."""
+    layers = Kpcn.layers() if layers is None else layers
+    out, off = [], 0
+    for cin, cout, _level, _relu in layers.tolist():
+        nw = 9 * cout * cin
+        out.append((slice(off, off + nw), slice(off + nw, off + nw + cout), (cout, cin)))
+        off += nw + cout
+    return out
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights over the used input channels, zero biases. The padded
+    input channels (10..15 of the first layer) and logits (25..31) stay zero.
+    The output layer starts at a tenth of He scale, so the first filters are
+    close to a box."""
+    rng = np.random.default_rng(seed)
+    layers = Kpcn.layers()
+    p = np.zeros(Kpcn.param_count(), dtype=np.float32)
+    sl = layer_slices(layers)
+    for i, (ws, _bs, (cout, cin)) in enumerate(sl):
+        used_in = 10 if i == 0 else cin
+        used_out = 25 if i == len(sl) - 1 else cout
+        std = np.sqrt(2.0 / (9 * used_in)) * (0.1 if i == len(sl) - 1 else 1.0)
+        w = np.zeros((9, cout, cin), dtype=np.float32)
+        w[:, :used_out, :used_in] = rng.normal(0.0, std, size=(9, used_out, used_in))
+        p[ws] = w.ravel()
+    return p
+
+
+def save(path: pathlib.Path | str, params: np.ndarray) -> None:
+    np.save(path, np.ascontiguousarray(params, dtype=np.float32), allow_pickle=False)
+
+
+def load(path: pathlib.Path | str = WEIGHTS) -> np.ndarray:
+    p = np.load(path, allow_pickle=False)
+    if p.dtype != np.float32 or p.size != Kpcn.param_count():
+        raise ValueError(f"{path}: not a denoiser weight file ({p.dtype}, {p.size} values)")
+    return p
+
+
+def crop_camera(w: int, h: int, x0: int, y0: int, cw: int, ch: int, **kw) -> np.ndarray:
+    """The camera of path_camera(w, h) restricted to the window [x0, x0+cw) x
+    [y0, y0+ch): same rays per pixel, through the view-plane offset `pan`."""
+    cam = S.path_camera(w, h, **kw)
+    full_scale = float(cam["scale"][0])
+    c = cam[0]
+    c["scale"] = full_scale * w / cw
+    c["aspect"] = cw / ch
+    # su_crop - pan0 = su_full * cw / w  (and the same for v, over the aspect).
+    c["pan"][0] = (w / 2 - cw / 2 - x0) / cw
+    c["pan"][1] = (h / 2 - ch / 2 - y0) / cw
+    return cam
+
+
+def train(gpu: Gpu, frames: GpuArray, targets: GpuArray, n_scenes: int, variants: int, crop: int, steps: int,
+          batch: int = 8, lr: float = 1e-3, seed: int = 0, params: np.ndarray | None = None, log=_log,
+          log_every: int = 200, budget_s: float | None = None) -> tuple[Kpcn, list[tuple[int, float]]]:
+    """Adam at `lr` over random batches of (scene, render) images; image
+    s * variants + j is render j of scene s and is scored against target s."""
+    net = Kpcn(gpu, init_params(seed) if params is None else params)
+    rng = np.random.default_rng(seed + 1)
+    hist, t0 = [], time.perf_counter()
+    for step in range(steps):
+        scenes = rng.integers(0, n_scenes, size=batch)
+        var = rng.integers(0, variants, size=batch)
+        want = (step + 1) % log_every == 0 or step == steps - 1
+        loss = net.train_indexed(frames, scenes * variants + var, targets, scenes, crop, crop, lr, want_loss=want)
+        if want:
+            hist.append((step + 1, loss))
+            el = time.perf_counter() - t0
+            log(f"[kpcn] step {step + 1}/{steps} loss {loss:.5f} ({el:.0f} s, {1e3 * el / (step + 1):.1f} ms/step)")
+            if budget_s is not None and el > budget_s:
+                log(f"[kpcn] time budget reached at step {step + 1}")
+                break
+    gpu.wait()
+    return net, hist
+
+
+def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
+    """mean((x - y)^2 / (y^2 + 0.01)) over pixels and channels."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    return float(np.mean((x - y) ** 2 / (y * y + 0.01)))
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Train the learned denoiser on path-traced pairs.")
+    ap.add_argument("--scenes", type=int, default=32, help="training scenes (seeds 1000, 1001, ...)")
+    ap.add_argument("--crop", type=int, default=512)
+    ap.add_argument("--spp-ref", type=int, default=4096)
+    ap.add_argument("--spp-in", type=str, default="1,2,4", help="input sample counts, one variant set each")
+    ap.add_argument("--variants", type=int, default=4, help="renders per scene per input sample count")
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--budget-s", type=float, default=None, help="stop training after this many seconds")
+    ap.add_argument("--out", type=str, default=str(WEIGHTS))
+    ap.add_argument("--log", type=str, default=None, help="JSON file for the loss history")
+    args = ap.parse_args(argv)
+    gpu = Gpu.new(0)
+    if gpu is None:
+        raise SystemExit("no HIP device")
+    spps = [int(s) for s in args.spp_in.split(",")]
+    seeds = list(range(1000, 1000 + args.scenes))
+    t0 = time.perf_counter()
+    frames, targets, meta = render_multi(gpu, seeds, args.crop, spps, args.spp_ref, args.variants)
+    _log(f"[kpcn] training set rendered in {time.perf_counter() - t0:.0f} s")
+    net, hist = train(gpu, frames, targets, len(seeds), len(spps) * args.variants, args.crop, args.steps,
+                      batch=args.batch, lr=args.lr, budget_s=args.budget_s)
+    save(args.out, net.params())
+    _log(f"[kpcn] weights written to {args.out}")
+    if args.log:
+        with open(args.log, "w") as f:
+            json.dump({"args": vars(args), "scenes": meta, "loss": hist}, f)
+    return 0
+
+
+def render_multi(gpu: Gpu, seeds, crop: int, spps, spp_ref: int, variants: int, full=(1920, 1080),
+                 log=_log) -> tuple[GpuArray, GpuArray, list[dict]]:
+    """Frames and targets rendered on the device. Each scene gets `variants`
+    renders at each input sample count in `spps`, so image s * V + j
+    (V = len(spps) * variants) is render j of scene s; target s is scene s at
+    `spp_ref` samples."""
+    w, h = full
+    npx = crop * crop
+    V = len(spps) * variants
+    frames = GpuArray.zeroed(gpu, np.float32, len(seeds) * V * npx * Kpcn.FRAME)
+    targets = GpuArray.zeroed(gpu, np.float32, len(seeds) * npx * 3)
+    meta = []
+    for si, seed in enumerate(seeds):
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(seed)
+        x0 = int(rng.integers(0, w - crop + 1))
+        y0 = int(rng.integers(0, h - crop + 1))
+        tr = Tracer.new(gpu, S.spheres_scene(seed=int(seed)), 4)
+        tr.set_samples(1 << 30)
+        cam = crop_camera(w, h, x0, y0, crop, crop, rr_from=3)
+        ref = cam.copy()
+        ref["stream"] = REF_STREAM
+        left = spp_ref
+        while left > 0:
+            step = min(64, left)
+            tr.sample(ref, crop, crop, step)
+            left -= step
+        targets.write_at(si * npx * 3, tr.radiance().ravel())
+        j, stream = 0, 1
+        for spp in spps:
+            for _ in range(variants):
+                c = cam.copy()
+                c["stream"] = stream
+                stream += spp
+                tr.sample(c, crop, crop, spp)
+                tr.frame_into(frames, (si * V + j) * npx * Kpcn.FRAME)
+                j += 1
+        tr.free()
+        meta.append({"seed": int(seed), "x0": x0, "y0": y0})
+        log(f"[kpcn] scene {si + 1}/{len(seeds)} seed {seed} crop ({x0},{y0}) in {time.perf_counter() - t0:.1f} s")
+    return frames, targets, meta
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
