@@ -368,6 +368,8 @@ def main() -> int:
         t0 = time.perf_counter()
         tracer = P.Tracer.new(gpu, tris, 4)
         log(f"Tracer::new {time.perf_counter() - t0:.2f} s, depth {tracer.depth}, {tracer.glow} emitters")
+        waves, tune_s = tracer.autotune(cam, w, h, 8, reps=2)
+        log(f"autotune: path kernel at {waves} waves a SIMD ({tune_s * 1e3:.1f} ms for 8 spp)")
         for i in range(args.warmup):
             tracer.sample(cam, w, h, spp)
             log(f"warmup frame {i}: {tracer.time()[1]:.3f} s")
@@ -396,6 +398,7 @@ def main() -> int:
                             "per_ray_bytes": "32 ray + 8 hit + 32 root + 64 x inner steps + 48 x triangle tests",
                             "launch_ms": kernel_s * 1e3}
         line["s_per_frame"] = per
+        line["path_waves_per_simd"] = tracer.path_waves
         line["rays_per_frame"] = rays / args.steps
         line["closest_per_frame"] = closest / args.steps
         line["shadow_per_frame"] = shadow / args.steps

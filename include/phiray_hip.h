@@ -10,7 +10,7 @@
  *    phr_last_error() describes the last failure on the calling thread;
  *  - plain pointers and sizes only; POD records cross by bytes in phiray's
  *    own layouts: Aabb 32 B, Node 32 B, Link 16 B, Hit 20 B, ray 32 B
- *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 176 B;
+ *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 192 B;
  *  - one HIP stream per phr_gpu; calls on one phr_gpu are serialised by the
  *    caller (phiray's `Tracer` is `&mut` from one render thread).
  */
@@ -198,7 +198,7 @@ int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array
 /* Tracer::set_samples (src/trace/mod.rs:242). */
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
 /* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
- * cam is the 176-byte camera record. */
+ * cam is the 192-byte camera record. */
 int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done);
 /* Tracer::time (src/trace/sample.rs:253-255). */
 int phr_tracer_time(phr_tracer* t, double* submit_s, double* gpu_s);
@@ -274,6 +274,19 @@ int phr_tracer_retree(phr_tracer* t, int* ok);
 phr_array* phr_tracer_nodes(phr_tracer* t);
 phr_array* phr_tracer_instances(phr_tracer* t);
 uint32_t phr_tracer_instance_count(const phr_tracer* t);
+
+/* Pixels Adaptive Sampling has not settled (src/trace/sample.rs `pixels
+ * left`, Camera::adapt = {threshold, min samples, step (a power of two), on});
+ * PHR_ERR_UNSUPPORTED when the last camera had it off. */
+int phr_tracer_open_pixels(phr_tracer* t, uint32_t* count);
+/* The path kernel variant (no counterpart in phiray): register allocation
+ * fitted to 2 (default), 3 or 4 waves a SIMD. */
+int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves);
+uint32_t phr_tracer_path_waves(const phr_tracer* t);
+/* Autotune (no counterpart in phiray): time each variant on `spp` samples of
+ * w x h under `cam` (best of `reps`), keep the fastest; the image starts over. */
+int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,
+                        uint32_t* chosen, double* best_s);
 
 /* ---- the learned denoiser (no counterpart in phiray: it stands beside
  * kernel::Denoise, src/kernel/denoise.rs:26-64) ------------------------------

@@ -104,6 +104,7 @@ typedef struct {
     uint32_t depth[4];
     float integrator[4];
     uint32_t w, h, rr_from, stream;  /* added to the sample index in the seed */
+    float adapt[4];  /* Adaptive Sampling: threshold, min samples, step, on (camera.rs `adapt`) */
 } orc_camera;
 
 /* Input triangle: src/trace/kernel.rs:11-40 (Tri) rows a..nc, color, emit —
@@ -205,6 +206,13 @@ void orc_stamp_camera(orc_camera* cam, uint32_t w, uint32_t h, float extent, uin
 void orc_sample(const orc_scene_t* s, const orc_camera* cam, uint32_t first, uint32_t count,
                 uint32_t* px, uint64_t* rays, int threads);
 
+
+/* Adaptive Sampling's tail stages after sample cam->frame
+ * (src/trace/wgsl/adapt.wgsl): the convergence check (stage 0, a no-op
+ * unless the sample is a step boundary past the minimum), then the
+ * four-neighbour dilation (stage 1). */
+void orc_adapt_check(const orc_camera* cam, uint32_t* px);
+void orc_adapt_dilate(const orc_camera* cam, uint32_t* px);
 
 /* One a-trous pass (src/kernel/denoise.wgsl:47-134) over the image buffer;
  * the display planes (view transform, packed pixel) are not restated. */

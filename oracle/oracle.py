@@ -52,6 +52,8 @@ _SIGS = {
     "orc_hardware_threads": (_I, []),
     "orc_log2": (C.c_float, [C.c_float]),
     "orc_denoise": (None, [_P, _U32, _U32, _U32]),
+    "orc_adapt_check": (None, [_P, _P]),
+    "orc_adapt_dilate": (None, [_P, _P]),
     "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P]),
     "orc_inst_inverse": (None, [_P, _P]),
     "orc_inst_mirrors": (_U32, [_P]),
@@ -267,6 +269,12 @@ class Forest:
         lib().orc_sample(C.byref(self._c), ptr(c), first, count, ptr(px), ptr(rays), nthreads)
         return int(rays[0]), int(rays[1])
 
+    def adapt_tail(self, cam: np.ndarray, w: int, h: int, px: np.ndarray, frame: int) -> None:
+        """Adaptive Sampling's check and dilation after sample `frame`."""
+        c = self.stamp(cam, w, h, frame)
+        lib().orc_adapt_check(ptr(c), ptr(px))
+        lib().orc_adapt_dilate(ptr(c), ptr(px))
+
 
 class Scene:
     """The oracle's tracer scene (all arrays kept alive here)."""
@@ -307,3 +315,9 @@ class Scene:
         count = n - first if count is None else count
         lib().orc_sample(C.byref(self._c), ptr(c), first, count, ptr(px), ptr(rays), nthreads)
         return int(rays[0]), int(rays[1])
+
+    def adapt_tail(self, cam: np.ndarray, w: int, h: int, px: np.ndarray, frame: int) -> None:
+        """Adaptive Sampling's check and dilation after sample `frame`."""
+        c = self.stamp(cam, w, h, frame)
+        lib().orc_adapt_check(ptr(c), ptr(px))
+        lib().orc_adapt_dilate(ptr(c), ptr(px))

@@ -29,6 +29,7 @@
 // undefined there); sin/cos/exp2 are the deterministic ones in walk.cpp.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -456,7 +457,11 @@ void guide(uint32_t* px, size_t n, size_t i, bool keep, V3 albedo, V3 nn, float 
     pxs(px, c, old_d + dist);
 }
 
-void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[4], float cover, float missed) {
+// util.wgsl:72-96: Adaptive Sampling's words and switch.
+inline bool adapt_on(const orc_camera* p) { return p->adapt[3] != 0.0f; }
+
+void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[4], float cover, float missed,
+          const orc_camera* p) {
     size_t a = n + 3 * i, g = 20 * n + 3 * i, gc = 23 * n + i, fc = 24 * n + i, mc = 33 * n + i, cn = 31 * n + i;
     float gone = (keep ? pxf(px, mc) : 0.0f) + missed;
     pxs(px, mc, gone);
@@ -470,6 +475,12 @@ void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[
     pxs(px, a, l.x); pxs(px, a + 1, l.y); pxs(px, a + 2, l.z);
     pxs(px, g, gs.x); pxs(px, g + 1, gs.y); pxs(px, g + 2, gs.z);
     pxs(px, gc, cov);
+    // util.wgsl:126-132: the half image takes the odd samples at double weight.
+    if (adapt_on(p) && (p->frame & 1u) == 1u) {
+        size_t x = 26 * n + 3 * i;
+        V3 half = (keep ? v3(pxf(px, x), pxf(px, x + 1), pxf(px, x + 2)) : v3(0, 0, 0)) + lin * 2.0f;
+        pxs(px, x, half.x); pxs(px, x + 1, half.y); pxs(px, x + 2, half.z);
+    }
 }
 
 // shade.wgsl:41-373 for one pixel and the camera's frame.
@@ -479,6 +490,13 @@ void shade(Ctx& c, uint32_t i, uint32_t* px) {
     size_t n = (size_t)p->w * p->h;
     bool keep = p->frame > 0u;
     if (p->frame == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
+    if (p->frame == 0u && adapt_on(p)) {
+        // A new image starts its half image at zero. The reference does not:
+        // its sample 1 adds to whatever the plane held (util.wgsl:126-132 keep
+        // it from frame 1 on, and frame 0 never writes it).
+        px[26 * n + 3 * i] = 0u; px[26 * n + 3 * i + 1] = 0u; px[26 * n + 3 * i + 2] = 0u;
+    }
+    if (adapt_on(p) && px[30 * n + i] != 0u) return;  // settled (shade.wgsl:64)
     float fw = (float)p->w, fh = (float)p->h;
     float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
     float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
@@ -514,7 +532,7 @@ void shade(Ctx& c, uint32_t i, uint32_t* px) {
                 // The theme's sky (black here) rides the overlay term, not the light.
                 static const float sky_over[4] = {0, 0, 0, 1};
                 guide(px, n, i, keep, v3(0, 0, 0), -d, -1.0f);
-                emit(px, n, i, keep, v3(0, 0, 0), sky_over, 1.0f, 1.0f);
+                emit(px, n, i, keep, v3(0, 0, 0), sky_over, 1.0f, 1.0f, p);
                 return;
             }
             col = col + thr * env_at(p, d);
@@ -573,7 +591,7 @@ void shade(Ctx& c, uint32_t i, uint32_t* px) {
         col = col + thr * cc;
         break;  // opaque: tw == 0 ends the camera path (shade.wgsl:345-346)
     }
-    emit(px, n, i, keep, col, none, 1.0f, 0.0f);
+    emit(px, n, i, keep, col, none, 1.0f, 0.0f, p);
 }
 
 }  // namespace
@@ -616,6 +634,47 @@ void orc_sample(const orc_scene_t* s, const orc_camera* cam, uint32_t first, uin
         for (auto& th : pool) th.join();
     }
     if (rays) { rays[0] += closest; rays[1] += shadow; }
+}
+
+
+// adapt.wgsl:47-79 (stage 0): Cycles' convergence check, at the last sample
+// of each step past the minimum, for pixels not yet settled.
+void orc_adapt_check(const orc_camera* p, uint32_t* px) {
+    if (!adapt_on(p)) return;
+    const uint32_t sample = p->frame, min_samples = (uint32_t)p->adapt[1];
+    const uint32_t step = std::max<uint32_t>((uint32_t)p->adapt[2], 1u);
+    if (sample <= min_samples || (sample & (step - 1u)) != step - 1u) return;
+    const size_t n = (size_t)p->w * p->h;
+    for (size_t i = 0; i < n; ++i) {
+        if (px[29 * n + i] != 0u) continue;
+        const size_t a = n + 3 * i, x = 26 * n + 3 * i;
+        const uint32_t taken = std::max<uint32_t>(px[31 * n + i], 1u);
+        const float scale = 1.0f / (float)taken;
+        const float d0 = std::fabs(pxf(px, a) - pxf(px, x)), d1 = std::fabs(pxf(px, a + 1) - pxf(px, x + 1)),
+                    d2 = std::fabs(pxf(px, a + 2) - pxf(px, x + 2));
+        const float err_diff = ((d0 + d1) + d2) * scale;
+        const float intensity = ((pxf(px, a) + pxf(px, a + 1)) + pxf(px, a + 2)) * scale;
+        const float norm = intensity < 1.0f ? std::sqrt(intensity) : intensity;
+        const float error = err_diff / (0.0001f + norm);
+        if (error < p->adapt[0]) px[29 * n + i] = 1u;
+    }
+}
+
+// adapt.wgsl:82-96 (stage 1): settled only where the pixel and its four
+// neighbours are.
+void orc_adapt_dilate(const orc_camera* p, uint32_t* px) {
+    if (!adapt_on(p)) return;
+    const uint32_t w = p->w, h = p->h;
+    const size_t n = (size_t)w * h;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t x = (uint32_t)(i % w), y = (uint32_t)(i / w);
+        uint32_t all = px[29 * n + i];
+        if (x > 0u) all = std::min(all, px[29 * n + i - 1]);
+        if (x + 1u < w) all = std::min(all, px[29 * n + i + 1]);
+        if (y > 0u) all = std::min(all, px[29 * n + i - w]);
+        if (y + 1u < h) all = std::min(all, px[29 * n + i + w]);
+        px[30 * n + i] = all;
+    }
 }
 
 }  // extern "C"

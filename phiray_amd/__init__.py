@@ -36,9 +36,10 @@ CAMERA = np.dtype([
     ("sun", "<f4", 3), ("dist", "<f4"), ("pan", "<f4", 2), ("ao", "<f4"), ("eps", "<f4"),
     ("env", "<f4", 4), ("frame", "<u4"), ("count", "<u4"), ("glow", "<u4"), ("bounces", "<u4"),
     ("depth", "<u4", 4), ("integrator", "<f4", 4), ("w", "<u4"), ("h", "<u4"), ("rr_from", "<u4"), ("stream", "<u4"),
+    ("adapt", "<f4", 4),
 ])
 assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
-assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 176
+assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 192
 
 # A scene of meshes under instances (src/trace/scene.rs:40-67, mesh.rs:12-22,
 # instance.rs:23-44): a vertex is its object-space position (w 1) then its
@@ -176,6 +177,10 @@ _SIGS = {
     "phr_kpcn_train_step": (_I, [_P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
+    "phr_tracer_set_path_waves": (_I, [_P, _U32]),
+    "phr_tracer_open_pixels": (_I, [_P, C.POINTER(_U32)]),
+    "phr_tracer_path_waves": (_U32, [_P]),
+    "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_tracer_denoise_learned": (_I, [_P, _P, _P]),
 }
 
@@ -708,6 +713,29 @@ class Tracer:
         _check(lib().phr_tracer_sample(self._h, _ptr(cam), w, h, spp, C.byref(frame), C.byref(done)), "Tracer::sample")
         self.size = (max(w, 1), max(h, 1))
         return int(frame.value), bool(done.value)
+
+    def open_pixels(self) -> int:
+        """Pixels Adaptive Sampling has not settled."""
+        c = C.c_uint32(0)
+        _check(lib().phr_tracer_open_pixels(self._h, C.byref(c)), "Tracer open pixels")
+        return int(c.value)
+
+    def set_path_waves(self, waves: int) -> None:
+        _check(lib().phr_tracer_set_path_waves(self._h, waves), "Tracer path waves")
+
+    @property
+    def path_waves(self) -> int:
+        return int(lib().phr_tracer_path_waves(self._h))
+
+    def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 2) -> tuple[int, float]:
+        """Time each path kernel variant on this workload and keep the fastest:
+        (waves a SIMD chosen, its seconds). The image starts over."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        chosen, best = C.c_uint32(0), C.c_double(0)
+        _check(lib().phr_tracer_autotune(self._h, _ptr(cam), w, h, spp, reps, C.byref(chosen), C.byref(best)),
+               "Tracer autotune")
+        self.size = (max(w, 1), max(h, 1))
+        return int(chosen.value), float(best.value)
 
     def time(self) -> tuple[float, float]:
         """`Tracer::time`: (submit seconds, GPU seconds) of the last sample call."""

@@ -30,10 +30,11 @@
 // image is padded to multiples of 8 by repeating its edge.
 //
 // Kernels are hand-written for gfx950:
-//  * conv: an implicit GEMM on v_mfma_f32_16x16x16_bf16. D[co][pixel] =
-//    W[co][tap, ci] * X[tap, ci][pixel]. A wave owns NF x 16 channels by
-//    MT x 16 pixels. The same kernel computes the input gradient, with the
-//    weights flipped and transposed.
+//  * conv: an implicit GEMM on v_mfma_f32_16x16x32_bf16 (16x16x16 for a last
+//    16-channel chunk). D[co][pixel] = W[co][tap, ci] * X[tap, ci][pixel]. A
+//    block owns a 32-column tile and all output channels; the input tile and
+//    its halo are staged in LDS. The same kernel computes the input gradient,
+//    with the weights flipped and transposed.
 //  * wgrad: dW[tap][co][ci] = sum_p dZ[p][co] X[p + off(tap)][ci]. The pixel
 //    tiles are staged transposed in LDS, so the reduction over pixels is the
 //    MFMA K dimension. Each block writes a partial for its pixel chunk, and a
@@ -85,6 +86,14 @@ __device__ __forceinline__ float from_bf16s(short h) { return from_bf16((uint16_
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// gfx950's 16x16x32 form: 8 bf16 of K a lane. A and B fragments are loaded
+// with the same lane-slot to channel map, so the product does not depend on
+// the order the hardware gives the K slots.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+                                                   0);
+}
 
 // ---- features ---------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_features(const float* __restrict__ fr, uint32_t B, uint32_t H, uint32_t W,
@@ -122,66 +131,79 @@ struct ConvArgs {
     uint32_t B, H, W, cin, cout, relu;
 };
 
-template <int NF, int MT>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+// A block computes a tile of 4 RW rows x 32 columns of one image, all output
+// channels. The input tile with its one-pixel halo is staged in LDS 32
+// channels at a time (16 B a load, coalesced along the channels). Every one of
+// the nine taps then reads its B fragments from LDS. The A fragments (weights)
+// come from global memory through the caches, and each is shared by the
+// wave's 2 RW pixel fragments.
+constexpr int TILE_W = 32;
+constexpr int CHUNK = 32;            // input channels staged at once
+constexpr int HALO_STRIDE = CHUNK + 8;  // shorts a staged pixel (80 B: 16-B aligned, banks spread)
+
+template <int NF, int RW>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+    constexpr int TH = 4 * RW, MT = 2 * RW;
+    constexpr int HWD = TILE_W + 2, HP = HWD * (TH + 2);
+    __shared__ __align__(16) uint16_t tile[HP * HALO_STRIDE];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t HW = (uint64_t)a.H * a.W, P = a.B * HW;
-    const uint64_t p0 = ((uint64_t)blockIdx.x * 4 + wave) * (16 * MT);
-    if (p0 >= P) return;
-    int py[MT], pxx[MT];
-    uint64_t pb[MT];
-    bool pv[MT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        uint64_t p = p0 + 16 * m + (lane & 15u);
-        pv[m] = p < P;
-        uint64_t q = pv[m] ? p : 0;
-        uint64_t b = q / HW, rr = q % HW;
-        pb[m] = b * HW;
-        py[m] = (int)(rr / a.W);
-        pxx[m] = (int)(rr % a.W);
-    }
+    uint32_t t = blockIdx.x;
+    const uint32_t tx = t % tiles_x;
+    t /= tiles_x;
+    const uint32_t ty = t % tiles_y, b = t / tiles_y;
+    const int x0 = (int)tx * TILE_W, y0 = (int)ty * TH;
+    const int H = (int)a.H, W = (int)a.W;
+    const uint64_t img = (uint64_t)b * a.H * a.W;
+    const uint32_t cin = a.cin;
     f32x4 acc[NF][MT];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[f][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint32_t kq = 4u * (lane >> 4);
-    const uint32_t cin = a.cin;
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const uint16_t* src[MT];
-        bool ok[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            int yy = py[m] + dy, xx = pxx[m] + dx;
-            ok[m] = pv[m] && yy >= 0 && yy < (int)a.H && xx >= 0 && xx < (int)a.W;
-            src[m] = a.x + (pb[m] + (uint64_t)(ok[m] ? yy : 0) * a.W + (uint64_t)(ok[m] ? xx : 0)) * cin + kq;
+    const uint32_t g = lane >> 4, col = lane & 15u;
+    for (uint32_t c0 = 0; c0 < cin; c0 += CHUNK) {
+        // A last chunk of 16 channels runs the same instruction with the K
+        // slots of lane groups 2 and 3 zero in both operands.
+        const uint32_t cc = min((uint32_t)CHUNK, cin - c0), parts = cc / 8;
+        const bool live = g < parts;
+        __syncthreads();  // the previous chunk's reads are done
+        for (uint32_t u = threadIdx.x; u < HP * parts; u += 256) {
+            const uint32_t pix = u / parts, part = u % parts;
+            const int gy = y0 + (int)(pix / HWD) - 1, gx = x0 + (int)(pix % HWD) - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *(const uint4*)(a.x + (img + (uint64_t)gy * a.W + gx) * cin + c0 + 8 * part);
+            *(uint4*)(tile + pix * HALO_STRIDE + 8 * part) = v;
         }
-        const uint16_t* wt = a.w + (uint64_t)tap * a.cout * cin + (uint64_t)(lane & 15u) * cin + kq;
-        for (uint32_t c0 = 0; c0 < cin; c0 += 16) {
-            bf16x4 xb[MT];
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // halo offsets
+            const uint16_t* wt = a.w + ((uint64_t)tap * a.cout + col) * cin + c0 + 8 * g;
+            uint4 bx[MT];
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
-                xb[m] = bf16x4{0, 0, 0, 0};
-                if (ok[m]) xb[m] = *(const bf16x4*)(src[m] + c0);
+                const int r = (int)wave * RW + m / 2, c = (m & 1) * 16 + (int)col;
+                bx[m] = live ? *(const uint4*)(tile + ((r + dy) * HWD + c + dx) * HALO_STRIDE + 8 * g)
+                             : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                bf16x4 wa = *(const bf16x4*)(wt + (uint64_t)(16 * f) * cin + c0);
+                const uint4 wa = live ? *(const uint4*)(wt + (uint64_t)(16 * f) * cin) : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-                for (int m = 0; m < MT; ++m) acc[f][m] = mfma16(wa, xb[m], acc[f][m]);
+                for (int m = 0; m < MT; ++m) acc[f][m] = mfma32(wa, bx[m], acc[f][m]);
             }
         }
     }
-    // Lane holds D[co = 16 f + kq + j][pixel = p0 + 16 m + lane % 16].
+    // Lane holds D[co = 16 f + 4 g + j][pixel = fragment m's column lane % 16].
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-        if (!pv[m]) continue;
-        const uint64_t p = p0 + 16 * m + (lane & 15u);
+        const int y = y0 + (int)wave * RW + m / 2, x = x0 + (m & 1) * 16 + (int)col;
+        if (y >= H || x >= W) continue;
+        const uint64_t p = img + (uint64_t)y * a.W + x;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const uint32_t co = 16u * f + kq;
+            const uint32_t co = 16u * f + 4u * g;
             float v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -632,11 +654,10 @@ int ensure_ws(phr_kpcn* k, uint32_t B, uint32_t H, uint32_t W, bool train) {
     return PHR_OK;
 }
 
-template <int NF, int MT>
+template <int NF, int RW>
 void launch_conv_t(const ConvArgs& a, hipStream_t s) {
-    const uint64_t P = (uint64_t)a.B * a.H * a.W;
-    const uint64_t blocks = (P + 64 * MT - 1) / (64 * MT);
-    hipLaunchKernelGGL((k_conv<NF, MT>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    const uint32_t tx = (a.W + TILE_W - 1) / TILE_W, ty = (a.H + 4 * RW - 1) / (4 * RW);
+    hipLaunchKernelGGL((k_conv<NF, RW>), dim3(tx * ty * a.B), dim3(256), 0, s, a, tx, ty);
 }
 
 int launch_conv(const ConvArgs& a, hipStream_t s) {
@@ -645,13 +666,13 @@ int launch_conv(const ConvArgs& a, hipStream_t s) {
         return PHR_ERR_ARG;
     }
     switch (a.cout / 16) {
-        case 2: launch_conv_t<2, 4>(a, s); break;
-        case 3: launch_conv_t<3, 4>(a, s); break;
+        case 2: launch_conv_t<2, 2>(a, s); break;
+        case 3: launch_conv_t<3, 2>(a, s); break;
         case 4: launch_conv_t<4, 2>(a, s); break;
         case 5: launch_conv_t<5, 2>(a, s); break;
         case 6: launch_conv_t<6, 2>(a, s); break;
-        case 7: launch_conv_t<7, 2>(a, s); break;
-        case 10: launch_conv_t<10, 2>(a, s); break;
+        case 7: launch_conv_t<7, 1>(a, s); break;
+        case 10: launch_conv_t<10, 1>(a, s); break;
         default: phr_set_error_msg("kpcn conv: unsupported output channel count"); return PHR_ERR_ARG;
     }
     PHR_HIP(hipGetLastError());

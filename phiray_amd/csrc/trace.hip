@@ -40,8 +40,9 @@ struct PhrCamera {
     uint32_t depth[4];
     float integrator[4];
     uint32_t w, h, rr_from, stream;  // stream: added to the sample index in the seed (0: the reference's seeds)
+    float adapt[4];  // Adaptive Sampling: threshold, min samples, step (a power of two), on (camera.rs `adapt`)
 };
-static_assert(sizeof(PhrCamera) == 176, "camera layout");
+static_assert(sizeof(PhrCamera) == 192, "camera layout");
 
 // Input triangle: Tri's rows a..nc, color and emit (src/trace/kernel.rs:11-40).
 struct PhrTri {
@@ -827,17 +828,35 @@ __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = _
 
 // Samples [frame0, frame0 + spp) of every pixel, summed into px in sample
 // order (util.wgsl:17-34, 101-145: sums restart at frame 0).
-template <bool F>
-__global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x,
+// WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
+// 3 and 4 trade a few spilled words for more waves in flight). The tracer's
+// autotune times the variants on the caller's workload and keeps the fastest.
+template <bool F, int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x,
                                                         uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                                        unsigned long long* __restrict__ rays) {
+                                                        unsigned long long* __restrict__ rays, uint32_t listed) {
     __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
     const PhrCamera& cam = *camp;
+    const size_t n = (size_t)cam.w * cam.h;
     // A block traces a 16 x 16 tile, a wave 16 x 4 pixels of it: neighbouring
-    // rays walk neighbouring nodes.
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
-    const bool inside = pxl < cam.w && pyl < cam.h;
+    // rays walk neighbouring nodes. With Adaptive Sampling past the first
+    // sample (`listed`), thread k takes the k-th pixel still open instead
+    // (shade.wgsl:47-50); the list holds them tile by tile.
+    uint32_t i = 0;
+    bool inside;
+    if (listed) {
+        const uint32_t k = blockIdx.x * TRACE_THREADS + threadIdx.x;
+        inside = k < min((size_t)px[34 * n], n);
+        if (inside) i = px[32 * n + k];
+        inside = inside && i < n;
+    } else {
+        const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+        const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
+        inside = pxl < cam.w && pyl < cam.h;
+        i = pyl * cam.w + pxl;
+    }
+    const bool adapt = cam.adapt[3] != 0.0f;
     Ctx c;
     c.s = sc;
     c.p = camp;
@@ -847,8 +866,6 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
     c.steps = 0;
     c.tests = 0;
     if (inside) {
-        const uint32_t i = pyl * cam.w + pxl;
-        const size_t n = (size_t)cam.w * cam.h;
         bool keep = frame0 > 0u;
         V3 l = keep ? v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2)) : v3(0, 0, 0);
         V3 ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
@@ -859,9 +876,14 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
         float film = keep ? ldf(px, 24 * n + i) : 0.f;
         uint32_t taken = keep ? px[31 * n + i] : 0u;
         float gone = keep ? ldf(px, 33 * n + i) : 0.f;
+        // The half image (util.wgsl:126-132): odd samples at double weight; a
+        // new image starts it at zero.
+        V3 ax = (adapt && keep) ? v3(ldf(px, 26 * n + 3 * i), ldf(px, 26 * n + 3 * i + 1), ldf(px, 26 * n + 3 * i + 2))
+                                : v3(0, 0, 0);
         uint32_t outline = 0;
         for (uint32_t s = 0; s < spp; ++s) {
             Sample sm = shade<F>(c, i, frame0 + s);
+            if (adapt && ((frame0 + s) & 1u) == 1u) ax = ax + sm.lin * 2.0f;
             ga = ga + sm.albedo;
             gn = gn + sm.normal;
             gd = gd + sm.depth;
@@ -884,6 +906,7 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
         px[25 * n + i] = outline;
         px[31 * n + i] = taken;
         stf(px, 33 * n + i, gone);
+        if (adapt) { stf(px, 26 * n + 3 * i, ax.x); stf(px, 26 * n + 3 * i + 1, ax.y); stf(px, 26 * n + 3 * i + 2, ax.z); }
     }
     // Rays traced, summed over the wave before one atomic.
     unsigned long long rc = c.closest, rs = c.shadow, st = c.steps, tt = c.tests;
@@ -901,6 +924,64 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_path(SceneDev sc, const PhrCa
     }
 }
 
+
+// ---- Adaptive Sampling's tail (src/trace/wgsl/adapt.wgsl, gather.wgsl) ----
+// Stage 0, Cycles' convergence check at a step boundary: the full and half
+// images' disagreement over the pixel's own count, normalised by intensity
+// (its square root below 1); under the threshold the pixel settles.
+__global__ __launch_bounds__(256) void k_adapt_check(uint32_t* __restrict__ px, uint32_t n, float threshold) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = n;
+    if (px[29 * N + i] != 0u) return;
+    const size_t a = N + 3 * (size_t)i, x = 26 * N + 3 * (size_t)i;
+    const uint32_t taken = max(px[31 * N + i], 1u);
+    const float scale = 1.0f / (float)taken;
+    const float f0 = __uint_as_float(px[a]), f1 = __uint_as_float(px[a + 1]), f2 = __uint_as_float(px[a + 2]);
+    const float d0 = fabsf(f0 - __uint_as_float(px[x])), d1 = fabsf(f1 - __uint_as_float(px[x + 1])),
+                d2 = fabsf(f2 - __uint_as_float(px[x + 2]));
+    const float err_diff = ((d0 + d1) + d2) * scale;
+    const float intensity = ((f0 + f1) + f2) * scale;
+    const float norm = intensity < 1.0f ? sqrtf(intensity) : intensity;
+    if (err_diff / (0.0001f + norm) < threshold) px[29 * N + i] = 1u;
+}
+
+// Stage 1: settled only where the pixel and its four neighbours are.
+__global__ __launch_bounds__(256) void k_adapt_dilate(uint32_t* __restrict__ px, uint32_t w, uint32_t h) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const size_t N = (size_t)w * h;
+    if (i >= N) return;
+    const uint32_t x = i % w, y = i / w;
+    const uint32_t* f = px + 29 * N;
+    uint32_t all = f[i];
+    if (x > 0u) all = min(all, f[i - 1]);
+    if (x + 1u < w) all = min(all, f[i + 1]);
+    if (y > 0u) all = min(all, f[i - w]);
+    if (y + 1u < h) all = min(all, f[i + w]);
+    px[30 * N + i] = all;
+}
+
+// Stages 2 and 3: the list of pixels still open for the next launch, one run
+// of the list a 16 x 16 tile, reserved with one atomic a block.
+__global__ __launch_bounds__(256) void k_gather_clear(uint32_t* __restrict__ px, size_t n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) px[34 * n] = 0u;
+}
+__global__ __launch_bounds__(256) void k_gather(uint32_t* __restrict__ px, uint32_t w, uint32_t h, uint32_t tiles_x) {
+    __shared__ uint32_t held, run;
+    const size_t N = (size_t)w * h;
+    if (threadIdx.x == 0) held = 0u;
+    __syncthreads();
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t x = tx * 16 + (threadIdx.x & 15u), y = ty * 16 + (threadIdx.x >> 4);
+    const uint32_t i = y * w + x;
+    const bool open = x < w && y < h && px[30 * N + i] == 0u;
+    uint32_t at = 0;
+    if (open) at = atomicAdd(&held, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) run = atomicAdd(&px[34 * N], held);
+    __syncthreads();
+    if (open) px[32 * N + run + at] = i;
+}
 
 // Each mesh tree's root box (object space), for the instances' world boxes.
 __global__ __launch_bounds__(256) void k_roots(const float4* __restrict__ nodes, const uint32_t* __restrict__ bases, uint32_t n,
@@ -955,6 +1036,7 @@ struct phr_tracer {
     std::vector<uint32_t> corner_vertex;  // input corner (3f + k) -> vertex id
     std::vector<uint32_t> order;          // slot -> input face
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
+    uint32_t path_waves = 2;  // which k_path variant runs (2, 3 or 4 waves a SIMD)
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -1507,6 +1589,10 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
               "phr_tracer_sample: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only phiray's own pixel filter (0)");
+    if (cam.adapt[3] != 0.0f) {
+        const uint32_t st = (uint32_t)cam.adapt[2];
+        PHR_CHECK((st & (st - 1u)) == 0u, PHR_ERR_ARG, "phr_tracer_sample: the adaptive step must be a power of two");
+    }
     cam.w = w;
     cam.h = h;
     if (cam.ao <= 0.f) cam.ao = t->extent * 0.08f;
@@ -1532,12 +1618,44 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->last, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
     PHR_HIP(hipEventRecord(t->ev0, s));
     const uint32_t tiles_x = (w + 15) / 16, tiles = tiles_x * ((h + 15) / 16);
-    if (t->forest)
-        hipLaunchKernelGGL(k_path<true>, dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, tiles_x, frame0,
-                           spp, (uint32_t*)t->px->ptr, t->rays);
-    else
-        hipLaunchKernelGGL(k_path<false>, dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, (const PhrCamera*)t->cam_dev, tiles_x, frame0,
-                           spp, (uint32_t*)t->px->ptr, t->rays);
+    const PhrCamera* cp = (const PhrCamera*)t->cam_dev;
+    uint32_t* pxp = (uint32_t*)t->px->ptr;
+    auto path = [&](uint32_t f0, uint32_t count, uint32_t listed) {
+        if (t->forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
+            hipLaunchKernelGGL((k_path<true, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp, t->rays,
+                               listed);
+        else if (t->path_waves == 4)
+            hipLaunchKernelGGL((k_path<false, 4>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                               t->rays, listed);
+        else if (t->path_waves == 3)
+            hipLaunchKernelGGL((k_path<false, 3>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                               t->rays, listed);
+        else
+            hipLaunchKernelGGL((k_path<false, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                               t->rays, listed);
+    };
+    if (cam.adapt[3] == 0.0f) {
+        path(frame0, spp, 0u);
+    } else {
+        // Adaptive Sampling (sample.rs `BTail::Adapt`): the tail runs after
+        // every sample, and only a step boundary past the minimum changes a
+        // flag, so samples run in launches that end at those boundaries. The
+        // dilation and the list follow each launch.
+        const uint32_t min_s = (uint32_t)cam.adapt[1], step = std::max<uint32_t>((uint32_t)cam.adapt[2], 1u);
+        const uint32_t ng = (uint32_t)((n + 255) / 256);
+        for (uint32_t f = frame0; f < frame0 + spp;) {
+            uint32_t b = std::max(f, min_s + 1u);  // the next checked sample: past the minimum, last of a step
+            b = b | (step - 1u);
+            const uint32_t e = std::min(frame0 + spp, b + 1u);
+            path(f, e - f, f > 0u ? 1u : 0u);
+            if (e == b + 1u)
+                hipLaunchKernelGGL(k_adapt_check, dim3(ng), dim3(256), 0, s, pxp, (uint32_t)n, cam.adapt[0]);
+            hipLaunchKernelGGL(k_adapt_dilate, dim3(ng), dim3(256), 0, s, pxp, w, h);
+            hipLaunchKernelGGL(k_gather_clear, dim3(1), dim3(256), 0, s, pxp, (size_t)n);
+            hipLaunchKernelGGL(k_gather, dim3(tiles), dim3(256), 0, s, pxp, w, h, tiles_x);
+            f = e;
+        }
+    }
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(t->ev1, s));
     t->last_submit_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1549,6 +1667,53 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     if (frame) *frame = t->frame;
     if (done) *done = (t->frame + 1 >= t->samples) ? 1 : 0;
     return PHR_OK;
+}
+
+// The path kernel variant: register allocation fitted to 2, 3 or 4 waves a
+// SIMD (no counterpart in phiray, whose WGSL pipeline has one).
+int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves) {
+    PHR_CHECK(t, PHR_ERR_ARG, "phr_tracer_set_path_waves: null");
+    PHR_CHECK(waves >= 2 && waves <= 4, PHR_ERR_ARG, "phr_tracer_set_path_waves: waves must be 2, 3 or 4");
+    t->path_waves = waves;
+    return PHR_OK;
+}
+uint32_t phr_tracer_path_waves(const phr_tracer* t) { return t ? t->path_waves : 0; }
+
+// Time every path kernel variant on the caller's workload (`spp` samples of
+// w x h under `cam`, best of `reps` launches each) and keep the fastest. The
+// image starts over; the variants compute the same sums.
+int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,
+                        uint32_t* chosen, double* best_s) {
+    PHR_CHECK(t && cam, PHR_ERR_ARG, "phr_tracer_autotune: null");
+    reps = std::max<uint32_t>(reps, 1);
+    uint32_t best_w = 2;
+    double best = 1e30;
+    for (uint32_t waves = 2; waves <= (t->forest ? 2u : 4u); ++waves) {
+        t->path_waves = waves;
+        double mine = 1e30;
+        for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
+            t->have_last = false;  // start the image over every launch
+            int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
+            if (rc != PHR_OK) return rc;
+            if (r > 0) mine = std::min(mine, t->last_gpu_s);
+        }
+        if (mine < best) best = mine, best_w = waves;
+    }
+    t->path_waves = best_w;
+    t->have_last = false;
+    if (chosen) *chosen = best_w;
+    if (best_s) *best_s = best;
+    return PHR_OK;
+}
+
+// Pixels Adaptive Sampling has not settled (Tracer::pixels_left,
+// src/trace/sample.rs): the list's length. PHR_ERR_UNSUPPORTED when the last
+// camera had it off.
+int phr_tracer_open_pixels(phr_tracer* t, uint32_t* count) {
+    PHR_CHECK(t && count, PHR_ERR_ARG, "phr_tracer_open_pixels: null");
+    PHR_CHECK(t->px && t->have_last && t->last.adapt[3] != 0.0f, PHR_ERR_UNSUPPORTED,
+              "phr_tracer_open_pixels: adaptive sampling is off");
+    return phr_array_read_range(t->px, 34ull * t->w * t->h, 1, count);
 }
 
 // Tracer::time (src/trace/sample.rs:253-255): submit and GPU seconds of the last sample call.

@@ -87,10 +87,14 @@ def crop_camera(w: int, h: int, x0: int, y0: int, cw: int, ch: int, **kw) -> np.
 
 
 def train(gpu: Gpu, frames: GpuArray, targets: GpuArray, n_scenes: int, variants: int, crop: int, steps: int,
-          batch: int = 8, lr: float = 1e-3, seed: int = 0, params: np.ndarray | None = None, log=_log,
-          log_every: int = 200, budget_s: float | None = None) -> tuple[Kpcn, list[tuple[int, float]]]:
-    """Adam at `lr` over random batches of (scene, render) images; image
-    s * variants + j is render j of scene s and is scored against target s."""
+          batch: int = 8, lr: float = 1e-3, lr_final: float | None = None, seed: int = 0,
+          params: np.ndarray | None = None, log=_log, log_every: int = 200,
+          budget_s: float | None = None) -> tuple[Kpcn, list[tuple[int, float]]]:
+    """Adam over random batches of (scene, render) images; image
+    s * variants + j is render j of scene s and is scored against target s.
+    The rate falls from `lr` to `lr_final` on a cosine over the steps (or
+    over `budget_s` seconds, whichever ends first)."""
+    lo = lr if lr_final is None else lr_final
     net = Kpcn(gpu, init_params(seed) if params is None else params)
     rng = np.random.default_rng(seed + 1)
     hist, t0 = [], time.perf_counter()
@@ -98,7 +102,11 @@ def train(gpu: Gpu, frames: GpuArray, targets: GpuArray, n_scenes: int, variants
         scenes = rng.integers(0, n_scenes, size=batch)
         var = rng.integers(0, variants, size=batch)
         want = (step + 1) % log_every == 0 or step == steps - 1
-        loss = net.train_indexed(frames, scenes * variants + var, targets, scenes, crop, crop, lr, want_loss=want)
+        frac = step / max(steps - 1, 1)
+        if budget_s is not None:
+            frac = max(frac, min(1.0, (time.perf_counter() - t0) / budget_s))
+        rate = lo + 0.5 * (lr - lo) * (1.0 + np.cos(np.pi * frac))
+        loss = net.train_indexed(frames, scenes * variants + var, targets, scenes, crop, crop, rate, want_loss=want)
         if want:
             hist.append((step + 1, loss))
             el = time.perf_counter() - t0
@@ -130,6 +138,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-4)
     ap.add_argument("--budget-s", type=float, default=None, help="stop training after this many seconds")
     ap.add_argument("--out", type=str, default=str(WEIGHTS))
     ap.add_argument("--log", type=str, default=None, help="JSON file for the loss history")
@@ -143,7 +152,7 @@ def main(argv=None) -> int:
     frames, targets, meta = render_multi(gpu, seeds, args.crop, spps, args.spp_ref, args.variants)
     _log(f"[kpcn] training set rendered in {time.perf_counter() - t0:.0f} s")
     net, hist = train(gpu, frames, targets, len(seeds), len(spps) * args.variants, args.crop, args.steps,
-                      batch=args.batch, lr=args.lr, budget_s=args.budget_s)
+                      batch=args.batch, lr=args.lr, lr_final=args.lr_final, budget_s=args.budget_s)
     save(args.out, net.params())
     _log(f"[kpcn] weights written to {args.out}")
     if args.log:

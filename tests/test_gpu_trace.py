@@ -105,3 +105,68 @@ def test_samples_accumulate_across_calls(gpu, orc):
     cam2["dist"] = cam2["dist"] * np.float32(1.1)
     frame, _ = tr.sample(cam2, w, h, 1)
     assert frame == 0
+
+
+def test_path_kernel_variants_agree(gpu, orc):
+    """The 2-, 3- and 4-waves-a-SIMD builds of the path kernel differ only in
+    register allocation: every plane equals the oracle's, for each."""
+    print("[trace] path variants", flush=True)
+    tris = S.spheres_scene(spheres=6, level=2, seed=9)
+    w, h = 48, 32
+    cam = S.path_camera(w, h, rr_from=3)
+    sc = orc.Scene(tris, 4, P.NODE)
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    for f in range(2):
+        sc.sample(cam, w, h, px, f, nthreads=8)
+    tr = P.Tracer.new(gpu, tris, 4)
+    assert tr.path_waves == 2
+    for waves in (2, 3, 4):
+        tr.set_path_waves(waves)
+        fresh = cam.copy()
+        fresh["stream"] = 0
+        tr.sample(fresh, w, h, 2)
+        compare_planes(tr.px(), px, w, h)
+        tr.sample(S.path_camera(w + 1, h, rr_from=3), w + 1, h, 1)  # another view: the next call starts over
+    with pytest.raises(P.PhrError):
+        tr.set_path_waves(5)
+    chosen, best = tr.autotune(cam, w, h, 1, reps=1)
+    assert chosen in (2, 3, 4) and best > 0 and tr.path_waves == chosen
+
+
+def test_adaptive_sampling_matches_the_twin(gpu, orc):
+    """Adaptive Sampling (adapt.wgsl, gather.wgsl): settled pixels stop, the
+    half image, the flags before and after the dilation, the per-pixel counts
+    and every integrator plane equal the oracle's after each call; the list
+    holds exactly the open pixels."""
+    print("[trace] adaptive sampling", flush=True)
+    tris = S.spheres_scene(spheres=8, level=2, seed=3)
+    w, h = 64, 48
+    n = w * h
+    cam = S.path_camera(w, h, rr_from=3)
+    cam["adapt"] = (0.08, 3.0, 2.0, 1.0)  # threshold, min samples, step, on
+    sc = orc.Scene(tris, 4, P.NODE)
+    tr = P.Tracer.new(gpu, tris, 4)
+    px = np.zeros(P.PX_LEN * n + 1, dtype=np.uint32)
+    f = 0
+    for spp in (5, 2, 9):
+        for _ in range(spp):
+            sc.sample(cam, w, h, px, f, nthreads=8)
+            sc.adapt_tail(cam, w, h, px, f)
+            f += 1
+        frame, _ = tr.sample(cam, w, h, spp)
+        assert frame == f - 1
+        got = tr.px()
+        compare_planes(got, px, w, h)
+        assert np.array_equal(got[26 * n:29 * n], px[26 * n:29 * n]), "half image"
+        assert np.array_equal(got[29 * n:31 * n], px[29 * n:31 * n]), "flags"
+        open_ = np.flatnonzero(px[30 * n:31 * n] == 0)
+        assert tr.open_pixels() == len(open_) == got[34 * n]
+        assert np.array_equal(np.sort(got[32 * n:32 * n + len(open_)]), open_)
+    settled = int((px[30 * n:31 * n] != 0).sum())
+    assert 0 < settled < n, settled  # the case exercises both kinds
+    counts = px[31 * n:32 * n]
+    assert counts.min() < f and counts.max() == f
+    with pytest.raises(P.PhrError):
+        bad = cam.copy()
+        bad["adapt"] = (0.08, 3.0, 3.0, 1.0)
+        tr.sample(bad, w, h, 1)
