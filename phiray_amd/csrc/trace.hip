@@ -1000,6 +1000,78 @@ __global__ __launch_bounds__(256) void k_inst_order(const uint32_t* __restrict__
     if (k < p) out[k] = placed[tlas_order[k]];
 }
 
+// Placed instance j's world box: its mesh tree's root box, eight corners
+// placed (instance.rs:116-124), read from the node array where the refit left it.
+__global__ __launch_bounds__(256) void k_world_boxes(const float* __restrict__ worlds, const uint32_t* __restrict__ placed,
+                                                     const uint32_t* __restrict__ inst_mesh, const uint32_t* __restrict__ bases,
+                                                     const float4* __restrict__ nodes, uint32_t p, float4* __restrict__ wb) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= p) return;
+    const uint32_t i = placed[j], r = bases[inst_mesh[i]];
+    const float4 a = nodes[2 * (size_t)r], b = nodes[2 * (size_t)r + 1];
+    const float* w = worlds + 16 * (size_t)i;
+    float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f}, hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (int k = 0; k < 8; ++k) {
+        const float c[3] = {(k & 1) ? b.x : a.x, (k & 2) ? b.y : a.y, (k & 4) ? b.z : a.z};
+        for (int q = 0; q < 3; ++q) {
+            const float v = ((w[q] * c[0] + w[4 + q] * c[1]) + w[8 + q] * c[2]) + w[12 + q];
+            lo[q] = fmin_(lo[q], v);
+            hi[q] = fmax_(hi[q], v);
+        }
+    }
+    wb[2 * (size_t)j] = make_float4(lo[0], lo[1], lo[2], 0.f);
+    wb[2 * (size_t)j + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+}
+
+// Each candidate emitter's power: its world triangle's area times its glow
+// (lamp.rs:51-97 over (face, instance)).
+__global__ __launch_bounds__(256) void k_light_power(const uint32_t* __restrict__ cand, uint32_t ncand,
+                                                     const uint4* __restrict__ faces, const float* __restrict__ verts,
+                                                     const float* __restrict__ worlds, float* __restrict__ pw) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncand) return;
+    const uint4 fc = faces[cand[3 * c]];
+    const float* w = worlds + 16 * (size_t)cand[3 * c + 1];
+    const float glow = __uint_as_float(cand[3 * c + 2]);
+    float v[3][3];
+    const uint32_t ids[3] = {fc.x, fc.y, fc.z};
+    for (int k = 0; k < 3; ++k) {
+        const float* p = verts + 8 * (size_t)ids[k];
+        for (int q = 0; q < 3; ++q) v[k][q] = ((w[q] * p[0] + w[4 + q] * p[1]) + w[8 + q] * p[2]) + w[12 + q];
+    }
+    const float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    const float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    float sq = -0.0f;
+    for (int k = 0; k < 3; ++k) sq = sq + cr[k] * cr[k];
+    pw[c] = (0.5f * sqrtf(sq)) * glow;
+}
+
+// The emitter table from the powers, in candidate order: the lit ones, then
+// the power CDF. One thread, so the sums run in the host's order.
+__global__ void k_light_table(const uint32_t* __restrict__ cand, const float* __restrict__ pw, uint32_t ncand,
+                              uint32_t* __restrict__ words) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t cnt = 0;
+    float total = -0.0f;
+    for (uint32_t c = 0; c < ncand; ++c)
+        if (pw[c] > 0.f) {
+            ++cnt;
+            total = total + pw[c];
+        }
+    words[0] = cnt;
+    uint32_t k = 0;
+    float run = 0.0f;
+    for (uint32_t c = 0; c < ncand; ++c) {
+        if (!(pw[c] > 0.f)) continue;
+        words[1 + k] = cand[3 * c];
+        words[1 + cnt + k] = cand[3 * c + 1];
+        run = run + pw[c];
+        words[1 + 2 * cnt + k] = __float_as_uint(run / total);
+        ++k;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1061,9 +1133,20 @@ struct phr_tracer {
     std::vector<uint32_t> glow_meshes;  // meshes holding an emissive face
     uint32_t* bases_dev = nullptr;      // each mesh tree's root node, or NONE
     std::vector<uint32_t> placed, node_base;
-    std::vector<PhrAabb> roots;  // each mesh tree's root box, object space
     float* frame_dev = nullptr;  // the learned denoiser's input and output, sized for w x h
     uint64_t frame_px = 0;
+    // A new pose is applied on the device without waiting: the instances'
+    // matrices, each instance's mesh, the emitter candidates (face slot,
+    // instance, glow) and their powers live there; the extent and emitter
+    // count the camera needs are read back at the next sample (`stale`).
+    phr_array* worlds_dev = nullptr;  // 16 floats an instance
+    phr_array* inst_mesh = nullptr;   // mesh id an instance
+    phr_array* lcand = nullptr;       // 3 words a candidate emitter (face slot, instance, glow bits)
+    phr_array* lpow = nullptr;        // its power
+    uint32_t ncand = 0;
+    bool stale = false;
+    std::vector<uint32_t> words_h;    // instance words staged for upload
+    std::vector<float> inst_worlds_h; // matrices staged for upload
 };
 
 namespace {
@@ -1096,7 +1179,7 @@ void tracer_release(phr_tracer* t) {
     if (t->bvh) phr_bvh_free(t->bvh);
     if (t->tlas) phr_bvh_free(t->tlas);
     for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->lights, t->px, t->nodes, t->wboxes, t->insts,
-                         t->placed_dev})
+                         t->placed_dev, t->worlds_dev, t->inst_mesh, t->lcand, t->lpow})
         phr_array_release(a);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->frame_dev) (void)hipFree(t->frame_dev);
@@ -1218,24 +1301,6 @@ bool inst_identity(const float w[16]) {
     return true;
 }
 
-// The box's eight corners placed (instance.rs:116-124).
-PhrAabb inst_world_box(const float w[16], const PhrAabb& b) {
-    PhrAabb out;
-    for (int j = 0; j < 3; ++j) { out.lo[j] = 3.40282347e38f; out.hi[j] = -3.40282347e38f; }
-    for (int k = 0; k < 8; ++k) {
-        const float c[3] = {(k & 1) ? b.hi[0] : b.lo[0], (k & 2) ? b.hi[1] : b.lo[1], (k & 4) ? b.hi[2] : b.lo[2]};
-        float q[3];
-        inst_to_world(w, c, q);
-        for (int j = 0; j < 3; ++j) {
-            out.lo[j] = fmin_(out.lo[j], q[j]);
-            out.hi[j] = fmax_(out.hi[j], q[j]);
-        }
-    }
-    out._a = 0.f;
-    out._b = 0.f;
-    return out;
-}
-
 // Instance::words (instance.rs:126-160).
 void inst_words(const phr_instance& in, uint32_t root, const phr_mesh& mesh, uint32_t* out) {
     for (uint32_t k = 0; k < INSTANCE_LEN; ++k) out[k] = 0u;
@@ -1263,7 +1328,16 @@ void inst_words(const phr_instance& in, uint32_t root, const phr_mesh& mesh, uin
 
 // lamp.rs:51-97 over the instances: each emissive face under each instance of
 // its mesh, by placed area times luminance, sorted by face then instance.
-int forest_lights(phr_tracer* t) {
+// The emitter table on the host (src/trace/lamp.rs:51-97 over (face, instance)),
+// for scenes with more candidate emitters than the one-thread table takes.
+int forest_lights_host(phr_tracer* t) {
+    for (uint32_t m : t->glow_meshes) {  // the emissive meshes' current object-space vertices
+        const phr_mesh& me = t->meshes[m];
+        if (me.vert_hi > me.vert_lo && me.vert_hi <= t->nverts)
+            PHR_HIP(hipMemcpyAsync(&t->pos[8 * (size_t)me.vert_lo], (const float*)t->verts->ptr + 8 * (size_t)me.vert_lo,
+                                   32ull * (me.vert_hi - me.vert_lo), hipMemcpyDeviceToHost, t->gpu->stream));
+    }
+    PHR_HIP(hipStreamSynchronize(t->gpu->stream));
     std::vector<std::vector<uint32_t>> by_mesh(t->nmeshes);
     for (uint32_t i = 0; i < t->ninsts; ++i)
         if (t->inst_in[i].mesh < t->nmeshes) by_mesh[t->inst_in[i].mesh].push_back(i);
@@ -1319,41 +1393,108 @@ int forest_lights(phr_tracer* t) {
     return PHR_OK;
 }
 
-// Each mesh tree's root box, read back once per new set of trees or bounds.
-int forest_roots(phr_tracer* t) {
-    float4* out = nullptr;
-    PHR_HIP(hipMalloc(&out, 32ull * std::max<uint32_t>(t->nmeshes, 1)));
-    hipLaunchKernelGGL(k_roots, dim3((t->nmeshes + 255) / 256), dim3(256), 0, t->gpu->stream, (const float4*)t->nodes->ptr,
-                       t->bases_dev, t->nmeshes, out);
-    t->roots.assign(t->nmeshes, PhrAabb{});
-    hipError_t e = hipMemcpyAsync(t->roots.data(), out, 32ull * t->nmeshes, hipMemcpyDeviceToHost, t->gpu->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(t->gpu->stream);
-    (void)hipFree(out);
-    if (e != hipSuccess) { phr_set_error("forest roots", e); return PHR_ERR_HIP; }
+constexpr uint32_t LIGHT_DEVICE_MAX = 4096;  // candidates the one-thread table kernel takes
+
+// The candidate emitters in the table's order: meshes in order, each emissive
+// face in slot order, each instance of its mesh in instance order. Built once
+// per face order (a new tree order changes the slots).
+int forest_light_cands(phr_tracer* t) {
+    std::vector<std::vector<uint32_t>> by_mesh(t->nmeshes);
+    for (uint32_t i = 0; i < t->ninsts; ++i)
+        if (t->inst_in[i].mesh < t->nmeshes) by_mesh[t->inst_in[i].mesh].push_back(i);
+    std::vector<uint32_t> cand;
+    for (uint32_t m : t->glow_meshes)
+        for (uint32_t f = t->meshes[m].face_lo; f < t->meshes[m].face_hi; ++f) {
+            const float* e = &t->mats_h[MAT_WORDS * (size_t)t->faces_slot[4 * (size_t)f + 3] + 8];
+            const float glow = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+            if (glow <= 0.f) continue;
+            for (uint32_t i : by_mesh[m]) {
+                cand.push_back(f);
+                cand.push_back(i);
+                cand.push_back(fbits(glow));
+            }
+        }
+    t->ncand = (uint32_t)(cand.size() / 3);
+    if (t->ncand == 0 || t->ncand > LIGHT_DEVICE_MAX) return PHR_OK;
+    int rc;
+    if (!t->lcand || t->lcand->cap < cand.size() || t->lpow->cap < t->ncand) {
+        phr_array_release(t->lcand);
+        phr_array_release(t->lpow);
+        t->lcand = t->lpow = nullptr;
+        if ((rc = phr_array_alloc(t->gpu, 4, cand.size(), &t->lcand)) != PHR_OK) return rc;
+        if ((rc = phr_array_alloc(t->gpu, 4, t->ncand, &t->lpow)) != PHR_OK) return rc;
+    }
+    PHR_HIP(hipMemcpyAsync(t->lcand->ptr, cand.data(), 4 * cand.size(), hipMemcpyHostToDevice, t->gpu->stream));
+    PHR_HIP(hipStreamSynchronize(t->gpu->stream));
+    return PHR_OK;
+}
+
+// The emitter table for the current pose: on the device, without waiting,
+// unless the scene has more candidates than the table kernel takes.
+int forest_lights(phr_tracer* t) {
+    if (t->ncand > LIGHT_DEVICE_MAX) return forest_lights_host(t);
+    const size_t words = 1 + 3 * (size_t)t->ncand;
+    if (!t->lights || t->lights->cap < words) {
+        phr_array_release(t->lights);
+        t->lights = nullptr;
+        int rc = phr_array_alloc(t->gpu, 4, std::max<size_t>(words, 4), &t->lights);
+        if (rc != PHR_OK) return rc;
+    }
+    hipStream_t s = t->gpu->stream;
+    if (t->ncand == 0) {
+        PHR_HIP(hipMemsetAsync(t->lights->ptr, 0, 4, s));
+        t->lights->len = 1;
+        t->glow = 0;
+        return PHR_OK;
+    }
+    hipLaunchKernelGGL(k_light_power, dim3((t->ncand + 255) / 256), dim3(256), 0, s, (const uint32_t*)t->lcand->ptr, t->ncand,
+                       (const uint4*)t->faces->ptr, (const float*)t->verts->ptr, (const float*)t->worlds_dev->ptr,
+                       (float*)t->lpow->ptr);
+    hipLaunchKernelGGL(k_light_table, dim3(1), dim3(64), 0, s, (const uint32_t*)t->lcand->ptr, (const float*)t->lpow->ptr,
+                       t->ncand, (uint32_t*)t->lights->ptr);
+    PHR_HIP(hipGetLastError());
+    t->stale = true;  // the count (and the table's length) is read back at the next sample
+    return PHR_OK;
+}
+
+// What the host keeps of a pose applied on the device: the extent (the top
+// tree's root box, forest.rs:189-199) and the emitter count.
+int forest_settle(phr_tracer* t) {
+    if (!t->forest || !t->stale) return PHR_OK;
+    float box[8];
+    uint32_t cnt = 0;
+    hipStream_t s = t->gpu->stream;
+    PHR_HIP(hipMemcpyAsync(box, t->nodes->ptr, 32, hipMemcpyDeviceToHost, s));
+    PHR_HIP(hipMemcpyAsync(&cnt, t->lights->ptr, 4, hipMemcpyDeviceToHost, s));
+    PHR_HIP(hipStreamSynchronize(s));
+    float sq = -0.0f;
+    for (int k = 0; k < 3; ++k) sq = sq + (box[4 + k] - box[k]) * (box[4 + k] - box[k]);
+    t->extent = std::max(std::sqrt(sq), 1e-3f);
+    if (t->ncand <= LIGHT_DEVICE_MAX) {
+        t->glow = cnt;
+        t->lights->len = 1 + 3 * (size_t)cnt;
+    }
+    t->stale = false;
     return PHR_OK;
 }
 
 // Forest::retlas + inst_words (src/trace/forest.rs:111-127, 168-181): the
-// placed instances' world boxes on the host, the top tree over them on the
-// device (into node slots [0, 2p - 1), leaves flagged INST), the instance
-// table, and the extent (the top tree's root box's diagonal, forest.rs:189-199).
+// placed instances' world boxes from their mesh trees' root boxes and the
+// top tree over them, both on the device (the top tree into node slots
+// [0, 2p - 1), leaves flagged INST), and the instance table. Nothing waits:
+// the extent (the top tree's root box, forest.rs:189-199) is read back at the
+// next sample.
 int forest_top(phr_tracer* t, bool fresh) {
-    const uint32_t p = (uint32_t)t->placed.size();
-    std::vector<PhrAabb> wb(p);
-    float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f}, hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-    for (uint32_t j = 0; j < p; ++j) {
-        const phr_instance& in = t->inst_in[t->placed[j]];
-        wb[j] = inst_world_box(in.world, t->roots[in.mesh]);
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin_(lo[k], wb[j].lo[k]);
-            hi[k] = fmax_(hi[k], wb[j].hi[k]);
-        }
-    }
-    float sq = -0.0f;
-    for (int k = 0; k < 3; ++k) sq = sq + (hi[k] - lo[k]) * (hi[k] - lo[k]);
-    t->extent = std::max(std::sqrt(sq), 1e-3f);
+    const uint32_t p = (uint32_t)t->placed.size(), n = t->ninsts;
     hipStream_t s = t->gpu->stream;
-    PHR_HIP(hipMemcpyAsync(t->wboxes->ptr, wb.data(), 32ull * p, hipMemcpyHostToDevice, s));
+    std::vector<float>& wl = t->inst_worlds_h;
+    wl.resize(16 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) std::memcpy(&wl[16 * (size_t)i], t->inst_in[i].world, 64);
+    PHR_HIP(hipMemcpyAsync(t->worlds_dev->ptr, wl.data(), 64ull * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_world_boxes, dim3((p + 255) / 256), dim3(256), 0, s, (const float*)t->worlds_dev->ptr,
+                       (const uint32_t*)t->placed_dev->ptr, (const uint32_t*)t->inst_mesh->ptr, t->bases_dev,
+                       (const float4*)t->nodes->ptr, p, (float4*)t->wboxes->ptr);
+    PHR_HIP(hipGetLastError());
     int rc;
     if (fresh) {
         const uint32_t seg[2] = {0, p}, base[1] = {0};
@@ -1362,8 +1503,8 @@ int forest_top(phr_tracer* t, bool fresh) {
     } else if ((rc = phr_bvh_internal_rebuild_async(t->tlas)) != PHR_OK) {
         return rc;
     }
-    const uint32_t n = t->ninsts;
-    std::vector<uint32_t> words((size_t)(INSTANCE_LEN + 2) * n, 0u);
+    std::vector<uint32_t>& words = t->words_h;
+    words.assign((size_t)(INSTANCE_LEN + 2) * n, 0u);
     phr_mesh unit{0, 0, 0, 0, {0.f, 0.f, 0.f}, {1.f, 1.f, 1.f}};
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t m = t->inst_in[i].mesh;
@@ -1382,7 +1523,7 @@ int forest_top(phr_tracer* t, bool fresh) {
     hipLaunchKernelGGL(k_inst_order, dim3((p + 255) / 256), dim3(256), 0, s, (const uint32_t*)t->placed_dev->ptr,
                        (const uint32_t*)phr_bvh_internal_order(t->tlas)->ptr, p, (uint32_t*)t->insts->ptr + INSTANCE_LEN * (size_t)n);
     PHR_HIP(hipGetLastError());
-    PHR_HIP(hipStreamSynchronize(s));
+    t->stale = true;
     return PHR_OK;
 }
 
@@ -1518,8 +1659,16 @@ int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, ph
 void phr_tracer_free(phr_tracer* t) { tracer_release(t); }
 uint32_t phr_tracer_len(const phr_tracer* t) { return t ? t->n : 0; }
 uint32_t phr_tracer_depth(const phr_tracer* t) { return t ? phr_bvh_depth(t->bvh) : 0; }
-float phr_tracer_extent(const phr_tracer* t) { return t ? t->extent : 0.f; }
-uint32_t phr_tracer_glow(const phr_tracer* t) { return t ? t->glow : 0; }
+float phr_tracer_extent(const phr_tracer* t) {
+    if (!t) return 0.f;
+    (void)forest_settle(const_cast<phr_tracer*>(t));  // a pose applied on the device is read back here
+    return t->extent;
+}
+uint32_t phr_tracer_glow(const phr_tracer* t) {
+    if (!t) return 0;
+    (void)forest_settle(const_cast<phr_tracer*>(t));
+    return t->glow;
+}
 phr_bvh* phr_tracer_bvh(phr_tracer* t) { return t ? t->bvh : nullptr; }
 
 // The slot -> input face map and the scene arrays, for tests and tooling.
@@ -1530,6 +1679,7 @@ int phr_tracer_order(phr_tracer* t, uint32_t* out) {
 }
 int phr_tracer_lights(phr_tracer* t, uint32_t* out, uint32_t cap, uint32_t* len) {
     PHR_CHECK(t && len, PHR_ERR_ARG, "phr_tracer_lights: null");
+    if (int rc = forest_settle(t); rc != PHR_OK) return rc;
     *len = (uint32_t)t->lights->len;
     if (out) {
         PHR_CHECK(cap >= *len, PHR_ERR_SIZE, "phr_tracer_lights: buffer too small");
@@ -1589,6 +1739,7 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
               "phr_tracer_sample: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only phiray's own pixel filter (0)");
+    if (int rc0 = forest_settle(t); rc0 != PHR_OK) return rc0;
     if (cam.adapt[3] != 0.0f) {
         const uint32_t st = (uint32_t)cam.adapt[2];
         PHR_CHECK((st & (st - 1u)) == 0u, PHR_ERR_ARG, "phr_tracer_sample: the adaptive step must be a power of two");
@@ -1973,6 +2124,10 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
     if ((rc = phr_array_alloc(g, 32, p, &t->wboxes)) != PHR_OK) return fail(rc);
     if ((rc = phr_array_alloc(g, 4, (size_t)(INSTANCE_LEN + 2) * t->ninsts, &t->insts)) != PHR_OK) return fail(rc);
     if ((rc = phr_array_alloc(g, 4, p, &t->placed_dev)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4, 16ull * t->ninsts, &t->worlds_dev)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4, t->ninsts, &t->inst_mesh)) != PHR_OK) return fail(rc);
+    std::vector<uint32_t> mesh_of(t->ninsts);
+    for (uint32_t i = 0; i < t->ninsts; ++i) mesh_of[i] = std::min(t->inst_in[i].mesh, t->nmeshes - 1);  // placed ones are valid
     if (hipMalloc(&t->rays, 48) != hipSuccess || hipMemsetAsync(t->rays, 0, 48, s) != hipSuccess ||
         hipMalloc(&t->cam_dev, sizeof(PhrCamera)) != hipSuccess || hipMalloc(&t->bases_dev, 4ull * t->nmeshes) != hipSuccess) {
         phr_set_error_msg("phr_tracer_from_scene: allocation failed");
@@ -1982,7 +2137,8 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
         hipMemcpyAsync(t->verts->ptr, d->verts, 32ull * t->nverts, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(t->mats->ptr, mts, 64ull * t->nmats, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(t->bases_dev, bases.data(), 4ull * t->nmeshes, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(t->placed_dev->ptr, t->placed.data(), 4ull * p, hipMemcpyHostToDevice, s) != hipSuccess) {
+        hipMemcpyAsync(t->placed_dev->ptr, t->placed.data(), 4ull * p, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(t->inst_mesh->ptr, mesh_of.data(), 4ull * t->ninsts, hipMemcpyHostToDevice, s) != hipSuccess) {
         phr_set_error_msg("phr_tracer_from_scene: upload failed");
         return fail(PHR_ERR_HIP);
     }
@@ -1991,8 +2147,8 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
     if ((rc = phr_bvh_internal_build_into(g, t->prims, leaf, seg_lo.data(), t->nmeshes, t->nodes, t->node_base.data(), PHR_LEAF,
                                           &t->bvh)) != PHR_OK)
         return fail(rc);
-    if ((rc = forest_gather(t)) != PHR_OK || (rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, true)) != PHR_OK ||
-        (rc = forest_lights(t)) != PHR_OK)
+    if ((rc = forest_gather(t)) != PHR_OK || (rc = forest_light_cands(t)) != PHR_OK || (rc = forest_top(t, true)) != PHR_OK ||
+        (rc = forest_lights(t)) != PHR_OK || (rc = forest_settle(t)) != PHR_OK)
         return fail(rc);
     if (hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess) {
         phr_set_error_msg("phr_tracer_from_scene: event creation failed");
@@ -2015,9 +2171,9 @@ int phr_tracer_reinstance(phr_tracer* t, const float* worlds, uint32_t n, int* o
     return PHR_OK;
 }
 
-// The moved vertices are in t->verts: boxes, refit, triangle cache, roots,
-// top tree, lights (the emissive meshes' vertices come back to the host).
-static int repose_device(phr_tracer* t, bool host_pos_current) {
+// The moved vertices are in t->verts: boxes, refit, triangle cache, then the
+// top tree and the emitter table, all queued on the device.
+static int repose_device(phr_tracer* t) {
     hipStream_t s = t->gpu->stream;
     const uint32_t n = t->n;
     hipLaunchKernelGGL(k_tri_boxes, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
@@ -2027,14 +2183,7 @@ static int repose_device(phr_tracer* t, bool host_pos_current) {
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr, (const float4*)t->verts->ptr,
                        (const uint32_t*)phr_bvh_internal_order(t->bvh)->ptr, n, (uint4*)t->faces->ptr, (float4*)t->tris->ptr);
     PHR_HIP(hipGetLastError());
-    if (!host_pos_current)
-        for (uint32_t m : t->glow_meshes) {
-            const phr_mesh& me = t->meshes[m];
-            if (me.vert_hi > me.vert_lo && me.vert_hi <= t->nverts)
-                PHR_HIP(hipMemcpyAsync(&t->pos[8 * (size_t)me.vert_lo], (const float*)t->verts->ptr + 8 * (size_t)me.vert_lo,
-                                       32ull * (me.vert_hi - me.vert_lo), hipMemcpyDeviceToHost, s));
-        }
-    if ((rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK) return rc;
+    if ((rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK) return rc;
     t->have_last = false;  // pose.rs:92-94
     t->frame = 0;
     return PHR_OK;
@@ -2046,7 +2195,7 @@ int phr_tracer_repose(phr_tracer* t, const void* verts, uint32_t nverts, int* ok
     if (!t->forest || nverts != t->nverts) return PHR_OK;
     std::memcpy(t->pos.data(), verts, 32ull * nverts);
     PHR_HIP(hipMemcpyAsync(t->verts->ptr, verts, 32ull * nverts, hipMemcpyHostToDevice, t->gpu->stream));
-    int rc = repose_device(t, true);
+    int rc = repose_device(t);
     if (rc == PHR_OK) *ok = 1;
     return rc;
 }
@@ -2056,7 +2205,7 @@ int phr_tracer_repose_on(phr_tracer* t, phr_array* verts, int* ok) {
     *ok = 0;
     if (!t->forest || verts->len != t->nverts) return PHR_OK;
     PHR_HIP(hipMemcpyAsync(t->verts->ptr, verts->ptr, 32ull * t->nverts, hipMemcpyDeviceToDevice, t->gpu->stream));
-    int rc = repose_device(t, false);
+    int rc = repose_device(t);
     if (rc == PHR_OK) *ok = 1;
     return rc;
 }
@@ -2067,7 +2216,7 @@ int phr_tracer_retree(phr_tracer* t, int* ok) {
     if (!t->forest) return PHR_OK;
     int rc;
     if ((rc = phr_bvh_internal_rebuild_async(t->bvh)) != PHR_OK || (rc = forest_gather(t)) != PHR_OK ||
-        (rc = forest_roots(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK)
+        (rc = forest_light_cands(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK)
         return rc;
     *ok = 1;  // the image keeps its samples (pose.rs:14-16)
     return PHR_OK;
