@@ -200,6 +200,13 @@ void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
 /* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
  * cam is the 192-byte camera record. */
 int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done);
+/* Tracer::sample_in (src/trace/sample.rs:104-249): the same in `slices`
+ * launches over the image's tiles, the device drained after each and go(user)
+ * asked whether to carry on; a zero abandons the call (*finished = 0, the
+ * reference's None) and the next call starts the image over. */
+typedef int (*phr_go_fn)(void* user);
+int phr_tracer_sample_in(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t slices,
+                         phr_go_fn go, void* user, uint32_t* frame, int* done, int* finished);
 /* Tracer::time (src/trace/sample.rs:253-255). */
 int phr_tracer_time(phr_tracer* t, double* submit_s, double* gpu_s);
 /* Rays traced so far: closest-hit and shadow (any-hit). */

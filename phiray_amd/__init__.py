@@ -178,6 +178,8 @@ _SIGS = {
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
+    "phr_tracer_sample_in": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, C.POINTER(_U32), C.POINTER(_I),
+                                  C.POINTER(_I)]),
     "phr_tracer_open_pixels": (_I, [_P, C.POINTER(_U32)]),
     "phr_tracer_path_waves": (_U32, [_P]),
     "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
@@ -736,6 +738,23 @@ class Tracer:
                "Tracer autotune")
         self.size = (max(w, 1), max(h, 1))
         return int(chosen.value), float(best.value)
+
+    _GO = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+    def sample_in(self, cam: np.ndarray, w: int, h: int, spp: int, slices: int, go=None) -> tuple[int, bool] | None:
+        """`Tracer::sample_in`: the launches in `slices` pieces, `go()` asked
+        after each whether to carry on; None when it said no (the next call
+        starts the image over)."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        frame, done, fin = C.c_uint32(0), C.c_int(0), C.c_int(0)
+        cb = self._GO(lambda _u: 1 if go() else 0) if go is not None else None
+        _check(lib().phr_tracer_sample_in(self._h, _ptr(cam), w, h, spp, slices,
+                                          C.cast(cb, C.c_void_p) if cb is not None else None, None,
+                                          C.byref(frame), C.byref(done), C.byref(fin)), "Tracer::sample_in")
+        self.size = (max(w, 1), max(h, 1))
+        if not fin.value:
+            return None
+        return int(frame.value), bool(done.value)
 
     def time(self) -> tuple[float, float]:
         """`Tracer::time`: (submit seconds, GPU seconds) of the last sample call."""

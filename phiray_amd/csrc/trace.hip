@@ -835,7 +835,8 @@ template <bool F, int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x,
                                                         uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                                        unsigned long long* __restrict__ rays, uint32_t listed) {
+                                                        unsigned long long* __restrict__ rays, uint32_t listed,
+                                                        uint32_t tile0) {
     __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
     const PhrCamera& cam = *camp;
     const size_t n = (size_t)cam.w * cam.h;
@@ -845,13 +846,14 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     // (shade.wgsl:47-50); the list holds them tile by tile.
     uint32_t i = 0;
     bool inside;
+    const uint32_t blk = blockIdx.x + tile0;  // a slice of the launch starts at tile0
     if (listed) {
-        const uint32_t k = blockIdx.x * TRACE_THREADS + threadIdx.x;
+        const uint32_t k = blk * TRACE_THREADS + threadIdx.x;
         inside = k < min((size_t)px[34 * n], n);
         if (inside) i = px[32 * n + k];
         inside = inside && i < n;
     } else {
-        const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+        const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
         const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
         inside = pxl < cam.w && pyl < cam.h;
         i = pyl * cam.w + pxl;
@@ -1729,7 +1731,13 @@ void phr_tracer_set_samples(phr_tracer* t, uint32_t n) { if (t) t->samples = n <
 // (ao from the extent when unset, eps, glow); a repeated camera at the same
 // size adds samples, any other starts the image over. `frame` gets the index of
 // the last sample taken, `done` whether the tracer's sample count is reached.
-int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done) {
+// Tracer::sample_in (src/trace/sample.rs:104-249): the samples' launches in
+// `slices` pieces over the image's tiles, the device drained after each and
+// `go` asked whether to carry on. When it says no, the call is abandoned
+// (*finished = 0) and the next one starts the image over, since a frame with
+// some pixels a sample ahead is no frame.
+int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, uint32_t slices,
+                         phr_go_fn go, void* user, uint32_t* frame, int* done, int* finished) {
     PHR_CHECK(t && cam_v, PHR_ERR_ARG, "phr_tracer_sample: null argument");
     PHR_CHECK(spp >= 1, PHR_ERR_ARG, "phr_tracer_sample: spp must be at least 1");
     w = std::max<uint32_t>(w, 1);
@@ -1771,22 +1779,37 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     const uint32_t tiles_x = (w + 15) / 16, tiles = tiles_x * ((h + 15) / 16);
     const PhrCamera* cp = (const PhrCamera*)t->cam_dev;
     uint32_t* pxp = (uint32_t*)t->px->ptr;
-    auto path = [&](uint32_t f0, uint32_t count, uint32_t listed) {
-        if (t->forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
-            hipLaunchKernelGGL((k_path<true, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp, t->rays,
-                               listed);
-        else if (t->path_waves == 4)
-            hipLaunchKernelGGL((k_path<false, 4>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                               t->rays, listed);
-        else if (t->path_waves == 3)
-            hipLaunchKernelGGL((k_path<false, 3>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                               t->rays, listed);
-        else
-            hipLaunchKernelGGL((k_path<false, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                               t->rays, listed);
+    if (finished) *finished = 1;
+    slices = std::min(std::max<uint32_t>(slices, 1u), tiles);
+    bool abandoned = false;
+    // One launch of `count` samples from f0, in `slices` pieces of the tiles.
+    auto path = [&](uint32_t f0, uint32_t count, uint32_t listed) -> int {
+        for (uint32_t k = 0; k < slices && !abandoned; ++k) {
+            const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
+            if (b == a) continue;
+            const dim3 grid(b - a);
+            if (t->forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
+                hipLaunchKernelGGL((k_path<true, 2>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp, t->rays,
+                                   listed, a);
+            else if (t->path_waves == 4)
+                hipLaunchKernelGGL((k_path<false, 4>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                                   t->rays, listed, a);
+            else if (t->path_waves == 3)
+                hipLaunchKernelGGL((k_path<false, 3>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                                   t->rays, listed, a);
+            else
+                hipLaunchKernelGGL((k_path<false, 2>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
+                                   t->rays, listed, a);
+            PHR_HIP(hipGetLastError());
+            if (slices > 1) {
+                PHR_HIP(hipStreamSynchronize(s));
+                if (go && !go(user)) abandoned = true;
+            }
+        }
+        return PHR_OK;
     };
     if (cam.adapt[3] == 0.0f) {
-        path(frame0, spp, 0u);
+        if (int rc = path(frame0, spp, 0u); rc != PHR_OK) return rc;
     } else {
         // Adaptive Sampling (sample.rs `BTail::Adapt`): the tail runs after
         // every sample, and only a step boundary past the minimum changes a
@@ -1798,7 +1821,8 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
             uint32_t b = std::max(f, min_s + 1u);  // the next checked sample: past the minimum, last of a step
             b = b | (step - 1u);
             const uint32_t e = std::min(frame0 + spp, b + 1u);
-            path(f, e - f, f > 0u ? 1u : 0u);
+            if (int rc = path(f, e - f, f > 0u ? 1u : 0u); rc != PHR_OK) return rc;
+            if (abandoned) break;
             if (e == b + 1u)
                 hipLaunchKernelGGL(k_adapt_check, dim3(ng), dim3(256), 0, s, pxp, (uint32_t)n, cam.adapt[0]);
             hipLaunchKernelGGL(k_adapt_dilate, dim3(ng), dim3(256), 0, s, pxp, w, h);
@@ -1811,6 +1835,11 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     PHR_HIP(hipEventRecord(t->ev1, s));
     t->last_submit_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     PHR_HIP(hipEventSynchronize(t->ev1));
+    if (abandoned) {
+        t->have_last = false;
+        if (finished) *finished = 0;
+        return PHR_OK;
+    }
     float ms = 0.f;
     PHR_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     t->last_gpu_s = ms * 1e-3;
@@ -1818,6 +1847,11 @@ int phr_tracer_sample(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, 
     if (frame) *frame = t->frame;
     if (done) *done = (t->frame + 1 >= t->samples) ? 1 : 0;
     return PHR_OK;
+}
+
+// Tracer::sample: `spp` samples in one piece.
+int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done) {
+    return phr_tracer_sample_in(t, cam, w, h, spp, 1, nullptr, nullptr, frame, done, nullptr);
 }
 
 // The path kernel variant: register allocation fitted to 2, 3 or 4 waves a

@@ -170,3 +170,25 @@ def test_adaptive_sampling_matches_the_twin(gpu, orc):
         bad = cam.copy()
         bad["adapt"] = (0.08, 3.0, 3.0, 1.0)
         tr.sample(bad, w, h, 1)
+
+
+def test_sample_in_slices(gpu, orc):
+    """Tracer::sample_in: slices give the same image as one launch; a `go`
+    that says no abandons the call and the next starts the image over."""
+    print("[trace] sample_in", flush=True)
+    tris = S.spheres_scene(spheres=6, level=2, seed=4)
+    w, h = 72, 40
+    cam = S.path_camera(w, h, rr_from=3)
+    sc = orc.Scene(tris, 4, P.NODE)
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    for f in range(3):
+        sc.sample(cam, w, h, px, f, nthreads=8)
+    tr = P.Tracer.new(gpu, tris, 4)
+    asked = []
+    assert tr.sample_in(cam, w, h, 2, 3, go=lambda: asked.append(1) or True) == (1, False)
+    assert len(asked) == 3
+    assert tr.sample_in(cam, w, h, 1, 5) == (2, False)
+    compare_planes(tr.px(), px, w, h)
+    assert tr.sample_in(cam, w, h, 1, 4, go=lambda: False) is None
+    frame, _ = tr.sample(cam, w, h, 1)
+    assert frame == 0  # started over
