@@ -140,6 +140,12 @@ int phr_bvh_codes(phr_bvh* b, uint32_t* sorted_codes);
 int phr_bvh_build_batch(phr_gpu* g, const void* prims, uint32_t n, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf,
                         phr_bvh** out);
 int phr_bvh_build_batch_on(phr_gpu* g, phr_array* prims, const uint32_t* seg_lo, uint32_t segs, uint32_t leaf, phr_bvh** out);
+/* Bvh::build_compact (src/bvh/build.rs:104-136) of the tree's primitives: the
+ * same reachable tree numbered breadth first, children adjacent, computed on
+ * the device from this build. *count nodes (32 B each) into `nodes` (cap
+ * nodes at most; null: only the count). One tree, below 2^26 primitives; the
+ * order and depth are phr_bvh_download's. */
+int phr_bvh_compact(phr_bvh* b, void* nodes, uint64_t cap, uint32_t* count);
 /* One segment's tree relative to itself, as GpuBvh::download would give it
  * for that segment's boxes alone: nodes[2 n_s - 1], order[n_s], depth. */
 int phr_bvh_download_segment(phr_bvh* b, uint32_t seg, void* nodes, uint32_t* order, uint32_t* depth);

@@ -178,6 +178,7 @@ _SIGS = {
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
+    "phr_bvh_compact": (_I, [_P, _P, _U64, C.POINTER(_U32)]),
     "phr_tracer_sample_in": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, C.POINTER(_U32), C.POINTER(_I),
                                   C.POINTER(_I)]),
     "phr_tracer_open_pixels": (_I, [_P, C.POINTER(_U32)]),
@@ -567,6 +568,15 @@ class GpuBvh:
         depth = C.c_uint32(0)
         _check(lib().phr_bvh_download(self._h, _ptr(nodes), _ptr(order), C.byref(depth)), "GpuBvh::download")
         return Bvh(nodes, order, int(depth.value))
+
+    def compact(self) -> np.ndarray:
+        """`Bvh::build_compact`'s nodes for the same primitives and leaf size:
+        the reachable tree numbered breadth first, computed on the device."""
+        n = len(self)
+        nodes = np.zeros(max(2 * n - 1, 0), dtype=NODE)
+        cnt = C.c_uint32(0)
+        _check(lib().phr_bvh_compact(self._h, _ptr(nodes), len(nodes), C.byref(cnt)), "GpuBvh compact")
+        return nodes[: cnt.value].copy()
 
     def sorted_codes(self) -> np.ndarray:
         out = np.zeros(len(self), dtype=np.uint32)

@@ -896,6 +896,82 @@ __global__ __launch_bounds__(256) void k_sort_keys(const uint32_t* __restrict__ 
 // ---------------------------------------------------------------------------
 // GpuBvh (src/bvh/gpu.rs:11-104), one tree or a batch of segments
 // ---------------------------------------------------------------------------
+// ---- breadth-first numbering (Bvh::build_compact, src/bvh/build.rs:104-136) ----
+// Breadth first visits a level's nodes left to right, and the nodes of one
+// level have disjoint ranges, so a reachable node's number is its rank by
+// (depth, first slot). Key per node id: that pair, or all ones when the node
+// is not reachable (k_pack's test: a packed ancestor or depth past the cap).
+constexpr uint32_t CP_FIRST_BITS = 26;  // trees up to 2^26 primitives
+__global__ __launch_bounds__(256) void k_compact_keys(const PhrLink* __restrict__ links, const uint4* __restrict__ up,
+                                                      const uint32_t* __restrict__ stab, uint32_t n, uint32_t leaf,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool reach = false;
+    uint32_t key = 0xffffffffu;
+    if (t < 2 * n) {
+        const bool inner = t < n;
+        const uint32_t slot = inner ? t : t - n;
+        if (!inner || slot + 1 < n) {
+            uint32_t d, first;
+            if (inner) {
+                const PhrLink l = links[t];
+                d = stab[l.left < n ? l.left : l.left - n] - 1u;
+                first = l.first;
+            } else {
+                first = slot;
+                d = max(slot > 0 ? stab[slot - 1] : 0u, stab[slot]);
+            }
+            bool muted = false;
+            if (d < PHR_STACK) {
+                const uint4 u = up[t];
+                muted = u.x != PHR_NONE && u.z - u.y < leaf;
+            }
+            reach = d < PHR_STACK && !muted;
+            if (reach) key = (d << CP_FIRST_BITS) | first;
+        }
+        keys[t] = key;
+    }
+    const uint64_t m = __ballot(reach);
+    if (__lane_id() == 0 && m) atomicAdd(count, (uint32_t)__popcll(m));
+}
+
+// Position of each reachable node id in the breadth-first order.
+__global__ __launch_bounds__(256) void k_compact_inv(const uint32_t* __restrict__ ids, uint32_t r, uint32_t* __restrict__ inv) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < r) inv[ids[p]] = p;
+}
+
+// Node p of the breadth-first tree: the packed node's box; an inner node that
+// was not packed points at its children's positions, the others are leaves.
+__global__ __launch_bounds__(256) void k_compact_write(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ skeys,
+                                                       uint32_t r, const PhrLink* __restrict__ links, const uint32_t* __restrict__ inv,
+                                                       uint32_t n, uint32_t leaf, const float4* __restrict__ nodes,
+                                                       float4* __restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= r) return;
+    const uint32_t id = ids[p], d = skeys[p] >> CP_FIRST_BITS;
+    const uint32_t at = id < n ? id : (n - 1) + (id - n);  // one tree: node_base 0
+    float4 lo = nodes[2 * (size_t)at], hi = nodes[2 * (size_t)at + 1];
+    uint32_t a, b;
+    if (id < n) {
+        const PhrLink l = links[id];
+        if (l.last - l.first < leaf || d + 1 >= PHR_STACK) {
+            a = l.first | PHR_LEAF;
+            b = l.last + 1;
+        } else {
+            a = inv[l.left];
+            b = inv[l.right];
+        }
+    } else {
+        a = (id - n) | PHR_LEAF;
+        b = id - n + 1;
+    }
+    lo.w = __uint_as_float(a);
+    hi.w = __uint_as_float(b);
+    out[2 * (size_t)p] = lo;
+    out[2 * (size_t)p + 1] = hi;
+}
+
 struct phr_bvh {
     phr_gpu* gpu = nullptr;
     uint32_t n = 0, leaf = 4, segs = 1, nonempty = 1, tiles = 0, bounds_blocks = 0, seg_passes = 0, leaf_flag = PHR_LEAF;
@@ -1249,6 +1325,75 @@ int phr_bvh_download(phr_bvh* b, void* nodes, uint32_t* order, uint32_t* depth) 
     return phr_bvh_download_segment(b, 0, nodes, order, depth);
 }
 
+int phr_sort_record(phr_gpu* g, phr_array* codes, uint32_t n, phr_sort** out);
+int phr_sort_run(phr_sort* s);
+void phr_sort_free(phr_sort* s);
+phr_array* phr_sort_code(phr_sort* s);
+static const uint32_t* sort_order_dev(phr_sort* s);
+
+// Bvh::build_compact (src/bvh/build.rs:104-136) of this tree's primitives: the
+// same reachable tree numbered breadth first, children adjacent, computed on
+// the device from the build (ranks of (depth, first slot) by the onesweep
+// sort) and copied out. The order and depth are the build's.
+int phr_bvh_compact(phr_bvh* b, void* nodes, uint64_t cap, uint32_t* count) {
+    PHR_CHECK(b && count, PHR_ERR_ARG, "phr_bvh_compact: null");
+    PHR_CHECK(b->segs == 1 && b->leaf_flag == PHR_LEAF, PHR_ERR_ARG, "phr_bvh_compact: one tree at a time");
+    const uint32_t n = b->n;
+    *count = 0;
+    if (n == 0) return PHR_OK;
+    PHR_CHECK(n < (1u << CP_FIRST_BITS), PHR_ERR_SIZE, "phr_bvh_compact: at most 2^26 primitives");
+    hipStream_t s = b->gpu->stream;
+    phr_array* keys = nullptr;
+    phr_sort* srt = nullptr;
+    uint32_t* aux = nullptr;  // count, then the inverse map (2n)
+    float4* out = nullptr;
+    int rc = phr_array_alloc(b->gpu, 4, 2ull * n, &keys);
+    auto done = [&](int code) {
+        if (srt) phr_sort_free(srt);
+        phr_array_release(keys);
+        if (aux) (void)hipFree(aux);
+        if (out) (void)hipFree(out);
+        return code;
+    };
+    if (rc != PHR_OK) return done(rc);
+    if (hipMalloc((void**)&aux, 4ull * (2ull * n + 1)) != hipSuccess || hipMalloc((void**)&out, 32ull * (2ull * n - 1)) != hipSuccess) {
+        phr_set_error_msg("phr_bvh_compact: allocation failed");
+        return done(PHR_ERR_HIP);
+    }
+    if (hipMemsetAsync(aux, 0, 4, s) != hipSuccess) return done(PHR_ERR_HIP);
+    hipLaunchKernelGGL(k_compact_keys, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->up, (const uint32_t*)b->keys_b, n,
+                       b->leaf, (uint32_t*)keys->ptr, aux);
+    if ((rc = phr_sort_record(b->gpu, keys, 2 * n, &srt)) != PHR_OK || (rc = phr_sort_run(srt)) != PHR_OK) return done(rc);
+    uint32_t r = 0;
+    if (hipMemcpyAsync(&r, aux, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        phr_set_error_msg("phr_bvh_compact: readback failed");
+        return done(PHR_ERR_HIP);
+    }
+    const uint32_t* ids = sort_order_dev(srt);
+    const uint32_t* skeys = (const uint32_t*)phr_sort_code(srt)->ptr;
+    hipLaunchKernelGGL(k_compact_inv, dim3((r + 255) / 256), dim3(256), 0, s, ids, r, aux + 1);
+    hipLaunchKernelGGL(k_compact_write, dim3((r + 255) / 256), dim3(256), 0, s, ids, skeys, r, b->links, aux + 1, n, b->leaf,
+                       (const float4*)b->nodes->ptr, out);
+    if (hipGetLastError() != hipSuccess) {
+        phr_set_error_msg("phr_bvh_compact: launch failed");
+        return done(PHR_ERR_HIP);
+    }
+    *count = r;
+    if (nodes) {
+        if (cap < r) {
+            phr_set_error_msg("phr_bvh_compact: node buffer too small");
+            return done(PHR_ERR_SIZE);
+        }
+        if (hipMemcpyAsync(nodes, out, 32ull * r, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+            phr_set_error_msg("phr_bvh_compact: readback failed");
+            return done(PHR_ERR_HIP);
+        }
+    } else if (hipStreamSynchronize(s) != hipSuccess) {
+        return done(PHR_ERR_HIP);
+    }
+    return done(PHR_OK);
+}
+
 // One segment's tree as Bvh::build of its boxes alone would hold it: node
 // indices, leaf ranges and the order all relative to the segment.
 int phr_bvh_download_segment(phr_bvh* b, uint32_t seg, void* nodes, uint32_t* order, uint32_t* depth) {
@@ -1315,6 +1460,8 @@ struct phr_sort {
     uint32_t n = 0, tiles = 0, zero_words = 0;
     uint32_t *order = nullptr, *kb = nullptr, *vb = nullptr, *zero = nullptr;
 };
+
+static const uint32_t* sort_order_dev(phr_sort* s) { return s->order; }
 
 static void sort_release(phr_sort* s) {
     if (!s) return;

@@ -110,3 +110,40 @@ def test_rebuild_is_deterministic(gpu):
         assert (a.order == b.order).all() and a.depth == b.depth
         r = np.array(a.reachable())
         assert (a.nodes[r].view(np.uint32) == b.nodes[r].view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("n,leaf", [(1, 4), (2, 1), (7, 1), (1000, 1), (5000, 4), (65536, 4)])
+def test_build_compact_matches_cpu(gpu, orc, n, leaf):
+    """Bvh::build_compact on the device: every node word equal to the oracle's
+    breadth-first tree (src/bvh/build.rs:104-136)."""
+    print(f"[lbvh] compact n={n} leaf={leaf}", flush=True)
+    if n == 65536:
+        corners, _ = S.lbvh_scene(65536, 0)
+        prims = S.corner_boxes(corners)
+    else:
+        prims = S.xorshift_prims_unit(n)
+    g = P.GpuBvh.build(gpu, prims, leaf)
+    got = g.compact()
+    want = np.zeros(2 * n - 1, dtype=P.NODE)
+    order = np.zeros(n, dtype=np.uint32)
+    d = np.zeros(1, dtype=np.uint32)
+    m = orc.lib().orc_build_compact(orc.ptr(prims), n, leaf, orc.ptr(want), orc.ptr(order), orc.ptr(d))
+    assert len(got) == m
+    assert np.array_equal(got.view(np.uint32), want[:m].view(np.uint32))
+    assert g.depth == int(d[0])
+
+
+def test_build_compact_depth_cap(gpu, orc):
+    """The 31-deep chain: nodes at the stack cap are packed in the compact tree too."""
+    print("[lbvh] compact depth cap", flush=True)
+    from tests.test_oracle_lbvh import deep_chain_prims
+
+    prims = deep_chain_prims()
+    n = len(prims)
+    got = P.GpuBvh.build(gpu, prims, 1).compact()
+    want = np.zeros(2 * n - 1, dtype=P.NODE)
+    order = np.zeros(n, dtype=np.uint32)
+    d = np.zeros(1, dtype=np.uint32)
+    m = orc.lib().orc_build_compact(orc.ptr(prims), n, 1, orc.ptr(want), orc.ptr(order), orc.ptr(d))
+    assert int(d[0]) == 31 and len(got) == m
+    assert np.array_equal(got.view(np.uint32), want[:m].view(np.uint32))
