@@ -171,6 +171,59 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
             "rays_per_frame": rays}
 
 
+def xpbd_rates(gpu: P.Gpu, frames: int = 30, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
+    """The rigid-body workload driven by the XPBD World: 1,024 bodies (boxes,
+    and icospheres colliding as spheres) dropped from heights U[1, 10] m onto
+    a plane. Every substep of a 1/60 s frame steps the world, refits every
+    mesh tree and rebuilds the top tree over the new poses; every frame is
+    traced at w x h, spp samples."""
+    heights = (1.0, 10.0)
+    b, _ = S.bodies_scene(1024, heights=heights)
+    v, f, m, me, ins = b.arrays()
+    tr = P.Tracer.from_scene(gpu, v, f, m, me, ins, 4)
+    world = P.World(gpu)
+    world.configure(substeps=1, iterations=10)
+    g = world.add_box((12.0, 12.0, 0.5))
+    world.add_body(kind=P.PASSIVE, shape=g, x=(0.0, 0.0, -0.5), friction=1.0)
+    for k, d in enumerate(S.bodies_defs(1024, heights=heights)):
+        r = float(d["r"])
+        s = world.add_box((r, r, r)) if d["shape"] == "box" else world.add_sphere(r)
+        world.add_body(shape=s, x=d["x"], q=d["q"], scale=(r, r, r), obj=k)
+    mats = np.ascontiguousarray(ins["world"].copy())
+    verts = P.GpuArray.from_(gpu, v)
+    cam = S.path_camera(w, h, eye=(0.0, -16.0, 9.0), target=(0.0, 0.0, 1.5), rr_from=3)
+    tr.set_samples(1 << 20)
+    phys, scene, trace, rays, pairs = 0.0, 0.0, 0.0, 0, 0
+    z0 = float(world.bodies()["x"][1:, 2].mean())
+    for fr in range(frames):
+        for _ in range(substeps):
+            t0 = time.perf_counter()
+            world.step(1.0 / 60.0 / substeps)
+            poses = world.poses()
+            t1 = time.perf_counter()
+            mats[:1024] = poses[1:]
+            assert tr.repose_on(verts) and tr.reinstance(mats)
+            gpu.wait()
+            t2 = time.perf_counter()
+            phys += t1 - t0
+            scene += t2 - t1
+            pairs += len(world.pairs())
+        c0 = tr.counters()
+        tr.sample(cam, w, h, spp)
+        trace += tr.time()[1]
+        c1 = tr.counters()
+        rays += int(c1[0] - c0[0] + c1[1] - c0[1])
+    n = frames * substeps
+    z1 = float(world.bodies()["x"][1:, 2].mean())
+    log(f"xpbd: {frames} frames, physics {phys / n * 1e3:.3f} ms + scene {scene / n * 1e3:.3f} ms a substep, "
+        f"trace {trace / frames * 1e3:.1f} ms a frame; mean height {z0:.2f} -> {z1:.2f} m")
+    return {"bodies": 1024, "frames": frames, "substeps_per_frame": substeps, "frame": f"{w}x{h}, {spp} spp",
+            "physics_ms_per_substep": phys / n * 1e3, "scene_ms_per_substep": scene / n * 1e3,
+            "trace_ms_per_frame": trace / frames * 1e3, "mrays_s": rays / trace / 1e6,
+            "pairs_per_substep": pairs / n, "contacts_last": world.contact_count(),
+            "mean_height_m": [z0, z1]}
+
+
 def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
     """mean((x - y)^2 / (y^2 + 0.01)) over pixels and channels."""
     x, y = x.astype(np.float64), y.astype(np.float64)
@@ -355,6 +408,7 @@ def main() -> int:
         log("forest of 1,024 bodies")
         out["forest_1024"] = forest_rates(gpu)
         log(f"forest: substep {out['forest_1024']['substep_ms']:.3f} ms, {out['forest_1024']['mrays_s']:.1f} Mrays/s")
+        out["xpbd_1024"] = xpbd_rates(gpu)
 
     line = {"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",

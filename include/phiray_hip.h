@@ -37,6 +37,7 @@ typedef struct phr_clock phr_clock;
 typedef struct phr_pass phr_pass;
 typedef struct phr_sort phr_sort;
 typedef struct phr_kpcn phr_kpcn;
+typedef struct phr_world phr_world;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* phr_last_error(void);
@@ -348,6 +349,47 @@ int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at);
  * floats) when given, else in place of the linear sums (Tracer::denoise's
  * keep = false, src/trace/read.rs:85-119). */
 int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out);
+
+/* ---- rigid bodies: World (src/sim/world.rs, build.rs, state.rs) -------------
+ * XPBD with substeps on the host, restated from phiray's CPU solver; the
+ * broadphase (broad.rs: Bvh::build over the swept boxes and the overlap walk)
+ * runs on the device. Shapes: box, sphere, capsule (not a box-capsule pair);
+ * the GJK/EPA kinds are PHR_ERR_UNSUPPORTED. */
+/* Shape (state.rs, 32 B): kind 0 box, 1 sphere, 2 capsule; half extents (box)
+ * or half length on z (capsule); radius. */
+typedef struct { uint32_t kind, data, count; float radius; float half[3]; float _a; } phr_shape;
+/* BodyDef (build.rs:14-61): kind 0 active, 1 passive, 2 kinematic; q is xyzw.
+ * Defaults as phiray's: mass 1, friction 0.5, restitution 0, margin 0,
+ * lin_damp 0.04, ang_damp 0.1, groups 1, scale 1. */
+typedef struct {
+    uint32_t kind, shape; float mass;
+    float x[3]; float q[4]; float v[3]; float w[3];
+    float scale[3]; float com[3];
+    float friction, restitution, margin, lin_damp, ang_damp;
+    float sleep[2]; uint32_t has_sleep, asleep, groups, obj;
+} phr_body_def;
+int phr_world_new(phr_gpu* g, phr_world** out);
+void phr_world_free(phr_world* w);
+/* World::substeps, iterations, time_scale, max_recover, sleep_after, gravity
+ * (null: unchanged). */
+int phr_world_configure(phr_world* w, uint32_t substeps, uint32_t iterations, float time_scale, float max_recover,
+                        float sleep_after, const float gravity[3]);
+int phr_world_add_shape(phr_world* w, const phr_shape* s, uint32_t* id);
+int phr_world_add_body(phr_world* w, const phr_body_def* d, uint32_t* id);
+/* World::step (world.rs:104-129): one frame of frame_dt seconds. */
+int phr_world_step(phr_world* w, float frame_dt);
+uint32_t phr_world_body_count(const phr_world* w);
+uint32_t phr_world_contact_count(const phr_world* w);
+/* The 64-byte Body records (state.rs). */
+int phr_world_bodies(const phr_world* w, void* out);
+/* World::poses (world.rs:225-228): 16 floats a body, m[col][row], for
+ * phr_tracer_reinstance. */
+int phr_world_poses(const phr_world* w, float* out);
+/* World::energy (world.rs:231-243). */
+float phr_world_energy(const phr_world* w);
+/* Pairs the last broadphase found (sorted, deduplicated): *n pairs, up to cap
+ * written as (i, j) with i < j. */
+int phr_world_pair_list(const phr_world* w, uint32_t* out, uint32_t cap, uint32_t* n);
 
 #ifdef __cplusplus
 }

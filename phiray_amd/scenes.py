@@ -382,12 +382,13 @@ def box_mesh() -> tuple[np.ndarray, np.ndarray]:
     return p, np.array(t, dtype=np.int64)
 
 
-def bodies_scene(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, light: float = 30.0):
+def bodies_scene(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, light: float = 30.0,
+                 heights=(0.6, 4.6)):
     """The rigid-body workload: `bodies` shapes, each its own mesh under one
     instance (a box or an icosphere of subdivision 0..5, picked per body),
     radius U[0.2, 0.6], turned about a random axis, placed uniformly over a
-    spread x spread x 4 box above a ground plane, under one emissive quad.
-    Returns (builder, world matrices [bodies, 4, 4])."""
+    spread x spread square at heights U[heights] above a ground plane, under
+    one emissive quad. Returns (builder, world matrices [bodies, 4, 4])."""
     b = SceneBuilder()
     u = uniform(seed, bodies * 9).reshape(bodies, 9).astype(np.float64)
     shapes = [box_mesh()] + [icosphere(k) for k in range(6)]
@@ -401,7 +402,7 @@ def bodies_scene(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, l
         r = 0.2 + 0.4 * u[i, 1]
         ang = 2 * np.pi * u[i, 2]
         axis = (u[i, 3] - 0.5, u[i, 4] - 0.5, 0.5)
-        move = ((u[i, 5] - 0.5) * spread, (u[i, 6] - 0.5) * spread, 0.6 + 4.0 * u[i, 7])
+        move = ((u[i, 5] - 0.5) * spread, (u[i, 6] - 0.5) * spread, heights[0] + (heights[1] - heights[0]) * u[i, 7])
         m = affine(rotation(axis, ang), r, move)
         worlds.append(m)
         b.instance(mesh, m)
@@ -413,3 +414,22 @@ def bodies_scene(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, l
                   b.material(color=(0, 0, 0, 1), emit=(light, light, light)))
     b.instance(lamp, affine(move=(0.0, 0.0, 8.0)))
     return b, np.array(worlds)
+
+
+def bodies_defs(bodies: int = 1024, seed: int = 0xB0D1, spread: float = 12.0, heights=(0.6, 4.6)) -> list[dict]:
+    """The rigid bodies of bodies_scene (the same draws): per body the
+    collision shape ("box" of half extent r, or "sphere" of radius r for an
+    icosphere), its centre, its orientation as a quaternion (xyzw) and its
+    scale r, for World::add_body."""
+    u = uniform(seed, bodies * 9).reshape(bodies, 9).astype(np.float64)
+    out = []
+    for i in range(bodies):
+        k = min(int(u[i, 0] * 7), 6)
+        r = 0.2 + 0.4 * u[i, 1]
+        ang = 2 * np.pi * u[i, 2]
+        axis = np.array((u[i, 3] - 0.5, u[i, 4] - 0.5, 0.5))
+        axis /= np.linalg.norm(axis)
+        q = (*(axis * np.sin(ang / 2)), np.cos(ang / 2))
+        move = ((u[i, 5] - 0.5) * spread, (u[i, 6] - 0.5) * spread, heights[0] + (heights[1] - heights[0]) * u[i, 7])
+        out.append({"shape": "box" if k == 0 else "sphere", "r": r, "x": move, "q": q})
+    return out
