@@ -343,6 +343,12 @@ int phr_kpcn_train_step(phr_kpcn* k, phr_array* frames, phr_array* targets, uint
  * PHR_ERR_ARG for an index past either set. */
 int phr_kpcn_train_indexed(phr_kpcn* k, phr_array* frames, const uint32_t* fidx, phr_array* targets,
                            const uint32_t* tidx, uint32_t b, uint32_t w, uint32_t h, float lr, float* loss);
+/* One frame as a hipGraph (no counterpart in phiray): spp samples of a fresh
+ * image under cam at w x h, then denoise 0 none, 1 a-trous (keep = false),
+ * 2 the learned net k; kernels only. The call renders it once eagerly first.
+ * Replays are refused (PHR_ERR_ARG) once the scene, image or network moved. */
+int phr_tracer_record_frame(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, int denoise, phr_kpcn* k);
+int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k);
 /* The tracer's frame (10 floats a pixel) at float offset `at` of `out`. */
 int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at);
 /* The learned denoiser over what has been traced: into `out` (w * h * 3

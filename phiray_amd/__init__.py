@@ -178,6 +178,8 @@ _SIGS = {
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
+    "phr_tracer_record_frame": (_I, [_P, _P, _U32, _U32, _U32, _I, _P]),
+    "phr_tracer_replay_frame": (_I, [_P, _P]),
     "phr_world_new": (_I, [_P, C.POINTER(_P)]),
     "phr_world_free": (None, [_P]),
     "phr_world_configure": (_I, [_P, _U32, _U32, C.c_float, C.c_float, C.c_float, _P]),
@@ -812,6 +814,17 @@ class Tracer:
         `denoise(keep=False)` leaves them."""
         _check(lib().phr_tracer_denoise_learned(self._h, kpcn._h, out._h if out is not None else None),
                "Tracer::denoise_learned")
+
+    def record_frame(self, cam: np.ndarray, w: int, h: int, spp: int, denoise: int = 0, kpcn: "Kpcn | None" = None) -> None:
+        """Capture one frame (fresh image, spp samples, then denoise 0 none /
+        1 a-trous / 2 learned) as a hipGraph."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        _check(lib().phr_tracer_record_frame(self._h, _ptr(cam), w, h, spp, denoise, kpcn._h if kpcn else None),
+               "Tracer record frame")
+        self.size = (max(w, 1), max(h, 1))
+
+    def replay_frame(self, kpcn: "Kpcn | None" = None) -> None:
+        _check(lib().phr_tracer_replay_frame(self._h, kpcn._h if kpcn else None), "Tracer replay frame")
 
     def frame_into(self, out: GpuArray, at: int = 0) -> None:
         """The denoiser's input frame (10 floats a pixel) at float `at` of `out`."""

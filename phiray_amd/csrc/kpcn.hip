@@ -890,6 +890,9 @@ int check_shape(phr_array* frames, uint32_t b, uint32_t w, uint32_t h, phr_array
 // ---- C ABI ----------------------------------------------------------------------------
 extern "C" {
 
+// The workspace's address (a recorded graph holds it).
+const void* phr_kpcn_internal_workspace(const phr_kpcn* k) { return k ? (const void*)k->ws : nullptr; }
+
 uint32_t phr_kpcn_layers(uint32_t* out) {
     if (out)
         for (int i = 0; i < NLAYERS; ++i) {

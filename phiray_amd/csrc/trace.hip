@@ -1149,6 +1149,14 @@ struct phr_tracer {
     bool stale = false;
     std::vector<uint32_t> words_h;    // instance words staged for upload
     std::vector<float> inst_worlds_h; // matrices staged for upload
+    // One recorded frame (phr_tracer_record_frame): its graph, the camera, and
+    // every device address it baked in, to refuse a replay after any moved.
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fexec = nullptr;
+    PhrCamera fcam{};
+    uint32_t fw = 0, fh = 0, fspp = 0;
+    SceneDev fsc{};
+    const void* fsig[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -1185,6 +1193,8 @@ void tracer_release(phr_tracer* t) {
         phr_array_release(a);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->frame_dev) (void)hipFree(t->frame_dev);
+    if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
+    if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
     if (t->rays) (void)hipFree(t->rays);
     if (t->cam_dev) (void)hipFree(t->cam_dev);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
@@ -1991,6 +2001,110 @@ int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at) {
                        (const uint32_t*)t->px->ptr, n, (float*)out->ptr + at);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipStreamSynchronize(t->gpu->stream));
+    return PHR_OK;
+}
+
+const void* phr_kpcn_internal_workspace(const phr_kpcn* k);
+
+namespace {
+bool same_scene(const SceneDev& a, const SceneDev& b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+}  // namespace
+
+// One frame as a hipGraph (no counterpart in phiray, whose Pass replays per
+// dispatch): `spp` samples of a fresh image under `cam` at w x h, then the
+// a-trous denoiser (denoise 1, keep = false) or the learned one (denoise 2,
+// net k) in place of the linear sums. Kernels only, no memory sets. Replays
+// draw the same samples; the graph holds the scene's, image's and network's
+// device addresses, so a replay after any of them moved is refused.
+int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, int denoise, phr_kpcn* k) {
+    PHR_CHECK(t && cam_v && spp >= 1 && denoise >= 0 && denoise <= 2 && (denoise != 2 || k), PHR_ERR_ARG,
+              "phr_tracer_record_frame: bad argument");
+    PhrCamera cam;
+    std::memcpy(&cam, cam_v, sizeof cam);
+    PHR_CHECK(cam.adapt[3] == 0.0f, PHR_ERR_UNSUPPORTED, "phr_tracer_record_frame: adaptive sampling changes its launches");
+    // An eager frame first: it stamps the camera, sizes the image, and gives
+    // the denoisers their workspaces, so the capture allocates nothing.
+    {
+        PhrCamera fresh = cam;
+        t->have_last = false;
+        int rc = phr_tracer_sample(t, &fresh, w, h, spp, nullptr, nullptr);
+        if (rc != PHR_OK) return rc;
+        if (denoise == 2 && (rc = phr_tracer_denoise_learned(t, k, nullptr)) != PHR_OK) return rc;
+    }
+    if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
+    if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
+    t->fexec = nullptr;
+    t->fgraph = nullptr;
+    const uint64_t n = (uint64_t)t->w * t->h;
+    hipStream_t s = t->gpu->stream;
+    const SceneDev sc = scene_dev(t);
+    const uint32_t tiles_x = (t->w + 15) / 16, tiles = tiles_x * ((t->h + 15) / 16);
+    const PhrCamera* cp = (const PhrCamera*)t->cam_dev;
+    uint32_t* pxp = (uint32_t*)t->px->ptr;
+    PHR_HIP(hipStreamSynchronize(s));
+    PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    if (t->forest)
+        hipLaunchKernelGGL((k_path<true, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    else if (t->path_waves == 4)
+        hipLaunchKernelGGL((k_path<false, 4>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    else if (t->path_waves == 3)
+        hipLaunchKernelGGL((k_path<false, 3>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    else
+        hipLaunchKernelGGL((k_path<false, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    int rc = PHR_OK;
+    if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0);
+    if (denoise == 2) {
+        const uint32_t g = (uint32_t)((n + 255) / 256);
+        float* rad = t->frame_dev + 10 * n;
+        hipLaunchKernelGGL(k_frame, dim3(g), dim3(256), 0, s, (const uint32_t*)pxp, n, t->frame_dev);
+        rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
+        if (rc == PHR_OK) hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, n, (const float*)rad);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc != PHR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    PHR_CHECK(e == hipSuccess && graph, PHR_ERR_HIP, "phr_tracer_record_frame: capture failed");
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        phr_set_error_msg("phr_tracer_record_frame: instantiation failed");
+        return PHR_ERR_HIP;
+    }
+    t->fgraph = graph;
+    t->fexec = exec;
+    t->fcam = t->last;  // as stamped
+    t->fw = t->w, t->fh = t->h, t->fspp = spp;
+    t->fsc = sc;
+    t->fsig[0] = t->px->ptr;
+    t->fsig[1] = denoise == 2 ? (const void*)t->frame_dev : nullptr;
+    t->fsig[2] = denoise == 2 ? phr_kpcn_internal_workspace(k) : nullptr;
+    t->fsig[3] = t->cam_dev;
+    return PHR_OK;
+}
+
+// Replay the recorded frame: the image holds its samples (denoised as
+// recorded) afterwards, and a later sample call under the same camera adds to them.
+int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k) {
+    PHR_CHECK(t && t->fexec, PHR_ERR_ARG, "phr_tracer_replay_frame: nothing recorded");
+    const bool moved = !t->px || t->px->ptr != t->fsig[0] || t->w != t->fw || t->h != t->fh || !same_scene(scene_dev(t), t->fsc) ||
+                       (t->fsig[1] && (t->frame_dev != t->fsig[1] || !k || phr_kpcn_internal_workspace(k) != t->fsig[2]));
+    PHR_CHECK(!moved, PHR_ERR_ARG, "phr_tracer_replay_frame: the scene, the image or the network moved; record again");
+    hipStream_t s = t->gpu->stream;
+    PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->fcam, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
+    PHR_HIP(hipEventRecord(t->ev0, s));
+    PHR_HIP(hipGraphLaunch(t->fexec, s));
+    PHR_HIP(hipEventRecord(t->ev1, s));
+    PHR_HIP(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    PHR_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    t->last_gpu_s = ms * 1e-3;
+    t->w = t->fw, t->h = t->fh;
+    t->last = t->fcam;
+    t->have_last = true;
+    t->frame = t->fspp - 1;
     return PHR_OK;
 }
 

@@ -159,3 +159,21 @@ def test_tracer_frame_and_learned_denoise(gpu):
     tr.denoise_learned(net)  # in place of the linear sums
     np.testing.assert_allclose(tr.radiance().ravel(), out.read(), rtol=1e-6, atol=1e-7)
     del frame_only
+
+
+def test_recorded_frame_with_the_learned_denoiser(gpu):
+    from phiray_amd import scenes as S
+
+    tris = S.spheres_scene(spheres=6, level=2, seed=5)
+    w, h = 56, 40
+    cam = S.path_camera(w, h, rr_from=3)
+    tr = P.Tracer.new(gpu, tris, 4)
+    net = P.Kpcn(gpu, _params(8))
+    tr.sample(cam, w, h, 2)
+    tr.denoise_learned(net)
+    want = tr.radiance()
+    tr.record_frame(cam, w, h, 2, denoise=2, kpcn=net)
+    for _ in range(3):
+        tr.sample(cam, w, h, 1)
+        tr.replay_frame(net)
+        assert np.array_equal(tr.radiance(), want)

@@ -192,3 +192,27 @@ def test_sample_in_slices(gpu, orc):
     assert tr.sample_in(cam, w, h, 1, 4, go=lambda: False) is None
     frame, _ = tr.sample(cam, w, h, 1)
     assert frame == 0  # started over
+
+
+def test_recorded_frame_replays_equal_eager(gpu):
+    """A frame captured as a hipGraph (path + a-trous) gives, replay after
+    replay with other work between, the image an eager frame gives."""
+    print("[trace] recorded frame", flush=True)
+    tris = S.spheres_scene(spheres=6, level=2, seed=8)
+    w, h = 64, 40
+    cam = S.path_camera(w, h, rr_from=3)
+    tr = P.Tracer.new(gpu, tris, 4)
+    tr.sample(cam, w, h, 3)
+    tr.denoise(keep=False)
+    want = tr.px()
+    other = S.path_camera(w, h, rr_from=3, eye=(1.0, -5.0, 2.0))
+    tr.record_frame(cam, w, h, 3, denoise=1)
+    for _ in range(3):
+        tr.sample(other, w, h, 2)  # eager work on the same image between replays
+        tr.replay_frame()
+        assert np.array_equal(tr.px()[: 25 * w * h], want[: 25 * w * h])
+    frame, _ = tr.sample(cam, w, h, 1)  # the recorded camera: samples add on
+    assert frame == 3
+    tr.sample(cam, w + 8, h, 1)  # a new size moves the image: the graph is refused
+    with pytest.raises(P.PhrError):
+        tr.replay_frame()
