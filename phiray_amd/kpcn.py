@@ -98,6 +98,7 @@ def train(gpu: Gpu, frames: GpuArray, targets: GpuArray, n_scenes: int, variants
     net = Kpcn(gpu, init_params(seed) if params is None else params)
     rng = np.random.default_rng(seed + 1)
     hist, t0 = [], time.perf_counter()
+    good, scale = net.params(), 1.0  # the last weights with a finite loss; a rate factor
     for step in range(steps):
         scenes = rng.integers(0, n_scenes, size=batch)
         var = rng.integers(0, variants, size=batch)
@@ -105,9 +106,15 @@ def train(gpu: Gpu, frames: GpuArray, targets: GpuArray, n_scenes: int, variants
         frac = step / max(steps - 1, 1)
         if budget_s is not None:
             frac = max(frac, min(1.0, (time.perf_counter() - t0) / budget_s))
-        rate = lo + 0.5 * (lr - lo) * (1.0 + np.cos(np.pi * frac))
+        rate = (lo + 0.5 * (lr - lo) * (1.0 + np.cos(np.pi * frac))) * scale
         loss = net.train_indexed(frames, scenes * variants + var, targets, scenes, crop, crop, rate, want_loss=want)
         if want:
+            if not np.isfinite(loss):  # diverged: back to the last finite weights at half the rate
+                scale *= 0.5
+                net.set_params(good)
+                log(f"[kpcn] step {step + 1}: loss {loss}; weights restored, rate x{scale}")
+                continue
+            good = net.params()
             hist.append((step + 1, loss))
             el = time.perf_counter() - t0
             log(f"[kpcn] step {step + 1}/{steps} loss {loss:.5f} ({el:.0f} s, {1e3 * el / (step + 1):.1f} ms/step)")
