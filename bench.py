@@ -248,8 +248,12 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
     log(f"convergence: {ref_spp}-spp reference in {time.perf_counter() - t0:.1f} s")
     clock = P.Clock(gpu)
     out = P.GpuArray.zeroed(gpu, np.float32, w * h * 3) if net is not None else None
-    if net is not None:  # warm the network's workspace at this size
+    tiles = None
+    if net is not None:  # warm the network's workspace at this size and pick its conv tiles
         tr.sample(cam, w, h, 1)
+        frames = P.GpuArray.zeroed(gpu, np.float32, w * h * P.Kpcn.FRAME)
+        tr.frame_into(frames)
+        tiles = net.autotune(frames, 1, w, h).tolist()
         tr.denoise_learned(net, out)
     trials, best = [], {"raw": None, "plain": None, "learned": None}
     for spp in (1, 2, 4, 8, 16, 32, 64, 128, 256):
@@ -282,7 +286,7 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
         if all(v is not None for k, v in best.items() if net is not None or k != "learned"):
             break
     res = {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
-           "plain": best["plain"], "learned": best["learned"], "trials": trials}
+           "plain": best["plain"], "learned": best["learned"], "trials": trials, "kpcn_rows_per_wave": tiles}
     if best["learned"] and best["raw"]:
         res["raw_over_learned"] = best["raw"]["s_per_frame"] / best["learned"]["s_per_frame"]
     if best["learned"] and best["plain"]:

@@ -338,6 +338,9 @@ int phr_kpcn_adam(phr_kpcn* k, float lr);
 /* phr_kpcn_backward then phr_kpcn_adam. */
 int phr_kpcn_train_step(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b, uint32_t w, uint32_t h,
                         float lr, float* loss);
+/* Autotune (no counterpart): each layer's forward conv tile, timed on b frames
+ * of w x h; chosen gets the rows a wave owns per layer (phr_kpcn_layers order). */
+int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, uint32_t reps, uint8_t* chosen);
 /* One training step over b images of w x h gathered from resident data sets:
  * frame image fidx[j] of `frames` against target image tidx[j] of `targets`.
  * PHR_ERR_ARG for an index past either set. */

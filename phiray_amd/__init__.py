@@ -175,6 +175,7 @@ _SIGS = {
     "phr_kpcn_backward": (_I, [_P, _P, _P, _U32, _U32, _U32, C.POINTER(C.c_float)]),
     "phr_kpcn_adam": (_I, [_P, C.c_float]),
     "phr_kpcn_train_step": (_I, [_P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
+    "phr_kpcn_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P]),
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
@@ -931,6 +932,13 @@ class Kpcn:
         _check(lib().phr_kpcn_train_step(self._h, frames._h, targets._h, b, w, h, lr,
                                          C.byref(v) if want_loss else None), "Kpcn::train_step")
         return float(v.value) if want_loss else None
+
+    def autotune(self, frames: GpuArray, b: int, w: int, h: int, reps: int = 3) -> np.ndarray:
+        """Time each layer's forward conv tiles on this frame size and keep
+        the fastest: rows a wave owns, per layer."""
+        out = np.zeros(len(self.layers()), dtype=np.uint8)
+        _check(lib().phr_kpcn_autotune(self._h, frames._h, b, w, h, reps, _ptr(out)), "Kpcn autotune")
+        return out
 
     def train_indexed(self, frames: GpuArray, fidx, targets: GpuArray, tidx, w: int, h: int, lr: float,
                       want_loss: bool = True) -> float | None:

@@ -573,6 +573,7 @@ struct phr_kpcn {
     std::vector<float> loss_host;
     char* batch = nullptr;  // gathered frames and targets of an indexed step
     uint64_t batch_bytes = 0;
+    uint8_t rw[NLAYERS] = {};  // forward conv tile per layer (rows a wave owns; 0 = default), set by autotune
 };
 
 namespace {
@@ -660,19 +661,29 @@ void launch_conv_t(const ConvArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_conv<NF, RW>), dim3(tx * ty * a.B), dim3(256), 0, s, a, tx, ty);
 }
 
-int launch_conv(const ConvArgs& a, hipStream_t s) {
+// rw: rows a wave owns (1 or 2; 0 = 2 up to 96 output channels, else 1).
+int launch_conv(const ConvArgs& a, hipStream_t s, int rw = 0) {
     if (a.cin % 16 || a.cout % 16) {
         phr_set_error_msg("kpcn conv: channels must be multiples of 16");
         return PHR_ERR_ARG;
     }
-    switch (a.cout / 16) {
-        case 2: launch_conv_t<2, 2>(a, s); break;
-        case 3: launch_conv_t<3, 2>(a, s); break;
-        case 4: launch_conv_t<4, 2>(a, s); break;
-        case 5: launch_conv_t<5, 2>(a, s); break;
-        case 6: launch_conv_t<6, 2>(a, s); break;
-        case 7: launch_conv_t<7, 1>(a, s); break;
-        case 10: launch_conv_t<10, 1>(a, s); break;
+    const uint32_t nf = a.cout / 16;
+    if (rw == 0) rw = nf <= 6 ? 2 : 1;
+    if (nf == 10) rw = 1;  // 40 accumulators a row: two rows would leave one wave a SIMD
+    switch (nf * 4 + (uint32_t)rw) {
+        case 2 * 4 + 1: launch_conv_t<2, 1>(a, s); break;
+        case 2 * 4 + 2: launch_conv_t<2, 2>(a, s); break;
+        case 3 * 4 + 1: launch_conv_t<3, 1>(a, s); break;
+        case 3 * 4 + 2: launch_conv_t<3, 2>(a, s); break;
+        case 4 * 4 + 1: launch_conv_t<4, 1>(a, s); break;
+        case 4 * 4 + 2: launch_conv_t<4, 2>(a, s); break;
+        case 5 * 4 + 1: launch_conv_t<5, 1>(a, s); break;
+        case 5 * 4 + 2: launch_conv_t<5, 2>(a, s); break;
+        case 6 * 4 + 1: launch_conv_t<6, 1>(a, s); break;
+        case 6 * 4 + 2: launch_conv_t<6, 2>(a, s); break;
+        case 7 * 4 + 1: launch_conv_t<7, 1>(a, s); break;
+        case 7 * 4 + 2: launch_conv_t<7, 2>(a, s); break;
+        case 10 * 4 + 1: launch_conv_t<10, 1>(a, s); break;
         default: phr_set_error_msg("kpcn conv: unsupported output channel count"); return PHR_ERR_ARG;
     }
     PHR_HIP(hipGetLastError());
@@ -691,7 +702,7 @@ int conv_fwd(phr_kpcn* k, int li, const uint16_t* x, uint16_t* y16, float* y32) 
     a.y32 = y32;
     a.B = k->B, a.H = k->Hp >> L.level, a.W = k->Wp >> L.level;
     a.cin = L.cin, a.cout = L.cout, a.relu = L.relu;
-    return launch_conv(a, k->gpu->stream);
+    return launch_conv(a, k->gpu->stream, k->rw[li]);
 }
 
 // Input gradient of layer `li`: conv of its output gradient with the flipped,
@@ -1030,6 +1041,47 @@ int phr_kpcn_adam(phr_kpcn* k, float lr) {
                        k->nparams, lr, b1, b2, eps, bc1, bc2);
     PHR_HIP(hipGetLastError());
     return pack_weights(k);
+}
+
+// Pick each layer's forward conv tile (4 or 8 rows of 32 pixels a block) by
+// timing both on b frames of w x h: the best of `reps` launches each, with HIP
+// events on the network's stream. chosen (optional) gets the rows a wave owns
+// per layer. The outputs do not depend on the tile.
+int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, uint32_t reps, uint8_t* chosen) {
+    PHR_CHECK(k, PHR_ERR_ARG, "phr_kpcn_autotune: null");
+    KP_TRY(check_shape(frames, b, w, h, nullptr, "phr_kpcn_autotune"));
+    KP_TRY(ensure_ws(k, b, h, w, false));
+    KP_TRY(forward(k, (const float*)frames->ptr));  // every layer's input in place
+    hipStream_t s = k->gpu->stream;
+    hipEvent_t e0, e1;
+    PHR_HIP(hipEventCreate(&e0));
+    PHR_HIP(hipEventCreate(&e1));
+    const uint16_t* in[NLAYERS] = {k->F, k->act[E0A], k->Q0, k->act[E1A], k->Q1, k->act[E2A], k->Q2, k->U2, k->U1, k->U0, k->act[D0]};
+    int rc = PHR_OK;
+    for (int li = 0; li < NLAYERS && rc == PHR_OK; ++li) {
+        float best = 3.0e38f;
+        uint8_t pick = 0;
+        for (uint8_t rw = 1; rw <= 2 && rc == PHR_OK; ++rw) {
+            if (LAYERS[li].cout == 160 && rw == 2) continue;
+            k->rw[li] = rw;
+            float mine = 3.0e38f;
+            for (uint32_t r = 0; r <= std::max<uint32_t>(reps, 1u) && rc == PHR_OK; ++r) {
+                hipEventRecord(e0, s);
+                rc = conv_fwd(k, li, in[li], li == OUT ? nullptr : k->act[li], li == OUT ? k->logits : nullptr);
+                hipEventRecord(e1, s);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = PHR_ERR_HIP;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (r > 0) mine = std::min(mine, ms);  // the first launch warms the variant
+            }
+            if (mine < best) best = mine, pick = rw;
+        }
+        k->rw[li] = pick;
+        if (chosen) chosen[li] = pick;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 int phr_kpcn_train_step(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b, uint32_t w, uint32_t h, float lr,

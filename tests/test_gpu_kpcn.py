@@ -177,3 +177,16 @@ def test_recorded_frame_with_the_learned_denoiser(gpu):
         tr.sample(cam, w, h, 1)
         tr.replay_frame(net)
         assert np.array_equal(tr.radiance(), want)
+
+
+def test_autotuned_tiles_give_the_same_output(gpu):
+    fr, _ = R.synthetic_frames(1, 72, 96, 17)
+    net = P.Kpcn(gpu, _params(9))
+    f = _upload(gpu, fr)
+    a = P.GpuArray.zeroed(gpu, np.float32, 72 * 96 * 3)
+    b = P.GpuArray.zeroed(gpu, np.float32, 72 * 96 * 3)
+    net.infer(f, 1, 96, 72, a)
+    rows = net.autotune(f, 1, 96, 72, reps=1)
+    assert set(rows.tolist()) <= {1, 2}
+    net.infer(f, 1, 96, 72, b)
+    assert np.array_equal(a.read(), b.read())
