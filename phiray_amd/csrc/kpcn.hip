@@ -30,11 +30,13 @@
 // image is padded to multiples of 8 by repeating its edge.
 //
 // Kernels are hand-written for gfx950:
-//  * conv: an implicit GEMM on v_mfma_f32_16x16x32_bf16 (16x16x16 for a last
-//    16-channel chunk). D[co][pixel] = W[co][tap, ci] * X[tap, ci][pixel]. A
-//    block owns a 32-column tile and all output channels; the input tile and
-//    its halo are staged in LDS. The same kernel computes the input gradient,
-//    with the weights flipped and transposed.
+//  * conv: an implicit GEMM on v_mfma_f32_16x16x32_bf16. D[co][pixel] =
+//    W[co][tap, ci] * X[tap, ci][pixel]. A block owns a 32-column tile and all
+//    output channels; the input tile and its halo are staged in LDS, read
+//    straight from the finer level (2x2 average) or from the coarser output
+//    beside the skip (upsampled concatenation), so the pooled and concatenated
+//    tensors exist only for training. The same kernel computes the input
+//    gradient, with the weights flipped and transposed.
 //  * wgrad: dW[tap][co][ci] = sum_p dZ[p][co] X[p + off(tap)][ci]. The pixel
 //    tiles are staged transposed in LDS, so the reduction over pixels is the
 //    MFMA K dimension. Each block writes a partial for its pixel chunk, and a
@@ -122,14 +124,46 @@ __global__ __launch_bounds__(256) void k_features(const float* __restrict__ fr, 
 
 // ---- 3x3 conv as an implicit GEMM ---------------------------------------------
 struct ConvArgs {
-    const uint16_t* x;    // [P][cin]
+    const uint16_t* x;    // [P][cin] (see the input forms below)
     const uint16_t* w;    // [9][cout][cin]
     const float* bias;    // [cout] or null
     const uint16_t* mask; // [P][cout] or null: output zeroed where mask <= 0 (ReLU')
     uint16_t* y16;        // [P][cout] or null
     float* y32;           // [P][cout] or null
     uint32_t B, H, W, cin, cout, relu;
+    // Input forms, read as they are staged (the standalone kernels' bits):
+    //  pooled: x is [B][2H][2W][cin]; the input is each 2x2 block's average;
+    //  xc:     the input's first c1 channels are xc [B][H/2][W/2][c1] at
+    //          (y/2, x/2) (nearest upsampling), the rest x [B][H][W][cin - c1].
+    uint32_t pooled;
+    const uint16_t* xc;
+    uint32_t c1;
 };
+
+// Eight input channels [c, c + 8) of pixel (gy, gx) of image img, in the
+// input form `a` describes.
+__device__ __forceinline__ uint4 conv_input(const ConvArgs& a, uint64_t b, int gy, int gx, uint32_t c) {
+    if (a.pooled) {  // k_pool's sum order and rounding
+        const uint64_t W2 = 2ull * a.W;
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const uint4 v = *(const uint4*)(a.x + ((b * 2 * a.H + 2 * gy + dy) * W2 + 2 * gx + dx) * a.cin + c);
+                const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+                for (int e = 0; e < 4; ++e) {
+                    acc[2 * e] += from_bf16((uint16_t)(w4[e] & 0xffffu));
+                    acc[2 * e + 1] += from_bf16((uint16_t)(w4[e] >> 16));
+                }
+            }
+        uint32_t o[4];
+        for (int e = 0; e < 4; ++e) o[e] = to_bf16(acc[2 * e] * 0.25f) | (uint32_t)to_bf16(acc[2 * e + 1] * 0.25f) << 16;
+        return make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    if (a.xc && c < a.c1)
+        return *(const uint4*)(a.xc + ((b * (a.H / 2) + gy / 2) * (a.W / 2) + gx / 2) * a.c1 + c);
+    if (a.xc) return *(const uint4*)(a.x + ((b * a.H + gy) * (uint64_t)a.W + gx) * (a.cin - a.c1) + (c - a.c1));
+    return *(const uint4*)(a.x + ((b * a.H + gy) * (uint64_t)a.W + gx) * a.cin + c);
+}
 
 // A block computes a tile of 4 RW rows x 32 columns of one image, all output
 // channels. The input tile with its one-pixel halo is staged in LDS 32
@@ -171,8 +205,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
             const uint32_t pix = u / parts, part = u % parts;
             const int gy = y0 + (int)(pix / HWD) - 1, gx = x0 + (int)(pix % HWD) - 1;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *(const uint4*)(a.x + (img + (uint64_t)gy * a.W + gx) * cin + c0 + 8 * part);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = conv_input(a, b, gy, gx, c0 + 8 * part);
             *(uint4*)(tile + pix * HALO_STRIDE + 8 * part) = v;
         }
         __syncthreads();
@@ -592,6 +625,9 @@ void free_ws(phr_kpcn* k) {
     k->ws = nullptr;
     k->ws_bytes = 0;
     k->B = k->H = k->W = 0;
+    k->Q0 = k->Q1 = k->Q2 = k->U2 = k->U1 = k->U0 = nullptr;
+    for (int i = 0; i < NLAYERS; ++i) k->dz[i] = nullptr;
+    k->G = k->Gs0 = k->Gs1 = k->Gs2 = k->part = k->bpart = nullptr;
 }
 
 int ensure_ws(phr_kpcn* k, uint32_t B, uint32_t H, uint32_t W, bool train) {
@@ -609,16 +645,16 @@ int ensure_ws(phr_kpcn* k, uint32_t B, uint32_t H, uint32_t W, bool train) {
     auto P = [&](uint32_t l) { return level_pixels(k, l); };
     add((void**)&k->F, P(0) * FEAT * 2);
     for (int i = 0; i < NLAYERS - 1; ++i) add((void**)&k->act[i], P(LAYERS[i].level) * LAYERS[i].cout * 2);
-    add((void**)&k->Q0, P(1) * 32 * 2);
-    add((void**)&k->Q1, P(2) * 48 * 2);
-    add((void**)&k->Q2, P(3) * 64 * 2);
-    add((void**)&k->U2, P(2) * 160 * 2);
-    add((void**)&k->U1, P(1) * 112 * 2);
-    add((void**)&k->U0, P(0) * 80 * 2);
     add((void**)&k->logits, P(0) * LOGITS * 4);
     k->loss_n = grid_of(P(0));
     add((void**)&k->loss_part, k->loss_n * 4);
-    if (train) {
+    if (train) {  // the pooled and concatenated inputs, whole, for the weight gradients
+        add((void**)&k->Q0, P(1) * 32 * 2);
+        add((void**)&k->Q1, P(2) * 48 * 2);
+        add((void**)&k->Q2, P(3) * 64 * 2);
+        add((void**)&k->U2, P(2) * 160 * 2);
+        add((void**)&k->U1, P(1) * 112 * 2);
+        add((void**)&k->U0, P(0) * 80 * 2);
         for (int i = 0; i < NLAYERS; ++i) add((void**)&k->dz[i], P(LAYERS[i].level) * LAYERS[i].cout * 2);
         uint64_t g = 0;
         for (int i = 1; i < NLAYERS; ++i) g = std::max<uint64_t>(g, P(LAYERS[i].level) * LAYERS[i].cin);
@@ -690,8 +726,10 @@ int launch_conv(const ConvArgs& a, hipStream_t s, int rw = 0) {
     return PHR_OK;
 }
 
-// Forward conv of layer `li` at its level.
-int conv_fwd(phr_kpcn* k, int li, const uint16_t* x, uint16_t* y16, float* y32) {
+// Forward conv of layer `li` at its level; `pooled` / (`xc`, `c1`) select the
+// fused input forms (ConvArgs).
+int conv_fwd(phr_kpcn* k, int li, const uint16_t* x, uint16_t* y16, float* y32, uint32_t pooled = 0,
+             const uint16_t* xc = nullptr, uint32_t c1 = 0) {
     const LayerDef& L = LAYERS[li];
     ConvArgs a{};
     a.x = x;
@@ -702,6 +740,7 @@ int conv_fwd(phr_kpcn* k, int li, const uint16_t* x, uint16_t* y16, float* y32) 
     a.y32 = y32;
     a.B = k->B, a.H = k->Hp >> L.level, a.W = k->Wp >> L.level;
     a.cin = L.cin, a.cout = L.cout, a.relu = L.relu;
+    a.pooled = pooled, a.xc = xc, a.c1 = c1;
     return launch_conv(a, k->gpu->stream, k->rw[li]);
 }
 
@@ -775,30 +814,44 @@ int upcat(phr_kpcn* k, const uint16_t* coarse, const uint16_t* skip, uint16_t* o
         if (rc_ != PHR_OK) return rc_; \
     } while (0)
 
-// Forward to the logits.
-int forward(phr_kpcn* k, const float* frames) {
+// Layer li's forward conv with its input as the network reads it: pooled for
+// the first conv of levels 1-3, the coarser output upsampled beside the skip
+// for the decoders.
+int conv_layer(phr_kpcn* k, int li) {
+    uint16_t** A = k->act;
+    switch (li) {
+        case E0A: return conv_fwd(k, E0A, k->F, A[E0A], nullptr);
+        case E0B: return conv_fwd(k, E0B, A[E0A], A[E0B], nullptr);
+        case E1A: return conv_fwd(k, E1A, A[E0B], A[E1A], nullptr, 1);
+        case E1B: return conv_fwd(k, E1B, A[E1A], A[E1B], nullptr);
+        case E2A: return conv_fwd(k, E2A, A[E1B], A[E2A], nullptr, 1);
+        case E2B: return conv_fwd(k, E2B, A[E2A], A[E2B], nullptr);
+        case E3: return conv_fwd(k, E3, A[E2B], A[E3], nullptr, 1);
+        case D2: return conv_fwd(k, D2, A[E2B], A[D2], nullptr, 0, A[E3], 96);
+        case D1: return conv_fwd(k, D1, A[E1B], A[D1], nullptr, 0, A[D2], 64);
+        case D0: return conv_fwd(k, D0, A[E0B], A[D0], nullptr, 0, A[D1], 48);
+        default: return conv_fwd(k, OUT, A[D0], nullptr, k->logits);
+    }
+}
+
+// Forward to the logits. The pooled and concatenated inputs are read by the
+// convolutions themselves; training also keeps them whole for the weight
+// gradients.
+int forward(phr_kpcn* k, const float* frames, bool keep) {
     hipStream_t s = k->gpu->stream;
     hipLaunchKernelGGL(k_features, dim3(grid_of(level_pixels(k, 0))), dim3(256), 0, s, frames, k->B, k->H, k->W, k->Hp,
                        k->Wp, k->F);
     PHR_HIP(hipGetLastError());
     uint16_t** A = k->act;
-    KP_TRY(conv_fwd(k, E0A, k->F, A[E0A], nullptr));
-    KP_TRY(conv_fwd(k, E0B, A[E0A], A[E0B], nullptr));
-    KP_TRY(pool(k, A[E0B], k->Q0, 1, 32));
-    KP_TRY(conv_fwd(k, E1A, k->Q0, A[E1A], nullptr));
-    KP_TRY(conv_fwd(k, E1B, A[E1A], A[E1B], nullptr));
-    KP_TRY(pool(k, A[E1B], k->Q1, 2, 48));
-    KP_TRY(conv_fwd(k, E2A, k->Q1, A[E2A], nullptr));
-    KP_TRY(conv_fwd(k, E2B, A[E2A], A[E2B], nullptr));
-    KP_TRY(pool(k, A[E2B], k->Q2, 3, 64));
-    KP_TRY(conv_fwd(k, E3, k->Q2, A[E3], nullptr));
-    KP_TRY(upcat(k, A[E3], A[E2B], k->U2, 2, 96, 64));
-    KP_TRY(conv_fwd(k, D2, k->U2, A[D2], nullptr));
-    KP_TRY(upcat(k, A[D2], A[E1B], k->U1, 1, 64, 48));
-    KP_TRY(conv_fwd(k, D1, k->U1, A[D1], nullptr));
-    KP_TRY(upcat(k, A[D1], A[E0B], k->U0, 0, 48, 32));
-    KP_TRY(conv_fwd(k, D0, k->U0, A[D0], nullptr));
-    KP_TRY(conv_fwd(k, OUT, A[D0], nullptr, k->logits));
+    for (int li = 0; li < NLAYERS; ++li) KP_TRY(conv_layer(k, li));
+    if (keep) {
+        KP_TRY(pool(k, A[E0B], k->Q0, 1, 32));
+        KP_TRY(pool(k, A[E1B], k->Q1, 2, 48));
+        KP_TRY(pool(k, A[E2B], k->Q2, 3, 64));
+        KP_TRY(upcat(k, A[E3], A[E2B], k->U2, 2, 96, 64));
+        KP_TRY(upcat(k, A[D2], A[E1B], k->U1, 1, 64, 48));
+        KP_TRY(upcat(k, A[D1], A[E0B], k->U0, 0, 48, 32));
+    }
     return PHR_OK;
 }
 
@@ -996,7 +1049,7 @@ int phr_kpcn_get_grads(phr_kpcn* k, float* host, uint64_t count) {
 // Device-pointer form of inference, shared with the tracer.
 int phr_kpcn_infer_dev(phr_kpcn* k, const float* frames, uint32_t b, uint32_t w, uint32_t h, float* out) {
     KP_TRY(ensure_ws(k, b, h, w, false));
-    KP_TRY(forward(k, frames));
+    KP_TRY(forward(k, frames, false));
     return apply(k, frames, out, nullptr, false);
 }
 
@@ -1014,7 +1067,7 @@ int phr_kpcn_loss(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32_t b
     KP_TRY(check_shape(frames, b, w, h, targets, "phr_kpcn_loss"));
     if (out) KP_TRY(check_shape(frames, b, w, h, out, "phr_kpcn_loss"));
     KP_TRY(ensure_ws(k, b, h, w, false));
-    KP_TRY(forward(k, (const float*)frames->ptr));
+    KP_TRY(forward(k, (const float*)frames->ptr, false));
     KP_TRY(apply(k, (const float*)frames->ptr, out ? (float*)out->ptr : nullptr, (const float*)targets->ptr, false));
     return read_loss(k, loss);
 }
@@ -1024,7 +1077,7 @@ int phr_kpcn_backward(phr_kpcn* k, phr_array* frames, phr_array* targets, uint32
     PHR_CHECK(k && targets, PHR_ERR_ARG, "phr_kpcn_backward: null");
     KP_TRY(check_shape(frames, b, w, h, targets, "phr_kpcn_backward"));
     KP_TRY(ensure_ws(k, b, h, w, true));
-    KP_TRY(forward(k, (const float*)frames->ptr));
+    KP_TRY(forward(k, (const float*)frames->ptr, true));
     KP_TRY(apply(k, (const float*)frames->ptr, nullptr, (const float*)targets->ptr, true));
     KP_TRY(backward(k));
     if (loss) return read_loss(k, loss);
@@ -1051,12 +1104,11 @@ int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, ui
     PHR_CHECK(k, PHR_ERR_ARG, "phr_kpcn_autotune: null");
     KP_TRY(check_shape(frames, b, w, h, nullptr, "phr_kpcn_autotune"));
     KP_TRY(ensure_ws(k, b, h, w, false));
-    KP_TRY(forward(k, (const float*)frames->ptr));  // every layer's input in place
+    KP_TRY(forward(k, (const float*)frames->ptr, false));  // every layer's input in place
     hipStream_t s = k->gpu->stream;
     hipEvent_t e0, e1;
     PHR_HIP(hipEventCreate(&e0));
     PHR_HIP(hipEventCreate(&e1));
-    const uint16_t* in[NLAYERS] = {k->F, k->act[E0A], k->Q0, k->act[E1A], k->Q1, k->act[E2A], k->Q2, k->U2, k->U1, k->U0, k->act[D0]};
     int rc = PHR_OK;
     for (int li = 0; li < NLAYERS && rc == PHR_OK; ++li) {
         float best = 3.0e38f;
@@ -1067,7 +1119,7 @@ int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, ui
             float mine = 3.0e38f;
             for (uint32_t r = 0; r <= std::max<uint32_t>(reps, 1u) && rc == PHR_OK; ++r) {
                 hipEventRecord(e0, s);
-                rc = conv_fwd(k, li, in[li], li == OUT ? nullptr : k->act[li], li == OUT ? k->logits : nullptr);
+                rc = conv_layer(k, li);
                 hipEventRecord(e1, s);
                 if (hipEventSynchronize(e1) != hipSuccess) rc = PHR_ERR_HIP;
                 float ms = 0.f;
@@ -1116,7 +1168,7 @@ int phr_kpcn_train_indexed(phr_kpcn* k, phr_array* frames, const uint32_t* fidx,
                                hipMemcpyDeviceToDevice, k->gpu->stream));
     }
     KP_TRY(ensure_ws(k, b, h, w, true));
-    KP_TRY(forward(k, (const float*)bf));
+    KP_TRY(forward(k, (const float*)bf, true));
     KP_TRY(apply(k, (const float*)bf, nullptr, (const float*)bt, true));
     KP_TRY(backward(k));
     KP_TRY(phr_kpcn_adam(k, lr));
