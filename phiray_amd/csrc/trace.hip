@@ -2007,7 +2007,12 @@ int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at) {
 const void* phr_kpcn_internal_workspace(const phr_kpcn* k);
 
 namespace {
-bool same_scene(const SceneDev& a, const SceneDev& b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+// Field by field: the struct's tail padding is never written, so a byte
+// compare of two copies can differ where the scenes do not.
+bool same_scene(const SceneDev& a, const SceneDev& b) {
+    return a.nodes == b.nodes && a.tris == b.tris && a.faces == b.faces && a.verts == b.verts && a.mats == b.mats &&
+           a.lights == b.lights && a.insts == b.insts && a.ninsts == b.ninsts;
+}
 }  // namespace
 
 // One frame as a hipGraph (no counterpart in phiray, whose Pass replays per
