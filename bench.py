@@ -38,15 +38,17 @@ HBM_PEAK_GBS = 8000.0
 RAY_BYTES, HIT_BYTES, ROOT_BYTES, STEP_BYTES, TEST_BYTES = 32, 8, 32, 64, 48
 
 
-def profiled_traffic(name: str, spp: int, w: int, h: int) -> tuple[float | None, str | None]:
+def profiled_traffic(name: str, spp: int, w: int, h: int, waves: int) -> tuple[float | None, str | None]:
     """HBM bytes of one k_path launch from the committed counter summary, when
-    it was measured on this same workload (FETCH_SIZE + WRITE_SIZE, each
-    scaled by what the counter reads for a copy of known size)."""
+    it was measured on this same workload and kernel variant (FETCH_SIZE +
+    WRITE_SIZE, each scaled by what the counter reads for a copy of known size)."""
     f = ROOT / "profiles" / name
     if not f.exists():
         return None, None
-    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", f.read_text())
-    if not m or (spp, w, h) != (64, 1920, 1080):
+    text = f.read_text()
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", text)
+    mw = re.search(r"path_waves_per_simd:\s*(\d+)", text)
+    if not m or not mw or int(mw.group(1)) != waves or (spp, w, h) != (64, 1920, 1080):
         return None, None
     return float(m.group(1)), f"profiles/{name}"
 
@@ -124,6 +126,43 @@ def batch_rates(gpu: P.Gpu, reps: int) -> dict:
     return {"scenes": len(seg_lo) - 1, "tris": n, "build_ms": tb * 1e3, "build_mtri_s": n / tb / 1e6,
             "build_gbs": build_bytes / tb / 1e9, "refit_ms": tr * 1e3, "refit_mtri_s": n / tr / 1e6,
             "max_depth": b.depth}
+
+
+def batch_render_rates(gpu: P.Gpu, scenes: int = 256, w: int = 256, h: int = 256, spp: int = 16, reps: int = 3) -> dict:
+    """The 256 scenes of uneven size (1k-200k triangles each) built by one
+    segmented LBVH build and rendered at 256x256, 16 spp, 4 bounces each, in
+    one launch; the whole pass (boxes, build, slot order, render) recorded as
+    one hipGraph and replayed. The scenes hold no emitter: the world colour
+    (1, 1, 1) lights them. Camera at (0.5, 0.5, -1.5) looking +z, 45 degrees."""
+    tris, seg_lo = S.batch_tris(scenes)
+    t0 = time.perf_counter()
+    b = P.Batch(gpu, tris, seg_lo)
+    setup_s = time.perf_counter() - t0
+    del tris
+    cam = S.path_camera(w, h, eye=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), rr_from=3)
+    cam["env"] = (1.0, 1.0, 1.0, 0.0)
+    waves, grouped, _ = b.autotune(cam, w, h, 4, reps=1)
+    log(f"batch render: kernel at {waves} waves a SIMD, grouped order {grouped}")
+    build_s = sorted(b.rebuild() for _ in range(5))[2]
+    b.render(cam, w, h, spp)
+    c0 = b.counters()
+    render_s = [b.render(cam, w, h, spp) for _ in range(reps)]
+    c1 = b.counters()
+    b.record(cam, w, h, spp, rebuild=True)
+    b.replay()
+    pass_s = [b.replay() for _ in range(reps)]
+    rays = int((c1[0] - c0[0]) + (c1[1] - c0[1])) / reps
+    steps, tests = int(c1[2] - c0[2]) / reps, int(c1[3] - c0[3]) / reps
+    t_render = float(np.median(render_s))
+    nbytes = rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES
+    n = int(seg_lo[-1])
+    return {"scenes": scenes, "tris": n, "image": f"{w}x{h}", "spp": spp, "bounces": 4, "light": "world colour 1.0",
+            "setup_s": setup_s, "path_waves_per_simd": waves, "grouped": grouped,
+            "build_ms": build_s * 1e3, "build_mtri_s": n / build_s / 1e6,
+            "render_ms": t_render * 1e3, "rays_per_pass": rays, "mrays_s": rays / t_render / 1e6,
+            "render_gbs": nbytes / t_render / 1e9, "render_hbm_frac": nbytes / t_render / 1e9 / HBM_PEAK_GBS,
+            "steps_per_ray": steps / max(rays, 1), "tests_per_ray": tests / max(rays, 1),
+            "graph_pass_ms": float(np.median(pass_s)) * 1e3}
 
 
 def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
@@ -246,6 +285,7 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
         tr.sample(ref_cam, w, h, 64)
     ref = tr.radiance()
     log(f"convergence: {ref_spp}-spp reference in {time.perf_counter() - t0:.1f} s")
+    waves, _ = tr.autotune(cam, w, h, 1, reps=3)  # the trials launch a few samples at a time
     clock = P.Clock(gpu)
     out = P.GpuArray.zeroed(gpu, np.float32, w * h * 3) if net is not None else None
     tiles = None
@@ -257,10 +297,13 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
         tr.denoise_learned(net, out)
     trials, best = [], {"raw": None, "plain": None, "learned": None}
     for spp in (1, 2, 4, 8, 16, 32, 64, 128, 256):
-        fresh = cam.copy()
-        fresh["stream"] = spp  # a new camera record: the image starts again
-        tr.sample(fresh, w, h, spp)
-        render_s = tr.time()[1]
+        times = []
+        for rep in range(3):  # the median of three renders; the last one is scored
+            fresh = cam.copy()
+            fresh["stream"] = spp + (rep << 16)  # a new camera record: the image starts again
+            tr.sample(fresh, w, h, spp)
+            times.append(tr.time()[1])
+        render_s = float(np.median(times))
         raw = rel_mse(tr.radiance(), ref)
         trial = {"spp": spp, "render_s": render_s, "relmse_raw": raw}
         if net is not None:
@@ -286,12 +329,62 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
         if all(v is not None for k, v in best.items() if net is not None or k != "learned"):
             break
     res = {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
-           "plain": best["plain"], "learned": best["learned"], "trials": trials, "kpcn_rows_per_wave": tiles}
+           "plain": best["plain"], "learned": best["learned"], "trials": trials, "kpcn_rows_per_wave": tiles,
+           "path_waves_per_simd": waves}
     if best["learned"] and best["raw"]:
         res["raw_over_learned"] = best["raw"]["s_per_frame"] / best["learned"]["s_per_frame"]
     if best["learned"] and best["plain"]:
         res["plain_over_learned"] = best["plain"]["s_per_frame"] / best["learned"]["s_per_frame"]
     return res
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def kpcn_rates(gpu: P.Gpu, net: "P.Kpcn | None", reps: int = 10) -> dict:
+    """The learned denoiser alone: inference on one 1920x1080 frame (conv
+    tiles autotuned) and one Adam training step over 8 crops of 512x512, on
+    random frames. FLOPs = 2 x 9 x cin x cout per pixel of each conv at its
+    level (a quarter of the pixels a level down); a training step does the
+    forward, the input gradients and the weight gradients (3x)."""
+    from phiray_amd import kpcn as K
+
+    if net is None:
+        net = P.Kpcn(gpu, K.init_params(0))
+    rng = np.random.default_rng(0)
+    lay = P.Kpcn.layers()
+    flop_px = float(sum(2 * 9 * int(c[0]) * int(c[1]) / 4 ** int(c[2]) for c in lay))
+    clock = P.Clock(gpu)
+    w, h = 1920, 1080
+    fr = P.GpuArray.from_(gpu, rng.uniform(0.05, 1.0, size=h * w * P.Kpcn.FRAME).astype(np.float32))
+    out = P.GpuArray.zeroed(gpu, np.float32, w * h * 3)
+    rows = net.autotune(fr, 1, w, h).tolist()
+    for _ in range(3):
+        net.infer(fr, 1, w, h, out)
+    clock.start()
+    for _ in range(reps):
+        net.infer(fr, 1, w, h, out)
+    t_inf, _ = clock.stop()
+    t_inf /= reps
+    del fr, out
+    b, c = 8, 512
+    fb = P.GpuArray.from_(gpu, rng.uniform(0.05, 1.0, size=b * c * c * P.Kpcn.FRAME).astype(np.float32))
+    tb = P.GpuArray.from_(gpu, rng.uniform(0.05, 1.0, size=b * c * c * 3).astype(np.float32))
+    trainer = P.Kpcn(gpu, K.init_params(1))
+    for _ in range(3):
+        trainer.train_step(fb, tb, b, c, c, 1e-4, want_loss=False)
+    gpu.wait()
+    clock.start()
+    for _ in range(reps):
+        trainer.train_step(fb, tb, b, c, c, 1e-4, want_loss=False)
+    t_step, _ = clock.stop()
+    t_step /= reps
+    inf_tflops = flop_px * w * h / t_inf / 1e12
+    train_tflops = 3 * flop_px * b * c * c / t_step / 1e12
+    return {"params": int(net.param_count()), "flop_per_px": flop_px, "rows_per_wave": rows,
+            "infer_1080p_ms": t_inf * 1e3, "infer_tflops": inf_tflops, "infer_mfma_frac": inf_tflops / MFMA_BF16_PEAK_TFLOPS,
+            "train_step_8x512_ms": t_step * 1e3, "train_tflops": train_tflops,
+            "train_mfma_frac": train_tflops / MFMA_BF16_PEAK_TFLOPS, "mfma_peak_tflops": MFMA_BF16_PEAK_TFLOPS}
 
 
 def load_net(gpu: P.Gpu) -> "P.Kpcn | None":
@@ -364,7 +457,8 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge"], default="all")
+    ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge", "batch", "kpcn"], default="all")
+    ap.add_argument("--path-waves", type=int, default=0, help="run this path kernel variant (2, 3, 4) instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -413,6 +507,17 @@ def main() -> int:
             log(f"{key} failed: {type(e).__name__}: {e}")
             out[key] = {"error": f"{type(e).__name__}: {e}"}
 
+    if args.only in ("all", "batch"):
+        log("batch of 256 scenes: build + render")
+        leg("batch_256_render", lambda: batch_render_rates(gpu))
+        if "error" not in out["batch_256_render"]:
+            r = out["batch_256_render"]
+            log(f"batch render: {r['render_ms']:.1f} ms, {r['mrays_s']:.1f} Mrays/s; graph pass {r['graph_pass_ms']:.1f} ms")
+    if args.only in ("all", "kpcn"):
+        log("learned denoiser: inference and a training step")
+        leg("kpcn", lambda: kpcn_rates(gpu, load_net(gpu)))
+        if "error" not in out["kpcn"]:
+            log(f"kpcn: infer 1080p {out['kpcn']['infer_1080p_ms']:.2f} ms, train step {out['kpcn']['train_step_8x512_ms']:.2f} ms")
     if args.only in ("all", "converge"):
         log("spp to relMSE <= 0.01")
         leg("s_per_frame_to_relmse_0.01", lambda: convergence(gpu, tris, S.path_camera(args.width, args.height, rr_from=3),
@@ -436,11 +541,15 @@ def main() -> int:
         t0 = time.perf_counter()
         tracer = P.Tracer.new(gpu, tris, 4)
         log(f"Tracer::new {time.perf_counter() - t0:.2f} s, depth {tracer.depth}, {tracer.glow} emitters")
-        try:
-            waves, tune_s = tracer.autotune(cam, w, h, 8, reps=2)
-            log(f"autotune: path kernel at {waves} waves a SIMD ({tune_s * 1e3:.1f} ms for 8 spp)")
-        except P.PhrError as e:
-            log(f"autotune failed ({e}); the path kernel stays at {tracer.path_waves} waves a SIMD")
+        if args.path_waves:
+            tracer.set_path_waves(args.path_waves)
+            log(f"path kernel at {args.path_waves} waves a SIMD (chosen on the command line)")
+        else:
+            try:
+                waves, tune_s = tracer.autotune(cam, w, h, 8, reps=2)
+                log(f"autotune: path kernel at {waves} waves a SIMD ({tune_s * 1e3:.1f} ms for 8 spp)")
+            except P.PhrError as e:
+                log(f"autotune failed ({e}); the path kernel stays at {tracer.path_waves} waves a SIMD")
         for i in range(args.warmup):
             tracer.sample(cam, w, h, spp)
             log(f"warmup frame {i}: {tracer.time()[1]:.3f} s")
@@ -462,7 +571,7 @@ def main() -> int:
         kernel_s = float(np.mean(gpu_s))
         nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
         achieved = nbytes / kernel_s / 1e9
-        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h)
+        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h, tracer.path_waves)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
                             "algorithmic_bytes": nbytes, "kernel": "k_path",

@@ -302,6 +302,41 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t);
 int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,
                         uint32_t* chosen, double* best_s);
 
+/* ---- a batch of scenes (no counterpart in phiray, which holds one Scene a
+ * Tracer, src/trace/mod.rs:50-245): many scenes of uneven size built by one
+ * segmented LBVH build (the segments of phr_bvh_build_batch) and traced by
+ * one launch. Scene k renders exactly as a Tracer over its triangles alone
+ * (phr_tracer_new) under the same camera. ----------------------------------- */
+typedef struct phr_batch phr_batch;
+/* Scene k is tris[seg_lo[k], seg_lo[k + 1]) (128-B Tri records as
+ * phr_tracer_new takes them); seg_lo[0] = 0 and no scene is empty. */
+int phr_batch_new(phr_gpu* g, const void* tris, const uint32_t* seg_lo, uint32_t scenes, uint32_t leaf, phr_batch** out);
+void phr_batch_free(phr_batch* b);
+uint32_t phr_batch_scenes(const phr_batch* b);
+uint32_t phr_batch_len(const phr_batch* b);
+phr_bvh* phr_batch_bvh(phr_batch* b);
+float phr_batch_extent(const phr_batch* b, uint32_t k);
+uint32_t phr_batch_glow(const phr_batch* b, uint32_t k);
+/* Boxes, the segmented build and the slot order again, on the device. */
+int phr_batch_rebuild(phr_batch* b, double* gpu_s);
+/* Every scene, spp samples of a fresh w x h image each, in one launch. */
+int phr_batch_render(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, double* gpu_s);
+/* The render kernel: waves a SIMD (2, 3, 4) and block order (grouped 1: the
+ * blocks of one XCD cover whole scenes; needs a multiple of 8 scenes). */
+int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped);
+int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped);
+/* Time every (waves, order) pair on this render and keep the fastest. */
+int phr_batch_autotune(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps, uint32_t* waves,
+                       uint32_t* grouped, double* best_s);
+/* The whole pass (rebuild 1: build, then render) as one hipGraph; kernels
+ * only. Replays are refused (PHR_ERR_ARG) once the images moved. */
+int phr_batch_record(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, int rebuild);
+int phr_batch_replay(phr_batch* b, double* gpu_s);
+/* Scene k's image words [from, from + count): its own PX_LEN * w * h + 1. */
+int phr_batch_px(phr_batch* b, uint32_t k, uint64_t from, uint64_t count, uint32_t* out);
+/* Since creation: closest rays, shadow rays, inner-node steps, triangle tests. */
+int phr_batch_counters(phr_batch* b, uint64_t* out4);
+
 /* ---- the learned denoiser (no counterpart in phiray: it stands beside
  * kernel::Denoise, src/kernel/denoise.rs:26-64) ------------------------------
  * A kernel-predicting convolutional network: a 4-level encoder/decoder of 3x3

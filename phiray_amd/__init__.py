@@ -200,6 +200,22 @@ _SIGS = {
     "phr_tracer_path_waves": (_U32, [_P]),
     "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_tracer_denoise_learned": (_I, [_P, _P, _P]),
+    "phr_batch_new": (_I, [_P, _P, _P, _U32, _U32, C.POINTER(_P)]),
+    "phr_batch_free": (None, [_P]),
+    "phr_batch_scenes": (_U32, [_P]),
+    "phr_batch_len": (_U32, [_P]),
+    "phr_batch_bvh": (_P, [_P]),
+    "phr_batch_extent": (C.c_float, [_P, _U32]),
+    "phr_batch_glow": (_U32, [_P, _U32]),
+    "phr_batch_rebuild": (_I, [_P, C.POINTER(C.c_double)]),
+    "phr_batch_render": (_I, [_P, _P, _U32, _U32, _U32, C.POINTER(C.c_double)]),
+    "phr_batch_set_path": (_I, [_P, _U32, _U32]),
+    "phr_batch_path": (_I, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "phr_batch_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(C.c_double)]),
+    "phr_batch_record": (_I, [_P, _P, _U32, _U32, _U32, _I]),
+    "phr_batch_replay": (_I, [_P, C.POINTER(C.c_double)]),
+    "phr_batch_px": (_I, [_P, _U32, _U64, _U64, _P]),
+    "phr_batch_counters": (_I, [_P, C.POINTER(_U64)]),
 }
 
 _lib = None
@@ -858,6 +874,110 @@ class Tracer:
     def free(self) -> None:
         if self._h:
             lib().phr_tracer_free(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenes in one segmented LBVH build and one path launch (no
+    counterpart in phiray, which holds one Scene a Tracer): scene k is
+    tris[seg_lo[k]:seg_lo[k+1]] and renders as `Tracer.new` over those
+    triangles alone does."""
+
+    def __init__(self, gpu: Gpu, tris: np.ndarray, seg_lo, leaf: int = 4):
+        tris = np.ascontiguousarray(tris, dtype=TRI)
+        lo = np.ascontiguousarray(seg_lo, dtype=np.uint32)
+        self.gpu = gpu
+        self.seg_lo = lo
+        self.size = (0, 0)
+        out = C.c_void_p()
+        self._h = C.c_void_p()
+        _check(lib().phr_batch_new(gpu._h, _ptr(tris), _ptr(lo), len(lo) - 1, leaf, C.byref(out)), "Batch::new")
+        self._h = out
+
+    @property
+    def scenes(self) -> int:
+        return int(lib().phr_batch_scenes(self._h))
+
+    def __len__(self) -> int:
+        return int(lib().phr_batch_len(self._h))
+
+    def extent(self, k: int) -> float:
+        return float(lib().phr_batch_extent(self._h, k))
+
+    def glow(self, k: int) -> int:
+        return int(lib().phr_batch_glow(self._h, k))
+
+    def tree(self, k: int) -> Bvh:
+        """Scene k's tree relative to itself (what Bvh::build of its boxes gives)."""
+        b = GpuBvh(self.gpu, lib().phr_batch_bvh(self._h))
+        b._seg_lo = self.seg_lo
+        try:
+            return b.download_segment(k)
+        finally:
+            b._h = C.c_void_p()  # borrowed: the batch owns it
+
+    def rebuild(self) -> float:
+        """Boxes, the segmented build and the slot order again; device seconds."""
+        t = C.c_double(0)
+        _check(lib().phr_batch_rebuild(self._h, C.byref(t)), "Batch rebuild")
+        return float(t.value)
+
+    def render(self, cam: np.ndarray, w: int, h: int, spp: int) -> float:
+        """Every scene, spp samples of a fresh image, one launch; device seconds."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        t = C.c_double(0)
+        _check(lib().phr_batch_render(self._h, _ptr(cam), w, h, spp, C.byref(t)), "Batch render")
+        self.size = (max(w, 1), max(h, 1))
+        return float(t.value)
+
+    def set_path(self, waves: int, grouped: int) -> None:
+        _check(lib().phr_batch_set_path(self._h, waves, grouped), "Batch set_path")
+
+    def path(self) -> tuple[int, int]:
+        wv, g = C.c_uint32(0), C.c_uint32(0)
+        _check(lib().phr_batch_path(self._h, C.byref(wv), C.byref(g)), "Batch path")
+        return int(wv.value), int(g.value)
+
+    def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 1) -> tuple[int, int, float]:
+        """(waves a SIMD, grouped order, seconds) of the fastest render kernel."""
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        wv, g, t = C.c_uint32(0), C.c_uint32(0), C.c_double(0)
+        _check(lib().phr_batch_autotune(self._h, _ptr(cam), w, h, spp, reps, C.byref(wv), C.byref(g), C.byref(t)),
+               "Batch autotune")
+        self.size = (max(w, 1), max(h, 1))
+        return int(wv.value), int(g.value), float(t.value)
+
+    def record(self, cam: np.ndarray, w: int, h: int, spp: int, rebuild: bool = True) -> None:
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        _check(lib().phr_batch_record(self._h, _ptr(cam), w, h, spp, 1 if rebuild else 0), "Batch record")
+        self.size = (max(w, 1), max(h, 1))
+
+    def replay(self) -> float:
+        t = C.c_double(0)
+        _check(lib().phr_batch_replay(self._h, C.byref(t)), "Batch replay")
+        return float(t.value)
+
+    def px(self, k: int) -> np.ndarray:
+        """Scene k's whole image buffer (PX_LEN * w * h + 1 words)."""
+        w, h = self.size
+        out = np.zeros(PX_LEN * w * h + 1, dtype=np.uint32)
+        _check(lib().phr_batch_px(self._h, k, 0, len(out), _ptr(out)), "Batch px")
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = (_U64 * 4)()
+        _check(lib().phr_batch_counters(self._h, out), "Batch counters")
+        return np.array(list(out), dtype=np.uint64)
+
+    def free(self) -> None:
+        if self._h:
+            lib().phr_batch_free(self._h)
         self._h = C.c_void_p()
 
     def __del__(self):

@@ -74,6 +74,9 @@ struct SceneDev {
     // root 24, face range 25-26, flags 27), then the top-tree slot order.
     const uint32_t* insts;
     uint32_t ninsts;
+    // One tree: its root node (0 for a tracer's own tree; a batch of scenes
+    // shares one node array, each scene's tree at its own base).
+    uint32_t root;
 };
 
 namespace {
@@ -141,19 +144,19 @@ __device__ __forceinline__ float box_t(float4 lo, float4 hi, V3 o, V3 inv, float
     return (tmax >= e && tmin < best) ? e : PHR_INF;
 }
 
-// trace_blas for one identity instance, root at node 0. `stk` is this
+// trace_blas for one identity instance, root at node `root`. `stk` is this
 // thread's LDS stack column (stride = threads per block).
 // Any-hit rays skip triangles of glass or cut-out materials (trans > 0.5 or
 // alpha < 0.5, bvh.wgsl:158-160): `faces` and `mats` give the material.
 // `steps` and `tests` count inner-node visits and triangle tests, the units
 // the walk's memory traffic is priced in.
 template <bool ANY>
-__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, uint32_t root, const float4* __restrict__ tris,
                                        const uint4* __restrict__ faces, const float* __restrict__ mats, V3 o, V3 d, float tmax,
                                        uint32_t* stk, uint32_t stride, uint32_t& steps, uint32_t& tests) {
     PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
     V3 inv = safe_inv(d);
-    float4 lo = nodes[0], hi = nodes[1];
+    float4 lo = nodes[2 * root], hi = nodes[2 * root + 1];
     if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
     RaySpace rs = ray_space(d);
     uint32_t sp = 0;
@@ -336,8 +339,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const Ph
             h = any ? walk_forest<true>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests)
                     : walk_forest<false>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests);
         else
-            h = any ? walk<true>(sc.nodes, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
-                    : walk<false>(sc.nodes, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+            h = any ? walk<true>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
+                    : walk<false>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
         hits[i] = h;
     }
     if (counters) {
@@ -403,11 +406,11 @@ __device__ __forceinline__ PhrHit trace(Ctx& c, V3 o, V3 d, float tmax, bool fir
     if (first) {
         c.shadow++;
         if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
-        return walk<true>(c.s.nodes, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+        return walk<true>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
     }
     c.closest++;
     if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
-    return walk<false>(c.s.nodes, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+    return walk<false>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
 }
 template <bool F>
 __device__ __forceinline__ float visible(Ctx& c, V3 org, V3 l, float dist) {
@@ -826,18 +829,12 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
 __device__ __forceinline__ float ldf(const uint32_t* px, size_t w) { return __uint_as_float(px[w]); }
 __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = __float_as_uint(v); }
 
-// Samples [frame0, frame0 + spp) of every pixel, summed into px in sample
-// order (util.wgsl:17-34, 101-145: sums restart at frame 0).
-// WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
-// 3 and 4 trade a few spilled words for more waves in flight). The tracer's
-// autotune times the variants on the caller's workload and keeps the fastest.
-template <bool F, int WPE>
-__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
-    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x,
-                                                        uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                                        unsigned long long* __restrict__ rays, uint32_t listed,
-                                                        uint32_t tile0) {
-    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
+// Samples [frame0, frame0 + spp) of every pixel of tile `blk`, summed into px
+// in sample order (util.wgsl:17-34, 101-145: sums restart at frame 0).
+template <bool F>
+__device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* __restrict__ camp, uint32_t blk, uint32_t tiles_x,
+                                           uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+                                           unsigned long long* __restrict__ rays, uint32_t listed, uint32_t* s_stack) {
     const PhrCamera& cam = *camp;
     const size_t n = (size_t)cam.w * cam.h;
     // A block traces a 16 x 16 tile, a wave 16 x 4 pixels of it: neighbouring
@@ -846,7 +843,6 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     // (shade.wgsl:47-50); the list holds them tile by tile.
     uint32_t i = 0;
     bool inside;
-    const uint32_t blk = blockIdx.x + tile0;  // a slice of the launch starts at tile0
     if (listed) {
         const uint32_t k = blk * TRACE_THREADS + threadIdx.x;
         inside = k < min((size_t)px[34 * n], n);
@@ -924,6 +920,42 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         atomicAdd(&rays[2], st);
         atomicAdd(&rays[3], tt);
     }
+}
+
+// One image: a block a tile, from tile0 on (a slice of the tiles).
+// WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
+// 3 and 4 trade a few spilled words for more waves in flight). The tracer's
+// autotune times the variants on the caller's workload and keeps the fastest.
+template <bool F, int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+    unsigned long long* __restrict__ rays, uint32_t listed, uint32_t tile0) {
+    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
+    path_block<F>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, listed, s_stack);
+}
+
+// A batch of scenes in one launch: `tiles` blocks a scene, each scene with
+// its own tree (root), tables, camera and image (px_stride words apart), all
+// starting fresh. Workgroups go to the eight XCDs in turn (block b to XCD
+// b mod 8); with `grouped` (scenes a multiple of 8) the blocks one XCD
+// receives cover whole scenes, scene x + 8q on XCD x, so each XCD's L2 holds
+// the trees of the scenes it traces rather than a share of every one.
+template <int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_batch(
+    const SceneDev* __restrict__ scenes, const PhrCamera* __restrict__ cams, uint32_t tiles, uint32_t tiles_x, uint32_t spp,
+    uint32_t* __restrict__ px, uint64_t px_stride, unsigned long long* __restrict__ rays, uint32_t grouped) {
+    __shared__ uint32_t s_stack[stack_words<false>() * TRACE_THREADS];
+    uint32_t scene, tile;
+    if (grouped) {
+        const uint32_t x = blockIdx.x % 8u, q = blockIdx.x / 8u;
+        scene = x + 8u * (q / tiles);
+        tile = q % tiles;
+    } else {
+        scene = blockIdx.x / tiles;
+        tile = blockIdx.x % tiles;
+    }
+    const SceneDev sc = scenes[scene];
+    path_block<false>(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, 0u, s_stack);
 }
 
 
@@ -1202,14 +1234,15 @@ void tracer_release(phr_tracer* t) {
     delete t;
 }
 
-// The emitter table (src/trace/lamp.rs:51-97) over one identity instance,
-// from the slot order.
-int build_lights(phr_tracer* t, const std::vector<uint32_t>& in_faces, const std::vector<float>& verts,
-                 const std::vector<float>& mats) {
+// The emitter table over slots [lo, hi) (src/trace/lamp.rs:51-97): slot s
+// holds input face order[s]; faces of positive glow and area, their power
+// CDF. Words: count, then the slots, the instances (0), the CDF bits.
+std::vector<uint32_t> light_words(const uint32_t* order, uint32_t lo, uint32_t hi, const std::vector<uint32_t>& in_faces,
+                                  const std::vector<float>& verts, const std::vector<float>& mats) {
     std::vector<uint32_t> ids;
     std::vector<float> pw;
-    for (uint32_t s = 0; s < t->n; ++s) {
-        const uint32_t* fc = &in_faces[4 * (size_t)t->order[s]];
+    for (uint32_t s = lo; s < hi; ++s) {
+        const uint32_t* fc = &in_faces[4 * (size_t)order[s]];
         const float* e = &mats[MAT_WORDS * (size_t)fc[3] + 8];
         float glow = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
         if (glow <= 0.f) continue;
@@ -1236,7 +1269,14 @@ int build_lights(phr_tracer* t, const std::vector<uint32_t>& in_faces, const std
         run = run + pw[k];
         words[1 + 2 * m + k] = fbits(run / total);
     }
-    t->glow = m;
+    return words;
+}
+
+// A tracer's table: one identity instance, every slot.
+int build_lights(phr_tracer* t, const std::vector<uint32_t>& in_faces, const std::vector<float>& verts,
+                 const std::vector<float>& mats) {
+    std::vector<uint32_t> words = light_words(t->order.data(), 0, t->n, in_faces, verts, mats);
+    t->glow = words[0];
     if (t->lights) phr_array_release(t->lights);
     t->lights = nullptr;
     int rc = phr_array_alloc(t->gpu, 4, words.size(), &t->lights);
@@ -1256,6 +1296,7 @@ SceneDev scene_dev(phr_tracer* t) {
     s.verts = (const float4*)t->verts->ptr;
     s.mats = (const float*)t->mats->ptr;
     s.lights = (const uint32_t*)t->lights->ptr;
+    s.root = 0;
     return s;
 }
 
@@ -2011,7 +2052,7 @@ namespace {
 // compare of two copies can differ where the scenes do not.
 bool same_scene(const SceneDev& a, const SceneDev& b) {
     return a.nodes == b.nodes && a.tris == b.tris && a.faces == b.faces && a.verts == b.verts && a.mats == b.mats &&
-           a.lights == b.lights && a.insts == b.insts && a.ninsts == b.ninsts;
+           a.lights == b.lights && a.insts == b.insts && a.ninsts == b.ninsts && a.root == b.root;
 }
 }  // namespace
 
@@ -2378,5 +2419,453 @@ int phr_tracer_retree(phr_tracer* t, int* ok) {
 phr_array* phr_tracer_nodes(phr_tracer* t) { return t ? (t->forest ? t->nodes : phr_bvh_internal_nodes(t->bvh)) : nullptr; }
 phr_array* phr_tracer_instances(phr_tracer* t) { return t && t->forest ? t->insts : nullptr; }
 uint32_t phr_tracer_instance_count(const phr_tracer* t) { return t ? (t->forest ? t->ninsts : 1u) : 0u; }
+
+}  // extern "C"
+
+// ---- a batch of scenes (many Tracers' worth in one build and one launch) ----
+struct phr_batch {
+    phr_gpu* gpu = nullptr;
+    uint32_t scenes = 0, n = 0, leaf = 4;
+    std::vector<uint32_t> seg_lo, node_base, light_at, glow;
+    std::vector<float> extent;
+    phr_array *in_faces = nullptr, *verts = nullptr, *mats = nullptr, *prims = nullptr, *faces = nullptr, *tris = nullptr,
+              *lights = nullptr, *px = nullptr;
+    phr_bvh* bvh = nullptr;
+    SceneDev* table = nullptr;    // one SceneDev a scene
+    PhrCamera* cams = nullptr;    // one stamped camera a scene (device)
+    uint32_t cams_cap = 0;
+    PhrCamera* gcams = nullptr;   // the recorded pass's own cameras (eager renders leave them alone)
+    unsigned long long* rays = nullptr;  // closest, shadow, inner steps, triangle tests
+    uint32_t w = 0, h = 0;
+    uint32_t path_waves = 2, grouped = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_gpu_s = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t gw = 0, gh = 0;
+    const void* gsig = nullptr;
+};
+
+namespace {
+
+void batch_release(phr_batch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->bvh) phr_bvh_free(b->bvh);
+    for (phr_array* a : {b->in_faces, b->verts, b->mats, b->prims, b->faces, b->tris, b->lights, b->px}) phr_array_release(a);
+    if (b->table) (void)hipFree(b->table);
+    if (b->cams) (void)hipFree(b->cams);
+    if (b->gcams) (void)hipFree(b->gcams);
+    if (b->rays) (void)hipFree(b->rays);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+}
+
+// Boxes from the faces, the segmented build, the faces into slot order: all
+// kernels, no host synchronisation (so a capture can hold it).
+int batch_enqueue_build(phr_batch* b) {
+    hipStream_t s = b->gpu->stream;
+    const uint32_t n = b->n;
+    hipLaunchKernelGGL(k_tri_boxes, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr,
+                       (const float4*)b->verts->ptr, n, (float4*)b->prims->ptr);
+    int rc = phr_bvh_internal_rebuild_async(b->bvh);
+    if (rc != PHR_OK) return rc;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr, (const float4*)b->verts->ptr,
+                       (const uint32_t*)phr_bvh_internal_order(b->bvh)->ptr, n, (uint4*)b->faces->ptr, (float4*)b->tris->ptr);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
+
+// Each scene's camera as a tracer of that scene alone stamps it.
+int batch_cameras(phr_batch* b, const PhrCamera& cam, uint32_t w, uint32_t h) {
+    std::vector<PhrCamera> c(b->scenes, cam);
+    for (uint32_t k = 0; k < b->scenes; ++k) {
+        c[k].w = w;
+        c[k].h = h;
+        if (c[k].ao <= 0.f) c[k].ao = b->extent[k] * 0.08f;
+        c[k].eps = b->extent[k] * 1e-5f;
+        c[k].glow = b->glow[k];
+        c[k].frame = 0;
+    }
+    if (b->cams_cap < b->scenes) {
+        if (b->cams) (void)hipFree(b->cams);
+        b->cams = nullptr;
+        b->cams_cap = 0;
+        PHR_HIP(hipMalloc((void**)&b->cams, sizeof(PhrCamera) * b->scenes));
+        b->cams_cap = b->scenes;
+    }
+    PHR_HIP(hipMemcpyAsync(b->cams, c.data(), sizeof(PhrCamera) * b->scenes, hipMemcpyHostToDevice, b->gpu->stream));
+    PHR_HIP(hipStreamSynchronize(b->gpu->stream));
+    return PHR_OK;
+}
+
+int batch_image(phr_batch* b, uint32_t w, uint32_t h) {
+    if (b->px && b->w == w && b->h == h) return PHR_OK;
+    if (b->px) phr_array_release(b->px);
+    b->px = nullptr;
+    const uint64_t words = (PX_LEN * (uint64_t)w * h + 1) * b->scenes;
+    int rc = phr_array_alloc(b->gpu, 4, words, &b->px);
+    if (rc != PHR_OK) return rc;
+    b->w = w;
+    b->h = h;
+    return PHR_OK;
+}
+
+void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t waves, uint32_t grouped) {
+    const uint32_t tiles_x = (b->w + 15) / 16, tiles = tiles_x * ((b->h + 15) / 16);
+    const dim3 grid(tiles * b->scenes);
+    const uint64_t stride = PX_LEN * (uint64_t)b->w * b->h + 1;
+    uint32_t* px = (uint32_t*)b->px->ptr;
+    hipStream_t s = b->gpu->stream;
+    if (waves == 4)
+        hipLaunchKernelGGL(k_path_batch<4>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
+    else if (waves == 3)
+        hipLaunchKernelGGL(k_path_batch<3>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
+    else
+        hipLaunchKernelGGL(k_path_batch<2>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
+}
+
+int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
+    PHR_CHECK(spp >= 1, PHR_ERR_ARG, "phr_batch_render: spp must be at least 1");
+    PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
+              "phr_batch_render: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
+    PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_batch_render: only phiray's own pixel filter (0)");
+    PHR_CHECK(cam.adapt[3] == 0.0f, PHR_ERR_UNSUPPORTED, "phr_batch_render: adaptive sampling is per tracer");
+    return PHR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Many scenes (no counterpart in phiray, which holds one Scene a Tracer):
+// scene k is tris[seg_lo[k], seg_lo[k + 1]), every scene non-empty. The
+// faces of all scenes share one vertex and one material table (vertices are
+// not shared between faces: the integrator reads them by value, so this
+// gives what Tracer::new's welded table gives); one segmented LBVH build
+// makes every scene's tree; each scene keeps its own emitter table and world
+// diagonal. A scene renders as a tracer of its triangles alone does.
+int phr_batch_new(phr_gpu* g, const void* tris_v, const uint32_t* seg_lo, uint32_t scenes, uint32_t leaf, phr_batch** out) {
+    PHR_CHECK(g && tris_v && seg_lo && out && scenes > 0, PHR_ERR_ARG, "phr_batch_new: bad argument");
+    PHR_CHECK(seg_lo[0] == 0, PHR_ERR_ARG, "phr_batch_new: the first scene starts at 0");
+    for (uint32_t k = 0; k < scenes; ++k)
+        PHR_CHECK(seg_lo[k + 1] > seg_lo[k], PHR_ERR_ARG, "phr_batch_new: every scene needs a triangle");
+    const uint32_t n = seg_lo[scenes];
+    const PhrTri* tris = (const PhrTri*)tris_v;
+    for (uint32_t f = 0; f < n; ++f)
+        PHR_CHECK(tris[f].trans == 0.f, PHR_ERR_UNSUPPORTED, "phr_batch_new: transmission (glass) is not supported");
+    phr_batch* b = new phr_batch();
+    b->gpu = g;
+    b->scenes = scenes;
+    b->n = n;
+    b->leaf = leaf;
+    b->seg_lo.assign(seg_lo, seg_lo + scenes + 1);
+    auto fail = [&](int rc) { batch_release(b); return rc; };
+    std::unordered_map<std::array<uint32_t, MAT_WORDS>, uint32_t, WordsHash<MAT_WORDS>> mmap;
+    std::vector<uint32_t> in_faces(4 * (size_t)n);
+    std::vector<float> verts(24 * (size_t)n), mats;
+    b->extent.resize(scenes);
+    std::array<uint32_t, MAT_WORDS> last_key{};
+    uint32_t last_mid = PHR_NONE;
+    for (uint32_t k = 0; k < scenes; ++k) {
+        float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f}, hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+        for (uint32_t f = seg_lo[k]; f < seg_lo[k + 1]; ++f) {
+            const PhrTri& tr = tris[f];
+            float m[MAT_WORDS] = {tr.color[0], tr.color[1], tr.color[2], tr.color[3], tr.rough, tr.metal, tr.trans, tr.ior,
+                                  tr.emit[0], tr.emit[1], tr.emit[2], tr.bump, 0, 0, 0, 0};
+            std::array<uint32_t, MAT_WORDS> mk;
+            for (uint32_t j = 0; j < MAT_WORDS; ++j) mk[j] = fbits(m[j]);
+            uint32_t mid;
+            if (last_mid != PHR_NONE && mk == last_key) {
+                mid = last_mid;
+            } else {
+                auto mi = mmap.find(mk);
+                if (mi == mmap.end()) {
+                    mid = (uint32_t)mmap.size();
+                    mmap.emplace(mk, mid);
+                    mats.insert(mats.end(), m, m + MAT_WORDS);
+                } else {
+                    mid = mi->second;
+                }
+                last_key = mk;
+                last_mid = mid;
+            }
+            const float* p[3] = {tr.a, tr.b, tr.c};
+            const float* nn[3] = {tr.na, tr.nb, tr.nc};
+            for (int c = 0; c < 3; ++c) {
+                float o[2];
+                oct_pack(nn[c], o);
+                float* v = &verts[8 * (3 * (size_t)f + c)];
+                v[0] = p[c][0], v[1] = p[c][1], v[2] = p[c][2], v[3] = 1.0f, v[4] = o[0], v[5] = o[1], v[6] = 0.f, v[7] = 0.f;
+                in_faces[4 * (size_t)f + c] = 3 * f + c;
+                for (int j = 0; j < 3; ++j) { lo[j] = std::min(lo[j], p[c][j]); hi[j] = std::max(hi[j], p[c][j]); }
+            }
+            in_faces[4 * (size_t)f + 3] = mid;
+        }
+        float sq = -0.0f;  // the scene's diagonal, as phr_tracer_new takes it
+        for (int j = 0; j < 3; ++j) sq = sq + (hi[j] - lo[j]) * (hi[j] - lo[j]);
+        b->extent[k] = std::max(std::sqrt(sq), 1e-3f);
+    }
+    int rc;
+    hipStream_t s = g->stream;
+    const uint32_t nmats = (uint32_t)mmap.size();
+    if ((rc = phr_array_alloc(g, 16, n, &b->in_faces)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, 3ull * n, &b->verts)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 4 * MAT_WORDS, nmats, &b->mats)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 32, n, &b->prims)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 16, n, &b->faces)) != PHR_OK) return fail(rc);
+    if ((rc = phr_array_alloc(g, 48, n, &b->tris)) != PHR_OK) return fail(rc);
+    if (hipMalloc((void**)&b->rays, 32) != hipSuccess || hipMemsetAsync(b->rays, 0, 32, s) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        phr_set_error_msg("phr_batch_new: allocation failed");
+        return fail(PHR_ERR_HIP);
+    }
+    if (hipMemcpyAsync(b->in_faces->ptr, in_faces.data(), 16ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->verts->ptr, verts.data(), 96ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->mats->ptr, mats.data(), 64ull * nmats, hipMemcpyHostToDevice, s) != hipSuccess) {
+        phr_set_error_msg("phr_batch_new: upload failed");
+        return fail(PHR_ERR_HIP);
+    }
+    hipLaunchKernelGGL(k_tri_boxes, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr,
+                       (const float4*)b->verts->ptr, n, (float4*)b->prims->ptr);
+    // Trees packed one after another: scene k's at the nodes of the scenes before it.
+    b->node_base.resize(scenes);
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < scenes; ++k) {
+        b->node_base[k] = (uint32_t)total;
+        total += 2ull * (seg_lo[k + 1] - seg_lo[k]) - 1;
+    }
+    if (total >= PHR_LEAF) {
+        phr_set_error_msg("phr_batch_new: too many nodes for 31-bit ids");
+        return fail(PHR_ERR_SIZE);
+    }
+    if ((rc = phr_bvh_internal_build_into(g, b->prims, leaf, seg_lo, scenes, nullptr, b->node_base.data(), PHR_LEAF, &b->bvh)) != PHR_OK)
+        return fail(rc);
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr, (const float4*)b->verts->ptr,
+                       (const uint32_t*)phr_bvh_internal_order(b->bvh)->ptr, n, (uint4*)b->faces->ptr, (float4*)b->tris->ptr);
+    std::vector<uint32_t> order(n);
+    if (hipMemcpyAsync(order.data(), phr_bvh_internal_order(b->bvh)->ptr, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        phr_set_error_msg("phr_batch_new: order readback failed");
+        return fail(PHR_ERR_HIP);
+    }
+    std::vector<uint32_t> words;
+    b->light_at.resize(scenes);
+    b->glow.resize(scenes);
+    for (uint32_t k = 0; k < scenes; ++k) {
+        std::vector<uint32_t> wk = light_words(order.data(), seg_lo[k], seg_lo[k + 1], in_faces, verts, mats);
+        b->light_at[k] = (uint32_t)words.size();
+        b->glow[k] = wk[0];
+        words.insert(words.end(), wk.begin(), wk.end());
+    }
+    if ((rc = phr_array_alloc(g, 4, words.size(), &b->lights)) != PHR_OK) return fail(rc);
+    std::vector<SceneDev> table(scenes);
+    for (uint32_t k = 0; k < scenes; ++k) {
+        SceneDev& d = table[k];
+        std::memset(&d, 0, sizeof d);
+        d.nodes = (const float4*)phr_bvh_internal_nodes(b->bvh)->ptr;
+        d.tris = (const float4*)b->tris->ptr;
+        d.faces = (const uint4*)b->faces->ptr;
+        d.verts = (const float4*)b->verts->ptr;
+        d.mats = (const float*)b->mats->ptr;
+        d.lights = (const uint32_t*)b->lights->ptr + b->light_at[k];
+        d.insts = nullptr;
+        d.ninsts = 0;
+        d.root = b->node_base[k];
+    }
+    if (hipMalloc((void**)&b->table, sizeof(SceneDev) * scenes) != hipSuccess ||
+        hipMemcpyAsync(b->lights->ptr, words.data(), 4 * words.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->table, table.data(), sizeof(SceneDev) * scenes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        phr_set_error_msg("phr_batch_new: table upload failed");
+        return fail(PHR_ERR_HIP);
+    }
+    b->grouped = scenes % 8 == 0 ? 1u : 0u;
+    *out = b;
+    return PHR_OK;
+}
+
+void phr_batch_free(phr_batch* b) { batch_release(b); }
+uint32_t phr_batch_scenes(const phr_batch* b) { return b ? b->scenes : 0; }
+uint32_t phr_batch_len(const phr_batch* b) { return b ? b->n : 0; }
+phr_bvh* phr_batch_bvh(phr_batch* b) { return b ? b->bvh : nullptr; }
+float phr_batch_extent(const phr_batch* b, uint32_t k) { return b && k < b->scenes ? b->extent[k] : 0.f; }
+uint32_t phr_batch_glow(const phr_batch* b, uint32_t k) { return b && k < b->scenes ? b->glow[k] : 0u; }
+
+// The whole build again from the triangles (boxes, segmented LBVH, slot
+// order); gpu_s gets its device time.
+int phr_batch_rebuild(phr_batch* b, double* gpu_s) {
+    PHR_CHECK(b, PHR_ERR_ARG, "phr_batch_rebuild: null");
+    hipStream_t s = b->gpu->stream;
+    PHR_HIP(hipEventRecord(b->ev0, s));
+    int rc = batch_enqueue_build(b);
+    if (rc != PHR_OK) return rc;
+    PHR_HIP(hipEventRecord(b->ev1, s));
+    PHR_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (gpu_s) *gpu_s = ms * 1e-3;
+    return PHR_OK;
+}
+
+// Every scene under `cam` at w x h, spp samples from a fresh image, in one
+// launch; gpu_s gets its device time.
+int phr_batch_render(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, double* gpu_s) {
+    PHR_CHECK(b && cam_v, PHR_ERR_ARG, "phr_batch_render: null argument");
+    w = std::max<uint32_t>(w, 1);
+    h = std::max<uint32_t>(h, 1);
+    PhrCamera cam;
+    std::memcpy(&cam, cam_v, sizeof cam);
+    int rc;
+    if ((rc = batch_check_camera(cam, spp)) != PHR_OK || (rc = batch_image(b, w, h)) != PHR_OK ||
+        (rc = batch_cameras(b, cam, w, h)) != PHR_OK)
+        return rc;
+    hipStream_t s = b->gpu->stream;
+    PHR_HIP(hipEventRecord(b->ev0, s));
+    batch_launch(b, b->cams, spp, b->path_waves, b->grouped);
+    PHR_HIP(hipGetLastError());
+    PHR_HIP(hipEventRecord(b->ev1, s));
+    PHR_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_gpu_s = ms * 1e-3;
+    if (gpu_s) *gpu_s = b->last_gpu_s;
+    return PHR_OK;
+}
+
+// The render kernel's build (waves a SIMD: 2, 3 or 4) and block order
+// (grouped: each XCD's blocks cover whole scenes; needs scenes % 8 == 0).
+int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped) {
+    PHR_CHECK(b && waves >= 2 && waves <= 4 && grouped <= 1, PHR_ERR_ARG, "phr_batch_set_path: waves 2-4, grouped 0 or 1");
+    PHR_CHECK(!grouped || b->scenes % 8 == 0, PHR_ERR_ARG, "phr_batch_set_path: grouped order needs a multiple of 8 scenes");
+    b->path_waves = waves;
+    b->grouped = grouped;
+    return PHR_OK;
+}
+int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped) {
+    PHR_CHECK(b, PHR_ERR_ARG, "phr_batch_path: null");
+    if (waves) *waves = b->path_waves;
+    if (grouped) *grouped = b->grouped;
+    return PHR_OK;
+}
+
+// Time every (waves, order) pair on the caller's render, `reps` launches
+// each after one warm-up, and keep the fastest.
+int phr_batch_autotune(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,
+                       uint32_t* waves, uint32_t* grouped, double* best_s) {
+    PHR_CHECK(b && cam_v, PHR_ERR_ARG, "phr_batch_autotune: null argument");
+    reps = std::max<uint32_t>(reps, 1);
+    w = std::max<uint32_t>(w, 1);
+    h = std::max<uint32_t>(h, 1);
+    PhrCamera cam;
+    std::memcpy(&cam, cam_v, sizeof cam);
+    int rc;
+    if ((rc = batch_check_camera(cam, spp)) != PHR_OK || (rc = batch_image(b, w, h)) != PHR_OK ||
+        (rc = batch_cameras(b, cam, w, h)) != PHR_OK)
+        return rc;
+    hipStream_t s = b->gpu->stream;
+    double best = 1e30;
+    uint32_t bw = b->path_waves, bg = b->grouped;
+    for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
+        for (uint32_t wv = 2; wv <= 4; ++wv) {
+            batch_launch(b, b->cams, spp, wv, g);
+            PHR_HIP(hipEventRecord(b->ev0, s));
+            for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g);
+            PHR_HIP(hipGetLastError());
+            PHR_HIP(hipEventRecord(b->ev1, s));
+            PHR_HIP(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            const double t = ms * 1e-3 / reps;
+            if (t < best) best = t, bw = wv, bg = g;
+        }
+    b->path_waves = bw;
+    b->grouped = bg;
+    if (waves) *waves = bw;
+    if (grouped) *grouped = bg;
+    if (best_s) *best_s = best;
+    return PHR_OK;
+}
+
+// The whole pass as one hipGraph: with `rebuild`, boxes + segmented build +
+// slot order, then every scene's render (fresh images under `cam`). The call
+// renders once eagerly first, which stamps the cameras and sizes the images.
+int phr_batch_record(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, int rebuild) {
+    PHR_CHECK(b && cam_v, PHR_ERR_ARG, "phr_batch_record: null argument");
+    int rc = phr_batch_render(b, cam_v, w, h, spp, nullptr);
+    if (rc != PHR_OK) return rc;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hipStream_t s = b->gpu->stream;
+    // The pass keeps its own copy of the stamped cameras: a later eager
+    // render restamps b->cams.
+    if (b->gcams) (void)hipFree(b->gcams);
+    b->gcams = nullptr;
+    PHR_HIP(hipMalloc((void**)&b->gcams, sizeof(PhrCamera) * b->scenes));
+    PHR_HIP(hipMemcpyAsync(b->gcams, b->cams, sizeof(PhrCamera) * b->scenes, hipMemcpyDeviceToDevice, s));
+    PHR_HIP(hipStreamSynchronize(s));
+    PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    if (rebuild) rc = batch_enqueue_build(b);
+    if (rc == PHR_OK) batch_launch(b, b->gcams, spp, b->path_waves, b->grouped);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc != PHR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    PHR_CHECK(e == hipSuccess && graph, PHR_ERR_HIP, "phr_batch_record: capture failed");
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        phr_set_error_msg("phr_batch_record: instantiation failed");
+        return PHR_ERR_HIP;
+    }
+    b->graph = graph;
+    b->exec = exec;
+    b->gw = b->w, b->gh = b->h;
+    b->gsig = b->px->ptr;
+    return PHR_OK;
+}
+
+int phr_batch_replay(phr_batch* b, double* gpu_s) {
+    PHR_CHECK(b && b->exec, PHR_ERR_ARG, "phr_batch_replay: nothing recorded");
+    PHR_CHECK(b->px && b->px->ptr == b->gsig && b->w == b->gw && b->h == b->gh, PHR_ERR_ARG,
+              "phr_batch_replay: the images moved; record again");
+    hipStream_t s = b->gpu->stream;
+    PHR_HIP(hipEventRecord(b->ev0, s));
+    PHR_HIP(hipGraphLaunch(b->exec, s));
+    PHR_HIP(hipEventRecord(b->ev1, s));
+    PHR_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_gpu_s = ms * 1e-3;
+    if (gpu_s) *gpu_s = b->last_gpu_s;
+    return PHR_OK;
+}
+
+// Scene k's image words [from, from + count) (its own PX_LEN * w * h + 1).
+int phr_batch_px(phr_batch* b, uint32_t k, uint64_t from, uint64_t count, uint32_t* out) {
+    PHR_CHECK(b && out && k < b->scenes, PHR_ERR_ARG, "phr_batch_px: bad argument");
+    PHR_CHECK(b->px, PHR_ERR_ARG, "phr_batch_px: nothing rendered yet");
+    const uint64_t stride = PX_LEN * (uint64_t)b->w * b->h + 1;
+    PHR_CHECK(from + count <= stride, PHR_ERR_SIZE, "phr_batch_px: range beyond the scene's image");
+    PHR_HIP(hipMemcpyAsync(out, (const uint32_t*)b->px->ptr + stride * k + from, 4 * count, hipMemcpyDeviceToHost, b->gpu->stream));
+    PHR_HIP(hipStreamSynchronize(b->gpu->stream));
+    return PHR_OK;
+}
+
+// Rays traced since the batch was made: closest, shadow, inner steps, triangle tests.
+int phr_batch_counters(phr_batch* b, uint64_t* out4) {
+    PHR_CHECK(b && out4, PHR_ERR_ARG, "phr_batch_counters: null");
+    unsigned long long v[4];
+    PHR_HIP(hipMemcpyAsync(v, b->rays, 32, hipMemcpyDeviceToHost, b->gpu->stream));
+    PHR_HIP(hipStreamSynchronize(b->gpu->stream));
+    for (int k = 0; k < 4; ++k) out4[k] = v[k];
+    return PHR_OK;
+}
 
 }  // extern "C"

@@ -147,6 +147,16 @@ def batch_scenes(count: int = 256, lo: float = 1e3, hi: float = 2e5, seed: int =
     return np.concatenate(boxes), seg_lo
 
 
+def batch_tris(count: int = 256, lo: float = 1e3, hi: float = 2e5, seed: int = 0x5C3,
+               color=(0.6, 0.6, 0.6, 1.0)) -> tuple[np.ndarray, np.ndarray]:
+    """The same scenes as triangle records (one grey Lambertian material), and
+    the segment starts."""
+    counts = batch_counts(count, lo, hi, seed)
+    tris = [tris_from_corners(lbvh_scene(int(m), i)[0], color) for i, m in enumerate(counts)]
+    seg_lo = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    return np.concatenate(tris), seg_lo
+
+
 def pinhole_rays(w: int = 256, h: int = 256, origin=(0.5, 0.5, -1.5), fov_deg: float = 45.0) -> np.ndarray:
     t = F32(np.tan(np.deg2rad(fov_deg) / 2.0))
     xs = (np.arange(w, dtype=F32) + F32(0.5)) / F32(w) - F32(0.5)
@@ -235,7 +245,7 @@ def spheres_scene(spheres: int = 64, level: int = 5, seed: int = 1, box=(4.0, 4.
 
 
 def path_camera(w: int, h: int, eye=(0.0, -6.0, 2.5), target=(0.0, 0.0, 0.3), fov_deg: float = 45.0,
-                bounces: int = 4, rr_from: int = 0, ao: float = 0.0) -> np.ndarray:
+                bounces: int = 4, rr_from: int = 0, ao: float = 0.0, up=(0.0, 0.0, 1.0)) -> np.ndarray:
     """A perspective `Camera` (src/trace/camera.rs) for the Rendered path:
     view basis from `eye` to `target`, horizontal field of view `fov_deg`,
     Max Bounces (total 4, transparent 8, transmission 8, glossy 0), diffuse
@@ -246,7 +256,7 @@ def path_camera(w: int, h: int, eye=(0.0, -6.0, 2.5), target=(0.0, 0.0, 0.3), fo
     fwd = target - eye
     dist = float(np.linalg.norm(fwd))
     fwd /= dist
-    right = np.cross(fwd, [0.0, 0.0, 1.0])
+    right = np.cross(fwd, np.array(up, dtype=np.float64))
     right /= np.linalg.norm(right)
     up = np.cross(right, fwd)
     cam = np.zeros(1, dtype=CAMERA)
