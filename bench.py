@@ -335,7 +335,33 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
         res["raw_over_learned"] = best["raw"]["s_per_frame"] / best["learned"]["s_per_frame"]
     if best["learned"] and best["plain"]:
         res["plain_over_learned"] = best["plain"]["s_per_frame"] / best["learned"]["s_per_frame"]
+    if best["learned"]:
+        res["equal_quality"] = equal_quality(tr, cam, w, h, ref, best["learned"], trials)
     return res
+
+
+def equal_quality(tr: "P.Tracer", cam: np.ndarray, w: int, h: int, ref: np.ndarray, learned: dict, trials: list,
+                  max_spp: int = 2048) -> dict:
+    """The samples the raw path needs to match the relMSE the learned frame
+    reaches (a stricter bound than 0.01, the same reference): the raw trials
+    already run, then doubling spp, one render each."""
+    target = next(t["relmse_learned"] for t in trials if t["spp"] == learned["spp"])
+    match = next(({"spp": t["spp"], "s_per_frame": t["render_s"], "relmse": t["relmse_raw"]}
+                  for t in trials if t["relmse_raw"] <= target), None)
+    spp = 2 * max(t["spp"] for t in trials)
+    while match is None and spp <= max_spp:
+        fresh = cam.copy()
+        fresh["stream"] = spp + (7 << 16)
+        tr.sample(fresh, w, h, spp)
+        raw = rel_mse(tr.radiance(), ref)
+        log(f"equal quality: raw {spp} spp relMSE {raw:.5f} (learned {target:.5f})")
+        if raw <= target:
+            match = {"spp": spp, "s_per_frame": tr.time()[1], "relmse": raw}
+        spp *= 2
+    out = {"relmse": target, "learned_spp": learned["spp"], "learned_s": learned["s_per_frame"], "raw": match}
+    if match:
+        out["raw_over_learned"] = match["s_per_frame"] / learned["s_per_frame"]
+    return out
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
