@@ -484,7 +484,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge", "batch", "kpcn"], default="all")
-    ap.add_argument("--path-waves", type=int, default=0, help="run this path kernel variant (2, 3, 4) instead of autotuning")
+    ap.add_argument("--path-waves", type=int, default=0, help="run this path kernel variant (2 to 5) instead of autotuning")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:

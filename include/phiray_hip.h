@@ -294,7 +294,7 @@ uint32_t phr_tracer_instance_count(const phr_tracer* t);
  * PHR_ERR_UNSUPPORTED when the last camera had it off. */
 int phr_tracer_open_pixels(phr_tracer* t, uint32_t* count);
 /* The path kernel variant (no counterpart in phiray): register allocation
- * fitted to 2 (default), 3 or 4 waves a SIMD. */
+ * fitted to 2 (default), 3, 4 or 5 waves a SIMD. */
 int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves);
 uint32_t phr_tracer_path_waves(const phr_tracer* t);
 /* Autotune (no counterpart in phiray): time each variant on `spp` samples of
@@ -321,7 +321,7 @@ uint32_t phr_batch_glow(const phr_batch* b, uint32_t k);
 int phr_batch_rebuild(phr_batch* b, double* gpu_s);
 /* Every scene, spp samples of a fresh w x h image each, in one launch. */
 int phr_batch_render(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, double* gpu_s);
-/* The render kernel: waves a SIMD (2, 3, 4) and block order (grouped 1: the
+/* The render kernel: waves a SIMD (2 to 5) and block order (grouped 1: the
  * blocks of one XCD cover whole scenes; needs a multiple of 8 scenes). */
 int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped);
 int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped);

@@ -869,10 +869,7 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
         V3 ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
         V3 gn = keep ? v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2)) : v3(0, 0, 0);
         float gd = keep ? ldf(px, 13 * n + i) : 0.f;
-        V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
         float cov = keep ? ldf(px, 23 * n + i) : 0.f;
-        float film = keep ? ldf(px, 24 * n + i) : 0.f;
-        uint32_t taken = keep ? px[31 * n + i] : 0u;
         float gone = keep ? ldf(px, 33 * n + i) : 0.f;
         // The half image (util.wgsl:126-132): odd samples at double weight; a
         // new image starts it at zero.
@@ -886,13 +883,21 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
             gn = gn + sm.normal;
             gd = gd + sm.depth;
             gone = gone + sm.missed;
-            taken = taken + 1u;
-            film = film + 1.0f;
             l = l + sm.lin;
-            gs = gs + v3(0.f, 0.f, 0.f);
             cov = cov + sm.cov;
             outline = sm.outline;
         }
+        // The grid plane, the film weight and the count take the same
+        // per-sample additions as the sums above (+0, +1.0, +1), done after
+        // the samples so they hold no registers across them.
+        V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
+        float film = keep ? ldf(px, 24 * n + i) : 0.f;
+        uint32_t taken = keep ? px[31 * n + i] : 0u;
+        for (uint32_t s = 0; s < spp; ++s) {
+            gs = gs + v3(0.f, 0.f, 0.f);
+            film = film + 1.0f;
+        }
+        taken = taken + spp;
         if (frame0 == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
         stf(px, n + 3 * i, l.x); stf(px, n + 3 * i + 1, l.y); stf(px, n + 3 * i + 2, l.z);
         stf(px, 7 * n + 3 * i, ga.x); stf(px, 7 * n + 3 * i + 1, ga.y); stf(px, 7 * n + 3 * i + 2, ga.z);
@@ -924,7 +929,7 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
 
 // One image: a block a tile, from tile0 on (a slice of the tiles).
 // WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
-// 3 and 4 trade a few spilled words for more waves in flight). The tracer's
+// 3 to 5 trade spilled words for more waves in flight). The tracer's
 // autotune times the variants on the caller's workload and keeps the fastest.
 template <bool F, int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
@@ -958,6 +963,36 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     path_block<false>(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, 0u, s_stack);
 }
 
+// The builds of the path kernels the tracer and the batch choose among.
+constexpr uint32_t PATH_WAVES_MIN = 2, PATH_WAVES_MAX = 5;
+
+void launch_path(uint32_t waves, bool forest, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp, uint32_t tiles_x,
+                 uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed, uint32_t tile0) {
+    const dim3 block(TRACE_THREADS);
+    if (forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
+        hipLaunchKernelGGL((k_path<true, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (waves == 5)
+        hipLaunchKernelGGL((k_path<false, 5>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (waves == 4)
+        hipLaunchKernelGGL((k_path<false, 4>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (waves == 3)
+        hipLaunchKernelGGL((k_path<false, 3>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else
+        hipLaunchKernelGGL((k_path<false, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+}
+
+void launch_path_batch(uint32_t waves, dim3 grid, hipStream_t s, const SceneDev* table, const PhrCamera* cams, uint32_t tiles,
+                       uint32_t tiles_x, uint32_t spp, uint32_t* px, uint64_t stride, unsigned long long* rays, uint32_t grouped) {
+    const dim3 block(TRACE_THREADS);
+    if (waves == 5)
+        hipLaunchKernelGGL(k_path_batch<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+    else if (waves == 4)
+        hipLaunchKernelGGL(k_path_batch<4>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+    else if (waves == 3)
+        hipLaunchKernelGGL(k_path_batch<3>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+    else
+        hipLaunchKernelGGL(k_path_batch<2>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+}
 
 // ---- Adaptive Sampling's tail (src/trace/wgsl/adapt.wgsl, gather.wgsl) ----
 // Stage 0, Cycles' convergence check at a step boundary: the full and half
@@ -1142,7 +1177,7 @@ struct phr_tracer {
     std::vector<uint32_t> corner_vertex;  // input corner (3f + k) -> vertex id
     std::vector<uint32_t> order;          // slot -> input face
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
-    uint32_t path_waves = 2;  // which k_path variant runs (2, 3 or 4 waves a SIMD)
+    uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -1838,19 +1873,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
         for (uint32_t k = 0; k < slices && !abandoned; ++k) {
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
             if (b == a) continue;
-            const dim3 grid(b - a);
-            if (t->forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
-                hipLaunchKernelGGL((k_path<true, 2>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp, t->rays,
-                                   listed, a);
-            else if (t->path_waves == 4)
-                hipLaunchKernelGGL((k_path<false, 4>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                                   t->rays, listed, a);
-            else if (t->path_waves == 3)
-                hipLaunchKernelGGL((k_path<false, 3>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                                   t->rays, listed, a);
-            else
-                hipLaunchKernelGGL((k_path<false, 2>), grid, dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, f0, count, pxp,
-                                   t->rays, listed, a);
+            launch_path(t->path_waves, t->forest, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
             PHR_HIP(hipGetLastError());
             if (slices > 1) {
                 PHR_HIP(hipStreamSynchronize(s));
@@ -1905,11 +1928,11 @@ int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, ui
     return phr_tracer_sample_in(t, cam, w, h, spp, 1, nullptr, nullptr, frame, done, nullptr);
 }
 
-// The path kernel variant: register allocation fitted to 2, 3 or 4 waves a
-// SIMD (no counterpart in phiray, whose WGSL pipeline has one).
+// The path kernel variant: register allocation fitted to 2, 3, 4 or 5 waves
+// a SIMD (no counterpart in phiray, whose WGSL pipeline has one).
 int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves) {
     PHR_CHECK(t, PHR_ERR_ARG, "phr_tracer_set_path_waves: null");
-    PHR_CHECK(waves >= 2 && waves <= 4, PHR_ERR_ARG, "phr_tracer_set_path_waves: waves must be 2, 3 or 4");
+    PHR_CHECK(waves >= PATH_WAVES_MIN && waves <= PATH_WAVES_MAX, PHR_ERR_ARG, "phr_tracer_set_path_waves: waves must be 2 to 5");
     t->path_waves = waves;
     return PHR_OK;
 }
@@ -1924,7 +1947,7 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     reps = std::max<uint32_t>(reps, 1);
     uint32_t best_w = 2;
     double best = 1e30;
-    for (uint32_t waves = 2; waves <= (t->forest ? 2u : 4u); ++waves) {
+    for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? PATH_WAVES_MIN : PATH_WAVES_MAX); ++waves) {
         t->path_waves = waves;
         double mine = 1e30;
         for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
@@ -2089,14 +2112,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     uint32_t* pxp = (uint32_t*)t->px->ptr;
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    if (t->forest)
-        hipLaunchKernelGGL((k_path<true, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
-    else if (t->path_waves == 4)
-        hipLaunchKernelGGL((k_path<false, 4>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
-    else if (t->path_waves == 3)
-        hipLaunchKernelGGL((k_path<false, 3>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
-    else
-        hipLaunchKernelGGL((k_path<false, 2>), dim3(tiles), dim3(TRACE_THREADS), 0, s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    launch_path(t->path_waves, t->forest, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0);
     if (denoise == 2) {
@@ -2517,14 +2533,7 @@ void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t wa
     const uint32_t tiles_x = (b->w + 15) / 16, tiles = tiles_x * ((b->h + 15) / 16);
     const dim3 grid(tiles * b->scenes);
     const uint64_t stride = PX_LEN * (uint64_t)b->w * b->h + 1;
-    uint32_t* px = (uint32_t*)b->px->ptr;
-    hipStream_t s = b->gpu->stream;
-    if (waves == 4)
-        hipLaunchKernelGGL(k_path_batch<4>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
-    else if (waves == 3)
-        hipLaunchKernelGGL(k_path_batch<3>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
-    else
-        hipLaunchKernelGGL(k_path_batch<2>, grid, dim3(TRACE_THREADS), 0, s, b->table, cams, tiles, tiles_x, spp, px, stride, b->rays, grouped);
+    launch_path_batch(waves, grid, b->gpu->stream, b->table, cams, tiles, tiles_x, spp, (uint32_t*)b->px->ptr, stride, b->rays, grouped);
 }
 
 int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
@@ -2735,10 +2744,11 @@ int phr_batch_render(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, ui
     return PHR_OK;
 }
 
-// The render kernel's build (waves a SIMD: 2, 3 or 4) and block order
+// The render kernel's build (waves a SIMD: 2 to 5) and block order
 // (grouped: each XCD's blocks cover whole scenes; needs scenes % 8 == 0).
 int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped) {
-    PHR_CHECK(b && waves >= 2 && waves <= 4 && grouped <= 1, PHR_ERR_ARG, "phr_batch_set_path: waves 2-4, grouped 0 or 1");
+    PHR_CHECK(b && waves >= PATH_WAVES_MIN && waves <= PATH_WAVES_MAX && grouped <= 1, PHR_ERR_ARG,
+              "phr_batch_set_path: waves 2-5, grouped 0 or 1");
     PHR_CHECK(!grouped || b->scenes % 8 == 0, PHR_ERR_ARG, "phr_batch_set_path: grouped order needs a multiple of 8 scenes");
     b->path_waves = waves;
     b->grouped = grouped;
@@ -2769,7 +2779,7 @@ int phr_batch_autotune(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, 
     double best = 1e30;
     uint32_t bw = b->path_waves, bg = b->grouped;
     for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
-        for (uint32_t wv = 2; wv <= 4; ++wv) {
+        for (uint32_t wv = PATH_WAVES_MIN; wv <= PATH_WAVES_MAX; ++wv) {
             batch_launch(b, b->cams, spp, wv, g);
             PHR_HIP(hipEventRecord(b->ev0, s));
             for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g);
