@@ -195,36 +195,65 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t tiles_x, uint
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[f][m] = f32x4{0.f, 0.f, 0.f, 0.f};
     const uint32_t g = lane >> 4, col = lane & 15u;
+    // Staging: a thread's share of a chunk's halo tile is loaded into
+    // registers (every load issued before the first LDS store, so their
+    // latencies overlap), and the next chunk's share is loaded while this
+    // chunk's MFMAs run.
+    constexpr int MAXU = (HP * (CHUNK / 8) + 255) / 256;
+    uint4 v[MAXU];
+    auto load = [&](uint32_t c0) {
+        const uint32_t parts = min((uint32_t)CHUNK, cin - c0) / 8, lp = parts == 4 ? 2u : 1u;  // parts: 4 or 2
+#pragma unroll
+        for (int k = 0; k < MAXU; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k;
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (u < HP * parts) {
+                const uint32_t pix = u >> lp, part = u & (parts - 1u);
+                const int gy = y0 + (int)(pix / HWD) - 1, gx = x0 + (int)(pix % HWD) - 1;
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W) v[k] = conv_input(a, b, gy, gx, c0 + 8 * part);
+            }
+        }
+    };
+    load(0);
     for (uint32_t c0 = 0; c0 < cin; c0 += CHUNK) {
         // A last chunk of 16 channels runs the same instruction with the K
         // slots of lane groups 2 and 3 zero in both operands.
         const uint32_t cc = min((uint32_t)CHUNK, cin - c0), parts = cc / 8;
+        const uint32_t lp = parts == 4 ? 2u : 1u;
         const bool live = g < parts;
         __syncthreads();  // the previous chunk's reads are done
-        for (uint32_t u = threadIdx.x; u < HP * parts; u += 256) {
-            const uint32_t pix = u / parts, part = u % parts;
-            const int gy = y0 + (int)(pix / HWD) - 1, gx = x0 + (int)(pix % HWD) - 1;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = conv_input(a, b, gy, gx, c0 + 8 * part);
-            *(uint4*)(tile + pix * HALO_STRIDE + 8 * part) = v;
+#pragma unroll
+        for (int k = 0; k < MAXU; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k;
+            if (u < HP * parts) *(uint4*)(tile + (u >> lp) * HALO_STRIDE + 8 * (u & (parts - 1u))) = v[k];
         }
         __syncthreads();
+        if (c0 + CHUNK < cin) load(c0 + CHUNK);
+        // A row of three taps at a time: its 3 NF weight fragments are loaded
+        // together (one wait on the caches a row, not one a tap).
 #pragma unroll 1
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3, dx = tap % 3;  // halo offsets
-            const uint16_t* wt = a.w + ((uint64_t)tap * a.cout + col) * cin + c0 + 8 * g;
-            uint4 bx[MT];
+        for (int dy = 0; dy < 3; ++dy) {
+            uint4 wa[3][NF];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const int r = (int)wave * RW + m / 2, c = (m & 1) * 16 + (int)col;
-                bx[m] = live ? *(const uint4*)(tile + ((r + dy) * HWD + c + dx) * HALO_STRIDE + 8 * g)
-                             : make_uint4(0u, 0u, 0u, 0u);
+            for (int dx = 0; dx < 3; ++dx) {
+                const uint16_t* wt = a.w + ((uint64_t)(3 * dy + dx) * a.cout + col) * cin + c0 + 8 * g;
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    wa[dx][f] = live ? *(const uint4*)(wt + (uint64_t)(16 * f) * cin) : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                const uint4 wa = live ? *(const uint4*)(wt + (uint64_t)(16 * f) * cin) : make_uint4(0u, 0u, 0u, 0u);
+            for (int dx = 0; dx < 3; ++dx) {
+                uint4 bx[MT];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) acc[f][m] = mfma32(wa, bx[m], acc[f][m]);
+                for (int m = 0; m < MT; ++m) {
+                    const int r = (int)wave * RW + m / 2, c = (m & 1) * 16 + (int)col;
+                    bx[m] = live ? *(const uint4*)(tile + ((r + dy) * HWD + c + dx) * HALO_STRIDE + 8 * g)
+                                 : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) acc[f][m] = mfma32(wa[dx][f], bx[m], acc[f][m]);
             }
         }
     }
@@ -281,29 +310,56 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, c
 #pragma unroll
     for (int i = 0; i < FW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bacc = 0.0f;
-    for (uint64_t t0 = lo; t0 < hi; t0 += TP) {
-        for (uint32_t u = threadIdx.x; u < (uint32_t)(TP * (COUT / 4)); u += 256) {
-            const uint32_t j = u / (COUT / 4), q = u % (COUT / 4);
-            const uint64_t p = t0 + j;
-            bf16x4 v = bf16x4{0, 0, 0, 0};
-            if (p < hi) v = *(const bf16x4*)(dz + p * COUT + 4 * q);
+    // A thread's share of a tile: NCO quads of dz and NCI quads of x (TP
+    // pixels x COUT / 4 quads = 256 NCO). The next tile's share is loaded
+    // into registers while this one's MFMAs run.
+    bf16x4 vdz[NCO], vx[NCI];
+    auto load = [&](uint64_t t0) {
+        // The tile's first pixel (the same for every thread); pixel t0 + j
+        // lies j columns on, wrapping into the next rows (and images).
+        const uint64_t b0 = t0 / HW, r0 = t0 % HW;
+        const uint32_t y0 = (uint32_t)(r0 / W), x0 = (uint32_t)(r0 % W);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sdz[(4 * q + e) * LD + j] = (uint16_t)v[e];
+        for (int k = 0; k < NCO; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k, j = u / (COUT / 4), q = u % (COUT / 4);
+            const uint64_t p = t0 + j;
+            vdz[k] = p < hi ? *(const bf16x4*)(dz + p * COUT + 4 * q) : bf16x4{0, 0, 0, 0};
         }
-        for (uint32_t u = threadIdx.x; u < (uint32_t)(TP * (CIN / 4)); u += 256) {
-            const uint32_t j = u / (CIN / 4), q = u % (CIN / 4);
+#pragma unroll
+        for (int k = 0; k < NCI; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k, j = u / (CIN / 4), q = u % (CIN / 4);
             const uint64_t p = t0 + j;
             bf16x4 v = bf16x4{0, 0, 0, 0};
             if (p < hi) {
-                const uint64_t b = p / HW, rr = p % HW;
-                const int yy = (int)(rr / W) + dy, xx = (int)(rr % W) + dx;
+                uint64_t b = b0;
+                uint32_t cy = y0, cx = x0 + j;
+                while (cx >= W) {
+                    cx -= W;
+                    if (++cy == H) cy = 0, ++b;
+                }
+                const int yy = (int)cy + dy, xx = (int)cx + dx;
                 if (yy >= 0 && yy < (int)H && xx >= 0 && xx < (int)W)
                     v = *(const bf16x4*)(x + (b * HW + (uint64_t)yy * W + xx) * CIN + 4 * q);
             }
+            vx[k] = v;
+        }
+    };
+    if (lo < hi) load(lo);
+    for (uint64_t t0 = lo; t0 < hi; t0 += TP) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sx[(4 * q + e) * LD + j] = (uint16_t)v[e];
+        for (int k = 0; k < NCO; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k, j = u / (COUT / 4), q = u % (COUT / 4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sdz[(4 * q + e) * LD + j] = (uint16_t)vdz[k][e];
+        }
+#pragma unroll
+        for (int k = 0; k < NCI; ++k) {
+            const uint32_t u = threadIdx.x + 256u * k, j = u / (CIN / 4), q = u % (CIN / 4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sx[(4 * q + e) * LD + j] = (uint16_t)vx[k][e];
         }
         __syncthreads();
+        if (t0 + TP < hi) load(t0 + TP);
 #pragma unroll
         for (int i = 0; i < FW; ++i) {
             const int fr = (int)wave + 4 * i;
