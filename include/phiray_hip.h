@@ -297,6 +297,13 @@ int phr_tracer_open_pixels(phr_tracer* t, uint32_t* count);
  * fitted to 2 (default), 3, 4 or 5 waves a SIMD. */
 int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves);
 uint32_t phr_tracer_path_waves(const phr_tracer* t);
+/* The path-regenerating build (no counterpart in phiray): 1 lets a lane whose
+ * sample ends start its pixel's next sample at once instead of waiting for
+ * the wave's longest path (the same sums, bit for bit). It serves one tree
+ * and a camera with lamps or a lit world and no Adaptive Sampling; other
+ * frames take the other build. The autotune tries both. */
+int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on);
+uint32_t phr_tracer_path_regen(const phr_tracer* t);
 /* Autotune (no counterpart in phiray): time each variant on `spp` samples of
  * w x h under `cam` (best of `reps`), keep the fastest; the image starts over. */
 int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,

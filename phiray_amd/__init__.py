@@ -198,6 +198,8 @@ _SIGS = {
                                   C.POINTER(_I)]),
     "phr_tracer_open_pixels": (_I, [_P, C.POINTER(_U32)]),
     "phr_tracer_path_waves": (_U32, [_P]),
+    "phr_tracer_set_path_regen": (_I, [_P, _U32]),
+    "phr_tracer_path_regen": (_U32, [_P]),
     "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_tracer_denoise_learned": (_I, [_P, _P, _P]),
     "phr_batch_new": (_I, [_P, _P, _P, _U32, _U32, C.POINTER(_P)]),
@@ -769,6 +771,15 @@ class Tracer:
     @property
     def path_waves(self) -> int:
         return int(lib().phr_tracer_path_waves(self._h))
+
+    def set_path_regen(self, on: bool) -> None:
+        """The path-regenerating build on or off (one tree, a camera with lamps
+        or a lit world, no Adaptive Sampling; other frames take the other)."""
+        _check(lib().phr_tracer_set_path_regen(self._h, 1 if on else 0), "Tracer set_path_regen")
+
+    @property
+    def path_regen(self) -> bool:
+        return bool(lib().phr_tracer_path_regen(self._h))
 
     def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 2) -> tuple[int, float]:
         """Time each path kernel variant on this workload and keep the fastest:

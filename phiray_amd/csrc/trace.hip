@@ -927,6 +927,329 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
     }
 }
 
+// The same samples as path_block for one tree, with no Adaptive Sampling
+// and no studio rig (the camera has lamps or a lit world), with path
+// regeneration: a lane's sample is a state advanced one segment a turn (a
+// closest ray, its shading, at most one shadow ray, which every lane's turn
+// shares), and a lane whose sample ends starts its pixel's next sample on
+// the next turn instead of waiting for the wave's longest path. Each
+// sample's arithmetic and random numbers are shade's and indirect's, in the
+// same order, so the sums are the same bits.
+template <int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_rg(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    __shared__ uint32_t s_stack[stack_words<false>() * TRACE_THREADS];
+    const PhrCamera* p = camp;
+    const size_t n = (size_t)p->w * p->h;
+    const uint32_t blk = blockIdx.x + tile0;
+    const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
+    const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
+    const bool inside = pxl < p->w && pyl < p->h;
+    const uint32_t i = pyl * p->w + pxl;
+    Ctx c;
+    c.s = sc;
+    c.p = camp;
+    c.stk = s_stack + threadIdx.x;
+    c.closest = 0;
+    c.shadow = 0;
+    c.steps = 0;
+    c.tests = 0;
+    if (inside) {
+        const bool keep = frame0 > 0u;
+        V3 l = keep ? v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2)) : v3(0, 0, 0);
+        V3 ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
+        V3 gn = keep ? v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2)) : v3(0, 0, 0);
+        float gd = keep ? ldf(px, 13 * n + i) : 0.f;
+        float cov = keep ? ldf(px, 23 * n + i) : 0.f;
+        float gone = keep ? ldf(px, 33 * n + i) : 0.f;
+        uint32_t outline = 0;
+        // The sample's constants (shade.wgsl:41-64, integrate.wgsl:111-123).
+        const bool depth_set = p->depth[0] != 0u;
+        const uint32_t total = depth_set ? p->depth[0] - 1u : 8u;
+        const uint32_t see_through = depth_set ? p->depth[1] - 1u : 8u;
+        const uint32_t through = depth_set ? min(p->depth[2] - 1u, total) : 8u;
+        const bool diffuse_on = !depth_set || (p->bounces > 0u && total > 0u);
+        const uint32_t passes = see_through + through + 1u;
+        const bool lit = world_lit(p);
+        const bool ao_only = p->glow == 0u && !lit;
+        const uint32_t depth = diffuse_on ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
+        const V3 one = v3(1, 1, 1);  // shade's throughput, never changed
+        // The sample in flight. phase 0: a new sample; 1: the camera ray's
+        // segments (shade's pass loop); 2: the diffuse bounces.
+        uint32_t s = 0, seed = 0, pass = 0, n_see = 0, b = 0, phase = 0;
+        V3 o = v3(0, 0, 0), d = v3(0, 0, 0), col = v3(0, 0, 0);
+        V3 g_alb = v3(0, 0, 0), g_nrm = v3(0, 0, 0);
+        float g_dep = 0.f, g_missed = 0.f, g_cov = 0.f;
+        uint32_t g_outline = 0u;
+        V3 cc = v3(0, 0, 0), alb = v3(0, 0, 0);
+        float p_diffuse = 1.0f;
+        V3 acc = v3(0, 0, 0), thr = v3(1, 1, 1), nrm = v3(0, 0, 0);
+        float min_pdf = 3.0e38f;
+        while (s < spp) {
+            if (phase == 0u) {  // shade.wgsl:65-95
+                const float fw = (float)p->w, fh = (float)p->h;
+                const float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
+                const float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
+                seed = pcg_hash(i * 9781u + ((frame0 + s) + p->stream) * 6271u + 1u);
+                const float ju = rnd(&seed);
+                const float jv = rnd(&seed);
+                const float jx = (ju - 0.5f) / fw;
+                const float jy = (jv - 0.5f) / fh;
+                const float vx = (su + jx - p->pan[0]) / p->scale;
+                const float vy = -((sv + jy) / p->aspect - p->pan[1]) / p->scale;
+                if (p->persp != 0u) {
+                    o = ld3(p->pivot) + ld3(p->back) * (p->dist / p->units);
+                    d = normalize(ld3(p->right) * vx + ld3(p->up) * vy - ld3(p->back) * p->dist);
+                } else {
+                    o = ld3(p->pivot) + (ld3(p->right) * vx + ld3(p->up) * vy + ld3(p->back) * 1.0e4f) / p->units;
+                    d = -ld3(p->back);
+                }
+                col = v3(0, 0, 0);
+                g_missed = 0.f;
+                g_cov = 0.f;
+                g_outline = 0u;
+                pass = 0u;
+                n_see = 0u;
+                phase = 1u;
+            }
+            const PhrHit h = trace<false>(c, o, d, PHR_INF, false);
+            // This turn's shadow ray: a lamp for the direct light (qdirect)
+            // or the ambient-occlusion ray of a world without light.
+            bool qon = false, qdirect = false;
+            V3 qpos = v3(0, 0, 0), qdir = v3(0, 0, 1), qc = v3(0, 0, 0);
+            float qdist = 0.f, near = 1.0f;
+            V3 thr_was = thr;
+            // What ends this turn: 0 the sample goes on; 1 it ends with col as
+            // it is; 2 the camera ray's direct light then no bounce; 3 the
+            // ambient-occlusion bounce; 4 the diffuse bounces end.
+            uint32_t end = 0u;
+            bool to_bounces = false;
+            if (phase == 1u) {  // shade.wgsl:96-302, one pass
+                if (h.tri == PHR_NONE) {
+                    if (pass == 0u) {
+                        g_alb = v3(0, 0, 0);
+                        g_nrm = -d;
+                        g_dep = -1.0f;
+                        col = v3(0, 0, 0);
+                        g_missed = 1.0f;
+                        g_cov = 1.0f;
+                    } else {
+                        col = col + one * env_rgb(p);
+                    }
+                    end = 1u;
+                } else {
+                    if (pass == 0u) g_outline = h.inst + 1u;
+                    const uint32_t m = mat_at(c.s, h.tri);
+                    const Tri t = tri_of<false>(c.s, h.tri, h.inst, m);
+                    const float w = 1.0f - h.u - h.v;
+                    const V3 hit = o + d * h.t;
+                    const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+                    const bool front = dot(ng0, d) < 0.f;
+                    const V3 ng = front ? ng0 : -ng0;
+                    V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
+                    sn = dot(sn, sn) > 1e-12f ? normalize(sn) * (front ? 1.0f : -1.0f) : ng;
+                    if (dot(sn, -d) < 0.f) sn = ng;
+                    const float start = 1.0e4f / p->units;
+                    const float lift = (p->persp == 0u && h.t > start * 0.5f) ? p->eps + fabsf(h.t - start) * 1e-5f + start * 2.4e-7f
+                                                                              : p->eps + h.t * 1e-5f;
+                    const V3 pos = hit + ng * lift;
+                    const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+                    const float alpha = t.color[3];
+                    if (pass == 0u) {
+                        g_alb = base;
+                        g_nrm = sn;
+                        g_dep = h.t;
+                    }
+                    if (rnd(&seed) >= alpha) {
+                        n_see++;
+                        if (n_see > see_through) {
+                            end = 1u;
+                        } else {
+                            const float reach = fmax_(fmax_(fabsf(hit.x), fabsf(hit.y)), fmax_(fabsf(hit.z), 0.5f));
+                            const float tl = lift - p->eps + reach * (256.0f / 8388608.0f);
+                            o = hit - ng * tl;
+                            if (++pass >= passes) end = 1u;
+                        }
+                    } else {
+                        col = col + one * t.emit;
+                        const V3 fd = dielectric_f0(t.ior);
+                        V3 diffuse = base * (1.0f - t.metal);
+                        if (maxc(diffuse) > 0.f)
+                            diffuse = diffuse * under_layer_real(t.ior, fmax_(t.rough, 0.04f), fmax_(dot(sn, -d), 1e-4f));
+                        const Surf sf{pos, sn, -d, diffuse, mixv(fd, base, t.metal), fmax_(t.rough, 0.04f), t.ior, 1.0f - t.metal};
+                        // direct (integrate.wgsl:66-85) without the rig: the
+                        // lamp pick's number, then one lamp.
+                        (void)rnd(&seed);
+                        const Shadowed q = glow_query<false>(c.s, sf, &seed);
+                        if (maxc(q.c) > 0.f) {
+                            qon = qdirect = true;
+                            qpos = sf.pos;
+                            qdir = q.dir;
+                            qdist = q.dist;
+                            qc = q.c;
+                        }
+                        const float pd = dot(vmax(sf.albedo, 0.0f), v3(0.2126f, 0.7152f, 0.0722f));
+                        const float pg = 0.0f;
+                        p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
+                        const bool take_diffuse = rnd(&seed) < p_diffuse;
+                        alb = sf.albedo;
+                        end = 2u;
+                        if (take_diffuse && pd > 0.f) {
+                            if (ao_only) {  // integrate.wgsl:113-117
+                                qon = true;
+                                qpos = pos;
+                                qdir = cosine_dir(sn, &seed);
+                                qdist = p->ao;
+                                end = 3u;
+                            } else {  // the first bounce's ray (integrate.wgsl:124-128)
+                                acc = v3(0, 0, 0);
+                                thr = v3(1, 1, 1);
+                                o = pos;
+                                nrm = sn;
+                                min_pdf = 3.0e38f;
+                                b = 0u;
+                                d = cosine_dir(nrm, &seed);
+                                const float pb = fmax_(dot(nrm, d), 0.0f) / PHR_PI;
+                                min_pdf = fmin_(min_pdf, pb);
+                                end = 0u;
+                                to_bounces = true;
+                            }
+                        }
+                    }
+                }
+            } else {  // phase 2: bounce b (integrate.wgsl:124-228)
+                if (h.tri == PHR_NONE) {
+                    acc = acc + thr * env_rgb(p) * 1.0f;
+                    end = 4u;
+                } else if (!diffuse_on) {
+                    end = 4u;
+                } else {
+                    const Tri t = tri_of<false>(c.s, h.tri, h.inst, mat_at(c.s, h.tri));
+                    const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+                    const V3 ng = dot(ng0, d) < 0.f ? ng0 : -ng0;
+                    const V3 hit = o + d * h.t + ng * (p->eps + h.t * 1e-5f);
+                    const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+                    const float rough = fmax_(t.rough, glossy_floor(p, min_pdf));
+                    const bool glossy = p->integrator[0] < 0.5f || (((uint32_t)(p->integrator[3] + 0.5f)) & 1u) != 0u;
+                    const V3 fd = glossy ? dielectric_f0(t.ior) : v3(0, 0, 0);
+                    const float metal = glossy ? t.metal : 0.0f;
+                    const float tw = t.trans * (1.0f - t.metal);
+                    V3 albedo = base * (1.0f - t.metal) * (1.0f - tw);
+                    if (maxc(albedo) > 0.f) albedo = albedo * under_layer(fd, rough, fmax_(dot(ng, -d), 1e-4f));
+                    acc = acc + thr * glow_mis(c.s, t, h, nrm, d);
+                    near = (!lit && h.t <= p->ao) ? 0.25f : 1.0f;
+                    const Surf sf{hit, ng, -d, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
+                    (void)rnd(&seed);
+                    const Shadowed q = glow_query<false>(c.s, sf, &seed);
+                    if (maxc(q.c) > 0.f) {
+                        qon = qdirect = true;
+                        qpos = sf.pos;
+                        qdir = q.dir;
+                        qdist = q.dist;
+                        qc = q.c;
+                    }
+                    thr_was = thr;
+                    thr = thr * albedo;
+                    bool more = b + 1u < depth;
+                    if (more && b + 1u >= p->rr_from) {
+                        const float keep_p = clampf(maxc(thr), 0.0f, 0.95f);
+                        if (keep_p < 1e-4f || rnd(&seed) > keep_p) more = false;
+                        else thr = thr / keep_p;
+                    }
+                    if (more) {
+                        o = hit;
+                        nrm = ng;
+                        ++b;
+                        d = cosine_dir(nrm, &seed);
+                        const float pb = fmax_(dot(nrm, d), 0.0f) / PHR_PI;
+                        min_pdf = fmin_(min_pdf, pb);
+                    } else {
+                        end = 5u;  // after this turn's lamp: the bounces end
+                    }
+                }
+            }
+            // The turn's shadow ray, one call site for every lane.
+            float vis = 1.0f;
+            if (qon) vis = trace<false>(c, qpos, qdir, qdist, true).tri != PHR_NONE ? 0.0f : 1.0f;
+            V3 dsum = v3(0, 0, 0);
+            if (qdirect) dsum = dsum + qc * vis;
+            if (phase == 1u && (end >= 2u || to_bounces)) {
+                cc = dsum * 1.0f;
+                if (end == 2u) {
+                    cc = cc + v3(0, 0, 0);
+                    col = col + one * cc;
+                    end = 1u;
+                } else if (end == 3u) {
+                    V3 bounced = v3(0, 0, 0);
+                    bounced = bounced + alb * (env_rgb(p) * mixf(0.25f, 1.0f, vis)) / p_diffuse;
+                    cc = cc + bounced;
+                    col = col + one * cc;
+                    end = 1u;
+                }
+            } else if (phase == 2u) {
+                if (end == 5u) {
+                    acc = acc + thr_was * dsum * near;
+                    end = 4u;
+                } else if (end == 0u) {
+                    acc = acc + thr_was * dsum * near;
+                }
+                if (end == 4u) {
+                    V3 bounced = v3(0, 0, 0);
+                    bounced = bounced + alb * acc / p_diffuse;
+                    cc = cc + bounced;
+                    col = col + one * cc;
+                    end = 1u;
+                }
+            }
+            if (to_bounces) phase = 2u;
+            if (end == 1u) {  // shade's `out`, summed as path_block sums it
+                ga = ga + g_alb;
+                gn = gn + g_nrm;
+                gd = gd + g_dep;
+                gone = gone + g_missed;
+                l = l + col;
+                cov = cov + g_cov;
+                outline = g_outline;
+                ++s;
+                phase = 0u;
+            }
+        }
+        V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
+        float film = keep ? ldf(px, 24 * n + i) : 0.f;
+        uint32_t taken = keep ? px[31 * n + i] : 0u;
+        for (uint32_t k = 0; k < spp; ++k) {
+            gs = gs + v3(0.f, 0.f, 0.f);
+            film = film + 1.0f;
+        }
+        taken = taken + spp;
+        if (frame0 == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
+        stf(px, n + 3 * i, l.x); stf(px, n + 3 * i + 1, l.y); stf(px, n + 3 * i + 2, l.z);
+        stf(px, 7 * n + 3 * i, ga.x); stf(px, 7 * n + 3 * i + 1, ga.y); stf(px, 7 * n + 3 * i + 2, ga.z);
+        stf(px, 10 * n + 3 * i, gn.x); stf(px, 10 * n + 3 * i + 1, gn.y); stf(px, 10 * n + 3 * i + 2, gn.z);
+        stf(px, 13 * n + i, gd);
+        stf(px, 20 * n + 3 * i, gs.x); stf(px, 20 * n + 3 * i + 1, gs.y); stf(px, 20 * n + 3 * i + 2, gs.z);
+        stf(px, 23 * n + i, cov);
+        stf(px, 24 * n + i, film);
+        px[25 * n + i] = outline;
+        px[31 * n + i] = taken;
+        stf(px, 33 * n + i, gone);
+    }
+    unsigned long long rc = c.closest, rs = c.shadow, st = c.steps, tt = c.tests;
+    for (int k = 32; k > 0; k >>= 1) {
+        rc += __shfl_xor(rc, k);
+        rs += __shfl_xor(rs, k);
+        st += __shfl_xor(st, k);
+        tt += __shfl_xor(tt, k);
+    }
+    if ((threadIdx.x & 63) == 0 && (rc | rs)) {
+        atomicAdd(&rays[0], rc);
+        atomicAdd(&rays[1], rs);
+        atomicAdd(&rays[2], st);
+        atomicAdd(&rays[3], tt);
+    }
+}
+
 // One image: a block a tile, from tile0 on (a slice of the tiles).
 // WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
 // 3 to 5 trade spilled words for more waves in flight). The tracer's
@@ -966,10 +1289,30 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 // The builds of the path kernels the tracer and the batch choose among.
 constexpr uint32_t PATH_WAVES_MIN = 2, PATH_WAVES_MAX = 5;
 
-void launch_path(uint32_t waves, bool forest, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp, uint32_t tiles_x,
-                 uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed, uint32_t tile0) {
+// Whether k_path_rg serves a camera: no Adaptive Sampling (its half image and
+// pixel list are path_block's) and no studio rig (whose three lamps a turn
+// would not fit one shadow ray): a camera with lamps, or a lit world.
+bool regen_allowed(const PhrCamera& cam) {
+    const bool lit = cam.env[0] * cam.env[0] + cam.env[1] * cam.env[1] + cam.env[2] * cam.env[2] > 0.f;
+    return cam.adapt[3] == 0.0f && (cam.count != 0u || lit);
+}
+
+// regen: the path-regenerating build (k_path_rg), for one tree, no list and
+// no Adaptive Sampling or studio rig (the caller checks the camera).
+void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp,
+                 uint32_t tiles_x, uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed,
+                 uint32_t tile0) {
     const dim3 block(TRACE_THREADS);
-    if (forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
+    if (regen && !forest && !listed) {
+        if (waves == 5)
+            hipLaunchKernelGGL(k_path_rg<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 4)
+            hipLaunchKernelGGL(k_path_rg<4>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 3)
+            hipLaunchKernelGGL(k_path_rg<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else
+            hipLaunchKernelGGL(k_path_rg<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+    } else if (forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
         hipLaunchKernelGGL((k_path<true, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
     else if (waves == 5)
         hipLaunchKernelGGL((k_path<false, 5>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
@@ -1178,6 +1521,7 @@ struct phr_tracer {
     std::vector<uint32_t> order;          // slot -> input face
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
+    uint32_t path_regen = 0;  // 1: the path-regenerating build where the camera allows it
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -1869,11 +2213,12 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
     slices = std::min(std::max<uint32_t>(slices, 1u), tiles);
     bool abandoned = false;
     // One launch of `count` samples from f0, in `slices` pieces of the tiles.
+    const bool regen = t->path_regen && regen_allowed(cam);
     auto path = [&](uint32_t f0, uint32_t count, uint32_t listed) -> int {
         for (uint32_t k = 0; k < slices && !abandoned; ++k) {
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
             if (b == a) continue;
-            launch_path(t->path_waves, t->forest, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
+            launch_path(t->path_waves, t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
             PHR_HIP(hipGetLastError());
             if (slices > 1) {
                 PHR_HIP(hipStreamSynchronize(s));
@@ -1938,6 +2283,16 @@ int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves) {
 }
 uint32_t phr_tracer_path_waves(const phr_tracer* t) { return t ? t->path_waves : 0; }
 
+// The path-regenerating build on (1) or off (0); it runs only for one tree
+// and a camera without Adaptive Sampling or the studio rig, the other build
+// otherwise (no counterpart in phiray).
+int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on) {
+    PHR_CHECK(t && on <= 1u, PHR_ERR_ARG, "phr_tracer_set_path_regen: null, or not 0 or 1");
+    t->path_regen = on;
+    return PHR_OK;
+}
+uint32_t phr_tracer_path_regen(const phr_tracer* t) { return t ? t->path_regen : 0; }
+
 // Time every path kernel variant on the caller's workload (`spp` samples of
 // w x h under `cam`, best of `reps` launches each) and keep the fastest. The
 // image starts over; the variants compute the same sums.
@@ -1945,19 +2300,25 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
                         uint32_t* chosen, double* best_s) {
     PHR_CHECK(t && cam, PHR_ERR_ARG, "phr_tracer_autotune: null");
     reps = std::max<uint32_t>(reps, 1);
-    uint32_t best_w = 2;
+    uint32_t best_w = 2, best_r = 0;
     double best = 1e30;
-    for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? PATH_WAVES_MIN : PATH_WAVES_MAX); ++waves) {
-        t->path_waves = waves;
-        double mine = 1e30;
-        for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
-            t->have_last = false;  // start the image over every launch
-            int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
-            if (rc != PHR_OK) return rc;
-            if (r > 0) mine = std::min(mine, t->last_gpu_s);
+    PhrCamera probe;
+    std::memcpy(&probe, cam, sizeof probe);
+    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 2u : 1u;
+    for (uint32_t regen = 0; regen < regens; ++regen)
+        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? PATH_WAVES_MIN : PATH_WAVES_MAX); ++waves) {
+            t->path_waves = waves;
+            t->path_regen = regen;
+            double mine = 1e30;
+            for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
+                t->have_last = false;  // start the image over every launch
+                int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
+                if (rc != PHR_OK) return rc;
+                if (r > 0) mine = std::min(mine, t->last_gpu_s);
+            }
+            if (mine < best) best = mine, best_w = waves, best_r = regen;
         }
-        if (mine < best) best = mine, best_w = waves;
-    }
+    t->path_regen = best_r;
     t->path_waves = best_w;
     t->have_last = false;
     if (chosen) *chosen = best_w;
@@ -2112,7 +2473,8 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     uint32_t* pxp = (uint32_t*)t->px->ptr;
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    launch_path(t->path_waves, t->forest, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u);
+    launch_path(t->path_waves, t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
+                t->rays, 0u, 0u);
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0);
     if (denoise == 2) {

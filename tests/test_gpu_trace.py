@@ -119,18 +119,61 @@ def test_path_kernel_variants_agree(gpu, orc):
     for f in range(2):
         sc.sample(cam, w, h, px, f, nthreads=8)
     tr = P.Tracer.new(gpu, tris, 4)
-    assert tr.path_waves == 2
-    for waves in (2, 3, 4, 5):
-        tr.set_path_waves(waves)
-        fresh = cam.copy()
-        fresh["stream"] = 0
-        tr.sample(fresh, w, h, 2)
-        compare_planes(tr.px(), px, w, h)
-        tr.sample(S.path_camera(w + 1, h, rr_from=3), w + 1, h, 1)  # another view: the next call starts over
+    assert tr.path_waves == 2 and not tr.path_regen
+    for regen in (False, True):
+        tr.set_path_regen(regen)
+        for waves in (2, 3, 4, 5):
+            tr.set_path_waves(waves)
+            fresh = cam.copy()
+            fresh["stream"] = 0
+            tr.sample(fresh, w, h, 2)
+            compare_planes(tr.px(), px, w, h)
+            tr.sample(S.path_camera(w + 1, h, rr_from=3), w + 1, h, 1)  # another view: the next call starts over
     with pytest.raises(P.PhrError):
         tr.set_path_waves(6)
     chosen, best = tr.autotune(cam, w, h, 1, reps=1)
     assert chosen in (2, 3, 4, 5) and best > 0 and tr.path_waves == chosen
+
+
+def soup(n, seed):
+    corners, _ = S.lbvh_scene(n, seed)
+    corners = (corners - np.float32(0.5)) * np.float32(3.0)  # a 3 m cube of triangles
+    corners[..., 2] += np.float32(1.0)
+    return S.tris_from_corners(corners, color=(0.7, 0.6, 0.5, 1.0))
+
+
+@pytest.mark.parametrize("case", ["lamps", "world", "occlusion"])
+def test_path_regeneration_matches_the_twin(gpu, orc, case):
+    """The path-regenerating build (a lane that ends a sample starts its
+    pixel's next one): every plane equals the oracle's, samples spread over
+    three calls, for a scene lit by lamps, a soup under the world colour,
+    and a soup in a black world (the ambient-occlusion bounce)."""
+    print(f"[trace] path regeneration, {case}", flush=True)
+    w, h = 56, 40
+    cam = S.path_camera(w, h, rr_from=3)
+    if case == "lamps":
+        tris = S.spheres_scene(spheres=8, level=2, seed=11)
+    else:
+        tris = soup(3000, 7)
+        if case == "world":
+            cam["env"] = (0.4, 0.5, 0.6, 0.0)
+    sc = orc.Scene(tris, 4, P.NODE)
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    rays = [0, 0]
+    for f in range(5):
+        r = sc.sample(cam, w, h, px, f, nthreads=8)
+        rays = [rays[0] + r[0], rays[1] + r[1]]
+    tr = P.Tracer.new(gpu, tris, 4)
+    tr.set_path_regen(True)
+    tr.set_path_waves(4)
+    for spp in (1, 3, 1):
+        tr.sample(cam, w, h, spp)
+    compare_planes(tr.px(), px, w, h)
+    assert list(tr.rays()) == rays
+    if case == "occlusion":
+        assert rays[1] > 0  # the occlusion rays ran (a black world adds no light)
+    else:
+        assert P.planes(px, w, h)["radiance"].max() > 0
 
 
 def test_adaptive_sampling_matches_the_twin(gpu, orc):
@@ -221,7 +264,8 @@ def test_recorded_frame_replays_equal_eager(gpu):
 def test_full_frame_bands_match_the_twin(gpu, orc):
     """The full path workload: the 1,310,724-triangle spheres scene at
     1920x1080 (2 samples of the 64 the bench takes). Three bands of rows (top,
-    middle, bottom) traced by the oracle equal the device frame's planes."""
+    middle, bottom) traced by the oracle equal the device frame's planes, for
+    both path kernel builds."""
     print("[trace] full frame bands, 1.31M tris, 1920x1080", flush=True)
     tris = S.spheres_scene(spheres=64, level=5, seed=1)
     assert len(tris) == 1_310_724
@@ -232,6 +276,10 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     frame, _ = tr.sample(cam, w, h, spp)
     assert frame == spp - 1
     got = P.planes(tr.px(), w, h)
+    tr.set_path_regen(True)
+    tr.sample(S.path_camera(64, 64), 64, 64, 1)  # another view: the next call starts over
+    tr.sample(cam, w, h, spp)
+    regen = P.planes(tr.px(), w, h)
     sc = orc.Scene(tris, 4, P.NODE, 8)
     px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
     bands = [(0, 6), (h // 2 - 3, 6), (h - 6, 6)]
@@ -243,11 +291,12 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     for row, rows in bands:
         sl = slice(row * w, (row + rows) * w)
         for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline"]:
-            a, b = got[k][sl], want[k][sl]
-            if a.dtype == np.float32:
-                bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
-                assert not bad.any(), f"rows {row}+{rows} {k}: {bad.sum()} words beyond 1e-5 relative"
-            else:
-                assert (a == b).all(), (row, k)
+            for build, planes in (("plain", got), ("regen", regen)):
+                a, b = planes[k][sl], want[k][sl]
+                if a.dtype == np.float32:
+                    bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
+                    assert not bad.any(), f"{build} rows {row}+{rows} {k}: {bad.sum()} words beyond 1e-5 relative"
+                else:
+                    assert (a == b).all(), (build, row, k)
         lit += int((want["radiance"][sl] > 0).any(axis=1).sum())
     assert lit > 1000  # the bands cross lit surfaces
