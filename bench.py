@@ -330,7 +330,7 @@ def convergence(gpu: P.Gpu, tris: np.ndarray, cam: np.ndarray, w: int, h: int, r
             break
     res = {"reference_spp": ref_spp, "relmse": "mean((x - y)^2 / (y^2 + 0.01))", "raw": best["raw"],
            "plain": best["plain"], "learned": best["learned"], "trials": trials, "kpcn_rows_per_wave": tiles,
-           "path_waves_per_simd": waves}
+           "path_waves_per_simd": waves, "path_regen": tr.path_regen}
     if best["learned"] and best["raw"]:
         res["raw_over_learned"] = best["raw"]["s_per_frame"] / best["learned"]["s_per_frame"]
     if best["learned"] and best["plain"]:
@@ -605,6 +605,7 @@ def main() -> int:
                             "launch_ms": kernel_s * 1e3}
         line["s_per_frame"] = per
         line["path_waves_per_simd"] = tracer.path_waves
+        line["path_regen"] = tracer.path_regen
         line["rays_per_frame"] = rays / args.steps
         line["closest_per_frame"] = closest / args.steps
         line["shadow_per_frame"] = shadow / args.steps

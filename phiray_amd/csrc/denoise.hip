@@ -11,13 +11,24 @@ using namespace phr;
 
 namespace {
 
+// The image buffer as the a-trous passes read it. CACHED: each pixel's guides
+// (sRGB albedo, unit normal, mean depth, 1 / count) and the first pass's
+// input come from a table the prep kernel wrote with these same functions,
+// instead of being derived again at every tap of every pass.
+template <bool CACHED>
 struct Img {
     const uint32_t* __restrict__ px;
     size_t n;
     uint32_t round;
+    const float4* __restrict__ feat;  // CACHED: 2 a pixel, (albedo, depth) (normal, 1 / count)
+    const float* __restrict__ col0;   // CACHED: the first pass's input, 3 a pixel
     __device__ float f_at(size_t i) const { return __uint_as_float(px[i]); }
     __device__ V3 v3_at(size_t base, size_t i) const { size_t o = base + 3 * i; return v3(f_at(o), f_at(o + 1), f_at(o + 2)); }
-    __device__ float inv_n(size_t i) const { uint32_t c = px[31 * n + i]; return 1.0f / (float)(c > 1u ? c : 1u); }
+    __device__ float inv_n(size_t i) const {
+        if constexpr (CACHED) return feat[2 * i + 1].w;
+        uint32_t c = px[31 * n + i];
+        return 1.0f / (float)(c > 1u ? c : 1u);
+    }
     __device__ static float srgb1(float c) {  // view.wgsl:88-93
         float x = fmin_(fmax_(c, 0.0f), 1.0f);
         float lo = x * 12.92f;
@@ -25,13 +36,27 @@ struct Img {
         return x <= 0.0031308f ? lo : hi;
     }
     __device__ V3 albedo_of(size_t i) const {
+        if constexpr (CACHED) {
+            const float4 f = feat[2 * i];
+            return v3(f.x, f.y, f.z);
+        }
         V3 a = v3_at(7 * n, i) * inv_n(i);
         return v3(srgb1(a.x), srgb1(a.y), srgb1(a.z));
     }
-    __device__ V3 normal_of(size_t i) const { return normalize(v3_at(10 * n, i) + v3(1e-9f, 1e-9f, 1e-9f)); }
-    __device__ float depth_of(size_t i) const { return f_at(13 * n + i) * inv_n(i); }
+    __device__ V3 normal_of(size_t i) const {
+        if constexpr (CACHED) {
+            const float4 f = feat[2 * i + 1];
+            return v3(f.x, f.y, f.z);
+        }
+        return normalize(v3_at(10 * n, i) + v3(1e-9f, 1e-9f, 1e-9f));
+    }
+    __device__ float depth_of(size_t i) const {
+        if constexpr (CACHED) return feat[2 * i].w;
+        return f_at(13 * n + i) * inv_n(i);
+    }
     __device__ V3 read_in(size_t i) const {
         if (round == 0u) {
+            if constexpr (CACHED) return v3(col0[3 * i], col0[3 * i + 1], col0[3 * i + 2]);
             V3 a = v3_at(1 * n, i) * inv_n(i);
             V3 b = vmax(albedo_of(i), 0.05f);
             return v3(a.x / b.x, a.y / b.y, a.z / b.z);
@@ -42,13 +67,29 @@ struct Img {
 
 __device__ __forceinline__ float luma_of(V3 c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }
 
+// The guide table and the first pass's input, once a denoise.
+__global__ __launch_bounds__(256) void k_denoise_prep(const uint32_t* __restrict__ px, size_t n, float4* __restrict__ feat,
+                                                      float* __restrict__ col0) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Img<false> im{px, n, 0u, nullptr, nullptr};
+    const V3 a = im.albedo_of(i), nr = im.normal_of(i), c = im.read_in(i);
+    feat[2 * i] = make_float4(a.x, a.y, a.z, im.depth_of(i));
+    feat[2 * i + 1] = make_float4(nr.x, nr.y, nr.z, im.inv_n(i));
+    col0[3 * i] = c.x;
+    col0[3 * i + 1] = c.y;
+    col0[3 * i + 2] = c.z;
+}
+
+template <bool CACHED>
 __global__ __launch_bounds__(256) void k_denoise(uint32_t* __restrict__ px, uint32_t w, uint32_t h, uint32_t step, uint32_t round,
-                                                 uint32_t last, uint32_t keep, float s0, float s1, float s2, float s3) {
+                                                 uint32_t last, uint32_t keep, float s0, float s1, float s2, float s3,
+                                                 const float4* __restrict__ feat, const float* __restrict__ col0) {
     const size_t n = (size_t)w * h;
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float KERNEL[3] = {0.375f, 0.25f, 0.0625f};
-    Img im{px, n, round};
+    Img<CACHED> im{px, n, round, feat, col0};
     int x = (int)(i % w), y = (int)(i / w);
     V3 centre = im.read_in(i);
     if (round == 0u) {
@@ -113,21 +154,33 @@ int phr_denoise_pass_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, u
                              uint32_t keep, const float s[4]) {
     const size_t n = (size_t)w * h;
     if (n == 0) return PHR_OK;
-    hipLaunchKernelGGL(k_denoise, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, px, w, h, step, round, last,
-                       keep, s[0], s[1], s[2], s[3]);
+    hipLaunchKernelGGL(k_denoise<false>, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, px, w, h, step, round, last,
+                       keep, s[0], s[1], s[2], s[3], nullptr, nullptr);
     PHR_HIP(hipGetLastError());
     return PHR_OK;
 }
 
 // Enqueue the four passes of Tracer::denoise_with over an image buffer of w x h.
-int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep) {
+// With `scratch` (DENOISE_SCRATCH_WORDS words a pixel, 16-B aligned) the
+// guides are derived once into it; without, every pass derives them again.
+int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep, float* scratch) {
     const float s[4] = {64.0f, 8.0f, 40.0f, 7.0f};
     const uint32_t steps[4] = {1, 2, 4, 8};
     const size_t n = (size_t)w * h;
     if (n == 0) return PHR_OK;
-    for (uint32_t r = 0; r < 4; ++r)
-        hipLaunchKernelGGL(k_denoise, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, px, w, h, steps[r], r,
-                           steps[r] == 8 ? 1u : 0u, keep ? 1u : 0u, s[0], s[1], s[2], s[3]);
+    const dim3 grid((uint32_t)((n + 255) / 256));
+    if (scratch) {
+        float4* feat = (float4*)scratch;
+        float* col0 = scratch + 8 * n;
+        hipLaunchKernelGGL(k_denoise_prep, grid, dim3(256), 0, g->stream, (const uint32_t*)px, n, feat, col0);
+        for (uint32_t r = 0; r < 4; ++r)
+            hipLaunchKernelGGL(k_denoise<true>, grid, dim3(256), 0, g->stream, px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u,
+                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], (const float4*)feat, (const float*)col0);
+    } else {
+        for (uint32_t r = 0; r < 4; ++r)
+            hipLaunchKernelGGL(k_denoise<false>, grid, dim3(256), 0, g->stream, px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u,
+                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], nullptr, nullptr);
+    }
     PHR_HIP(hipGetLastError());
     return PHR_OK;
 }

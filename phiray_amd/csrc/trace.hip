@@ -1500,7 +1500,8 @@ phr_array* phr_bvh_internal_order(phr_bvh* b);
 extern "C" void phr_bvh_free(phr_bvh* b);
 extern "C" int phr_bvh_refit_async(phr_bvh* b);
 extern "C" uint32_t phr_bvh_depth(const phr_bvh* b);
-int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep);
+int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep, float* scratch);
+constexpr uint64_t DENOISE_SCRATCH_WORDS = 11;  // guides (8) and the first pass's input (3)
 
 struct phr_tracer {
     phr_gpu* gpu = nullptr;
@@ -1547,6 +1548,8 @@ struct phr_tracer {
     uint32_t* bases_dev = nullptr;      // each mesh tree's root node, or NONE
     std::vector<uint32_t> placed, node_base;
     float* frame_dev = nullptr;  // the learned denoiser's input and output, sized for w x h
+    float* dn_scratch = nullptr;  // the a-trous passes' guide table, DENOISE_SCRATCH_WORDS a pixel
+    uint64_t dn_px = 0;
     uint64_t frame_px = 0;
     // A new pose is applied on the device without waiting: the instances'
     // matrices, each instance's mesh, the emitter candidates (face slot,
@@ -1568,6 +1571,7 @@ struct phr_tracer {
     uint32_t fw = 0, fh = 0, fspp = 0;
     SceneDev fsc{};
     const void* fsig[4] = {nullptr, nullptr, nullptr, nullptr};
+    int fdenoise = 0;  // the recorded frame's denoiser: 0 none, 1 a-trous, 2 learned
 };
 
 namespace {
@@ -1604,6 +1608,7 @@ void tracer_release(phr_tracer* t) {
         phr_array_release(a);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->frame_dev) (void)hipFree(t->frame_dev);
+    if (t->dn_scratch) (void)hipFree(t->dn_scratch);
     if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
     if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
     if (t->rays) (void)hipFree(t->rays);
@@ -2401,6 +2406,18 @@ __global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, u
     for (int j = 0; j < 3; ++j) px[1 * n + 3 * i + j] = __float_as_uint(rad[3 * i + j] * k);
 }
 
+// The a-trous passes' guide table for the current image size.
+int ensure_denoise_scratch(phr_tracer* t) {
+    const uint64_t n = (uint64_t)t->w * t->h;
+    if (t->dn_scratch && t->dn_px == n) return PHR_OK;
+    if (t->dn_scratch) (void)hipFree(t->dn_scratch);
+    t->dn_scratch = nullptr;
+    t->dn_px = 0;
+    PHR_HIP(hipMalloc((void**)&t->dn_scratch, std::max<uint64_t>(n, 1) * DENOISE_SCRATCH_WORDS * 4));
+    t->dn_px = n;
+    return PHR_OK;
+}
+
 int ensure_frame(phr_tracer* t) {
     const uint64_t n = (uint64_t)t->w * t->h;
     if (t->frame_dev && t->frame_px == n) return PHR_OK;
@@ -2460,6 +2477,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
         int rc = phr_tracer_sample(t, &fresh, w, h, spp, nullptr, nullptr);
         if (rc != PHR_OK) return rc;
         if (denoise == 2 && (rc = phr_tracer_denoise_learned(t, k, nullptr)) != PHR_OK) return rc;
+        if (denoise == 1 && (rc = ensure_denoise_scratch(t)) != PHR_OK) return rc;
     }
     if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
     if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
@@ -2476,7 +2494,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     launch_path(t->path_waves, t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
                 t->rays, 0u, 0u);
     int rc = PHR_OK;
-    if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0);
+    if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch);
     if (denoise == 2) {
         const uint32_t g = (uint32_t)((n + 255) / 256);
         float* rad = t->frame_dev + 10 * n;
@@ -2503,9 +2521,10 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     t->fw = t->w, t->fh = t->h, t->fspp = spp;
     t->fsc = sc;
     t->fsig[0] = t->px->ptr;
-    t->fsig[1] = denoise == 2 ? (const void*)t->frame_dev : nullptr;
+    t->fsig[1] = denoise == 2 ? (const void*)t->frame_dev : denoise == 1 ? (const void*)t->dn_scratch : nullptr;
     t->fsig[2] = denoise == 2 ? phr_kpcn_internal_workspace(k) : nullptr;
     t->fsig[3] = t->cam_dev;
+    t->fdenoise = denoise;
     return PHR_OK;
 }
 
@@ -2514,7 +2533,8 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
 int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k) {
     PHR_CHECK(t && t->fexec, PHR_ERR_ARG, "phr_tracer_replay_frame: nothing recorded");
     const bool moved = !t->px || t->px->ptr != t->fsig[0] || t->w != t->fw || t->h != t->fh || !same_scene(scene_dev(t), t->fsc) ||
-                       (t->fsig[1] && (t->frame_dev != t->fsig[1] || !k || phr_kpcn_internal_workspace(k) != t->fsig[2]));
+                       (t->fdenoise == 2 && (t->frame_dev != t->fsig[1] || !k || phr_kpcn_internal_workspace(k) != t->fsig[2])) ||
+                       (t->fdenoise == 1 && t->dn_scratch != t->fsig[1]);
     PHR_CHECK(!moved, PHR_ERR_ARG, "phr_tracer_replay_frame: the scene, the image or the network moved; record again");
     hipStream_t s = t->gpu->stream;
     PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->fcam, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
@@ -2562,7 +2582,9 @@ int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
 int phr_tracer_denoise(phr_tracer* t, int keep) {
     PHR_CHECK(t, PHR_ERR_ARG, "phr_tracer_denoise: null");
     if (!t->px) return PHR_OK;  // nothing traced: nothing to filter
-    int rc = phr_denoise_enqueue(t->gpu, (uint32_t*)t->px->ptr, t->w, t->h, keep);
+    int rc = ensure_denoise_scratch(t);
+    if (rc != PHR_OK) return rc;
+    rc = phr_denoise_enqueue(t->gpu, (uint32_t*)t->px->ptr, t->w, t->h, keep, t->dn_scratch);
     if (rc != PHR_OK) return rc;
     PHR_HIP(hipStreamSynchronize(t->gpu->stream));
     return PHR_OK;
