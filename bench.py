@@ -197,6 +197,7 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
     g_s, wall_s = clock.stop()
     cam = S.path_camera(w, h, eye=(0.0, -16.0, 9.0), target=(0.0, 0.0, 1.5), rr_from=3)
     tr.set_samples(1 << 20)
+    waves, _ = tr.autotune(cam, w, h, spp, reps=1)
     tr.sample(cam, w, h, spp)
     c0 = tr.counters()
     clock.start()
@@ -207,7 +208,7 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
     return {"bodies": len(worlds), "meshes": len(me), "tris": len(f), "build_s": build_s,
             "substep_ms": wall_s / substeps * 1e3, "substep_gpu_span_ms": g_s / substeps * 1e3,
             "frame": f"{w}x{h}, {spp} spp", "frame_ms": t_frame * 1e3, "mrays_s": rays / t_frame / 1e6,
-            "rays_per_frame": rays}
+            "rays_per_frame": rays, "path_waves_per_simd": waves}
 
 
 def xpbd_rates(gpu: P.Gpu, frames: int = 30, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
@@ -232,6 +233,7 @@ def xpbd_rates(gpu: P.Gpu, frames: int = 30, substeps: int = 8, w: int = 1280, h
     verts = P.GpuArray.from_(gpu, v)
     cam = S.path_camera(w, h, eye=(0.0, -16.0, 9.0), target=(0.0, 0.0, 1.5), rr_from=3)
     tr.set_samples(1 << 20)
+    tr.autotune(cam, w, h, spp, reps=1)
     phys, scene, trace, rays, pairs = 0.0, 0.0, 0.0, 0, 0
     z0 = float(world.bodies()["x"][1:, 2].mean())
     for fr in range(frames):

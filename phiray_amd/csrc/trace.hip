@@ -275,17 +275,17 @@ __device__ __forceinline__ bool walk_blas(const SceneDev& s, uint32_t i, V3 ow, 
 
 // trace (bvh.wgsl:194-232): the top tree over the instances' world boxes; a
 // leaf's slots name the instances (inst_order) whose trees the ray walks.
-// The top tree's stack is the column's first PHR_STACK words, a mesh tree's
-// the next PHR_STACK.
+// The top tree's stack holds 16-bit node ids (its nodes are the array's
+// first 2 insts - 1, fewer than 65,536: the tracer refuses more instances),
+// a mesh tree's stack 32-bit ones; each is the thread's column of stride.
 template <bool ANY>
-__device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint32_t* stk, uint32_t stride, uint32_t& steps,
-                              uint32_t& tests) {
+__device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint16_t* tstk, uint32_t* bstk, uint32_t stride,
+                              uint32_t& steps, uint32_t& tests) {
     PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
     const uint32_t* inst_order = s.insts + INSTANCE_LEN * (size_t)s.ninsts;
     const V3 inv = safe_inv(d);
     float4 lo = s.nodes[0], hi = s.nodes[1];
     if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
-    uint32_t* bstk = stk + PHR_STACK * stride;
     uint32_t sp = 0;
     for (;;) {
         uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
@@ -301,8 +301,8 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint32_
             float tl = box_t(llo, lhi, o, inv, best.t);
             float tr = box_t(rlo, rhi, o, inv, best.t);
             if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) { stk[sp * stride] = ri; lo = llo; hi = lhi; }
-                else { stk[sp * stride] = li; lo = rlo; hi = rhi; }
+                if (tl <= tr) { tstk[sp * stride] = (uint16_t)ri; lo = llo; hi = lhi; }
+                else { tstk[sp * stride] = (uint16_t)li; lo = rlo; hi = rhi; }
                 ++sp;
                 continue;
             }
@@ -311,7 +311,7 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint32_
         }
         if (sp == 0) break;
         --sp;
-        uint32_t x = stk[sp * stride];
+        uint32_t x = tstk[sp * stride];
         lo = s.nodes[2 * x];
         hi = s.nodes[2 * x + 1];
     }
@@ -319,14 +319,16 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint32_
 }
 
 constexpr uint32_t TRACE_THREADS = 256;
-// LDS stack words a thread: one tree, or the top tree and a mesh tree.
+// LDS stacks a thread: PHR_STACK words for one tree or a mesh tree, and for a
+// forest PHR_STACK 16-bit entries for the top tree.
 template <bool F>
-constexpr uint32_t stack_words() { return F ? 2 * PHR_STACK : PHR_STACK; }
+constexpr uint32_t top_stack_entries() { return F ? PHR_STACK : 1u; }
 
 template <bool F>
 __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const PhrRay* __restrict__ rays, uint32_t nrays, uint32_t any,
                                                            PhrHit* __restrict__ hits, unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint16_t s_tstack[top_stack_entries<F>() * TRACE_THREADS];
     const uint32_t i = blockIdx.x * TRACE_THREADS + threadIdx.x;
     uint32_t steps = 0, tests = 0;
     if (i < nrays) {
@@ -336,8 +338,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const Ph
         uint32_t* stk = s_stack + threadIdx.x;
         PhrHit h;
         if constexpr (F)
-            h = any ? walk_forest<true>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests)
-                    : walk_forest<false>(sc, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+            h = any ? walk_forest<true>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests)
+                    : walk_forest<false>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests);
         else
             h = any ? walk<true>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
                     : walk<false>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
@@ -396,6 +398,7 @@ struct Ctx {
     SceneDev s;
     const PhrCamera* p;
     uint32_t* stk;
+    uint16_t* tstk;  // a forest's top-tree stack
     uint32_t closest, shadow, steps, tests;
 };
 
@@ -405,11 +408,11 @@ template <bool F>
 __device__ __forceinline__ PhrHit trace(Ctx& c, V3 o, V3 d, float tmax, bool first) {
     if (first) {
         c.shadow++;
-        if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+        if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
         return walk<true>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
     }
     c.closest++;
-    if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+    if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
     return walk<false>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
 }
 template <bool F>
@@ -834,7 +837,8 @@ __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = _
 template <bool F>
 __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* __restrict__ camp, uint32_t blk, uint32_t tiles_x,
                                            uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                           unsigned long long* __restrict__ rays, uint32_t listed, uint32_t* s_stack) {
+                                           unsigned long long* __restrict__ rays, uint32_t listed, uint32_t* s_stack,
+                                           uint16_t* s_tstack) {
     const PhrCamera& cam = *camp;
     const size_t n = (size_t)cam.w * cam.h;
     // A block traces a 16 x 16 tile, a wave 16 x 4 pixels of it: neighbouring
@@ -859,6 +863,7 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
     c.s = sc;
     c.p = camp;
     c.stk = s_stack + threadIdx.x;
+    c.tstk = s_tstack + (F ? threadIdx.x : 0u);
     c.closest = 0;
     c.shadow = 0;
     c.steps = 0;
@@ -939,7 +944,7 @@ template <int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_rg(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
     unsigned long long* __restrict__ rays, uint32_t tile0) {
-    __shared__ uint32_t s_stack[stack_words<false>() * TRACE_THREADS];
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
     const PhrCamera* p = camp;
     const size_t n = (size_t)p->w * p->h;
     const uint32_t blk = blockIdx.x + tile0;
@@ -951,6 +956,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     c.s = sc;
     c.p = camp;
     c.stk = s_stack + threadIdx.x;
+    c.tstk = nullptr;
     c.closest = 0;
     c.shadow = 0;
     c.steps = 0;
@@ -1258,8 +1264,9 @@ template <bool F, int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
     unsigned long long* __restrict__ rays, uint32_t listed, uint32_t tile0) {
-    __shared__ uint32_t s_stack[stack_words<F>() * TRACE_THREADS];
-    path_block<F>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, listed, s_stack);
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint16_t s_tstack[top_stack_entries<F>() * TRACE_THREADS];
+    path_block<F>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, listed, s_stack, s_tstack);
 }
 
 // A batch of scenes in one launch: `tiles` blocks a scene, each scene with
@@ -1272,7 +1279,8 @@ template <int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_batch(
     const SceneDev* __restrict__ scenes, const PhrCamera* __restrict__ cams, uint32_t tiles, uint32_t tiles_x, uint32_t spp,
     uint32_t* __restrict__ px, uint64_t px_stride, unsigned long long* __restrict__ rays, uint32_t grouped) {
-    __shared__ uint32_t s_stack[stack_words<false>() * TRACE_THREADS];
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint16_t s_tstack[1];
     uint32_t scene, tile;
     if (grouped) {
         const uint32_t x = blockIdx.x % 8u, q = blockIdx.x / 8u;
@@ -1283,7 +1291,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         tile = blockIdx.x % tiles;
     }
     const SceneDev sc = scenes[scene];
-    path_block<false>(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, 0u, s_stack);
+    path_block<false>(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, 0u, s_stack, s_tstack);
 }
 
 // The builds of the path kernels the tracer and the batch choose among.
@@ -1312,7 +1320,9 @@ void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t
             hipLaunchKernelGGL(k_path_rg<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
         else
             hipLaunchKernelGGL(k_path_rg<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
-    } else if (forest)  // its 64-word LDS stack holds it at two waves a SIMD anyway
+    } else if (forest && waves >= 3)  // its stacks (192 B a thread) hold it at three waves a SIMD at most
+        hipLaunchKernelGGL((k_path<true, 3>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (forest)
         hipLaunchKernelGGL((k_path<true, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
     else if (waves == 5)
         hipLaunchKernelGGL((k_path<false, 5>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
@@ -2311,7 +2321,7 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     std::memcpy(&probe, cam, sizeof probe);
     const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 2u : 1u;
     for (uint32_t regen = 0; regen < regens; ++regen)
-        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? PATH_WAVES_MIN : PATH_WAVES_MAX); ++waves) {
+        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? 3u : PATH_WAVES_MAX); ++waves) {
             t->path_waves = waves;
             t->path_regen = regen;
             double mine = 1e30;
@@ -2643,6 +2653,8 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
     PHR_CHECK(d->nfaces > 0 && d->nmeshes > 0 && d->ninsts > 0 && d->verts && d->faces && d->mats && d->meshes && d->insts,
               PHR_ERR_ARG, "phr_tracer_from_scene: empty scene");
     PHR_CHECK(d->nmeshes <= 65536, PHR_ERR_SIZE, "phr_tracer_from_scene: at most 65,536 meshes");
+    // The top tree's nodes (2 insts - 1) are walked with 16-bit stack entries.
+    PHR_CHECK(d->ninsts <= 32768, PHR_ERR_SIZE, "phr_tracer_from_scene: at most 32,768 instances");
     const uint32_t* fcs = (const uint32_t*)d->faces;
     const float* mts = (const float*)d->mats;
     for (uint32_t f = 0; f < d->nfaces; ++f) {

@@ -154,17 +154,20 @@ def test_forest_path_matches_the_twin(gpu, scene, spp):
     w, h = 48, 32
     cam = S.path_camera(w, h, eye=(0.0, -7.0, 3.5), target=(0.0, 0.5, 0.8), rr_from=2)
     t.set_samples(64)
-    t.sample(cam, w, h, spp)
-    got = t.px()
     want = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
     for f in range(spp):
         fo.sample(cam, w, h, want, f, 8)
     n = w * h
-    for lo, hi in [(1 * n, 4 * n), (7 * n, 14 * n)]:
-        a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)
-        bad = ~np.isclose(a, b, rtol=1e-5, atol=1e-7)
-        assert not bad.any(), f"words {lo}..{hi}: {bad.sum()} beyond 1e-5, first {np.nonzero(bad)[0][:5] + lo}"
-    assert (got[31 * n:32 * n] == want[31 * n:32 * n]).all()
+    for waves in (2, 3):  # the forest kernel's two register fits
+        t.set_path_waves(waves)
+        t.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next call starts over
+        t.sample(cam, w, h, spp)
+        got = t.px()
+        for lo, hi in [(1 * n, 4 * n), (7 * n, 14 * n)]:
+            a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)
+            bad = ~np.isclose(a, b, rtol=1e-5, atol=1e-7)
+            assert not bad.any(), f"waves {waves} words {lo}..{hi}: {bad.sum()} beyond 1e-5, first {np.nonzero(bad)[0][:5] + lo}"
+        assert (got[31 * n:32 * n] == want[31 * n:32 * n]).all()
 
 
 def test_reinstance_matches_the_oracle(gpu, orc):
