@@ -323,6 +323,12 @@ constexpr uint32_t TRACE_THREADS = 256;
 // forest PHR_STACK 16-bit entries for the top tree.
 template <bool F>
 constexpr uint32_t top_stack_entries() { return F ? PHR_STACK : 1u; }
+// The forest's 4-waves-a-SIMD build keeps 24 mesh-tree entries (40 KB of LDS
+// a block, four blocks a CU): the tracer runs it only when every mesh tree
+// is at most 24 deep, a walk holding at most one pending node a level.
+constexpr uint32_t FOREST_SHORT_STACK = 24;
+template <bool F, int WPE>
+constexpr uint32_t mesh_stack_entries() { return (F && WPE >= 4) ? FOREST_SHORT_STACK : PHR_STACK; }
 
 template <bool F>
 __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const PhrRay* __restrict__ rays, uint32_t nrays, uint32_t any,
@@ -1264,7 +1270,7 @@ template <bool F, int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
     unsigned long long* __restrict__ rays, uint32_t listed, uint32_t tile0) {
-    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    __shared__ uint32_t s_stack[mesh_stack_entries<F, WPE>() * TRACE_THREADS];
     __shared__ uint16_t s_tstack[top_stack_entries<F>() * TRACE_THREADS];
     path_block<F>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, listed, s_stack, s_tstack);
 }
@@ -1320,7 +1326,9 @@ void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t
             hipLaunchKernelGGL(k_path_rg<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
         else
             hipLaunchKernelGGL(k_path_rg<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
-    } else if (forest && waves >= 3)  // its stacks (192 B a thread) hold it at three waves a SIMD at most
+    } else if (forest && waves >= 4)  // the caller checked the mesh trees' depth
+        hipLaunchKernelGGL((k_path<true, 4>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (forest && waves == 3)  // 32 + 32 16-bit entries (192 B a thread): three blocks a CU
         hipLaunchKernelGGL((k_path<true, 3>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
     else if (forest)
         hipLaunchKernelGGL((k_path<true, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
@@ -1504,6 +1512,8 @@ int phr_bvh_internal_build(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh*
 int phr_bvh_internal_build_into(phr_gpu* g, phr_array* prims, uint32_t leaf, const uint32_t* seg_lo, uint32_t segs,
                                 phr_array* nodes, const uint32_t* node_base, uint32_t leaf_flag, phr_bvh** out);
 int phr_bvh_internal_rebuild_async(phr_bvh* b);
+int phr_bvh_internal_read_depth(phr_bvh* b);
+uint32_t phr_bvh_internal_segment_depth(const phr_bvh* b, uint32_t s);
 int phr_bvh_internal_refit_async(phr_bvh* b);
 phr_array* phr_bvh_internal_nodes(phr_bvh* b);
 phr_array* phr_bvh_internal_order(phr_bvh* b);
@@ -1533,6 +1543,7 @@ struct phr_tracer {
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
     uint32_t path_regen = 0;  // 1: the path-regenerating build where the camera allows it
+    uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -1608,6 +1619,23 @@ struct WordsHash {  // FNV-1a over the words
         return (size_t)h;
     }
 };
+
+// The build the path launches take: a forest's 4-wave build only over mesh
+// trees of at most FOREST_SHORT_STACK levels, else its 3-wave one.
+uint32_t path_build_waves(const phr_tracer* t) {
+    if (t->forest && t->path_waves >= 4) return t->mesh_depth <= FOREST_SHORT_STACK ? 4u : 3u;
+    return t->path_waves;
+}
+
+// The deepest mesh tree of a forest, read back after a build.
+int forest_mesh_depth(phr_tracer* t) {
+    int rc = phr_bvh_internal_read_depth(t->bvh);
+    if (rc != PHR_OK) return rc;
+    uint32_t d = 0;
+    for (uint32_t m = 0; m < t->nmeshes; ++m) d = std::max(d, phr_bvh_internal_segment_depth(t->bvh, m));
+    t->mesh_depth = d;
+    return PHR_OK;
+}
 
 void tracer_release(phr_tracer* t) {
     if (!t) return;
@@ -2233,7 +2261,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
         for (uint32_t k = 0; k < slices && !abandoned; ++k) {
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
             if (b == a) continue;
-            launch_path(t->path_waves, t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
+            launch_path(path_build_waves(t), t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
             PHR_HIP(hipGetLastError());
             if (slices > 1) {
                 PHR_HIP(hipStreamSynchronize(s));
@@ -2321,7 +2349,8 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     std::memcpy(&probe, cam, sizeof probe);
     const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 2u : 1u;
     for (uint32_t regen = 0; regen < regens; ++regen)
-        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? 3u : PATH_WAVES_MAX); ++waves) {
+        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? (t->mesh_depth <= FOREST_SHORT_STACK ? 4u : 3u) : PATH_WAVES_MAX);
+             ++waves) {
             t->path_waves = waves;
             t->path_regen = regen;
             double mine = 1e30;
@@ -2501,7 +2530,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     uint32_t* pxp = (uint32_t*)t->px->ptr;
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    launch_path(t->path_waves, t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
+    launch_path(path_build_waves(t), t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
                 t->rays, 0u, 0u);
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch);
@@ -2753,6 +2782,7 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
     if ((rc = phr_bvh_internal_build_into(g, t->prims, leaf, seg_lo.data(), t->nmeshes, t->nodes, t->node_base.data(), PHR_LEAF,
                                           &t->bvh)) != PHR_OK)
         return fail(rc);
+    if ((rc = forest_mesh_depth(t)) != PHR_OK) return fail(rc);
     if ((rc = forest_gather(t)) != PHR_OK || (rc = forest_light_cands(t)) != PHR_OK || (rc = forest_top(t, true)) != PHR_OK ||
         (rc = forest_lights(t)) != PHR_OK || (rc = forest_settle(t)) != PHR_OK)
         return fail(rc);
@@ -2821,7 +2851,7 @@ int phr_tracer_retree(phr_tracer* t, int* ok) {
     *ok = 0;
     if (!t->forest) return PHR_OK;
     int rc;
-    if ((rc = phr_bvh_internal_rebuild_async(t->bvh)) != PHR_OK || (rc = forest_gather(t)) != PHR_OK ||
+    if ((rc = phr_bvh_internal_rebuild_async(t->bvh)) != PHR_OK || (rc = forest_mesh_depth(t)) != PHR_OK || (rc = forest_gather(t)) != PHR_OK ||
         (rc = forest_light_cands(t)) != PHR_OK || (rc = forest_top(t, false)) != PHR_OK || (rc = forest_lights(t)) != PHR_OK)
         return rc;
     *ok = 1;  // the image keeps its samples (pose.rs:14-16)

@@ -158,7 +158,7 @@ def test_forest_path_matches_the_twin(gpu, scene, spp):
     for f in range(spp):
         fo.sample(cam, w, h, want, f, 8)
     n = w * h
-    for waves in (2, 3):  # the forest kernel's two register fits
+    for waves in (2, 3, 4):  # the forest kernel's register fits (4: mesh trees at most 24 deep)
         t.set_path_waves(waves)
         t.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next call starts over
         t.sample(cam, w, h, spp)
