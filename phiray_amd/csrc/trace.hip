@@ -201,80 +201,19 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, uint32_
     return best;
 }
 
-// trace_blas (bvh.wgsl:120-190) for instance i: the ray taken into the
-// mesh's space by the instance's inverse rows (direction left unnormalised,
-// so t stays the world's) unless the instance sits at the identity (word 27
-// bit 0); a shadow ray passes an instance that casts none (bit 1). The
-// nearest hit so far rides in `best`, so this tree only takes strictly
-// nearer hits. Returns true when an any-hit ray stopped.
+// An instance's row times (x, y, z, w), summed in trace_blas's order.
 __device__ __forceinline__ float dot4(float4 r, float x, float y, float z, float w) {
     return ((r.x * x + r.y * y) + r.z * z) + r.w * w;
 }
 
-template <bool ANY>
-__device__ __forceinline__ bool walk_blas(const SceneDev& s, uint32_t i, V3 ow, V3 dw, PhrHit& best, uint32_t* stk,
-                                          uint32_t stride, uint32_t& steps, uint32_t& tests) {
-    const uint32_t* w = s.insts + INSTANCE_LEN * (size_t)i;
-    const uint4 info = *(const uint4*)(w + 24);
-    const uint32_t root = info.x, flags = info.w;
-    if (root == PHR_NONE) return false;
-    if (ANY && (flags & 2u)) return false;
-    V3 o = ow, d = dw;
-    if ((flags & 1u) == 0u) {
-        const float4 r0 = *(const float4*)(w + 12), r1 = *(const float4*)(w + 16), r2 = *(const float4*)(w + 20);
-        o = v3(dot4(r0, ow.x, ow.y, ow.z, 1.0f), dot4(r1, ow.x, ow.y, ow.z, 1.0f), dot4(r2, ow.x, ow.y, ow.z, 1.0f));
-        d = v3((r0.x * dw.x + r0.y * dw.y) + r0.z * dw.z, (r1.x * dw.x + r1.y * dw.y) + r1.z * dw.z,
-               (r2.x * dw.x + r2.y * dw.y) + r2.z * dw.z);
-    }
-    const V3 inv = safe_inv(d);
-    float4 lo = s.nodes[2 * root], hi = s.nodes[2 * root + 1];
-    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return false;
-    const RaySpace rs = ray_space(d);
-    uint32_t sp = 0;
-    for (;;) {
-        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
-        if (li & PHR_LEAF) {
-            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
-                ++tests;
-                float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
-                float u, v;
-                float t = tri_t(t0, t1, t2, o, rs, u, v);
-                if (t < best.t) {
-                    if (ANY) {
-                        const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
-                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
-                        best = PhrHit{t, id, u, v, i};
-                        return true;
-                    }
-                    best = PhrHit{t, id, u, v, i};
-                }
-            }
-        } else {
-            ++steps;
-            float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
-            float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
-            float tl = box_t(llo, lhi, o, inv, best.t);
-            float tr = box_t(rlo, rhi, o, inv, best.t);
-            if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) { stk[sp * stride] = ri; lo = llo; hi = lhi; }
-                else { stk[sp * stride] = li; lo = rlo; hi = rhi; }
-                ++sp;
-                continue;
-            }
-            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
-            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
-        }
-        if (sp == 0) break;
-        --sp;
-        uint32_t x = stk[sp * stride];
-        lo = s.nodes[2 * x];
-        hi = s.nodes[2 * x + 1];
-    }
-    return false;
-}
-
 // trace (bvh.wgsl:194-232): the top tree over the instances' world boxes; a
 // leaf's slots name the instances (inst_order) whose trees the ray walks.
+// trace_blas (bvh.wgsl:120-190) for instance i: the ray taken into the
+// mesh's space by the instance's inverse rows (direction left unnormalised,
+// so t stays the world's) unless the instance sits at the identity (word 27
+// bit 0); a shadow ray passes an instance that casts none (bit 1). The
+// nearest hit so far rides in `best`, so a tree only takes strictly nearer
+// hits.
 // The top tree's stack holds 16-bit node ids (its nodes are the array's
 // first 2 insts - 1, fewer than 65,536: the tracer refuses more instances),
 // a mesh tree's stack 32-bit ones; each is the thread's column of stride.
@@ -283,37 +222,110 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint16_
                               uint32_t& steps, uint32_t& tests) {
     PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
     const uint32_t* inst_order = s.insts + INSTANCE_LEN * (size_t)s.ninsts;
-    const V3 inv = safe_inv(d);
+    const V3 winv = safe_inv(d);
     float4 lo = s.nodes[0], hi = s.nodes[1];
-    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
-    uint32_t sp = 0;
+    if (box_t(lo, hi, o, winv, best.t) >= PHR_INF) return best;
+    // One loop over both levels (the nested walk's visits in its order): a
+    // step is a node of the top tree or of the current instance's mesh tree,
+    // the same code either way, so lanes at different levels still step
+    // together. At a top-tree leaf the ray goes into its instances' trees one
+    // after another (trace_blas); when the last is done the top tree pops.
+    uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
+    uint32_t tsp = 0, bsp = 0;
+    bool in_mesh = false;
+    uint32_t inst = 0, k = 0, kend = 0;
+    V3 ro = o, rinv = winv;  // the ray the current level walks with
+    RaySpace rs{0u, false, 0.f, 0.f, 0.f};
     for (;;) {
-        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
-        if (li & PHR_LEAF) {
-            for (uint32_t k = li & ~(PHR_LEAF | PHR_INST); k < ri; ++k) {
-                walk_blas<ANY>(s, inst_order[k], o, d, best, bstk, stride, steps, tests);
-                if (ANY && best.tri != PHR_NONE) return best;
-            }
-        } else {
+        bool pop = false;
+        if (!(li & PHR_LEAF)) {
             ++steps;
-            float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
-            float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
-            float tl = box_t(llo, lhi, o, inv, best.t);
-            float tr = box_t(rlo, rhi, o, inv, best.t);
+            const float4 llo = s.nodes[2 * li], lhi = s.nodes[2 * li + 1];
+            const float4 rlo = s.nodes[2 * ri], rhi = s.nodes[2 * ri + 1];
+            const float tl = box_t(llo, lhi, ro, rinv, best.t);
+            const float tr = box_t(rlo, rhi, ro, rinv, best.t);
             if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) { tstk[sp * stride] = (uint16_t)ri; lo = llo; hi = lhi; }
-                else { tstk[sp * stride] = (uint16_t)li; lo = rlo; hi = rhi; }
-                ++sp;
+                const uint32_t far = tl <= tr ? ri : li;
+                if (in_mesh) bstk[bsp++ * stride] = far;
+                else tstk[tsp++ * stride] = (uint16_t)far;
+                const float4 nl = tl <= tr ? llo : rlo, nh = tl <= tr ? lhi : rhi;
+                li = __float_as_uint(nl.w);
+                ri = __float_as_uint(nh.w);
                 continue;
             }
-            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
-            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
+            if (tl < PHR_INF) { li = __float_as_uint(llo.w); ri = __float_as_uint(lhi.w); continue; }
+            if (tr < PHR_INF) { li = __float_as_uint(rlo.w); ri = __float_as_uint(rhi.w); continue; }
+            pop = true;
+        } else if (in_mesh) {
+            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
+                ++tests;
+                const float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
+                float u, v;
+                const float t = tri_t(t0, t1, t2, ro, rs, u, v);
+                if (t < best.t) {
+                    if (ANY) {
+                        const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
+                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
+                        return PhrHit{t, id, u, v, inst};
+                    }
+                    best = PhrHit{t, id, u, v, inst};
+                }
+            }
+            pop = true;
+        } else {  // a top-tree leaf: its instances, in slot order
+            k = li & ~(PHR_LEAF | PHR_INST);
+            kend = ri;
         }
-        if (sp == 0) break;
-        --sp;
-        uint32_t x = tstk[sp * stride];
-        lo = s.nodes[2 * x];
-        hi = s.nodes[2 * x + 1];
+        if (pop && in_mesh && bsp > 0) {
+            const uint32_t x = bstk[--bsp * stride];
+            li = __float_as_uint(s.nodes[2 * x].w);
+            ri = __float_as_uint(s.nodes[2 * x + 1].w);
+            continue;
+        }
+        if (pop && in_mesh) {  // this instance's tree is done
+            in_mesh = false;
+            ++k;
+        }
+        // Into the next instance of the leaf whose root box the ray meets
+        // (trace_blas's prologue, bvh.wgsl:120-150).
+        while (!in_mesh && k < kend) {
+            const uint32_t i = inst_order[k];
+            const uint32_t* w = s.insts + INSTANCE_LEN * (size_t)i;
+            const uint4 info = *(const uint4*)(w + 24);
+            const uint32_t root = info.x, flags = info.w;
+            if (root != PHR_NONE && !(ANY && (flags & 2u))) {
+                V3 mo = o, md = d;
+                if ((flags & 1u) == 0u) {
+                    const float4 r0 = *(const float4*)(w + 12), r1 = *(const float4*)(w + 16), r2 = *(const float4*)(w + 20);
+                    mo = v3(dot4(r0, o.x, o.y, o.z, 1.0f), dot4(r1, o.x, o.y, o.z, 1.0f), dot4(r2, o.x, o.y, o.z, 1.0f));
+                    md = v3((r0.x * d.x + r0.y * d.y) + r0.z * d.z, (r1.x * d.x + r1.y * d.y) + r1.z * d.z,
+                            (r2.x * d.x + r2.y * d.y) + r2.z * d.z);
+                }
+                const V3 minv = safe_inv(md);
+                const float4 rlo = s.nodes[2 * root], rhi = s.nodes[2 * root + 1];
+                if (box_t(rlo, rhi, mo, minv, best.t) < PHR_INF) {
+                    in_mesh = true;
+                    inst = i;
+                    ro = mo;
+                    rinv = minv;
+                    rs = ray_space(md);
+                    bsp = 0;
+                    li = __float_as_uint(rlo.w);
+                    ri = __float_as_uint(rhi.w);
+                    break;
+                }
+            }
+            ++k;
+        }
+        if (in_mesh) continue;
+        // The leaf's instances are done, or an inner node met nothing: the
+        // top tree pops.
+        ro = o;
+        rinv = winv;
+        if (tsp == 0) break;
+        const uint32_t x = tstk[--tsp * stride];
+        li = __float_as_uint(s.nodes[2 * x].w);
+        ri = __float_as_uint(s.nodes[2 * x + 1].w);
     }
     return best;
 }
