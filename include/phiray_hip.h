@@ -10,7 +10,7 @@
  *    phr_last_error() describes the last failure on the calling thread;
  *  - plain pointers and sizes only; POD records cross by bytes in phiray's
  *    own layouts: Aabb 32 B, Node 32 B, Link 16 B, Hit 20 B, ray 32 B
- *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 192 B;
+ *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 208 B;
  *  - one HIP stream per phr_gpu; calls on one phr_gpu are serialised by the
  *    caller (phiray's `Tracer` is `&mut` from one render thread).
  */
@@ -205,7 +205,10 @@ int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array
 /* Tracer::set_samples (src/trace/mod.rs:242). */
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
 /* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
- * cam is the 192-byte camera record. */
+ * cam is the 208-byte camera record. Each launch also writes the display
+ * plane and the packed RGBA8 word (src/trace/wgsl/util.wgsl:101-145) under
+ * the camera's view words; only the Standard transform (view[0] = 0) is
+ * carried, others are refused with PHR_ERR_UNSUPPORTED. */
 int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done);
 /* Tracer::sample_in (src/trace/sample.rs:104-249): the same in `slices`
  * launches over the image's tiles, the device drained after each and go(user)
@@ -234,6 +237,11 @@ int phr_tracer_radiance(phr_tracer* t, float* out);
 int phr_tracer_albedo(phr_tracer* t, float* out);
 int phr_tracer_normal(phr_tracer* t, float* out);
 int phr_tracer_depth_plane(phr_tracer* t, float* out);
+/* Tracer::pixels (src/trace/read.rs:322-327): w*h RGBA8 pixels, 4*w*h bytes.
+ * Tracer::render (src/trace/sample.rs:88-91) is phr_tracer_sample then this. */
+int phr_tracer_pixels(phr_tracer* t, uint8_t* out);
+/* Tracer::display (src/trace/read.rs:257-263): the shown value, w*h rgb floats. */
+int phr_tracer_display(phr_tracer* t, float* out);
 
 /* ---- a scene of meshes under instances: the forest -------------------------- */
 /* A mesh (src/trace/mesh.rs:12-22): its face and vertex ranges and the box of

@@ -5,7 +5,8 @@
 // src/trace/wgsl/view.wgsl:88-93, and the four-pass driver of
 // src/trace/read.rs:85-119. pow(x, y) is exp2(y log2 x) and exp(x) is
 // exp2(x log2 e), both through the deterministic exp2/log2 of walk.cpp. The
-// last pass's display planes (view transform, packed pixel) are not restated.
+// last pass writes the display plane and the packed pixel (denoise.wgsl:
+// 124-133) through path.cpp's Standard view transform.
 #include "oracle.h"
 
 #include <cmath>
@@ -65,7 +66,7 @@ inline float luma_of(V3 c) { return dot(c, V3{0.2126f, 0.7152f, 0.0722f}); }
 }  // namespace
 
 extern "C" void orc_denoise_pass(uint32_t* px, uint32_t w, uint32_t h, uint32_t step, uint32_t round,
-                                 uint32_t last, uint32_t keep, const float strength[4]) {
+                                 uint32_t last, uint32_t keep, const float strength[4], const float view[4]) {
     const float KERNEL[3] = {0.375f, 0.25f, 0.0625f};
     Img im{px, (size_t)w * h, w, h, round};
     size_t n = im.n;
@@ -121,11 +122,20 @@ extern "C" void orc_denoise_pass(uint32_t* px, uint32_t w, uint32_t h, uint32_t 
             size_t o = 1 * n + 3 * i;
             px[o] = f_bits(v.x); px[o + 1] = f_bits(v.y); px[o + 2] = f_bits(v.z);
         }
+        float cov = im.f_at(23 * n + i) * k;
+        V3 grid = mul(im.v3_at(20 * n, i), k);
+        V3 m = divs(lin, fmax_(met, k));
+        float mv[3] = {m.x, m.y, m.z}, d[3];
+        orc_view_standard(mv, view[1], view[2], d);
+        V3 shown = add(mul(V3{d[0], d[1], d[2]}, 1.0f - cov), grid);
+        px[4 * n + 3 * i] = f_bits(shown.x); px[4 * n + 3 * i + 1] = f_bits(shown.y); px[4 * n + 3 * i + 2] = f_bits(shown.z);
+        float sv[3] = {shown.x, shown.y, shown.z};
+        px[i] = orc_pack_rgba8(sv, im.f_at(24 * n + i) * k);
     }
 }
 
-extern "C" void orc_denoise(uint32_t* px, uint32_t w, uint32_t h, uint32_t keep) {
+extern "C" void orc_denoise(uint32_t* px, uint32_t w, uint32_t h, uint32_t keep, const float view[4]) {
     const float strength[4] = {64.0f, 8.0f, 40.0f, 7.0f};
     const uint32_t steps[4] = {1, 2, 4, 8};
-    for (uint32_t r = 0; r < 4; ++r) orc_denoise_pass(px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u, keep, strength);
+    for (uint32_t r = 0; r < 4; ++r) orc_denoise_pass(px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u, keep, strength, view);
 }

@@ -105,7 +105,14 @@ typedef struct {
     float integrator[4];
     uint32_t w, h, rr_from, stream;  /* added to the sample index in the seed */
     float adapt[4];  /* Adaptive Sampling: threshold, min samples, step, on (camera.rs `adapt`) */
+    float view[4];   /* colour management: transform, exposure (stops), gamma (0: 1), look (camera.rs:53-56) */
 } orc_camera;
+
+/* The display pixel (src/trace/wgsl/view.wgsl:62-107) under the Standard view
+ * transform: exposure, sRGB encoding, gamma (0 means 1); and the RGBA8 word,
+ * red in the low byte, each channel rounded to nearest. */
+void orc_view_standard(const float lin[3], float exposure, float gamma, float out[3]);
+uint32_t orc_pack_rgba8(const float rgb[3], float a);
 
 /* Input triangle: src/trace/kernel.rs:11-40 (Tri) rows a..nc, color, emit —
  * the texture, UV and layered-material rows are not carried. 128 bytes. */
@@ -215,12 +222,12 @@ void orc_adapt_check(const orc_camera* cam, uint32_t* px);
 void orc_adapt_dilate(const orc_camera* cam, uint32_t* px);
 
 /* One a-trous pass (src/kernel/denoise.wgsl:47-134) over the image buffer;
- * the display planes (view transform, packed pixel) are not restated. */
+ * the last writes the display planes under view (Standard transform). */
 void orc_denoise_pass(uint32_t* px, uint32_t w, uint32_t h, uint32_t step, uint32_t round,
-                      uint32_t last, uint32_t keep, const float strength[4]);
+                      uint32_t last, uint32_t keep, const float strength[4], const float view[4]);
 /* Tracer::denoise_with (src/trace/read.rs:85-119): steps 1, 2, 4, 8 with
  * strengths [64, 8, 40, 7]. */
-void orc_denoise(uint32_t* px, uint32_t w, uint32_t h, uint32_t keep);
+void orc_denoise(uint32_t* px, uint32_t w, uint32_t h, uint32_t keep, const float view[4]);
 
 int orc_hardware_threads(void);
 

@@ -51,10 +51,10 @@ _SIGS = {
     "orc_sample": (None, [_P, _P, _U32, _U32, _P, _P, _I]),
     "orc_hardware_threads": (_I, []),
     "orc_log2": (C.c_float, [C.c_float]),
-    "orc_denoise": (None, [_P, _U32, _U32, _U32]),
+    "orc_denoise": (None, [_P, _U32, _U32, _U32, _P]),
     "orc_adapt_check": (None, [_P, _P]),
     "orc_adapt_dilate": (None, [_P, _P]),
-    "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P]),
+    "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P, _P]),
     "orc_inst_inverse": (None, [_P, _P]),
     "orc_inst_mirrors": (_U32, [_P]),
     "orc_inst_to_world": (None, [_P, _P, _P]),

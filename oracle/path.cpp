@@ -5,8 +5,9 @@
 // scenes use, operation for operation, with the same random-number draws:
 //   src/trace/wgsl/util.wgsl:1-10        hash, rnd (PCG-RXS-M-XS)
 //   src/trace/wgsl/util.wgsl:17-34       guide
-//   src/trace/wgsl/util.wgsl:101-145     emit (linear sums, counts; the display
-//                                        transform is not restated)
+//   src/trace/wgsl/util.wgsl:101-145     emit (linear sums, counts, the display
+//                                        plane and the packed pixel)
+//   src/trace/wgsl/view.wgsl:62-107      view_transform_of (Standard only), srgb, pack
 //   src/trace/wgsl/util.wgsl:162-179     pixel, frame_size, frame_point
 //   src/trace/wgsl/shade.wgsl:41-373     main (camera ray, the camera path)
 //   src/trace/wgsl/integrate.wgsl:6-23   cosine_dir, glow_mis
@@ -481,6 +482,16 @@ void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[
         V3 half = (keep ? v3(pxf(px, x), pxf(px, x + 1), pxf(px, x + 2)) : v3(0, 0, 0)) + lin * 2.0f;
         pxs(px, x, half.x); pxs(px, x + 1, half.y); pxs(px, x + 2, half.z);
     }
+    // util.wgsl:139-144: the light's mean over the samples that met something,
+    // transformed once, under the grid's coverage, the grid laid over.
+    float k = (float)taken;
+    V3 m = l / fmax_(k - gone, 1.0f);
+    float mv[3] = {m.x, m.y, m.z}, d[3];
+    orc_view_standard(mv, p->view[1], p->view[2], d);
+    V3 shown = v3(d[0], d[1], d[2]) * (1.0f - cov / k) + gs / k;
+    pxs(px, 4 * n + 3 * i, shown.x); pxs(px, 4 * n + 3 * i + 1, shown.y); pxs(px, 4 * n + 3 * i + 2, shown.z);
+    float sv[3] = {shown.x, shown.y, shown.z};
+    px[i] = orc_pack_rgba8(sv, film / k);
 }
 
 // shade.wgsl:41-373 for one pixel and the camera's frame.
@@ -678,3 +689,33 @@ void orc_adapt_dilate(const orc_camera* p, uint32_t* px) {
 }
 
 }  // extern "C"
+
+// view.wgsl:62-99 for transform 0 and view.wgsl:88-93 (srgb): exposure as a
+// power of two, the sRGB curve on the clamped channel, then the gamma power.
+// pow(x, y) is exp2(y log2 x), 0 for x <= 0.
+static float orc_srgb1(float c) {
+    float x = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    if (x <= 0.0031308f) return x * 12.92f;
+    return 1.055f * orc_exp2((1.0f / 2.4f) * orc_log2(x)) - 0.055f;
+}
+
+void orc_view_standard(const float lin[3], float exposure, float gamma, float out[3]) {
+    float s = orc_exp2(exposure);
+    float g = 1.0f / (gamma <= 0.0f ? 1.0f : gamma);
+    for (int j = 0; j < 3; ++j) {
+        float d = orc_srgb1((lin[j] > 0.0f ? lin[j] : 0.0f) * s);
+        d = d < 0.0f ? 0.0f : (d > 1.0f ? 1.0f : d);
+        out[j] = d > 0.0f ? orc_exp2(g * orc_log2(d)) : 0.0f;
+    }
+}
+
+// view.wgsl:101-107.
+uint32_t orc_pack_rgba8(const float rgb[3], float a) {
+    uint32_t q[4];
+    for (int j = 0; j < 4; ++j) {
+        float c = j < 3 ? rgb[j] : a;
+        c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+        q[j] = (uint32_t)(c * 255.0f + 0.5f);
+    }
+    return q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+}

@@ -36,10 +36,10 @@ CAMERA = np.dtype([
     ("sun", "<f4", 3), ("dist", "<f4"), ("pan", "<f4", 2), ("ao", "<f4"), ("eps", "<f4"),
     ("env", "<f4", 4), ("frame", "<u4"), ("count", "<u4"), ("glow", "<u4"), ("bounces", "<u4"),
     ("depth", "<u4", 4), ("integrator", "<f4", 4), ("w", "<u4"), ("h", "<u4"), ("rr_from", "<u4"), ("stream", "<u4"),
-    ("adapt", "<f4", 4),
+    ("adapt", "<f4", 4), ("view", "<f4", 4),
 ])
 assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
-assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 192
+assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 208
 
 # A scene of meshes under instances (src/trace/scene.rs:40-67, mesh.rs:12-22,
 # instance.rs:23-44): a vertex is its object-space position (w 1) then its
@@ -153,6 +153,8 @@ _SIGS = {
     "phr_tracer_radiance": (_I, [_P, _P]),
     "phr_tracer_denoise": (_I, [_P, _I]),
     "phr_tracer_albedo": (_I, [_P, _P]),
+    "phr_tracer_pixels": (_I, [_P, _P]),
+    "phr_tracer_display": (_I, [_P, _P]),
     "phr_tracer_normal": (_I, [_P, _P]),
     "phr_tracer_depth_plane": (_I, [_P, _P]),
     "phr_tracer_from_scene": (_I, [_P, _P, _U32, C.POINTER(_P)]),
@@ -864,6 +866,26 @@ class Tracer:
         _check(lib().phr_tracer_radiance(self._h, _ptr(out)), "Tracer::radiance")
         return out
 
+    def pixels(self) -> np.ndarray:
+        """`Tracer::pixels`: the RGBA8 image, (h, w, 4) bytes, row 0 at the top."""
+        w, h = self.size
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        _check(lib().phr_tracer_pixels(self._h, _ptr(out)), "Tracer::pixels")
+        return out
+
+    def display(self) -> np.ndarray:
+        """`Tracer::display`: the shown value before quantising, (w*h, 3) floats."""
+        w, h = self.size
+        out = np.zeros((w * h, 3), dtype=np.float32)
+        _check(lib().phr_tracer_display(self._h, _ptr(out)), "Tracer::display")
+        return out
+
+    def render(self, cam: np.ndarray, w: int, h: int) -> tuple[np.ndarray, bool]:
+        """`Tracer::render`: one sample, then the RGBA8 image and whether
+        refinement is done."""
+        _, done = self.sample(cam, w, h, 1)
+        return self.pixels(), done
+
     def albedo(self) -> np.ndarray:
         w, h = self.size
         out = np.zeros((w * h, 3), dtype=np.float32)
@@ -1206,7 +1228,9 @@ def planes(px: np.ndarray, w: int, h: int) -> dict[str, np.ndarray]:
     n = w * h
     f = px.view(np.float32)
     return {
+        "pixel": px[:n].view(np.uint8).reshape(n, 4),
         "radiance": f[n:4 * n].reshape(n, 3),
+        "display": f[4 * n:7 * n].reshape(n, 3),
         "albedo": f[7 * n:10 * n].reshape(n, 3),
         "normal": f[10 * n:13 * n].reshape(n, 3),
         "depth": f[13 * n:14 * n],

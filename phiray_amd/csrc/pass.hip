@@ -250,7 +250,7 @@ const KernelInfo KERNELS[] = {
 }  // namespace
 
 int phr_denoise_pass_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, uint32_t step, uint32_t round, uint32_t last,
-                             uint32_t keep, const float s[4]);
+                             uint32_t keep, const float s[4], const float view[4]);
 
 struct phr_pass {
     struct Step {
@@ -316,7 +316,7 @@ void launch(phr_pass* ps, const phr_pass::Step& s, hipStream_t st) {
         case 8: hipLaunchKernelGGL(kp_spread, s.grid, dim3(BLOCK), 0, st, param_of<RadixParam>(s), (const uint2*)P(0), (uint32_t*)P(1)); break;
         case 9: {
             DenoiseParam d = param_of<DenoiseParam>(s);
-            (void)phr_denoise_pass_enqueue(ps->gpu, (uint32_t*)P(0), d.w, d.h, d.step, d.round, d.last, d.keep, d.strength);
+            (void)phr_denoise_pass_enqueue(ps->gpu, (uint32_t*)P(0), d.w, d.h, d.step, d.round, d.last, d.keep, d.strength, d.view);
             break;
         }
         default: break;
@@ -368,6 +368,11 @@ int phr_pass_add(phr_pass* p, const char* name, uint32_t n, uint32_t items, cons
     PHR_CHECK(param_len == ki.param_size, PHR_ERR_ARG, std::string(name) + ": param size");
     PHR_CHECK(count == ki.binds, PHR_ERR_ARG, std::string(name) + ": binding count");
     for (uint32_t k = 0; k < count; ++k) PHR_CHECK(arrays && arrays[k] && arrays[k]->gpu == p->gpu, PHR_ERR_ARG, std::string(name) + ": binding");
+    if (kind == 9) {
+        DenoiseParam d;
+        std::memcpy(&d, params, sizeof d);
+        PHR_CHECK(d.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "denoise: only the Standard view transform (view[0] = 0)");
+    }
     phr_pass::Step s;
     s.kind = kind;
     s.params.assign((const uint8_t*)params, (const uint8_t*)params + param_len);

@@ -152,6 +152,36 @@ PHR_HD float dlog2(float x) {
 PHR_HD float dpow(float x, float y) { return x > 0.f ? dexp2(y * dlog2(x)) : 0.0f; }
 PHR_HD float dexp(float x) { return dexp2(x * 1.4426950408889634f); }
 
+// ---- the display pixel (src/trace/wgsl/view.wgsl:62-107) ----
+// Blender's Standard view: sRGB encoding of a channel clamped to [0, 1].
+PHR_HD float srgb_encode(float c) {
+    float x = clampf(c, 0.0f, 1.0f);
+    return x <= 0.0031308f ? x * 12.92f : 1.055f * dpow(x, 1.0f / 2.4f) - 0.055f;
+}
+// Scene-linear to display under view words (transform, exposure in stops,
+// gamma with 0 meaning 1, look): exposure, the Standard transform, gamma.
+// Only transform 0 (Standard) is carried; the host refuses the others.
+PHR_HD V3 view_standard(V3 lin, float exposure, float gamma) {
+    float s = dexp2(exposure);
+    V3 d = v3(srgb_encode(fmax_(lin.x, 0.0f) * s), srgb_encode(fmax_(lin.y, 0.0f) * s), srgb_encode(fmax_(lin.z, 0.0f) * s));
+    float g = 1.0f / (gamma <= 0.0f ? 1.0f : gamma);
+    return v3(dpow(clampf(d.x, 0.0f, 1.0f), g), dpow(clampf(d.y, 0.0f, 1.0f), g), dpow(clampf(d.z, 0.0f, 1.0f), g));
+}
+// RGBA8 in one word, red in the low byte; each channel rounded to nearest.
+PHR_HD uint32_t pack_rgba8(V3 c, float a) {
+    uint32_t r = (uint32_t)(clampf(c.x, 0.0f, 1.0f) * 255.0f + 0.5f);
+    uint32_t g = (uint32_t)(clampf(c.y, 0.0f, 1.0f) * 255.0f + 0.5f);
+    uint32_t b = (uint32_t)(clampf(c.z, 0.0f, 1.0f) * 255.0f + 0.5f);
+    uint32_t q = (uint32_t)(clampf(a, 0.0f, 1.0f) * 255.0f + 0.5f);
+    return r | (g << 8) | (b << 16) | (q << 24);
+}
+// The value shown for a pixel from its sums (util.wgsl:133-144): the view
+// transform of the light's mean over the samples that met something, under
+// the grid's coverage, with the grid laid over. k: samples taken.
+PHR_HD V3 shown_of(V3 l, V3 gs, float cov, float gone, float k, float exposure, float gamma) {
+    return view_standard(l / fmax_(k - gone, 1.0f), exposure, gamma) * (1.0f - cov / k) + gs / k;
+}
+
 // ---- the per-pixel random stream (PCG-RXS-M-XS hash) ----
 PHR_HD uint32_t pcg_hash(uint32_t x) {
     uint32_t h = x * 747796405u + 2891336453u;

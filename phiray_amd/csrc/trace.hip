@@ -41,8 +41,9 @@ struct PhrCamera {
     float integrator[4];
     uint32_t w, h, rr_from, stream;  // stream: added to the sample index in the seed (0: the reference's seeds)
     float adapt[4];  // Adaptive Sampling: threshold, min samples, step (a power of two), on (camera.rs `adapt`)
+    float view[4];   // colour management: transform (0 Standard), exposure in stops, gamma (0: 1), look (camera.rs:53-56)
 };
-static_assert(sizeof(PhrCamera) == 192, "camera layout");
+static_assert(sizeof(PhrCamera) == 208, "camera layout");
 
 // Input triangle: Tri's rows a..nc, color and emit (src/trace/kernel.rs:11-40).
 struct PhrTri {
@@ -850,6 +851,18 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
 __device__ __forceinline__ float ldf(const uint32_t* px, size_t w) { return __uint_as_float(px[w]); }
 __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = __float_as_uint(v); }
 
+// The display plane (words 4n..7n) and the packed RGBA8 word (word i) from
+// a pixel's new sums, as util.wgsl:139-144 writes them after each sample.
+__device__ __forceinline__ void put_shown(uint32_t* px, size_t n, size_t i, const PhrCamera& cam, V3 l, V3 gs, float cov,
+                                          float gone, float film, uint32_t taken) {
+    const float k = (float)taken;
+    const V3 s = shown_of(l, gs, cov, gone, k, cam.view[1], cam.view[2]);
+    stf(px, 4 * n + 3 * i, s.x);
+    stf(px, 4 * n + 3 * i + 1, s.y);
+    stf(px, 4 * n + 3 * i + 2, s.z);
+    px[i] = pack_rgba8(s, film / k);
+}
+
 // Samples [frame0, frame0 + spp) of every pixel of tile `blk`, summed into px
 // in sample order (util.wgsl:17-34, 101-145: sums restart at frame 0).
 template <bool F>
@@ -932,6 +945,7 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
         px[25 * n + i] = outline;
         px[31 * n + i] = taken;
         stf(px, 33 * n + i, gone);
+        put_shown(px, n, i, cam, l, gs, cov, gone, film, taken);
         if (adapt) { stf(px, 26 * n + 3 * i, ax.x); stf(px, 26 * n + 3 * i + 1, ax.y); stf(px, 26 * n + 3 * i + 2, ax.z); }
     }
     // Rays traced, summed over the wave before one atomic.
@@ -1258,6 +1272,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         px[25 * n + i] = outline;
         px[31 * n + i] = taken;
         stf(px, 33 * n + i, gone);
+        put_shown(px, n, i, *p, l, gs, cov, gone, film, taken);
     }
     unsigned long long rc = c.closest, rs = c.shadow, st = c.steps, tt = c.tests;
     for (int k = 32; k > 0; k >>= 1) {
@@ -1532,7 +1547,7 @@ phr_array* phr_bvh_internal_order(phr_bvh* b);
 extern "C" void phr_bvh_free(phr_bvh* b);
 extern "C" int phr_bvh_refit_async(phr_bvh* b);
 extern "C" uint32_t phr_bvh_depth(const phr_bvh* b);
-int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep, float* scratch);
+int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep, float* scratch, const float view[4]);
 constexpr uint64_t DENOISE_SCRATCH_WORDS = 11;  // guides (8) and the first pass's input (3)
 
 struct phr_tracer {
@@ -2232,6 +2247,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
     PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
               "phr_tracer_sample: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only phiray's own pixel filter (0)");
+    PHR_CHECK(cam.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only the Standard view transform (view[0] = 0)");
     if (int rc0 = forest_settle(t); rc0 != PHR_OK) return rc0;
     if (cam.adapt[3] != 0.0f) {
         const uint32_t st = (uint32_t)cam.adapt[2];
@@ -2448,13 +2464,25 @@ __global__ __launch_bounds__(256) void k_frame(const uint32_t* __restrict__ px, 
     o[9] = __uint_as_float(px[13 * n + i]) / k;
 }
 
-// The denoised radiance back into the linear sums: sum = value * count.
-__global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, uint64_t n, const float* __restrict__ rad) {
+// The denoised radiance back into the linear sums (sum = value * count), and
+// the display plane and packed word from the new sums as the tracer writes
+// them (util.wgsl:139-144) under exposure and gamma.
+__global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, uint64_t n, const float* __restrict__ rad,
+                                                    float exposure, float gamma) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = px[31 * n + i];
     const float k = (float)(c > 1u ? c : 1u);
-    for (int j = 0; j < 3; ++j) px[1 * n + 3 * i + j] = __float_as_uint(rad[3 * i + j] * k);
+    const V3 l = v3(rad[3 * i] * k, rad[3 * i + 1] * k, rad[3 * i + 2] * k);
+    stf(px, n + 3 * i, l.x);
+    stf(px, n + 3 * i + 1, l.y);
+    stf(px, n + 3 * i + 2, l.z);
+    const V3 gs = v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2));
+    const V3 sh = shown_of(l, gs, ldf(px, 23 * n + i), ldf(px, 33 * n + i), k, exposure, gamma);
+    stf(px, 4 * n + 3 * i, sh.x);
+    stf(px, 4 * n + 3 * i + 1, sh.y);
+    stf(px, 4 * n + 3 * i + 2, sh.z);
+    px[i] = pack_rgba8(sh, ldf(px, 24 * n + i) / k);
 }
 
 // The a-trous passes' guide table for the current image size.
@@ -2545,13 +2573,14 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     launch_path(path_build_waves(t), t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
                 t->rays, 0u, 0u);
     int rc = PHR_OK;
-    if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch);
+    if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
     if (denoise == 2) {
         const uint32_t g = (uint32_t)((n + 255) / 256);
         float* rad = t->frame_dev + 10 * n;
         hipLaunchKernelGGL(k_frame, dim3(g), dim3(256), 0, s, (const uint32_t*)pxp, n, t->frame_dev);
         rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
-        if (rc == PHR_OK) hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, n, (const float*)rad);
+        if (rc == PHR_OK)
+            hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, n, (const float*)rad, t->last.view[1], t->last.view[2]);
     }
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s, &graph);
@@ -2621,7 +2650,8 @@ int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
     rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
     if (rc != PHR_OK) return rc;
     if (!out) {
-        hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, t->gpu->stream, (uint32_t*)t->px->ptr, n, rad);
+        hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, t->gpu->stream, (uint32_t*)t->px->ptr, n, rad,
+                           t->last.view[1], t->last.view[2]);
         PHR_HIP(hipGetLastError());
     }
     PHR_HIP(hipStreamSynchronize(t->gpu->stream));
@@ -2635,7 +2665,7 @@ int phr_tracer_denoise(phr_tracer* t, int keep) {
     if (!t->px) return PHR_OK;  // nothing traced: nothing to filter
     int rc = ensure_denoise_scratch(t);
     if (rc != PHR_OK) return rc;
-    rc = phr_denoise_enqueue(t->gpu, (uint32_t*)t->px->ptr, t->w, t->h, keep, t->dn_scratch);
+    rc = phr_denoise_enqueue(t->gpu, (uint32_t*)t->px->ptr, t->w, t->h, keep, t->dn_scratch, t->last.view);
     if (rc != PHR_OK) return rc;
     PHR_HIP(hipStreamSynchronize(t->gpu->stream));
     return PHR_OK;
@@ -2679,6 +2709,21 @@ static int guide_plane(phr_tracer* t, uint64_t at, int comps, float* out, const 
     }
     return PHR_OK;
 }
+// Tracer::pixels (src/trace/read.rs:322-327): the packed RGBA8 words, 4 bytes
+// a pixel, red first, row 0 at the top.
+int phr_tracer_pixels(phr_tracer* t, uint8_t* out) {
+    PHR_CHECK(t && t->px && out, PHR_ERR_ARG, "phr_tracer_pixels: nothing traced yet");
+    return phr_array_read_range(t->px, 0, (uint64_t)t->w * t->h, out);
+}
+
+// Tracer::display (src/trace/read.rs:257-263): the shown value, 3 floats a
+// pixel, words 4n..7n as the tracer or the denoiser's last pass left them.
+int phr_tracer_display(phr_tracer* t, float* out) {
+    PHR_CHECK(t && t->px && out, PHR_ERR_ARG, "phr_tracer_display: nothing traced yet");
+    const uint64_t n = (uint64_t)t->w * t->h;
+    return phr_array_read_range(t->px, 4 * n, 3 * n, out);
+}
+
 int phr_tracer_albedo(phr_tracer* t, float* out) { return guide_plane(t, 7, 3, out, "phr_tracer_albedo: nothing traced yet"); }
 int phr_tracer_normal(phr_tracer* t, float* out) { return guide_plane(t, 10, 3, out, "phr_tracer_normal: nothing traced yet"); }
 int phr_tracer_depth_plane(phr_tracer* t, float* out) { return guide_plane(t, 13, 1, out, "phr_tracer_depth_plane: nothing traced yet"); }
@@ -2980,6 +3025,7 @@ int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
               "phr_batch_render: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_batch_render: only phiray's own pixel filter (0)");
     PHR_CHECK(cam.adapt[3] == 0.0f, PHR_ERR_UNSUPPORTED, "phr_batch_render: adaptive sampling is per tracer");
+    PHR_CHECK(cam.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_batch_render: only the Standard view transform (view[0] = 0)");
     return PHR_OK;
 }
 

@@ -6,7 +6,10 @@ images are outside the restated subset):
 * denoising a 12-sample render moves it towards the 1,024-sample one (mean
   absolute difference below 0.75x the noisy one's);
 * the albedo edge between a dark and a pale quad survives and stays sharp;
-* `denoise_shown` (keep) leaves the linear sums alone, `denoise` replaces them.
+* `denoise_shown` (keep) leaves the linear sums alone, `denoise` replaces them;
+* the last pass's display plane is the Standard view transform (sRGB curve
+  in float64, exposure, gamma) of the filtered light over the share of
+  samples that met something, and the packed word its 8-bit rounding.
 The restatement's own arithmetic has no independent pin beyond these
 behaviours (parity unpinned below the qualitative bar); the GPU kernel is held
 to it within 1e-5 relative.
@@ -65,10 +68,19 @@ def radiance(px):
     return pl["radiance"] / np.maximum(pl["count"], 1)[:, None]
 
 
-def denoise(orc, px, keep=False):
+def denoise(orc, px, keep=False, view=(0.0, 0.0, 0.0, 0.0)):
     out = px.copy()
-    orc.lib().orc_denoise(orc.ptr(out), SIDE, SIDE, 1 if keep else 0)
+    v = np.array(view, dtype=np.float32)
+    orc.lib().orc_denoise(orc.ptr(out), SIDE, SIDE, 1 if keep else 0, orc.ptr(v))
     return out
+
+
+def srgb_view(lin, exposure=0.0, gamma=0.0):
+    """view.wgsl:62-99 for the Standard transform, in float64."""
+    c = np.maximum(lin.astype(np.float64), 0.0) * 2.0 ** exposure
+    x = np.clip(c, 0.0, 1.0)
+    d = np.where(x <= 0.0031308, x * 12.92, 1.055 * x ** (1 / 2.4) - 0.055)
+    return np.clip(d, 0.0, 1.0) ** (1.0 / (gamma if gamma > 0 else 1.0))
 
 
 @pytest.fixture(scope="module")
@@ -103,6 +115,25 @@ def test_denoising_what_is_shown_keeps_the_samples(orc, renders):
     assert not (radiance(denoise(orc, noisy)) == radiance(noisy)).all()
 
 
+@pytest.mark.parametrize("exposure,gamma", [(0.0, 0.0), (1.5, 2.2), (-2.0, 0.8)])
+def test_last_pass_writes_the_view_transform(orc, renders, exposure, gamma):
+    _, noisy = renders
+    out = denoise(orc, noisy, view=(0.0, exposure, gamma, 0.0))
+    pl = P.planes(out, SIDE, SIDE)
+    k = np.maximum(pl["count"], 1).astype(np.float64)
+    met = 1.0 - pl["missed"] / k
+    lin = pl["radiance"] / k[:, None]  # keep=False: the filtered light's sums
+    # Under the sky's share of the samples (`cov`) and its colour (`grid`).
+    want = srgb_view(lin / np.maximum(met, 1.0 / k)[:, None], exposure, gamma) * (1.0 - pl["cov"] / k)[:, None]
+    want = want + pl["grid"] / k[:, None]
+    assert pl["cov"].max() > 0
+    assert np.allclose(pl["display"], want, rtol=2e-6, atol=2e-7)
+    q = np.floor(np.clip(want, 0, 1) * 255.0 + 0.5)
+    assert (np.abs(pl["pixel"][:, :3].astype(np.int32) - q) <= 1).all()
+    assert (pl["pixel"][:, 3] == 255).all()  # every sample covers the film
+    assert pl["display"].max() > 0.5
+
+
 def test_log2_against_libm(orc):
     L = orc.lib()
     for x in np.concatenate([np.linspace(1e-3, 4.0, 4001), np.geomspace(1e-30, 1e30, 2001)]).astype(np.float32):
@@ -124,7 +155,11 @@ def test_gpu_denoise_matches_the_restatement(gpu, orc, keep):
     got = tr.px()
     want = denoise(orc, before, keep)
     n = SIDE * SIDE
-    for lo, hi in [(1 * n, 4 * n), (14 * n, 20 * n)]:
+    for lo, hi in [(1 * n, 4 * n), (4 * n, 7 * n), (14 * n, 20 * n)]:
         a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)
         bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
         assert not bad.any(), f"words {lo}..{hi}: {bad.sum()} beyond 1e-5"
+    a, b = got[:n].view(np.uint8).astype(np.int32), want[:n].view(np.uint8).astype(np.int32)
+    assert (np.abs(a - b) <= 1).all()
+    assert (tr.pixels().reshape(-1) == got[:n].view(np.uint8)).all()
+    assert (tr.display().reshape(-1) == got[4 * n:7 * n].view(np.float32)).all()

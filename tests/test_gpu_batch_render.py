@@ -38,9 +38,11 @@ def camera():
 
 def compare_planes(got, want, w, h):
     pg, pw = P.planes(got, w, h), P.planes(want, w, h)
-    for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline"]:
+    for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline", "display", "pixel"]:
         a, b = pg[k], pw[k]
-        if a.dtype == np.float32:
+        if k == "pixel":  # 8-bit rounding of display values within 1e-5 of each other
+            assert (np.abs(a.astype(np.int32) - b.astype(np.int32)) <= 1).all(), k
+        elif a.dtype == np.float32:
             bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
             assert not bad.any(), f"{k}: {bad.sum()} words beyond 1e-5 relative"
         else:

@@ -158,6 +158,17 @@ def test_tracer_frame_and_learned_denoise(gpu):
     assert np.array_equal(out.read(), via_net.read())
     tr.denoise_learned(net)  # in place of the linear sums
     np.testing.assert_allclose(tr.radiance().ravel(), out.read(), rtol=1e-6, atol=1e-7)
+    # The display is refreshed from the new sums as the tracer composes it:
+    # the Standard curve (float64 here) of the light over the samples that
+    # met something, under the sky's share.
+    pl = P.planes(tr.px(), w, h)
+    k = pl["count"].astype(np.float64)
+    c = np.clip(pl["radiance"] / np.maximum(k - pl["missed"], 1.0)[:, None], 0.0, 1.0)
+    d = np.where(c <= 0.0031308, c * 12.92, 1.055 * c ** (1 / 2.4) - 0.055)
+    want = d * (1.0 - pl["cov"] / k)[:, None] + pl["grid"] / k[:, None]
+    np.testing.assert_allclose(pl["display"], want, rtol=1e-5, atol=1e-6)
+    q = np.floor(np.clip(want, 0, 1) * 255.0 + 0.5)
+    assert (np.abs(pl["pixel"][:, :3].astype(np.int32) - q) <= 1).all()
     del frame_only
 
 

@@ -55,9 +55,11 @@ def test_any_hit_rays(gpu, orc):
 
 def compare_planes(got, want, w, h):
     pg, pw = P.planes(got, w, h), P.planes(want, w, h)
-    for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline"]:
+    for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline", "display", "pixel"]:
         a, b = pg[k], pw[k]
-        if a.dtype == np.float32:
+        if k == "pixel":  # 8-bit rounding of display values within 1e-5 of each other
+            assert (np.abs(a.astype(np.int32) - b.astype(np.int32)) <= 1).all(), k
+        elif a.dtype == np.float32:
             bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
             assert not bad.any(), f"{k}: {bad.sum()} words beyond 1e-5 relative, e.g. {a[bad][:4]} vs {b[bad][:4]}"
         else:
@@ -290,13 +292,45 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     lit = 0
     for row, rows in bands:
         sl = slice(row * w, (row + rows) * w)
-        for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline"]:
+        for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline", "display", "pixel"]:
             for build, planes in (("plain", got), ("regen", regen)):
                 a, b = planes[k][sl], want[k][sl]
-                if a.dtype == np.float32:
+                if k == "pixel":
+                    assert (np.abs(a.astype(np.int32) - b.astype(np.int32)) <= 1).all(), (build, row, k)
+                elif a.dtype == np.float32:
                     bad = ~np.isclose(a, b, rtol=1e-5, atol=0)
                     assert not bad.any(), f"{build} rows {row}+{rows} {k}: {bad.sum()} words beyond 1e-5 relative"
                 else:
                     assert (a == b).all(), (build, row, k)
         lit += int((want["radiance"][sl] > 0).any(axis=1).sum())
     assert lit > 1000  # the bands cross lit surfaces
+
+
+def test_display_under_exposure_and_gamma_matches_the_twin(gpu, orc):
+    """The display plane and the packed pixel under a non-default view;
+    `render` and `pixels` read back the packed words, `display` the plane;
+    a view transform other than Standard is refused."""
+    print("[trace] display exposure/gamma", flush=True)
+    tris = S.spheres_scene(spheres=5, level=1, seed=8)
+    w, h = 40, 24
+    cam = S.path_camera(w, h, rr_from=3)
+    cam["view"] = (0.0, 1.5, 2.2, 0.0)
+    tr = P.Tracer.new(gpu, tris, 4)
+    tr.sample(cam, w, h, 2)
+    sc = orc.Scene(tris, 4, P.NODE)
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    for f in range(2):
+        sc.sample(cam, w, h, px, f, 8)
+    got = tr.px()
+    compare_planes(got, px, w, h)
+    n = w * h
+    assert (tr.pixels().reshape(-1) == got[:n].view(np.uint8)).all()
+    assert (tr.display().reshape(-1) == got[4 * n:7 * n].view(np.float32)).all()
+    img, _ = tr.render(cam, w, h)  # a third sample onto the same image
+    assert img.shape == (h, w, 4) and (img.reshape(-1) == tr.px()[:n].view(np.uint8)).all()
+    assert P.planes(tr.px(), w, h)["count"].min() == 3
+    bad = cam.copy()
+    bad["view"] = (1.0, 0.0, 0.0, 0.0)  # Filmic
+    with pytest.raises(P.PhrError):
+        tr.sample(bad, w, h, 1)
+    tr.free()

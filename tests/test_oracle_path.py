@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 import phiray_amd as P
+from phiray_amd import scenes as S
 
 
 def ulps(a, b):
@@ -128,3 +129,35 @@ def test_emitters_are_counted_once_near_and_far(orc, case):
     rad = pl["radiance"] / pl["count"][:, None]
     got = center_mean(rad)
     assert abs(got - want) < want * tol, f"{name}: converged {got} vs expected {want}"
+
+
+@pytest.mark.parametrize("exposure,gamma", [(0.0, 0.0), (1.0, 2.2)])
+def test_display_is_the_view_transform_of_the_mean(orc, exposure, gamma):
+    """util.wgsl:133-144 and view.wgsl:62-107 (Standard): the shown value is
+    the sRGB curve (here in float64) of the light's mean over the samples
+    that met something, after exposure and before gamma, under the sky's
+    share of the samples; the packed word is its 8-bit rounding with the
+    film's coverage as alpha."""
+    tris = S.spheres_scene(spheres=3, level=1, seed=5)
+    w, h = 24, 16
+    cam = S.path_camera(w, h, rr_from=3)
+    cam["view"] = (0.0, exposure, gamma, 0.0)
+    sc = orc.Scene(tris, 4, P.NODE)
+    px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
+    for f in range(3):
+        sc.sample(cam, w, h, px, f, 8)
+    pl = P.planes(px, w, h)
+    k = pl["count"].astype(np.float64)
+    assert (k == 3).all() and pl["missed"].max() > 0 and pl["missed"].min() == 0
+    m = pl["radiance"] / np.maximum(k - pl["missed"], 1.0)[:, None]
+    c = np.clip(np.maximum(m, 0.0) * 2.0 ** exposure, 0.0, 1.0)
+    d = np.where(c <= 0.0031308, c * 12.92, 1.055 * c ** (1 / 2.4) - 0.055)
+    want = np.clip(d, 0.0, 1.0) ** (1.0 / (gamma if gamma > 0 else 1.0))
+    # A miss lays the sky over the light: coverage in `cov`, colour in `grid`.
+    want = want * (1.0 - pl["cov"] / k)[:, None] + pl["grid"] / k[:, None]
+    assert (pl["cov"] == pl["missed"]).all()
+    assert np.allclose(pl["display"], want, rtol=2e-6, atol=2e-7)
+    q = np.floor(np.clip(want, 0, 1) * 255.0 + 0.5)
+    assert (np.abs(pl["pixel"][:, :3].astype(np.int32) - q) <= 1).all()
+    assert (pl["pixel"][:, 3] == 255).all()
+    assert pl["display"].max() > 0.3
