@@ -242,6 +242,13 @@ int phr_tracer_depth_plane(phr_tracer* t, float* out);
 int phr_tracer_pixels(phr_tracer* t, uint8_t* out);
 /* Tracer::display (src/trace/read.rs:257-263): the shown value, w*h rgb floats. */
 int phr_tracer_display(phr_tracer* t, float* out);
+/* Tracer::show (src/trace/read.rs:182-229, src/trace/wgsl/show.wgsl): put a
+ * scene-linear image (count rgb triples, truncated or zero-filled to w*h) in
+ * place of the traced mean and run the display step; *shown = 0 when there
+ * is no image or camera yet (the reference's false). */
+int phr_tracer_show(phr_tracer* t, const float* linear, uint64_t count, int* shown);
+/* Tracer::size (src/trace/read.rs:162-164): the image's w, h (0, 0 before a sample). */
+int phr_tracer_size(phr_tracer* t, uint32_t* w, uint32_t* h);
 
 /* ---- a scene of meshes under instances: the forest -------------------------- */
 /* A mesh (src/trace/mesh.rs:12-22): its face and vertex ranges and the box of

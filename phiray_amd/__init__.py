@@ -155,6 +155,8 @@ _SIGS = {
     "phr_tracer_albedo": (_I, [_P, _P]),
     "phr_tracer_pixels": (_I, [_P, _P]),
     "phr_tracer_display": (_I, [_P, _P]),
+    "phr_tracer_show": (_I, [_P, _P, _U64, _P]),
+    "phr_tracer_size": (_I, [_P, _P, _P]),
     "phr_tracer_normal": (_I, [_P, _P]),
     "phr_tracer_depth_plane": (_I, [_P, _P]),
     "phr_tracer_from_scene": (_I, [_P, _P, _U32, C.POINTER(_P)]),
@@ -879,6 +881,20 @@ class Tracer:
         out = np.zeros((w * h, 3), dtype=np.float32)
         _check(lib().phr_tracer_display(self._h, _ptr(out)), "Tracer::display")
         return out
+
+    def show(self, linear: np.ndarray) -> bool:
+        """`Tracer::show`: a scene-linear image (rgb a pixel, row 0 first) in
+        place of the traced mean, then the display step; False with no image."""
+        lin = np.ascontiguousarray(np.asarray(linear, dtype=np.float32).reshape(-1, 3))
+        shown = C.c_int(0)
+        _check(lib().phr_tracer_show(self._h, _ptr(lin), len(lin), C.byref(shown)), "Tracer::show")
+        return bool(shown.value)
+
+    def image_size(self) -> tuple[int, int]:
+        """`Tracer::size`: the image's (w, h), (0, 0) before the first sample."""
+        w, h = C.c_uint32(0), C.c_uint32(0)
+        _check(lib().phr_tracer_size(self._h, C.byref(w), C.byref(h)), "Tracer::size")
+        return int(w.value), int(h.value)
 
     def render(self, cam: np.ndarray, w: int, h: int) -> tuple[np.ndarray, bool]:
         """`Tracer::render`: one sample, then the RGBA8 image and whether

@@ -2464,9 +2464,10 @@ __global__ __launch_bounds__(256) void k_frame(const uint32_t* __restrict__ px, 
     o[9] = __uint_as_float(px[13 * n + i]) / k;
 }
 
-// The denoised radiance back into the linear sums (sum = value * count), and
-// the display plane and packed word from the new sums as the tracer writes
-// them (util.wgsl:139-144) under exposure and gamma.
+// A mean image back into the linear sums (sum = value * count), and the
+// display plane and packed word from the new sums as the tracer writes them
+// (util.wgsl:139-144, show.wgsl:26-45) under exposure and gamma: the learned
+// denoiser's output, or a host's image for Tracer::show.
 __global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, uint64_t n, const float* __restrict__ rad,
                                                     float exposure, float gamma) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -2636,6 +2637,41 @@ int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k) {
 // stands beside Tracer::denoise). With `out`, the denoised radiance goes there
 // (w * h * 3 floats) and the image is left as it is; without, it replaces the
 // linear sums, as Tracer::denoise with keep = false does.
+// Tracer::show (src/trace/read.rs:182-229, src/trace/wgsl/show.wgsl): a
+// host's scene-linear image put back in place of the traced mean (sum = value
+// times the pixel's own count) and run through the display step. `linear`
+// holds `count` rgb triples, truncated or zero-filled to the image's size;
+// *shown = 0, as the reference's false, when there is no image or camera yet.
+int phr_tracer_show(phr_tracer* t, const float* linear, uint64_t count, int* shown) {
+    PHR_CHECK(t && shown && (linear || count == 0), PHR_ERR_ARG, "phr_tracer_show: null argument");
+    *shown = 0;
+    const uint64_t n = (uint64_t)t->w * t->h;
+    if (!t->px || !t->have_last || n == 0) return PHR_OK;
+    int rc = ensure_frame(t);
+    if (rc != PHR_OK) return rc;
+    float* rad = t->frame_dev + 10 * n;
+    const uint64_t m = std::min<uint64_t>(count, n);
+    hipStream_t s = t->gpu->stream;
+    if (m) PHR_HIP(hipMemcpyAsync(rad, linear, m * 12, hipMemcpyHostToDevice, s));
+    if (m < n) PHR_HIP(hipMemsetAsync(rad + 3 * m, 0, (n - m) * 12, s));
+    const uint32_t g = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, (uint32_t*)t->px->ptr, n, (const float*)rad, t->last.view[1],
+                       t->last.view[2]);
+    PHR_HIP(hipGetLastError());
+    PHR_HIP(hipStreamSynchronize(s));
+    *shown = 1;
+    return PHR_OK;
+}
+
+// Tracer::size (src/trace/read.rs:162-164): the image's size, 0 x 0 before
+// the first sample.
+int phr_tracer_size(phr_tracer* t, uint32_t* w, uint32_t* h) {
+    PHR_CHECK(t && w && h, PHR_ERR_ARG, "phr_tracer_size: null argument");
+    *w = t->px ? t->w : 0u;
+    *h = t->px ? t->h : 0u;
+    return PHR_OK;
+}
+
 int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
     PHR_CHECK(t && k, PHR_ERR_ARG, "phr_tracer_denoise_learned: null");
     if (!t->px) return PHR_OK;

@@ -334,3 +334,44 @@ def test_display_under_exposure_and_gamma_matches_the_twin(gpu, orc):
     with pytest.raises(P.PhrError):
         tr.sample(bad, w, h, 1)
     tr.free()
+
+
+def test_showing_the_image_its_own_radiance_changes_nothing(gpu):
+    """tests/show.rs:47-82 under the Standard view: `show` of the image's own
+    radiance gives the radiance, the display and the bytes back; before any
+    sample it shows nothing."""
+    print("[trace] show round trip", flush=True)
+    tris = S.spheres_scene(spheres=4, level=1, seed=11)
+    w, h = 32, 24
+    tr = P.Tracer.new(gpu, tris, 4)
+    assert tr.image_size() == (0, 0)
+    assert not tr.show(np.zeros((w * h, 3), np.float32))
+    cam = S.path_camera(w, h, rr_from=3)
+    cam["view"] = (0.0, 0.5, 1.8, 0.0)
+    tr.sample(cam, w, h, 4)
+    assert tr.image_size() == (w, h)
+    was_px, was_display, was_linear = tr.pixels(), tr.display(), tr.radiance()
+    assert (was_linear[:, 0] > 0).any()
+    assert tr.show(was_linear)
+    np.testing.assert_allclose(tr.radiance(), was_linear, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(tr.display(), was_display, rtol=0, atol=1e-4)
+    assert (tr.pixels() == was_px).all()
+
+
+def test_a_change_to_the_linear_pixels_shows_through_the_transform(gpu):
+    """tests/show.rs:87-118 under the Standard view: one stop down in the
+    linear pixels dims every lit pixel's display by less than half."""
+    print("[trace] show darker", flush=True)
+    tris = S.spheres_scene(spheres=4, level=1, seed=11)
+    w, h = 32, 24
+    tr = P.Tracer.new(gpu, tris, 4)
+    tr.sample(S.path_camera(w, h, rr_from=3), w, h, 4)
+    before, linear = tr.display(), tr.radiance()
+    assert tr.show(linear * 0.5)
+    after = tr.display()
+    lit = np.where(linear[:, 0] > 0.05)[0]
+    assert len(lit) > 0
+    assert (after[lit, 0] < before[lit, 0]).all()
+    assert after[lit[0], 0] / max(before[lit[0], 0], 1e-6) > 0.5
+    assert tr.show(linear[: w])  # zero-filled past the first row
+    assert (tr.radiance()[w:] == 0).all() and (tr.display()[w:, 0] <= before[w:, 0] + 1e-6).all()
