@@ -47,6 +47,9 @@ uint32_t phr_abi_version(void);
 /* Gpu::new (src/gpu/mod.rs:108); PHR_ERR_UNSUPPORTED = None (no device). */
 int phr_gpu_open(int device, phr_gpu** out);
 void phr_gpu_close(phr_gpu* g);
+/* Gpu::shared (src/gpu/mod.rs:113-118): the process's device 0, opened on
+ * first use; phr_gpu_close leaves it open. */
+int phr_gpu_shared(phr_gpu** out);
 /* Gpu::wait (src/gpu/mod.rs:272). */
 int phr_gpu_wait(phr_gpu* g);
 /* Gpu::wait_for (src/gpu/mod.rs:292): *done = 1 when drained within limit. */
@@ -66,8 +69,18 @@ int phr_array_zeroed(phr_gpu* g, uint32_t elem, uint64_t len, phr_array** out);
 int phr_array_from(phr_gpu* g, uint32_t elem, const void* items, uint64_t len, phr_array** out);
 /* GpuArray::write (src/gpu/array.rs:47-56): *grown = reallocated. */
 int phr_array_write(phr_array* a, const void* items, uint64_t len, int* grown);
-/* GpuArray::write_at (src/gpu/array.rs:88-107). */
-int phr_array_write_at(phr_array* a, uint64_t at, const void* items, uint64_t len, int* grown);
+/* GpuArray::write_at (src/gpu/array.rs:96-107): *written = 0 (the reference's
+ * false), and nothing written, when at + len passes the array's length. */
+int phr_array_write_at(phr_array* a, uint64_t at, const void* items, uint64_t len, int* written);
+/* GpuArray::write_front (src/gpu/array.rs:85-94): write_at from item 0. */
+int phr_array_write_front(phr_array* a, const void* items, uint64_t len, int* written);
+/* GpuArray::split (src/gpu/array.rs:58-83): items over `chunks` new arrays of
+ * at most limit / elem items, *cut items each; PHR_ERR_SIZE (None) when they
+ * do not fit. out has room for `chunks` handles. */
+int phr_array_split(phr_gpu* g, uint32_t elem, const void* items, uint64_t len, uint64_t limit, uint32_t chunks,
+                    phr_array** out, uint32_t* cut);
+/* GpuArray::read_first (src/gpu/array.rs:138-142): min(count, len) items, *got of them. */
+int phr_array_read_first(phr_array* a, uint64_t count, void* out, uint64_t* got);
 /* GpuArray::read_range (src/gpu/array.rs:136-187), blocking. */
 int phr_array_read_range(phr_array* a, uint64_t from, uint64_t count, void* out);
 uint64_t phr_array_len(const phr_array* a);
@@ -110,6 +123,8 @@ int phr_pass_run_step(phr_pass* p, uint32_t step);
 /* Pass::clear / len (src/gpu/pass.rs:140-148). */
 void phr_pass_clear(phr_pass* p);
 uint32_t phr_pass_len(const phr_pass* p);
+/* Pass::compiled (src/gpu/pass.rs:151-154): kernel types the pass uses. */
+uint32_t phr_pass_compiled(const phr_pass* p);
 
 /* ---- BVH: GpuBvh (src/bvh/gpu.rs:11-104) ------------------------------------ */
 /* GpuBvh::build (src/bvh/gpu.rs:23-75): prims are n Aabb (32 B) on the host.
@@ -284,6 +299,17 @@ typedef struct {
  * PHR_ERR_ARG for an empty scene or one whose instances place nothing (None),
  * PHR_ERR_UNSUPPORTED for a material the path does not carry (transmission). */
 int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* scene, uint32_t leaf, phr_tracer** out);
+/* Tracer::with_chunks (src/trace/build.rs:136-200): from_scene under phiray's
+ * per-binding byte limit and binding count (2, or 4 when chunks > 2);
+ * PHR_ERR_SIZE (the reference's None) when the scene does not fit them. The
+ * kernels here index whole arrays: the split is bookkeeping for callers. */
+int phr_tracer_with_chunks(phr_gpu* g, const phr_scene_desc* scene, uint32_t leaf, uint64_t limit, uint32_t chunks,
+                           phr_tracer** out);
+/* Tracer::split / Tracer::chunks (src/trace/mod.rs:118-127): items a binding
+ * for nodes, faces, vertices, and the atlas offset (0: no atlas); bindings an
+ * array. */
+int phr_tracer_split(phr_tracer* t, uint32_t out[4]);
+uint32_t phr_tracer_chunks(phr_tracer* t);
 /* Tracer::reinstance (src/trace/pose.rs:102-125): new matrices (16 floats each,
  * m[col][row]); the top tree, instance table and light table follow, the
  * image starts over. *ok = 0 when the count differs or the tracer was not

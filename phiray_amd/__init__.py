@@ -70,6 +70,7 @@ _SIGS = {
     "phr_last_error": (C.c_char_p, []),
     "phr_abi_version": (_U32, []),
     "phr_gpu_open": (_I, [_I, C.POINTER(_P)]),
+    "phr_gpu_shared": (_I, [C.POINTER(_P)]),
     "phr_gpu_close": (None, [_P]),
     "phr_gpu_wait": (_I, [_P]),
     "phr_gpu_wait_for": (_I, [_P, _U64, C.POINTER(_I)]),
@@ -82,6 +83,9 @@ _SIGS = {
     "phr_array_from": (_I, [_P, _U32, _P, _U64, C.POINTER(_P)]),
     "phr_array_write": (_I, [_P, _P, _U64, C.POINTER(_I)]),
     "phr_array_write_at": (_I, [_P, _U64, _P, _U64, C.POINTER(_I)]),
+    "phr_array_write_front": (_I, [_P, _P, _U64, C.POINTER(_I)]),
+    "phr_array_split": (_I, [_P, _U32, _P, _U64, _U64, _U32, _P, C.POINTER(_U32)]),
+    "phr_array_read_first": (_I, [_P, _U64, _P, C.POINTER(_U64)]),
     "phr_array_read_range": (_I, [_P, _U64, _U64, _P]),
     "phr_array_len": (_U64, [_P]),
     "phr_array_cap": (_U64, [_P]),
@@ -102,6 +106,7 @@ _SIGS = {
     "phr_pass_run_step": (_I, [_P, _U32]),
     "phr_pass_clear": (None, [_P]),
     "phr_pass_len": (_U32, [_P]),
+    "phr_pass_compiled": (_U32, [_P]),
     "phr_bvh_build": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
     "phr_bvh_build_on": (_I, [_P, _P, _U32, C.POINTER(_P)]),
     "phr_bvh_rebuild_async": (_I, [_P]),
@@ -160,6 +165,9 @@ _SIGS = {
     "phr_tracer_normal": (_I, [_P, _P]),
     "phr_tracer_depth_plane": (_I, [_P, _P]),
     "phr_tracer_from_scene": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "phr_tracer_with_chunks": (_I, [_P, _P, _U32, _U64, _U32, C.POINTER(_P)]),
+    "phr_tracer_split": (_I, [_P, _P]),
+    "phr_tracer_chunks": (_U32, [_P]),
     "phr_tracer_reinstance": (_I, [_P, _P, _U32, C.POINTER(_I)]),
     "phr_tracer_repose": (_I, [_P, _P, _U32, C.POINTER(_I)]),
     "phr_tracer_repose_on": (_I, [_P, _P, C.POINTER(_I)]),
@@ -275,6 +283,14 @@ class Gpu:
             return None
         return cls(out.value)
 
+    @classmethod
+    def shared(cls) -> "Gpu | None":
+        """`Gpu::shared`: the process's device, the same one for every caller."""
+        out = C.c_void_p()
+        if lib().phr_gpu_shared(C.byref(out)) != 0:
+            return None
+        return cls(out.value)
+
     def wait(self) -> None:
         _check(lib().phr_gpu_wait(self._h), "Gpu::wait")
 
@@ -339,10 +355,38 @@ class GpuArray:
         return bool(grown.value)
 
     def write_at(self, at: int, items: np.ndarray) -> bool:
+        """`GpuArray::write_at`: False, and nothing written, past the length."""
         items = np.ascontiguousarray(items, dtype=self.dtype)
-        grown = C.c_int(0)
-        _check(lib().phr_array_write_at(self._h, at, _ptr(items), len(items), C.byref(grown)), "GpuArray::write_at")
-        return bool(grown.value)
+        written = C.c_int(0)
+        _check(lib().phr_array_write_at(self._h, at, _ptr(items), len(items), C.byref(written)), "GpuArray::write_at")
+        return bool(written.value)
+
+    def write_front(self, items: np.ndarray) -> bool:
+        """`GpuArray::write_front`: False when the items do not fit."""
+        items = np.ascontiguousarray(items, dtype=self.dtype)
+        written = C.c_int(0)
+        _check(lib().phr_array_write_front(self._h, _ptr(items), len(items), C.byref(written)), "GpuArray::write_front")
+        return bool(written.value)
+
+    def read_first(self, count: int) -> np.ndarray:
+        """`GpuArray::read_first`: the first min(count, len) items."""
+        out = np.empty(max(count, 0), dtype=self.dtype)
+        got = C.c_uint64(0)
+        _check(lib().phr_array_read_first(self._h, count, _ptr(out), C.byref(got)), "GpuArray::read_first")
+        return out[: got.value]
+
+    @classmethod
+    def split(cls, gpu: "Gpu", items: np.ndarray, limit: int, chunks: int) -> "tuple[list[GpuArray], int] | None":
+        """`GpuArray::split`: the items over `chunks` arrays of at most
+        limit bytes each and the items a chunk; None when they do not fit."""
+        items = np.ascontiguousarray(items)
+        hs = (C.c_void_p * chunks)()
+        cut = C.c_uint32(0)
+        rc = lib().phr_array_split(gpu._h, items.dtype.itemsize, _ptr(items), len(items), limit, chunks, hs, C.byref(cut))
+        if rc == -4:  # PHR_ERR_SIZE
+            return None
+        _check(rc, "GpuArray::split")
+        return [cls(gpu, hs[k], items.dtype) for k in range(chunks)], int(cut.value)
 
     def __len__(self) -> int:
         return int(lib().phr_array_len(self._h))
@@ -439,6 +483,10 @@ class Pass:
 
     def __len__(self) -> int:
         return int(lib().phr_pass_len(self._h))
+
+    def compiled(self) -> int:
+        """`Pass::compiled`: kernel types this pass uses."""
+        return int(lib().phr_pass_compiled(self._h))
 
     def __del__(self):
         try:
@@ -672,6 +720,36 @@ class Tracer:
         t = cls(gpu, out.value)
         t._scene = arrs
         return t
+
+    @classmethod
+    def with_chunks(cls, gpu: Gpu, verts: np.ndarray, faces: np.ndarray, mats: np.ndarray, meshes: np.ndarray,
+                    insts: np.ndarray, limit: int, chunks: int, leaf: int = 4) -> "Tracer | None":
+        """`Tracer::with_chunks`: None when the scene does not fit `chunks`
+        bindings of `limit` bytes."""
+        arrs = [np.ascontiguousarray(verts, dtype=VERT), np.ascontiguousarray(faces, dtype=FACE),
+                np.ascontiguousarray(mats, dtype=MAT), np.ascontiguousarray(meshes, dtype=MESH),
+                np.ascontiguousarray(insts, dtype=INSTANCE)]
+        if len(arrs[1]) == 0 or len(arrs[4]) == 0:
+            return None
+        desc = _SceneDesc(*[x for a in arrs for x in (_ptr(a), len(a))])
+        out = C.c_void_p()
+        rc = lib().phr_tracer_with_chunks(gpu._h, C.byref(desc), leaf, limit, chunks, C.byref(out))
+        if rc == -4:  # PHR_ERR_SIZE
+            return None
+        _check(rc, "Tracer::with_chunks")
+        t = cls(gpu, out.value)
+        t._scene = arrs
+        return t
+
+    def split(self) -> tuple[int, int, int, int]:
+        """`Tracer::split`: items a binding for nodes, faces, vertices; atlas offset."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().phr_tracer_split(self._h, out), "Tracer::split")
+        return tuple(int(v) for v in out)
+
+    def chunks(self) -> int:
+        """`Tracer::chunks`: bindings a large array (2 or 4)."""
+        return int(lib().phr_tracer_chunks(self._h))
 
     def reinstance(self, worlds: np.ndarray) -> bool:
         """`Tracer::reinstance`: worlds [n, 16] as m[col][row]; False = build again."""

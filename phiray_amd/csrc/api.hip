@@ -85,8 +85,27 @@ int phr_gpu_open(int device, phr_gpu** out) {
     return PHR_OK;
 }
 
+// Gpu::shared (src/gpu/mod.rs:113-118): the process's device 0, opened on
+// first use and kept until the process ends; every caller gets the same one.
+int phr_gpu_shared(phr_gpu** out) {
+    PHR_CHECK(out, PHR_ERR_ARG, "phr_gpu_shared: null out");
+    static std::mutex m;
+    static phr_gpu* dev = nullptr;
+    std::lock_guard<std::mutex> lock(m);
+    if (!dev) {
+        int rc = phr_gpu_open(0, &dev);
+        if (rc != PHR_OK) {
+            *out = nullptr;
+            return rc;
+        }
+        dev->shared = true;
+    }
+    *out = dev;
+    return PHR_OK;
+}
+
 void phr_gpu_close(phr_gpu* g) {
-    if (!g) return;
+    if (!g || g->shared) return;
     if (g->stream) {
         (void)hipStreamSynchronize(g->stream);
         (void)hipStreamDestroy(g->stream);
@@ -176,26 +195,46 @@ int phr_array_write(phr_array* a, const void* items, uint64_t len, int* grown) {
 
 // GpuArray::write_at (src/gpu/array.rs:88-107): items from `at`, growing the
 // length; `grown` as for write (the old contents are kept across a regrow).
-int phr_array_write_at(phr_array* a, uint64_t at, const void* items, uint64_t len, int* grown) {
-    PHR_CHECK(a && (items || len == 0), PHR_ERR_ARG, "phr_array_write_at: null");
-    int g = 0;
-    uint64_t end = at + len;
-    if (end > a->cap) {
-        uint64_t cap = 1;
-        while (cap < end) cap <<= 1;
-        void* p = nullptr;
-        PHR_HIP(hipMalloc(&p, cap * a->elem));
-        if (a->len) PHR_HIP(hipMemcpyAsync(p, a->ptr, a->len * a->elem, hipMemcpyDeviceToDevice, a->gpu->stream));
-        PHR_HIP(hipStreamSynchronize(a->gpu->stream));
-        (void)hipFree(a->ptr);
-        a->ptr = p;
-        a->cap = cap;
-        g = 1;
-    }
+// GpuArray::write_at (src/gpu/array.rs:96-107): overwrite len items from
+// item `at`, keeping the rest; *written = 0, and nothing written, when they
+// reach past the array's length.
+int phr_array_write_at(phr_array* a, uint64_t at, const void* items, uint64_t len, int* written) {
+    PHR_CHECK(a && written && (items || len == 0), PHR_ERR_ARG, "phr_array_write_at: null");
+    *written = 0;
+    if (at + len > a->len || at + len < at) return PHR_OK;
     if (len) PHR_HIP(hipMemcpyAsync((char*)a->ptr + at * a->elem, items, len * a->elem, hipMemcpyHostToDevice, a->gpu->stream));
-    a->len = std::max(a->len, end);
     PHR_HIP(hipStreamSynchronize(a->gpu->stream));
-    if (grown) *grown = g;
+    *written = 1;
+    return PHR_OK;
+}
+
+// GpuArray::write_front (src/gpu/array.rs:85-94): write_at from item 0.
+int phr_array_write_front(phr_array* a, const void* items, uint64_t len, int* written) {
+    return phr_array_write_at(a, 0, items, len, written);
+}
+
+// GpuArray::split (src/gpu/array.rs:58-83): items over `chunks` arrays of at
+// most limit / elem items each, `cut` items a chunk (at least 1); an unused
+// chunk holds one placeholder item. PHR_ERR_SIZE (the reference's None) when
+// they do not fit. out receives `chunks` handles.
+int phr_array_split(phr_gpu* g, uint32_t elem, const void* items, uint64_t len, uint64_t limit, uint32_t chunks,
+                    phr_array** out, uint32_t* cut) {
+    PHR_CHECK(g && elem && out && cut && (items || len == 0) && chunks >= 1, PHR_ERR_ARG, "phr_array_split: bad argument");
+    const uint64_t cap = std::max<uint64_t>(limit / elem, 1);
+    PHR_CHECK(len <= (uint64_t)chunks * cap, PHR_ERR_SIZE, "phr_array_split: the items do not fit the chunks");
+    const uint64_t c = std::min<uint64_t>(std::max<uint64_t>(len, 1), cap);
+    for (uint32_t k = 0; k < chunks; ++k) out[k] = nullptr;
+    for (uint32_t k = 0; k < chunks; ++k) {
+        const uint64_t lo = k * c, hi = std::min<uint64_t>((k + 1) * c, len);
+        int rc = lo < hi ? phr_array_from(g, elem, (const char*)items + lo * elem, hi - lo, &out[k])
+                         : phr_array_zeroed(g, elem, 1, &out[k]);
+        if (rc != PHR_OK) {
+            for (uint32_t j = 0; j < k; ++j) phr_array_free(out[j]);
+            for (uint32_t j = 0; j < chunks; ++j) out[j] = nullptr;
+            return rc;
+        }
+    }
+    *cut = (uint32_t)c;
     return PHR_OK;
 }
 
@@ -207,6 +246,14 @@ int phr_array_read_range(phr_array* a, uint64_t from, uint64_t count, void* out)
         PHR_HIP(hipMemcpyAsync(out, (const char*)a->ptr + from * a->elem, count * a->elem, hipMemcpyDeviceToHost, a->gpu->stream));
     PHR_HIP(hipStreamSynchronize(a->gpu->stream));
     return PHR_OK;
+}
+
+// GpuArray::read_first (src/gpu/array.rs:138-142): the first min(count, len)
+// items, their number in *got.
+int phr_array_read_first(phr_array* a, uint64_t count, void* out, uint64_t* got) {
+    PHR_CHECK(a && got, PHR_ERR_ARG, "phr_array_read_first: null");
+    *got = std::min<uint64_t>(count, a->len);
+    return phr_array_read_range(a, 0, *got, out);
 }
 
 // ---- Clock (src/gpu/clock.rs:17-87) ------------------------------------------

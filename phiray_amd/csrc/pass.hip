@@ -440,4 +440,17 @@ void phr_pass_clear(phr_pass* p) {
 }
 uint32_t phr_pass_len(const phr_pass* p) { return p ? (uint32_t)p->steps.size() : 0; }
 
+// Pass::compiled (src/gpu/pass.rs:151-154): kernels this pass uses, one per
+// kernel type however many steps run it.
+uint32_t phr_pass_compiled(const phr_pass* p) {
+    if (!p) return 0;
+    uint32_t seen = 0, n = 0;
+    for (const auto& s : p->steps)
+        if (!(seen & (1u << s.kind))) {
+            seen |= 1u << s.kind;
+            ++n;
+        }
+    return n;
+}
+
 }  // extern "C"

@@ -12,6 +12,7 @@ struct phr_gpu {
     int cu_count = 0;
     uint64_t max_alloc = 0;
     char name[256] = {0};
+    bool shared = false;  // the process's device (Gpu::shared): close leaves it open
 };
 
 // A device buffer grown by doubling (src/gpu/array.rs:7-56): `len` items in

@@ -1571,6 +1571,11 @@ struct phr_tracer {
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
     uint32_t path_regen = 0;  // 1: the path-regenerating build where the camera allows it
     uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
+    // The binding split phiray's WGSL needs (items a binding for nodes, faces,
+    // vertices; the atlas offset in the data table) and bindings an array.
+    // The kernels here index whole arrays; the numbers are kept for callers.
+    uint32_t split[4] = {0, 0, 0, 0};
+    uint32_t chunks = 2;
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -2055,6 +2060,29 @@ extern "C" {
 // built on the device, the faces put in leaf order, and the emitter table.
 // PHR_ERR_ARG for an empty scene (the reference's None); PHR_ERR_UNSUPPORTED
 // for a triangle the restated integrator does not carry (transmission).
+namespace {
+
+// The items a binding of `limit` bytes takes for an array of `len` items of
+// `elem` bytes over `chunks` bindings (GpuArray::split, src/gpu/array.rs:70-83);
+// 0 when they do not fit.
+uint32_t split_cut(uint64_t len, uint64_t elem, uint64_t limit, uint32_t chunks) {
+    const uint64_t cap = std::max<uint64_t>(limit / elem, 1);
+    if (len > (uint64_t)chunks * cap) return 0;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(len, 1), cap);
+}
+
+// A new tracer's split at the device's largest allocation over 2 bindings.
+void default_split(phr_tracer* t, uint64_t nodes, uint64_t faces, uint64_t verts) {
+    const uint64_t lim = std::max<uint64_t>(t->gpu->max_alloc, 1);
+    t->split[0] = std::max<uint32_t>(split_cut(nodes, 32, lim, 2), 1);
+    t->split[1] = std::max<uint32_t>(split_cut(faces, 16, lim, 2), 1);
+    t->split[2] = std::max<uint32_t>(split_cut(verts, 32, lim, 2), 1);
+    t->split[3] = 0;
+    t->chunks = 2;
+}
+
+}  // namespace
+
 int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, phr_tracer** out) {
     PHR_CHECK(g && out && tris_v, PHR_ERR_ARG, "phr_tracer_new: null argument");
     PHR_CHECK(n > 0, PHR_ERR_ARG, "phr_tracer_new: empty scene");
@@ -2154,6 +2182,7 @@ int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, ph
         phr_set_error_msg("phr_tracer_new: event creation failed");
         return fail(PHR_ERR_HIP);
     }
+    default_split(t, 2ull * n - 1, n, t->nverts);
     *out = t;
     return PHR_OK;
 }
@@ -2663,6 +2692,46 @@ int phr_tracer_show(phr_tracer* t, const float* linear, uint64_t count, int* sho
     return PHR_OK;
 }
 
+// Tracer::with_chunks (src/trace/build.rs:136-200): Tracer::from_scene with
+// phiray's per-binding byte limit and binding count (2, or 4 when more than
+// 2 are asked for) checked as phiray checks them: PHR_ERR_SIZE, the
+// reference's None, when the scene's largest array (vertices 32 B, faces
+// 16 B, materials) passes limit x chunks, or the nodes, faces or vertices do
+// not split over the bindings.
+int phr_tracer_with_chunks(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, uint64_t limit, uint32_t chunks,
+                           phr_tracer** out) {
+    PHR_CHECK(g && d && out && limit > 0, PHR_ERR_ARG, "phr_tracer_with_chunks: bad argument");
+    *out = nullptr;
+    const uint32_t c = chunks > 2 ? 4u : 2u;
+    const uint64_t biggest = std::max<uint64_t>({(uint64_t)d->nverts * 32, (uint64_t)d->nfaces * 16, (uint64_t)d->nmats * 64 + 64});
+    PHR_CHECK(biggest <= limit * c, PHR_ERR_SIZE, "phr_tracer_with_chunks: the scene's largest array passes the bindings");
+    phr_tracer* t = nullptr;
+    int rc = phr_tracer_from_scene(g, d, leaf, &t);
+    if (rc != PHR_OK) return rc;
+    const uint32_t nc = split_cut(t->nodes->len, 32, limit, c), fc = split_cut(d->nfaces, 16, limit, c),
+                   vc = split_cut(d->nverts, 32, limit, c);
+    if (!nc || !fc || !vc) {
+        phr_tracer_free(t);
+        phr_set_error_msg("phr_tracer_with_chunks: the nodes, faces or vertices do not split over the bindings");
+        return PHR_ERR_SIZE;
+    }
+    t->split[0] = nc;
+    t->split[1] = fc;
+    t->split[2] = vc;
+    t->split[3] = 0;
+    t->chunks = c;
+    *out = t;
+    return PHR_OK;
+}
+
+// Tracer::split / Tracer::chunks (src/trace/mod.rs:118-127).
+int phr_tracer_split(phr_tracer* t, uint32_t out[4]) {
+    PHR_CHECK(t && out, PHR_ERR_ARG, "phr_tracer_split: null argument");
+    std::memcpy(out, t->split, sizeof t->split);
+    return PHR_OK;
+}
+uint32_t phr_tracer_chunks(phr_tracer* t) { return t ? t->chunks : 0u; }
+
 // Tracer::size (src/trace/read.rs:162-164): the image's size, 0 x 0 before
 // the first sample.
 int phr_tracer_size(phr_tracer* t, uint32_t* w, uint32_t* h) {
@@ -2883,6 +2952,7 @@ int phr_tracer_from_scene(phr_gpu* g, const phr_scene_desc* d, uint32_t leaf, ph
         phr_set_error_msg("phr_tracer_from_scene: event creation failed");
         return fail(PHR_ERR_HIP);
     }
+    default_split(t, t->nodes->len, d->nfaces, d->nverts);
     *out = t;
     return PHR_OK;
 }
