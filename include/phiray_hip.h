@@ -222,8 +222,8 @@ void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
 /* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
  * cam is the 208-byte camera record. Each launch also writes the display
  * plane and the packed RGBA8 word (src/trace/wgsl/util.wgsl:101-145) under
- * the camera's view words; only the Standard transform (view[0] = 0) is
- * carried, others are refused with PHR_ERR_UNSUPPORTED. */
+ * the camera's view words (src/trace/wgsl/view.wgsl: Standard, Filmic and
+ * its looks, AgX; exposure; gamma). */
 int phr_tracer_sample(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t* frame, int* done);
 /* Tracer::sample_in (src/trace/sample.rs:104-249): the same in `slices`
  * launches over the image's tiles, the device drained after each and go(user)

@@ -6,7 +6,7 @@
 // src/trace/read.rs:85-119. pow(x, y) is exp2(y log2 x) and exp(x) is
 // exp2(x log2 e), both through the deterministic exp2/log2 of walk.cpp. The
 // last pass writes the display plane and the packed pixel (denoise.wgsl:
-// 124-133) through path.cpp's Standard view transform.
+// 124-133) through path.cpp's view transform.
 #include "oracle.h"
 
 #include <cmath>
@@ -126,7 +126,7 @@ extern "C" void orc_denoise_pass(uint32_t* px, uint32_t w, uint32_t h, uint32_t 
         V3 grid = mul(im.v3_at(20 * n, i), k);
         V3 m = divs(lin, fmax_(met, k));
         float mv[3] = {m.x, m.y, m.z}, d[3];
-        orc_view_standard(mv, view[1], view[2], d);
+        orc_view_transform(mv, view, d);
         V3 shown = add(mul(V3{d[0], d[1], d[2]}, 1.0f - cov), grid);
         px[4 * n + 3 * i] = f_bits(shown.x); px[4 * n + 3 * i + 1] = f_bits(shown.y); px[4 * n + 3 * i + 2] = f_bits(shown.z);
         float sv[3] = {shown.x, shown.y, shown.z};

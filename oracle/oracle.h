@@ -108,10 +108,11 @@ typedef struct {
     float view[4];   /* colour management: transform, exposure (stops), gamma (0: 1), look (camera.rs:53-56) */
 } orc_camera;
 
-/* The display pixel (src/trace/wgsl/view.wgsl:62-107) under the Standard view
- * transform: exposure, sRGB encoding, gamma (0 means 1); and the RGBA8 word,
- * red in the low byte, each channel rounded to nearest. */
-void orc_view_standard(const float lin[3], float exposure, float gamma, float out[3]);
+/* The display pixel (src/trace/wgsl/view.wgsl:5-107) under the view words
+ * (transform 0 Standard, 1 Filmic, 2 AgX; exposure; gamma, 0 meaning 1;
+ * Filmic look 1..6): exposure, the transform, gamma; and the RGBA8 word, red
+ * in the low byte, each channel rounded to nearest. */
+void orc_view_transform(const float lin[3], const float view[4], float out[3]);
 uint32_t orc_pack_rgba8(const float rgb[3], float a);
 
 /* Input triangle: src/trace/kernel.rs:11-40 (Tri) rows a..nc, color, emit —
@@ -222,7 +223,7 @@ void orc_adapt_check(const orc_camera* cam, uint32_t* px);
 void orc_adapt_dilate(const orc_camera* cam, uint32_t* px);
 
 /* One a-trous pass (src/kernel/denoise.wgsl:47-134) over the image buffer;
- * the last writes the display planes under view (Standard transform). */
+ * the last writes the display planes under the view words. */
 void orc_denoise_pass(uint32_t* px, uint32_t w, uint32_t h, uint32_t step, uint32_t round,
                       uint32_t last, uint32_t keep, const float strength[4], const float view[4]);
 /* Tracer::denoise_with (src/trace/read.rs:85-119): steps 1, 2, 4, 8 with

@@ -55,6 +55,7 @@ _SIGS = {
     "orc_adapt_check": (None, [_P, _P]),
     "orc_adapt_dilate": (None, [_P, _P]),
     "orc_denoise_pass": (None, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _P, _P]),
+    "orc_view_transform": (None, [_P, _P, _P]),
     "orc_inst_inverse": (None, [_P, _P]),
     "orc_inst_mirrors": (_U32, [_P]),
     "orc_inst_to_world": (None, [_P, _P, _P]),

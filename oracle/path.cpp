@@ -7,7 +7,7 @@
 //   src/trace/wgsl/util.wgsl:17-34       guide
 //   src/trace/wgsl/util.wgsl:101-145     emit (linear sums, counts, the display
 //                                        plane and the packed pixel)
-//   src/trace/wgsl/view.wgsl:62-107      view_transform_of (Standard only), srgb, pack
+//   src/trace/wgsl/view.wgsl:5-107       view_transform_of (Standard, Filmic and its looks, AgX), srgb, pack
 //   src/trace/wgsl/util.wgsl:162-179     pixel, frame_size, frame_point
 //   src/trace/wgsl/shade.wgsl:41-373     main (camera ray, the camera path)
 //   src/trace/wgsl/integrate.wgsl:6-23   cosine_dir, glow_mis
@@ -487,7 +487,7 @@ void emit(uint32_t* px, size_t n, size_t i, bool keep, V3 lin, const float over[
     float k = (float)taken;
     V3 m = l / fmax_(k - gone, 1.0f);
     float mv[3] = {m.x, m.y, m.z}, d[3];
-    orc_view_standard(mv, p->view[1], p->view[2], d);
+    orc_view_transform(mv, p->view, d);
     V3 shown = v3(d[0], d[1], d[2]) * (1.0f - cov / k) + gs / k;
     pxs(px, 4 * n + 3 * i, shown.x); pxs(px, 4 * n + 3 * i + 1, shown.y); pxs(px, 4 * n + 3 * i + 2, shown.z);
     float sv[3] = {shown.x, shown.y, shown.z};
@@ -690,23 +690,124 @@ void orc_adapt_dilate(const orc_camera* p, uint32_t* px) {
 
 }  // extern "C"
 
-// view.wgsl:62-99 for transform 0 and view.wgsl:88-93 (srgb): exposure as a
-// power of two, the sRGB curve on the clamped channel, then the gamma power.
-// pow(x, y) is exp2(y log2 x), 0 for x <= 0.
+// view.wgsl:5-99: exposure as a power of two; Standard (the sRGB curve on the
+// clamped channel), Filmic (its curve, or a look's) or AgX (its curve between
+// the inset and outset matrices, sRGB-decoded, mixed, encoded again); then
+// the gamma power. pow(x, y) is exp2(y log2 x), 0 for x <= 0. The curves
+// are phiray's samples over log2(x) in [-14.947861, 10.052138], 64 a curve.
+static const float ORC_FILMIC[64] = {
+    0.000015f, 0.000015f, 0.000015f, 0.000015f, 0.000015f, 0.000015f, 0.000015f, 0.001109f,
+    0.002798f, 0.004849f, 0.007337f, 0.010324f, 0.013937f, 0.018276f, 0.023498f, 0.029783f,
+    0.037296f, 0.046297f, 0.057015f, 0.069747f, 0.084803f, 0.102511f, 0.123240f, 0.147317f,
+    0.175060f, 0.206725f, 0.242476f, 0.282333f, 0.326138f, 0.373576f, 0.424074f, 0.476835f,
+    0.530496f, 0.584086f, 0.636811f, 0.687598f, 0.735542f, 0.779871f, 0.820052f, 0.855760f,
+    0.886888f, 0.913539f, 0.935939f, 0.954360f, 0.969213f, 0.980878f, 0.989775f, 0.996255f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+};
+static const float ORC_AGX[64] = {
+    0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000015f,
+    0.000094f, 0.000307f, 0.000714f, 0.001401f, 0.002474f, 0.004091f, 0.006433f, 0.009787f,
+    0.014516f, 0.021097f, 0.030173f, 0.042574f, 0.057306f, 0.073883f, 0.092617f, 0.114256f,
+    0.138886f, 0.167071f, 0.199246f, 0.235968f, 0.277732f, 0.324850f, 0.377462f, 0.435243f,
+    0.495499f, 0.553566f, 0.607904f, 0.657973f, 0.703691f, 0.745144f, 0.782605f, 0.816304f,
+    0.846638f, 0.873680f, 0.897952f, 0.919880f, 0.939710f, 0.957668f, 0.973975f, 0.988673f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+};
+static const float ORC_LOOKS[6][64] = {
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000070f,
+    0.000193f, 0.000363f, 0.000601f, 0.000931f, 0.001389f, 0.002026f, 0.002910f, 0.004138f,
+    0.005841f, 0.008200f, 0.011463f, 0.015966f, 0.022162f, 0.030652f, 0.042221f, 0.057863f,
+    0.078793f, 0.106414f, 0.142202f, 0.187489f, 0.243112f, 0.308980f, 0.383689f, 0.464395f,
+    0.547092f, 0.627392f, 0.701407f, 0.766409f, 0.821117f, 0.865535f, 0.900558f, 0.927541f,
+    0.947961f, 0.963206f, 0.974473f, 0.982737f, 0.988766f, 0.993146f, 0.996319f, 0.998612f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f},
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000233f,
+    0.000620f, 0.001128f, 0.001793f, 0.002666f, 0.003810f, 0.005309f, 0.007269f, 0.009833f,
+    0.013179f, 0.017541f, 0.023213f, 0.030568f, 0.040067f, 0.052275f, 0.067866f, 0.087615f,
+    0.112373f, 0.143010f, 0.180318f, 0.224866f, 0.276828f, 0.335802f, 0.400681f, 0.469646f,
+    0.540236f, 0.609827f, 0.675954f, 0.736553f, 0.790239f, 0.836358f, 0.874903f, 0.906341f,
+    0.931432f, 0.951064f, 0.966136f, 0.977482f, 0.985838f, 0.991826f, 0.995958f, 0.998647f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f},
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000510f,
+    0.001330f, 0.002363f, 0.003665f, 0.005306f, 0.007372f, 0.009971f, 0.013237f, 0.017335f,
+    0.022469f, 0.028884f, 0.036880f, 0.046810f, 0.059090f, 0.074194f, 0.092650f, 0.115017f,
+    0.141861f, 0.173696f, 0.210924f, 0.253744f, 0.302067f, 0.355438f, 0.412989f, 0.473463f,
+    0.535105f, 0.596234f, 0.655290f, 0.710781f, 0.761550f, 0.806854f, 0.846356f, 0.880079f,
+    0.908313f, 0.931529f, 0.950296f, 0.965209f, 0.976851f, 0.985758f, 0.992407f, 0.997209f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f},
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.001808f,
+    0.004533f, 0.007732f, 0.011485f, 0.015885f, 0.021036f, 0.027059f, 0.034090f, 0.042283f,
+    0.051810f, 0.062859f, 0.075637f, 0.090362f, 0.107263f, 0.126574f, 0.148519f, 0.173307f,
+    0.201111f, 0.232056f, 0.266193f, 0.303481f, 0.343773f, 0.386797f, 0.432150f, 0.479308f,
+    0.526904f, 0.574394f, 0.621585f, 0.667736f, 0.712148f, 0.754204f, 0.793394f, 0.829342f,
+    0.861800f, 0.890655f, 0.915907f, 0.937653f, 0.956064f, 0.971365f, 0.983812f, 0.993680f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f},
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.003014f,
+    0.007419f, 0.012418f, 0.018087f, 0.024509f, 0.031775f, 0.039988f, 0.049257f, 0.059700f,
+    0.071444f, 0.084623f, 0.099377f, 0.115848f, 0.134178f, 0.154507f, 0.176963f, 0.201658f,
+    0.228681f, 0.258089f, 0.289897f, 0.324071f, 0.360515f, 0.399071f, 0.439508f, 0.481524f,
+    0.523400f, 0.565001f, 0.606881f, 0.648556f, 0.689527f, 0.729308f, 0.767438f, 0.803505f,
+    0.837159f, 0.868121f, 0.896191f, 0.921245f, 0.943236f, 0.962183f, 0.978162f, 0.991297f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f},
+    {0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.000000f, 0.004685f,
+    0.011326f, 0.018609f, 0.026591f, 0.035334f, 0.044905f, 0.055372f, 0.066812f, 0.079302f,
+    0.092923f, 0.107759f, 0.123898f, 0.141424f, 0.160426f, 0.180986f, 0.203184f, 0.227091f,
+    0.252769f, 0.280265f, 0.309608f, 0.340804f, 0.373832f, 0.408639f, 0.445135f, 0.483188f,
+    0.520119f, 0.556217f, 0.593106f, 0.630501f, 0.668084f, 0.705500f, 0.742373f, 0.778309f,
+    0.812910f, 0.845788f, 0.876574f, 0.904934f, 0.930579f, 0.953271f, 0.972832f, 0.989149f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f,
+    1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f}
+};
+static const float ORC_AGX_IN[3][3] = {0.498585f, 0.125929f, 0.097476f, 0.346742f, 0.681639f, 0.191866f, 0.097839f, 0.123183f, 0.678532f};   /* [column][row] */
+static const float ORC_AGX_OUT[3][3] = {2.055217f, -0.287805f, -0.221530f, -0.912124f, 1.396234f, -0.286374f, -0.141522f, -0.097287f, 1.507346f};
+
+static float orc_pow(float x, float y) { return x > 0.0f ? orc_exp2(y * orc_log2(x)) : 0.0f; }
+static float orc_clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+
 static float orc_srgb1(float c) {
-    float x = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    float x = orc_clamp01(c);
     if (x <= 0.0031308f) return x * 12.92f;
-    return 1.055f * orc_exp2((1.0f / 2.4f) * orc_log2(x)) - 0.055f;
+    return 1.055f * orc_pow(x, 1.0f / 2.4f) - 0.055f;
+}
+static float orc_srgb_inv(float c) { return c <= 0.04045f ? c / 12.92f : orc_pow((c + 0.055f) / 1.055f, 2.4f); }
+
+// view.wgsl:37-54: the curve linearly interpolated at log2(max(x, 1e-10)).
+static float orc_curve(const float* curve, float x) {
+    float l = orc_log2(x > 1e-10f ? x : 1e-10f);
+    float t = orc_clamp01((l - (-14.947861f)) / (10.052138f - (-14.947861f))) * 63.0f;
+    float fl = std::floor(t);
+    unsigned i = (unsigned)fl, j = i + 1u < 63u ? i + 1u : 63u;
+    float f = t - fl;
+    return curve[i] * (1.0f - f) + curve[j] * f;
 }
 
-void orc_view_standard(const float lin[3], float exposure, float gamma, float out[3]) {
-    float s = orc_exp2(exposure);
-    float g = 1.0f / (gamma <= 0.0f ? 1.0f : gamma);
-    for (int j = 0; j < 3; ++j) {
-        float d = orc_srgb1((lin[j] > 0.0f ? lin[j] : 0.0f) * s);
-        d = d < 0.0f ? 0.0f : (d > 1.0f ? 1.0f : d);
-        out[j] = d > 0.0f ? orc_exp2(g * orc_log2(d)) : 0.0f;
+void orc_view_transform(const float lin[3], const float view[4], float out[3]) {
+    float s = orc_exp2(view[1]);
+    float c[3], d[3];
+    for (int j = 0; j < 3; ++j) c[j] = (lin[j] > 0.0f ? lin[j] : 0.0f) * s;
+    unsigned kind = (unsigned)(view[0] + 0.5f);
+    if (kind == 1u) {
+        unsigned look = (unsigned)(view[3] + 0.5f);
+        const float* curve = (look >= 1u && look <= 6u) ? ORC_LOOKS[look - 1u] : ORC_FILMIC;
+        for (int j = 0; j < 3; ++j) d[j] = orc_curve(curve, c[j]);
+    } else if (kind == 2u) {
+        float in[3], y[3], o[3];
+        for (int r = 0; r < 3; ++r) in[r] = ORC_AGX_IN[0][r] * c[0] + ORC_AGX_IN[1][r] * c[1] + ORC_AGX_IN[2][r] * c[2];
+        for (int j = 0; j < 3; ++j) y[j] = orc_srgb_inv(orc_curve(ORC_AGX, in[j]));
+        for (int r = 0; r < 3; ++r) o[r] = ORC_AGX_OUT[0][r] * y[0] + ORC_AGX_OUT[1][r] * y[1] + ORC_AGX_OUT[2][r] * y[2];
+        for (int j = 0; j < 3; ++j) d[j] = orc_srgb1(o[j]);
+    } else {
+        for (int j = 0; j < 3; ++j) d[j] = orc_srgb1(c[j]);
     }
+    float g = 1.0f / (view[2] <= 0.0f ? 1.0f : view[2]);
+    for (int j = 0; j < 3; ++j) out[j] = orc_pow(orc_clamp01(d[j]), g);
 }
 
 // view.wgsl:101-107.

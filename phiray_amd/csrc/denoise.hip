@@ -5,7 +5,7 @@
 // albedo back into the linear sums. One thread per pixel; passes ping-pong
 // between the two scratch planes, so a pass never reads what it writes.
 // The last pass also writes the display plane and the packed RGBA8 word
-// under the camera's exposure and gamma (Standard view transform).
+// under the camera's view transform, exposure and gamma.
 #include "phr_runtime.h"
 
 using namespace phr;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_denoise_prep(const uint32_t* __restrict
 template <bool CACHED>
 __global__ __launch_bounds__(256) void k_denoise(uint32_t* __restrict__ px, uint32_t w, uint32_t h, uint32_t step, uint32_t round,
                                                  uint32_t last, uint32_t keep, float s0, float s1, float s2, float s3,
-                                                 float exposure, float gamma, const float4* __restrict__ feat,
+                                                 float4 view, const float4* __restrict__ feat,
                                                  const float* __restrict__ col0) {
     const size_t n = (size_t)w * h;
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_denoise(uint32_t* __restrict__ px, uint
     // tracer composes an unfiltered pixel (denoise.wgsl:124-133).
     const float cov = im.f_at(23 * n + i) * k;
     const V3 grid = im.v3_at(20 * n, i) * k;
-    const V3 shown = view_standard(lin / fmax_(met, k), exposure, gamma) * (1.0f - cov) + grid;
+    const V3 shown = view_transform(lin / fmax_(met, k), view.x, view.y, view.z, view.w) * (1.0f - cov) + grid;
     px[4 * n + 3 * i] = __float_as_uint(shown.x);
     px[4 * n + 3 * i + 1] = __float_as_uint(shown.y);
     px[4 * n + 3 * i + 2] = __float_as_uint(shown.z);
@@ -166,7 +166,7 @@ int phr_denoise_pass_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, u
     const size_t n = (size_t)w * h;
     if (n == 0) return PHR_OK;
     hipLaunchKernelGGL(k_denoise<false>, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, px, w, h, step, round, last,
-                       keep, s[0], s[1], s[2], s[3], view[1], view[2], nullptr, nullptr);
+                       keep, s[0], s[1], s[2], s[3], make_float4(view[0], view[1], view[2], view[3]), nullptr, nullptr);
     PHR_HIP(hipGetLastError());
     return PHR_OK;
 }
@@ -187,12 +187,12 @@ int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int ke
         hipLaunchKernelGGL(k_denoise_prep, grid, dim3(256), 0, g->stream, (const uint32_t*)px, n, feat, col0);
         for (uint32_t r = 0; r < 4; ++r)
             hipLaunchKernelGGL(k_denoise<true>, grid, dim3(256), 0, g->stream, px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u,
-                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], view[1], view[2], (const float4*)feat,
+                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], make_float4(view[0], view[1], view[2], view[3]), (const float4*)feat,
                                (const float*)col0);
     } else {
         for (uint32_t r = 0; r < 4; ++r)
             hipLaunchKernelGGL(k_denoise<false>, grid, dim3(256), 0, g->stream, px, w, h, steps[r], r, steps[r] == 8 ? 1u : 0u,
-                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], view[1], view[2], nullptr, nullptr);
+                               keep ? 1u : 0u, s[0], s[1], s[2], s[3], make_float4(view[0], view[1], view[2], view[3]), nullptr, nullptr);
     }
     PHR_HIP(hipGetLastError());
     return PHR_OK;

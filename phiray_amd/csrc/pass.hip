@@ -368,11 +368,6 @@ int phr_pass_add(phr_pass* p, const char* name, uint32_t n, uint32_t items, cons
     PHR_CHECK(param_len == ki.param_size, PHR_ERR_ARG, std::string(name) + ": param size");
     PHR_CHECK(count == ki.binds, PHR_ERR_ARG, std::string(name) + ": binding count");
     for (uint32_t k = 0; k < count; ++k) PHR_CHECK(arrays && arrays[k] && arrays[k]->gpu == p->gpu, PHR_ERR_ARG, std::string(name) + ": binding");
-    if (kind == 9) {
-        DenoiseParam d;
-        std::memcpy(&d, params, sizeof d);
-        PHR_CHECK(d.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "denoise: only the Standard view transform (view[0] = 0)");
-    }
     phr_pass::Step s;
     s.kind = kind;
     s.params.assign((const uint8_t*)params, (const uint8_t*)params + param_len);

@@ -856,7 +856,7 @@ __device__ __forceinline__ void stf(uint32_t* px, size_t w, float v) { px[w] = _
 __device__ __forceinline__ void put_shown(uint32_t* px, size_t n, size_t i, const PhrCamera& cam, V3 l, V3 gs, float cov,
                                           float gone, float film, uint32_t taken) {
     const float k = (float)taken;
-    const V3 s = shown_of(l, gs, cov, gone, k, cam.view[1], cam.view[2]);
+    const V3 s = shown_of(l, gs, cov, gone, k, make_float4(cam.view[0], cam.view[1], cam.view[2], cam.view[3]));
     stf(px, 4 * n + 3 * i, s.x);
     stf(px, 4 * n + 3 * i + 1, s.y);
     stf(px, 4 * n + 3 * i + 2, s.z);
@@ -1548,6 +1548,7 @@ extern "C" void phr_bvh_free(phr_bvh* b);
 extern "C" int phr_bvh_refit_async(phr_bvh* b);
 extern "C" uint32_t phr_bvh_depth(const phr_bvh* b);
 int phr_denoise_enqueue(phr_gpu* g, uint32_t* px, uint32_t w, uint32_t h, int keep, float* scratch, const float view[4]);
+inline float4 view4(const PhrCamera& c) { return make_float4(c.view[0], c.view[1], c.view[2], c.view[3]); }
 constexpr uint64_t DENOISE_SCRATCH_WORDS = 11;  // guides (8) and the first pass's input (3)
 
 struct phr_tracer {
@@ -2276,7 +2277,6 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
     PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
               "phr_tracer_sample: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only phiray's own pixel filter (0)");
-    PHR_CHECK(cam.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only the Standard view transform (view[0] = 0)");
     if (int rc0 = forest_settle(t); rc0 != PHR_OK) return rc0;
     if (cam.adapt[3] != 0.0f) {
         const uint32_t st = (uint32_t)cam.adapt[2];
@@ -2495,10 +2495,10 @@ __global__ __launch_bounds__(256) void k_frame(const uint32_t* __restrict__ px, 
 
 // A mean image back into the linear sums (sum = value * count), and the
 // display plane and packed word from the new sums as the tracer writes them
-// (util.wgsl:139-144, show.wgsl:26-45) under exposure and gamma: the learned
+// (util.wgsl:139-144, show.wgsl:26-45) under the camera's view words: the learned
 // denoiser's output, or a host's image for Tracer::show.
 __global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, uint64_t n, const float* __restrict__ rad,
-                                                    float exposure, float gamma) {
+                                                    float4 view) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = px[31 * n + i];
@@ -2508,7 +2508,7 @@ __global__ __launch_bounds__(256) void k_frame_back(uint32_t* __restrict__ px, u
     stf(px, n + 3 * i + 1, l.y);
     stf(px, n + 3 * i + 2, l.z);
     const V3 gs = v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2));
-    const V3 sh = shown_of(l, gs, ldf(px, 23 * n + i), ldf(px, 33 * n + i), k, exposure, gamma);
+    const V3 sh = shown_of(l, gs, ldf(px, 23 * n + i), ldf(px, 33 * n + i), k, view);
     stf(px, 4 * n + 3 * i, sh.x);
     stf(px, 4 * n + 3 * i + 1, sh.y);
     stf(px, 4 * n + 3 * i + 2, sh.z);
@@ -2610,7 +2610,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
         hipLaunchKernelGGL(k_frame, dim3(g), dim3(256), 0, s, (const uint32_t*)pxp, n, t->frame_dev);
         rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
         if (rc == PHR_OK)
-            hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, n, (const float*)rad, t->last.view[1], t->last.view[2]);
+            hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, n, (const float*)rad, view4(t->last));
     }
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s, &graph);
@@ -2684,8 +2684,7 @@ int phr_tracer_show(phr_tracer* t, const float* linear, uint64_t count, int* sho
     if (m) PHR_HIP(hipMemcpyAsync(rad, linear, m * 12, hipMemcpyHostToDevice, s));
     if (m < n) PHR_HIP(hipMemsetAsync(rad + 3 * m, 0, (n - m) * 12, s));
     const uint32_t g = (uint32_t)((n + 255) / 256);
-    hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, (uint32_t*)t->px->ptr, n, (const float*)rad, t->last.view[1],
-                       t->last.view[2]);
+    hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, (uint32_t*)t->px->ptr, n, (const float*)rad, view4(t->last));
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipStreamSynchronize(s));
     *shown = 1;
@@ -2755,8 +2754,7 @@ int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
     rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
     if (rc != PHR_OK) return rc;
     if (!out) {
-        hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, t->gpu->stream, (uint32_t*)t->px->ptr, n, rad,
-                           t->last.view[1], t->last.view[2]);
+        hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, t->gpu->stream, (uint32_t*)t->px->ptr, n, rad, view4(t->last));
         PHR_HIP(hipGetLastError());
     }
     PHR_HIP(hipStreamSynchronize(t->gpu->stream));
@@ -3131,7 +3129,6 @@ int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
               "phr_batch_render: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_batch_render: only phiray's own pixel filter (0)");
     PHR_CHECK(cam.adapt[3] == 0.0f, PHR_ERR_UNSUPPORTED, "phr_batch_render: adaptive sampling is per tracer");
-    PHR_CHECK(cam.view[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_batch_render: only the Standard view transform (view[0] = 0)");
     return PHR_OK;
 }
 

@@ -147,13 +147,15 @@ def test_log2_against_libm(orc):
 def test_gpu_denoise_matches_the_restatement(gpu, orc, keep):
     print(f"[denoise] keep={keep}", flush=True)
     tris, cam = scene()
+    if keep:  # the display under a Filmic look (Medium Low Contrast)
+        cam["view"] = (1.0, 0.5, 0.0, 4.0)
     tr = P.Tracer.new(gpu, tris, 4)
     tr.set_samples(64)
     tr.sample(cam, SIDE, SIDE, 12)
     before = tr.px()
     tr.denoise(keep=keep)
     got = tr.px()
-    want = denoise(orc, before, keep)
+    want = denoise(orc, before, keep, view=tuple(float(v) for v in cam[0]["view"]))
     n = SIDE * SIDE
     for lo, hi in [(1 * n, 4 * n), (4 * n, 7 * n), (14 * n, 20 * n)]:
         a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)

@@ -161,3 +161,42 @@ def test_display_is_the_view_transform_of_the_mean(orc, exposure, gamma):
     assert (np.abs(pl["pixel"][:, :3].astype(np.int32) - q) <= 1).all()
     assert (pl["pixel"][:, 3] == 255).all()
     assert pl["display"].max() > 0.3
+
+
+def view_of(orc, lin, view):
+    lin = np.ascontiguousarray(lin, dtype=np.float32)
+    v = np.asarray(view, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    orc.lib().orc_view_transform(orc.ptr(lin), orc.ptr(v), orc.ptr(out))
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_view_transforms_behave(orc, kind):
+    """view.wgsl:5-99 under each transform: the display rises with the light
+    and saturates, halving the light does not halve the display (the
+    transforms are concave), and the exposure is a factor of two a stop.
+    The curve samples themselves are phiray's table (parity unpinned beyond
+    these behaviours)."""
+    ramp = np.geomspace(1e-4, 64.0, 200)
+    shown = np.array([view_of(orc, (x, x, x), (kind, 0, 0, 0)) for x in ramp])
+    # Per channel under Standard and Filmic; AgX's outset matrix has negative
+    # terms, so there the luminance rises while a channel may dip.
+    luma = shown @ np.array([0.2126, 0.7152, 0.0722])
+    assert (np.diff(shown if kind < 2 else luma[:, None], axis=0) >= (-1e-7 if kind < 2 else -1e-4)).all()
+    assert shown[-1].min() > 0.98 and shown[0].max() < 0.05
+    for x in (0.05, 0.18, 0.6):
+        full, half = view_of(orc, (x, x, x), (kind, 0, 0, 0)), view_of(orc, (x / 2, x / 2, x / 2), (kind, 0, 0, 0))
+        assert (half < full).all() and (half > 0.5 * full).all(), (x, full, half)
+        up = view_of(orc, (x, x, x), (kind, 1.0, 0, 0))
+        assert np.allclose(up, view_of(orc, (2 * x, 2 * x, 2 * x), (kind, 0, 0, 0)), rtol=2e-6, atol=1e-7)
+
+
+def test_filmic_looks_bend_the_midtones_their_way(orc):
+    """tests/show.rs:120-150: against the base Filmic curve, Very High
+    Contrast (look 1) pushes a dim value darker and Very Low Contrast (6)
+    lifts it; the looks do nothing under Standard."""
+    dim = (0.05, 0.05, 0.05)
+    base = view_of(orc, dim, (1, 0, 0, 0))[0]
+    assert view_of(orc, dim, (1, 0, 0, 1))[0] < base < view_of(orc, dim, (1, 0, 0, 6))[0]
+    assert (view_of(orc, dim, (0, 0, 0, 1)) == view_of(orc, dim, (0, 0, 0, 0))).all()
