@@ -219,6 +219,12 @@ int phr_tracer_raycast(phr_tracer* t, const void* rays, uint32_t nrays, int any_
 int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array* hits);
 /* Tracer::set_samples (src/trace/mod.rs:242). */
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
+/* Tracer::samples (src/trace/mod.rs:152-155). */
+uint32_t phr_tracer_samples(phr_tracer* t);
+/* Tracer::set_adaptive (src/trace/mod.rs:235-240): Adaptive Sampling's
+ * [threshold, min samples, step, on], stamped onto every later camera
+ * (before the first call, the camera's own `adapt` words are used). */
+int phr_tracer_set_adaptive(phr_tracer* t, const float adapt[4]);
 /* Tracer::sample (src/trace/sample.rs:96-99), spp samples in one launch;
  * cam is the 208-byte camera record. Each launch also writes the display
  * plane and the packed RGBA8 word (src/trace/wgsl/util.wgsl:101-145) under

@@ -161,6 +161,8 @@ _SIGS = {
     "phr_tracer_pixels": (_I, [_P, _P]),
     "phr_tracer_display": (_I, [_P, _P]),
     "phr_tracer_show": (_I, [_P, _P, _U64, _P]),
+    "phr_tracer_samples": (_U32, [_P]),
+    "phr_tracer_set_adaptive": (_I, [_P, _P]),
     "phr_tracer_size": (_I, [_P, _P, _P]),
     "phr_tracer_normal": (_I, [_P, _P]),
     "phr_tracer_depth_plane": (_I, [_P, _P]),
@@ -967,6 +969,15 @@ class Tracer:
         shown = C.c_int(0)
         _check(lib().phr_tracer_show(self._h, _ptr(lin), len(lin), C.byref(shown)), "Tracer::show")
         return bool(shown.value)
+
+    def samples(self) -> int:
+        """`Tracer::samples`: samples averaged before a still view stops refining."""
+        return int(lib().phr_tracer_samples(self._h))
+
+    def set_adaptive(self, adapt) -> None:
+        """`Tracer::set_adaptive`: [threshold, min samples, step, on] on every later camera."""
+        a = np.ascontiguousarray(adapt, dtype=np.float32).reshape(4)
+        _check(lib().phr_tracer_set_adaptive(self._h, _ptr(a)), "Tracer::set_adaptive")
 
     def image_size(self) -> tuple[int, int]:
         """`Tracer::size`: the image's (w, h), (0, 0) before the first sample."""

@@ -1577,6 +1577,10 @@ struct phr_tracer {
     // The kernels here index whole arrays; the numbers are kept for callers.
     uint32_t split[4] = {0, 0, 0, 0};
     uint32_t chunks = 2;
+    // Tracer::set_adaptive (src/trace/mod.rs:235-240): stamped onto every
+    // camera once set; until then the camera's own `adapt` words are used.
+    float adapt[4] = {0.f, 0.f, 0.f, 0.f};
+    bool adapt_set = false;
     bool have_last = false;
     PhrCamera last{};
     PhrCamera* cam_dev = nullptr;  // the stamped camera the path kernel reads
@@ -2255,6 +2259,16 @@ int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array
 
 // Tracer::set_samples (src/trace/mod.rs:242).
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n) { if (t) t->samples = n < 1 ? 1 : n; }
+// Tracer::samples (src/trace/mod.rs:152-155).
+uint32_t phr_tracer_samples(phr_tracer* t) { return t ? t->samples : 0u; }
+// Tracer::set_adaptive (src/trace/mod.rs:235-240): [threshold, min samples,
+// step, on] stamped onto every later camera; all zero turns it off.
+int phr_tracer_set_adaptive(phr_tracer* t, const float adapt[4]) {
+    PHR_CHECK(t && adapt, PHR_ERR_ARG, "phr_tracer_set_adaptive: null argument");
+    std::memcpy(t->adapt, adapt, sizeof t->adapt);
+    t->adapt_set = true;
+    return PHR_OK;
+}
 
 // Tracer::sample_in (src/trace/sample.rs:104-249) without slices: `spp`
 // samples in one launch. The camera is stamped as the reference stamps it
@@ -2274,6 +2288,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
     h = std::max<uint32_t>(h, 1);
     PhrCamera cam;
     std::memcpy(&cam, cam_v, sizeof cam);
+    if (t->adapt_set) std::memcpy(cam.adapt, t->adapt, sizeof cam.adapt);
     PHR_CHECK(!(cam.depth[0] == 0u || cam.depth[3] > 1u), PHR_ERR_UNSUPPORTED,
               "phr_tracer_sample: the glossy lobe is not restated; set Max Bounces (depth) with glossy 0");
     PHR_CHECK(cam.integrator[0] < 0.5f, PHR_ERR_UNSUPPORTED, "phr_tracer_sample: only phiray's own pixel filter (0)");

@@ -215,6 +215,19 @@ def test_adaptive_sampling_matches_the_twin(gpu, orc):
         bad = cam.copy()
         bad["adapt"] = (0.08, 3.0, 3.0, 1.0)
         tr.sample(bad, w, h, 1)
+    # Tracer::set_adaptive stamps the same words onto a camera without them.
+    t2 = P.Tracer.new(gpu, tris, 4)
+    t2.set_samples(40)
+    assert t2.samples() == 40
+    t2.set_adaptive((0.08, 3.0, 2.0, 1.0))
+    plain = S.path_camera(w, h, rr_from=3)
+    for spp in (5, 2, 9):
+        t2.sample(plain, w, h, spp)
+    g2 = t2.px()  # the open-pixel list (32n..33n) is in atomic order: its open entries compared sorted
+    assert np.array_equal(g2[:32 * n], got[:32 * n]) and np.array_equal(g2[33 * n:], got[33 * n:])
+    k = int(got[34 * n])  # past the open count the list holds older entries
+    assert np.array_equal(np.sort(g2[32 * n:32 * n + k]), np.sort(got[32 * n:32 * n + k]))
+    t2.free()
 
 
 def test_sample_in_slices(gpu, orc):
