@@ -219,6 +219,10 @@ int phr_tracer_raycast(phr_tracer* t, const void* rays, uint32_t nrays, int any_
 int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array* hits);
 /* Tracer::set_samples (src/trace/mod.rs:242). */
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n);
+/* Tracer::warm (src/trace/build.rs:27-69): load every path-kernel build on
+ * the device ahead of its first frame; names[i] (static strings) and ms[i]
+ * per build, *count entries (at most cap). */
+int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, uint32_t* count);
 /* Tracer::samples (src/trace/mod.rs:152-155). */
 uint32_t phr_tracer_samples(phr_tracer* t);
 /* Tracer::set_adaptive (src/trace/mod.rs:235-240): Adaptive Sampling's

@@ -162,6 +162,7 @@ _SIGS = {
     "phr_tracer_display": (_I, [_P, _P]),
     "phr_tracer_show": (_I, [_P, _P, _U64, _P]),
     "phr_tracer_samples": (_U32, [_P]),
+    "phr_tracer_warm": (_I, [_P, _U32, _P, _P, C.POINTER(_U32)]),
     "phr_tracer_set_adaptive": (_I, [_P, _P]),
     "phr_tracer_size": (_I, [_P, _P, _P]),
     "phr_tracer_normal": (_I, [_P, _P]),
@@ -969,6 +970,16 @@ class Tracer:
         shown = C.c_int(0)
         _check(lib().phr_tracer_show(self._h, _ptr(lin), len(lin), C.byref(shown)), "Tracer::show")
         return bool(shown.value)
+
+    @staticmethod
+    def warm(gpu: Gpu) -> list[tuple[str, float]]:
+        """`Tracer::warm`: each path-kernel build loaded, with its load time in ms."""
+        cap = 64
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        count = C.c_uint32(0)
+        _check(lib().phr_tracer_warm(gpu._h, cap, names, ms, C.byref(count)), "Tracer::warm")
+        return [(names[i].decode(), float(ms[i])) for i in range(count.value)]
 
     def samples(self) -> int:
         """`Tracer::samples`: samples averaged before a still view stops refining."""

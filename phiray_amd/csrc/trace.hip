@@ -2259,6 +2259,41 @@ int phr_tracer_raycast_on(phr_tracer* t, phr_array* rays, int any_hit, phr_array
 
 // Tracer::set_samples (src/trace/mod.rs:242).
 void phr_tracer_set_samples(phr_tracer* t, uint32_t n) { if (t) t->samples = n < 1 ? 1 : n; }
+// Tracer::warm (src/trace/build.rs:27-69): load every path-kernel build on
+// the device before its first frame, so a switch of build (the autotune, a
+// forest, path regeneration) does not wait for its code object. Each entry
+// gets the kernel's name and the milliseconds its load took; a build loaded
+// already costs next to nothing. *count: entries written (at most cap).
+int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, uint32_t* count) {
+    PHR_CHECK(g && count && (cap == 0 || (names && ms)), PHR_ERR_ARG, "phr_tracer_warm: bad argument");
+    PHR_HIP(hipSetDevice(g->device));
+    struct K { const char* name; const void* fn; };
+    const K ks[] = {
+        {"k_path<2>", (const void*)k_path<false, 2>}, {"k_path<3>", (const void*)k_path<false, 3>},
+        {"k_path<4>", (const void*)k_path<false, 4>}, {"k_path<5>", (const void*)k_path<false, 5>},
+        {"k_path_forest<2>", (const void*)k_path<true, 2>}, {"k_path_forest<3>", (const void*)k_path<true, 3>},
+        {"k_path_forest<4>", (const void*)k_path<true, 4>}, {"k_path_rg<2>", (const void*)k_path_rg<2>},
+        {"k_path_rg<3>", (const void*)k_path_rg<3>}, {"k_path_rg<4>", (const void*)k_path_rg<4>},
+        {"k_path_rg<5>", (const void*)k_path_rg<5>}, {"k_path_batch<2>", (const void*)k_path_batch<2>},
+        {"k_path_batch<3>", (const void*)k_path_batch<3>}, {"k_path_batch<4>", (const void*)k_path_batch<4>},
+        {"k_path_batch<5>", (const void*)k_path_batch<5>},
+    };
+    uint32_t n = 0;
+    for (const K& k : ks) {
+        auto t0 = std::chrono::steady_clock::now();
+        hipFuncAttributes attr;
+        PHR_HIP(hipFuncGetAttributes(&attr, k.fn));
+        const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (n < cap) {
+            names[n] = k.name;
+            ms[n] = dt;
+        }
+        ++n;
+    }
+    *count = std::min(n, cap);
+    return PHR_OK;
+}
+
 // Tracer::samples (src/trace/mod.rs:152-155).
 uint32_t phr_tracer_samples(phr_tracer* t) { return t ? t->samples : 0u; }
 // Tracer::set_adaptive (src/trace/mod.rs:235-240): [threshold, min samples,

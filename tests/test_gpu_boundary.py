@@ -85,3 +85,19 @@ def test_with_chunks_applies_the_binding_limits(gpu):
     assert P.Tracer.with_chunks(gpu, *arrs, limit=64, chunks=2) is None
     t.free()
     base.free()
+
+
+def test_warm_loads_every_path_build(gpu):
+    """Tracer::warm (src/trace/build.rs:27-69): every build named once with
+    its load time; a tracer renders after it as before."""
+    print("[boundary] warm", flush=True)
+    first = P.Tracer.warm(gpu)
+    names = [n for n, _ in first]
+    assert len(names) == len(set(names)) == 15 and "k_path<4>" in names and "k_path_batch<5>" in names
+    assert all(ms >= 0.0 for _, ms in first)
+    again = P.Tracer.warm(gpu)
+    assert [n for n, _ in again] == names
+    tr = P.Tracer.new(gpu, S.spheres_scene(spheres=2, level=1, seed=4), 4)
+    tr.sample(S.path_camera(16, 16, rr_from=3), 16, 16, 1)
+    assert P.planes(tr.px(), 16, 16)["count"].min() == 1
+    tr.free()
