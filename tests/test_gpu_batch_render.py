@@ -166,3 +166,21 @@ def test_full_batch_scenes_match_their_tracers(gpu):
         assert P.planes(want, w, h)["radiance"].max() > 0
         tr.free()
     b.free()
+
+
+def test_batch_display_under_agx_matches_the_tracer(gpu):
+    """The display step in the batch launch follows the camera's view words
+    (AgX, exposure, gamma) exactly as a tracer's launch does."""
+    print("[batch render] AgX display", flush=True)
+    parts, tris, lo = batch_scenes()
+    b = P.Batch(gpu, tris[: lo[4]], lo[:5])
+    cam = camera()
+    cam["view"] = (2.0, 0.7, 1.4, 0.0)
+    b.set_path(3, 0)
+    b.render(cam, W, H, SPP)
+    for k in (0, 2):
+        tr = P.Tracer.new(gpu, parts[k], 4)
+        tr.sample(cam, W, H, SPP)
+        assert np.array_equal(b.px(k), tr.px()), k
+        tr.free()
+    b.free()
