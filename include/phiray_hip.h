@@ -10,7 +10,10 @@
  *    phr_last_error() describes the last failure on the calling thread;
  *  - plain pointers and sizes only; POD records cross by bytes in phiray's
  *    own layouts: Aabb 32 B, Node 32 B, Link 16 B, Hit 20 B, ray 32 B
- *    {o.xyz, tmax, d.xyz, pad}, triangle 128 B, camera 208 B;
+ *    {o.xyz, tmax, d.xyz, pad}; the tracer takes a 128 B triangle and a 208 B
+ *    camera (the fields the Rendered path reads), made from phiray's own
+ *    760 B `Tri` and 5,776 B `Camera` by phr_tri_from_phiray and
+ *    phr_camera_from_phiray;
  *  - one HIP stream per phr_gpu; calls on one phr_gpu are serialised by the
  *    caller (phiray's `Tracer` is `&mut` from one render thread).
  */
@@ -196,6 +199,26 @@ uint64_t phr_sort_bytes(const phr_sort* s);
 /* GpuSort::order / read (src/bvh/sort.rs:99-111), blocking; either may be null. */
 int phr_sort_read(phr_sort* s, uint32_t* order, uint32_t* sorted);
 void phr_sort_free(phr_sort* s);
+
+/* ---- phiray's own records ------------------------------------------------------ */
+/* phiray's `Tri` (src/trace/kernel.rs:11-65, 760 bytes a record) into the
+ * tracer's 128-byte triangle: positions, normals, colour, roughness,
+ * metallic, transmission, IOR, emission and bump copied; PHR_ERR_UNSUPPORTED
+ * (and nothing more written) for a record with a texture, normal map,
+ * texture-driven input, subsurface, sheen, coat, a second lobe or a
+ * procedural, which the restated path does not shade. Host only. What
+ * Tracer::new (src/trace/build.rs:11-13) is handed goes through this first. */
+#define PHR_PHIRAY_TRI_BYTES 760
+#define PHR_PHIRAY_CAMERA_BYTES 5776
+int phr_tri_from_phiray(const void* tris, uint64_t n, void* out);
+/* phiray's `Camera` (src/trace/camera.rs:12-228, 5,776 bytes) into the
+ * tracer's 208-byte camera, roulette and sample streams as the reference's
+ * (rr_from 0, stream 0); PHR_ERR_UNSUPPORTED for a camera whose frame the
+ * restated path would not draw: a non-black theme sky, the grid overlay, a
+ * transparent film or shown world, an environment image, depth of field, a
+ * tile of a larger frame, TAA jitter, light clamping or a lit lamp. Host
+ * only. Tracer::sample's camera (src/trace/sample.rs:96-99) goes through it. */
+int phr_camera_from_phiray(const void* camera, void* out);
 
 /* ---- tracer: Tracer (src/trace/mod.rs:50-245) -------------------------------- */
 /* Tracer::new (src/trace/build.rs:11-13) over n triangle records (128 B each,

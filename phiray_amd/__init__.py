@@ -41,6 +41,83 @@ CAMERA = np.dtype([
 assert AABB.itemsize == 32 and NODE.itemsize == 32 and HIT.itemsize == 20
 assert RAY.itemsize == 32 and TRI.itemsize == 128 and CAMERA.itemsize == 208
 
+# phiray's own records, byte for byte: `Tri` (src/trace/kernel.rs:11-65, with
+# three 44-word procedural slots, src/trace/proc.rs:41-254), `Light`
+# (src/trace/light.rs:14-43) and `Camera` (src/trace/camera.rs:12-228, 64
+# lamps). phr_tri_from_phiray / phr_camera_from_phiray turn them into TRI and
+# CAMERA above.
+NO_TEX = 0xFFFFFFFF  # src/trace/kernel.rs:70
+NO_MAT = 0xFFFFFFFF  # src/trace/mat.rs:49
+MAX_LIGHT = 64  # src/trace/camera.rs:332
+PROC = np.dtype([("kind", "<u4"), ("rest", "<u4", 43)])
+PHIRAY_TRI = np.dtype([
+    ("a", "<f4", 3), ("rough", "<f4"), ("na", "<f4", 3), ("trans", "<f4"),
+    ("b", "<f4", 3), ("metal", "<f4"), ("nb", "<f4", 3), ("ior", "<f4"),
+    ("c", "<f4", 3), ("_c", "<f4"), ("nc", "<f4", 3), ("_nc", "<f4"),
+    ("color", "<f4", 4), ("us", "<f4", 3), ("tex", "<u4"), ("vs", "<f4", 3), ("nmap", "<u4"),
+    ("emit", "<f4", 3), ("bump", "<f4"), ("maps", "<u4", 3), ("sss", "<f4"), ("sss_radius", "<f4", 3),
+    ("sheen", "<f4"), ("sheen_rough", "<f4"), ("sheen_tint", "<f4", 3), ("coat", "<f4"), ("coat_rough", "<f4"),
+    ("aniso", "<f4"), ("aniso_rot", "<f4"), ("second", "<u4"), ("mix", "<f4"),
+    ("color_proc", PROC), ("bump_proc", PROC), ("color_proc2", PROC),
+])
+PHIRAY_LIGHT = np.dtype([
+    ("pos", "<f4", 3), ("kind", "<u4"), ("color", "<f4", 3), ("size", "<f4"), ("dir", "<f4", 3), ("cone", "<f4"),
+    ("right", "<f4", 3), ("size_y", "<f4"), ("shape", "<u4"), ("blend", "<f4"), ("spot_scale", "<f4", 2),
+])
+PHIRAY_CAMERA = np.dtype([
+    ("n", "<u4"), ("groups_x", "<u4"), ("w", "<u4"), ("h", "<u4"),
+    ("right", "<f4", 3), ("persp", "<u4"), ("up", "<f4", 3), ("scale", "<f4"),
+    ("back", "<f4", 3), ("units", "<f4"), ("pivot", "<f4", 3), ("aspect", "<f4"),
+    ("sun", "<f4", 3), ("dist", "<f4"), ("sky", "<f4", 4), ("pan", "<f4", 2), ("ao", "<f4"), ("eps", "<f4"),
+    ("frame", "<u4"), ("grid", "<f4"), ("count", "<u4"), ("glow", "<u4"), ("grid_color", "<f4", 4),
+    ("env", "<f4", 4), ("split", "<u4", 4), ("view", "<f4", 4),
+    ("envmap", "<u4"), ("env_rot", "<f4"), ("bounces", "<u4"), ("env_cdf", "<u4"),
+    ("aperture", "<f4"), ("focus", "<f4"), ("blades", "<u4"), ("blade_spin", "<f4"),
+    ("aperture_ratio", "<f4"), ("env_hidden", "<u4"), ("see_through", "<f4"), ("grid_hide", "<u4"),
+    ("tile", "<u4", 4), ("film", "<u4", 4), ("preview", "<u4", 4), ("slice", "<u4", 4), ("insts", "<u4", 4),
+    ("vis", "<u4", 4), ("taa", "<f4", 4), ("fx", "<f4", 4), ("outline", "<f4", 4), ("shade", "<u4", 4),
+    ("single", "<f4", 4), ("cavity", "<f4", 4), ("ssao", "<u4", 4), ("shadow", "<f4", 4),
+    ("shadow_fade", "<f4", 4), ("matcap", "<u4", 4), ("dof", "<f4", 4), ("outline_fx", "<f4", 4),
+    ("outline_pick", "<f4", 4), ("outline_active", "<f4", 4), ("outline_move", "<f4", 4),
+    ("adapt", "<f4", 4), ("limit", "<f4", 4), ("depth", "<u4", 4), ("integrator", "<f4", 4),
+    ("lights", PHIRAY_LIGHT, MAX_LIGHT),
+])
+assert PROC.itemsize == 176 and PHIRAY_TRI.itemsize == 760
+assert PHIRAY_LIGHT.itemsize == 80 and PHIRAY_CAMERA.itemsize == 5776
+
+
+def phiray_tris(n: int) -> np.ndarray:
+    """n zeroed phiray `Tri` records with `Tri::new`'s defaults
+    (src/trace/kernel.rs:72-74): rough 0.5, IOR 1.5, no textures, sheen
+    roughness 0.5 and white tint, coat roughness 0.03, the default
+    subsurface radius, no second material."""
+    t = np.zeros(n, dtype=PHIRAY_TRI)
+    t["rough"], t["ior"] = 0.5, 1.5
+    t["tex"], t["nmap"], t["maps"] = NO_TEX, NO_TEX, NO_TEX
+    t["sheen_rough"], t["sheen_tint"], t["coat_rough"] = 0.5, 1.0, 0.03
+    t["sss_radius"] = (0.05, 0.01, 0.005)
+    t["second"] = NO_MAT
+    return t
+
+
+def tris_from_phiray(tris: np.ndarray) -> np.ndarray:
+    """phiray `Tri` records (PHIRAY_TRI) into the tracer's TRI records
+    through phr_tri_from_phiray; PhrError for a record outside the subset."""
+    src = np.ascontiguousarray(tris, dtype=PHIRAY_TRI)
+    out = np.zeros(len(src), dtype=TRI)
+    _check(lib().phr_tri_from_phiray(_ptr(src), len(src), _ptr(out)), "phr_tri_from_phiray")
+    return out
+
+
+def camera_from_phiray(cam: np.ndarray) -> np.ndarray:
+    """A phiray `Camera` record (PHIRAY_CAMERA) into the tracer's CAMERA
+    through phr_camera_from_phiray; PhrError for a frame outside the subset."""
+    src = np.ascontiguousarray(np.asarray(cam, dtype=PHIRAY_CAMERA).reshape(1))
+    out = np.zeros(1, dtype=CAMERA)
+    _check(lib().phr_camera_from_phiray(_ptr(src), _ptr(out)), "phr_camera_from_phiray")
+    return out
+
+
 # A scene of meshes under instances (src/trace/scene.rs:40-67, mesh.rs:12-22,
 # instance.rs:23-44): a vertex is its object-space position (w 1) then its
 # packed normal and uv; a face three vertex ids and its material; a material
@@ -233,6 +310,8 @@ _SIGS = {
     "phr_batch_replay": (_I, [_P, C.POINTER(C.c_double)]),
     "phr_batch_px": (_I, [_P, _U32, _U64, _U64, _P]),
     "phr_batch_counters": (_I, [_P, C.POINTER(_U64)]),
+    "phr_tri_from_phiray": (_I, [_P, _U64, _P]),
+    "phr_camera_from_phiray": (_I, [_P, _P]),
 }
 
 _lib = None

@@ -13,6 +13,7 @@
 // with the per-pixel PCG stream seeded as the WGSL seeds it, and adds each
 // sample into the image buffer's sums in order. The restated feature subset
 // and every operation order are those of the CPU twin.
+#include "phr_records.h"
 #include "phr_runtime.h"
 
 #include <algorithm>
@@ -27,36 +28,6 @@
 
 using namespace phr;
 
-// Camera subset (src/trace/camera.rs:12-228): the fields the Rendered path reads.
-struct PhrCamera {
-    float right[3]; uint32_t persp;
-    float up[3]; float scale;
-    float back[3]; float units;
-    float pivot[3]; float aspect;
-    float sun[3]; float dist;
-    float pan[2]; float ao; float eps;
-    float env[4];
-    uint32_t frame, count, glow, bounces;
-    uint32_t depth[4];
-    float integrator[4];
-    uint32_t w, h, rr_from, stream;  // stream: added to the sample index in the seed (0: the reference's seeds)
-    float adapt[4];  // Adaptive Sampling: threshold, min samples, step (a power of two), on (camera.rs `adapt`)
-    float view[4];   // colour management: transform (0 Standard), exposure in stops, gamma (0: 1), look (camera.rs:53-56)
-};
-static_assert(sizeof(PhrCamera) == 208, "camera layout");
-
-// Input triangle: Tri's rows a..nc, color and emit (src/trace/kernel.rs:11-40).
-struct PhrTri {
-    float a[3]; float rough;
-    float na[3]; float trans;
-    float b[3]; float metal;
-    float nb[3]; float ior;
-    float c[3]; float _c;
-    float nc[3]; float _nc;
-    float color[4];
-    float emit[3]; float bump;
-};
-static_assert(sizeof(PhrTri) == 128, "tri layout");
 
 constexpr uint32_t MAT_WORDS = 16;
 constexpr uint32_t PX_LEN = 34;  // src/trace/read.rs:56
