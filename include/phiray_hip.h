@@ -179,6 +179,21 @@ phr_array* phr_bvh_nodes(phr_bvh* b);
 phr_array* phr_bvh_order(phr_bvh* b);
 phr_array* phr_bvh_prims(phr_bvh* b);
 void phr_bvh_free(phr_bvh* b);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): phr_bvh_build_on's tree over n Aabb (32 B) at the device
+ * pointer `prims`, written to caller-owned device memory — `nodes` (2n - 1
+ * Node, 32 B; only reachable nodes are written), `order` (n u32) and, when
+ * not null, `depth` (one u32, device memory). Enqueued on the device's
+ * stream; nothing is allocated, freed or waited on. The build's scratch is
+ * one static device array of PHR_STATIC_BUILD_SCRATCH_BYTES per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race. n above PHR_STATIC_BUILD_MAX is refused with
+ * PHR_ERR_SIZE. */
+#define PHR_STATIC_BUILD_MAX 262144u
+#define PHR_STATIC_BUILD_SCRATCH_BYTES 22570240ull
+int phr_bvh_build_static(phr_gpu* g, const void* prims, uint32_t n, uint32_t leaf, void* nodes, uint32_t* order,
+                         uint32_t* depth);
 
 /* ---- sort: GpuSort (src/bvh/sort.rs:13-111) ---------------------------------- */
 /* GpuSort::record (src/bvh/sort.rs:30-51): a sort of n u32 codes held in a

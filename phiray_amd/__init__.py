@@ -134,6 +134,8 @@ INSTANCE = np.dtype([("world", "<f4", 16), ("mesh", "<u4"), ("casts_shadow", "<u
 INSTANCE_LEN = 40  # words of an instance record (src/trace/instance.rs:18)
 INST = 0x40000000  # a top-tree leaf (src/trace/forest.rs:17)
 LEAF = 0x80000000
+# phr_bvh_build_static: the most boxes its static scratch holds.
+STATIC_BUILD_MAX = 262144
 NONE = 0xFFFFFFFF
 STACK = 32
 PX_LEN = 34  # words a pixel in the image buffer (src/trace/read.rs:56)
@@ -186,6 +188,7 @@ _SIGS = {
     "phr_pass_compiled": (_U32, [_P]),
     "phr_bvh_build": (_I, [_P, _P, _U32, _U32, C.POINTER(_P)]),
     "phr_bvh_build_on": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "phr_bvh_build_static": (_I, [_P, _P, _U32, _U32, _P, _P, _P]),
     "phr_bvh_rebuild_async": (_I, [_P]),
     "phr_bvh_rebuild": (_I, [_P]),
     "phr_bvh_refit_async": (_I, [_P]),
@@ -655,6 +658,23 @@ class GpuBvh:
         b = cls(gpu, out.value)
         b._prims = prims  # keep the borrowed array alive
         return b
+
+    @staticmethod
+    def build_static(gpu: Gpu, prims: GpuArray, nodes: GpuArray, order: GpuArray, depth: "GpuArray | None" = None,
+                     leaf: int = 4, n: int | None = None) -> None:
+        """The build of `build_on` into caller-owned device arrays, with no
+        allocation: scratch is the library's static device array (at most
+        STATIC_BUILD_MAX boxes; calls on one GPU share it). Enqueued."""
+        n = len(prims) if n is None else n
+        cap = lambda a: int(lib().phr_array_cap(a._h))  # noqa: E731
+        if prims.dtype.itemsize != 32 or nodes.dtype.itemsize != 32 or order.dtype.itemsize != 4 or (
+                depth is not None and depth.dtype.itemsize != 4):
+            raise ValueError("build_static: 32-byte boxes and nodes, u32 order and depth")
+        if n and (n > cap(prims) or 2 * n - 1 > cap(nodes) or n > cap(order) or (depth is not None and cap(depth) < 1)):
+            raise ValueError("build_static: an array is smaller than the tree")
+        dptr = lib().phr_array_device_ptr
+        _check(lib().phr_bvh_build_static(gpu._h, dptr(prims._h), n, leaf, dptr(nodes._h), dptr(order._h),
+                                          dptr(depth._h) if depth is not None else None), "GpuBvh::build_static")
 
     @classmethod
     def build_batch(cls, gpu: Gpu, prims: np.ndarray, seg_lo, leaf: int = 4) -> "GpuBvh":

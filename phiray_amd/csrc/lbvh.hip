@@ -83,6 +83,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -1323,6 +1324,181 @@ int phr_bvh_download(phr_bvh* b, void* nodes, uint32_t* order, uint32_t* depth) 
     PHR_CHECK(b, PHR_ERR_ARG, "phr_bvh_download: null");
     PHR_CHECK(b->segs == 1, PHR_ERR_ARG, "phr_bvh_download: a batch downloads one segment at a time");
     return phr_bvh_download_segment(b, 0, nodes, order, depth);
+}
+
+}  // extern "C"
+
+// ---- a build that allocates nothing -----------------------------------------
+// The scratch a one-tree build of at most PHR_STATIC_BUILD_MAX boxes needs,
+// carved from one file-scope device array: every GPU the library runs on holds
+// its own copy until the process ends. Calls on one GPU share it, so two such
+// builds in flight on one device race (the library's stream serialises the
+// calls it issues itself).
+namespace {
+
+constexpr uint64_t ST_ALIGN = 256;
+constexpr uint64_t st_round(uint64_t b) { return (b + ST_ALIGN - 1) & ~(ST_ALIGN - 1); }
+constexpr uint32_t ST_N = PHR_STATIC_BUILD_MAX;
+constexpr uint32_t ST_TILES = (ST_N + OS_TILE - 1) / OS_TILE;
+constexpr uint32_t ST_STATUS_AT = (Z_DEPTH + 1 + 63) & ~63u;
+constexpr uint32_t ST_SCAN_STATUS = (ST_STATUS_AT + CODE_PASSES * ST_TILES * 256 + 1) & ~1u;
+constexpr uint32_t ST_ZERO_WORDS = ST_SCAN_STATUS + 2 * ((ST_N + SC_TILE - 1) / SC_TILE) + 2;
+constexpr uint32_t ST_LEVEL0 = (ST_N + RF_SLOTS - 1) / RF_SLOTS;
+// Offsets of each buffer (bytes), as bvh_alloc sizes them for one segment.
+constexpr uint64_t ST_SEG_LO = 0;
+constexpr uint64_t ST_NODE_BASE = ST_SEG_LO + st_round(8);
+constexpr uint64_t ST_CODES = ST_NODE_BASE + st_round(4);
+constexpr uint64_t ST_KEYS_B = ST_CODES + st_round(4ull * ST_N);
+constexpr uint64_t ST_VALS_B = ST_KEYS_B + st_round(4ull * ST_N);
+constexpr uint64_t ST_ZERO = ST_VALS_B + st_round(4ull * ST_N);
+constexpr uint64_t ST_ACC = ST_ZERO + st_round(4ull * ST_ZERO_WORDS);
+constexpr uint64_t ST_PARTS = ST_ACC + st_round(24);
+constexpr uint64_t ST_LINKS = ST_PARTS + st_round(32ull * BOUNDS_BLOCKS_MAX);
+constexpr uint64_t ST_UP = ST_LINKS + st_round(16ull * ST_N);
+constexpr uint64_t ST_CNT = ST_UP + st_round(32ull * ST_N);
+constexpr uint64_t ST_LEAF_AT = ST_CNT + st_round(4ull * ST_N);
+constexpr uint64_t ST_OWN_END = ST_LEAF_AT + st_round(4ull * ST_N);
+constexpr uint64_t ST_FAR_ENDS = ST_OWN_END + st_round(4ull * ST_N);
+constexpr uint64_t ST_FRONT0 = ST_FAR_ENDS + st_round(4ull * ST_N);
+constexpr uint64_t ST_FRONT1 = ST_FRONT0 + st_round(4ull * ST_N);
+constexpr uint64_t ST_FCOUNT0 = ST_FRONT1 + st_round(4ull * ST_N);
+constexpr uint64_t ST_FCOUNT1 = ST_FCOUNT0 + st_round(4ull * ST_LEVEL0);
+constexpr uint64_t ST_BYTES = ST_FCOUNT1 + st_round(4ull * ST_LEVEL0);
+static_assert(ST_BYTES == PHR_STATIC_BUILD_SCRATCH_BYTES, "the header's scratch figure follows the carve");
+
+}  // namespace
+
+// The scratch itself. In host code this name is a stand-in, not a device
+// address: the build takes each device's address of it from
+// hipGetSymbolAddress, once per device, and passes only that to kernels.
+static __device__ __align__(256) uint8_t scratch[ST_BYTES];
+
+namespace {
+
+constexpr int ST_MAX_DEVICES = 64;
+std::mutex g_st_mutex;
+uint8_t* g_st_ptr[ST_MAX_DEVICES] = {};
+
+// This device's address of `scratch`, looked up on its first use with the
+// device current.
+int static_scratch(phr_gpu* g, uint8_t** out) {
+    PHR_CHECK(g->device >= 0 && g->device < ST_MAX_DEVICES, PHR_ERR_UNSUPPORTED, "phr_bvh_build_static: device index too large");
+    std::lock_guard<std::mutex> lock(g_st_mutex);
+    if (!g_st_ptr[g->device]) {
+        PHR_HIP(hipSetDevice(g->device));
+        void* p = nullptr;
+        PHR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PHR_CHECK(p, PHR_ERR_HIP, "phr_bvh_build_static: no device address for the scratch array");
+        g_st_ptr[g->device] = (uint8_t*)p;
+    }
+    *out = g_st_ptr[g->device];
+    return PHR_OK;
+}
+
+// What must hold before a build starts: the segment table of one tree, the
+// scene-box words at Aabb::default(), the arrival counters of the wide refit
+// levels at zero. A kernel, not a memset, so the call can be captured.
+__global__ __launch_bounds__(256) void k_static_init(uint32_t n, uint32_t* __restrict__ seg_lo, uint32_t* __restrict__ node_base,
+                                                     uint32_t* __restrict__ acc, uint32_t* __restrict__ cnt) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gsize = gridDim.x * blockDim.x;
+    if (gtid == 0) {
+        seg_lo[0] = 0u;
+        seg_lo[1] = n;
+        node_base[0] = 0u;
+        for (int j = 0; j < 6; ++j) acc[j] = j < 3 ? KEY_MAX : KEY_MIN;
+    }
+    for (uint32_t i = gtid; i < n; i += gsize) cnt[i] = 0u;
+}
+
+// The depth the packing launch left, or 0 for an empty tree (no zero block).
+__global__ void k_static_depth(const uint32_t* __restrict__ zero, uint32_t* __restrict__ depth) {
+    if (threadIdx.x == 0) depth[0] = zero ? zero[Z_DEPTH] : 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phr_bvh_build_static(phr_gpu* g, const void* prims, uint32_t n, uint32_t leaf, void* nodes, uint32_t* order,
+                         uint32_t* depth) {
+    PHR_CHECK(g && (n == 0 || (prims && nodes && order)), PHR_ERR_ARG, "phr_bvh_build_static: null argument");
+    PHR_CHECK(n <= PHR_STATIC_BUILD_MAX, PHR_ERR_SIZE,
+              "phr_bvh_build_static: more boxes than the static scratch holds (PHR_STATIC_BUILD_MAX = 262,144)");
+    PHR_CHECK(leaf >= 1, PHR_ERR_ARG, "phr_bvh_build_static: leaf size must be at least 1");
+    uint8_t* base = nullptr;
+    int rc = static_scratch(g, &base);
+    if (rc != PHR_OK) return rc;
+    hipStream_t s = g->stream;
+    if (n == 0) {
+        // An empty tree: nothing to build, depth 0.
+        if (depth) hipLaunchKernelGGL(k_static_depth, dim3(1), dim3(64), 0, s, (const uint32_t*)nullptr, depth);
+        PHR_HIP(hipGetLastError());
+        return PHR_OK;
+    }
+    // The build's state over caller memory and the scratch; nothing here
+    // reaches the allocator (the vectors stay empty).
+    phr_array a_prims, a_order, a_nodes;
+    a_prims.gpu = a_order.gpu = a_nodes.gpu = g;
+    a_prims.ptr = const_cast<void*>(prims);
+    a_prims.elem = 32;
+    a_prims.len = a_prims.cap = n;
+    a_order.ptr = order;
+    a_order.elem = 4;
+    a_order.len = a_order.cap = n;
+    a_nodes.ptr = nodes;
+    a_nodes.elem = 32;
+    a_nodes.len = a_nodes.cap = 2ull * n - 1;
+    phr_bvh b;
+    b.gpu = g;
+    b.n = n;
+    b.leaf = leaf;
+    b.segs = 1;
+    b.nonempty = 1;
+    b.seg_passes = 0;
+    b.tiles = (n + OS_TILE - 1) / OS_TILE;
+    b.bounds_blocks = std::min<uint32_t>(BOUNDS_BLOCKS_MAX, std::max<uint32_t>(1, (n + 2047) / 2048));
+    b.status_at = ST_STATUS_AT;
+    b.scan_status = (b.status_at + CODE_PASSES * b.tiles * 256 + 1) & ~1u;
+    b.zero_words = b.scan_status + 2 * ((n + SC_TILE - 1) / SC_TILE) + 2;
+    b.prims = &a_prims;
+    b.order = &a_order;
+    b.nodes = &a_nodes;
+    b.own_prims = false;
+    b.own_nodes = false;
+    b.seg_lo = (uint32_t*)(base + ST_SEG_LO);
+    b.node_base = (uint32_t*)(base + ST_NODE_BASE);
+    b.codes = (uint32_t*)(base + ST_CODES);
+    b.keys_b = (uint32_t*)(base + ST_KEYS_B);
+    b.vals_b = (uint32_t*)(base + ST_VALS_B);
+    b.zero = (uint32_t*)(base + ST_ZERO);
+    b.acc = (uint32_t*)(base + ST_ACC);
+    b.parts = base + ST_PARTS;
+    b.links = (PhrLink*)(base + ST_LINKS);
+    b.up = (uint4*)(base + ST_UP);
+    b.cnt = (uint32_t*)(base + ST_CNT);
+    b.leaf_at = (uint32_t*)(base + ST_LEAF_AT);
+    b.own_end = (uint32_t*)(base + ST_OWN_END);
+    b.far_ends = (uint32_t*)(base + ST_FAR_ENDS);
+    b.front[0] = (uint32_t*)(base + ST_FRONT0);
+    b.front[1] = (uint32_t*)(base + ST_FRONT1);
+    b.fcount[0] = (uint32_t*)(base + ST_FCOUNT0);
+    b.fcount[1] = (uint32_t*)(base + ST_FCOUNT1);
+    b.level0_blocks = (n + RF_SLOTS - 1) / RF_SLOTS;
+    hipLaunchKernelGGL(k_static_init, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, s, n, b.seg_lo, b.node_base,
+                       b.acc, b.cnt);
+    PHR_HIP(hipGetLastError());
+    rc = bvh_enqueue_build(&b);
+    // b's members point at the caller's memory and the scratch: nothing to free.
+    b.prims = b.order = b.nodes = nullptr;
+    b.seg_lo = b.node_base = b.codes = b.keys_b = b.vals_b = b.zero = b.acc = b.cnt = b.leaf_at = b.own_end = b.far_ends = nullptr;
+    b.parts = nullptr;
+    b.links = nullptr;
+    b.up = nullptr;
+    b.front[0] = b.front[1] = b.fcount[0] = b.fcount[1] = nullptr;
+    if (rc != PHR_OK) return rc;
+    if (depth) hipLaunchKernelGGL(k_static_depth, dim3(1), dim3(64), 0, s, (const uint32_t*)(base + ST_ZERO), depth);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
 }
 
 int phr_sort_record(phr_gpu* g, phr_array* codes, uint32_t n, phr_sort** out);
