@@ -136,6 +136,14 @@ uint32_t phr_pass_compiled(const phr_pass* p);
 int phr_bvh_build(phr_gpu* g, const void* prims, uint32_t n, uint32_t leaf, phr_bvh** out);
 /* The same over an Aabb array already on the device (borrowed, not owned). */
 int phr_bvh_build_on(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out);
+/* No counterpart in the reference (the tuner the gfx950 build adds): keys a
+ * thread holds in a sort pass, 8 (2,048-key tiles, the default) or 16
+ * (4,096); every choice gives the same tree. phr_bvh_autotune times `reps`
+ * rebuilds of this tree with each, keeps the faster, and reports it and the
+ * two times (ms[2], may be null). */
+int phr_bvh_set_sort_tile(phr_bvh* b, uint32_t items);
+uint32_t phr_bvh_sort_tile(const phr_bvh* b);
+int phr_bvh_autotune(phr_bvh* b, uint32_t reps, uint32_t* items, double* ms);
 /* Build again from the current prims (same count), enqueued / then synced. */
 int phr_bvh_rebuild_async(phr_bvh* b);
 int phr_bvh_rebuild(phr_bvh* b);
@@ -393,6 +401,11 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t);
  * frames take the other build. The autotune tries both. */
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on);
 uint32_t phr_tracer_path_regen(const phr_tracer* t);
+/* No counterpart in the reference: the mesh-tree stack entries a thread of
+ * the path launches holds (24 or 32; the autotune picks the 24-entry stack
+ * at 3 waves too when every mesh tree of a forest is at most 24 deep). */
+int phr_tracer_set_path_stack(phr_tracer* t, uint32_t entries);
+uint32_t phr_tracer_path_stack(const phr_tracer* t);
 /* Autotune (no counterpart in phiray): time each variant on `spp` samples of
  * w x h under `cam` (best of `reps`), keep the fastest; the image starts over. */
 int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps,
@@ -472,6 +485,18 @@ int phr_kpcn_train_step(phr_kpcn* k, phr_array* frames, phr_array* targets, uint
 /* Autotune (no counterpart): each layer's forward conv tile, timed on b frames
  * of w x h; chosen gets the rows a wave owns per layer (phr_kpcn_layers order). */
 int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, uint32_t reps, uint8_t* chosen);
+/* No counterpart in the reference: the learned frame's held-out check. For
+ * two independent halves A and B of a frame (FRAME-layout arrays, n pixels)
+ * and `denoised` (3n floats) made from A alone: *err estimates the relMSE of
+ * `denoised` against the converged image, mean(((D - B)^2 - (A - B)^2 / 2) /
+ * (D^2 + 0.01)); *noise (may be null) is B's own term, mean((A - B)^2 / 2 /
+ * (D^2 + 0.01)). Blocking. */
+int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_array* frame_b, uint64_t n, double* err,
+                      double* noise);
+/* No counterpart in the reference: the frame of both halves (equal sample
+ * counts) as one: each channel's mean, depth over the halves that met
+ * something; channels 10 (a frame) or 3 (the radiance alone). Enqueued. */
+int phr_kpcn_frame_mean(phr_gpu* g, phr_array* frame_a, phr_array* frame_b, uint64_t n, uint32_t channels, phr_array* out);
 /* One training step over b images of w x h gathered from resident data sets:
  * frame image fidx[j] of `frames` against target image tidx[j] of `targets`.
  * PHR_ERR_ARG for an index past either set. */

@@ -232,6 +232,39 @@ void orc_denoise(uint32_t* px, uint32_t w, uint32_t h, uint32_t keep, const floa
 
 int orc_hardware_threads(void);
 
+/* ---- the rigid-body World (src/sim/), sim.cpp ---------------------------- */
+/* Shape (src/sim/state.rs, 32 B): kind 0 box, 1 sphere, 2 capsule; half
+ * extents (box) or half length on z (capsule); radius. */
+typedef struct { uint32_t kind, data, count; float radius; float half[3]; float _a; } orc_shape;
+/* BodyDef (src/sim/build.rs:14-61): kind 0 active, 1 passive, 2 kinematic;
+ * q is xyzw. */
+typedef struct {
+    uint32_t kind, shape; float mass;
+    float x[3]; float q[4]; float v[3]; float w[3];
+    float scale[3]; float com[3];
+    float friction, restitution, margin, lin_damp, ang_damp;
+    float sleep[2]; uint32_t has_sleep, asleep, groups, obj;
+} orc_body_def;
+typedef struct orc_world orc_world;
+orc_world* orc_world_new(void);
+void orc_world_free(orc_world* w);
+void orc_world_configure(orc_world* w, uint32_t substeps, uint32_t iterations, float time_scale, float max_recover,
+                         float sleep_after, const float gravity[3]);
+int orc_world_add_shape(orc_world* w, const orc_shape* s, uint32_t* id);
+int orc_world_add_body(orc_world* w, const orc_body_def* d, uint32_t* id);
+/* World::step (src/sim/world.rs:104-129). */
+int orc_world_step(orc_world* w, float frame_dt);
+uint32_t orc_world_body_count(const orc_world* w);
+uint32_t orc_world_contact_count(const orc_world* w);
+/* The 64-byte Body records (src/sim/state.rs). */
+int orc_world_bodies(const orc_world* w, void* out);
+/* World::poses (src/sim/world.rs:225-228): 16 floats a body, m[col][row]. */
+int orc_world_poses(const orc_world* w, float* out);
+/* World::energy (src/sim/world.rs:231-243). */
+float orc_world_energy(const orc_world* w);
+/* The last broadphase's pairs (i < j, sorted). */
+int orc_world_pair_list(const orc_world* w, uint32_t* out, uint32_t cap, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,18 @@ _SIGS = {
     "orc_forest_build": (_U32, [_P, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _I]),
     "orc_forest_reinstance": (None, [_P, _P, _P, _P, _P, _P, _P]),
     "orc_trace_scene": (None, [_P, _P, _U32, _I, _P, _P, _I]),
+    "orc_world_new": (_P, []),
+    "orc_world_free": (None, [_P]),
+    "orc_world_configure": (None, [_P, _U32, _U32, C.c_float, C.c_float, C.c_float, _P]),
+    "orc_world_add_shape": (_I, [_P, _P, C.POINTER(_U32)]),
+    "orc_world_add_body": (_I, [_P, _P, C.POINTER(_U32)]),
+    "orc_world_step": (_I, [_P, C.c_float]),
+    "orc_world_body_count": (_U32, [_P]),
+    "orc_world_contact_count": (_U32, [_P]),
+    "orc_world_bodies": (_I, [_P, _P]),
+    "orc_world_poses": (_I, [_P, _P]),
+    "orc_world_energy": (C.c_float, [_P]),
+    "orc_world_pair_list": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
 }
 
 _lib = None
@@ -322,3 +334,104 @@ class Scene:
         c = self.stamp(cam, w, h, frame)
         lib().orc_adapt_check(ptr(c), ptr(px))
         lib().orc_adapt_dilate(ptr(c), ptr(px))
+
+
+# ---- the rigid-body World (src/sim/), sim.cpp ----------------------------------
+SHAPE = np.dtype([("kind", "<u4"), ("data", "<u4"), ("count", "<u4"), ("radius", "<f4"), ("half", "<f4", 3), ("_a", "<f4")])
+BODY = np.dtype([("x", "<f4", 3), ("inv_mass", "<f4"), ("q", "<f4", 4), ("v", "<f4", 3), ("shape", "<u4"),
+                 ("w", "<f4", 3), ("flags", "<u4")])
+BODY_DEF = np.dtype([("kind", "<u4"), ("shape", "<u4"), ("mass", "<f4"), ("x", "<f4", 3), ("q", "<f4", 4), ("v", "<f4", 3),
+                     ("w", "<f4", 3), ("scale", "<f4", 3), ("com", "<f4", 3), ("friction", "<f4"), ("restitution", "<f4"),
+                     ("margin", "<f4"), ("lin_damp", "<f4"), ("ang_damp", "<f4"), ("sleep", "<f4", 2), ("has_sleep", "<u4"),
+                     ("asleep", "<u4"), ("groups", "<u4"), ("obj", "<u4")])
+PASSIVE = 1
+
+
+class World:
+    """phiray's XPBD `World` (src/sim/world.rs) restated on the CPU, with the
+    same methods as the product's World."""
+
+    BOX, SPHERE, CAPSULE = 0, 1, 2
+
+    def __init__(self):
+        self._h = C.c_void_p(lib().orc_world_new())
+
+    def configure(self, substeps: int = 10, iterations: int = 10, time_scale: float = 1.0, max_recover: float = 2.0,
+                  sleep_after: float = 2.0, gravity=(0.0, 0.0, -9.81)) -> None:
+        g = np.array(gravity, dtype=np.float32)
+        lib().orc_world_configure(self._h, substeps, iterations, time_scale, max_recover, sleep_after, ptr(g))
+
+    def add_shape(self, kind: int, half=(0.0, 0.0, 0.0), radius: float = 0.0) -> int:
+        s = np.zeros(1, dtype=SHAPE)
+        s["kind"], s["half"], s["radius"] = kind, half, radius
+        out = C.c_uint32(0)
+        if lib().orc_world_add_shape(self._h, ptr(s), C.byref(out)) != 0:
+            raise ValueError("orc_world_add_shape refused the shape")
+        return int(out.value)
+
+    def add_box(self, half) -> int:
+        return self.add_shape(self.BOX, half=half)
+
+    def add_sphere(self, radius: float) -> int:
+        return self.add_shape(self.SPHERE, radius=radius)
+
+    def add_capsule(self, radius: float, half: float) -> int:
+        return self.add_shape(self.CAPSULE, half=(radius, radius, half), radius=radius)
+
+    @staticmethod
+    def body_def(**kw) -> np.ndarray:
+        """A BodyDef with phiray's defaults (src/sim/build.rs:39-61)."""
+        d = np.zeros(1, dtype=BODY_DEF)
+        d["mass"], d["q"], d["scale"] = 1.0, (0.0, 0.0, 0.0, 1.0), (1.0, 1.0, 1.0)
+        d["friction"], d["lin_damp"], d["ang_damp"], d["groups"] = 0.5, 0.04, 0.1, 1
+        for k, v in kw.items():
+            if k == "sleep":
+                if v is not None:
+                    d["sleep"], d["has_sleep"] = v, 1
+                continue
+            d[k] = v
+        return d
+
+    def add_body(self, **kw) -> int:
+        d = self.body_def(**kw)
+        out = C.c_uint32(0)
+        if lib().orc_world_add_body(self._h, ptr(d), C.byref(out)) != 0:
+            raise ValueError("orc_world_add_body refused the body")
+        return int(out.value)
+
+    def step(self, dt: float) -> None:
+        lib().orc_world_step(self._h, dt)
+
+    def __len__(self) -> int:
+        return int(lib().orc_world_body_count(self._h))
+
+    def bodies(self) -> np.ndarray:
+        out = np.zeros(len(self), dtype=BODY)
+        lib().orc_world_bodies(self._h, ptr(out))
+        return out
+
+    def poses(self) -> np.ndarray:
+        out = np.zeros((len(self), 16), dtype=np.float32)
+        lib().orc_world_poses(self._h, ptr(out))
+        return out
+
+    def energy(self) -> float:
+        return float(lib().orc_world_energy(self._h))
+
+    def contact_count(self) -> int:
+        return int(lib().orc_world_contact_count(self._h))
+
+    def pairs(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        lib().orc_world_pair_list(self._h, None, 0, C.byref(n))
+        out = np.zeros((n.value, 2), dtype=np.uint32)
+        lib().orc_world_pair_list(self._h, ptr(out), n.value, C.byref(n))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_world_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass

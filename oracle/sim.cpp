@@ -1,25 +1,36 @@
-// phiray's rigid-body driver for the instanced workload: XPBD with substeps
-// (src/sim/world.rs, contact.rs, broad.rs, shape.rs, math.rs, build.rs,
-// narrow/closed.rs, narrow/sat.rs, narrow/manifold.rs), restated in C++ on the
-// host, with the broadphase on the device. The broadphase builds this
-// library's LBVH over the bodies' swept boxes and walks it with one thread a
-// body (k_overlap), the kernel phiray's broad.rs names as its GPU successor.
-// World::poses feeds Tracer::reinstance.
+// The rigid-body World of phiray restated on the CPU: the checker the
+// device solver (phiray_amd/csrc/world.hip) is held to.
+//
+// TEST INFRASTRUCTURE ONLY, like the rest of this folder: only the tests load
+// it, and the product library never links or calls it.
+//
+// It restates phiray's XPBD world (src/sim/world.rs, contact.rs, broad.rs,
+// shape.rs, math.rs, build.rs, narrow/closed.rs, narrow/sat.rs,
+// narrow/manifold.rs) operation by operation: left-to-right sums, no fused
+// multiply-add (built with -ffp-contract=off), ties in min/max selections as
+// Rust's Iterator::min_by (first) and max_by (last) send them. The
+// broadphase is brute force over the swept boxes (broad.rs walks a Bvh over
+// the same boxes; the pair set does not depend on the tree).
 //
 // Shapes: box, sphere and capsule (closed forms, SAT with clipping). Hulls,
 // cylinders, cones and triangle meshes, which phiray takes through GJK/EPA
-// and its mesh midphase, are refused with PHR_ERR_UNSUPPORTED.
-//
-// Arithmetic follows the Rust source operation by operation (left-to-right
-// sums, no fused multiply-add: the library is built with -ffp-contract=off),
-// and ties in min/max selections go as Rust's Iterator::min_by (first) and
-// max_by (last) send them.
-#include "phr_runtime.h"
+// and its mesh midphase, are refused, as is a box-capsule pair.
+#include "oracle.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
+
+#define ORC_OK 0
+#define ORC_ERR_ARG (-1)
+#define ORC_ERR_UNSUPPORTED (-3)
+#define ORC_ERR_SIZE (-4)
+#define ORC_CHECK(cond, code, msg) \
+    do {                           \
+        if (!(cond)) return (code); \
+    } while (0)
 
 namespace {
 
@@ -130,7 +141,7 @@ struct Aabb {
 };
 
 // ---- shapes (src/sim/shape.rs) --------------------------------------------------
-V3 support(const phr_shape& s, V3 d) {
+V3 support(const orc_shape& s, V3 d) {
     switch (s.kind) {
         case S_SPHERE: return scale(norm(d), s.radius);
         case S_BOX: return mk(copysign_(s.half[0], d[0]), copysign_(s.half[1], d[1]), copysign_(s.half[2], d[2]));
@@ -138,9 +149,9 @@ V3 support(const phr_shape& s, V3 d) {
             return add(scale(norm(d), s.radius), mk(0.0f, 0.0f, copysign_(s.half[2], d[2])));
     }
 }
-V3 support_world(const phr_shape& s, V3 x, Q4 q, V3 d) { return add(x, rotate(q, support(s, unrotate(q, d)))); }
+V3 support_world(const orc_shape& s, V3 x, Q4 q, V3 d) { return add(x, rotate(q, support(s, unrotate(q, d)))); }
 
-Aabb shape_bounds(const phr_shape& s, V3 x, Q4 q) {
+Aabb shape_bounds(const orc_shape& s, V3 x, Q4 q) {
     auto axis = [&](int k, float sign) {
         V3 e = mk(0, 0, 0);
         e[k] = sign;
@@ -149,7 +160,7 @@ Aabb shape_bounds(const phr_shape& s, V3 x, Q4 q) {
     return Aabb{mk(axis(0, -1.0f), axis(1, -1.0f), axis(2, -1.0f)), mk(axis(0, 1.0f), axis(1, 1.0f), axis(2, 1.0f))};
 }
 
-V3 inertia(const phr_shape& s, float mass) {
+V3 inertia(const orc_shape& s, float mass) {
     auto boxed = [&](V3 h) {
         float x = 4.0f * h[0] * h[0], y = 4.0f * h[1] * h[1], z = 4.0f * h[2] * h[2];
         return scale(mk(y + z, x + z, x + y), mass / 12.0f);
@@ -165,7 +176,7 @@ V3 inertia(const phr_shape& s, float mass) {
     }
 }
 
-void segment(const phr_shape& s, V3 x, Q4 q, V3& p0, V3& p1) {
+void segment(const orc_shape& s, V3 x, Q4 q, V3& p0, V3& p1) {
     V3 h = rotate(q, mk(0.0f, 0.0f, s.half[2]));
     p0 = sub(x, h);
     p1 = add(x, h);
@@ -499,7 +510,7 @@ namespace {
 constexpr float SLOP = 0.01f;
 
 struct View {
-    const phr_shape* s;
+    const orc_shape* s;
     V3 x;
     Q4 q;
     float margin;
@@ -741,57 +752,21 @@ void velocity_pass(std::vector<Body>& bodies, const std::vector<BodyConst>& cons
     });
 }
 
-// ---- broadphase on the device (src/sim/broad.rs) ---------------------------------------
+// ---- broadphase (src/sim/broad.rs:52-91) -----------------------------------------------
+// Every pair of swept boxes that overlap, by brute force: broad.rs walks a
+// Bvh::build tree over the same boxes, and the pair set does not depend on the
+// tree. Listed as (i, j), i < j, sorted, when either body moves and one is
+// active and their collision groups meet.
 constexpr float PAD = SLOP + 0.005f;
-
-// Body i's flags word for the overlap walk: bit 0 moves, bit 1 active.
-__global__ __launch_bounds__(256) void k_overlap(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
-                                                 const float4* __restrict__ boxes, const uint32_t* __restrict__ info,
-                                                 const uint32_t* __restrict__ groups, uint32_t n, uint2* __restrict__ pairs,
-                                                 uint32_t cap, uint32_t* __restrict__ count) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n || (info[i] & 1u) == 0u) return;
-    const float4 blo = boxes[2 * (size_t)i], bhi = boxes[2 * (size_t)i + 1];
-    auto meets = [&](float4 lo, float4 hi) {
-        return lo.x <= bhi.x && hi.x >= blo.x && lo.y <= bhi.y && hi.y >= blo.y && lo.z <= bhi.z && hi.z >= blo.z;
-    };
-    uint32_t stack[PHR_STACK + 1];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-        const uint32_t k = stack[--sp];
-        const float4 lo = nodes[2 * (size_t)k], hi = nodes[2 * (size_t)k + 1];
-        if (!meets(lo, hi)) continue;
-        const uint32_t a = __float_as_uint(lo.w), b = __float_as_uint(hi.w);
-        if (a & PHR_LEAF) {
-            for (uint32_t s = a & ~PHR_LEAF; s < b; ++s) {
-                const uint32_t j = order[s];
-                if (j == i || !meets(boxes[2 * (size_t)j], boxes[2 * (size_t)j + 1])) continue;
-                if (((info[i] | info[j]) & 2u) == 0u || (groups[i] & groups[j]) == 0u) continue;
-                const uint32_t at = atomicAdd(count, 1u);
-                if (at < cap) pairs[at] = make_uint2(min(i, j), max(i, j));
-            }
-        } else if (sp + 2 <= (int)PHR_STACK + 1) {
-            stack[sp++] = a;
-            stack[sp++] = b;
-        }
-    }
-}
 
 }  // namespace
 
-int phr_bvh_internal_build(phr_gpu* g, phr_array* prims, uint32_t leaf, phr_bvh** out);
-int phr_bvh_internal_rebuild_async(phr_bvh* b);
-phr_array* phr_bvh_internal_nodes(phr_bvh* b);
-phr_array* phr_bvh_internal_order(phr_bvh* b);
-
 // ---- the world (src/sim/world.rs, build.rs) --------------------------------------------
-struct phr_world {
-    phr_gpu* gpu = nullptr;
+struct orc_world {
     std::vector<Body> bodies;
     std::vector<BodyConst> consts;
     std::vector<BodyPrev> prev;
-    std::vector<phr_shape> shapes;
+    std::vector<orc_shape> shapes;
     std::vector<Contact> contacts;
     std::vector<float> vn0;
     std::vector<V3> push, spin;
@@ -801,81 +776,30 @@ struct phr_world {
     V3 gravity = mk(0.0f, 0.0f, -9.81f);
     uint32_t substeps = 10, iterations = 10;
     float time_scale = 1.0f, max_recover = 2.0f, sleep_after = 2.0f;
-    // device broadphase
-    phr_array* boxes = nullptr;   // 2 float4 a body
-    phr_array* info = nullptr;    // flags word a body
-    phr_array* grp = nullptr;     // groups a body
-    phr_array* pairs = nullptr;   // uint2
-    uint32_t* count = nullptr;
-    phr_bvh* tree = nullptr;
-    uint32_t tree_n = 0;
-    uint64_t pairs_found = 0;
     std::vector<std::pair<uint32_t, uint32_t>> last_pairs;
 };
 
 namespace {
 
-int world_pairs(phr_world* w, const std::vector<Aabb>& boxes, std::vector<std::pair<uint32_t, uint32_t>>& out) {
+void world_pairs(orc_world* w, const std::vector<Aabb>& boxes, std::vector<std::pair<uint32_t, uint32_t>>& out) {
     const uint32_t n = (uint32_t)w->bodies.size();
     out.clear();
-    if (n < 2) return PHR_OK;
-    std::vector<float> bx(8 * (size_t)n);
-    std::vector<uint32_t> info(n), grp(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const Aabb& b = boxes[i];
-        float* o = &bx[8 * (size_t)i];
-        o[0] = b.lo[0], o[1] = b.lo[1], o[2] = b.lo[2], o[3] = 0.0f;
-        o[4] = b.hi[0], o[5] = b.hi[1], o[6] = b.hi[2], o[7] = 0.0f;
-        info[i] = (w->bodies[i].moves() ? 1u : 0u) | (w->bodies[i].kind() == K_ACTIVE ? 2u : 0u);
-        grp[i] = w->consts[i].groups;
-    }
-    int grown = 0, rc;
-    if (!w->boxes) {
-        if ((rc = phr_array_alloc(w->gpu, 32, n, &w->boxes)) != PHR_OK || (rc = phr_array_alloc(w->gpu, 4, n, &w->info)) != PHR_OK ||
-            (rc = phr_array_alloc(w->gpu, 4, n, &w->grp)) != PHR_OK || (rc = phr_array_alloc(w->gpu, 8, 4ull * n, &w->pairs)) != PHR_OK)
-            return rc;
-        PHR_HIP(hipMalloc((void**)&w->count, 4));
-    }
-    if ((rc = phr_array_write(w->boxes, bx.data(), n, &grown)) != PHR_OK || (rc = phr_array_write(w->info, info.data(), n, &grown)) != PHR_OK ||
-        (rc = phr_array_write(w->grp, grp.data(), n, &grown)) != PHR_OK)
-        return rc;
-    if (!w->tree || w->tree_n != n) {
-        if (w->tree) phr_bvh_free(w->tree);
-        w->tree = nullptr;
-        if ((rc = phr_bvh_internal_build(w->gpu, w->boxes, 4, &w->tree)) != PHR_OK) return rc;  // Bvh::build(aabbs, 4)
-        w->tree_n = n;
-    } else if ((rc = phr_bvh_internal_rebuild_async(w->tree)) != PHR_OK) {
-        return rc;
-    }
-    hipStream_t s = w->gpu->stream;
-    for (;;) {
-        const uint32_t cap = (uint32_t)w->pairs->cap;
-        uint32_t found = 0;
-        PHR_HIP(hipMemsetAsync(w->count, 0, 4, s));
-        hipLaunchKernelGGL(k_overlap, dim3((n + 255) / 256), dim3(256), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
-                           (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->boxes->ptr,
-                           (const uint32_t*)w->info->ptr, (const uint32_t*)w->grp->ptr, n, (uint2*)w->pairs->ptr, cap, w->count);
-        PHR_HIP(hipGetLastError());
-        PHR_HIP(hipMemcpyAsync(&found, w->count, 4, hipMemcpyDeviceToHost, s));
-        PHR_HIP(hipStreamSynchronize(s));
-        if (found <= cap) {
-            std::vector<uint32_t> raw(2 * (size_t)found);
-            if (found) PHR_HIP(hipMemcpy(raw.data(), w->pairs->ptr, 8ull * found, hipMemcpyDeviceToHost));
-            out.resize(found);
-            for (uint32_t k = 0; k < found; ++k) out[k] = {raw[2 * (size_t)k], raw[2 * (size_t)k + 1]};
-            std::sort(out.begin(), out.end());
-            out.erase(std::unique(out.begin(), out.end()), out.end());
-            w->pairs_found = out.size();
-            return PHR_OK;
+    auto meets = [&](const Aabb& p, const Aabb& q) {
+        return p.lo[0] <= q.hi[0] && p.hi[0] >= q.lo[0] && p.lo[1] <= q.hi[1] && p.hi[1] >= q.lo[1] && p.lo[2] <= q.hi[2] &&
+               p.hi[2] >= q.lo[2];
+    };
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = i + 1; j < n; ++j) {
+            const Body &bi = w->bodies[i], &bj = w->bodies[j];
+            if (!bi.moves() && !bj.moves()) continue;
+            if (bi.kind() != K_ACTIVE && bj.kind() != K_ACTIVE) continue;
+            if ((w->consts[i].groups & w->consts[j].groups) == 0u) continue;
+            if (meets(boxes[i], boxes[j])) out.push_back({i, j});
         }
-        phr_array_release(w->pairs);  // more pairs than room: grow and walk again
-        w->pairs = nullptr;
-        if ((rc = phr_array_alloc(w->gpu, 8, 2ull * found, &w->pairs)) != PHR_OK) return rc;
-    }
 }
 
 // Bounds, broadphase, narrowphase; a contact wakes what it touches (world.rs:131-151).
-int collide(phr_world* w, float dt) {
+void collide(orc_world* w, float dt) {
     const size_t n = w->bodies.size();
     std::vector<Aabb> boxes(n);
     for (size_t i = 0; i < n; ++i) {  // broad.rs:52-70: swept by the motion over dt, grown by the margin
@@ -889,8 +813,7 @@ int collide(phr_world* w, float dt) {
         boxes[i] = Aabb{mk(swept.lo[0] - pad, swept.lo[1] - pad, swept.lo[2] - pad), mk(swept.hi[0] + pad, swept.hi[1] + pad, swept.hi[2] + pad)};
     }
     std::vector<std::pair<uint32_t, uint32_t>>& pairs = w->last_pairs;
-    int rc = world_pairs(w, boxes, pairs);
-    if (rc != PHR_OK) return rc;
+    world_pairs(w, boxes, pairs);
     w->contacts.clear();
     for (auto& pr : pairs) {
         const uint32_t i = pr.first, j = pr.second;
@@ -921,62 +844,47 @@ int collide(phr_world* w, float dt) {
     w->vn0.assign(w->contacts.size(), 0.0f);
     w->push.assign(n, mk(0, 0, 0));
     w->spin.assign(n, mk(0, 0, 0));
-    return PHR_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
-int phr_world_new(phr_gpu* g, phr_world** out) {
-    PHR_CHECK(g && out, PHR_ERR_ARG, "phr_world_new: null");
-    phr_world* w = new phr_world();
-    w->gpu = g;
-    *out = w;
-    return PHR_OK;
-}
+orc_world* orc_world_new(void) { return new orc_world(); }
 
-void phr_world_free(phr_world* w) {
-    if (!w) return;
-    if (w->tree) phr_bvh_free(w->tree);
-    for (phr_array* a : {w->boxes, w->info, w->grp, w->pairs}) phr_array_release(a);
-    if (w->count) (void)hipFree(w->count);
-    delete w;
-}
+void orc_world_free(orc_world* w) { delete w; }
 
 // World::substeps / iterations / time_scale / max_recover / sleep_after / gravity.
-int phr_world_configure(phr_world* w, uint32_t substeps, uint32_t iterations, float time_scale, float max_recover,
-                        float sleep_after, const float gravity[3]) {
-    PHR_CHECK(w, PHR_ERR_ARG, "phr_world_configure: null");
+void orc_world_configure(orc_world* w, uint32_t substeps, uint32_t iterations, float time_scale, float max_recover,
+                         float sleep_after, const float gravity[3]) {
     w->substeps = substeps;
     w->iterations = iterations;
     w->time_scale = time_scale;
     w->max_recover = max_recover;
     w->sleep_after = sleep_after;
     if (gravity) w->gravity = mk(gravity[0], gravity[1], gravity[2]);
-    return PHR_OK;
 }
 
 // World::add_shape (build.rs): box, sphere and capsule only.
-int phr_world_add_shape(phr_world* w, const phr_shape* s, uint32_t* id) {
-    PHR_CHECK(w && s && id, PHR_ERR_ARG, "phr_world_add_shape: null");
-    PHR_CHECK(s->kind == S_BOX || s->kind == S_SPHERE || s->kind == S_CAPSULE, PHR_ERR_UNSUPPORTED,
-              "phr_world_add_shape: hulls, cylinders, cones and meshes take GJK/EPA in phiray and are not restated");
+int orc_world_add_shape(orc_world* w, const orc_shape* s, uint32_t* id) {
+    ORC_CHECK(w && s && id, ORC_ERR_ARG, "orc_world_add_shape: null");
+    ORC_CHECK(s->kind == S_BOX || s->kind == S_SPHERE || s->kind == S_CAPSULE, ORC_ERR_UNSUPPORTED,
+              "orc_world_add_shape: hulls, cylinders, cones and meshes take GJK/EPA in phiray and are not restated");
     w->shapes.push_back(*s);
     *id = (uint32_t)w->shapes.size() - 1;
-    return PHR_OK;
+    return ORC_OK;
 }
 
 // World::add_body (build.rs:99-131).
-int phr_world_add_body(phr_world* w, const phr_body_def* d, uint32_t* id) {
-    PHR_CHECK(w && d && id, PHR_ERR_ARG, "phr_world_add_body: null");
-    PHR_CHECK(d->shape < w->shapes.size(), PHR_ERR_ARG, "phr_world_add_body: no such shape");
+int orc_world_add_body(orc_world* w, const orc_body_def* d, uint32_t* id) {
+    ORC_CHECK(w && d && id, ORC_ERR_ARG, "orc_world_add_body: null");
+    ORC_CHECK(d->shape < w->shapes.size(), ORC_ERR_ARG, "orc_world_add_body: no such shape");
     if (w->shapes[d->shape].kind == S_CAPSULE)
         for (const Body& b : w->bodies)
-            PHR_CHECK(w->shapes[b.shape].kind != S_BOX, PHR_ERR_UNSUPPORTED, "phr_world_add_body: box-capsule pairs take GJK/EPA");
+            ORC_CHECK(w->shapes[b.shape].kind != S_BOX, ORC_ERR_UNSUPPORTED, "orc_world_add_body: box-capsule pairs take GJK/EPA");
     if (w->shapes[d->shape].kind == S_BOX)
         for (const Body& b : w->bodies)
-            PHR_CHECK(w->shapes[b.shape].kind != S_CAPSULE, PHR_ERR_UNSUPPORTED, "phr_world_add_body: box-capsule pairs take GJK/EPA");
+            ORC_CHECK(w->shapes[b.shape].kind != S_CAPSULE, ORC_ERR_UNSUPPORTED, "orc_world_add_body: box-capsule pairs take GJK/EPA");
     const bool movable = d->kind == K_ACTIVE && d->mass > 0.0f;
     const float inv_mass = movable ? 1.0f / d->mass : 0.0f;
     V3 in = inertia(w->shapes[d->shape], d->mass);
@@ -1011,16 +919,15 @@ int phr_world_add_body(phr_world* w, const phr_body_def* d, uint32_t* id) {
     w->sleep_ang.push_back(d->has_sleep ? d->sleep[1] : 0.0f);
     w->timers.push_back(0.0f);
     *id = (uint32_t)w->bodies.size() - 1;
-    return PHR_OK;
+    return ORC_OK;
 }
 
 // World::step (world.rs:104-129): one frame of frame_dt seconds.
-int phr_world_step(phr_world* w, float frame_dt) {
-    PHR_CHECK(w, PHR_ERR_ARG, "phr_world_step: null");
+int orc_world_step(orc_world* w, float frame_dt) {
+    ORC_CHECK(w, ORC_ERR_ARG, "orc_world_step: null");
     const float dt = frame_dt * w->time_scale;
-    if (dt <= 0.0f || w->substeps == 0) return PHR_OK;
-    int rc = collide(w, dt);
-    if (rc != PHR_OK) return rc;
+    if (dt <= 0.0f || w->substeps == 0) return ORC_OK;
+    collide(w, dt);
     const float h = dt / (float)w->substeps;
     const float g = len(w->gravity);
     std::vector<Body>& B = w->bodies;
@@ -1068,23 +975,23 @@ int phr_world_step(phr_world* w, float frame_dt) {
             b.w = mk(0, 0, 0);
         }
     }
-    return PHR_OK;
+    return ORC_OK;
 }
 
-uint32_t phr_world_body_count(const phr_world* w) { return w ? (uint32_t)w->bodies.size() : 0u; }
-uint32_t phr_world_contact_count(const phr_world* w) { return w ? (uint32_t)w->contacts.size() : 0u; }
+uint32_t orc_world_body_count(const orc_world* w) { return w ? (uint32_t)w->bodies.size() : 0u; }
+uint32_t orc_world_contact_count(const orc_world* w) { return w ? (uint32_t)w->contacts.size() : 0u; }
 
 // The bodies' 64-byte records (state.rs Body).
-int phr_world_bodies(const phr_world* w, void* out) {
-    PHR_CHECK(w && out, PHR_ERR_ARG, "phr_world_bodies: null");
+int orc_world_bodies(const orc_world* w, void* out) {
+    ORC_CHECK(w && out, ORC_ERR_ARG, "orc_world_bodies: null");
     std::memcpy(out, w->bodies.data(), sizeof(Body) * w->bodies.size());
-    return PHR_OK;
+    return ORC_OK;
 }
 
 // World::poses (world.rs:225-228): each body's object matrix, m[col][row]
 // (math.rs pose_matrix), 16 floats a body.
-int phr_world_poses(const phr_world* w, float* out) {
-    PHR_CHECK(w && out, PHR_ERR_ARG, "phr_world_poses: null");
+int orc_world_poses(const orc_world* w, float* out) {
+    ORC_CHECK(w && out, ORC_ERR_ARG, "orc_world_poses: null");
     for (size_t i = 0; i < w->bodies.size(); ++i) {
         const Body& b = w->bodies[i];
         const V3 s = w->scales[i];
@@ -1098,11 +1005,11 @@ int phr_world_poses(const phr_world* w, float* out) {
         V3 t = sub(b.x, rotate(b.q, w->consts[i].com_local));
         m[12] = t[0], m[13] = t[1], m[14] = t[2], m[15] = 1.0f;
     }
-    return PHR_OK;
+    return ORC_OK;
 }
 
 // World::energy (world.rs:231-243): kinetic plus potential of the active bodies.
-float phr_world_energy(const phr_world* w) {
+float orc_world_energy(const orc_world* w) {
     if (!w) return 0.0f;
     float total = 0.0f;
     for (size_t i = 0; i < w->bodies.size(); ++i) {
@@ -1119,14 +1026,14 @@ float phr_world_energy(const phr_world* w) {
 }
 
 // The pairs the last broadphase found, sorted and deduplicated (broad.rs:73-91).
-int phr_world_pair_list(const phr_world* w, uint32_t* out, uint32_t cap, uint32_t* n) {
-    PHR_CHECK(w && n, PHR_ERR_ARG, "phr_world_pair_list: null");
+int orc_world_pair_list(const orc_world* w, uint32_t* out, uint32_t cap, uint32_t* n) {
+    ORC_CHECK(w && n, ORC_ERR_ARG, "orc_world_pair_list: null");
     *n = (uint32_t)w->last_pairs.size();
     if (out) {
-        PHR_CHECK(cap >= *n, PHR_ERR_SIZE, "phr_world_pair_list: buffer too small");
+        ORC_CHECK(cap >= *n, ORC_ERR_SIZE, "orc_world_pair_list: buffer too small");
         for (uint32_t k = 0; k < *n; ++k) out[2 * k] = w->last_pairs[k].first, out[2 * k + 1] = w->last_pairs[k].second;
     }
-    return PHR_OK;
+    return ORC_OK;
 }
 
 }  // extern "C"

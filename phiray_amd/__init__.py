@@ -190,6 +190,9 @@ _SIGS = {
     "phr_bvh_build_on": (_I, [_P, _P, _U32, C.POINTER(_P)]),
     "phr_bvh_build_static": (_I, [_P, _P, _U32, _U32, _P, _P, _P]),
     "phr_bvh_rebuild_async": (_I, [_P]),
+    "phr_bvh_set_sort_tile": (_I, [_P, _U32]),
+    "phr_bvh_sort_tile": (_U32, [_P]),
+    "phr_bvh_autotune": (_I, [_P, _U32, C.POINTER(_U32), _P]),
     "phr_bvh_rebuild": (_I, [_P]),
     "phr_bvh_refit_async": (_I, [_P]),
     "phr_bvh_refit": (_I, [_P]),
@@ -271,6 +274,8 @@ _SIGS = {
     "phr_kpcn_adam": (_I, [_P, C.c_float]),
     "phr_kpcn_train_step": (_I, [_P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_kpcn_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P]),
+    "phr_kpcn_held_out": (_I, [_P, _P, _P, _P, _U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "phr_kpcn_frame_mean": (_I, [_P, _P, _P, _U64, _U32, _P]),
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
@@ -295,6 +300,8 @@ _SIGS = {
     "phr_tracer_path_waves": (_U32, [_P]),
     "phr_tracer_set_path_regen": (_I, [_P, _U32]),
     "phr_tracer_path_regen": (_U32, [_P]),
+    "phr_tracer_path_stack": (_U32, [_P]),
+    "phr_tracer_set_path_stack": (_I, [_P, _U32]),
     "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_tracer_denoise_learned": (_I, [_P, _P, _P]),
     "phr_batch_new": (_I, [_P, _P, _P, _U32, _U32, C.POINTER(_P)]),
@@ -735,6 +742,21 @@ class GpuBvh:
     def rebuild_async(self) -> None:
         _check(lib().phr_bvh_rebuild_async(self._h), "GpuBvh::rebuild_async")
 
+    @property
+    def sort_tile(self) -> int:
+        """Keys a thread holds in a sort pass (the tile is 256 x this)."""
+        return int(lib().phr_bvh_sort_tile(self._h))
+
+    def set_sort_tile(self, items: int) -> None:
+        _check(lib().phr_bvh_set_sort_tile(self._h, items), "GpuBvh sort tile")
+
+    def autotune(self, reps: int = 5) -> tuple[int, list[float]]:
+        """Time rebuilds at each sort tile, keep the faster: (items, ms each)."""
+        items = C.c_uint32(0)
+        ms = np.zeros(2, dtype=np.float64)
+        _check(lib().phr_bvh_autotune(self._h, reps, C.byref(items), _ptr(ms)), "GpuBvh autotune")
+        return int(items.value), ms.tolist()
+
     def refit(self) -> None:
         _check(lib().phr_bvh_refit(self._h), "GpuBvh::refit")
 
@@ -964,6 +986,15 @@ class Tracer:
     @property
     def path_regen(self) -> bool:
         return bool(lib().phr_tracer_path_regen(self._h))
+
+    @property
+    def path_stack(self) -> int:
+        """Mesh-tree stack entries a thread of the path launches holds."""
+        return int(lib().phr_tracer_path_stack(self._h))
+
+    def set_path_stack(self, entries: int) -> None:
+        """24 (a forest whose mesh trees are at most 24 deep) or 32 entries."""
+        _check(lib().phr_tracer_set_path_stack(self._h, entries), "Tracer path stack")
 
     def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 2) -> tuple[int, float]:
         """Time each path kernel variant on this workload and keep the fastest:
@@ -1276,6 +1307,22 @@ class Kpcn:
         out = np.zeros(self.param_count(), dtype=np.float32)
         _check(lib().phr_kpcn_get_grads(self._h, _ptr(out), out.size), "Kpcn::grads")
         return out
+
+    @staticmethod
+    def held_out(gpu: Gpu, denoised: GpuArray, half_a: GpuArray, half_b: GpuArray, n: int) -> tuple[float, float]:
+        """(estimated relMSE of `denoised`, made from half A alone, against the
+        converged image; half B's own noise term), from two independent
+        halves of a frame."""
+        err, noise = C.c_double(0), C.c_double(0)
+        _check(lib().phr_kpcn_held_out(gpu._h, denoised._h, half_a._h, half_b._h, n, C.byref(err), C.byref(noise)),
+               "Kpcn held-out check")
+        return float(err.value), float(noise.value)
+
+    @staticmethod
+    def frame_mean(gpu: Gpu, half_a: GpuArray, half_b: GpuArray, n: int, out: GpuArray, channels: int = 10) -> None:
+        """Both halves' frame as one (equal sample counts): all 10 channels,
+        or the radiance alone (channels=3)."""
+        _check(lib().phr_kpcn_frame_mean(gpu._h, half_a._h, half_b._h, n, channels, out._h), "Kpcn frame mean")
 
     def infer(self, frames: GpuArray, b: int, w: int, h: int, out: GpuArray) -> None:
         _check(lib().phr_kpcn_infer(self._h, frames._h, b, w, h, out._h), "Kpcn::infer")

@@ -1005,6 +1005,60 @@ int check_shape(phr_array* frames, uint32_t b, uint32_t w, uint32_t h, phr_array
     return PHR_OK;
 }
 
+// ---- the held-out check -----------------------------------------------------------
+// Two independent halves A and B of a frame (the same samples a pixel, other
+// streams), D the network's output from A alone. B is unbiased and
+// independent of D, and A's noise equals B's in distribution, so per value
+//   E[(D - B)^2] - E[(A - B)^2] / 2 = E[(D - T)^2]
+// for the converged image T: the sum below estimates D's relMSE against T,
+// with D^2 + 0.01 standing in for T^2 + 0.01. The second sum is B's own
+// noise on the same scale. Per-block partials in double, summed on the host.
+constexpr uint32_t HO_BLOCKS = 512;
+__global__ __launch_bounds__(256) void k_held_out(const float* __restrict__ d, const float* __restrict__ fa,
+                                                  const float* __restrict__ fb, uint64_t n, double* __restrict__ part) {
+    __shared__ double s_e[256], s_v[256];
+    double e = 0.0, v = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        for (int c = 0; c < 3; ++c) {
+            const float y = d[3 * i + c], a = fa[FRAME * i + c], b = fb[FRAME * i + c];
+            const float den = y * y + 0.01f;
+            const float ab = (a - b) * (a - b) * 0.5f;
+            e += (double)(((y - b) * (y - b) - ab) / den);
+            v += (double)(ab / den);
+        }
+    s_e[threadIdx.x] = e;
+    s_v[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_e[threadIdx.x] += s_e[threadIdx.x + o];
+            s_v[threadIdx.x] += s_v[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s_e[0];
+        part[2 * blockIdx.x + 1] = s_v[0];
+    }
+}
+
+// The frame of both halves together (equal sample counts): the mean of each
+// channel, depth over the halves that met something (-1 when neither did).
+// With channels = 3 only the radiance, 3 floats a pixel.
+__global__ __launch_bounds__(256) void k_frame_mean(const float* __restrict__ fa, const float* __restrict__ fb, uint64_t n,
+                                                    uint32_t channels, float* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float* a = fa + FRAME * i;
+    const float* b = fb + FRAME * i;
+    float* o = out + (uint64_t)channels * i;
+    for (uint32_t c = 0; c < min(channels, 9u); ++c) o[c] = 0.5f * (a[c] + b[c]);
+    if (channels == FRAME) {
+        const float za = a[9], zb = b[9];
+        o[9] = za < 0.f ? zb : (zb < 0.f ? za : 0.5f * (za + zb));
+    }
+}
+
 }  // namespace
 
 // ---- C ABI ----------------------------------------------------------------------------
@@ -1156,6 +1210,45 @@ int phr_kpcn_adam(phr_kpcn* k, float lr) {
 // timing both on b frames of w x h: the best of `reps` launches each, with HIP
 // events on the network's stream. chosen (optional) gets the rows a wave owns
 // per layer. The outputs do not depend on the tile.
+int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_array* frame_b, uint64_t n, double* err,
+                      double* noise) {
+    PHR_CHECK(g && denoised && frame_a && frame_b && err && n, PHR_ERR_ARG, "phr_kpcn_held_out: null");
+    PHR_CHECK(denoised->len * denoised->elem >= n * 12 && frame_a->len * frame_a->elem >= n * FRAME * 4 &&
+                  frame_b->len * frame_b->elem >= n * FRAME * 4,
+              PHR_ERR_SIZE, "phr_kpcn_held_out: arrays smaller than n pixels");
+    double* part = nullptr;
+    PHR_HIP(hipMalloc((void**)&part, 16ull * HO_BLOCKS));
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(HO_BLOCKS, (n + 255) / 256);
+    hipLaunchKernelGGL(k_held_out, dim3(blocks), dim3(256), 0, g->stream, (const float*)denoised->ptr, (const float*)frame_a->ptr,
+                       (const float*)frame_b->ptr, n, part);
+    std::vector<double> h(2ull * blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 16ull * blocks, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) {
+        phr_set_error("phr_kpcn_held_out", e);
+        return PHR_ERR_HIP;
+    }
+    double se = 0.0, sv = 0.0;
+    for (uint32_t i = 0; i < blocks; ++i) se += h[2 * i], sv += h[2 * i + 1];
+    *err = se / (3.0 * (double)n);
+    if (noise) *noise = sv / (3.0 * (double)n);
+    return PHR_OK;
+}
+
+int phr_kpcn_frame_mean(phr_gpu* g, phr_array* frame_a, phr_array* frame_b, uint64_t n, uint32_t channels, phr_array* out) {
+    PHR_CHECK(g && frame_a && frame_b && out && n, PHR_ERR_ARG, "phr_kpcn_frame_mean: null");
+    PHR_CHECK(channels == FRAME || channels == 3, PHR_ERR_ARG, "phr_kpcn_frame_mean: 10 channels (a frame) or 3 (radiance)");
+    PHR_CHECK(frame_a->len * frame_a->elem >= n * FRAME * 4 && frame_b->len * frame_b->elem >= n * FRAME * 4 &&
+                  out->len * out->elem >= n * channels * 4,
+              PHR_ERR_SIZE, "phr_kpcn_frame_mean: arrays smaller than n pixels");
+    hipLaunchKernelGGL(k_frame_mean, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, g->stream, (const float*)frame_a->ptr,
+                       (const float*)frame_b->ptr, n, channels, (float*)out->ptr);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
+
 int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, uint32_t reps, uint8_t* chosen) {
     PHR_CHECK(k, PHR_ERR_ARG, "phr_kpcn_autotune: null");
     KP_TRY(check_shape(frames, b, w, h, nullptr, "phr_kpcn_autotune"));

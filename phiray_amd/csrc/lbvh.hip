@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ prims
 // ---- 3. onesweep pass ------------------------------------------------------
 // FIRST: the values are the input indices. SEGD: the digit is a byte of the
 // value's segment id instead of a byte of the key (the key rides along).
-template <bool FIRST, bool SEGD>
+template <bool FIRST, bool SEGD, uint32_t ITEMS>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                   const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* ticket,
@@ -401,9 +401,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     __shared__ uint32_t s_whist[OS_WAVES][256];
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
-    __shared__ uint32_t s_keys[OS_TILE];
-    __shared__ uint32_t s_vals[OS_TILE];
-    __shared__ uint8_t s_dig[OS_TILE];
+    __shared__ uint32_t s_keys[(OS_THREADS * ITEMS)];
+    __shared__ uint32_t s_vals[(OS_THREADS * ITEMS)];
+    __shared__ uint8_t s_dig[(OS_THREADS * ITEMS)];
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_tile;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -411,10 +411,10 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     for (uint32_t i = tid; i < OS_WAVES * 256; i += 256) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t wbase = tile * OS_TILE + wave * (OS_ITEMS * 64);
-    uint32_t key[OS_ITEMS], val[OS_ITEMS], rank[OS_ITEMS], dig[OS_ITEMS];
+    const uint32_t wbase = tile * (OS_THREADS * ITEMS) + wave * (ITEMS * 64);
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS], dig[ITEMS];
 #pragma unroll
-    for (uint32_t r = 0; r < OS_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         bool ok = idx < n;
         key[r] = ok ? kin[idx] : 0u;
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     }
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (uint32_t r = 0; r < OS_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         bool ok = idx < n;
         uint32_t d = dig[r];
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     s_global[d] = binbase + excl;
     __syncthreads();
 #pragma unroll
-    for (uint32_t r = 0; r < OS_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint32_t idx = wbase + r * 64 + lane;
         if (idx < n) {
             uint32_t dd = dig[r];
@@ -479,9 +479,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
         }
     }
     __syncthreads();
-    const uint32_t count = min(OS_TILE, n - tile * OS_TILE);
+    const uint32_t count = min((OS_THREADS * ITEMS), n - tile * (OS_THREADS * ITEMS));
 #pragma unroll
-    for (uint32_t r = 0; r < OS_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint32_t idx = r * 256 + tid;
         if (idx < count) {
             uint32_t dd = s_dig[idx];
@@ -976,6 +976,7 @@ __global__ __launch_bounds__(256) void k_compact_write(const uint32_t* __restric
 struct phr_bvh {
     phr_gpu* gpu = nullptr;
     uint32_t n = 0, leaf = 4, segs = 1, nonempty = 1, tiles = 0, bounds_blocks = 0, seg_passes = 0, leaf_flag = PHR_LEAF;
+    uint32_t items = OS_ITEMS;  // keys a thread holds in a sort pass (the tile is 256 x items)
     std::vector<uint32_t> seg_lo_h, node_base_h, depth_h;
     phr_array* prims = nullptr;
     bool own_prims = false;
@@ -1109,6 +1110,22 @@ int bvh_alloc(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, const uint3
 
 int bvh_enqueue_refit(phr_bvh* b);
 
+// One onesweep pass of tiles of 256 x ITEMS keys.
+template <uint32_t ITEMS>
+void launch_pass(bool first, bool seg, uint32_t tiles, hipStream_t s, uint32_t* kin, uint32_t* vin, uint32_t* kout,
+                 uint32_t* vout, const uint32_t* hist, uint32_t* st, uint32_t* ticket, uint32_t n, uint32_t shift,
+                 const uint32_t* seg_of) {
+    if (first)
+        hipLaunchKernelGGL((k_onesweep<true, false, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
+                           ticket, n, shift, seg_of);
+    else if (!seg)
+        hipLaunchKernelGGL((k_onesweep<false, false, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
+                           ticket, n, shift, seg_of);
+    else
+        hipLaunchKernelGGL((k_onesweep<false, true, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
+                           ticket, n, shift, seg_of);
+}
+
 // Enqueue the whole build on the device's stream (no host synchronisation).
 int bvh_enqueue_build(phr_bvh* b) {
     const uint32_t n = b->n;
@@ -1130,22 +1147,20 @@ int bvh_enqueue_build(phr_bvh* b) {
                        n, unsorted, hist, b->far_ends, b->seg_of);
     uint32_t* kin = unsorted;
     uint32_t* vin = nullptr;
+    // The status words are sized for 2048-key tiles; a larger tile uses fewer.
+    const uint32_t sort_tiles = (n + OS_THREADS * b->items - 1) / (OS_THREADS * b->items);
     for (uint32_t p = 0; p < passes; ++p) {
         const bool to_codes = ((passes - p) % 2) == 1;
         uint32_t* kout = to_codes ? b->codes : b->keys_b;
         uint32_t* vout = to_codes ? order : b->vals_b;
         const bool seg = p >= CODE_PASSES;
         const uint32_t shift = seg ? 8 * (p - CODE_PASSES) : 8 * p;
-        uint32_t* st = status + (size_t)p * b->tiles * 256;
-        if (p == 0)
-            hipLaunchKernelGGL((k_onesweep<true, false>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist + 256 * p,
-                               st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
-        else if (!seg)
-            hipLaunchKernelGGL((k_onesweep<false, false>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout,
-                               hist + 256 * p, st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
+        uint32_t* st = status + (size_t)p * sort_tiles * 256;
+        if (b->items == 16)
+            launch_pass<16>(p == 0, seg, sort_tiles, s, kin, vin, kout, vout, hist + 256 * p, st, ticket + p, n, shift, b->seg_of);
         else
-            hipLaunchKernelGGL((k_onesweep<false, true>), dim3(b->tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout,
-                               hist + 256 * p, st, ticket + p, n, shift, (const uint32_t*)b->seg_of);
+            launch_pass<OS_ITEMS>(p == 0, seg, sort_tiles, s, kin, vin, kout, vout, hist + 256 * p, st, ticket + p, n, shift,
+                                  b->seg_of);
         kin = kout;
         vin = vout;
     }
@@ -1281,6 +1296,51 @@ int phr_bvh_build_batch_on(phr_gpu* g, phr_array* prims, const uint32_t* seg_lo,
     PHR_CHECK(g && prims && out && seg_lo && prims->elem == 32, PHR_ERR_ARG,
               "phr_bvh_build_batch_on: prims must be an array of 32-byte boxes");
     return bvh_new(g, prims, false, leaf, seg_lo, segs, nullptr, nullptr, PHR_LEAF, out);
+}
+
+int phr_bvh_set_sort_tile(phr_bvh* b, uint32_t items) {
+    PHR_CHECK(b, PHR_ERR_ARG, "phr_bvh_set_sort_tile: null");
+    PHR_CHECK(items == 8 || items == 16, PHR_ERR_ARG, "phr_bvh_set_sort_tile: 8 or 16 keys a thread");
+    b->items = items;
+    return PHR_OK;
+}
+
+uint32_t phr_bvh_sort_tile(const phr_bvh* b) { return b ? b->items : 0u; }
+
+int phr_bvh_autotune(phr_bvh* b, uint32_t reps, uint32_t* items, double* ms) {
+    PHR_CHECK(b && reps > 0, PHR_ERR_ARG, "phr_bvh_autotune: null or no repetitions");
+    hipStream_t s = b->gpu->stream;
+    hipEvent_t e0, e1;
+    PHR_HIP(hipEventCreate(&e0));
+    PHR_HIP(hipEventCreate(&e1));
+    const uint32_t cand[2] = {8, 16};
+    float best = 3.0e38f;
+    uint32_t pick = b->items;
+    int rc = PHR_OK;
+    for (int c = 0; c < 2 && rc == PHR_OK; ++c) {
+        b->items = cand[c];
+        if ((rc = bvh_enqueue_build(b)) != PHR_OK) break;  // warm
+        float t = 0.f;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = PHR_ERR_HIP; break; }
+        for (uint32_t r = 0; r < reps && rc == PHR_OK; ++r) rc = bvh_enqueue_build(b);
+        if (rc != PHR_OK) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = PHR_ERR_HIP;
+            break;
+        }
+        t /= (float)reps;
+        if (ms) ms[c] = t;
+        if (t < best) { best = t; pick = cand[c]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PHR_OK) {
+        if (rc == PHR_ERR_HIP) phr_set_error_msg("phr_bvh_autotune: timing failed");
+        return rc;
+    }
+    b->items = pick;
+    if (items) *items = pick;
+    return PHR_OK;
 }
 
 int phr_bvh_rebuild_async(phr_bvh* b) {
@@ -1696,10 +1756,10 @@ int phr_sort_run(phr_sort* s) {
         uint32_t* vout = to_final ? s->order : s->vb;
         uint32_t* stp = status + (size_t)p * s->tiles * 256;
         if (p == 0)
-            hipLaunchKernelGGL((k_onesweep<true, false>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout, hist, stp,
+            hipLaunchKernelGGL((k_onesweep<true, false, OS_ITEMS>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout, hist, stp,
                                ticket, n, 0u, (const uint32_t*)nullptr);
         else
-            hipLaunchKernelGGL((k_onesweep<false, false>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout,
+            hipLaunchKernelGGL((k_onesweep<false, false, OS_ITEMS>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout,
                                hist + 256 * p, stp, ticket + p, n, 8 * p, (const uint32_t*)nullptr);
         kin = kout;
         vin = vout;

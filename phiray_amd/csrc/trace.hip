@@ -1264,11 +1264,13 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 // WPE: waves per SIMD the register allocation is fitted to (2 needs no spill;
 // 3 to 5 trade spilled words for more waves in flight). The tracer's
 // autotune times the variants on the caller's workload and keeps the fastest.
-template <bool F, int WPE>
+// MSTK: the mesh-tree stack's entries (a forest over mesh trees no deeper
+// than that; the autotune may pick the short stack at 3 waves too).
+template <bool F, int WPE, uint32_t MSTK = mesh_stack_entries<F, WPE>()>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
     unsigned long long* __restrict__ rays, uint32_t listed, uint32_t tile0) {
-    __shared__ uint32_t s_stack[mesh_stack_entries<F, WPE>() * TRACE_THREADS];
+    __shared__ uint32_t s_stack[MSTK * TRACE_THREADS];
     __shared__ uint16_t s_tstack[top_stack_entries<F>() * TRACE_THREADS];
     path_block<F>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, listed, s_stack, s_tstack);
 }
@@ -1311,9 +1313,11 @@ bool regen_allowed(const PhrCamera& cam) {
 
 // regen: the path-regenerating build (k_path_rg), for one tree, no list and
 // no Adaptive Sampling or studio rig (the caller checks the camera).
+// short_stack: the forest's 3-wave build with FOREST_SHORT_STACK mesh-tree
+// entries (the caller checked the mesh trees' depth).
 void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp,
                  uint32_t tiles_x, uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed,
-                 uint32_t tile0) {
+                 uint32_t tile0, bool short_stack = false) {
     const dim3 block(TRACE_THREADS);
     if (regen && !forest && !listed) {
         if (waves == 5)
@@ -1326,6 +1330,8 @@ void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t
             hipLaunchKernelGGL(k_path_rg<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
     } else if (forest && waves >= 4)  // the caller checked the mesh trees' depth
         hipLaunchKernelGGL((k_path<true, 4>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
+    else if (forest && waves == 3 && short_stack)
+        hipLaunchKernelGGL((k_path<true, 3, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
     else if (forest && waves == 3)  // 32 + 32 16-bit entries (192 B a thread): three blocks a CU
         hipLaunchKernelGGL((k_path<true, 3>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
     else if (forest)
@@ -1543,6 +1549,7 @@ struct phr_tracer {
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
     uint32_t path_regen = 0;  // 1: the path-regenerating build where the camera allows it
     uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
+    bool short_stack = false;         // the forest's 3-wave build with the 24-entry mesh-tree stack
     // The binding split phiray's WGSL needs (items a binding for nodes, faces,
     // vertices; the atlas offset in the data table) and bindings an array.
     // The kernels here index whole arrays; the numbers are kept for callers.
@@ -1634,6 +1641,9 @@ uint32_t path_build_waves(const phr_tracer* t) {
     if (t->forest && t->path_waves >= 4) return t->mesh_depth <= FOREST_SHORT_STACK ? 4u : 3u;
     return t->path_waves;
 }
+
+// Whether the forest's 3-wave build takes the short mesh-tree stack.
+bool path_short_stack(const phr_tracer* t) { return t->forest && t->short_stack && t->mesh_depth <= FOREST_SHORT_STACK; }
 
 // The deepest mesh tree of a forest, read back after a build.
 int forest_mesh_depth(phr_tracer* t) {
@@ -2243,7 +2253,8 @@ int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, ui
         {"k_path<2>", (const void*)k_path<false, 2>}, {"k_path<3>", (const void*)k_path<false, 3>},
         {"k_path<4>", (const void*)k_path<false, 4>}, {"k_path<5>", (const void*)k_path<false, 5>},
         {"k_path_forest<2>", (const void*)k_path<true, 2>}, {"k_path_forest<3>", (const void*)k_path<true, 3>},
-        {"k_path_forest<4>", (const void*)k_path<true, 4>}, {"k_path_rg<2>", (const void*)k_path_rg<2>},
+        {"k_path_forest<4>", (const void*)k_path<true, 4>},
+        {"k_path_forest<3, short stack>", (const void*)k_path<true, 3, FOREST_SHORT_STACK>}, {"k_path_rg<2>", (const void*)k_path_rg<2>},
         {"k_path_rg<3>", (const void*)k_path_rg<3>}, {"k_path_rg<4>", (const void*)k_path_rg<4>},
         {"k_path_rg<5>", (const void*)k_path_rg<5>}, {"k_path_batch<2>", (const void*)k_path_batch<2>},
         {"k_path_batch<3>", (const void*)k_path_batch<3>}, {"k_path_batch<4>", (const void*)k_path_batch<4>},
@@ -2339,7 +2350,8 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
         for (uint32_t k = 0; k < slices && !abandoned; ++k) {
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
             if (b == a) continue;
-            launch_path(path_build_waves(t), t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a);
+            launch_path(path_build_waves(t), t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a,
+                        path_short_stack(t));
             PHR_HIP(hipGetLastError());
             if (slices > 1) {
                 PHR_HIP(hipStreamSynchronize(s));
@@ -2414,6 +2426,24 @@ int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on) {
 }
 uint32_t phr_tracer_path_regen(const phr_tracer* t) { return t ? t->path_regen : 0; }
 
+// The mesh-tree stack entries the path launches take: the 4-wave forest
+// build's 24, the 3-wave build's 24 or 32 as the autotune chose, 32 otherwise.
+int phr_tracer_set_path_stack(phr_tracer* t, uint32_t entries) {
+    PHR_CHECK(t, PHR_ERR_ARG, "phr_tracer_set_path_stack: null");
+    PHR_CHECK(entries == FOREST_SHORT_STACK || entries == PHR_STACK, PHR_ERR_ARG, "phr_tracer_set_path_stack: 24 or 32 entries");
+    PHR_CHECK(entries == PHR_STACK || (t->forest && t->mesh_depth <= FOREST_SHORT_STACK), PHR_ERR_UNSUPPORTED,
+              "phr_tracer_set_path_stack: the short stack serves a forest whose mesh trees are at most 24 deep");
+    t->short_stack = entries == FOREST_SHORT_STACK;
+    return PHR_OK;
+}
+
+uint32_t phr_tracer_path_stack(const phr_tracer* t) {
+    if (!t) return 0u;
+    if (t->forest && path_build_waves(t) >= 4) return FOREST_SHORT_STACK;
+    if (t->forest && path_build_waves(t) == 3 && path_short_stack(t)) return FOREST_SHORT_STACK;
+    return PHR_STACK;
+}
+
 // Time every path kernel variant on the caller's workload (`spp` samples of
 // w x h under `cam`, best of `reps` launches each) and keep the fastest. The
 // image starts over; the variants compute the same sums.
@@ -2426,22 +2456,26 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     PhrCamera probe;
     std::memcpy(&probe, cam, sizeof probe);
     const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 2u : 1u;
+    const bool fits_short = t->forest && t->mesh_depth <= FOREST_SHORT_STACK;
+    bool best_short = false;
     for (uint32_t regen = 0; regen < regens; ++regen)
-        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? (t->mesh_depth <= FOREST_SHORT_STACK ? 4u : 3u) : PATH_WAVES_MAX);
-             ++waves) {
-            t->path_waves = waves;
-            t->path_regen = regen;
-            double mine = 1e30;
-            for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
-                t->have_last = false;  // start the image over every launch
-                int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
-                if (rc != PHR_OK) return rc;
-                if (r > 0) mine = std::min(mine, t->last_gpu_s);
+        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? (fits_short ? 4u : 3u) : PATH_WAVES_MAX); ++waves)
+            for (uint32_t stk = 0; stk < ((fits_short && waves == 3) ? 2u : 1u); ++stk) {
+                t->path_waves = waves;
+                t->path_regen = regen;
+                t->short_stack = stk == 1;
+                double mine = 1e30;
+                for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
+                    t->have_last = false;  // start the image over every launch
+                    int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
+                    if (rc != PHR_OK) return rc;
+                    if (r > 0) mine = std::min(mine, t->last_gpu_s);
+                }
+                if (mine < best) best = mine, best_w = waves, best_r = regen, best_short = stk == 1;
             }
-            if (mine < best) best = mine, best_w = waves, best_r = regen;
-        }
     t->path_regen = best_r;
     t->path_waves = best_w;
+    t->short_stack = best_short;
     t->have_last = false;
     if (chosen) *chosen = best_w;
     if (best_s) *best_s = best;
@@ -2622,7 +2656,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     launch_path(path_build_waves(t), t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
-                t->rays, 0u, 0u);
+                t->rays, 0u, 0u, path_short_stack(t));
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
     if (denoise == 2) {

@@ -6,11 +6,18 @@ Everything that computes runs in the library's HIP kernels (`Kpcn`,
 batch holds. The network stands beside phiray's `kernel::Denoise`
 (src/kernel/denoise.rs:26-64), which it does not replace.
 
-Training data: seeded spheres scenes (the Rendered-mode generator) seen by
-the 1920x1080 camera, cropped to `crop` x `crop` windows. Inputs are
-`spp_in`-sample frames, several independent renders per scene. Targets are
-`spp_ref`-sample renders of the same window on sample streams the inputs
-never draw.
+Training data: seeded scenes of the LBVH workload's generator (65,536
+triangles in the unit cube, albedo U[0.2, 0.8]^3, a white world; see
+scenes.cloud_scene) seen by the 1920x1080 camera, cropped to `crop` x `crop`
+windows. Inputs are `spp_in`-sample frames, several independent renders per
+scene. Targets are `spp_ref`-sample renders of the same window on sample
+streams the inputs never draw.
+
+The learned frame (`LearnedFrame`) is guarded by the plain render: a frame
+is traced as two independent halves, the network denoises one, the other
+judges it (Kpcn.held_out), and more samples are taken while the estimated
+error is over the bound; past the sample budget the plain render of every
+sample is the answer.
 """
 from __future__ import annotations
 
@@ -29,6 +36,8 @@ def _log(msg: str) -> None:
 HERE = pathlib.Path(__file__).resolve().parent
 WEIGHTS = HERE / "kpcn_weights.npy"
 REF_STREAM = 1 << 24  # sample streams of the targets; inputs use small ones
+HALF_STREAM = 1 << 20  # the judging half of a learned frame
+SCENES = {"cloud": (S.cloud_scene, S.cloud_camera), "spheres": (S.spheres_scene, S.path_camera)}
 
 
 def layer_slices(layers: np.ndarray | None = None) -> list[tuple[slice, slice, tuple[int, int]]]:
@@ -72,10 +81,10 @@ def load(path: pathlib.Path | str = WEIGHTS) -> np.ndarray:
     return p
 
 
-def crop_camera(w: int, h: int, x0: int, y0: int, cw: int, ch: int, **kw) -> np.ndarray:
-    """The camera of path_camera(w, h) restricted to the window [x0, x0+cw) x
+def crop_camera(w: int, h: int, x0: int, y0: int, cw: int, ch: int, camera=S.path_camera, **kw) -> np.ndarray:
+    """The camera `camera(w, h)` restricted to the window [x0, x0+cw) x
     [y0, y0+ch): same rays per pixel, through the view-plane offset `pan`."""
-    cam = S.path_camera(w, h, **kw)
+    cam = camera(w, h, **kw)
     full_scale = float(cam["scale"][0])
     c = cam[0]
     c["scale"] = full_scale * w / cw
@@ -138,9 +147,10 @@ def main(argv=None) -> int:
 
     ap = argparse.ArgumentParser(description="Train the learned denoiser on path-traced pairs.")
     ap.add_argument("--scenes", type=int, default=32, help="training scenes (seeds 1000, 1001, ...)")
+    ap.add_argument("--kind", choices=sorted(SCENES), default="cloud", help="the scene generator")
     ap.add_argument("--crop", type=int, default=512)
     ap.add_argument("--spp-ref", type=int, default=4096)
-    ap.add_argument("--spp-in", type=str, default="1,2,4", help="input sample counts, one variant set each")
+    ap.add_argument("--spp-in", type=str, default="2,4", help="input sample counts, one variant set each")
     ap.add_argument("--variants", type=int, default=4, help="renders per scene per input sample count")
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=8)
@@ -156,7 +166,7 @@ def main(argv=None) -> int:
     spps = [int(s) for s in args.spp_in.split(",")]
     seeds = list(range(1000, 1000 + args.scenes))
     t0 = time.perf_counter()
-    frames, targets, meta = render_multi(gpu, seeds, args.crop, spps, args.spp_ref, args.variants)
+    frames, targets, meta = render_multi(gpu, seeds, args.crop, spps, args.spp_ref, args.variants, kind=args.kind)
     _log(f"[kpcn] training set rendered in {time.perf_counter() - t0:.0f} s")
     net, hist = train(gpu, frames, targets, len(seeds), len(spps) * args.variants, args.crop, args.steps,
                       batch=args.batch, lr=args.lr, lr_final=args.lr_final, budget_s=args.budget_s)
@@ -169,7 +179,7 @@ def main(argv=None) -> int:
 
 
 def render_multi(gpu: Gpu, seeds, crop: int, spps, spp_ref: int, variants: int, full=(1920, 1080),
-                 log=_log) -> tuple[GpuArray, GpuArray, list[dict]]:
+                 log=_log, kind: str = "cloud") -> tuple[GpuArray, GpuArray, list[dict]]:
     """Frames and targets rendered on the device. Each scene gets `variants`
     renders at each input sample count in `spps`, so image s * V + j
     (V = len(spps) * variants) is render j of scene s; target s is scene s at
@@ -185,9 +195,10 @@ def render_multi(gpu: Gpu, seeds, crop: int, spps, spp_ref: int, variants: int, 
         rng = np.random.default_rng(seed)
         x0 = int(rng.integers(0, w - crop + 1))
         y0 = int(rng.integers(0, h - crop + 1))
-        tr = Tracer.new(gpu, S.spheres_scene(seed=int(seed)), 4)
+        scene, camera = SCENES[kind]
+        tr = Tracer.new(gpu, scene(seed=int(seed)), 4)
         tr.set_samples(1 << 30)
-        cam = crop_camera(w, h, x0, y0, crop, crop, rr_from=3)
+        cam = crop_camera(w, h, x0, y0, crop, crop, camera=camera, rr_from=3)
         ref = cam.copy()
         ref["stream"] = REF_STREAM
         left = spp_ref
@@ -209,6 +220,69 @@ def render_multi(gpu: Gpu, seeds, crop: int, spps, spp_ref: int, variants: int, 
         meta.append({"seed": int(seed), "x0": x0, "y0": y0})
         log(f"[kpcn] scene {si + 1}/{len(seeds)} seed {seed} crop ({x0},{y0}) in {time.perf_counter() - t0:.1f} s")
     return frames, targets, meta
+
+
+class LearnedFrame:
+    """Frames of one scene made by the learned denoiser under its held-out
+    check. Two tracers hold the two halves of every frame (the same samples
+    a pixel on other streams); `render` denoises half A, estimates that
+    image's relMSE from half B, and takes twice the samples while the
+    estimate is over `bound`. The answer is the network's output from both
+    halves together, or, once `max_spp` samples would be passed, the plain
+    render of every sample taken."""
+
+    def __init__(self, gpu: Gpu, tris: np.ndarray, net: Kpcn, leaf: int = 4):
+        self.gpu, self.net = gpu, net
+        self.a, self.b = Tracer.new(gpu, tris, leaf), Tracer.new(gpu, tris, leaf)
+        for t in (self.a, self.b):
+            t.set_samples(1 << 30)
+        self._size = None
+
+    def _buffers(self, w: int, h: int) -> None:
+        if self._size == (w, h):
+            return
+        n = w * h
+        self.fa = GpuArray.zeroed(self.gpu, np.float32, n * Kpcn.FRAME)
+        self.fb = GpuArray.zeroed(self.gpu, np.float32, n * Kpcn.FRAME)
+        self.fm = GpuArray.zeroed(self.gpu, np.float32, n * Kpcn.FRAME)
+        self.da = GpuArray.zeroed(self.gpu, np.float32, n * 3)
+        self.out = GpuArray.zeroed(self.gpu, np.float32, n * 3)
+        self._size = (w, h)
+
+    def render(self, cam: np.ndarray, w: int, h: int, spp: int = 4, bound: float = 0.01, max_spp: int = 256) -> dict:
+        """Render, check and denoise; the image is left in `self.out`
+        (3 floats a pixel). Returns what was done: samples taken, each
+        check's estimate, whether the plain render was the answer."""
+        self._buffers(w, h)
+        n = w * h
+        half = max(spp // 2, 1)
+        ca, cb = cam.copy(), cam.copy()
+        cb["stream"] = int(cam["stream"][0]) + HALF_STREAM
+        self.a.sample(ca, w, h, half)
+        self.b.sample(cb, w, h, half)
+        taken, checks = half, []
+        while True:
+            self.a.frame_into(self.fa)
+            self.b.frame_into(self.fb)
+            self.net.infer(self.fa, 1, w, h, self.da)
+            err, noise = Kpcn.held_out(self.gpu, self.da, self.fa, self.fb, n)
+            checks.append({"spp": 2 * taken, "estimate": err, "half_noise": noise})
+            if err <= bound:
+                Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.fm)
+                self.net.infer(self.fm, 1, w, h, self.out)
+                return {"spp": 2 * taken, "checks": checks, "plain": False}
+            if 4 * taken > max_spp:
+                break
+            self.a.sample(ca, w, h, taken)  # the same view: the halves go on accumulating
+            self.b.sample(cb, w, h, taken)
+            taken *= 2
+        Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.out, channels=3)
+        self.gpu.wait()
+        return {"spp": 2 * taken, "checks": checks, "plain": True}
+
+    def free(self) -> None:
+        self.a.free()
+        self.b.free()
 
 
 if __name__ == "__main__":

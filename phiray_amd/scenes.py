@@ -129,6 +129,26 @@ def tris_from_corners(corners: np.ndarray, color=(0.5, 0.5, 0.5, 1.0)) -> np.nda
     return t
 
 
+def cloud_scene(seed: int, n: int = 65536) -> np.ndarray:
+    """The learned denoiser's scenes: the LBVH workload's triangles (n of
+    them, seed `seed`: centroids uniform in the unit cube, corners within
+    0.01) as Lambertian triangles of albedo U[0.2, 0.8]^3 each, drawn from
+    the stream that follows the geometry's draws."""
+    corners, _ = lbvh_scene(n, seed)
+    t = tris_from_corners(corners)
+    a = uniform(seed, 3 * n, start=21 * n).reshape(n, 3) * F32(0.6) + F32(0.2)
+    t["color"][:, :3] = a
+    return t
+
+
+def cloud_camera(w: int, h: int, **kw) -> np.ndarray:
+    """The view of cloud_scene: from (0.5, 0.5, -1.5) towards the cube's
+    centre, 45 degrees, lit by a white world of radiance 1 (no emitter)."""
+    cam = path_camera(w, h, eye=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), **kw)
+    cam["env"] = (1.0, 1.0, 1.0, 0.0)
+    return cam
+
+
 def corner_boxes(corners: np.ndarray) -> np.ndarray:
     return of_tri(corners[:, 0], corners[:, 1], corners[:, 2])
 

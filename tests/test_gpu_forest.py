@@ -158,8 +158,10 @@ def test_forest_path_matches_the_twin(gpu, scene, spp):
     for f in range(spp):
         fo.sample(cam, w, h, want, f, 8)
     n = w * h
-    for waves in (2, 3, 4):  # the forest kernel's register fits (4: mesh trees at most 24 deep)
+    for waves, stack in ((2, 32), (3, 32), (3, 24), (4, 24)):  # register fits and mesh-tree stacks
         t.set_path_waves(waves)
+        t.set_path_stack(stack)
+        assert t.path_stack == stack
         t.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next call starts over
         t.sample(cam, w, h, spp)
         got = t.px()

@@ -147,3 +147,19 @@ def test_build_compact_depth_cap(gpu, orc):
     m = orc.lib().orc_build_compact(orc.ptr(prims), n, 1, orc.ptr(want), orc.ptr(order), orc.ptr(d))
     assert int(d[0]) == 31 and len(got) == m
     assert np.array_equal(got.view(np.uint32), want[:m].view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [5000, 1_310_724])
+def test_every_sort_tile_gives_the_same_tree(gpu, orc, n):
+    print(f"[lbvh] sort tiles of 2,048 and 4,096 keys, n={n}", flush=True)
+    p = S.xorshift_prims_gpu_test(n, 0.0)
+    nodes, order, depth = orc.build_bvh(p, 4, P.NODE, 8)
+    g = P.GpuBvh.build(gpu, p, 4)
+    for items in (16, 8):
+        g.set_sort_tile(items)
+        g.rebuild()
+        same(g.download(), nodes, order, depth, f"tile {items} n={n}")
+    items, ms = g.autotune(2)
+    assert items in (8, 16) and g.sort_tile == items and min(ms) > 0
+    g.rebuild()
+    same(g.download(), nodes, order, depth, f"autotuned n={n}")
