@@ -215,12 +215,12 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
             "rays_per_frame": rays, "path_waves_per_simd": waves, "mesh_stack_entries": tr.path_stack}
 
 
-def xpbd_rates(gpu: P.Gpu, frames: int = 30, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
-    """The rigid-body workload driven by the XPBD World: 1,024 bodies (boxes,
-    and icospheres colliding as spheres) dropped from heights U[1, 10] m onto
-    a plane. Every substep of a 1/60 s frame steps the world, refits every
-    mesh tree and rebuilds the top tree over the new poses; every frame is
-    traced at w x h, spp samples."""
+def xpbd_rates(gpu: P.Gpu, frames: int = 120, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
+    """The rigid-body workload driven by the device XPBD World: 1,024 bodies
+    (boxes, and icospheres colliding as spheres) dropped from heights
+    U[1, 10] m onto a plane, 120 frames. Every substep of a 1/60 s frame
+    steps the world, refits every mesh tree and rebuilds the top tree over
+    the new poses; every frame is traced at w x h, spp samples."""
     heights = (1.0, 10.0)
     b, _ = S.bodies_scene(1024, heights=heights)
     v, f, m, me, ins = b.arrays()

@@ -508,6 +508,17 @@ int phr_kpcn_train_indexed(phr_kpcn* k, phr_array* frames, const uint32_t* fidx,
  * Replays are refused (PHR_ERR_ARG) once the scene, image or network moved. */
 int phr_tracer_record_frame(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, uint32_t spp, int denoise, phr_kpcn* k);
 int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k);
+/* No counterpart in the reference (the frame of src/trace/pose.rs:77-155 and
+ * sample.rs as one recorded graph): a forest's new pose — vertices from the
+ * device array `verts` when not null (refit of the mesh trees), n matrices
+ * (the top tree, instance and emitter tables) — then the camera's stamp, the
+ * path launch and the denoiser (0 none, 1 a-trous, 2 learned with k), all in
+ * one hipGraph. The emitter table must be one the device builds (1 to 4,096
+ * candidates). Replay with new matrices (and whatever `verts` then holds);
+ * the next sample call starts the image over. */
+int phr_tracer_record_pose_frame(phr_tracer* t, phr_array* verts, const float* worlds, uint32_t n, const void* cam,
+                                 uint32_t w, uint32_t h, uint32_t spp, int denoise, phr_kpcn* k);
+int phr_tracer_replay_pose_frame(phr_tracer* t, const float* worlds, uint32_t n, phr_kpcn* k);
 /* The tracer's frame (10 floats a pixel) at float offset `at` of `out`. */
 int phr_tracer_frame(phr_tracer* t, phr_array* out, uint64_t at);
 /* The learned denoiser over what has been traced: into `out` (w * h * 3

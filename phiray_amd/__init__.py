@@ -281,6 +281,8 @@ _SIGS = {
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
     "phr_tracer_record_frame": (_I, [_P, _P, _U32, _U32, _U32, _I, _P]),
     "phr_tracer_replay_frame": (_I, [_P, _P]),
+    "phr_tracer_record_pose_frame": (_I, [_P, _P, _P, _U32, _P, _U32, _U32, _U32, _I, _P]),
+    "phr_tracer_replay_pose_frame": (_I, [_P, _P, _U32, _P]),
     "phr_world_new": (_I, [_P, C.POINTER(_P)]),
     "phr_world_free": (None, [_P]),
     "phr_world_configure": (_I, [_P, _U32, _U32, C.c_float, C.c_float, C.c_float, _P]),
@@ -1068,6 +1070,25 @@ class Tracer:
 
     def replay_frame(self, kpcn: "Kpcn | None" = None) -> None:
         _check(lib().phr_tracer_replay_frame(self._h, kpcn._h if kpcn else None), "Tracer replay frame")
+
+    def record_pose_frame(self, verts: "GpuArray | None", worlds: np.ndarray, cam: np.ndarray, w: int, h: int, spp: int,
+                          denoise: int = 0, kpcn: "Kpcn | None" = None) -> None:
+        """Capture a forest's whole frame as one hipGraph: the new pose
+        (vertices from `verts` when given, then the matrices), the camera's
+        stamp, the path launch and the denoiser."""
+        worlds = np.ascontiguousarray(worlds, dtype=np.float32)
+        cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
+        _check(lib().phr_tracer_record_pose_frame(self._h, verts._h if verts is not None else None, _ptr(worlds),
+                                                   len(worlds.reshape(-1, 16)), _ptr(cam), w, h, spp, denoise,
+                                                   kpcn._h if kpcn else None), "Tracer record pose frame")
+        self.size = (max(w, 1), max(h, 1))
+
+    def replay_pose_frame(self, worlds: np.ndarray, kpcn: "Kpcn | None" = None) -> None:
+        """The recorded posed frame under new matrices (the vertices are read
+        again from the array given at recording)."""
+        worlds = np.ascontiguousarray(worlds, dtype=np.float32)
+        _check(lib().phr_tracer_replay_pose_frame(self._h, _ptr(worlds), len(worlds.reshape(-1, 16)),
+                                                  kpcn._h if kpcn else None), "Tracer replay pose frame")
 
     def frame_into(self, out: GpuArray, at: int = 0) -> None:
         """The denoiser's input frame (10 floats a pixel) at float `at` of `out`."""

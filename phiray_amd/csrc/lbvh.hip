@@ -336,12 +336,10 @@ __device__ __forceinline__ float recip_extent(float lo, float hi) {
     return e > 0.f ? 1.0f / e : 0.f;  // src/bvh/build.rs:76-80
 }
 
-// seg_of (input index -> segment) is written only when the sort has segment
-// passes (seg_passes > 0).
 __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ prims, const uint32_t* __restrict__ acc,
                                                 const uint32_t* __restrict__ seg_lo, uint32_t segs, uint32_t seg_passes, uint32_t n,
                                                 uint32_t* __restrict__ codes, uint32_t* __restrict__ hist,
-                                                uint32_t* __restrict__ far_ends, uint32_t* __restrict__ seg_of) {
+                                                uint32_t* __restrict__ far_ends) {
     __shared__ uint32_t s_h[MAX_PASSES * 256];
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < MAX_PASSES * 256; i += 256) s_h[i] = 0u;
@@ -379,7 +377,6 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ prims
             atomicAdd(&s_h[512 + ((code >> 16) & 0xFFu)], 1u);
             atomicAdd(&s_h[768 + (code >> 24)], 1u);
             if (seg_passes) {
-                seg_of[i] = s;
                 atomicAdd(&s_h[1024 + (s & 0xFFu)], 1u);
                 if (seg_passes > 1) atomicAdd(&s_h[1280 + ((s >> 8) & 0xFFu)], 1u);
             }
@@ -397,7 +394,7 @@ template <bool FIRST, bool SEGD, uint32_t ITEMS>
 __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                   const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* ticket,
-                                                  uint32_t n, uint32_t shift, const uint32_t* __restrict__ seg_of) {
+                                                  uint32_t n, uint32_t shift, const uint32_t* __restrict__ seg_lo, uint32_t segs) {
     __shared__ uint32_t s_whist[OS_WAVES][256];
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
@@ -419,7 +416,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
         bool ok = idx < n;
         key[r] = ok ? kin[idx] : 0u;
         val[r] = FIRST ? idx : (ok ? vin[idx] : 0u);
-        if constexpr (SEGD) dig[r] = ok ? (seg_of[val[r]] >> shift) & 0xFFu : 0u;
+        // The segment of input index val: a search of the small segment
+        // table (cached), not a gather from an n-word array.
+        if constexpr (SEGD) dig[r] = ok ? (seg_of_slot(seg_lo, segs, val[r]) >> shift) & 0xFFu : 0u;
         else dig[r] = (key[r] >> shift) & 0xFFu;
     }
     const uint64_t lt = lanemask_lt();
@@ -493,11 +492,28 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
 }
 
 // ---- 4. Karras links --------------------------------------------------------
-// delta over the segment's own slots (src/bvh/mod.rs:33-43): i, j local.
-__device__ __forceinline__ int32_t kdelta(const uint32_t* __restrict__ codes, int64_t n, int64_t i, int64_t j) {
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = codes[i], b = codes[j];
-    if (a != b) return (int32_t)__clz(a ^ b);
+// The codes around a block's slots, staged in LDS: most searches stay within
+// KW_HALO slots of their own (a low node's range is short), so their
+// dependent reads are LDS reads; the rest read the codes array.
+constexpr int32_t KW_HALO = 1024;
+constexpr int32_t KW_SIZE = 256 + 2 * KW_HALO;
+
+struct CodeWindow {
+    const uint32_t* g;  // the codes, global slots
+    const uint32_t* s;  // s[k] = g[w0 + k] for k in [0, KW_SIZE)
+    int64_t w0;
+    __device__ __forceinline__ uint32_t at(int64_t slot) const {
+        const int64_t k = slot - w0;
+        return (k >= 0 && k < KW_SIZE) ? s[k] : g[slot];
+    }
+};
+
+// delta (src/bvh/mod.rs:33-43) of local slot i, whose code is ci, against
+// local slot j of the segment starting at global slot lo.
+__device__ __forceinline__ int32_t wdelta(const CodeWindow& w, int64_t lo, int64_t m, int64_t i, uint32_t ci, int64_t j) {
+    if (j < 0 || j >= m) return -1;
+    const uint32_t b = w.at(lo + j);
+    if (ci != b) return (int32_t)__clz(ci ^ b);
     return 32 + (int32_t)__clz((uint32_t)i ^ (uint32_t)j);
 }
 
@@ -505,6 +521,14 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
                                                 uint32_t segs, PhrLink* __restrict__ links, uint4* __restrict__ up,
                                                 uint32_t* __restrict__ leaf_at, uint32_t* __restrict__ own_end,
                                                 uint32_t* __restrict__ far_ends) {
+    __shared__ uint32_t s_codes[KW_SIZE];
+    const int64_t w0 = (int64_t)blockIdx.x * 256 - KW_HALO;
+    for (int32_t k = threadIdx.x; k < KW_SIZE; k += 256) {
+        const int64_t g = w0 + k;
+        s_codes[k] = (g >= 0 && g < (int64_t)n) ? codes[g] : 0u;
+    }
+    __syncthreads();
+    const CodeWindow win{codes, s_codes, w0};
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     leaf_at[p] = PHR_NONE;
@@ -515,24 +539,25 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
         own_end[p] = 0u;  // the last slot of a segment starts no inner node
         return;
     }
-    const uint32_t* c = codes + lo;
     const int64_t m = ns, i = k;
-    int64_t d = (kdelta(c, m, i, i + 1) - kdelta(c, m, i, i - 1) > 0) ? 1 : -1;
-    int32_t dmin = kdelta(c, m, i, i - d);
+    const uint32_t ci = win.at((int64_t)p);
+    auto delta = [&](int64_t j) { return wdelta(win, lo, m, i, ci, j); };
+    int64_t d = (delta(i + 1) - delta(i - 1) > 0) ? 1 : -1;
+    int32_t dmin = delta(i - d);
     int64_t lmax = 2;
-    while (kdelta(c, m, i, i + lmax * d) > dmin) lmax *= 2;
+    while (delta(i + lmax * d) > dmin) lmax *= 2;
     int64_t l = 0, t = lmax / 2;
     while (t >= 1) {
-        if (kdelta(c, m, i, i + (l + t) * d) > dmin) l += t;
+        if (delta(i + (l + t) * d) > dmin) l += t;
         t /= 2;
     }
     int64_t j = i + l * d;
-    int32_t dnode = kdelta(c, m, i, j);
+    int32_t dnode = delta(j);
     int64_t s = 0;
     t = l;
     for (;;) {
         t = (t + 1) >> 1;
-        if (kdelta(c, m, i, i + (s + t) * d) > dnode) s += t;
+        if (delta(i + (s + t) * d) > dnode) s += t;
         if (t <= 1) break;
     }
     int64_t g = i + s * d + (d < 0 ? d : 0);
@@ -632,10 +657,14 @@ __global__ __launch_bounds__(256) void k_stab(const uint32_t* __restrict__ own_e
 // larger of the coverings of edges k - 1 and k (an edge across a segment's end
 // is covered by nothing). Deeper than 31 means an ancestor at depth 31 was
 // packed (d + 1 >= STACK), so the node is unreachable.
+// With `inner` (one byte a node of the node array, or null) each inner
+// node's byte says whether a walk reaches it as an inner node (not packed,
+// not under a packed leaf): the trees' walk records are made for those.
 __global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links, const uint4* __restrict__ up,
                                               const uint32_t* __restrict__ stab, uint32_t n, const uint32_t* __restrict__ seg_lo,
-                                              uint32_t segs, uint32_t leaf, uint32_t* __restrict__ leaf_at,
-                                              uint32_t* __restrict__ depth_out, uint32_t* __restrict__ acc) {
+                                              const uint32_t* __restrict__ node_base, uint32_t segs, uint32_t leaf,
+                                              uint32_t* __restrict__ leaf_at, uint32_t* __restrict__ depth_out,
+                                              uint32_t* __restrict__ acc, uint8_t* __restrict__ inner_out) {
     __shared__ uint32_t s_seg[4], s_depth[4];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t j = t; j < 6 * segs; j += gridDim.x * blockDim.x) acc[j] = (j % 6) < 3 ? KEY_MAX : KEY_MIN;
@@ -670,6 +699,10 @@ __global__ __launch_bounds__(256) void k_pack(const PhrLink* __restrict__ links,
                 listed = inner ? ((l.last - l.first < leaf) || (d + 1 >= PHR_STACK)) : true;
                 if (listed) leaf_at[first] = t;
                 depth = d;
+            }
+            if (inner && inner_out) {
+                const Seg g = seg_at(seg_lo, node_base, sg);
+                inner_out[out_of(t, n, g)] = (d < PHR_STACK && !muted && !listed) ? 1u : 0u;
             }
         }
     }
@@ -985,7 +1018,6 @@ struct phr_bvh {
     bool own_nodes = true;
     uint32_t* seg_lo = nullptr;     // segs + 1
     uint32_t* node_base = nullptr;  // segs
-    uint32_t* seg_of = nullptr;     // n, only with segment passes
     uint32_t* codes = nullptr;      // n: sorted codes after the last pass
     uint32_t* keys_b = nullptr;
     uint32_t* vals_b = nullptr;
@@ -1002,6 +1034,7 @@ struct phr_bvh {
     uint32_t* front[2] = {nullptr, nullptr};   // n each: nodes handed from one refit level to the next
     uint32_t* fcount[2] = {nullptr, nullptr};  // per block of a level: how many it handed over
     uint32_t level0_blocks = 0;
+    uint8_t* inner = nullptr;  // a byte a node of the node array: an inner node a walk reaches
 };
 
 namespace {
@@ -1011,10 +1044,10 @@ void bvh_release(phr_bvh* b) {
     if (b->own_prims) phr_array_release(b->prims);
     phr_array_release(b->order);
     if (b->own_nodes) phr_array_release(b->nodes);
-    for (void* p : {(void*)b->seg_lo, (void*)b->node_base, (void*)b->seg_of, (void*)b->codes, (void*)b->keys_b, (void*)b->vals_b,
+    for (void* p : {(void*)b->seg_lo, (void*)b->node_base, (void*)b->codes, (void*)b->keys_b, (void*)b->vals_b,
                     (void*)b->zero, (void*)b->acc, b->parts, (void*)b->links, (void*)b->up, (void*)b->cnt, (void*)b->leaf_at,
                     (void*)b->own_end, (void*)b->far_ends, (void*)b->front[0], (void*)b->front[1], (void*)b->fcount[0],
-                    (void*)b->fcount[1]})
+                    (void*)b->fcount[1], (void*)b->inner})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -1085,13 +1118,14 @@ int bvh_alloc(phr_gpu* g, phr_array* prims, bool own, uint32_t leaf, const uint3
     };
     b->level0_blocks = (uint32_t)((nn + RF_SLOTS - 1) / RF_SLOTS);
     bool ok = dev((void**)&b->seg_lo, 4ull * (segs + 1)) && dev((void**)&b->node_base, 4ull * segs) &&
-              (b->seg_passes == 0 || dev((void**)&b->seg_of, 4 * nn)) && dev((void**)&b->codes, 4 * nn) &&
+              dev((void**)&b->codes, 4 * nn) &&
               dev((void**)&b->keys_b, 4 * nn) && dev((void**)&b->vals_b, 4 * nn) && dev((void**)&b->zero, 4ull * b->zero_words) &&
               dev((void**)&b->acc, 24ull * segs) && dev(&b->parts, 32ull * b->bounds_blocks) &&
               dev((void**)&b->links, 16 * nn) && dev((void**)&b->up, 32 * nn) &&
               dev((void**)&b->cnt, 4 * nn) && dev((void**)&b->leaf_at, 4 * nn) && dev((void**)&b->own_end, 4 * nn) &&
               dev((void**)&b->far_ends, 4 * nn) && dev((void**)&b->front[0], 4 * nn) && dev((void**)&b->front[1], 4 * nn) &&
-              dev((void**)&b->fcount[0], 4ull * b->level0_blocks) && dev((void**)&b->fcount[1], 4ull * b->level0_blocks);
+              dev((void**)&b->fcount[0], 4ull * b->level0_blocks) && dev((void**)&b->fcount[1], 4ull * b->level0_blocks) &&
+              dev((void**)&b->inner, std::max<uint64_t>(b->nodes->len, 1));
     // Aabb::default() as keys for every segment: fkey(f32::MAX) x 3, fkey(f32::MIN) x 3.
     std::vector<uint32_t> acc0(6ull * segs);
     for (size_t j = 0; j < acc0.size(); ++j) acc0[j] = (j % 6) < 3 ? KEY_MAX : KEY_MIN;
@@ -1114,16 +1148,16 @@ int bvh_enqueue_refit(phr_bvh* b);
 template <uint32_t ITEMS>
 void launch_pass(bool first, bool seg, uint32_t tiles, hipStream_t s, uint32_t* kin, uint32_t* vin, uint32_t* kout,
                  uint32_t* vout, const uint32_t* hist, uint32_t* st, uint32_t* ticket, uint32_t n, uint32_t shift,
-                 const uint32_t* seg_of) {
+                 const uint32_t* seg_lo, uint32_t segs) {
     if (first)
         hipLaunchKernelGGL((k_onesweep<true, false, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
-                           ticket, n, shift, seg_of);
+                           ticket, n, shift, seg_lo, segs);
     else if (!seg)
         hipLaunchKernelGGL((k_onesweep<false, false, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
-                           ticket, n, shift, seg_of);
+                           ticket, n, shift, seg_lo, segs);
     else
         hipLaunchKernelGGL((k_onesweep<false, true, ITEMS>), dim3(tiles), dim3(OS_THREADS), 0, s, kin, vin, kout, vout, hist, st,
-                           ticket, n, shift, seg_of);
+                           ticket, n, shift, seg_lo, segs);
 }
 
 // Enqueue the whole build on the device's stream (no host synchronisation).
@@ -1144,7 +1178,7 @@ int bvh_enqueue_build(phr_bvh* b) {
     // must end in (codes, order): with an odd count the codes start in keys_b.
     uint32_t* unsorted = (passes % 2) ? b->keys_b : b->codes;
     hipLaunchKernelGGL(k_morton, dim3(b->tiles), dim3(256), 0, s, prims, (const uint32_t*)b->acc, b->seg_lo, b->segs, b->seg_passes,
-                       n, unsorted, hist, b->far_ends, b->seg_of);
+                       n, unsorted, hist, b->far_ends);
     uint32_t* kin = unsorted;
     uint32_t* vin = nullptr;
     // The status words are sized for 2048-key tiles; a larger tile uses fewer.
@@ -1157,10 +1191,11 @@ int bvh_enqueue_build(phr_bvh* b) {
         const uint32_t shift = seg ? 8 * (p - CODE_PASSES) : 8 * p;
         uint32_t* st = status + (size_t)p * sort_tiles * 256;
         if (b->items == 16)
-            launch_pass<16>(p == 0, seg, sort_tiles, s, kin, vin, kout, vout, hist + 256 * p, st, ticket + p, n, shift, b->seg_of);
+            launch_pass<16>(p == 0, seg, sort_tiles, s, kin, vin, kout, vout, hist + 256 * p, st, ticket + p, n, shift, b->seg_lo,
+                            b->segs);
         else
             launch_pass<OS_ITEMS>(p == 0, seg, sort_tiles, s, kin, vin, kout, vout, hist + 256 * p, st, ticket + p, n, shift,
-                                  b->seg_of);
+                                  b->seg_lo, b->segs);
         kin = kout;
         vin = vout;
     }
@@ -1170,8 +1205,8 @@ int bvh_enqueue_build(phr_bvh* b) {
     uint32_t* stab = b->keys_b;
     hipLaunchKernelGGL(k_stab, dim3((n + SC_TILE - 1) / SC_TILE), dim3(256), 0, s, b->own_end, b->far_ends, n, stab,
                        (uint64_t*)(b->zero + b->scan_status), b->zero + Z_SCAN_TICKET);
-    hipLaunchKernelGGL(k_pack, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->up, stab, n, b->seg_lo, b->segs,
-                       b->leaf, b->leaf_at, b->zero + Z_DEPTH, b->acc);
+    hipLaunchKernelGGL(k_pack, dim3((2 * n + 255) / 256), dim3(256), 0, s, b->links, b->up, stab, n, b->seg_lo, b->node_base,
+                       b->segs, b->leaf, b->leaf_at, b->zero + Z_DEPTH, b->acc, b->inner);
     PHR_HIP(hipGetLastError());
     return bvh_enqueue_refit(b);
 }
@@ -1264,6 +1299,7 @@ int phr_bvh_internal_refit_async(phr_bvh* b) { return bvh_enqueue_refit(b); }
 int phr_bvh_internal_read_depth(phr_bvh* b) { return bvh_read_depth(b); }
 phr_array* phr_bvh_internal_nodes(phr_bvh* b) { return b->nodes; }
 phr_array* phr_bvh_internal_order(phr_bvh* b) { return b->order; }
+const uint8_t* phr_bvh_internal_inner(phr_bvh* b) { return b->inner; }
 uint32_t phr_bvh_internal_segment_depth(const phr_bvh* b, uint32_t s) { return s < b->segs ? b->depth_h[s] : 0; }
 
 extern "C" {
@@ -1757,10 +1793,10 @@ int phr_sort_run(phr_sort* s) {
         uint32_t* stp = status + (size_t)p * s->tiles * 256;
         if (p == 0)
             hipLaunchKernelGGL((k_onesweep<true, false, OS_ITEMS>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout, hist, stp,
-                               ticket, n, 0u, (const uint32_t*)nullptr);
+                               ticket, n, 0u, (const uint32_t*)nullptr, 0u);
         else
             hipLaunchKernelGGL((k_onesweep<false, false, OS_ITEMS>), dim3(s->tiles), dim3(OS_THREADS), 0, st, kin, vin, kout, vout,
-                               hist + 256 * p, stp, ticket + p, n, 8 * p, (const uint32_t*)nullptr);
+                               hist + 256 * p, stp, ticket + p, n, 8 * p, (const uint32_t*)nullptr, 0u);
         kin = kout;
         vin = vout;
     }

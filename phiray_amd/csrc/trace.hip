@@ -37,6 +37,7 @@ constexpr uint32_t PHR_INST = 1u << 30;  // a top-tree leaf's range is instance 
 
 struct SceneDev {
     const float4* nodes;
+    const float4* recs;    // one tree's walk records (4 float4 a node index); a forest walks `nodes`
     const float4* tris;    // 3 float4 per slot: (a.xyz, b.x) (b.yz, c.xy) (c.z, -, -, -)
     const uint4* faces;    // slot order: 3 vertex ids + material
     const float4* verts;   // 2 float4 per vertex: (pos, 1) (oct.xy, u, v)
@@ -116,28 +117,42 @@ __device__ __forceinline__ float box_t(float4 lo, float4 hi, V3 o, V3 inv, float
     return (tmax >= e && tmin < best) ? e : PHR_INF;
 }
 
-// trace_blas for one identity instance, root at node `root`. `stk` is this
-// thread's LDS stack column (stride = threads per block).
+// trace_blas for one identity instance, root at node `root`, over the tree's
+// walk records: record x (node x an inner node) holds x's two children side
+// by side as 32-byte nodes, an inner child's words replaced by (its own node
+// index, 0) and a leaf child's kept (first | LEAF, end). A step reads one
+// aligned 64-byte record and tests both boxes; the stack holds the far
+// child's first word, so a pop needs no load of its own. A leaf taken from
+// the stack learns its end from the cache row of its first triangle (row 2,
+// word 1). The visits, and so the hits and counts, are those of the node
+// walk: both children tested, the nearer entered first (a tie to the left),
+// the farther pushed. `stk` is this thread's LDS stack column (stride =
+// threads per block).
 // Any-hit rays skip triangles of glass or cut-out materials (trans > 0.5 or
 // alpha < 0.5, bvh.wgsl:158-160): `faces` and `mats` give the material.
 // `steps` and `tests` count inner-node visits and triangle tests, the units
 // the walk's memory traffic is priced in.
 template <bool ANY>
-__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, uint32_t root, const float4* __restrict__ tris,
-                                       const uint4* __restrict__ faces, const float* __restrict__ mats, V3 o, V3 d, float tmax,
-                                       uint32_t* stk, uint32_t stride, uint32_t& steps, uint32_t& tests) {
+__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const float4* __restrict__ recs, uint32_t root,
+                                       const float4* __restrict__ tris, const uint4* __restrict__ faces, const float* __restrict__ mats,
+                                       V3 o, V3 d, float tmax, uint32_t* stk, uint32_t stride, uint32_t& steps, uint32_t& tests) {
     PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
     V3 inv = safe_inv(d);
-    float4 lo = nodes[2 * root], hi = nodes[2 * root + 1];
-    if (box_t(lo, hi, o, inv, best.t) >= PHR_INF) return best;
+    const float4 rlo = nodes[2 * root], rhi = nodes[2 * root + 1];
+    if (box_t(rlo, rhi, o, inv, best.t) >= PHR_INF) return best;
     RaySpace rs = ray_space(d);
     uint32_t sp = 0;
+    // The entry in hand: an inner node's index (its record), or a leaf's
+    // first | LEAF with its end (PHR_NONE when it came off the stack).
+    uint32_t cur = __float_as_uint(rlo.w), end = __float_as_uint(rhi.w);
+    if (!(cur & PHR_LEAF)) cur = root;
     for (;;) {
-        uint32_t li = __float_as_uint(lo.w), ri = __float_as_uint(hi.w);
-        if (li & PHR_LEAF) {
-            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
+        if (cur & PHR_LEAF) {
+            uint32_t id = cur & ~PHR_LEAF;
+            do {
                 ++tests;
                 float4 t0 = tris[3 * id], t1 = tris[3 * id + 1], t2 = tris[3 * id + 2];
+                if (end == PHR_NONE) end = __float_as_uint(t2.y);
                 float u, v;
                 float t = tri_t(t0, t1, t2, o, rs, u, v);
                 if (t < best.t) {
@@ -148,29 +163,58 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, uint32_
                     }
                     best = PhrHit{t, id, u, v, 0u};
                 }
-            }
+            } while (++id < end);
         } else {
             ++steps;
-            float4 llo = nodes[2 * li], lhi = nodes[2 * li + 1];
-            float4 rlo = nodes[2 * ri], rhi = nodes[2 * ri + 1];
+            const float4* r = recs + 4 * (size_t)cur;
+            float4 llo = r[0], lhi = r[1], qlo = r[2], qhi = r[3];
             float tl = box_t(llo, lhi, o, inv, best.t);
-            float tr = box_t(rlo, rhi, o, inv, best.t);
+            float tr = box_t(qlo, qhi, o, inv, best.t);
             if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) { stk[sp * stride] = ri; lo = llo; hi = lhi; }
-                else { stk[sp * stride] = li; lo = rlo; hi = rhi; }
+                if (tl <= tr) {
+                    stk[sp * stride] = __float_as_uint(qlo.w);
+                    cur = __float_as_uint(llo.w);
+                    end = __float_as_uint(lhi.w);
+                } else {
+                    stk[sp * stride] = __float_as_uint(llo.w);
+                    cur = __float_as_uint(qlo.w);
+                    end = __float_as_uint(qhi.w);
+                }
                 ++sp;
                 continue;
             }
-            if (tl < PHR_INF) { lo = llo; hi = lhi; continue; }
-            if (tr < PHR_INF) { lo = rlo; hi = rhi; continue; }
+            if (tl < PHR_INF) { cur = __float_as_uint(llo.w); end = __float_as_uint(lhi.w); continue; }
+            if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); continue; }
         }
         if (sp == 0) break;
         --sp;
-        uint32_t x = stk[sp * stride];
-        lo = nodes[2 * x];
-        hi = nodes[2 * x + 1];
+        cur = stk[sp * stride];
+        end = PHR_NONE;
     }
     return best;
+}
+
+// The walk records of a tree's nodes (one entry per node index; only inner
+// nodes a walk reaches get one, as `inner` marks them), and each reachable
+// leaf's end in the cache row of its first triangle.
+__global__ __launch_bounds__(256) void k_walk_records(const float4* __restrict__ nodes, const uint8_t* __restrict__ inner,
+                                                      uint32_t count, float4* __restrict__ recs, float4* __restrict__ tris) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= count || !inner[x]) return;
+    const uint32_t kid[2] = {__float_as_uint(nodes[2 * (size_t)x].w), __float_as_uint(nodes[2 * (size_t)x + 1].w)};
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t y = kid[c];
+        float4 lo = nodes[2 * (size_t)y], hi = nodes[2 * (size_t)y + 1];
+        const uint32_t w0 = __float_as_uint(lo.w);
+        if (w0 & PHR_LEAF) {
+            tris[3 * (size_t)(w0 & ~PHR_LEAF) + 2].y = hi.w;  // the leaf's end
+        } else {
+            lo.w = __uint_as_float(y);
+            hi.w = 0.f;
+        }
+        recs[4 * (size_t)x + 2 * c] = lo;
+        recs[4 * (size_t)x + 2 * c + 1] = hi;
+    }
 }
 
 // An instance's row times (x, y, z, w), summed in trace_blas's order.
@@ -331,8 +375,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const Ph
             h = any ? walk_forest<true>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests)
                     : walk_forest<false>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests);
         else
-            h = any ? walk<true>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
-                    : walk<false>(sc.nodes, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+            h = any ? walk<true>(sc.nodes, sc.recs, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
+                    : walk<false>(sc.nodes, sc.recs, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
         hits[i] = h;
     }
     if (counters) {
@@ -399,11 +443,13 @@ __device__ __forceinline__ PhrHit trace(Ctx& c, V3 o, V3 d, float tmax, bool fir
     if (first) {
         c.shadow++;
         if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
-        return walk<true>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+        return walk<true>(c.s.nodes, c.s.recs, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
+                          c.tests);
     }
     c.closest++;
     if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
-    return walk<false>(c.s.nodes, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps, c.tests);
+    return walk<false>(c.s.nodes, c.s.recs, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
+                       c.tests);
 }
 template <bool F>
 __device__ __forceinline__ float visible(Ctx& c, V3 org, V3 l, float dist) {
@@ -1521,6 +1567,23 @@ uint32_t phr_bvh_internal_segment_depth(const phr_bvh* b, uint32_t s);
 int phr_bvh_internal_refit_async(phr_bvh* b);
 phr_array* phr_bvh_internal_nodes(phr_bvh* b);
 phr_array* phr_bvh_internal_order(phr_bvh* b);
+const uint8_t* phr_bvh_internal_inner(phr_bvh* b);
+
+// The walk records of a built tree (after its gather wrote the triangle
+// cache), into `recs`, allocated on first use.
+static int enqueue_walk_records(phr_gpu* g, phr_bvh* bvh, phr_array** recs, phr_array* tris) {
+    phr_array* nodes = phr_bvh_internal_nodes(bvh);
+    const uint64_t count = nodes->len;
+    if (!*recs) {
+        int rc = phr_array_alloc(g, 64, std::max<uint64_t>(count, 1), recs);
+        if (rc != PHR_OK) return rc;
+    }
+    if (count)
+        hipLaunchKernelGGL(k_walk_records, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, g->stream, (const float4*)nodes->ptr,
+                           phr_bvh_internal_inner(bvh), (uint32_t)count, (float4*)(*recs)->ptr, (float4*)tris->ptr);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
 extern "C" void phr_bvh_free(phr_bvh* b);
 extern "C" int phr_bvh_refit_async(phr_bvh* b);
 extern "C" uint32_t phr_bvh_depth(const phr_bvh* b);
@@ -1539,6 +1602,7 @@ struct phr_tracer {
     phr_array* prims = nullptr;     // Aabb per input face
     phr_array* faces = nullptr;     // uint4 per slot
     phr_array* tris = nullptr;      // 3 float4 per slot
+    phr_array* recs = nullptr;      // one tree's walk records: 64 B a node index
     phr_array* lights = nullptr;    // [count, ids, insts, cdf]
     phr_array* px = nullptr;        // PX_LEN * w * h + 1 words
     unsigned long long* rays = nullptr;  // closest, shadow
@@ -1608,6 +1672,12 @@ struct phr_tracer {
     SceneDev fsc{};
     const void* fsig[4] = {nullptr, nullptr, nullptr, nullptr};
     int fdenoise = 0;  // the recorded frame's denoiser: 0 none, 1 a-trous, 2 learned
+    // A recorded posed frame: the pose's copies read these pinned buffers, so
+    // a replay takes new matrices without recording again.
+    float* pin_worlds = nullptr;      // 16 floats an instance
+    uint32_t* pin_words = nullptr;    // the instance table as uploaded
+    bool fposed = false;              // the recorded frame applies a pose first
+    const void* fverts = nullptr;     // its vertex source (null: matrices only)
 };
 
 namespace {
@@ -1659,14 +1729,16 @@ void tracer_release(phr_tracer* t) {
     if (!t) return;
     if (t->bvh) phr_bvh_free(t->bvh);
     if (t->tlas) phr_bvh_free(t->tlas);
-    for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->lights, t->px, t->nodes, t->wboxes, t->insts,
-                         t->placed_dev, t->worlds_dev, t->inst_mesh, t->lcand, t->lpow})
+    for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->recs, t->lights, t->px, t->nodes, t->wboxes,
+                         t->insts, t->placed_dev, t->worlds_dev, t->inst_mesh, t->lcand, t->lpow})
         phr_array_release(a);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->frame_dev) (void)hipFree(t->frame_dev);
     if (t->dn_scratch) (void)hipFree(t->dn_scratch);
     if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
     if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
+    if (t->pin_worlds) (void)hipHostFree(t->pin_worlds);
+    if (t->pin_words) (void)hipHostFree(t->pin_words);
     if (t->rays) (void)hipFree(t->rays);
     if (t->cam_dev) (void)hipFree(t->cam_dev);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
@@ -1729,6 +1801,7 @@ int build_lights(phr_tracer* t, const std::vector<uint32_t>& in_faces, const std
 SceneDev scene_dev(phr_tracer* t) {
     SceneDev s;
     s.nodes = (const float4*)(t->forest ? t->nodes : phr_bvh_internal_nodes(t->bvh))->ptr;
+    s.recs = t->forest ? nullptr : (const float4*)t->recs->ptr;
     s.insts = t->forest ? (const uint32_t*)t->insts->ptr : nullptr;
     s.ninsts = t->ninsts;
     s.tris = (const float4*)t->tris->ptr;
@@ -1950,6 +2023,24 @@ int forest_lights(phr_tracer* t) {
     return PHR_OK;
 }
 
+// A recorded posed frame's camera stamp on the device (what the host does
+// at a sample from the values forest_settle reads back): the extent from the
+// top tree's root box, the occlusion distance when the caller left it unset,
+// the ray offset, and the emitter count.
+__global__ void k_stamp_pose(const float4* __restrict__ nodes, const uint32_t* __restrict__ lights, uint32_t glow_on_device,
+                             uint32_t ao_auto, PhrCamera* __restrict__ cam) {
+    if (threadIdx.x != 0) return;
+    const float4 lo = nodes[0], hi = nodes[1];
+    float sq = -0.0f;
+    sq = sq + (hi.x - lo.x) * (hi.x - lo.x);
+    sq = sq + (hi.y - lo.y) * (hi.y - lo.y);
+    sq = sq + (hi.z - lo.z) * (hi.z - lo.z);
+    const float extent = fmaxf(sqrtf(sq), 1e-3f);
+    if (ao_auto) cam->ao = extent * 0.08f;
+    cam->eps = extent * 1e-5f;
+    if (glow_on_device) cam->glow = lights[0];
+}
+
 // What the host keeps of a pose applied on the device: the extent (the top
 // tree's root box, forest.rs:189-199) and the emitter count.
 int forest_settle(phr_tracer* t) {
@@ -1977,25 +2068,14 @@ int forest_settle(phr_tracer* t) {
 // [0, 2p - 1), leaves flagged INST), and the instance table. Nothing waits:
 // the extent (the top tree's root box, forest.rs:189-199) is read back at the
 // next sample.
-int forest_top(phr_tracer* t, bool fresh) {
-    const uint32_t p = (uint32_t)t->placed.size(), n = t->ninsts;
-    hipStream_t s = t->gpu->stream;
+// The host's half of a new pose: the matrices and the instance table staged
+// for upload (into the pinned buffers when a posed frame is recorded, which
+// its copies read). Returns the upload sources.
+void forest_pose_host(phr_tracer* t, const float** wsrc, const uint32_t** isrc) {
+    const uint32_t n = t->ninsts;
     std::vector<float>& wl = t->inst_worlds_h;
     wl.resize(16 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i) std::memcpy(&wl[16 * (size_t)i], t->inst_in[i].world, 64);
-    PHR_HIP(hipMemcpyAsync(t->worlds_dev->ptr, wl.data(), 64ull * n, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_world_boxes, dim3((p + 255) / 256), dim3(256), 0, s, (const float*)t->worlds_dev->ptr,
-                       (const uint32_t*)t->placed_dev->ptr, (const uint32_t*)t->inst_mesh->ptr, t->bases_dev,
-                       (const float4*)t->nodes->ptr, p, (float4*)t->wboxes->ptr);
-    PHR_HIP(hipGetLastError());
-    int rc;
-    if (fresh) {
-        const uint32_t seg[2] = {0, p}, base[1] = {0};
-        if ((rc = phr_bvh_internal_build_into(t->gpu, t->wboxes, 1, seg, 1, t->nodes, base, PHR_LEAF | PHR_INST, &t->tlas)) != PHR_OK)
-            return rc;
-    } else if ((rc = phr_bvh_internal_rebuild_async(t->tlas)) != PHR_OK) {
-        return rc;
-    }
     std::vector<uint32_t>& words = t->words_h;
     words.assign((size_t)(INSTANCE_LEN + 2) * n, 0u);
     phr_mesh unit{0, 0, 0, 0, {0.f, 0.f, 0.f}, {1.f, 1.f, 1.f}};
@@ -2012,7 +2092,42 @@ int forest_top(phr_tracer* t, bool fresh) {
         if (t->inst_in[i].mesh < t->nmeshes) by_mesh[t->inst_in[i].mesh].push_back(i);
     for (uint32_t m = 0; m < t->nmeshes; ++m)
         for (uint32_t i : by_mesh[m]) draw[at++] = i;
-    PHR_HIP(hipMemcpyAsync(t->insts->ptr, words.data(), 4 * words.size(), hipMemcpyHostToDevice, s));
+    *wsrc = wl.data();
+    *isrc = words.data();
+    if (t->pin_worlds) {
+        std::memcpy(t->pin_worlds, wl.data(), 64ull * n);
+        std::memcpy(t->pin_words, words.data(), 4 * words.size());
+        *wsrc = t->pin_worlds;
+        *isrc = t->pin_words;
+    }
+}
+
+// Forest::retlas + inst_words (src/trace/forest.rs:111-127, 168-181): the
+// placed instances' world boxes from their mesh trees' root boxes and the
+// top tree over them, both on the device (the top tree into node slots
+// [0, 2p - 1), leaves flagged INST), and the instance table. Nothing waits:
+// the extent (the top tree's root box, forest.rs:189-199) is read back at the
+// next sample.
+int forest_top(phr_tracer* t, bool fresh) {
+    const uint32_t p = (uint32_t)t->placed.size(), n = t->ninsts;
+    hipStream_t s = t->gpu->stream;
+    const float* wsrc = nullptr;
+    const uint32_t* isrc = nullptr;
+    forest_pose_host(t, &wsrc, &isrc);
+    PHR_HIP(hipMemcpyAsync(t->worlds_dev->ptr, wsrc, 64ull * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_world_boxes, dim3((p + 255) / 256), dim3(256), 0, s, (const float*)t->worlds_dev->ptr,
+                       (const uint32_t*)t->placed_dev->ptr, (const uint32_t*)t->inst_mesh->ptr, t->bases_dev,
+                       (const float4*)t->nodes->ptr, p, (float4*)t->wboxes->ptr);
+    PHR_HIP(hipGetLastError());
+    int rc;
+    if (fresh) {
+        const uint32_t seg[2] = {0, p}, base[1] = {0};
+        if ((rc = phr_bvh_internal_build_into(t->gpu, t->wboxes, 1, seg, 1, t->nodes, base, PHR_LEAF | PHR_INST, &t->tlas)) != PHR_OK)
+            return rc;
+    } else if ((rc = phr_bvh_internal_rebuild_async(t->tlas)) != PHR_OK) {
+        return rc;
+    }
+    PHR_HIP(hipMemcpyAsync(t->insts->ptr, isrc, 4ull * (INSTANCE_LEN + 2) * n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_inst_order, dim3((p + 255) / 256), dim3(256), 0, s, (const uint32_t*)t->placed_dev->ptr,
                        (const uint32_t*)phr_bvh_internal_order(t->tlas)->ptr, p, (uint32_t*)t->insts->ptr + INSTANCE_LEN * (size_t)n);
     PHR_HIP(hipGetLastError());
@@ -2157,6 +2272,7 @@ int phr_tracer_new(phr_gpu* g, const void* tris_v, uint32_t n, uint32_t leaf, ph
     phr_array* order = phr_bvh_internal_order(t->bvh);
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
                        (const float4*)t->verts->ptr, (const uint32_t*)order->ptr, n, (uint4*)t->faces->ptr, (float4*)t->tris->ptr);
+    if ((rc = enqueue_walk_records(g, t->bvh, &t->recs, t->tris)) != PHR_OK) return fail(rc);
     t->order.resize(n);
     if (hipMemcpyAsync(t->order.data(), order->ptr, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -2690,13 +2806,14 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     t->fsig[2] = denoise == 2 ? phr_kpcn_internal_workspace(k) : nullptr;
     t->fsig[3] = t->cam_dev;
     t->fdenoise = denoise;
+    t->fposed = false;
     return PHR_OK;
 }
 
 // Replay the recorded frame: the image holds its samples (denoised as
 // recorded) afterwards, and a later sample call under the same camera adds to them.
 int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k) {
-    PHR_CHECK(t && t->fexec, PHR_ERR_ARG, "phr_tracer_replay_frame: nothing recorded");
+    PHR_CHECK(t && t->fexec && !t->fposed, PHR_ERR_ARG, "phr_tracer_replay_frame: nothing recorded (a posed frame replays with its own call)");
     const bool moved = !t->px || t->px->ptr != t->fsig[0] || t->w != t->fw || t->h != t->fh || !same_scene(scene_dev(t), t->fsc) ||
                        (t->fdenoise == 2 && (t->frame_dev != t->fsig[1] || !k || phr_kpcn_internal_workspace(k) != t->fsig[2])) ||
                        (t->fdenoise == 1 && t->dn_scratch != t->fsig[1]);
@@ -2713,6 +2830,141 @@ int phr_tracer_replay_frame(phr_tracer* t, phr_kpcn* k) {
     t->w = t->fw, t->h = t->fh;
     t->last = t->fcam;
     t->have_last = true;
+    t->frame = t->fspp - 1;
+    return PHR_OK;
+}
+
+// A posed frame recorded as one graph: the new vertices when `verts` is
+// given (triangle boxes, the mesh trees' refit, the triangle cache), the new
+// matrices (world boxes, the top tree, the instance and emitter tables), the
+// camera's stamp on the device, the path launch and the denoiser: what
+// reinstance / repose_on and then sample (+ denoise) do one call after
+// another (src/trace/pose.rs:77-155), with no host wait between them. An
+// eager posed frame first sizes the image, stamps the camera and gives the
+// denoisers their workspaces. A forest whose emitter table is built on the
+// device only (at most 4,096 candidates, at least one).
+int phr_tracer_record_pose_frame(phr_tracer* t, phr_array* verts, const float* worlds, uint32_t n, const void* cam_v, uint32_t w,
+                                 uint32_t h, uint32_t spp, int denoise, phr_kpcn* k) {
+    PHR_CHECK(t && worlds && cam_v && spp >= 1 && denoise >= 0 && denoise <= 2 && (denoise != 2 || k), PHR_ERR_ARG,
+              "phr_tracer_record_pose_frame: bad argument");
+    PHR_CHECK(t->forest && n == t->ninsts, PHR_ERR_ARG, "phr_tracer_record_pose_frame: a forest and one matrix an instance");
+    PHR_CHECK(!verts || (verts->elem == 32 && verts->len == t->nverts), PHR_ERR_ARG,
+              "phr_tracer_record_pose_frame: vertices are the scene's 32-byte records");
+    PHR_CHECK(t->ncand > 0 && t->ncand <= LIGHT_DEVICE_MAX, PHR_ERR_UNSUPPORTED,
+              "phr_tracer_record_pose_frame: the emitter table must be built on the device (1 to 4,096 candidates)");
+    PhrCamera cam;
+    std::memcpy(&cam, cam_v, sizeof cam);
+    PHR_CHECK(cam.adapt[3] == 0.0f, PHR_ERR_UNSUPPORTED, "phr_tracer_record_pose_frame: adaptive sampling changes its launches");
+    if (!t->pin_worlds) {
+        PHR_HIP(hipHostMalloc((void**)&t->pin_worlds, 64ull * n, hipHostMallocDefault));
+        PHR_HIP(hipHostMalloc((void**)&t->pin_words, 4ull * (INSTANCE_LEN + 2) * n, hipHostMallocDefault));
+    }
+    int ok = 0, rc;
+    if (verts && ((rc = phr_tracer_repose_on(t, verts, &ok)) != PHR_OK || !ok)) return rc != PHR_OK ? rc : PHR_ERR_ARG;
+    if ((rc = phr_tracer_reinstance(t, worlds, n, &ok)) != PHR_OK || !ok) return rc != PHR_OK ? rc : PHR_ERR_ARG;
+    {
+        PhrCamera fresh = cam;
+        t->have_last = false;
+        if ((rc = phr_tracer_sample(t, &fresh, w, h, spp, nullptr, nullptr)) != PHR_OK) return rc;
+        if (denoise == 2 && (rc = phr_tracer_denoise_learned(t, k, nullptr)) != PHR_OK) return rc;
+        if (denoise == 1 && (rc = ensure_denoise_scratch(t)) != PHR_OK) return rc;
+    }
+    if (t->fexec) (void)hipGraphExecDestroy(t->fexec);
+    if (t->fgraph) (void)hipGraphDestroy(t->fgraph);
+    t->fexec = nullptr;
+    t->fgraph = nullptr;
+    const uint64_t np = (uint64_t)t->w * t->h;
+    const uint32_t nf = t->n;
+    hipStream_t s = t->gpu->stream;
+    const SceneDev sc = scene_dev(t);
+    const uint32_t tiles_x = (t->w + 15) / 16, tiles = tiles_x * ((t->h + 15) / 16);
+    PhrCamera* cp = (PhrCamera*)t->cam_dev;
+    uint32_t* pxp = (uint32_t*)t->px->ptr;
+    PHR_HIP(hipStreamSynchronize(s));
+    PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    rc = PHR_OK;
+    if (verts) {
+        (void)hipMemcpyAsync(t->verts->ptr, verts->ptr, 32ull * t->nverts, hipMemcpyDeviceToDevice, s);
+        hipLaunchKernelGGL(k_tri_boxes, dim3((nf + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
+                           (const float4*)t->verts->ptr, nf, (float4*)t->prims->ptr);
+        rc = phr_bvh_internal_refit_async(t->bvh);
+        if (rc == PHR_OK)
+            hipLaunchKernelGGL(k_gather, dim3((nf + 255) / 256), dim3(256), 0, s, (const uint4*)t->in_faces->ptr,
+                               (const float4*)t->verts->ptr, (const uint32_t*)phr_bvh_internal_order(t->bvh)->ptr, nf,
+                               (uint4*)t->faces->ptr, (float4*)t->tris->ptr);
+    }
+    if (rc == PHR_OK) rc = forest_top(t, false);
+    if (rc == PHR_OK) rc = forest_lights(t);
+    if (rc == PHR_OK) {
+        hipLaunchKernelGGL(k_stamp_pose, dim3(1), dim3(64), 0, s, (const float4*)t->nodes->ptr, (const uint32_t*)t->lights->ptr, 1u,
+                           cam.ao <= 0.f ? 1u : 0u, cp);
+        launch_path(path_build_waves(t), t->forest, false, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u,
+                    path_short_stack(t));
+        if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
+        if (denoise == 2) {
+            const uint32_t g = (uint32_t)((np + 255) / 256);
+            float* rad = t->frame_dev + 10 * np;
+            hipLaunchKernelGGL(k_frame, dim3(g), dim3(256), 0, s, (const uint32_t*)pxp, np, t->frame_dev);
+            rc = phr_kpcn_infer_dev(k, t->frame_dev, 1, t->w, t->h, rad);
+            if (rc == PHR_OK)
+                hipLaunchKernelGGL(k_frame_back, dim3(g), dim3(256), 0, s, pxp, np, (const float*)rad, view4(t->last));
+        }
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc != PHR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    PHR_CHECK(e == hipSuccess && graph, PHR_ERR_HIP, "phr_tracer_record_pose_frame: capture failed");
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        phr_set_error_msg("phr_tracer_record_pose_frame: instantiation failed");
+        return PHR_ERR_HIP;
+    }
+    t->fgraph = graph;
+    t->fexec = exec;
+    t->fcam = t->last;  // as stamped; the graph's own stamp updates ao, eps and glow on the device
+    t->fw = t->w, t->fh = t->h, t->fspp = spp;
+    t->fsc = sc;
+    t->fsig[0] = t->px->ptr;
+    t->fsig[1] = denoise == 2 ? (const void*)t->frame_dev : denoise == 1 ? (const void*)t->dn_scratch : nullptr;
+    t->fsig[2] = denoise == 2 ? phr_kpcn_internal_workspace(k) : nullptr;
+    t->fsig[3] = t->cam_dev;
+    t->fdenoise = denoise;
+    t->fposed = true;
+    t->fverts = verts ? verts->ptr : nullptr;
+    return PHR_OK;
+}
+
+// Replay the recorded posed frame under new matrices (one an instance; the
+// vertices are read again from the array given at recording). The image is
+// the new pose's, traced and denoised as recorded; the next sample call
+// starts the image over.
+int phr_tracer_replay_pose_frame(phr_tracer* t, const float* worlds, uint32_t n, phr_kpcn* k) {
+    PHR_CHECK(t && worlds && t->fexec && t->fposed, PHR_ERR_ARG, "phr_tracer_replay_pose_frame: no posed frame recorded");
+    PHR_CHECK(n == t->ninsts, PHR_ERR_ARG, "phr_tracer_replay_pose_frame: one matrix an instance");
+    const bool moved = !t->px || t->px->ptr != t->fsig[0] || t->w != t->fw || t->h != t->fh || !same_scene(scene_dev(t), t->fsc) ||
+                       (t->fdenoise == 2 && (t->frame_dev != t->fsig[1] || !k || phr_kpcn_internal_workspace(k) != t->fsig[2])) ||
+                       (t->fdenoise == 1 && t->dn_scratch != t->fsig[1]);
+    PHR_CHECK(!moved, PHR_ERR_ARG, "phr_tracer_replay_pose_frame: the scene, the image or the network moved; record again");
+    for (uint32_t i = 0; i < n; ++i) std::memcpy(t->inst_in[i].world, worlds + 16 * (size_t)i, 64);
+    const float* wsrc = nullptr;
+    const uint32_t* isrc = nullptr;
+    forest_pose_host(t, &wsrc, &isrc);  // into the pinned buffers the graph's copies read
+    hipStream_t s = t->gpu->stream;
+    PHR_HIP(hipMemcpyAsync(t->cam_dev, &t->fcam, sizeof(PhrCamera), hipMemcpyHostToDevice, s));
+    PHR_HIP(hipEventRecord(t->ev0, s));
+    PHR_HIP(hipGraphLaunch(t->fexec, s));
+    PHR_HIP(hipEventRecord(t->ev1, s));
+    PHR_HIP(hipEventSynchronize(t->ev1));
+    float ms = 0.f;
+    PHR_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    t->last_gpu_s = ms * 1e-3;
+    t->w = t->fw, t->h = t->fh;
+    t->stale = true;       // the extent and emitter count are the device's now
+    t->have_last = false;  // the next sample starts over
     t->frame = t->fspp - 1;
     return PHR_OK;
 }
@@ -3087,7 +3339,7 @@ struct phr_batch {
     std::vector<uint32_t> seg_lo, node_base, light_at, glow;
     std::vector<float> extent;
     phr_array *in_faces = nullptr, *verts = nullptr, *mats = nullptr, *prims = nullptr, *faces = nullptr, *tris = nullptr,
-              *lights = nullptr, *px = nullptr;
+              *lights = nullptr, *px = nullptr, *recs = nullptr;
     phr_bvh* bvh = nullptr;
     SceneDev* table = nullptr;    // one SceneDev a scene
     PhrCamera* cams = nullptr;    // one stamped camera a scene (device)
@@ -3111,7 +3363,7 @@ void batch_release(phr_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->bvh) phr_bvh_free(b->bvh);
-    for (phr_array* a : {b->in_faces, b->verts, b->mats, b->prims, b->faces, b->tris, b->lights, b->px}) phr_array_release(a);
+    for (phr_array* a : {b->in_faces, b->verts, b->mats, b->prims, b->faces, b->tris, b->lights, b->px, b->recs}) phr_array_release(a);
     if (b->table) (void)hipFree(b->table);
     if (b->cams) (void)hipFree(b->cams);
     if (b->gcams) (void)hipFree(b->gcams);
@@ -3133,7 +3385,7 @@ int batch_enqueue_build(phr_batch* b) {
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr, (const float4*)b->verts->ptr,
                        (const uint32_t*)phr_bvh_internal_order(b->bvh)->ptr, n, (uint4*)b->faces->ptr, (float4*)b->tris->ptr);
     PHR_HIP(hipGetLastError());
-    return PHR_OK;
+    return enqueue_walk_records(b->gpu, b->bvh, &b->recs, b->tris);
 }
 
 // Each scene's camera as a tracer of that scene alone stamps it.
@@ -3296,6 +3548,7 @@ int phr_batch_new(phr_gpu* g, const void* tris_v, const uint32_t* seg_lo, uint32
         return fail(rc);
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, (const uint4*)b->in_faces->ptr, (const float4*)b->verts->ptr,
                        (const uint32_t*)phr_bvh_internal_order(b->bvh)->ptr, n, (uint4*)b->faces->ptr, (float4*)b->tris->ptr);
+    if ((rc = enqueue_walk_records(g, b->bvh, &b->recs, b->tris)) != PHR_OK) return fail(rc);
     std::vector<uint32_t> order(n);
     if (hipMemcpyAsync(order.data(), phr_bvh_internal_order(b->bvh)->ptr, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -3317,6 +3570,7 @@ int phr_batch_new(phr_gpu* g, const void* tris_v, const uint32_t* seg_lo, uint32
         SceneDev& d = table[k];
         std::memset(&d, 0, sizeof d);
         d.nodes = (const float4*)phr_bvh_internal_nodes(b->bvh)->ptr;
+        d.recs = (const float4*)b->recs->ptr;
         d.tris = (const float4*)b->tris->ptr;
         d.faces = (const uint4*)b->faces->ptr;
         d.verts = (const float4*)b->verts->ptr;

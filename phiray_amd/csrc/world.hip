@@ -101,46 +101,46 @@ static_assert(sizeof(Body) == 64 && sizeof(Konst) == 48 && sizeof(PairRec) == 32
 struct V {
     float x, y, z;
 };
-__device__ __forceinline__ V mk(float x, float y, float z) { return V{x, y, z}; }
-__device__ __forceinline__ V operator+(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ V operator*(V a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-__device__ __forceinline__ V hadamard(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ float norm2(V a) { return dot(a, a); }
-__device__ __forceinline__ float length(V a) { return sqrtf(norm2(a)); }
-__device__ __forceinline__ float comp(V a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
-__device__ __forceinline__ V unit_axis(int k, float s) { return mk(k == 0 ? s : 0.f, k == 1 ? s : 0.f, k == 2 ? s : 0.f); }
+__host__ __device__ __forceinline__ V mk(float x, float y, float z) { return V{x, y, z}; }
+__host__ __device__ __forceinline__ V operator+(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ V operator-(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ V operator-(V a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ V operator*(V a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ __forceinline__ float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ V cross(V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__host__ __device__ __forceinline__ V hadamard(V a, V b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__host__ __device__ __forceinline__ float norm2(V a) { return dot(a, a); }
+__host__ __device__ __forceinline__ float length(V a) { return sqrtf(norm2(a)); }
+__host__ __device__ __forceinline__ float comp(V a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__host__ __device__ __forceinline__ V unit_axis(int k, float s) { return mk(k == 0 ? s : 0.f, k == 1 ? s : 0.f, k == 2 ? s : 0.f); }
 
 struct Q {
     float x, y, z, w;
 };
 // q v q^-1 for a unit quaternion: v + 2 w (u x v) + 2 u x (u x v).
-__device__ __forceinline__ V rot(Q q, V v) {
+__host__ __device__ __forceinline__ V rot(Q q, V v) {
     const V u = mk(q.x, q.y, q.z);
     const V t = cross(u, v) * 2.0f;
     return v + t * q.w + cross(u, t);
 }
-__device__ __forceinline__ V unrot(Q q, V v) { return rot(Q{-q.x, -q.y, -q.z, q.w}, v); }
-__device__ __forceinline__ Q qmul(Q a, Q b) {
+__host__ __device__ __forceinline__ V unrot(Q q, V v) { return rot(Q{-q.x, -q.y, -q.z, q.w}, v); }
+__host__ __device__ __forceinline__ Q qmul(Q a, Q b) {
     return Q{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
 }
-__device__ __forceinline__ Q qunit(Q q) {
+__host__ __device__ __forceinline__ Q qunit(Q q) {
     const float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     return l > 0.f ? Q{q.x / l, q.y / l, q.z / l, q.w / l} : Q{0.f, 0.f, 0.f, 1.f};
 }
 // The orientation after turning at angular velocity om for time h (first
 // order, renormalised; Muller 2020 eq. 8).
-__device__ __forceinline__ Q qstep(Q q, V om, float h) {
+__host__ __device__ __forceinline__ Q qstep(Q q, V om, float h) {
     const Q d = qmul(Q{om.x, om.y, om.z, 0.f}, q);
     const float s = 0.5f * h;
     return qunit(Q{q.x + d.x * s, q.y + d.y * s, q.z + d.z * s, q.w + d.w * s});
 }
 // The angular velocity that turns `from` into `to` over h (Muller 2020 eq. 9).
-__device__ __forceinline__ V qrate(Q to, Q from, float h) {
+__host__ __device__ __forceinline__ V qrate(Q to, Q from, float h) {
     const Q d = qmul(to, Q{-from.x, -from.y, -from.z, from.w});
     const float s = (d.w >= 0.f ? 2.0f : -2.0f) / h;
     return mk(d.x * s, d.y * s, d.z * s);
@@ -149,28 +149,28 @@ __device__ __forceinline__ V qrate(Q to, Q from, float h) {
 // ---- a body's state in LDS: (x, inv_mass), q, (v, shape), (w, flags) -----------------------
 struct Lds {
     float4* s;
-    __device__ __forceinline__ V x(uint32_t i) const { const float4 r = s[4 * i]; return mk(r.x, r.y, r.z); }
-    __device__ __forceinline__ float im(uint32_t i) const { return s[4 * i].w; }
-    __device__ __forceinline__ Q q(uint32_t i) const { const float4 r = s[4 * i + 1]; return Q{r.x, r.y, r.z, r.w}; }
-    __device__ __forceinline__ V v(uint32_t i) const { const float4 r = s[4 * i + 2]; return mk(r.x, r.y, r.z); }
-    __device__ __forceinline__ V w(uint32_t i) const { const float4 r = s[4 * i + 3]; return mk(r.x, r.y, r.z); }
-    __device__ __forceinline__ void set_x(uint32_t i, V a) { s[4 * i].x = a.x; s[4 * i].y = a.y; s[4 * i].z = a.z; }
-    __device__ __forceinline__ void set_q(uint32_t i, Q a) { s[4 * i + 1] = make_float4(a.x, a.y, a.z, a.w); }
-    __device__ __forceinline__ void set_v(uint32_t i, V a) { s[4 * i + 2].x = a.x; s[4 * i + 2].y = a.y; s[4 * i + 2].z = a.z; }
-    __device__ __forceinline__ void set_w(uint32_t i, V a) { s[4 * i + 3].x = a.x; s[4 * i + 3].y = a.y; s[4 * i + 3].z = a.z; }
+    __host__ __device__ __forceinline__ V x(uint32_t i) const { const float4 r = s[4 * i]; return mk(r.x, r.y, r.z); }
+    __host__ __device__ __forceinline__ float im(uint32_t i) const { return s[4 * i].w; }
+    __host__ __device__ __forceinline__ Q q(uint32_t i) const { const float4 r = s[4 * i + 1]; return Q{r.x, r.y, r.z, r.w}; }
+    __host__ __device__ __forceinline__ V v(uint32_t i) const { const float4 r = s[4 * i + 2]; return mk(r.x, r.y, r.z); }
+    __host__ __device__ __forceinline__ V w(uint32_t i) const { const float4 r = s[4 * i + 3]; return mk(r.x, r.y, r.z); }
+    __host__ __device__ __forceinline__ void set_x(uint32_t i, V a) { s[4 * i].x = a.x; s[4 * i].y = a.y; s[4 * i].z = a.z; }
+    __host__ __device__ __forceinline__ void set_q(uint32_t i, Q a) { s[4 * i + 1] = make_float4(a.x, a.y, a.z, a.w); }
+    __host__ __device__ __forceinline__ void set_v(uint32_t i, V a) { s[4 * i + 2].x = a.x; s[4 * i + 2].y = a.y; s[4 * i + 2].z = a.z; }
+    __host__ __device__ __forceinline__ void set_w(uint32_t i, V a) { s[4 * i + 3].x = a.x; s[4 * i + 3].y = a.y; s[4 * i + 3].z = a.z; }
 };
 
-__device__ __forceinline__ uint32_t kind_of(uint32_t flags) {
+__host__ __device__ __forceinline__ uint32_t kind_of(uint32_t flags) {
     const uint32_t k = flags & KIND_MASK;
     return k == 1u ? K_PASSIVE : (k == 2u ? K_KINEMATIC : K_ACTIVE);
 }
-__device__ __forceinline__ bool moves_f(uint32_t f) { return kind_of(f) != K_PASSIVE && (f & SLEEPING) == 0u; }
-__device__ __forceinline__ bool dynamic_f(uint32_t f) { return kind_of(f) == K_ACTIVE && (f & SLEEPING) == 0u; }
+__host__ __device__ __forceinline__ bool moves_f(uint32_t f) { return kind_of(f) != K_PASSIVE && (f & SLEEPING) == 0u; }
+__host__ __device__ __forceinline__ bool dynamic_f(uint32_t f) { return kind_of(f) == K_ACTIVE && (f & SLEEPING) == 0u; }
 
-__device__ __forceinline__ V ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+__host__ __device__ __forceinline__ V ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 
 // ---- 1. swept boxes ----------------------------------------------------------------
-__device__ V shape_reach(const phr_shape& s, Q q) {
+__host__ __device__ V shape_reach(const phr_shape& s, Q q) {
     // Half the extent of the shape's world box along each axis.
     if (s.kind == S_SPHERE) return mk(s.radius, s.radius, s.radius);
     if (s.kind == S_CAPSULE) {
@@ -259,7 +259,7 @@ struct Found {
     V n;
     V pa[4], pb[4];
     int m;
-    __device__ void add(V a, V b) {
+    __host__ __device__ void add(V a, V b) {
         if (m < 4) {
             pa[m] = a;
             pb[m] = b;
@@ -268,7 +268,7 @@ struct Found {
     }
 };
 
-__device__ void spheres(V ca, float ra, V cb, float rb, float slop, Found& f) {
+__host__ __device__ void spheres(V ca, float ra, V cb, float rb, float slop, Found& f) {
     const V d = cb - ca;
     const float l = length(d);
     if (l - ra - rb > slop) return;
@@ -278,7 +278,7 @@ __device__ void spheres(V ca, float ra, V cb, float rb, float slop, Found& f) {
 }
 
 // Closest points of segments [p0, p1] and [q0, q1] (parameters s, t in [0, 1]).
-__device__ void segment_pair(V p0, V p1, V q0, V q1, float& s, float& t) {
+__host__ __device__ void segment_pair(V p0, V p1, V q0, V q1, float& s, float& t) {
     const V d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
     const float a = norm2(d1), e = norm2(d2), f = dot(d2, r);
     if (a <= 1e-12f && e <= 1e-12f) { s = t = 0.f; return; }
@@ -292,7 +292,7 @@ __device__ void segment_pair(V p0, V p1, V q0, V q1, float& s, float& t) {
     else if (t > 1.f) { t = 1.f; s = fminf(fmaxf((b - c) / a, 0.f), 1.f); }
 }
 
-__device__ void capsule_ends(const phr_shape& s, V x, Q q, V& p0, V& p1) {
+__host__ __device__ void capsule_ends(const phr_shape& s, V x, Q q, V& p0, V& p1) {
     const V h = rot(q, mk(0.f, 0.f, s.half[2]));
     p0 = x - h;
     p1 = x + h;
@@ -300,7 +300,7 @@ __device__ void capsule_ends(const phr_shape& s, V x, Q q, V& p0, V& p1) {
 
 // Sphere (a) against box (b): the nearest point of the box, or, from inside,
 // the nearest face.
-__device__ void sphere_box(V c, float r, V xb, Q qb, V h, float slop, Found& f) {
+__host__ __device__ void sphere_box(V c, float r, V xb, Q qb, V h, float slop, Found& f) {
     const V l = unrot(qb, c - xb);
     const V k = mk(fminf(fmaxf(l.x, -h.x), h.x), fminf(fmaxf(l.y, -h.y), h.y), fminf(fmaxf(l.z, -h.z), h.z));
     const V d = l - k;
@@ -333,14 +333,15 @@ struct Box {
     V c, ax[3];
     float h[3];
 };
-__device__ __forceinline__ float box_radius(const Box& b, V l) {
+__host__ __device__ __forceinline__ float box_radius(const Box& b, V l) {
     return b.h[0] * fabsf(dot(b.ax[0], l)) + b.h[1] * fabsf(dot(b.ax[1], l)) + b.h[2] * fabsf(dot(b.ax[2], l));
 }
 
+constexpr float CLIP_TOL = 1e-5f;
 // The incident face of `inc` against the reference face (axis k, outward
 // normal n) of `ref`, clipped to the reference face's sides; points on the
 // incident box (p) and their projections on the reference face (p - n s).
-__device__ void face_contact(const Box& ref, int k, V n, const Box& inc, float slop, bool flip, Found& f) {
+__host__ __device__ void face_contact(const Box& ref, int k, V n, const Box& inc, float slop, bool flip, Found& f) {
     int j = 0;
     float best = -1.f;
     for (int t = 0; t < 3; ++t) {
@@ -362,13 +363,15 @@ __device__ void face_contact(const Box& ref, int k, V n, const Box& inc, float s
         const int m = (k + 1 + (side >> 1)) % 3;
         const float sg = (side & 1) ? -1.f : 1.f;
         const V pn = ref.ax[m] * sg;
-        const float off = ref.h[m];
+        // A corner on the side plane (boxes of one size stacked) stays, to
+        // within rounding: the plane sits CLIP_TOL (relative) outside.
+        const float off = ref.h[m] + CLIP_TOL * fmaxf(ref.h[m], 1.0f);
         int nn = 0;
         for (int e = 0; e < np; ++e) {
             const V p = poly[e], q = poly[(e + 1) % np];
             const float dp = dot(p - ref.c, pn) - off, dq = dot(q - ref.c, pn) - off;
             if (dp <= 0.f && nn < 8) next[nn++] = p;
-            if ((dp < 0.f) != (dq < 0.f) && nn < 8) {
+            if ((dp <= 0.f) != (dq <= 0.f) && nn < 8) {
                 const float t = dp / (dp - dq);
                 next[nn++] = p + (q - p) * t;
             }
@@ -432,7 +435,7 @@ __device__ void face_contact(const Box& ref, int k, V n, const Box& inc, float s
 // Box a against box b by separating axes; faces are preferred to edge pairs
 // unless an edge pair separates the boxes by EDGE_PREF more.
 constexpr float EDGE_PREF = 5e-4f;
-__device__ void boxes(const Box& A, const Box& B, float slop, Found& f) {
+__host__ __device__ void boxes(const Box& A, const Box& B, float slop, Found& f) {
     const V d = B.c - A.c;
     float face_s = -3.0e38f;
     int face = -1;  // 0..2: a face of A; 3..5: a face of B
@@ -482,7 +485,7 @@ __device__ void boxes(const Box& A, const Box& B, float slop, Found& f) {
     }
 }
 
-__device__ Box box_of(const phr_shape& s, V x, Q q) {
+__host__ __device__ Box box_of(const phr_shape& s, V x, Q q) {
     Box b;
     b.c = x;
     b.ax[0] = rot(q, mk(1.f, 0.f, 0.f));
@@ -495,7 +498,7 @@ __device__ Box box_of(const phr_shape& s, V x, Q q) {
 }
 
 // The points of one pair with a's shape kind <= b's where that matters.
-__device__ void manifold(const phr_shape& sa, V xa, Q qa, const phr_shape& sb, V xb, Q qb, float slop, Found& f) {
+__host__ __device__ void manifold(const phr_shape& sa, V xa, Q qa, const phr_shape& sb, V xb, Q qb, float slop, Found& f) {
     const uint32_t ka = sa.kind, kb = sb.kind;
     if (ka == S_SPHERE && kb == S_SPHERE) return spheres(xa, sa.radius, xb, sb.radius, slop, f);
     if (ka == S_BOX && kb == S_BOX) return boxes(box_of(sa, xa, qa), box_of(sb, xb, qb), slop, f);
@@ -583,13 +586,13 @@ struct Params {
 
 // The generalized inverse mass of body i along direction l at arm r (Muller
 // 2020 eq. 2-3), zero for a body the solve does not move.
-__device__ __forceinline__ float w_along(const Lds& L, const Konst* kc, const uint32_t* fl, uint32_t i, V r, V l) {
+__host__ __device__ __forceinline__ float w_along(const Lds& L, const Konst* kc, const uint32_t* fl, uint32_t i, V r, V l) {
     if (!dynamic_f(fl[i])) return 0.f;
     const V rl = unrot(L.q(i), cross(r, l));
     return L.im(i) + dot(hadamard(rl, ld3(kc[i].inv_inertia)), rl);
 }
 // The turn an impulse p at arm r gives body i (world frame).
-__device__ __forceinline__ V turn_of(const Lds& L, const Konst* kc, uint32_t i, V r, V p) {
+__host__ __device__ __forceinline__ V turn_of(const Lds& L, const Konst* kc, uint32_t i, V r, V p) {
     const Q q = L.q(i);
     return rot(q, hadamard(unrot(q, cross(r, p)), ld3(kc[i].inv_inertia)));
 }
@@ -604,7 +607,7 @@ struct Kicks {
 struct Move {
     V da, ta, db, tb;  // translation and turn of a and b
 };
-__device__ Move summed(const Lds& L, const Konst* kc, const uint32_t* fl, uint32_t a, uint32_t b, const V* ra, const V* rb,
+__host__ __device__ Move summed(const Lds& L, const Konst* kc, const uint32_t* fl, uint32_t a, uint32_t b, const V* ra, const V* rb,
                        const Kicks& k, int m, const float* weight) {
     Move mv{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool da = dynamic_f(fl[a]), db = dynamic_f(fl[b]);
@@ -623,7 +626,7 @@ __device__ Move summed(const Lds& L, const Konst* kc, const uint32_t* fl, uint32
     return mv;
 }
 // The largest scale <= 1 under which no point moves by more than it asked.
-__device__ float safe_scale(const Move& mv, const V* ra, const V* rb, const Kicks& k, int m) {
+__host__ __device__ float safe_scale(const Move& mv, const V* ra, const V* rb, const Kicks& k, int m) {
     float s = 1.0f;
     for (int e = 0; e < m; ++e) {
         if (!k.on[e]) continue;
@@ -633,7 +636,7 @@ __device__ float safe_scale(const Move& mv, const V* ra, const V* rb, const Kick
     return s;
 }
 
-__device__ void project_pair(Lds& L, const Konst* kc, const uint32_t* fl, const float4* prev, float4* push, const PairRec& pr,
+__host__ __device__ void project_pair(Lds& L, const Konst* kc, const uint32_t* fl, const float4* prev, float4* push, const PairRec& pr,
                              PointRec* pts, float cap) {
     const uint32_t a = pr.a, b = pr.b;
     const int m = (int)pr.m;
@@ -757,7 +760,7 @@ __device__ void project_pair(Lds& L, const Konst* kc, const uint32_t* fl, const 
 }
 
 // Dynamic friction, then restitution, on the velocities (Muller 2020 §3.6).
-__device__ void velocity_pair(Lds& L, const Konst* kc, const uint32_t* fl, const PairRec& pr, const PointRec* pts, float h,
+__host__ __device__ void velocity_pair(Lds& L, const Konst* kc, const uint32_t* fl, const PairRec& pr, const PointRec* pts, float h,
                               float g) {
     const uint32_t a = pr.a, b = pr.b;
     const int m = (int)pr.m;
@@ -824,7 +827,7 @@ __device__ void velocity_pair(Lds& L, const Konst* kc, const uint32_t* fl, const
 // fewer than 2^11. Colouring by it, a pair joins a round only once every
 // lower pair sharing one of its moving bodies has its colour, so the colours
 // in sequence do exactly what the pairs one after another in that order do.
-__device__ __forceinline__ uint32_t priority(const PairRec& pr) { return (pr.a << 11) | pr.b; }
+__host__ __device__ __forceinline__ uint32_t priority(const PairRec& pr) { return (pr.a << 11) | pr.b; }
 
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodies, const Konst* __restrict__ kc,
                                                          Aux* __restrict__ aux, const uint32_t* __restrict__ count,

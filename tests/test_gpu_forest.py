@@ -276,3 +276,44 @@ def test_one_identity_instance_traces_as_the_triangle_tracer(gpu):
         t.set_samples(64)
         t.sample(cam, 32, 24, 2)
     assert (tf.px()[:34 * 32 * 24] == tt.px()[:34 * 32 * 24]).all()
+
+
+@pytest.mark.parametrize("denoise", [0, 1])
+def test_a_recorded_posed_frame_equals_the_eager_one(gpu, denoise):
+    """One hipGraph holding a new pose (vertices, matrices), the camera's
+    stamp, the path launch and the denoiser gives, replay after replay, the
+    image a reinstance + repose + sample (+ denoise) call by call gives."""
+    print(f"[forest] recorded posed frame, denoise {denoise}", flush=True)
+    arrs = forest_scene().arrays()
+    t = P.Tracer.from_scene(gpu, *arrs, leaf=4)
+    eager = P.Tracer.from_scene(gpu, *arrs, leaf=4)
+    w, h, spp = 40, 28, 2
+    cam = S.path_camera(w, h, eye=(0.0, -7.0, 3.5), target=(0.0, 0.5, 0.8), rr_from=2)
+    verts = P.GpuArray.from_(gpu, arrs[0])
+    base = arrs[4]["world"].copy()
+    rng = np.random.default_rng(31)
+
+    def pose(k):
+        wk = base.copy()
+        for i in range(7):
+            wk[i] = S.world_of(S.affine(S.rotation(rng.normal(size=3), rng.uniform(0, 6)), rng.uniform(0.4, 1.0),
+                                        rng.uniform(-3, 3, 3) + np.array([0, 0, 3.5])))
+        v = arrs[0].copy()
+        v["p"][:, :3] *= np.float32(1.0 + 0.01 * k)
+        return wk, v
+
+    w0, v0 = pose(0)
+    verts.write(v0)
+    t.record_pose_frame(verts, w0, cam, w, h, spp, denoise)
+    for k in range(1, 4):
+        wk, vk = pose(k)
+        verts.write(vk)
+        t.replay_pose_frame(wk)
+        got = t.px()
+        assert eager.repose_on(verts) and eager.reinstance(wk)
+        eager.sample(cam, w, h, spp)
+        if denoise == 1:
+            eager.denoise(keep=False)
+        want = eager.px()
+        assert (got == want).all(), f"pose {k}: {(got != want).sum()} words differ"
+        eager.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next pose starts over
