@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 RAY_BYTES, HIT_BYTES, ROOT_BYTES, STEP_BYTES, TEST_BYTES = 32, 8, 32, 64, 48
 
 
-def profiled_traffic(name: str, spp: int, w: int, h: int, waves: int) -> tuple[float | None, str | None]:
+def profiled_traffic(name: str, spp: int, w: int, h: int, waves: int, build: str) -> tuple[float | None, str | None]:
     """HBM bytes of one k_path launch from the committed counter summary, when
     it was measured on this same workload and kernel variant (FETCH_SIZE +
     WRITE_SIZE, each scaled by what the counter reads for a copy of known size)."""
@@ -48,7 +48,10 @@ def profiled_traffic(name: str, spp: int, w: int, h: int, waves: int) -> tuple[f
     text = f.read_text()
     m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", text)
     mw = re.search(r"path_waves_per_simd:\s*(\d+)", text)
+    mb = re.search(r"path_build:\s*(\w+)", text)
     if not m or not mw or int(mw.group(1)) != waves or (spp, w, h) != (64, 1920, 1080):
+        return None, None
+    if (mb.group(1) if mb else "nested") != build:
         return None, None
     return float(m.group(1)), f"profiles/{name}"
 
@@ -281,8 +284,8 @@ def learned_vs_plain(gpu: P.Gpu, net: "P.Kpcn", seeds, w: int = 1920, h: int = 1
     on held-out scenes of the learned denoiser's generator (65,536 triangles,
     albedo U[0.2, 0.8]^3, a white world; seeds the training never saw):
 
-    * plain: the raw path tracer, the fewest samples (doubling from 4) whose
-      image reaches the bound; its GPU time;
+    * plain: the raw path tracer, the fewest samples whose image reaches the
+      bound (doubling from 4, then bisecting); its GPU time;
     * learned: LearnedFrame at `spp` samples (two halves, the network, the
       held-out check, more samples while the check fails, the plain render
       past its budget), wall time of the whole call;
@@ -323,6 +326,22 @@ def learned_vs_plain(gpu: P.Gpu, net: "P.Kpcn", seeds, w: int = 1920, h: int = 1
             if a_trous is None and at <= 0.01:
                 a_trous = {"spp": n, "s_per_frame": render_s + den_s, "relmse": at}
             n *= 2
+        if plain and plain["spp"] > 4:
+            # The fewest samples: bisect between the doubling's last failing
+            # and first passing counts.
+            lo, hi = plain["spp"] // 2, plain["spp"]
+            while hi - lo > 1:
+                mid = (lo + hi) // 2
+                fresh = cam.copy()
+                fresh["stream"] = mid + (5 << 16)
+                tr.sample(fresh, w, h, mid)
+                render_s = tr.time()[1]
+                raw = rel_mse(tr.radiance(), ref)
+                trials.append({"spp": mid, "render_s": render_s, "relmse_raw": raw})
+                if raw <= 0.01:
+                    hi, plain = mid, {"spp": mid, "s_per_frame": render_s, "relmse": raw}
+                else:
+                    lo = mid
         tr.free()
         lf = K.LearnedFrame(gpu, tris, net)
         lf.render(cam, w, h, spp=spp)  # warm the network's workspace at this size
@@ -534,6 +553,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge", "batch", "kpcn"], default="all")
     ap.add_argument("--path-waves", type=int, default=0, help="run this path kernel variant (2 to 5) instead of autotuning")
+    ap.add_argument("--path-build", choices=["auto", "nested", "regen", "wavefront"], default="auto",
+                    help="with --path-waves: which path build runs (default: the nested one)")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     ap.add_argument("--c5-seeds", type=int, default=8, help="held-out scenes of the learned-vs-plain leg")
     ap.add_argument("--c5-ref-spp", type=int, default=4096, help="its reference render's samples")
@@ -622,11 +643,25 @@ def main() -> int:
         log(f"Tracer::new {time.perf_counter() - t0:.2f} s, depth {tracer.depth}, {tracer.glow} emitters")
         if args.path_waves:
             tracer.set_path_waves(args.path_waves)
-            log(f"path kernel at {args.path_waves} waves a SIMD (chosen on the command line)")
+            build = {"auto": 0, "nested": 0, "regen": 1, "wavefront": 2}[args.path_build]
+            tracer.set_path_regen(build)
+            log(f"path kernel at {args.path_waves} waves a SIMD, build {args.path_build} (chosen on the command line)")
         else:
             try:
                 waves, tune_s = tracer.autotune(cam, w, h, 8, reps=2)
-                log(f"autotune: path kernel at {waves} waves a SIMD ({tune_s * 1e3:.1f} ms for 8 spp)")
+                build_name = ("nested", "regen", "wavefront")[tracer.path_regen]
+                log(f"autotune: path kernel at {waves} waves a SIMD, {build_name} build ({tune_s * 1e3:.1f} ms for 8 spp)")
+                # Each build at the chosen waves, 8 samples, for the line.
+                chosen = tracer.path_regen
+                by_build = {}
+                for b, name in enumerate(("nested", "regen", "wavefront")):
+                    tracer.set_path_regen(b)
+                    tracer.sample(S.path_camera(64, 64), 64, 64, 1)  # another view: the next call starts over
+                    tracer.sample(cam, w, h, 8)
+                    by_build[name] = tracer.time()[1] * 1e3
+                tracer.set_path_regen(chosen)
+                out["path_build_ms_8spp"] = by_build
+                log(f"builds at {waves} waves, 8 spp: " + ", ".join(f"{k} {v:.1f} ms" for k, v in by_build.items()))
             except P.PhrError as e:
                 log(f"autotune failed ({e}); the path kernel stays at {tracer.path_waves} waves a SIMD")
         for i in range(args.warmup):
@@ -650,10 +685,13 @@ def main() -> int:
         kernel_s = float(np.mean(gpu_s))
         nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
         achieved = nbytes / kernel_s / 1e9
-        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h, tracer.path_waves)
+        build = ("nested", "regen", "wavefront")[tracer.path_regen]
+        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h, tracer.path_waves, build)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
-                            "algorithmic_bytes": nbytes, "kernel": "k_path",
+                            "algorithmic_bytes": nbytes,
+                            "kernel": {"nested": "k_path", "regen": "k_path_rg",
+                                       "wavefront": "k_wf_walk + k_wf_shade (one frame's turns)"}[build],
                             "per_ray_bytes": "32 ray + 8 hit + 32 root + 64 x inner steps + 48 x triangle tests",
                             "launch_ms": kernel_s * 1e3}
         if traffic:
@@ -665,6 +703,7 @@ def main() -> int:
         line["s_per_frame"] = per
         line["path_waves_per_simd"] = tracer.path_waves
         line["path_regen"] = tracer.path_regen
+        line["path_build"] = ("nested", "regen", "wavefront")[tracer.path_regen]
         line["rays_per_frame"] = rays / args.steps
         line["closest_per_frame"] = closest / args.steps
         line["shadow_per_frame"] = shadow / args.steps

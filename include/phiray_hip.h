@@ -394,11 +394,15 @@ int phr_tracer_open_pixels(phr_tracer* t, uint32_t* count);
  * fitted to 2 (default), 3, 4 or 5 waves a SIMD. */
 int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves);
 uint32_t phr_tracer_path_waves(const phr_tracer* t);
-/* The path-regenerating build (no counterpart in phiray): 1 lets a lane whose
+/* The path-regenerating builds (no counterpart in phiray): 1 lets a lane whose
  * sample ends start its pixel's next sample at once instead of waiting for
- * the wave's longest path (the same sums, bit for bit). It serves one tree
- * and a camera with lamps or a lit world and no Adaptive Sampling; other
- * frames take the other build. The autotune tries both. */
+ * the wave's longest path; 2 is the wavefront build: each turn walks every
+ * pixel's ray in a persistent walker (a lane whose walk ends takes the next
+ * queued ray), shades in a kernel of its own, and appends the next rays and
+ * the shadow rays to queues by wave ballots. Both give the same sums, bit
+ * for bit. They serve one tree and a camera with lamps or a lit world and no
+ * Adaptive Sampling; other frames take the nested build, and a sliced sample
+ * or a recorded frame takes 1 for 2. The autotune tries all three. */
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on);
 uint32_t phr_tracer_path_regen(const phr_tracer* t);
 /* No counterpart in the reference: the mesh-tree stack entries a thread of
@@ -566,6 +570,10 @@ float phr_world_energy(const phr_world* w);
 /* Pairs the last broadphase found (sorted, deduplicated): *n pairs, up to cap
  * written as (i, j) with i < j. */
 int phr_world_pair_list(const phr_world* w, uint32_t* out, uint32_t cap, uint32_t* n);
+/* No counterpart in the reference: lanes a pair in the solve's projections,
+ * 4 (default: a point a lane) or 1 (a pair a lane); both give the same bits. */
+int phr_world_set_lanes(phr_world* w, uint32_t lanes);
+uint32_t phr_world_lanes(const phr_world* w);
 
 #ifdef __cplusplus
 }

@@ -295,6 +295,8 @@ _SIGS = {
     "phr_world_poses": (_I, [_P, _P]),
     "phr_world_energy": (C.c_float, [_P]),
     "phr_world_pair_list": (_I, [_P, _P, _U32, C.POINTER(_U32)]),
+    "phr_world_set_lanes": (_I, [_P, _U32]),
+    "phr_world_lanes": (_U32, [_P]),
     "phr_bvh_compact": (_I, [_P, _P, _U64, C.POINTER(_U32)]),
     "phr_tracer_sample_in": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, C.POINTER(_U32), C.POINTER(_I),
                                   C.POINTER(_I)]),
@@ -980,14 +982,19 @@ class Tracer:
     def path_waves(self) -> int:
         return int(lib().phr_tracer_path_waves(self._h))
 
-    def set_path_regen(self, on: bool) -> None:
-        """The path-regenerating build on or off (one tree, a camera with lamps
-        or a lit world, no Adaptive Sampling; other frames take the other)."""
-        _check(lib().phr_tracer_set_path_regen(self._h, 1 if on else 0), "Tracer set_path_regen")
+    PATH_NESTED, PATH_REGEN, PATH_WAVEFRONT = 0, 1, 2
+
+    def set_path_regen(self, mode: "bool | int") -> None:
+        """Which build the path frames take: the nested one (False, 0), the
+        path-regenerating one (True, 1) or the wavefront one (2). The last two
+        serve one tree and a camera with lamps or a lit world, no Adaptive
+        Sampling; other frames take the nested build."""
+        m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
+        _check(lib().phr_tracer_set_path_regen(self._h, m), "Tracer set_path_regen")
 
     @property
-    def path_regen(self) -> bool:
-        return bool(lib().phr_tracer_path_regen(self._h))
+    def path_regen(self) -> int:
+        return int(lib().phr_tracer_path_regen(self._h))
 
     @property
     def path_stack(self) -> int:
@@ -1493,6 +1500,14 @@ class World:
         out = np.zeros((n.value, 2), dtype=np.uint32)
         _check(lib().phr_world_pair_list(self._h, _ptr(out), n.value, C.byref(n)), "World pairs")
         return out
+
+    @property
+    def lanes(self) -> int:
+        """Lanes a pair in the solve's projections (4, or 1)."""
+        return int(lib().phr_world_lanes(self._h))
+
+    def set_lanes(self, lanes: int) -> None:
+        _check(lib().phr_world_set_lanes(self._h, lanes), "World set_lanes")
 
     def free(self) -> None:
         if self._h:

@@ -193,8 +193,6 @@ int phr_array_write(phr_array* a, const void* items, uint64_t len, int* grown) {
     return PHR_OK;
 }
 
-// GpuArray::write_at (src/gpu/array.rs:88-107): items from `at`, growing the
-// length; `grown` as for write (the old contents are kept across a regrow).
 // GpuArray::write_at (src/gpu/array.rs:96-107): overwrite len items from
 // item `at`, keeping the rest; *written = 0, and nothing written, when they
 // reach past the array's length.

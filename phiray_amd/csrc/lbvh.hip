@@ -113,7 +113,7 @@ constexpr uint32_t Z_DEPTH = Z_SCAN_TICKET + 8;
 // Refit levels: slots owned by a level-0 block, and blocks of one level per
 // block of the next. A level's inner-node counters live in LDS (2 bits each)
 // while its block owns at most RF_LDS_SLOTS slots, in global memory above.
-constexpr uint32_t RF_THREADS = 256;
+constexpr uint32_t RF_THREADS = 1024;  // every handed-over climb of a block in flight at once
 constexpr uint32_t RF_LEAF_THREADS = 1024;  // level 0 is latency-bound: more waves in flight
 constexpr uint32_t RF_SLOTS = 2048;
 constexpr uint32_t RF_GROUP = 64;
@@ -454,12 +454,24 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint32_t* __restrict__ k
     uint32_t binbase = block_exclusive_scan(hist[d], s_scan);
     uint32_t excl = 0;
     if (tile > 0) {
+        // Four earlier tiles' words a round trip, taken in order up to the
+        // nearest inclusive prefix (before tile 0: a prefix of zero).
         int32_t j = (int32_t)tile - 1;
-        for (;;) {
-            uint32_t s = __hip_atomic_load(status + (size_t)j * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (s & FLAG_PRE) { excl += s & VAL_MASK; break; }
-            if (s & FLAG_AGG) { excl += s & VAL_MASK; --j; continue; }
-            __builtin_amdgcn_s_sleep(1);
+        for (bool done = false; !done;) {
+            uint32_t st[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                st[k] = j - k >= 0 ? __hip_atomic_load(status + (size_t)(j - k) * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : FLAG_PRE;
+            int k = 0;
+            for (; k < 4; ++k) {
+                if (st[k] & FLAG_PRE) { excl += st[k] & VAL_MASK; done = true; break; }
+                if (!(st[k] & FLAG_AGG)) break;  // not published yet
+                excl += st[k] & VAL_MASK;
+            }
+            if (done) break;
+            j -= k;
+            if (k < 4) __builtin_amdgcn_s_sleep(1);
         }
         __hip_atomic_store(my, FLAG_PRE | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -624,23 +636,39 @@ __global__ __launch_bounds__(256) void k_stab(const uint32_t* __restrict__ own_e
         if (w < wave) before += s_wave[w];
         total += s_wave[w];
     }
-    if (tid == 0) {
+    if (wave == 0) {
+        // The look-back by the first wave: 64 earlier tiles' words a read,
+        // summed up to the nearest inclusive prefix once every word up to it
+        // is published.
         uint64_t* my = status + tile;
         int64_t excl = 0;
         if (tile == 0) {
-            __hip_atomic_store(my, SC_PRE | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(my, SC_PRE | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(my, SC_AGG | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int32_t j = (int32_t)tile - 1;
+            if (lane == 0) __hip_atomic_store(my, SC_AGG | ((uint64_t)total & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j0 = (int64_t)tile - 1;  // the window's nearest tile, at lane 0
             for (;;) {
-                uint64_t st = __hip_atomic_load(status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (st & SC_PRE) { excl += sc_value(st); break; }
-                if (st & SC_AGG) { excl += sc_value(st); --j; continue; }
-                __builtin_amdgcn_s_sleep(1);
+                const int64_t j = j0 - (int64_t)lane;
+                // Before tile 0: an inclusive prefix of zero.
+                const uint64_t st = j >= 0 ? __hip_atomic_load(status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SC_PRE;
+                const uint64_t pre = __ballot((st & SC_PRE) != 0ull);
+                const uint64_t pub = __ballot((st & (SC_PRE | SC_AGG)) != 0ull);
+                const uint32_t first = pre ? (uint32_t)__ffsll((unsigned long long)pre) - 1u : 64u;
+                const uint64_t need = first >= 63u ? ~0ull : ((1ull << (first + 1u)) - 1ull);
+                if ((pub & need) != need) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                int64_t v = lane <= first ? sc_value(st) : 0;
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                excl += v;
+                if (first < 64u) break;
+                j0 -= 64;
             }
-            __hip_atomic_store(my, SC_PRE | ((uint64_t)(excl + total) & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(my, SC_PRE | ((uint64_t)(excl + total) & SC_MASK), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_excl = excl;
+        if (lane == 0) s_excl = excl;
     }
     __syncthreads();
     int64_t run = s_excl + before + x - mine;

@@ -38,6 +38,7 @@ constexpr uint32_t PHR_INST = 1u << 30;  // a top-tree leaf's range is instance 
 struct SceneDev {
     const float4* nodes;
     const float4* recs;    // one tree's walk records (4 float4 a node index); a forest walks `nodes`
+    const float4* kids;    // the records' second level (8 float4 a node index), after the first
     const float4* tris;    // 3 float4 per slot: (a.xyz, b.x) (b.yz, c.xy) (c.z, -, -, -)
     const uint4* faces;    // slot order: 3 vertex ids + material
     const float4* verts;   // 2 float4 per vertex: (pos, 1) (oct.xy, u, v)
@@ -132,8 +133,101 @@ __device__ __forceinline__ float box_t(float4 lo, float4 hi, V3 o, V3 inv, float
 // alpha < 0.5, bvh.wgsl:158-160): `faces` and `mats` give the material.
 // `steps` and `tests` count inner-node visits and triangle tests, the units
 // the walk's memory traffic is priced in.
-template <bool ANY>
-__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const float4* __restrict__ recs, uint32_t root,
+// Walk records, for each node index x of an inner node a walk reaches: its
+// children A and B as two Nodes (4 float4 at recs[4x], dense, what the
+// binary step reads), and, for each child that is inner, that child's two
+// children (8 float4 at kids[8x]: A's, then B's). An inner child's words
+// are (its node index, 0); a leaf child keeps (first | LEAF, end). Both
+// live in one array of REC_ROWS float4 a node: recs for every node, then
+// kids for every node.
+constexpr uint32_t REC_ROWS = 12;
+
+// The walk's visit of inner node `cur` from its record: the binary step at
+// cur and, when it goes into an inner child, the step at that child too,
+// whose children's boxes ride in the same record. No triangle test comes
+// between the two, so both use the same best.t and make exactly the binary
+// walk's decisions and pushes; the record's three 64-B lines are read at
+// once (r = recs + 4 cur, k = kids + 8 cur). False when nothing is entered
+// (the caller pops).
+// visit1 is the binary step alone (the record's first line), for kernels
+// whose registers have no room for the other two lines.
+__device__ __forceinline__ bool visit1(const float4* __restrict__ r, V3 o, V3 inv, float best_t, uint32_t& cur, uint32_t& end,
+                                       uint32_t* stk, uint32_t stride, uint32_t& sp, uint32_t& steps) {
+    const float4 llo = r[0], lhi = r[1], qlo = r[2], qhi = r[3];
+    ++steps;
+    const float tl = box_t(llo, lhi, o, inv, best_t);
+    const float tr = box_t(qlo, qhi, o, inv, best_t);
+    if (tl < PHR_INF && tr < PHR_INF) {
+        if (tl <= tr) {
+            stk[sp * stride] = __float_as_uint(qlo.w);
+            cur = __float_as_uint(llo.w);
+            end = __float_as_uint(lhi.w);
+        } else {
+            stk[sp * stride] = __float_as_uint(llo.w);
+            cur = __float_as_uint(qlo.w);
+            end = __float_as_uint(qhi.w);
+        }
+        ++sp;
+        return true;
+    }
+    if (tl < PHR_INF) { cur = __float_as_uint(llo.w); end = __float_as_uint(lhi.w); return true; }
+    if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); return true; }
+    return false;
+}
+__device__ __forceinline__ bool visit2(const float4* __restrict__ r, const float4* __restrict__ k, V3 o, V3 inv, float best_t,
+                                       uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp, uint32_t& steps) {
+    const float4 alo = r[0], ahi = r[1], blo = r[2], bhi = r[3];
+    const float4 a0lo = k[0], a0hi = k[1], a1lo = k[2], a1hi = k[3];
+    const float4 b0lo = k[4], b0hi = k[5], b1lo = k[6], b1hi = k[7];
+    ++steps;
+    const float ta = box_t(alo, ahi, o, inv, best_t);
+    const float tb = box_t(blo, bhi, o, inv, best_t);
+    bool into_b;
+    if (ta < PHR_INF && tb < PHR_INF) {
+        into_b = !(ta <= tb);
+        stk[sp * stride] = __float_as_uint(into_b ? alo.w : blo.w);
+        ++sp;
+    } else if (ta < PHR_INF) {
+        into_b = false;
+    } else if (tb < PHR_INF) {
+        into_b = true;
+    } else {
+        return false;
+    }
+    const uint32_t cw = __float_as_uint(into_b ? blo.w : alo.w);
+    if (cw & PHR_LEAF) {
+        cur = cw;
+        end = __float_as_uint(into_b ? bhi.w : ahi.w);
+        return true;
+    }
+    ++steps;
+    const float4 llo = into_b ? b0lo : a0lo, lhi = into_b ? b0hi : a0hi;
+    const float4 qlo = into_b ? b1lo : a1lo, qhi = into_b ? b1hi : a1hi;
+    const float tl = box_t(llo, lhi, o, inv, best_t);
+    const float tr = box_t(qlo, qhi, o, inv, best_t);
+    if (tl < PHR_INF && tr < PHR_INF) {
+        if (tl <= tr) {
+            stk[sp * stride] = __float_as_uint(qlo.w);
+            cur = __float_as_uint(llo.w);
+            end = __float_as_uint(lhi.w);
+        } else {
+            stk[sp * stride] = __float_as_uint(llo.w);
+            cur = __float_as_uint(qlo.w);
+            end = __float_as_uint(qhi.w);
+        }
+        ++sp;
+        return true;
+    }
+    if (tl < PHR_INF) { cur = __float_as_uint(llo.w); end = __float_as_uint(lhi.w); return true; }
+    if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); return true; }
+    return false;
+}
+
+// TWO: a visit takes two levels from the record (visit2), else one (visit1);
+// the same decisions either way.
+template <bool ANY, bool TWO = false>
+__device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const float4* __restrict__ recs, const float4* __restrict__ kids,
+                                       uint32_t root,
                                        const float4* __restrict__ tris, const uint4* __restrict__ faces, const float* __restrict__ mats,
                                        V3 o, V3 d, float tmax, uint32_t* stk, uint32_t stride, uint32_t& steps, uint32_t& tests) {
     PhrHit best{tmax, PHR_NONE, 0.f, 0.f, 0u};
@@ -164,27 +258,9 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const f
                     best = PhrHit{t, id, u, v, 0u};
                 }
             } while (++id < end);
-        } else {
-            ++steps;
-            const float4* r = recs + 4 * (size_t)cur;
-            float4 llo = r[0], lhi = r[1], qlo = r[2], qhi = r[3];
-            float tl = box_t(llo, lhi, o, inv, best.t);
-            float tr = box_t(qlo, qhi, o, inv, best.t);
-            if (tl < PHR_INF && tr < PHR_INF) {
-                if (tl <= tr) {
-                    stk[sp * stride] = __float_as_uint(qlo.w);
-                    cur = __float_as_uint(llo.w);
-                    end = __float_as_uint(lhi.w);
-                } else {
-                    stk[sp * stride] = __float_as_uint(llo.w);
-                    cur = __float_as_uint(qlo.w);
-                    end = __float_as_uint(qhi.w);
-                }
-                ++sp;
-                continue;
-            }
-            if (tl < PHR_INF) { cur = __float_as_uint(llo.w); end = __float_as_uint(lhi.w); continue; }
-            if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); continue; }
+        } else if (TWO ? visit2(recs + 4 * (size_t)cur, kids + 8 * (size_t)cur, o, inv, best.t, cur, end, stk, stride, sp, steps)
+                       : visit1(recs + 4 * (size_t)cur, o, inv, best.t, cur, end, stk, stride, sp, steps)) {
+            continue;
         }
         if (sp == 0) break;
         --sp;
@@ -197,23 +273,36 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const f
 // The walk records of a tree's nodes (one entry per node index; only inner
 // nodes a walk reaches get one, as `inner` marks them), and each reachable
 // leaf's end in the cache row of its first triangle.
+__device__ __forceinline__ void rec_child(const float4* __restrict__ nodes, uint32_t y, float4* __restrict__ out,
+                                          float4* __restrict__ tris) {
+    float4 lo = nodes[2 * (size_t)y], hi = nodes[2 * (size_t)y + 1];
+    const uint32_t w0 = __float_as_uint(lo.w);
+    if (w0 & PHR_LEAF) {
+        tris[3 * (size_t)(w0 & ~PHR_LEAF) + 2].y = hi.w;  // the leaf's end
+    } else {
+        lo.w = __uint_as_float(y);
+        hi.w = 0.f;
+    }
+    out[0] = lo;
+    out[1] = hi;
+}
 __global__ __launch_bounds__(256) void k_walk_records(const float4* __restrict__ nodes, const uint8_t* __restrict__ inner,
                                                       uint32_t count, float4* __restrict__ recs, float4* __restrict__ tris) {
     const uint32_t x = blockIdx.x * 256 + threadIdx.x;
     if (x >= count || !inner[x]) return;
+    float4* r = recs + 4 * (size_t)x;
+    float4* k = recs + 4 * (size_t)count + 8 * (size_t)x;
     const uint32_t kid[2] = {__float_as_uint(nodes[2 * (size_t)x].w), __float_as_uint(nodes[2 * (size_t)x + 1].w)};
     for (int c = 0; c < 2; ++c) {
         const uint32_t y = kid[c];
-        float4 lo = nodes[2 * (size_t)y], hi = nodes[2 * (size_t)y + 1];
-        const uint32_t w0 = __float_as_uint(lo.w);
-        if (w0 & PHR_LEAF) {
-            tris[3 * (size_t)(w0 & ~PHR_LEAF) + 2].y = hi.w;  // the leaf's end
-        } else {
-            lo.w = __uint_as_float(y);
-            hi.w = 0.f;
-        }
-        recs[4 * (size_t)x + 2 * c] = lo;
-        recs[4 * (size_t)x + 2 * c + 1] = hi;
+        rec_child(nodes, y, r + 2 * c, tris);
+        const float4 ylo = nodes[2 * (size_t)y];
+        if (__float_as_uint(ylo.w) & PHR_LEAF) continue;
+        // An inner child's own children (their leaf ends are written by
+        // that child's record too).
+        const uint32_t g0 = __float_as_uint(ylo.w), g1 = __float_as_uint(nodes[2 * (size_t)y + 1].w);
+        rec_child(nodes, g0, k + 4 * c, tris);
+        rec_child(nodes, g1, k + 4 * c + 2, tris);
     }
 }
 
@@ -375,8 +464,8 @@ __global__ __launch_bounds__(TRACE_THREADS) void k_raycast(SceneDev sc, const Ph
             h = any ? walk_forest<true>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests)
                     : walk_forest<false>(sc, o, d, a.w, s_tstack + threadIdx.x, stk, TRACE_THREADS, steps, tests);
         else
-            h = any ? walk<true>(sc.nodes, sc.recs, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
-                    : walk<false>(sc.nodes, sc.recs, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
+            h = any ? walk<true, true>(sc.nodes, sc.recs, sc.kids, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests)
+                    : walk<false, true>(sc.nodes, sc.recs, sc.kids, sc.root, sc.tris, sc.faces, sc.mats, o, d, a.w, stk, TRACE_THREADS, steps, tests);
         hits[i] = h;
     }
     if (counters) {
@@ -443,12 +532,12 @@ __device__ __forceinline__ PhrHit trace(Ctx& c, V3 o, V3 d, float tmax, bool fir
     if (first) {
         c.shadow++;
         if constexpr (F) return walk_forest<true>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
-        return walk<true>(c.s.nodes, c.s.recs, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
+        return walk<true>(c.s.nodes, c.s.recs, c.s.kids, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
                           c.tests);
     }
     c.closest++;
     if constexpr (F) return walk_forest<false>(c.s, o, d, tmax, c.tstk, c.stk, TRACE_THREADS, c.steps, c.tests);
-    return walk<false>(c.s.nodes, c.s.recs, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
+    return walk<false>(c.s.nodes, c.s.recs, c.s.kids, c.s.root, c.s.tris, c.s.faces, c.s.mats, o, d, tmax, c.stk, TRACE_THREADS, c.steps,
                        c.tests);
 }
 template <bool F>
@@ -981,14 +1070,302 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
     }
 }
 
-// The same samples as path_block for one tree, with no Adaptive Sampling
-// and no studio rig (the camera has lamps or a lit world), with path
-// regeneration: a lane's sample is a state advanced one segment a turn (a
-// closest ray, its shading, at most one shadow ray, which every lane's turn
-// shares), and a lane whose sample ends starts its pixel's next sample on
-// the next turn instead of waiting for the wave's longest path. Each
+// ---- one segment of a sample: the regenerating builds' unit --------------------
+// A sample of shade + indirect for one tree, with no Adaptive Sampling and no
+// studio rig (the camera has lamps or a lit world), as a state advanced one
+// segment a turn: a closest ray, its shading, at most one shadow ray. Each
 // sample's arithmetic and random numbers are shade's and indirect's, in the
-// same order, so the sums are the same bits.
+// same order, so the sums are the same bits. k_path_rg keeps the state in
+// registers; the wavefront build (k_wf_*) keeps it in HBM between kernels.
+struct RgConst {  // the camera's constants (shade.wgsl:41-64, integrate.wgsl:111-123)
+    uint32_t see_through, passes, depth;
+    bool diffuse_on, lit, ao_only;
+};
+__device__ __forceinline__ RgConst rg_const(const PhrCamera* p) {
+    RgConst k;
+    const bool depth_set = p->depth[0] != 0u;
+    const uint32_t total = depth_set ? p->depth[0] - 1u : 8u;
+    k.see_through = depth_set ? p->depth[1] - 1u : 8u;
+    const uint32_t through = depth_set ? min(p->depth[2] - 1u, total) : 8u;
+    k.diffuse_on = !depth_set || (p->bounces > 0u && total > 0u);
+    k.passes = k.see_through + through + 1u;
+    k.lit = world_lit(p);
+    k.ao_only = p->glow == 0u && !k.lit;
+    k.depth = k.diffuse_on ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
+    return k;
+}
+// The sample in flight. phase 0: a new sample; 1: the camera ray's segments
+// (shade's pass loop); 2: the diffuse bounces.
+struct RgSample {
+    uint32_t seed, pass, n_see, b, phase;
+    V3 o, d, col, cc, alb, acc, thr, nrm;
+    float p_diffuse, min_pdf;
+};
+// The sample's guides, set by its first segment (shade's `out`).
+struct RgGuide {
+    V3 alb, nrm;
+    float dep, missed, cov;
+    uint32_t outline;
+};
+// What a turn leaves for its shadow ray and for after it. end: 0 the sample
+// goes on; 1 it ends with col as it is; 2 the camera ray's direct light then
+// no bounce; 3 the ambient-occlusion bounce; 4 the diffuse bounces end; 5
+// they end after this turn's lamp.
+struct RgTurn {
+    bool qon, qdirect, to_bounces;
+    uint32_t end;
+    V3 qpos, qdir, qc, thr_was;
+    float qdist, near;
+};
+
+// Phase 0: sample `frame` of pixel i starts (shade.wgsl:65-95).
+__device__ __forceinline__ void rg_start(RgSample& st, RgGuide& g, const PhrCamera* p, uint32_t i, uint32_t frame) {
+    const float fw = (float)p->w, fh = (float)p->h;
+    const float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
+    const float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
+    st.seed = pcg_hash(i * 9781u + (frame + p->stream) * 6271u + 1u);
+    const float ju = rnd(&st.seed);
+    const float jv = rnd(&st.seed);
+    const float jx = (ju - 0.5f) / fw;
+    const float jy = (jv - 0.5f) / fh;
+    const float vx = (su + jx - p->pan[0]) / p->scale;
+    const float vy = -((sv + jy) / p->aspect - p->pan[1]) / p->scale;
+    if (p->persp != 0u) {
+        st.o = ld3(p->pivot) + ld3(p->back) * (p->dist / p->units);
+        st.d = normalize(ld3(p->right) * vx + ld3(p->up) * vy - ld3(p->back) * p->dist);
+    } else {
+        st.o = ld3(p->pivot) + (ld3(p->right) * vx + ld3(p->up) * vy + ld3(p->back) * 1.0e4f) / p->units;
+        st.d = -ld3(p->back);
+    }
+    st.col = v3(0, 0, 0);
+    g.missed = 0.f;
+    g.cov = 0.f;
+    g.outline = 0u;
+    st.pass = 0u;
+    st.n_see = 0u;
+    st.phase = 1u;
+}
+
+// The turn's closest hit h: shading up to the shadow ray (shade.wgsl:96-302
+// for a camera segment, integrate.wgsl:124-228 for a bounce).
+__device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, const SceneDev& sc, const PhrCamera* p, const RgConst& k,
+                                       const PhrHit& h) {
+    const V3 one = v3(1, 1, 1);  // shade's throughput, never changed
+    q.qon = q.qdirect = q.to_bounces = false;
+    q.qpos = v3(0, 0, 0);
+    q.qdir = v3(0, 0, 1);
+    q.qc = v3(0, 0, 0);
+    q.qdist = 0.f;
+    q.near = 1.0f;
+    q.thr_was = st.thr;
+    q.end = 0u;
+    if (st.phase == 1u) {
+        if (h.tri == PHR_NONE) {
+            if (st.pass == 0u) {
+                g.alb = v3(0, 0, 0);
+                g.nrm = -st.d;
+                g.dep = -1.0f;
+                st.col = v3(0, 0, 0);
+                g.missed = 1.0f;
+                g.cov = 1.0f;
+            } else {
+                st.col = st.col + one * env_rgb(p);
+            }
+            q.end = 1u;
+            return;
+        }
+        if (st.pass == 0u) g.outline = h.inst + 1u;
+        const uint32_t m = mat_at(sc, h.tri);
+        const Tri t = tri_of<false>(sc, h.tri, h.inst, m);
+        const float w = 1.0f - h.u - h.v;
+        const V3 hit = st.o + st.d * h.t;
+        const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        const bool front = dot(ng0, st.d) < 0.f;
+        const V3 ng = front ? ng0 : -ng0;
+        V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
+        sn = dot(sn, sn) > 1e-12f ? normalize(sn) * (front ? 1.0f : -1.0f) : ng;
+        if (dot(sn, -st.d) < 0.f) sn = ng;
+        const float start = 1.0e4f / p->units;
+        const float lift = (p->persp == 0u && h.t > start * 0.5f) ? p->eps + fabsf(h.t - start) * 1e-5f + start * 2.4e-7f
+                                                                  : p->eps + h.t * 1e-5f;
+        const V3 pos = hit + ng * lift;
+        const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+        const float alpha = t.color[3];
+        if (st.pass == 0u) {
+            g.alb = base;
+            g.nrm = sn;
+            g.dep = h.t;
+        }
+        if (rnd(&st.seed) >= alpha) {
+            st.n_see++;
+            if (st.n_see > k.see_through) {
+                q.end = 1u;
+            } else {
+                const float reach = fmax_(fmax_(fabsf(hit.x), fabsf(hit.y)), fmax_(fabsf(hit.z), 0.5f));
+                const float tl = lift - p->eps + reach * (256.0f / 8388608.0f);
+                st.o = hit - ng * tl;
+                if (++st.pass >= k.passes) q.end = 1u;
+            }
+            return;
+        }
+        st.col = st.col + one * t.emit;
+        const V3 fd = dielectric_f0(t.ior);
+        V3 diffuse = base * (1.0f - t.metal);
+        if (maxc(diffuse) > 0.f) diffuse = diffuse * under_layer_real(t.ior, fmax_(t.rough, 0.04f), fmax_(dot(sn, -st.d), 1e-4f));
+        const Surf sf{pos, sn, -st.d, diffuse, mixv(fd, base, t.metal), fmax_(t.rough, 0.04f), t.ior, 1.0f - t.metal};
+        // direct (integrate.wgsl:66-85) without the rig: the lamp pick's
+        // number, then one lamp.
+        (void)rnd(&st.seed);
+        const Shadowed sh = glow_query<false>(sc, sf, &st.seed);
+        if (maxc(sh.c) > 0.f) {
+            q.qon = q.qdirect = true;
+            q.qpos = sf.pos;
+            q.qdir = sh.dir;
+            q.qdist = sh.dist;
+            q.qc = sh.c;
+        }
+        const float pd = dot(vmax(sf.albedo, 0.0f), v3(0.2126f, 0.7152f, 0.0722f));
+        const float pg = 0.0f;
+        st.p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
+        const bool take_diffuse = rnd(&st.seed) < st.p_diffuse;
+        st.alb = sf.albedo;
+        q.end = 2u;
+        if (take_diffuse && pd > 0.f) {
+            if (k.ao_only) {  // integrate.wgsl:113-117
+                q.qon = true;
+                q.qpos = pos;
+                q.qdir = cosine_dir(sn, &st.seed);
+                q.qdist = p->ao;
+                q.end = 3u;
+            } else {  // the first bounce's ray (integrate.wgsl:124-128)
+                st.acc = v3(0, 0, 0);
+                st.thr = v3(1, 1, 1);
+                st.o = pos;
+                st.nrm = sn;
+                st.min_pdf = 3.0e38f;
+                st.b = 0u;
+                st.d = cosine_dir(st.nrm, &st.seed);
+                const float pb = fmax_(dot(st.nrm, st.d), 0.0f) / PHR_PI;
+                st.min_pdf = fmin_(st.min_pdf, pb);
+                q.end = 0u;
+                q.to_bounces = true;
+            }
+        }
+        return;
+    }
+    // phase 2: bounce b
+    if (h.tri == PHR_NONE) {
+        st.acc = st.acc + st.thr * env_rgb(p) * 1.0f;
+        q.end = 4u;
+        return;
+    }
+    if (!k.diffuse_on) {
+        q.end = 4u;
+        return;
+    }
+    const Tri t = tri_of<false>(sc, h.tri, h.inst, mat_at(sc, h.tri));
+    const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+    const V3 ng = dot(ng0, st.d) < 0.f ? ng0 : -ng0;
+    const V3 hit = st.o + st.d * h.t + ng * (p->eps + h.t * 1e-5f);
+    const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
+    const float rough = fmax_(t.rough, glossy_floor(p, st.min_pdf));
+    const bool glossy = p->integrator[0] < 0.5f || (((uint32_t)(p->integrator[3] + 0.5f)) & 1u) != 0u;
+    const V3 fd = glossy ? dielectric_f0(t.ior) : v3(0, 0, 0);
+    const float metal = glossy ? t.metal : 0.0f;
+    const float tw = t.trans * (1.0f - t.metal);
+    V3 albedo = base * (1.0f - t.metal) * (1.0f - tw);
+    if (maxc(albedo) > 0.f) albedo = albedo * under_layer(fd, rough, fmax_(dot(ng, -st.d), 1e-4f));
+    st.acc = st.acc + st.thr * glow_mis(sc, t, h, st.nrm, st.d);
+    q.near = (!k.lit && h.t <= p->ao) ? 0.25f : 1.0f;
+    const Surf sf{hit, ng, -st.d, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
+    (void)rnd(&st.seed);
+    const Shadowed sh = glow_query<false>(sc, sf, &st.seed);
+    if (maxc(sh.c) > 0.f) {
+        q.qon = q.qdirect = true;
+        q.qpos = sf.pos;
+        q.qdir = sh.dir;
+        q.qdist = sh.dist;
+        q.qc = sh.c;
+    }
+    q.thr_was = st.thr;
+    st.thr = st.thr * albedo;
+    bool more = st.b + 1u < k.depth;
+    if (more && st.b + 1u >= p->rr_from) {
+        const float keep_p = clampf(maxc(st.thr), 0.0f, 0.95f);
+        if (keep_p < 1e-4f || rnd(&st.seed) > keep_p) more = false;
+        else st.thr = st.thr / keep_p;
+    }
+    if (more) {
+        st.o = hit;
+        st.nrm = ng;
+        ++st.b;
+        st.d = cosine_dir(st.nrm, &st.seed);
+        const float pb = fmax_(dot(st.nrm, st.d), 0.0f) / PHR_PI;
+        st.min_pdf = fmin_(st.min_pdf, pb);
+    } else {
+        q.end = 5u;  // after this turn's lamp: the bounces end
+    }
+}
+
+// What a turn's end changes of its sample: the radiance, the camera hit's
+// light and the bounces' sum, and the phase.
+struct RgPost {
+    V3 col, cc, acc;
+    uint32_t phase;
+};
+// After the turn's shadow ray (vis: 1 lit or no ray, 0 blocked). True when
+// the sample has ended; its radiance is then r.col.
+__device__ __forceinline__ bool rg_post_of(RgPost& r, V3 alb, float p_diffuse, const RgTurn& q, const PhrCamera* p, float vis) {
+    const V3 one = v3(1, 1, 1);
+    uint32_t end = q.end;
+    V3 dsum = v3(0, 0, 0);
+    if (q.qdirect) dsum = dsum + q.qc * vis;
+    if (r.phase == 1u && (end >= 2u || q.to_bounces)) {
+        r.cc = dsum * 1.0f;
+        if (end == 2u) {
+            r.cc = r.cc + v3(0, 0, 0);
+            r.col = r.col + one * r.cc;
+            end = 1u;
+        } else if (end == 3u) {
+            V3 bounced = v3(0, 0, 0);
+            bounced = bounced + alb * (env_rgb(p) * mixf(0.25f, 1.0f, vis)) / p_diffuse;
+            r.cc = r.cc + bounced;
+            r.col = r.col + one * r.cc;
+            end = 1u;
+        }
+    } else if (r.phase == 2u) {
+        if (end == 5u) {
+            r.acc = r.acc + q.thr_was * dsum * q.near;
+            end = 4u;
+        } else if (end == 0u) {
+            r.acc = r.acc + q.thr_was * dsum * q.near;
+        }
+        if (end == 4u) {
+            V3 bounced = v3(0, 0, 0);
+            bounced = bounced + alb * r.acc / p_diffuse;
+            r.cc = r.cc + bounced;
+            r.col = r.col + one * r.cc;
+            end = 1u;
+        }
+    }
+    if (q.to_bounces) r.phase = 2u;
+    return end == 1u;
+}
+__device__ __forceinline__ bool rg_post(RgSample& st, const RgTurn& q, const PhrCamera* p, float vis) {
+    RgPost r{st.col, st.cc, st.acc, st.phase};
+    const bool ended = rg_post_of(r, st.alb, st.p_diffuse, q, p, vis);
+    st.col = r.col;
+    st.cc = r.cc;
+    st.acc = r.acc;
+    st.phase = r.phase;
+    return ended;
+}
+
+// The same samples as path_block for one tree, with no Adaptive Sampling
+// and no studio rig, with path regeneration: each lane advances its sample
+// one segment a turn (every lane's closest ray and at most one shadow ray
+// share a call site), and a lane whose sample ends starts its pixel's next
+// sample on the next turn instead of waiting for the wave's longest path.
 template <int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_rg(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
@@ -1019,261 +1396,40 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         float cov = keep ? ldf(px, 23 * n + i) : 0.f;
         float gone = keep ? ldf(px, 33 * n + i) : 0.f;
         uint32_t outline = 0;
-        // The sample's constants (shade.wgsl:41-64, integrate.wgsl:111-123).
-        const bool depth_set = p->depth[0] != 0u;
-        const uint32_t total = depth_set ? p->depth[0] - 1u : 8u;
-        const uint32_t see_through = depth_set ? p->depth[1] - 1u : 8u;
-        const uint32_t through = depth_set ? min(p->depth[2] - 1u, total) : 8u;
-        const bool diffuse_on = !depth_set || (p->bounces > 0u && total > 0u);
-        const uint32_t passes = see_through + through + 1u;
-        const bool lit = world_lit(p);
-        const bool ao_only = p->glow == 0u && !lit;
-        const uint32_t depth = diffuse_on ? (p->bounces != 0u ? p->bounces : 1u) : 1u;
-        const V3 one = v3(1, 1, 1);  // shade's throughput, never changed
-        // The sample in flight. phase 0: a new sample; 1: the camera ray's
-        // segments (shade's pass loop); 2: the diffuse bounces.
-        uint32_t s = 0, seed = 0, pass = 0, n_see = 0, b = 0, phase = 0;
-        V3 o = v3(0, 0, 0), d = v3(0, 0, 0), col = v3(0, 0, 0);
-        V3 g_alb = v3(0, 0, 0), g_nrm = v3(0, 0, 0);
-        float g_dep = 0.f, g_missed = 0.f, g_cov = 0.f;
-        uint32_t g_outline = 0u;
-        V3 cc = v3(0, 0, 0), alb = v3(0, 0, 0);
-        float p_diffuse = 1.0f;
-        V3 acc = v3(0, 0, 0), thr = v3(1, 1, 1), nrm = v3(0, 0, 0);
-        float min_pdf = 3.0e38f;
+        const RgConst k = rg_const(p);
+        RgSample st;
+        st.seed = 0;
+        st.pass = st.n_see = st.b = st.phase = 0u;
+        st.o = st.d = st.col = st.cc = st.alb = st.acc = st.nrm = v3(0, 0, 0);
+        st.thr = v3(1, 1, 1);
+        st.p_diffuse = 1.0f;
+        st.min_pdf = 3.0e38f;
+        RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
+        uint32_t s = 0;
         while (s < spp) {
-            if (phase == 0u) {  // shade.wgsl:65-95
-                const float fw = (float)p->w, fh = (float)p->h;
-                const float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
-                const float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
-                seed = pcg_hash(i * 9781u + ((frame0 + s) + p->stream) * 6271u + 1u);
-                const float ju = rnd(&seed);
-                const float jv = rnd(&seed);
-                const float jx = (ju - 0.5f) / fw;
-                const float jy = (jv - 0.5f) / fh;
-                const float vx = (su + jx - p->pan[0]) / p->scale;
-                const float vy = -((sv + jy) / p->aspect - p->pan[1]) / p->scale;
-                if (p->persp != 0u) {
-                    o = ld3(p->pivot) + ld3(p->back) * (p->dist / p->units);
-                    d = normalize(ld3(p->right) * vx + ld3(p->up) * vy - ld3(p->back) * p->dist);
-                } else {
-                    o = ld3(p->pivot) + (ld3(p->right) * vx + ld3(p->up) * vy + ld3(p->back) * 1.0e4f) / p->units;
-                    d = -ld3(p->back);
-                }
-                col = v3(0, 0, 0);
-                g_missed = 0.f;
-                g_cov = 0.f;
-                g_outline = 0u;
-                pass = 0u;
-                n_see = 0u;
-                phase = 1u;
-            }
-            const PhrHit h = trace<false>(c, o, d, PHR_INF, false);
-            // This turn's shadow ray: a lamp for the direct light (qdirect)
-            // or the ambient-occlusion ray of a world without light.
-            bool qon = false, qdirect = false;
-            V3 qpos = v3(0, 0, 0), qdir = v3(0, 0, 1), qc = v3(0, 0, 0);
-            float qdist = 0.f, near = 1.0f;
-            V3 thr_was = thr;
-            // What ends this turn: 0 the sample goes on; 1 it ends with col as
-            // it is; 2 the camera ray's direct light then no bounce; 3 the
-            // ambient-occlusion bounce; 4 the diffuse bounces end.
-            uint32_t end = 0u;
-            bool to_bounces = false;
-            if (phase == 1u) {  // shade.wgsl:96-302, one pass
-                if (h.tri == PHR_NONE) {
-                    if (pass == 0u) {
-                        g_alb = v3(0, 0, 0);
-                        g_nrm = -d;
-                        g_dep = -1.0f;
-                        col = v3(0, 0, 0);
-                        g_missed = 1.0f;
-                        g_cov = 1.0f;
-                    } else {
-                        col = col + one * env_rgb(p);
-                    }
-                    end = 1u;
-                } else {
-                    if (pass == 0u) g_outline = h.inst + 1u;
-                    const uint32_t m = mat_at(c.s, h.tri);
-                    const Tri t = tri_of<false>(c.s, h.tri, h.inst, m);
-                    const float w = 1.0f - h.u - h.v;
-                    const V3 hit = o + d * h.t;
-                    const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
-                    const bool front = dot(ng0, d) < 0.f;
-                    const V3 ng = front ? ng0 : -ng0;
-                    V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
-                    sn = dot(sn, sn) > 1e-12f ? normalize(sn) * (front ? 1.0f : -1.0f) : ng;
-                    if (dot(sn, -d) < 0.f) sn = ng;
-                    const float start = 1.0e4f / p->units;
-                    const float lift = (p->persp == 0u && h.t > start * 0.5f) ? p->eps + fabsf(h.t - start) * 1e-5f + start * 2.4e-7f
-                                                                              : p->eps + h.t * 1e-5f;
-                    const V3 pos = hit + ng * lift;
-                    const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
-                    const float alpha = t.color[3];
-                    if (pass == 0u) {
-                        g_alb = base;
-                        g_nrm = sn;
-                        g_dep = h.t;
-                    }
-                    if (rnd(&seed) >= alpha) {
-                        n_see++;
-                        if (n_see > see_through) {
-                            end = 1u;
-                        } else {
-                            const float reach = fmax_(fmax_(fabsf(hit.x), fabsf(hit.y)), fmax_(fabsf(hit.z), 0.5f));
-                            const float tl = lift - p->eps + reach * (256.0f / 8388608.0f);
-                            o = hit - ng * tl;
-                            if (++pass >= passes) end = 1u;
-                        }
-                    } else {
-                        col = col + one * t.emit;
-                        const V3 fd = dielectric_f0(t.ior);
-                        V3 diffuse = base * (1.0f - t.metal);
-                        if (maxc(diffuse) > 0.f)
-                            diffuse = diffuse * under_layer_real(t.ior, fmax_(t.rough, 0.04f), fmax_(dot(sn, -d), 1e-4f));
-                        const Surf sf{pos, sn, -d, diffuse, mixv(fd, base, t.metal), fmax_(t.rough, 0.04f), t.ior, 1.0f - t.metal};
-                        // direct (integrate.wgsl:66-85) without the rig: the
-                        // lamp pick's number, then one lamp.
-                        (void)rnd(&seed);
-                        const Shadowed q = glow_query<false>(c.s, sf, &seed);
-                        if (maxc(q.c) > 0.f) {
-                            qon = qdirect = true;
-                            qpos = sf.pos;
-                            qdir = q.dir;
-                            qdist = q.dist;
-                            qc = q.c;
-                        }
-                        const float pd = dot(vmax(sf.albedo, 0.0f), v3(0.2126f, 0.7152f, 0.0722f));
-                        const float pg = 0.0f;
-                        p_diffuse = (pd + pg > 0.f) ? pd / (pd + pg) : 1.0f;
-                        const bool take_diffuse = rnd(&seed) < p_diffuse;
-                        alb = sf.albedo;
-                        end = 2u;
-                        if (take_diffuse && pd > 0.f) {
-                            if (ao_only) {  // integrate.wgsl:113-117
-                                qon = true;
-                                qpos = pos;
-                                qdir = cosine_dir(sn, &seed);
-                                qdist = p->ao;
-                                end = 3u;
-                            } else {  // the first bounce's ray (integrate.wgsl:124-128)
-                                acc = v3(0, 0, 0);
-                                thr = v3(1, 1, 1);
-                                o = pos;
-                                nrm = sn;
-                                min_pdf = 3.0e38f;
-                                b = 0u;
-                                d = cosine_dir(nrm, &seed);
-                                const float pb = fmax_(dot(nrm, d), 0.0f) / PHR_PI;
-                                min_pdf = fmin_(min_pdf, pb);
-                                end = 0u;
-                                to_bounces = true;
-                            }
-                        }
-                    }
-                }
-            } else {  // phase 2: bounce b (integrate.wgsl:124-228)
-                if (h.tri == PHR_NONE) {
-                    acc = acc + thr * env_rgb(p) * 1.0f;
-                    end = 4u;
-                } else if (!diffuse_on) {
-                    end = 4u;
-                } else {
-                    const Tri t = tri_of<false>(c.s, h.tri, h.inst, mat_at(c.s, h.tri));
-                    const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
-                    const V3 ng = dot(ng0, d) < 0.f ? ng0 : -ng0;
-                    const V3 hit = o + d * h.t + ng * (p->eps + h.t * 1e-5f);
-                    const V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
-                    const float rough = fmax_(t.rough, glossy_floor(p, min_pdf));
-                    const bool glossy = p->integrator[0] < 0.5f || (((uint32_t)(p->integrator[3] + 0.5f)) & 1u) != 0u;
-                    const V3 fd = glossy ? dielectric_f0(t.ior) : v3(0, 0, 0);
-                    const float metal = glossy ? t.metal : 0.0f;
-                    const float tw = t.trans * (1.0f - t.metal);
-                    V3 albedo = base * (1.0f - t.metal) * (1.0f - tw);
-                    if (maxc(albedo) > 0.f) albedo = albedo * under_layer(fd, rough, fmax_(dot(ng, -d), 1e-4f));
-                    acc = acc + thr * glow_mis(c.s, t, h, nrm, d);
-                    near = (!lit && h.t <= p->ao) ? 0.25f : 1.0f;
-                    const Surf sf{hit, ng, -d, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
-                    (void)rnd(&seed);
-                    const Shadowed q = glow_query<false>(c.s, sf, &seed);
-                    if (maxc(q.c) > 0.f) {
-                        qon = qdirect = true;
-                        qpos = sf.pos;
-                        qdir = q.dir;
-                        qdist = q.dist;
-                        qc = q.c;
-                    }
-                    thr_was = thr;
-                    thr = thr * albedo;
-                    bool more = b + 1u < depth;
-                    if (more && b + 1u >= p->rr_from) {
-                        const float keep_p = clampf(maxc(thr), 0.0f, 0.95f);
-                        if (keep_p < 1e-4f || rnd(&seed) > keep_p) more = false;
-                        else thr = thr / keep_p;
-                    }
-                    if (more) {
-                        o = hit;
-                        nrm = ng;
-                        ++b;
-                        d = cosine_dir(nrm, &seed);
-                        const float pb = fmax_(dot(nrm, d), 0.0f) / PHR_PI;
-                        min_pdf = fmin_(min_pdf, pb);
-                    } else {
-                        end = 5u;  // after this turn's lamp: the bounces end
-                    }
-                }
-            }
+            if (st.phase == 0u) rg_start(st, g, p, i, frame0 + s);
+            const PhrHit h = trace<false>(c, st.o, st.d, PHR_INF, false);
+            RgTurn q;
+            rg_hit(st, g, q, c.s, p, k, h);
             // The turn's shadow ray, one call site for every lane.
             float vis = 1.0f;
-            if (qon) vis = trace<false>(c, qpos, qdir, qdist, true).tri != PHR_NONE ? 0.0f : 1.0f;
-            V3 dsum = v3(0, 0, 0);
-            if (qdirect) dsum = dsum + qc * vis;
-            if (phase == 1u && (end >= 2u || to_bounces)) {
-                cc = dsum * 1.0f;
-                if (end == 2u) {
-                    cc = cc + v3(0, 0, 0);
-                    col = col + one * cc;
-                    end = 1u;
-                } else if (end == 3u) {
-                    V3 bounced = v3(0, 0, 0);
-                    bounced = bounced + alb * (env_rgb(p) * mixf(0.25f, 1.0f, vis)) / p_diffuse;
-                    cc = cc + bounced;
-                    col = col + one * cc;
-                    end = 1u;
-                }
-            } else if (phase == 2u) {
-                if (end == 5u) {
-                    acc = acc + thr_was * dsum * near;
-                    end = 4u;
-                } else if (end == 0u) {
-                    acc = acc + thr_was * dsum * near;
-                }
-                if (end == 4u) {
-                    V3 bounced = v3(0, 0, 0);
-                    bounced = bounced + alb * acc / p_diffuse;
-                    cc = cc + bounced;
-                    col = col + one * cc;
-                    end = 1u;
-                }
-            }
-            if (to_bounces) phase = 2u;
-            if (end == 1u) {  // shade's `out`, summed as path_block sums it
-                ga = ga + g_alb;
-                gn = gn + g_nrm;
-                gd = gd + g_dep;
-                gone = gone + g_missed;
-                l = l + col;
-                cov = cov + g_cov;
-                outline = g_outline;
+            if (q.qon) vis = trace<false>(c, q.qpos, q.qdir, q.qdist, true).tri != PHR_NONE ? 0.0f : 1.0f;
+            if (rg_post(st, q, p, vis)) {  // shade's `out`, summed as path_block sums it
+                ga = ga + g.alb;
+                gn = gn + g.nrm;
+                gd = gd + g.dep;
+                gone = gone + g.missed;
+                l = l + st.col;
+                cov = cov + g.cov;
+                outline = g.outline;
                 ++s;
-                phase = 0u;
+                st.phase = 0u;
             }
         }
         V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
         float film = keep ? ldf(px, 24 * n + i) : 0.f;
         uint32_t taken = keep ? px[31 * n + i] : 0u;
-        for (uint32_t k = 0; k < spp; ++k) {
+        for (uint32_t kk = 0; kk < spp; ++kk) {
             gs = gs + v3(0.f, 0.f, 0.f);
             film = film + 1.0f;
         }
@@ -1291,19 +1447,455 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         stf(px, 33 * n + i, gone);
         put_shown(px, n, i, *p, l, gs, cov, gone, film, taken);
     }
-    unsigned long long rc = c.closest, rs = c.shadow, st = c.steps, tt = c.tests;
-    for (int k = 32; k > 0; k >>= 1) {
-        rc += __shfl_xor(rc, k);
-        rs += __shfl_xor(rs, k);
-        st += __shfl_xor(st, k);
-        tt += __shfl_xor(tt, k);
+    unsigned long long rc = c.closest, rs = c.shadow, st_ = c.steps, tt = c.tests;
+    for (int kk = 32; kk > 0; kk >>= 1) {
+        rc += __shfl_xor(rc, kk);
+        rs += __shfl_xor(rs, kk);
+        st_ += __shfl_xor(st_, kk);
+        tt += __shfl_xor(tt, kk);
     }
     if ((threadIdx.x & 63) == 0 && (rc | rs)) {
         atomicAdd(&rays[0], rc);
         atomicAdd(&rays[1], rs);
-        atomicAdd(&rays[2], st);
+        atomicAdd(&rays[2], st_);
         atomicAdd(&rays[3], tt);
     }
+}
+
+// ---- the wavefront build --------------------------------------------------------
+// The regenerating build's samples with the walks taken out of the shading.
+// A path slot a pixel of the 16 x 16 tiles keeps its sample's state in HBM;
+// a turn is three launches:
+//  * k_wf_walk<false> walks every queued slot's ray to its closest hit;
+//  * k_wf_shade advances each slot one segment (rg_hit). A turn with no
+//    shadow ray finishes there (rg_post); otherwise rg_post runs for both
+//    visibilities and the field they change (the ended sample's radiance,
+//    or the bounce sums) rides with the shadow ray. The next ray and the
+//    shadow ray are appended to their queues by wave ballots (one atomic a
+//    wave), and an ended sample's pixel starts its next sample at once;
+//  * k_wf_walk<true> walks the shadow rays and applies what each turn left.
+// The walkers are persistent: a lane whose walk ends takes the next queued
+// ray while the rest of its wave walks on (refilled once half the wave is
+// idle), so no wave waits for its longest walk, and they carry no shading
+// registers. Each pixel has one sample in flight, so its sums are taken in
+// sample order and are the bits k_path_rg writes.
+constexpr uint32_t WF_THREADS = 256;
+constexpr uint32_t WF_CTRS = 8;  // next-ray counts [2], extend head, shadow count, shadow head
+enum : uint32_t { WF_END = 0, WF_CC = 1, WF_ACC = 2 };
+
+struct WfDev {
+    float4* s0;   // o.xyz, min_pdf
+    float4* s1;   // d.xyz, p_diffuse
+    float4* s2;   // col.xyz, seed
+    float4* s3;   // cc.xyz, sample index
+    float4* s4;   // alb.xyz, phase | b << 2
+    float4* s5;   // acc.xyz, -
+    float4* s6;   // thr.xyz, pass
+    float4* s7;   // nrm.xyz, n_see
+    float4* hit;  // t, tri, u, v
+    uint32_t* q0;  // ray queues of slots, by turn parity
+    uint32_t* q1;
+    uint32_t* sq;  // the shadow queue of slots
+    float4* sh;    // a slot's shadow ray: (pos, dist) (dir, -) (candidate if blocked, kind) (candidate if lit, -)
+    uint8_t* vis;  // a slot's last shadow ray: 1 lit, 0 blocked
+    uint8_t* pend; // 1 + kind while a slot's shadow result waits to be taken in
+    uint32_t* ctr;
+};
+constexpr uint32_t WF_CHUNK = 256;       // queue entries a walker wave takes with one atomic
+constexpr uint32_t WF_SHADE_ITEMS = 4;   // entries a shading thread takes: a block's appends cost one atomic a queue
+
+__device__ __forceinline__ uint32_t wf_pixel(uint32_t slot, uint32_t tiles_x, uint32_t w, uint32_t h, bool& inside) {
+    const uint32_t tile = slot >> 8, r = slot & 255u;
+    const uint32_t x = (tile % tiles_x) * 16u + (r & 15u), y = (tile / tiles_x) * 16u + (r >> 4);
+    inside = x < w && y < h;
+    return y * w + x;
+}
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// The wave's lanes with `want` take consecutive places of a queue: one
+// atomic a wave. Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0u;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63u) == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    return base + lanes_below(m);
+}
+
+__device__ __forceinline__ void wf_load(const WfDev& w, uint32_t slot, RgSample& st, uint32_t& s) {
+    const float4 a = w.s0[slot], b = w.s1[slot], c = w.s2[slot], d = w.s3[slot];
+    const float4 e = w.s4[slot], f = w.s5[slot], g = w.s6[slot], k = w.s7[slot];
+    st.o = v3(a.x, a.y, a.z);
+    st.min_pdf = a.w;
+    st.d = v3(b.x, b.y, b.z);
+    st.p_diffuse = b.w;
+    st.col = v3(c.x, c.y, c.z);
+    st.seed = __float_as_uint(c.w);
+    st.cc = v3(d.x, d.y, d.z);
+    s = __float_as_uint(d.w);
+    st.alb = v3(e.x, e.y, e.z);
+    st.phase = __float_as_uint(e.w) & 3u;
+    st.b = __float_as_uint(e.w) >> 2;
+    st.acc = v3(f.x, f.y, f.z);
+    st.thr = v3(g.x, g.y, g.z);
+    st.pass = __float_as_uint(g.w);
+    st.nrm = v3(k.x, k.y, k.z);
+    st.n_see = __float_as_uint(k.w);
+}
+__device__ __forceinline__ void wf_store(const WfDev& w, uint32_t slot, const RgSample& st, uint32_t s) {
+    w.s0[slot] = make_float4(st.o.x, st.o.y, st.o.z, st.min_pdf);
+    w.s1[slot] = make_float4(st.d.x, st.d.y, st.d.z, st.p_diffuse);
+    w.s2[slot] = make_float4(st.col.x, st.col.y, st.col.z, __uint_as_float(st.seed));
+    w.s3[slot] = make_float4(st.cc.x, st.cc.y, st.cc.z, __uint_as_float(s));
+    w.s4[slot] = make_float4(st.alb.x, st.alb.y, st.alb.z, __uint_as_float(st.phase | (st.b << 2)));
+    w.s5[slot] = make_float4(st.acc.x, st.acc.y, st.acc.z, 0.f);
+    w.s6[slot] = make_float4(st.thr.x, st.thr.y, st.thr.z, __uint_as_float(st.pass));
+    w.s7[slot] = make_float4(st.nrm.x, st.nrm.y, st.nrm.z, __uint_as_float(st.n_see));
+}
+__device__ __forceinline__ void add3(uint32_t* px, size_t at, V3 v) {
+    stf(px, at, ldf(px, at) + v.x);
+    stf(px, at + 1, ldf(px, at + 1) + v.y);
+    stf(px, at + 2, ldf(px, at + 2) + v.z);
+}
+
+// Counters zeroed before a frame's first turn.
+__global__ void k_wf_zero(uint32_t* ctr) {
+    if (threadIdx.x < WF_CTRS) ctr[threadIdx.x] = 0u;
+}
+
+// Every slot of a pixel starts its first sample and joins the first queue;
+// a new image's sums start at zero.
+__global__ __launch_bounds__(WF_THREADS) void k_wf_init(const PhrCamera* __restrict__ camp, WfDev w, uint32_t tiles_x,
+                                                        uint32_t nslots, uint32_t frame0, uint32_t* __restrict__ px) {
+    const PhrCamera* p = camp;
+    const size_t n = (size_t)p->w * p->h;
+    const uint32_t slot = blockIdx.x * WF_THREADS + threadIdx.x;
+    bool inside = false;
+    const uint32_t i = slot < nslots ? wf_pixel(slot, tiles_x, p->w, p->h, inside) : 0u;
+    if (inside) {
+        if (frame0 == 0u) {
+            for (int c = 0; c < 3; ++c) {
+                stf(px, n + 3 * (size_t)i + c, 0.f);
+                stf(px, 7 * n + 3 * (size_t)i + c, 0.f);
+                stf(px, 10 * n + 3 * (size_t)i + c, 0.f);
+            }
+            stf(px, 13 * n + i, 0.f);
+            stf(px, 23 * n + i, 0.f);
+            stf(px, 33 * n + i, 0.f);
+            px[25 * n + i] = 0u;
+        }
+        RgSample st;
+        st.seed = 0;
+        st.pass = st.n_see = st.b = st.phase = 0u;
+        st.o = st.d = st.col = st.cc = st.alb = st.acc = st.nrm = v3(0, 0, 0);
+        st.thr = v3(1, 1, 1);
+        st.p_diffuse = 1.0f;
+        st.min_pdf = 3.0e38f;
+        RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
+        rg_start(st, g, p, i, frame0);
+        wf_store(w, slot, st, 0u);
+        w.pend[slot] = 0u;
+    }
+    // The block's slots in order, one atomic a block.
+    __shared__ uint32_t s_cnt, s_base;
+    if (threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
+    const uint32_t at = wave_append(inside, &s_cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(&w.ctr[0], s_cnt) : 0u;
+    __syncthreads();
+    if (inside) w.q0[s_base + at] = slot;
+}
+
+// The walks of one turn: closest hits of the queued rays (ANY false), or the
+// shadow rays and what each turn left (ANY true). Persistent: each wave takes
+// rays from the queue's head until it is drained.
+template <bool ANY>
+__global__ __launch_bounds__(WF_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_wf_walk(SceneDev sc, const PhrCamera* __restrict__ camp, WfDev w, uint32_t parity,
+                                                        uint32_t tiles_x, uint32_t* __restrict__ px,
+                                                        unsigned long long* __restrict__ rays) {
+    __shared__ uint32_t s_stack[PHR_STACK * WF_THREADS];
+    uint32_t* stk = s_stack + threadIdx.x;
+    uint32_t* ctr = w.ctr;
+    if (!ANY && blockIdx.x == 0 && threadIdx.x == 0) {  // for this turn's shade and shadow walk
+        ctr[1u - parity] = 0u;
+        ctr[3] = 0u;
+        ctr[4] = 0u;
+    }
+    const uint32_t count = ANY ? ctr[3] : ctr[parity];
+    uint32_t* head = ANY ? &ctr[4] : &ctr[2];
+    const uint32_t* queue = ANY ? w.sq : (parity ? w.q1 : w.q0);
+    const uint32_t lane = threadIdx.x & 63u;
+    PhrHit best{PHR_INF, PHR_NONE, 0.f, 0.f, 0u};
+    V3 o = v3(0, 0, 0), inv = o;
+    RaySpace rs{};
+    uint32_t cur = 0, end = 0, sp = 0, slot = 0;
+    uint32_t steps = 0, tests = 0, nrays = 0;
+    bool live = false, drained = count == 0u;
+    uint32_t cbase = 0, cend = 0;  // the wave's chunk of the queue
+    const float4 rlo = sc.nodes[2 * sc.root], rhi = sc.nodes[2 * sc.root + 1];
+    // A finished walk's result: the hit for the shading, or the shadow ray's
+    // visibility taken into the field its turn left.
+    // A finished walk only stores: the hit for the shading, or the shadow
+    // ray's visibility, which the slot's next shading (or the frame's last
+    // kernel) takes in.
+    auto finish = [&]() {
+        if constexpr (!ANY) w.hit[slot] = make_float4(best.t, __uint_as_float(best.tri), best.u, best.v);
+        else w.vis[slot] = best.tri != PHR_NONE ? 0u : 1u;
+    };
+    for (;;) {
+        const uint64_t idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!drained && nidle >= 32u && cbase >= cend) {  // the wave's chunk is used up: take the next
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(head, WF_CHUNK);
+            b = __shfl(b, 0);
+            if (b >= count) {
+                drained = true;
+            } else {
+                cbase = b;
+                cend = min(b + WF_CHUNK, count);
+            }
+        }
+        if (nidle >= 32u && cbase < cend) {  // refill the idle lanes from the chunk
+            const uint32_t take = min(nidle, cend - cbase);
+            const uint32_t r = lanes_below(idle);
+            const uint32_t my = cbase + r;
+            cbase += take;
+            if (!live && r < take) {
+                slot = queue[my];
+                V3 d;
+                float tmax;
+                if constexpr (ANY) {
+                    const float4 a = w.sh[4 * (size_t)slot], b = w.sh[4 * (size_t)slot + 1];
+                    o = v3(a.x, a.y, a.z);
+                    tmax = a.w;
+                    d = v3(b.x, b.y, b.z);
+                } else {
+                    const float4 a = w.s0[slot], b = w.s1[slot];
+                    o = v3(a.x, a.y, a.z);
+                    d = v3(b.x, b.y, b.z);
+                    tmax = PHR_INF;
+                }
+                ++nrays;
+                best = PhrHit{tmax, PHR_NONE, 0.f, 0.f, 0u};
+                inv = safe_inv(d);
+                if (box_t(rlo, rhi, o, inv, best.t) >= PHR_INF) {
+                    finish();
+                } else {
+                    rs = ray_space(d);
+                    sp = 0;
+                    cur = __float_as_uint(rlo.w);
+                    end = __float_as_uint(rhi.w);
+                    if (!(cur & PHR_LEAF)) cur = sc.root;
+                    live = true;
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (!live) continue;
+        // One visit of walk<ANY>: a leaf's triangles or an inner node, then a
+        // pop if nothing was entered.
+        bool more = true, popped = true;
+        if (cur & PHR_LEAF) {
+            uint32_t id = cur & ~PHR_LEAF;
+            do {
+                ++tests;
+                const float4 t0 = sc.tris[3 * id], t1 = sc.tris[3 * id + 1], t2 = sc.tris[3 * id + 2];
+                if (end == PHR_NONE) end = __float_as_uint(t2.y);
+                float u, v;
+                const float t = tri_t(t0, t1, t2, o, rs, u, v);
+                if (t < best.t) {
+                    if (ANY) {
+                        const float* m = sc.mats + MAT_WORDS * (size_t)sc.faces[id].w;
+                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
+                        best = PhrHit{t, id, u, v, 0u};
+                        more = false;
+                        break;
+                    }
+                    best = PhrHit{t, id, u, v, 0u};
+                }
+            } while (++id < end);
+        } else {
+            popped = !visit2(sc.recs + 4 * (size_t)cur, sc.kids + 8 * (size_t)cur, o, inv, best.t, cur, end, stk, WF_THREADS, sp, steps);
+        }
+        if (more && popped) {
+            if (sp == 0) {
+                more = false;
+            } else {
+                --sp;
+                cur = stk[sp * WF_THREADS];
+                end = PHR_NONE;
+            }
+        }
+        if (!more) {
+            finish();
+            live = false;
+        }
+    }
+    unsigned long long rc = ANY ? 0ull : nrays, rsh = ANY ? nrays : 0ull, st_ = steps, tt = tests;
+    for (int k = 32; k > 0; k >>= 1) {
+        rc += __shfl_xor(rc, k);
+        rsh += __shfl_xor(rsh, k);
+        st_ += __shfl_xor(st_, k);
+        tt += __shfl_xor(tt, k);
+    }
+    if (lane == 0 && (rc | rsh)) {
+        atomicAdd(&rays[0], rc);
+        atomicAdd(&rays[1], rsh);
+        atomicAdd(&rays[2], st_);
+        atomicAdd(&rays[3], tt);
+    }
+}
+
+// One segment of every queued slot (rg_hit, rg_post), its next ray and its
+// shadow ray appended to the queues; an ended sample adds its radiance and
+// its pixel starts the next one. A block takes WF_SHADE_ITEMS x 256
+// consecutive entries and gathers its appends in LDS (wave ballots on LDS
+// counters), then reserves its places in each queue with one atomic.
+__global__ __launch_bounds__(WF_THREADS) void k_wf_shade(SceneDev sc, const PhrCamera* __restrict__ camp, WfDev w, uint32_t parity,
+                                                         uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px) {
+    __shared__ uint32_t s_next[WF_SHADE_ITEMS * WF_THREADS];
+    __shared__ uint32_t s_shadow[WF_SHADE_ITEMS * WF_THREADS];
+    __shared__ uint32_t s_cnt[2], s_base[2];
+    const PhrCamera* p = camp;
+    const size_t n = (size_t)p->w * p->h;
+    uint32_t* ctr = w.ctr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[2] = 0u;  // the next turn's extend head
+    const uint32_t count = ctr[parity];
+    const uint32_t first_e = blockIdx.x * WF_SHADE_ITEMS * WF_THREADS;
+    if (first_e >= count) return;
+    const uint32_t* queue = parity ? w.q1 : w.q0;
+    uint32_t* next_q = parity ? w.q0 : w.q1;
+    const RgConst k = rg_const(p);
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t r = 0; r < WF_SHADE_ITEMS; ++r) {
+        const uint32_t e = first_e + r * WF_THREADS + threadIdx.x;
+        bool next = false, shadow = false;
+        uint32_t slot = 0;
+        if (e < count) {
+            slot = queue[e];
+            bool inside;
+            const uint32_t i = wf_pixel(slot, tiles_x, p->w, p->h, inside);
+            RgSample st;
+            uint32_t s;
+            wf_load(w, slot, st, s);
+            const float4 hr = w.hit[slot];
+            const PhrHit h{hr.x, __float_as_uint(hr.y), hr.z, hr.w, 0u};
+            if (const uint32_t pk = w.pend[slot]) {  // the last turn's shadow ray, taken in first
+                const float4 c = w.sh[4 * (size_t)slot + (w.vis[slot] ? 3 : 2)];
+                const V3 v = v3(c.x, c.y, c.z);
+                if (pk - 1u == WF_END) add3(px, n + 3 * (size_t)i, v);
+                else if (pk - 1u == WF_CC) st.cc = v;
+                else st.acc = v;
+            }
+            const bool first = st.phase == 1u && st.pass == 0u;
+            RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
+            RgTurn q;
+            rg_hit(st, g, q, sc, p, k, h);
+            if (first) {  // the sample's guides, in sample order
+                add3(px, 7 * n + 3 * (size_t)i, g.alb);
+                add3(px, 10 * n + 3 * (size_t)i, g.nrm);
+                stf(px, 13 * n + i, ldf(px, 13 * n + i) + g.dep);
+                stf(px, 33 * n + i, ldf(px, 33 * n + i) + g.missed);
+                stf(px, 23 * n + i, ldf(px, 23 * n + i) + g.cov);
+                px[25 * n + i] = g.outline;
+            }
+            bool ended;
+            uint32_t pk_new = 0u;
+            if (!q.qon) {
+                ended = rg_post(st, q, p, 1.0f);
+                if (ended) add3(px, n + 3 * (size_t)i, st.col);
+            } else {
+                RgPost blocked{st.col, st.cc, st.acc, st.phase};
+                ended = rg_post_of(blocked, st.alb, st.p_diffuse, q, p, 0.0f);
+                rg_post(st, q, p, 1.0f);
+                const uint32_t kind = ended ? WF_END : (q.to_bounces ? WF_CC : WF_ACC);
+                pk_new = 1u + kind;
+                float4 c0, c1;
+                if (ended) {
+                    c0 = make_float4(blocked.col.x, blocked.col.y, blocked.col.z, __uint_as_float(kind));
+                    c1 = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
+                } else if (kind == WF_CC) {
+                    c0 = make_float4(blocked.cc.x, blocked.cc.y, blocked.cc.z, __uint_as_float(kind));
+                    c1 = make_float4(st.cc.x, st.cc.y, st.cc.z, 0.f);
+                } else {
+                    c0 = make_float4(blocked.acc.x, blocked.acc.y, blocked.acc.z, __uint_as_float(kind));
+                    c1 = make_float4(st.acc.x, st.acc.y, st.acc.z, 0.f);
+                }
+                float4* d = w.sh + 4 * (size_t)slot;
+                d[0] = make_float4(q.qpos.x, q.qpos.y, q.qpos.z, q.qdist);
+                d[1] = make_float4(q.qdir.x, q.qdir.y, q.qdir.z, 0.f);
+                d[2] = c0;
+                d[3] = c1;
+                shadow = true;
+            }
+            w.pend[slot] = (uint8_t)pk_new;
+            if (ended) {
+                if (s + 1u < spp) {
+                    ++s;
+                    rg_start(st, g, p, i, frame0 + s);
+                    next = true;
+                }
+            } else {
+                next = true;
+            }
+            wf_store(w, slot, st, s);
+        }
+        const uint32_t at = wave_append(next, &s_cnt[0]);
+        if (next) s_next[at] = slot;
+        const uint32_t sa = wave_append(shadow, &s_cnt[1]);
+        if (shadow) s_shadow[sa] = slot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_base[0] = s_cnt[0] ? atomicAdd(&ctr[1u - parity], s_cnt[0]) : 0u;
+        s_base[1] = s_cnt[1] ? atomicAdd(&ctr[3], s_cnt[1]) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t k2 = threadIdx.x; k2 < s_cnt[0]; k2 += WF_THREADS) next_q[s_base[0] + k2] = s_next[k2];
+    for (uint32_t k2 = threadIdx.x; k2 < s_cnt[1]; k2 += WF_THREADS) w.sq[s_base[1] + k2] = s_shadow[k2];
+}
+
+// The frame's last words a pixel: the last sample's radiance if a shadow ray
+// still held it, the grid plane, film weight and count (added once a sample,
+// as path_block adds them), the display.
+__global__ __launch_bounds__(256) void k_wf_final(const PhrCamera* __restrict__ camp, WfDev w, uint32_t tiles_x, uint32_t frame0,
+                                                  uint32_t spp, uint32_t* __restrict__ px) {
+    const PhrCamera* p = camp;
+    const size_t n = (size_t)p->w * p->h;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    // The last sample's radiance when its turn ended with a shadow ray.
+    const uint32_t x = i % p->w, y = i / p->w;
+    const uint32_t slot = ((y >> 4) * tiles_x + (x >> 4)) * 256u + (y & 15u) * 16u + (x & 15u);
+    if (w.pend[slot] == 1u + WF_END) {
+        const float4 c = w.sh[4 * (size_t)slot + (w.vis[slot] ? 3 : 2)];
+        add3(px, n + 3 * (size_t)i, v3(c.x, c.y, c.z));
+    }
+    const bool keep = frame0 > 0u;
+    const V3 l = v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2));
+    const float cov = ldf(px, 23 * n + i), gone = ldf(px, 33 * n + i);
+    V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
+    float film = keep ? ldf(px, 24 * n + i) : 0.f;
+    uint32_t taken = keep ? px[31 * n + i] : 0u;
+    for (uint32_t kk = 0; kk < spp; ++kk) {
+        gs = gs + v3(0.f, 0.f, 0.f);
+        film = film + 1.0f;
+    }
+    taken = taken + spp;
+    if (frame0 == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
+    stf(px, 20 * n + 3 * i, gs.x); stf(px, 20 * n + 3 * i + 1, gs.y); stf(px, 20 * n + 3 * i + 2, gs.z);
+    stf(px, 24 * n + i, film);
+    px[31 * n + i] = taken;
+    put_shown(px, n, i, *p, l, gs, cov, gone, film, taken);
 }
 
 // One image: a block a tile, from tile0 on (a slice of the tiles).
@@ -1575,7 +2167,7 @@ static int enqueue_walk_records(phr_gpu* g, phr_bvh* bvh, phr_array** recs, phr_
     phr_array* nodes = phr_bvh_internal_nodes(bvh);
     const uint64_t count = nodes->len;
     if (!*recs) {
-        int rc = phr_array_alloc(g, 64, std::max<uint64_t>(count, 1), recs);
+        int rc = phr_array_alloc(g, 16 * REC_ROWS, std::max<uint64_t>(count, 1), recs);
         if (rc != PHR_OK) return rc;
     }
     if (count)
@@ -1611,7 +2203,7 @@ struct phr_tracer {
     std::vector<uint32_t> order;          // slot -> input face
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
-    uint32_t path_regen = 0;  // 1: the path-regenerating build where the camera allows it
+    uint32_t path_regen = 0;  // 1: the path-regenerating build, 2: the wavefront build, where the camera allows it
     uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
     bool short_stack = false;         // the forest's 3-wave build with the 24-entry mesh-tree stack
     // The binding split phiray's WGSL needs (items a binding for nodes, faces,
@@ -1678,6 +2270,14 @@ struct phr_tracer {
     uint32_t* pin_words = nullptr;    // the instance table as uploaded
     bool fposed = false;              // the recorded frame applies a pose first
     const void* fverts = nullptr;     // its vertex source (null: matrices only)
+    // The wavefront build's buffers, sized for wf_slots path slots.
+    phr_array* wf_state = nullptr;    // 9 float4 a slot (state, hit)
+    phr_array* wf_queue = nullptr;    // 2 ray queues and the shadow queue, of slots
+    phr_array* wf_shadow = nullptr;   // 4 float4 a slot: its shadow ray
+    uint32_t* wf_ctr = nullptr;       // WF_CTRS counters
+    uint32_t* wf_pin = nullptr;       // pinned: the next turn's ray count, read every few turns
+    uint32_t wf_slots = 0;
+    uint32_t wf_turns = 0;            // turns the last wavefront frame took
 };
 
 namespace {
@@ -1730,8 +2330,11 @@ void tracer_release(phr_tracer* t) {
     if (t->bvh) phr_bvh_free(t->bvh);
     if (t->tlas) phr_bvh_free(t->tlas);
     for (phr_array* a : {t->in_faces, t->verts, t->mats, t->prims, t->faces, t->tris, t->recs, t->lights, t->px, t->nodes, t->wboxes,
-                         t->insts, t->placed_dev, t->worlds_dev, t->inst_mesh, t->lcand, t->lpow})
+                         t->insts, t->placed_dev, t->worlds_dev, t->inst_mesh, t->lcand, t->lpow, t->wf_state, t->wf_queue,
+                         t->wf_shadow})
         phr_array_release(a);
+    if (t->wf_ctr) (void)hipFree(t->wf_ctr);
+    if (t->wf_pin) (void)hipHostFree(t->wf_pin);
     if (t->bases_dev) (void)hipFree(t->bases_dev);
     if (t->frame_dev) (void)hipFree(t->frame_dev);
     if (t->dn_scratch) (void)hipFree(t->dn_scratch);
@@ -1802,6 +2405,7 @@ SceneDev scene_dev(phr_tracer* t) {
     SceneDev s;
     s.nodes = (const float4*)(t->forest ? t->nodes : phr_bvh_internal_nodes(t->bvh))->ptr;
     s.recs = t->forest ? nullptr : (const float4*)t->recs->ptr;
+    s.kids = t->forest ? nullptr : s.recs + 4 * t->recs->len;  // recs holds len nodes' records
     s.insts = t->forest ? (const uint32_t*)t->insts->ptr : nullptr;
     s.ninsts = t->ninsts;
     s.tris = (const float4*)t->tris->ptr;
@@ -2413,6 +3017,78 @@ int phr_tracer_set_adaptive(phr_tracer* t, const float adapt[4]) {
 // `go` asked whether to carry on. When it says no, the call is abandoned
 // (*finished = 0) and the next one starts the image over, since a frame with
 // some pixels a sample ahead is no frame.
+// The wavefront build (k_wf_*) over every tile: samples [frame0, frame0 +
+// spp) of the image. The host reads the next turn's ray count every 16 turns
+// and stops once no slot has a sample left; the turns past the last hold no
+// rays and end at once.
+int wf_frame(phr_tracer* t, const SceneDev& sc, const PhrCamera* cp, const PhrCamera& cam, uint32_t tiles_x, uint32_t tiles,
+             uint32_t frame0, uint32_t spp, uint32_t* pxp) {
+    hipStream_t s = t->gpu->stream;
+    const uint32_t slots = tiles * WF_THREADS;
+    if (t->wf_slots != slots) {
+        for (phr_array** a : {&t->wf_state, &t->wf_queue, &t->wf_shadow}) {
+            if (*a) phr_array_release(*a);
+            *a = nullptr;
+        }
+        t->wf_slots = 0;
+        int rc;
+        if ((rc = phr_array_alloc(t->gpu, 16, 9ull * slots, &t->wf_state)) != PHR_OK) return rc;
+        if ((rc = phr_array_alloc(t->gpu, 4, 3ull * slots, &t->wf_queue)) != PHR_OK) return rc;
+        if ((rc = phr_array_alloc(t->gpu, 16, 4ull * slots + (2ull * slots + 15) / 16, &t->wf_shadow)) != PHR_OK) return rc;
+        t->wf_slots = slots;
+    }
+    if (!t->wf_ctr) PHR_HIP(hipMalloc((void**)&t->wf_ctr, 4 * WF_CTRS));
+    if (!t->wf_pin) PHR_HIP(hipHostMalloc((void**)&t->wf_pin, 4, hipHostMallocDefault));
+    float4* st = (float4*)t->wf_state->ptr;
+    WfDev w;
+    w.s0 = st;
+    w.s1 = st + (size_t)slots;
+    w.s2 = st + 2 * (size_t)slots;
+    w.s3 = st + 3 * (size_t)slots;
+    w.s4 = st + 4 * (size_t)slots;
+    w.s5 = st + 5 * (size_t)slots;
+    w.s6 = st + 6 * (size_t)slots;
+    w.s7 = st + 7 * (size_t)slots;
+    w.hit = st + 8 * (size_t)slots;
+    w.q0 = (uint32_t*)t->wf_queue->ptr;
+    w.q1 = w.q0 + slots;
+    w.sq = w.q1 + slots;
+    w.sh = (float4*)t->wf_shadow->ptr;
+    w.vis = (uint8_t*)(w.sh + 4 * (size_t)slots);
+    w.pend = w.vis + slots;
+    w.ctr = t->wf_ctr;
+    hipLaunchKernelGGL(k_wf_zero, dim3(1), dim3(64), 0, s, w.ctr);
+    hipLaunchKernelGGL(k_wf_init, dim3(tiles), dim3(WF_THREADS), 0, s, cp, w, tiles_x, slots, frame0, pxp);
+    PHR_HIP(hipGetLastError());
+    const uint32_t cus = (uint32_t)std::max(t->gpu->cu_count, 1);
+    const uint32_t walk_blocks = cus * 5u;  // the LDS stacks fit five 256-thread blocks a CU
+    const uint32_t shade_blocks = (slots + WF_SHADE_ITEMS * WF_THREADS - 1) / (WF_SHADE_ITEMS * WF_THREADS);
+    // A sample takes at most one turn a camera segment and one a bounce.
+    const bool depth_set = cam.depth[0] != 0u;
+    const uint64_t total = depth_set ? cam.depth[0] - 1u : 8u;
+    const uint64_t passes = (depth_set ? cam.depth[1] - 1u : 8u) + (depth_set ? std::min<uint64_t>(cam.depth[2] - 1u, total) : 8u) + 1u;
+    const uint64_t limit = (uint64_t)spp * (passes + (cam.bounces ? cam.bounces : 1u) + 1u) + 32u;
+    uint64_t turn = 0;
+    for (;; ++turn) {
+        const uint32_t par = (uint32_t)(turn & 1u);
+        hipLaunchKernelGGL(k_wf_walk<false>, dim3(walk_blocks), dim3(WF_THREADS), 0, s, sc, cp, w, par, tiles_x, pxp, t->rays);
+        hipLaunchKernelGGL(k_wf_shade, dim3(shade_blocks), dim3(WF_THREADS), 0, s, sc, cp, w, par, tiles_x, frame0, spp, pxp);
+        hipLaunchKernelGGL(k_wf_walk<true>, dim3(walk_blocks), dim3(WF_THREADS), 0, s, sc, cp, w, par, tiles_x, pxp, t->rays);
+        PHR_HIP(hipGetLastError());
+        if ((turn & 15u) == 15u || turn + 1 >= limit) {
+            PHR_HIP(hipMemcpyAsync(t->wf_pin, w.ctr + (1u - par), 4, hipMemcpyDeviceToHost, s));
+            PHR_HIP(hipStreamSynchronize(s));
+            if (*t->wf_pin == 0u) break;
+            PHR_CHECK(turn + 1 < limit, PHR_ERR_SIZE, "phr_tracer_sample: the wavefront build ran past its turn bound");
+        }
+    }
+    t->wf_turns = (uint32_t)(turn + 1);
+    const uint64_t n = (uint64_t)cam.w * cam.h;
+    hipLaunchKernelGGL(k_wf_final, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, cp, w, tiles_x, frame0, spp, pxp);
+    PHR_HIP(hipGetLastError());
+    return PHR_OK;
+}
+
 int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t h, uint32_t spp, uint32_t slices,
                          phr_go_fn go, void* user, uint32_t* frame, int* done, int* finished) {
     PHR_CHECK(t && cam_v, PHR_ERR_ARG, "phr_tracer_sample: null argument");
@@ -2476,7 +3152,9 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
         }
         return PHR_OK;
     };
-    if (cam.adapt[3] == 0.0f) {
+    if (cam.adapt[3] == 0.0f && regen && t->path_regen == 2u && !t->forest && slices == 1u) {
+        if (int rc = wf_frame(t, sc, cp, cam, tiles_x, tiles, frame0, spp, pxp); rc != PHR_OK) return rc;
+    } else if (cam.adapt[3] == 0.0f) {
         if (int rc = path(frame0, spp, 0u); rc != PHR_OK) return rc;
     } else {
         // Adaptive Sampling (sample.rs `BTail::Adapt`): the tail runs after
@@ -2532,11 +3210,13 @@ int phr_tracer_set_path_waves(phr_tracer* t, uint32_t waves) {
 }
 uint32_t phr_tracer_path_waves(const phr_tracer* t) { return t ? t->path_waves : 0; }
 
-// The path-regenerating build on (1) or off (0); it runs only for one tree
-// and a camera without Adaptive Sampling or the studio rig, the other build
-// otherwise (no counterpart in phiray).
+// The path-regenerating build (1), the wavefront build (2) or neither (0);
+// they run only for one tree and a camera without Adaptive Sampling or the
+// studio rig, the nested build otherwise. A sliced sample (sample_in with
+// several slices) and a recorded frame take the regenerating build for 2
+// (no counterpart in phiray).
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on) {
-    PHR_CHECK(t && on <= 1u, PHR_ERR_ARG, "phr_tracer_set_path_regen: null, or not 0 or 1");
+    PHR_CHECK(t && on <= 2u, PHR_ERR_ARG, "phr_tracer_set_path_regen: null, or not 0, 1 or 2");
     t->path_regen = on;
     return PHR_OK;
 }
@@ -2571,11 +3251,14 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     double best = 1e30;
     PhrCamera probe;
     std::memcpy(&probe, cam, sizeof probe);
-    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 2u : 1u;
+    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 3u : 1u;
     const bool fits_short = t->forest && t->mesh_depth <= FOREST_SHORT_STACK;
     bool best_short = false;
     for (uint32_t regen = 0; regen < regens; ++regen)
-        for (uint32_t waves = PATH_WAVES_MIN; waves <= (t->forest ? (fits_short ? 4u : 3u) : PATH_WAVES_MAX); ++waves)
+        // The wavefront build's kernels have one register fit: timed once,
+        // keeping the waves the others chose for its recorded-frame stand-in.
+        for (uint32_t waves = regen == 2u ? best_w : PATH_WAVES_MIN;
+             waves <= (regen == 2u ? best_w : (t->forest ? (fits_short ? 4u : 3u) : PATH_WAVES_MAX)); ++waves)
             for (uint32_t stk = 0; stk < ((fits_short && waves == 3) ? 2u : 1u); ++stk) {
                 t->path_waves = waves;
                 t->path_regen = regen;
@@ -2969,10 +3652,6 @@ int phr_tracer_replay_pose_frame(phr_tracer* t, const float* worlds, uint32_t n,
     return PHR_OK;
 }
 
-// The learned denoiser over what has been traced (no counterpart in phiray; it
-// stands beside Tracer::denoise). With `out`, the denoised radiance goes there
-// (w * h * 3 floats) and the image is left as it is; without, it replaces the
-// linear sums, as Tracer::denoise with keep = false does.
 // Tracer::show (src/trace/read.rs:182-229, src/trace/wgsl/show.wgsl): a
 // host's scene-linear image put back in place of the traced mean (sum = value
 // times the pixel's own count) and run through the display step. `linear`
@@ -3047,6 +3726,10 @@ int phr_tracer_size(phr_tracer* t, uint32_t* w, uint32_t* h) {
     return PHR_OK;
 }
 
+// The learned denoiser over what has been traced (no counterpart in phiray; it
+// stands beside Tracer::denoise). With `out`, the denoised radiance goes there
+// (w * h * 3 floats) and the image is left as it is; without, it replaces the
+// linear sums, as Tracer::denoise with keep = false does.
 int phr_tracer_denoise_learned(phr_tracer* t, phr_kpcn* k, phr_array* out) {
     PHR_CHECK(t && k, PHR_ERR_ARG, "phr_tracer_denoise_learned: null");
     if (!t->px) return PHR_OK;
@@ -3571,6 +4254,7 @@ int phr_batch_new(phr_gpu* g, const void* tris_v, const uint32_t* seg_lo, uint32
         std::memset(&d, 0, sizeof d);
         d.nodes = (const float4*)phr_bvh_internal_nodes(b->bvh)->ptr;
         d.recs = (const float4*)b->recs->ptr;
+        d.kids = d.recs + 4 * b->recs->len;
         d.tris = (const float4*)b->tris->ptr;
         d.faces = (const uint4*)b->faces->ptr;
         d.verts = (const float4*)b->verts->ptr;

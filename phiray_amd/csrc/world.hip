@@ -52,6 +52,7 @@
 namespace {
 
 constexpr uint32_t MAX_BODIES = 2048;    // the bodies' state in LDS: 128 KB
+constexpr uint32_t LPUSH_BODIES = 1280;  // bodies whose state and pushed-out offsets fit LDS together (120 KB)
 constexpr uint32_t PAIR_SLOTS = 32;      // pairs a body may own
 constexpr uint32_t MAX_COLOURS = 1024;
 constexpr uint32_t SOLVE_THREADS = 512;
@@ -251,6 +252,93 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ nodes,
         }
     }
     count[i] = found;
+}
+
+// The same walk with the tree, the swept boxes and the bodies' flags and groups
+// in LDS: a block of one wave copies them in, then each lane walks for its
+// body with its stack and its pair run in LDS too. Every visit is an LDS
+// read instead of an L2 round trip. For worlds of at most LDS_PAIR_BODIES.
+constexpr uint32_t LDS_PAIR_BODIES = 1280;
+__global__ __launch_bounds__(64) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
+                                                  const float4* __restrict__ boxes, const Body* __restrict__ bodies,
+                                                  const Konst* __restrict__ konst, uint32_t n, uint32_t* __restrict__ count,
+                                                  uint32_t* __restrict__ other, uint32_t* __restrict__ overflow) {
+    __shared__ float4 s_nodes[2 * (2 * LDS_PAIR_BODIES - 1)];
+    __shared__ float4 s_boxes[2 * LDS_PAIR_BODIES];
+    __shared__ uint32_t s_order[LDS_PAIR_BODIES], s_flags[LDS_PAIR_BODIES], s_groups[LDS_PAIR_BODIES];
+    __shared__ uint32_t s_stack[PHR_STACK * 64];
+    __shared__ uint32_t s_mine[PAIR_SLOTS * 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nn = 2 * n - 1;
+    // Eight loads in flight a lane: one wave copies ~110 KB.
+    auto stage = [&](float4* dst, const float4* src, uint32_t len) {
+        for (uint32_t k = tid; k < len; k += 64 * 8) {
+            float4 v[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (k + 64 * j < len) v[j] = src[k + 64 * j];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (k + 64 * j < len) dst[k + 64 * j] = v[j];
+        }
+    };
+    stage(s_nodes, nodes, 2 * nn);
+    stage(s_boxes, boxes, 2 * n);
+    for (uint32_t k = tid; k < n; k += 64) {
+        s_order[k] = order[k];
+        s_flags[k] = bodies[k].flags;
+        s_groups[k] = konst[k].groups;
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 64 + tid;
+    if (i >= n) return;
+    const uint32_t fi = s_flags[i];
+    uint32_t found = 0;
+    uint32_t* mine = s_mine + tid;  // slot k at mine[64 k]
+    if (moves_f(fi)) {
+        const float4 blo = s_boxes[2 * i], bhi = s_boxes[2 * i + 1];
+        const uint32_t gi = s_groups[i];
+        const bool act_i = kind_of(fi) == K_ACTIVE;
+        auto meets = [&](float4 lo, float4 hi) {
+            return lo.x <= bhi.x && hi.x >= blo.x && lo.y <= bhi.y && hi.y >= blo.y && lo.z <= bhi.z && hi.z >= blo.z;
+        };
+        uint32_t* stack = s_stack + tid;  // entry k at stack[64 k]
+        int sp = 0;
+        stack[0] = 0;
+        sp = 1;
+        while (sp > 0) {
+            const uint32_t k = stack[64 * --sp];
+            const float4 lo = s_nodes[2 * k], hi = s_nodes[2 * k + 1];
+            if (!meets(lo, hi)) continue;
+            const uint32_t c0 = __float_as_uint(lo.w), c1 = __float_as_uint(hi.w);
+            if (c0 & PHR_LEAF) {
+                for (uint32_t sl = c0 & ~PHR_LEAF; sl < c1; ++sl) {
+                    const uint32_t j = s_order[sl];
+                    if (j == i) continue;
+                    const uint32_t fj = s_flags[j];
+                    if (moves_f(fj) && j < i) continue;  // j owns it
+                    if (!act_i && kind_of(fj) != K_ACTIVE) continue;
+                    if ((gi & s_groups[j]) == 0u) continue;
+                    if (!meets(s_boxes[2 * j], s_boxes[2 * j + 1])) continue;
+                    if (found == PAIR_SLOTS) {
+                        atomicOr(overflow, 1u);
+                        continue;
+                    }
+                    uint32_t at = found++;
+                    while (at > 0 && mine[64 * (at - 1)] > j) {
+                        mine[64 * at] = mine[64 * (at - 1)];
+                        --at;
+                    }
+                    mine[64 * at] = j;
+                }
+            } else if (sp + 2 <= (int)PHR_STACK) {  // an LBVH is at most 31 deep: never full
+                stack[64 * sp++] = c0;
+                stack[64 * sp++] = c1;
+            }
+        }
+    }
+    count[i] = found;
+    for (uint32_t k = 0; k < found; ++k) other[(size_t)i * PAIR_SLOTS + k] = mine[64 * k];
 }
 
 // ---- 4. contacts ---------------------------------------------------------------------
@@ -822,6 +910,289 @@ __host__ __device__ void velocity_pair(Lds& L, const Konst* kc, const uint32_t* 
     if (any) apply(k);
 }
 
+// ---- the same two projections on four lanes a pair -------------------------------------
+// Lane e of an aligned group of four holds the pair's point e (lanes past the
+// pair's points hold none and add zeros). A sum over the points is taken on
+// every lane from the four lanes' terms in point order, through DPP quad
+// broadcasts, starting from zero as `summed` does; a point without a term
+// adds +0, which leaves a sum that never is -0 unchanged. So each lane holds
+// the bits the one-lane functions compute, and lane 0 alone writes the
+// bodies. A colour's projection then waits on about a quarter of the
+// one-lane version's dependent arithmetic.
+template <int K>
+__device__ __forceinline__ float qlane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ int qlane_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float qsum(float v) {
+    float r = 0.f;
+    r = r + qlane<0>(v);
+    r = r + qlane<1>(v);
+    r = r + qlane<2>(v);
+    r = r + qlane<3>(v);
+    return r;
+}
+__device__ __forceinline__ float qneg_sum(float v) {  // 0 - v0 - v1 - v2 - v3, in that order
+    float r = 0.f;
+    r = r - qlane<0>(v);
+    r = r - qlane<1>(v);
+    r = r - qlane<2>(v);
+    r = r - qlane<3>(v);
+    return r;
+}
+__device__ __forceinline__ V qsum(V v) { return mk(qsum(v.x), qsum(v.y), qsum(v.z)); }
+__device__ __forceinline__ V qneg_sum(V v) { return mk(qneg_sum(v.x), qneg_sum(v.y), qneg_sum(v.z)); }
+__device__ __forceinline__ bool qany(bool on) {
+    const int x = on ? 1 : 0;
+    return (qlane_i<0>(x) | qlane_i<1>(x) | qlane_i<2>(x) | qlane_i<3>(x)) != 0;
+}
+// What a projection reads of its pair besides the bodies' state in LDS,
+// loaded for the next colour while the current one is solved: the pair
+// record, this lane's point, both bodies' poses at the substep's start and
+// their inverse inertia, friction and restitution. None of it changes
+// within a substep's colours (a point's multipliers change only under its
+// own pair, on these same lanes).
+struct PairPre {
+    PairRec pr;
+    PointRec pt;
+    float4 pa0, pa1, pb0, pb1;
+    V ia, ib;
+    float fa, fb, ea, eb;
+};
+__device__ __forceinline__ void pair_pre(PairPre& P, const PairRec* __restrict__ spairs, const PointRec* __restrict__ spoints,
+                                         const float4* __restrict__ prev, const Konst* __restrict__ kc, uint32_t e, uint32_t lane) {
+    P.pr = spairs[e];
+    const uint32_t a = P.pr.a, b = P.pr.b;
+    P.pt = lane < P.pr.m ? spoints[4 * (size_t)e + lane] : PointRec{};
+    P.pa0 = prev[2 * a];
+    P.pa1 = prev[2 * a + 1];
+    P.pb0 = prev[2 * b];
+    P.pb1 = prev[2 * b + 1];
+    P.ia = ld3(kc[a].inv_inertia);
+    P.ib = ld3(kc[b].inv_inertia);
+    P.fa = kc[a].friction;
+    P.fb = kc[b].friction;
+    P.ea = kc[a].restitution;
+    P.eb = kc[b].restitution;
+}
+// w_along and turn_of with the body's inverse inertia in hand.
+__device__ __forceinline__ float w_along_k(const Lds& L, const uint32_t* fl, uint32_t i, V inv_i, V r, V l) {
+    if (!dynamic_f(fl[i])) return 0.f;
+    const V rl = unrot(L.q(i), cross(r, l));
+    return L.im(i) + dot(hadamard(rl, inv_i), rl);
+}
+__device__ __forceinline__ V turn_of_k(const Lds& L, uint32_t i, V inv_i, V r, V p) {
+    const Q q = L.q(i);
+    return rot(q, hadamard(unrot(q, cross(r, p)), inv_i));
+}
+// `summed` with this lane's point: its impulse p (weighted already), on or off.
+__device__ __forceinline__ Move quad_summed(const Lds& L, const PairPre& K, bool da, bool db, uint32_t a, uint32_t b, V ra, V rb,
+                                            bool on, V p) {
+    const V z = mk(0.f, 0.f, 0.f);
+    const V ca = (on && da) ? p * L.im(a) : z, ta = (on && da) ? turn_of_k(L, a, K.ia, ra, p) : z;
+    const V cb = (on && db) ? p * L.im(b) : z, tb = (on && db) ? turn_of_k(L, b, K.ib, rb, p) : z;
+    return Move{qsum(ca), qsum(ta), qneg_sum(cb), qneg_sum(tb)};
+}
+// `safe_scale`: the smallest of 1 and each asking point's need / got.
+__device__ __forceinline__ float quad_scale(const Move& mv, V ra, V rb, bool on, V dir, float need) {
+    float s = 1.0f;
+    if (on) {
+        const float got = dot((mv.da + cross(mv.ta, ra)) - (mv.db + cross(mv.tb, rb)), dir);
+        if (got * need > 0.f) s = need / got;
+    }
+    return fminf(fminf(fminf(fminf(1.0f, qlane<0>(s)), qlane<1>(s)), qlane<2>(s)), qlane<3>(s));
+}
+
+__device__ __forceinline__ void project_quad(Lds& L, const PairPre& K, const uint32_t* fl, float4* push, PointRec* pts, float cap,
+                                             uint32_t e) {
+    const PairRec& pr = K.pr;
+    const uint32_t a = pr.a, b = pr.b;
+    const bool mine = e < pr.m;
+    const bool da = dynamic_f(fl[a]), db = dynamic_f(fl[b]);
+    const V n = mk(pr.n.x, pr.n.y, pr.n.z);
+    const Q qa = L.q(a), qb = L.q(b);
+    const V xa = L.x(a), xb = L.x(b);
+    const float4 pa0 = K.pa0, pa1 = K.pa1, pb0 = K.pb0, pb1 = K.pb1;
+    const V xa_prev = mk(pa0.x, pa0.y, pa0.z), xb_prev = mk(pb0.x, pb0.y, pb0.z);
+    const Q qa_prev{pa1.x, pa1.y, pa1.z, pa1.w}, qb_prev{pb1.x, pb1.y, pb1.z, pb1.w};
+    V la = mk(0.f, 0.f, 0.f), lb = la;
+    float lam_n = 0.f, lam_t = 0.f;
+    if (mine) {
+        const float4 aa = K.pt.aa, ab = K.pt.ab;
+        la = mk(aa.x, aa.y, aa.z);
+        lb = mk(ab.x, ab.y, ab.z);
+        lam_n = aa.w;
+        lam_t = ab.w;
+    }
+    V ra = rot(qa, la), rb = rot(qb, lb);
+    // Normal.
+    bool on = false;
+    float dl = 0.f, rec = 0.f, need = 0.f;
+    V p = mk(0.f, 0.f, 0.f), dir = p;
+    if (mine) {
+        const float depth = dot((xa + ra) - (xb + rb), n);
+        const float wsum = w_along_k(L, fl, a, K.ia, ra, n) + w_along_k(L, fl, b, K.ib, rb, n);
+        if (depth > 0.f && wsum != 0.f) {
+            const V was_a = xa_prev + rot(qa_prev, la), was_b = xb_prev + rot(qb_prev, lb);
+            const float closed = fmaxf(dot(((xa + ra) - was_a) - ((xb + rb) - was_b), n), 0.f);
+            const float hit = fminf(depth, closed);
+            const float fix = hit + fminf(depth - hit, cap);
+            dl = -fix / wsum;
+            rec = fix > 0.f ? (fix - hit) / fix : 0.f;
+            p = n * dl;
+            dir = -n;
+            need = fix;
+            on = true;
+        }
+    }
+    V xa2 = xa, xb2 = xb;
+    Q qa2 = qa, qb2 = qb;
+    if (qany(on)) {
+        const Move mv = quad_summed(L, K, da, db, a, b, ra, rb, on, p);
+        const float s = quad_scale(mv, ra, rb, on, dir, need);
+        const Move rv = quad_summed(L, K, da, db, a, b, ra, rb, on, p * rec);
+        if (da) {
+            xa2 = xa + mv.da * s;
+            qa2 = qstep(qa, mv.ta * s, 1.0f);
+        }
+        if (db) {
+            xb2 = xb + mv.db * s;
+            qb2 = qstep(qb, mv.tb * s, 1.0f);
+        }
+        if (e == 0) {
+            if (da) {
+                L.set_x(a, xa2);
+                L.set_q(a, qa2);
+                const float4 p0 = push[2 * a], p1 = push[2 * a + 1];
+                push[2 * a] = make_float4(p0.x + rv.da.x * s, p0.y + rv.da.y * s, p0.z + rv.da.z * s, 0.f);
+                push[2 * a + 1] = make_float4(p1.x + rv.ta.x * s, p1.y + rv.ta.y * s, p1.z + rv.ta.z * s, 0.f);
+            }
+            if (db) {
+                L.set_x(b, xb2);
+                L.set_q(b, qb2);
+                const float4 p0 = push[2 * b], p1 = push[2 * b + 1];
+                push[2 * b] = make_float4(p0.x + rv.db.x * s, p0.y + rv.db.y * s, p0.z + rv.db.z * s, 0.f);
+                push[2 * b + 1] = make_float4(p1.x + rv.tb.x * s, p1.y + rv.tb.y * s, p1.z + rv.tb.z * s, 0.f);
+            }
+        }
+        lam_n += dl * s;
+    }
+    // Static friction.
+    const float mu = K.fa * K.fb;
+    on = false;
+    dl = 0.f;
+    V tdir = mk(0.f, 0.f, 0.f);
+    ra = rot(qa2, la);
+    rb = rot(qb2, lb);
+    if (mine && lam_n != 0.f) {
+        const V drift = ((xa2 + ra) - (xa_prev + rot(qa_prev, la))) - ((xb2 + rb) - (xb_prev + rot(qb_prev, lb)));
+        const V tang = drift - n * dot(drift, n);
+        const float lt = length(tang);
+        if (lt >= 1e-9f) {
+            const V t = tang * (1.0f / lt);
+            const float wsum = w_along_k(L, fl, a, K.ia, ra, t) + w_along_k(L, fl, b, K.ib, rb, t);
+            if (wsum != 0.f) {
+                tdir = t;
+                dl = -lt / wsum;
+                p = t * dl;
+                dir = -t;
+                need = lt;
+                on = true;
+            }
+        }
+    }
+    if (!qany(on)) {
+        if (mine) pts[e].aa.w = lam_n;
+        return;
+    }
+    const float share = quad_scale(quad_summed(L, K, da, db, a, b, ra, rb, on, p), ra, rb, on, dir, need);
+    if (on) {
+        const float want = dl * share;
+        const float room = fmaxf(mu * fabsf(lam_n) - fabsf(lam_t), 0.f);
+        dl = copysignf(fminf(fabsf(want), room), want);
+        p = tdir * dl;
+    }
+    const Move mv = quad_summed(L, K, da, db, a, b, ra, rb, on, p);
+    const float s = quad_scale(mv, ra, rb, on, dir, need);
+    if (e == 0) {
+        if (da) {
+            L.set_x(a, xa2 + mv.da * s);
+            L.set_q(a, qstep(qa2, mv.ta * s, 1.0f));
+        }
+        if (db) {
+            L.set_x(b, xb2 + mv.db * s);
+            L.set_q(b, qstep(qb2, mv.tb * s, 1.0f));
+        }
+    }
+    lam_t += dl * s;
+    if (mine) {
+        pts[e].aa.w = lam_n;
+        pts[e].ab.w = lam_t;
+    }
+}
+
+__device__ __forceinline__ void velocity_quad(Lds& L, const PairPre& K, const uint32_t* fl, float h, float g, uint32_t e) {
+    const PairRec& pr = K.pr;
+    const uint32_t a = pr.a, b = pr.b;
+    const bool mine = e < pr.m;
+    const V n = mk(pr.n.x, pr.n.y, pr.n.z);
+    const Q qa = L.q(a), qb = L.q(b);
+    const PointRec pt = K.pt;
+    const V ra = rot(qa, mk(pt.aa.x, pt.aa.y, pt.aa.z)), rb = rot(qb, mk(pt.ab.x, pt.ab.y, pt.ab.z));
+    const float mu = K.fa * K.fb;
+    const bool da = dynamic_f(fl[a]), db = dynamic_f(fl[b]);
+    V va = L.v(a), wa = L.w(a), vb = L.v(b), wb = L.w(b);
+    auto rel = [&]() { return (va + cross(wa, ra)) - (vb + cross(wb, rb)); };
+    auto apply = [&](bool on, V p, V dir, float need) {
+        const Move mv = quad_summed(L, K, da, db, a, b, ra, rb, on, p);
+        const float s = quad_scale(mv, ra, rb, on, dir, need);
+        if (da) { va = va + mv.da * s; wa = wa + mv.ta * s; }
+        if (db) { vb = vb + mv.db * s; wb = wb + mv.tb * s; }
+        if (e == 0) {
+            if (da) { L.set_v(a, va); L.set_w(a, wa); }
+            if (db) { L.set_v(b, vb); L.set_w(b, wb); }
+        }
+    };
+    bool on = false;
+    V p = mk(0.f, 0.f, 0.f), dir = p;
+    float need = 0.f;
+    const float lam = pt.aa.w;
+    if (mine && lam != 0.f) {
+        const V vr = rel();
+        const V vt = vr - n * dot(vr, n);
+        const float lt = length(vt);
+        if (lt >= 1e-9f) {
+            const V t = vt * (1.0f / lt);
+            const float wsum = w_along_k(L, fl, a, K.ia, ra, t) + w_along_k(L, fl, b, K.ib, rb, t);
+            const float j = fminf(lt / wsum, mu * fabsf(lam) / h);
+            p = t * (-j);
+            dir = -t;
+            need = j * wsum;
+            on = true;
+        }
+    }
+    if (qany(on)) apply(on, p, dir, need);
+    on = false;
+    const float e_pair = K.ea * K.eb;
+    if (mine && lam != 0.f) {
+        const float vn0 = pt.e.x;
+        const float vn = dot(rel(), n);
+        const float rest = fabsf(vn0) <= 2.0f * g * h ? 0.f : e_pair;
+        const float wsum = w_along_k(L, fl, a, K.ia, ra, n) + w_along_k(L, fl, b, K.ib, rb, n);
+        const float nd = -rest * fmaxf(vn0, 0.f) - vn;
+        if (wsum > 0.f && nd < -1e-12f) {
+            p = n * (nd / wsum);
+            dir = n;
+            need = nd;
+            on = true;
+        }
+    }
+    if (qany(on)) apply(on, p, dir, need);
+}
+
 // A pair's place in phiray's pair order (broad.rs sorts the pairs (a, b),
 // a < b; contact.rs walks them in that order): unique, since bodies number
 // fewer than 2^11. Colouring by it, a pair joins a round only once every
@@ -829,18 +1200,31 @@ __host__ __device__ void velocity_pair(Lds& L, const Konst* kc, const uint32_t* 
 // in sequence do exactly what the pairs one after another in that order do.
 __host__ __device__ __forceinline__ uint32_t priority(const PairRec& pr) { return (pr.a << 11) | pr.b; }
 
+// QUAD: four lanes a pair in the projections. NB: bodies the LDS arrays hold.
+// LPUSH: the pushed-out offsets (read and written at every projection) in
+// LDS too, for worlds of at most LPUSH_BODIES.
+template <bool QUAD, uint32_t NB, bool LPUSH>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodies, const Konst* __restrict__ kc,
                                                          Aux* __restrict__ aux, const uint32_t* __restrict__ count,
                                                          const PairRec* __restrict__ pairs, PointRec* __restrict__ points,
                                                          uint32_t* __restrict__ live, uint32_t* __restrict__ order,
-                                                         float4* __restrict__ prev, float4* __restrict__ push,
+                                                         PairRec* __restrict__ spairs, PointRec* __restrict__ spoints,
+                                                         float4* __restrict__ prev, float4* __restrict__ gpush,
                                                          float* __restrict__ poses, uint32_t* __restrict__ stats, Params P) {
-    __shared__ float4 s_body[4 * MAX_BODIES];
-    __shared__ uint32_t s_flags[MAX_BODIES];
-    __shared__ uint32_t s_claim[MAX_BODIES];
+    __shared__ float4 s_body[4 * NB];
+    __shared__ uint32_t s_flags[NB];
+    __shared__ uint32_t s_claim[NB];
+    __shared__ float4 s_push[LPUSH ? 2 * NB : 1];
     __shared__ uint32_t s_cstart[MAX_COLOURS + 1];
+    constexpr uint32_t LIVE = LPUSH ? 5632u : 1792u;  // live pairs whose bodies the LDS left over holds
+    __shared__ uint32_t s_ab[LIVE];
+    float4* push = LPUSH ? s_push : gpush;
     __shared__ uint32_t s_nlive, s_cursor, s_colours, s_contacts, s_pairs;
     const uint32_t tid = threadIdx.x, T = SOLVE_THREADS, n = P.n;
+    if (stats[4] != 0u) {  // a body listed more pairs than it has slots: report, change nothing
+        if (tid == 0) stats[3] = 1u;
+        return;
+    }
     Lds L{s_body};
     for (uint32_t i = tid; i < n; i += T) {
         const float4* src = (const float4*)&bodies[i];
@@ -857,16 +1241,18 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
     // on which thread looks first.
     for (uint32_t i = tid; i < n; i += T) s_claim[i] = 0u;
     __syncthreads();
-    const uint32_t slots = n * PAIR_SLOTS;
-    for (uint32_t s = tid; s < slots; s += T) {
-        if (s % PAIR_SLOTS >= count[s / PAIR_SLOTS]) continue;
-        const PairRec pr = pairs[s];
-        atomicAdd(&s_pairs, 1u);
-        if (pr.m == 0) continue;
-        atomicAdd(&s_contacts, pr.m);
-        const uint32_t fa = s_flags[pr.a], fb = s_flags[pr.b];
-        if ((fa & SLEEPING) && moves_f(fb)) s_claim[pr.a] = 1u;
-        if ((fb & SLEEPING) && moves_f(fa)) s_claim[pr.b] = 1u;
+    // A thread a body, over the pairs it owns (its count of its 32 slots).
+    for (uint32_t i = tid; i < n; i += T) {
+        const uint32_t c = count[i];
+        for (uint32_t k = 0; k < c; ++k) {
+            const PairRec pr = pairs[(size_t)i * PAIR_SLOTS + k];
+            atomicAdd(&s_pairs, 1u);
+            if (pr.m == 0) continue;
+            atomicAdd(&s_contacts, pr.m);
+            const uint32_t fa = s_flags[pr.a], fb = s_flags[pr.b];
+            if ((fa & SLEEPING) && moves_f(fb)) s_claim[pr.a] = 1u;
+            if ((fb & SLEEPING) && moves_f(fa)) s_claim[pr.b] = 1u;
+        }
     }
     __syncthreads();
     for (uint32_t i = tid; i < n; i += T)
@@ -876,37 +1262,65 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
         }
     __syncthreads();
     // The pairs the solve works on: points, and a body that moves under them.
-    for (uint32_t s = tid; s < slots; s += T) {
-        if (s % PAIR_SLOTS >= count[s / PAIR_SLOTS]) continue;
-        const PairRec pr = pairs[s];
-        if (pr.m == 0 || !(dynamic_f(s_flags[pr.a]) || dynamic_f(s_flags[pr.b]))) continue;
-        live[atomicAdd(&s_nlive, 1u)] = s;
+    for (uint32_t i = tid; i < n; i += T) {
+        const uint32_t c = count[i];
+        for (uint32_t k = 0; k < c; ++k) {
+            const uint32_t s = i * PAIR_SLOTS + k;
+            const PairRec pr = pairs[s];
+            if (pr.m == 0 || !(dynamic_f(s_flags[pr.a]) || dynamic_f(s_flags[pr.b]))) continue;
+            live[atomicAdd(&s_nlive, 1u)] = s;
+        }
     }
     __syncthreads();
-    // Colours: rounds of a matching in phiray's pair order.
+    // Colours: rounds of a matching in phiray's pair order. The live pairs'
+    // bodies are kept in LDS (a | b << 16) when they fit, so a round reads
+    // no global memory.
     const uint32_t nlive = s_nlive;
+    const bool cached = nlive <= LIVE;
+    if (cached) {
+        for (uint32_t e = tid; e < nlive; e += T) {
+            const PairRec pr = pairs[live[e]];
+            s_ab[e] = pr.a | (pr.b << 16);
+        }
+        __syncthreads();
+    }
+    // The bodies of live pair e, false once it has its colour.
+    auto bodies_of = [&](uint32_t e, uint32_t& a, uint32_t& b) {
+        if (cached) {
+            const uint32_t ab = s_ab[e];
+            if (ab == NONE) return false;
+            a = ab & 0xFFFFu;
+            b = ab >> 16;
+            return true;
+        }
+        const uint32_t s = live[e];
+        if (s == NONE) return false;
+        const PairRec pr = pairs[s];
+        a = pr.a;
+        b = pr.b;
+        return true;
+    };
     for (uint32_t round = 0; round < nlive; ++round) {
         if (s_cursor == nlive || s_colours == MAX_COLOURS) break;
         for (uint32_t i = tid; i < n; i += T) s_claim[i] = NONE;
         __syncthreads();
         for (uint32_t e = tid; e < nlive; e += T) {
-            const uint32_t s = live[e];
-            if (s == NONE) continue;
-            const PairRec pr = pairs[s];
-            const uint32_t key = priority(pr);
-            if (dynamic_f(s_flags[pr.a])) atomicMin(&s_claim[pr.a], key);
-            if (dynamic_f(s_flags[pr.b])) atomicMin(&s_claim[pr.b], key);
+            uint32_t a, b;
+            if (!bodies_of(e, a, b)) continue;
+            const uint32_t key = (a << 11) | b;  // priority()
+            if (dynamic_f(s_flags[a])) atomicMin(&s_claim[a], key);
+            if (dynamic_f(s_flags[b])) atomicMin(&s_claim[b], key);
         }
         __syncthreads();
         for (uint32_t e = tid; e < nlive; e += T) {
-            const uint32_t s = live[e];
-            if (s == NONE) continue;
-            const PairRec pr = pairs[s];
-            const uint32_t key = priority(pr);
-            const bool ok = (!dynamic_f(s_flags[pr.a]) || s_claim[pr.a] == key) && (!dynamic_f(s_flags[pr.b]) || s_claim[pr.b] == key);
+            uint32_t a, b;
+            if (!bodies_of(e, a, b)) continue;
+            const uint32_t key = (a << 11) | b;
+            const bool ok = (!dynamic_f(s_flags[a]) || s_claim[a] == key) && (!dynamic_f(s_flags[b]) || s_claim[b] == key);
             if (ok) {
-                order[atomicAdd(&s_cursor, 1u)] = s;
-                live[e] = NONE;
+                order[atomicAdd(&s_cursor, 1u)] = live[e];
+                if (cached) s_ab[e] = NONE;
+                else live[e] = NONE;
             }
         }
         __syncthreads();
@@ -918,15 +1332,23 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
         if (tid == 0) { stats[0] = s_pairs; stats[1] = s_contacts; stats[2] = ncol; stats[3] = 2u; }
         return;
     }
+    // The pairs and their points copied into colour order: a projection then
+    // reads its records by its own index, one dependent load fewer.
+    for (uint32_t e = tid; e < nlive; e += T) {
+        const uint32_t s = order[e];
+        const PairRec pr = pairs[s];
+        spairs[e] = pr;
+        for (uint32_t c = 0; c < pr.m; ++c) spoints[4 * (size_t)e + c] = points[4 * (size_t)s + c];
+    }
+    __syncthreads();
     const V grav = mk(P.gx, P.gy, P.gz);
     const float h = P.h;
     for (uint32_t sub = 0; sub < P.substeps; ++sub) {
         // Each point's closing speed before the substep; multipliers cleared.
         for (uint32_t e = tid; e < nlive; e += T) {
-            const uint32_t s = order[e];
-            const PairRec pr = pairs[s];
+            const PairRec pr = spairs[e];
             const V nn = mk(pr.n.x, pr.n.y, pr.n.z);
-            PointRec* pts = points + 4 * (size_t)s;
+            PointRec* pts = spoints + 4 * (size_t)e;
             const Q qa = L.q(pr.a), qb = L.q(pr.b);
             for (uint32_t c = 0; c < pr.m; ++c) {
                 const V ra = rot(qa, mk(pts[c].aa.x, pts[c].aa.y, pts[c].aa.z)), rb = rot(qb, mk(pts[c].ab.x, pts[c].ab.y, pts[c].ab.z));
@@ -954,14 +1376,50 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
             L.set_q(i, qstep(q, L.w(i), h));
         }
         __syncthreads();
-        for (uint32_t it = 0; it < P.iterations; ++it)
-            for (uint32_t c = 0; c < ncol; ++c) {
-                for (uint32_t e = s_cstart[c] + tid; e < s_cstart[c + 1]; e += T) {
-                    const uint32_t s = order[e];
-                    project_pair(L, kc, s_flags, prev, push, pairs[s], points + 4 * (size_t)s, P.cap);
+        if constexpr (QUAD) {
+            // A group of four lanes takes the colour's entries g, g + T/4, ...;
+            // its first entry of the next colour is read while this one is
+            // solved (the next iteration's colour 0 after the last colour).
+            // With one colour the next entry is this one, whose multipliers
+            // this projection changes: nothing is read ahead then.
+            const uint32_t g = tid / 4, lane = tid & 3u;
+            const bool ahead_ok = ncol > 1;
+            PairPre nxt;
+            bool have = ahead_ok && s_cstart[0] + g < s_cstart[1];
+            if (have) pair_pre(nxt, spairs, spoints, prev, kc, s_cstart[0] + g, lane);
+            for (uint32_t it = 0; it < P.iterations; ++it)
+                for (uint32_t c = 0; c < ncol; ++c) {
+                    const uint32_t e0 = s_cstart[c] + g, e1 = s_cstart[c + 1];
+                    if (!ahead_ok) {
+                        for (uint32_t e = e0; e < e1; e += T / 4) {
+                            PairPre one;
+                            pair_pre(one, spairs, spoints, prev, kc, e, lane);
+                            project_quad(L, one, s_flags, push, spoints + 4 * (size_t)e, P.cap, lane);
+                        }
+                        __syncthreads();
+                        continue;
+                    }
+                    const PairPre cur = nxt;
+                    const bool mine = have;
+                    const uint32_t nc = c + 1 < ncol ? c + 1 : 0u;
+                    have = !(c + 1 == ncol && it + 1 == P.iterations) && s_cstart[nc] + g < s_cstart[nc + 1];
+                    if (have) pair_pre(nxt, spairs, spoints, prev, kc, s_cstart[nc] + g, lane);
+                    if (mine) project_quad(L, cur, s_flags, push, spoints + 4 * (size_t)e0, P.cap, lane);
+                    for (uint32_t e = e0 + T / 4; e < e1; e += T / 4) {
+                        PairPre more;
+                        pair_pre(more, spairs, spoints, prev, kc, e, lane);
+                        project_quad(L, more, s_flags, push, spoints + 4 * (size_t)e, P.cap, lane);
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
+        } else {
+            for (uint32_t it = 0; it < P.iterations; ++it)
+                for (uint32_t c = 0; c < ncol; ++c) {
+                    for (uint32_t e = s_cstart[c] + tid; e < s_cstart[c + 1]; e += T)
+                        project_pair(L, kc, s_flags, prev, push, spairs[e], spoints + 4 * (size_t)e, P.cap);
+                    __syncthreads();
+                }
+        }
         // Velocities from the pose change against the drift's prediction,
         // less what the projections pushed out of standing penetrations.
         for (uint32_t i = tid; i < n; i += T) {
@@ -974,12 +1432,31 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
             L.set_w(i, w + (qrate(L.q(i), q_pred, h) - mk(u1.x, u1.y, u1.z) * (1.0f / h)));
         }
         __syncthreads();
-        for (uint32_t c = 0; c < ncol; ++c) {
-            for (uint32_t e = s_cstart[c] + tid; e < s_cstart[c + 1]; e += T) {
-                const uint32_t s = order[e];
-                velocity_pair(L, kc, s_flags, pairs[s], points + 4 * (size_t)s, h, P.g);
+        if constexpr (QUAD) {
+            const uint32_t g = tid / 4, lane = tid & 3u;
+            PairPre nxt;
+            bool have = ncol > 0 && s_cstart[0] + g < s_cstart[1];
+            if (have) pair_pre(nxt, spairs, spoints, prev, kc, s_cstart[0] + g, lane);
+            for (uint32_t c = 0; c < ncol; ++c) {
+                const uint32_t e0 = s_cstart[c] + g, e1 = s_cstart[c + 1];
+                const PairPre cur = nxt;
+                const bool mine = have;
+                have = c + 1 < ncol && s_cstart[c + 1] + g < s_cstart[c + 2];
+                if (have) pair_pre(nxt, spairs, spoints, prev, kc, s_cstart[c + 1] + g, lane);
+                if (mine) velocity_quad(L, cur, s_flags, h, P.g, lane);
+                for (uint32_t e = e0 + T / 4; e < e1; e += T / 4) {
+                    PairPre more;
+                    pair_pre(more, spairs, spoints, prev, kc, e, lane);
+                    velocity_quad(L, more, s_flags, h, P.g, lane);
+                }
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+            for (uint32_t c = 0; c < ncol; ++c) {
+                for (uint32_t e = s_cstart[c] + tid; e < s_cstart[c + 1]; e += T)
+                    velocity_pair(L, kc, s_flags, spairs[e], spoints + 4 * (size_t)e, h, P.g);
+                __syncthreads();
+            }
         }
         // The other half kick, then Blender's damping.
         for (uint32_t i = tid; i < n; i += T) {
@@ -1082,6 +1559,8 @@ struct phr_world {
     uint32_t* d_other = nullptr;
     PairRec* d_pairs = nullptr;
     PointRec* d_points = nullptr;
+    PairRec* d_spairs = nullptr;    // the pairs in colour order
+    PointRec* d_spoints = nullptr;  // their points
     uint32_t* d_live = nullptr;
     uint32_t* d_order = nullptr;
     float4* d_prev = nullptr;
@@ -1091,17 +1570,20 @@ struct phr_world {
     phr_bvh* tree = nullptr;
     uint32_t tree_n = 0;
     uint32_t stats[4] = {0, 0, 0, 0};
+    bool quad = true;  // four lanes a pair in the solve's projections
 };
 
 namespace {
 
 void world_release_device(phr_world* w) {
     for (void* p : {(void*)w->d_bodies, (void*)w->d_konst, (void*)w->d_aux, (void*)w->d_shapes, (void*)w->d_count,
-                    (void*)w->d_other, (void*)w->d_pairs, (void*)w->d_points, (void*)w->d_live, (void*)w->d_order,
+                    (void*)w->d_other, (void*)w->d_pairs, (void*)w->d_points, (void*)w->d_spairs, (void*)w->d_spoints,
+                    (void*)w->d_live, (void*)w->d_order,
                     (void*)w->d_prev, (void*)w->d_push, (void*)w->d_poses, (void*)w->d_stats})
         if (p) (void)hipFree(p);
     w->d_bodies = nullptr; w->d_konst = nullptr; w->d_aux = nullptr; w->d_shapes = nullptr; w->d_count = nullptr;
     w->d_other = nullptr; w->d_pairs = nullptr; w->d_points = nullptr; w->d_live = nullptr; w->d_order = nullptr;
+    w->d_spairs = nullptr; w->d_spoints = nullptr;
     w->d_prev = nullptr; w->d_push = nullptr; w->d_poses = nullptr; w->d_stats = nullptr;
     phr_array_release(w->d_boxes);
     w->d_boxes = nullptr;
@@ -1136,6 +1618,7 @@ int world_push(phr_world* w) {
                         dev((void**)&w->d_aux, sizeof(Aux) * cap) && dev((void**)&w->d_shapes, sizeof(phr_shape) * caps) &&
                         dev((void**)&w->d_count, 4ull * cap) && dev((void**)&w->d_other, 4 * slots) &&
                         dev((void**)&w->d_pairs, sizeof(PairRec) * slots) && dev((void**)&w->d_points, sizeof(PointRec) * 4 * slots) &&
+                        dev((void**)&w->d_spairs, sizeof(PairRec) * slots) && dev((void**)&w->d_spoints, sizeof(PointRec) * 4 * slots) &&
                         dev((void**)&w->d_live, 4 * slots) && dev((void**)&w->d_order, 4 * slots) &&
                         dev((void**)&w->d_prev, 32ull * cap) && dev((void**)&w->d_push, 32ull * cap) &&
                         dev((void**)&w->d_poses, 64ull * cap) && dev((void**)&w->d_stats, 32);
@@ -1189,6 +1672,14 @@ int phr_world_configure(phr_world* w, uint32_t substeps, uint32_t iterations, fl
     if (gravity) std::memcpy(w->gravity, gravity, sizeof w->gravity);
     return PHR_OK;
 }
+
+int phr_world_set_lanes(phr_world* w, uint32_t lanes) {
+    PHR_CHECK(w, PHR_ERR_ARG, "phr_world_set_lanes: null");
+    PHR_CHECK(lanes == 1u || lanes == 4u, PHR_ERR_ARG, "phr_world_set_lanes: 1 or 4 lanes a pair");
+    w->quad = lanes == 4u;
+    return PHR_OK;
+}
+uint32_t phr_world_lanes(const phr_world* w) { return w ? (w->quad ? 4u : 1u) : 0u; }
 
 int phr_world_add_shape(phr_world* w, const phr_shape* s, uint32_t* id) {
     PHR_CHECK(w && s && id, PHR_ERR_ARG, "phr_world_add_shape: null");
@@ -1280,9 +1771,15 @@ int phr_world_step(phr_world* w, float frame_dt) {
     }
     uint32_t* overflow = w->d_stats + 4;
     PHR_HIP(hipMemsetAsync(overflow, 0, 4, s));
-    hipLaunchKernelGGL(k_pairs, dim3((n + 255) / 256), dim3(256), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
-                       (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->d_boxes->ptr, w->d_bodies,
-                       w->d_konst, n, w->d_count, w->d_other, overflow);
+    PHR_CHECK(phr_bvh_internal_nodes(w->tree)->len >= 2ull * n - 1, PHR_ERR_SIZE, "phr_world_step: the broadphase tree is short");
+    if (n <= LDS_PAIR_BODIES)
+        hipLaunchKernelGGL(k_pairs_lds, dim3((n + 63) / 64), dim3(64), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
+                           (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->d_boxes->ptr, w->d_bodies,
+                           w->d_konst, n, w->d_count, w->d_other, overflow);
+    else
+        hipLaunchKernelGGL(k_pairs, dim3((n + 255) / 256), dim3(256), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
+                           (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->d_boxes->ptr, w->d_bodies,
+                           w->d_konst, n, w->d_count, w->d_other, overflow);
     const uint32_t slots = n * PAIR_SLOTS;
     hipLaunchKernelGGL(k_contacts, dim3((slots + 127) / 128), dim3(128), 0, s, w->d_bodies, w->d_konst, w->d_shapes, w->d_count,
                        w->d_other, n, dt, w->d_pairs, w->d_points);
@@ -1298,18 +1795,25 @@ int phr_world_step(phr_world* w, float frame_dt) {
     P.gy = w->gravity[1];
     P.gz = w->gravity[2];
     P.sleep_after = w->sleep_after;
-    // A pair list that overflowed is caught before the solve changes anything.
-    uint32_t ovf = 0;
-    PHR_HIP(hipMemcpyAsync(&ovf, overflow, 4, hipMemcpyDeviceToHost, s));
-    PHR_HIP(hipStreamSynchronize(s));
-    PHR_CHECK(ovf == 0, PHR_ERR_SIZE, "phr_world_step: a body meets more than 32 others");
-    hipLaunchKernelGGL(k_solve, dim3(1), dim3(SOLVE_THREADS), 0, s, w->d_bodies, w->d_konst, w->d_aux, w->d_count, w->d_pairs,
-                       w->d_points, w->d_live, w->d_order, w->d_prev, w->d_push, w->d_poses, w->d_stats, P);
+    // A pair list that overflowed stops the solve before it changes anything
+    // (status 1); more colours than it can hold, too (status 2).
+    auto solve = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(1), dim3(SOLVE_THREADS), 0, s, w->d_bodies, w->d_konst, w->d_aux, w->d_count, w->d_pairs,
+                           w->d_points, w->d_live, w->d_order, w->d_spairs, w->d_spoints, w->d_prev, w->d_push, w->d_poses,
+                           w->d_stats, P);
+    };
+    if (w->quad && n <= LPUSH_BODIES)
+        solve(k_solve<true, LPUSH_BODIES, true>);
+    else if (w->quad)
+        solve(k_solve<true, MAX_BODIES, false>);
+    else
+        solve(k_solve<false, MAX_BODIES, false>);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipMemcpyAsync(w->stats, w->d_stats, 16, hipMemcpyDeviceToHost, s));
     PHR_HIP(hipStreamSynchronize(s));
     w->host_stale = true;
-    PHR_CHECK(w->stats[3] == 0, PHR_ERR_SIZE, "phr_world_step: the pairs need more than 1,024 colours");
+    PHR_CHECK(w->stats[3] != 1u, PHR_ERR_SIZE, "phr_world_step: a body meets more than 32 others");
+    PHR_CHECK(w->stats[3] == 0u, PHR_ERR_SIZE, "phr_world_step: the pairs need more than 1,024 colours");
     return PHR_OK;
 }
 

@@ -21,6 +21,7 @@ sample is the answer.
 """
 from __future__ import annotations
 
+import math
 import pathlib
 import time
 
@@ -229,7 +230,9 @@ class LearnedFrame:
     image's relMSE from half B, and takes twice the samples while the
     estimate is over `bound`. The answer is the network's output from both
     halves together, or, once `max_spp` samples would be passed, the plain
-    render of every sample taken."""
+    render of every sample taken. A failed check takes as many more samples
+    as its estimate asks for, the error falling as 1 / samples, at least one
+    and at most as many as were taken."""
 
     def __init__(self, gpu: Gpu, tris: np.ndarray, net: Kpcn, leaf: int = 4):
         self.gpu, self.net = gpu, net
@@ -252,7 +255,9 @@ class LearnedFrame:
     def render(self, cam: np.ndarray, w: int, h: int, spp: int = 4, bound: float = 0.01, max_spp: int = 256) -> dict:
         """Render, check and denoise; the image is left in `self.out`
         (3 floats a pixel). Returns what was done: samples taken, each
-        check's estimate, whether the plain render was the answer."""
+        check's estimate, whether the plain render was the answer. Past the
+        first check each half goes to min(2 t, max(t + 1, ceil(t err /
+        bound))) samples from t."""
         self._buffers(w, h)
         n = w * h
         half = max(spp // 2, 1)
@@ -271,11 +276,12 @@ class LearnedFrame:
                 Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.fm)
                 self.net.infer(self.fm, 1, w, h, self.out)
                 return {"spp": 2 * taken, "checks": checks, "plain": False}
-            if 4 * taken > max_spp:
+            nxt = min(2 * taken, max(taken + 1, math.ceil(taken * err / bound)))
+            if 2 * nxt > max_spp:
                 break
-            self.a.sample(ca, w, h, taken)  # the same view: the halves go on accumulating
-            self.b.sample(cb, w, h, taken)
-            taken *= 2
+            self.a.sample(ca, w, h, nxt - taken)  # the same view: the halves go on accumulating
+            self.b.sample(cb, w, h, nxt - taken)
+            taken = nxt
         Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.out, channels=3)
         self.gpu.wait()
         return {"spp": 2 * taken, "checks": checks, "plain": True}

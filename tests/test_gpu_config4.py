@@ -155,3 +155,23 @@ def test_the_world_steps_the_1024_bodies(gpu, orc):
     radius = np.array([float(d["r"]) for d in defs], dtype=np.float32)
     assert (b["x"][1:, 2] > 0.5 * radius).all(), b["x"][1:, 2].min()
     assert dev.contact_count() > 100
+
+
+def test_the_four_lane_solve_equals_the_one_lane_solve(gpu):
+    """The solve's projections on four lanes a pair (a point a lane, sums
+    gathered in point order) give the bits of the one-lane build: the 1,024
+    bodies over 60 frames of 8 substeps, every body record compared every 10
+    frames, once most of the heap has landed and pairs share bodies."""
+    print("[config4] four-lane solve against the one-lane solve, 60 frames", flush=True)
+    defs = S.bodies_defs(1024, heights=(1.0, 10.0))
+    one, four = _world(lambda: P.World(gpu), defs), _world(lambda: P.World(gpu), defs)
+    one.set_lanes(1)
+    assert one.lanes == 1 and four.lanes == 4
+    for f in range(60):
+        for _ in range(8):
+            one.step(1.0 / 480.0)
+            four.step(1.0 / 480.0)
+        if f % 10 == 9:
+            a, b = one.bodies(), four.bodies()
+            assert a.tobytes() == b.tobytes(), (f, int((a != b).sum()))
+    assert four.contact_count() > 200

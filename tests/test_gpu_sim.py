@@ -96,3 +96,30 @@ def test_trajectories_follow_the_oracle(gpu, orc, which, frames):
     dx = np.abs(a["x"] - b["x"]).max()
     dv = np.abs(a["v"] - b["v"]).max()
     assert dx < 2e-3 and dv < 2e-2, (which, dx, dv, a["x"], b["x"])
+
+
+@pytest.mark.parametrize("which", ["drop", "head_on", "box_slide", "stack"])
+def test_four_lanes_give_the_one_lane_bits(gpu, which):
+    """The solve's four-lanes-a-pair projections (with the next colour's
+    records read ahead) against the one-lane build, bit for bit, on worlds of
+    one or two colours (where the next entry to read is the one being
+    solved) and the ten-box stack."""
+    print(f"[sim] four lanes against one: {which}", flush=True)
+    worlds = []
+    for lanes in (1, 4):
+        w = P.World(gpu)
+        w.set_lanes(lanes)
+        if which == "stack":
+            C.ground(w)
+            b = w.add_box((0.5, 0.5, 0.5))
+            for k in range(10):
+                w.add_body(shape=b, x=(0.0, 0.0, 0.5 + k))
+        else:
+            _scene(w, which)
+        worlds.append(w)
+    for f in range(120):
+        for w in worlds:
+            w.step(C.DT)
+        if f % 30 == 29:
+            a, b = worlds[0].bodies(), worlds[1].bodies()
+            assert a.tobytes() == b.tobytes(), (which, f)

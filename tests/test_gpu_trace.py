@@ -122,8 +122,9 @@ def test_path_kernel_variants_agree(gpu, orc):
         sc.sample(cam, w, h, px, f, nthreads=8)
     tr = P.Tracer.new(gpu, tris, 4)
     assert tr.path_waves == 2 and not tr.path_regen
-    for regen in (False, True):
+    for regen in (False, True, P.Tracer.PATH_WAVEFRONT):
         tr.set_path_regen(regen)
+        assert tr.path_regen == int(regen)
         for waves in (2, 3, 4, 5):
             tr.set_path_waves(waves)
             fresh = cam.copy()
@@ -144,13 +145,16 @@ def soup(n, seed):
     return S.tris_from_corners(corners, color=(0.7, 0.6, 0.5, 1.0))
 
 
+@pytest.mark.parametrize("build", [1, 2], ids=["regen", "wavefront"])
 @pytest.mark.parametrize("case", ["lamps", "world", "occlusion"])
-def test_path_regeneration_matches_the_twin(gpu, orc, case):
+def test_path_regeneration_matches_the_twin(gpu, orc, case, build):
     """The path-regenerating build (a lane that ends a sample starts its
-    pixel's next one): every plane equals the oracle's, samples spread over
-    three calls, for a scene lit by lamps, a soup under the world colour,
-    and a soup in a black world (the ambient-occlusion bounce)."""
-    print(f"[trace] path regeneration, {case}", flush=True)
+    pixel's next one) and the wavefront build (walks, shading and shadow
+    walks as kernels of their own over queues of slots): every plane and the
+    ray counts equal the oracle's, samples spread over three calls, for a
+    scene lit by lamps, a soup under the world colour, and a soup in a black
+    world (the ambient-occlusion bounce)."""
+    print(f"[trace] path regeneration, {case}, build {build}", flush=True)
     w, h = 56, 40
     cam = S.path_camera(w, h, rr_from=3)
     if case == "lamps":
@@ -166,7 +170,7 @@ def test_path_regeneration_matches_the_twin(gpu, orc, case):
         r = sc.sample(cam, w, h, px, f, nthreads=8)
         rays = [rays[0] + r[0], rays[1] + r[1]]
     tr = P.Tracer.new(gpu, tris, 4)
-    tr.set_path_regen(True)
+    tr.set_path_regen(build)
     tr.set_path_waves(4)
     for spp in (1, 3, 1):
         tr.sample(cam, w, h, spp)
@@ -280,7 +284,7 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     """The full path workload: the 1,310,724-triangle spheres scene at
     1920x1080 (2 samples of the 64 the bench takes). Three bands of rows (top,
     middle, bottom) traced by the oracle equal the device frame's planes, for
-    both path kernel builds."""
+    the nested, regenerating and wavefront builds."""
     print("[trace] full frame bands, 1.31M tris, 1920x1080", flush=True)
     tris = S.spheres_scene(spheres=64, level=5, seed=1)
     assert len(tris) == 1_310_724
@@ -295,6 +299,10 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     tr.sample(S.path_camera(64, 64), 64, 64, 1)  # another view: the next call starts over
     tr.sample(cam, w, h, spp)
     regen = P.planes(tr.px(), w, h)
+    tr.set_path_regen(P.Tracer.PATH_WAVEFRONT)
+    tr.sample(S.path_camera(64, 64), 64, 64, 1)
+    tr.sample(cam, w, h, spp)
+    wave = P.planes(tr.px(), w, h)
     sc = orc.Scene(tris, 4, P.NODE, 8)
     px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
     bands = [(0, 6), (h // 2 - 3, 6), (h - 6, 6)]
@@ -306,7 +314,7 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     for row, rows in bands:
         sl = slice(row * w, (row + rows) * w)
         for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline", "display", "pixel"]:
-            for build, planes in (("plain", got), ("regen", regen)):
+            for build, planes in (("plain", got), ("regen", regen), ("wavefront", wave)):
                 a, b = planes[k][sl], want[k][sl]
                 if k == "pixel":
                     assert (np.abs(a.astype(np.int32) - b.astype(np.int32)) <= 1).all(), (build, row, k)
