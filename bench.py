@@ -36,6 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # the root node (32), both children at every inner step (2 x 32), one
 # triangle record per test (48: three corners in 16-byte rows).
 RAY_BYTES, HIT_BYTES, ROOT_BYTES, STEP_BYTES, TEST_BYTES = 32, 8, 32, 64, 48
+PATH_BUILDS = ("nested", "regen", "wavefront", "split")  # the tracer's path_regen modes 0-3
 
 
 def profiled_traffic(name: str, spp: int, w: int, h: int, waves: int, build: str) -> tuple[float | None, str | None]:
@@ -553,7 +554,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "path", "lbvh", "forest", "converge", "batch", "kpcn"], default="all")
     ap.add_argument("--path-waves", type=int, default=0, help="run this path kernel variant (2 to 5) instead of autotuning")
-    ap.add_argument("--path-build", choices=["auto", "nested", "regen", "wavefront"], default="auto",
+    ap.add_argument("--path-build", choices=["auto", "nested", "regen", "wavefront", "split"], default="auto",
                     help="with --path-waves: which path build runs (default: the nested one)")
     ap.add_argument("--copy-mb", type=int, default=0, help="also run the calibration copy kernel over this many MiB")
     ap.add_argument("--c5-seeds", type=int, default=8, help="held-out scenes of the learned-vs-plain leg")
@@ -643,18 +644,18 @@ def main() -> int:
         log(f"Tracer::new {time.perf_counter() - t0:.2f} s, depth {tracer.depth}, {tracer.glow} emitters")
         if args.path_waves:
             tracer.set_path_waves(args.path_waves)
-            build = {"auto": 0, "nested": 0, "regen": 1, "wavefront": 2}[args.path_build]
+            build = {"auto": 0, "nested": 0, "regen": 1, "wavefront": 2, "split": 3}[args.path_build]
             tracer.set_path_regen(build)
             log(f"path kernel at {args.path_waves} waves a SIMD, build {args.path_build} (chosen on the command line)")
         else:
             try:
                 waves, tune_s = tracer.autotune(cam, w, h, 8, reps=2)
-                build_name = ("nested", "regen", "wavefront")[tracer.path_regen]
+                build_name = PATH_BUILDS[tracer.path_regen]
                 log(f"autotune: path kernel at {waves} waves a SIMD, {build_name} build ({tune_s * 1e3:.1f} ms for 8 spp)")
                 # Each build at the chosen waves, 8 samples, for the line.
                 chosen = tracer.path_regen
                 by_build = {}
-                for b, name in enumerate(("nested", "regen", "wavefront")):
+                for b, name in enumerate(PATH_BUILDS):
                     tracer.set_path_regen(b)
                     tracer.sample(S.path_camera(64, 64), 64, 64, 1)  # another view: the next call starts over
                     tracer.sample(cam, w, h, 8)
@@ -685,12 +686,12 @@ def main() -> int:
         kernel_s = float(np.mean(gpu_s))
         nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
         achieved = nbytes / kernel_s / 1e9
-        build = ("nested", "regen", "wavefront")[tracer.path_regen]
+        build = PATH_BUILDS[tracer.path_regen]
         traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h, tracer.path_waves, build)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
                             "algorithmic_bytes": nbytes,
-                            "kernel": {"nested": "k_path", "regen": "k_path_rg",
+                            "kernel": {"nested": "k_path", "regen": "k_path_rg", "split": "k_path_sw",
                                        "wavefront": "k_wf_walk + k_wf_shade (one frame's turns)"}[build],
                             "per_ray_bytes": "32 ray + 8 hit + 32 root + 64 x inner steps + 48 x triangle tests",
                             "launch_ms": kernel_s * 1e3}
@@ -703,7 +704,7 @@ def main() -> int:
         line["s_per_frame"] = per
         line["path_waves_per_simd"] = tracer.path_waves
         line["path_regen"] = tracer.path_regen
-        line["path_build"] = ("nested", "regen", "wavefront")[tracer.path_regen]
+        line["path_build"] = PATH_BUILDS[tracer.path_regen]
         line["rays_per_frame"] = rays / args.steps
         line["closest_per_frame"] = closest / args.steps
         line["shadow_per_frame"] = shadow / args.steps

@@ -982,13 +982,15 @@ class Tracer:
     def path_waves(self) -> int:
         return int(lib().phr_tracer_path_waves(self._h))
 
-    PATH_NESTED, PATH_REGEN, PATH_WAVEFRONT = 0, 1, 2
+    PATH_NESTED, PATH_REGEN, PATH_WAVEFRONT, PATH_SPLIT = 0, 1, 2, 3
 
     def set_path_regen(self, mode: "bool | int") -> None:
         """Which build the path frames take: the nested one (False, 0), the
-        path-regenerating one (True, 1) or the wavefront one (2). The last two
-        serve one tree and a camera with lamps or a lit world, no Adaptive
-        Sampling; other frames take the nested build."""
+        path-regenerating one (True, 1), the wavefront one (2) or the split
+        one (3: a wave walks until half its lanes have finished their walk,
+        those shade and start their next walk while the others walk on). The
+        last three serve one tree and a camera with lamps or a lit world, no
+        Adaptive Sampling; other frames take the nested build."""
         m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
         _check(lib().phr_tracer_set_path_regen(self._h, m), "Tracer set_path_regen")
 

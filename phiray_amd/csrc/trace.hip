@@ -243,21 +243,34 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const f
     for (;;) {
         if (cur & PHR_LEAF) {
             uint32_t id = cur & ~PHR_LEAF;
-            do {
+            float4 t0 = tris[3 * id], t1 = tris[3 * id + 1], t2 = tris[3 * id + 2];
+            if (end == PHR_NONE) end = __float_as_uint(t2.y);
+            for (;;) {
                 ++tests;
-                float4 t0 = tris[3 * id], t1 = tris[3 * id + 1], t2 = tris[3 * id + 2];
-                if (end == PHR_NONE) end = __float_as_uint(t2.y);
+                // The leaf's next triangle is loaded while this one is tested.
+                const bool more = id + 1u < end;
+                float4 n0 = t0, n1 = t1, n2 = t2;
+                if (more) {
+                    n0 = tris[3 * (id + 1u)];
+                    n1 = tris[3 * (id + 1u) + 1];
+                    n2 = tris[3 * (id + 1u) + 2];
+                }
                 float u, v;
                 float t = tri_t(t0, t1, t2, o, rs, u, v);
                 if (t < best.t) {
                     if (ANY) {
                         const float* m = mats + MAT_WORDS * (size_t)faces[id].w;
-                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
-                        return PhrHit{t, id, u, v, 0u};
+                        if (!(m[6] > 0.5f || m[3] < 0.5f)) return PhrHit{t, id, u, v, 0u};
+                    } else {
+                        best = PhrHit{t, id, u, v, 0u};
                     }
-                    best = PhrHit{t, id, u, v, 0u};
                 }
-            } while (++id < end);
+                if (!more) break;
+                ++id;
+                t0 = n0;
+                t1 = n1;
+                t2 = n2;
+            }
         } else if (TWO ? visit2(recs + 4 * (size_t)cur, kids + 8 * (size_t)cur, o, inv, best.t, cur, end, stk, stride, sp, steps)
                        : visit1(recs + 4 * (size_t)cur, o, inv, best.t, cur, end, stk, stride, sp, steps)) {
             continue;
@@ -268,6 +281,92 @@ __device__ __forceinline__ PhrHit walk(const float4* __restrict__ nodes, const f
         end = PHR_NONE;
     }
     return best;
+}
+
+// A walk that can stop after any unit and go on later (the split build):
+// the ray, its best hit, where it is (cur, end) and its stack (sp entries of
+// the thread's column). A unit is an inner node's record (both boxes, the
+// nearer child entered and the farther pushed) or one triangle, so a walk
+// makes walk<ANY>'s visits, decisions and pushes in walk<ANY>'s order and
+// ends with its hit. `any`: a shadow ray's walk, which ends at the first
+// triangle that blocks it.
+struct Walker {
+    V3 o, inv;
+    RaySpace rs;
+    float t, u, v;  // closest: the nearest hit so far; shadow: t is the ray's length
+    uint32_t tri, cur, end, sp;
+    bool on, any;
+};
+__device__ __forceinline__ void walker_begin(Walker& w, const SceneDev& s, V3 o, V3 d, float tmax, bool any) {
+    w.o = o;
+    w.inv = safe_inv(d);
+    w.rs = ray_space(d);
+    w.t = tmax;
+    w.u = w.v = 0.f;
+    w.tri = PHR_NONE;
+    w.sp = 0;
+    w.any = any;
+    const float4 rlo = s.nodes[2 * s.root], rhi = s.nodes[2 * s.root + 1];
+    w.on = box_t(rlo, rhi, o, w.inv, tmax) < PHR_INF;
+    w.cur = __float_as_uint(rlo.w);
+    w.end = __float_as_uint(rhi.w);
+    if (!(w.cur & PHR_LEAF)) w.cur = s.root;
+}
+__device__ __forceinline__ void walker_step(Walker& w, const SceneDev& s, uint32_t* stk, uint32_t stride, uint32_t& steps,
+                                            uint32_t& tests) {
+    const bool leaf = (w.cur & PHR_LEAF) != 0u;
+    const float4* q = leaf ? s.tris + 3 * (size_t)(w.cur & ~PHR_LEAF) : s.recs + 4 * (size_t)w.cur;
+    const float4 a = q[0], b = q[1], c = q[2];
+    float4 d = a;
+    if (!leaf) d = q[3];
+    if (leaf) {
+        const uint32_t id = w.cur & ~PHR_LEAF;
+        ++tests;
+        if (w.end == PHR_NONE) w.end = __float_as_uint(c.y);
+        float u, v;
+        const float t = tri_t(a, b, c, w.o, w.rs, u, v);
+        if (t < w.t) {
+            if (w.any) {
+                const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
+                if (!(m[6] > 0.5f || m[3] < 0.5f)) {
+                    w.tri = id;
+                    w.on = false;
+                    return;
+                }
+            } else {
+                w.t = t;
+                w.tri = id;
+                w.u = u;
+                w.v = v;
+            }
+        }
+        if (id + 1u < w.end) {
+            w.cur = (id + 1u) | PHR_LEAF;
+            return;
+        }
+    } else {
+        ++steps;
+        const float tl = box_t(a, b, w.o, w.inv, w.t);
+        const float tr = box_t(c, d, w.o, w.inv, w.t);
+        const bool hl = tl < PHR_INF, hr = tr < PHR_INF;
+        if (hl || hr) {
+            const bool left = hl && (!hr || tl <= tr);
+            if (hl && hr) {
+                stk[w.sp * stride] = __float_as_uint(left ? c.w : a.w);
+                ++w.sp;
+            }
+            w.cur = __float_as_uint(left ? a.w : c.w);
+            w.end = __float_as_uint(left ? b.w : d.w);
+            return;
+        }
+    }
+    if (w.sp == 0) {
+        w.on = false;
+        return;
+    }
+    --w.sp;
+    w.cur = stk[w.sp * stride];
+    w.end = PHR_NONE;
 }
 
 // The walk records of a tree's nodes (one entry per node index; only inner
@@ -1462,6 +1561,167 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     }
 }
 
+// The same samples as k_path_rg with walks that span turns. Each lane keeps
+// one walk (a Walker: the sample's closest ray, or its shadow ray) and the
+// wave walks only until half of its lanes still at work have finished
+// theirs. Those lanes then shade (rg_hit after a closest walk, the turn's end
+// after a shadow walk) and begin their next walk, while the others keep
+// their walk where it stopped for the next turn. A wave no longer waits on
+// its longest walk, and closest and shadow walks share one loop. A lane's
+// walks and shading come in k_path_rg's order, so the sums are the same bits.
+// A turn with a shadow ray takes its end both ways (lamp seen, lamp blocked)
+// before the walk: a sample that ended waits with its radiance both ways; a
+// sample that goes on is advanced as lit and, if the ray was blocked, gets
+// back the two sums that differ (the camera hit's light, the bounces' sum).
+template <int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    const PhrCamera* p = camp;
+    const size_t n = (size_t)p->w * p->h;
+    const uint32_t blk = blockIdx.x + tile0;
+    const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
+    const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
+    const bool inside = pxl < p->w && pyl < p->h;
+    const uint32_t i = pyl * p->w + pxl;
+    uint32_t* stk = s_stack + threadIdx.x;
+    uint32_t closest = 0, shadow = 0, steps = 0, tests = 0;
+    const bool keep = frame0 > 0u;
+    V3 l = v3(0, 0, 0), ga = l, gn = l;
+    float gd = 0.f, cov = 0.f, gone = 0.f;
+    if (inside && keep) {
+        l = v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2));
+        ga = v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2));
+        gn = v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2));
+        gd = ldf(px, 13 * n + i);
+        cov = ldf(px, 23 * n + i);
+        gone = ldf(px, 33 * n + i);
+    }
+    uint32_t outline = 0;
+    const RgConst k = rg_const(p);
+    RgSample st;
+    st.seed = 0;
+    st.pass = st.n_see = st.b = st.phase = 0u;
+    st.o = st.d = st.col = st.cc = st.alb = st.acc = st.nrm = v3(0, 0, 0);
+    st.thr = v3(1, 1, 1);
+    st.p_diffuse = 1.0f;
+    st.min_pdf = 3.0e38f;
+    RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
+    Walker w;
+    w.on = w.any = false;
+    bool done = !inside || spp == 0u, pfinal = false;
+    V3 pa = v3(0, 0, 0), pb = pa;  // what a shadow walk's answer picks between
+    uint32_t s = 0;
+    if (!done) {
+        rg_start(st, g, p, i, frame0);
+        walker_begin(w, sc, st.o, st.d, PHR_INF, false);
+        ++closest;
+    }
+    while (__ballot(!done) != 0ull) {
+        for (;;) {  // walk until half the lanes at work wait to shade, or all do
+            const uint32_t waiting = (uint32_t)__popcll(__ballot(!done && !w.on));
+            const uint32_t working = (uint32_t)__popcll(__ballot(!done));
+            if (waiting == working || 2u * waiting >= working) break;
+            if (w.on) walker_step(w, sc, stk, TRACE_THREADS, steps, tests);
+        }
+        if (done || w.on) continue;
+        bool next = false;
+        if (w.any) {  // the turn's shadow ray has its answer
+            const bool blocked = w.tri != PHR_NONE;
+            if (pfinal) {
+                l = l + (blocked ? pb : pa);
+            } else if (blocked) {
+                st.cc = pa;
+                st.acc = pb;
+            }
+            next = true;
+        } else {  // the closest hit: the segment's shading
+            const PhrHit h{w.t, w.tri, w.u, w.v, 0u};
+            RgTurn q;
+            rg_hit(st, g, q, sc, p, k, h);
+            bool ended;
+            if (q.qon) {
+                RgPost lit{st.col, st.cc, st.acc, st.phase}, dark = lit;
+                ended = rg_post_of(lit, st.alb, st.p_diffuse, q, p, 1.0f);
+                rg_post_of(dark, st.alb, st.p_diffuse, q, p, 0.0f);
+                pfinal = ended;
+                if (ended) {
+                    pa = lit.col;
+                    pb = dark.col;
+                } else {
+                    st.col = lit.col;
+                    st.cc = lit.cc;
+                    st.acc = lit.acc;
+                    st.phase = lit.phase;
+                    pa = dark.cc;
+                    pb = dark.acc;
+                }
+                walker_begin(w, sc, q.qpos, q.qdir, q.qdist, true);
+                ++shadow;
+            } else {
+                ended = rg_post(st, q, p, 1.0f);
+                if (ended) l = l + st.col;
+                next = true;
+            }
+            if (ended) {  // shade's `out`, summed as path_block sums it
+                ga = ga + g.alb;
+                gn = gn + g.nrm;
+                gd = gd + g.dep;
+                gone = gone + g.missed;
+                cov = cov + g.cov;
+                outline = g.outline;
+                ++s;
+                st.phase = 0u;
+            }
+        }
+        if (next) {
+            if (s < spp) {
+                if (st.phase == 0u) rg_start(st, g, p, i, frame0 + s);
+                walker_begin(w, sc, st.o, st.d, PHR_INF, false);
+                ++closest;
+            } else {
+                done = true;
+            }
+        }
+    }
+    if (inside) {
+        V3 gs = keep ? v3(ldf(px, 20 * n + 3 * i), ldf(px, 20 * n + 3 * i + 1), ldf(px, 20 * n + 3 * i + 2)) : v3(0, 0, 0);
+        float film = keep ? ldf(px, 24 * n + i) : 0.f;
+        uint32_t taken = keep ? px[31 * n + i] : 0u;
+        for (uint32_t kk = 0; kk < spp; ++kk) {
+            gs = gs + v3(0.f, 0.f, 0.f);
+            film = film + 1.0f;
+        }
+        taken = taken + spp;
+        if (frame0 == 0u) { px[29 * n + i] = 0u; px[30 * n + i] = 0u; }
+        stf(px, n + 3 * i, l.x); stf(px, n + 3 * i + 1, l.y); stf(px, n + 3 * i + 2, l.z);
+        stf(px, 7 * n + 3 * i, ga.x); stf(px, 7 * n + 3 * i + 1, ga.y); stf(px, 7 * n + 3 * i + 2, ga.z);
+        stf(px, 10 * n + 3 * i, gn.x); stf(px, 10 * n + 3 * i + 1, gn.y); stf(px, 10 * n + 3 * i + 2, gn.z);
+        stf(px, 13 * n + i, gd);
+        stf(px, 20 * n + 3 * i, gs.x); stf(px, 20 * n + 3 * i + 1, gs.y); stf(px, 20 * n + 3 * i + 2, gs.z);
+        stf(px, 23 * n + i, cov);
+        stf(px, 24 * n + i, film);
+        px[25 * n + i] = outline;
+        px[31 * n + i] = taken;
+        stf(px, 33 * n + i, gone);
+        put_shown(px, n, i, *p, l, gs, cov, gone, film, taken);
+    }
+    unsigned long long rc = closest, rs = shadow, st_ = steps, tt = tests;
+    for (int kk = 32; kk > 0; kk >>= 1) {
+        rc += __shfl_xor(rc, kk);
+        rs += __shfl_xor(rs, kk);
+        st_ += __shfl_xor(st_, kk);
+        tt += __shfl_xor(tt, kk);
+    }
+    if ((threadIdx.x & 63) == 0 && (rc | rs)) {
+        atomicAdd(&rays[0], rc);
+        atomicAdd(&rays[1], rs);
+        atomicAdd(&rays[2], st_);
+        atomicAdd(&rays[3], tt);
+    }
+}
+
 // ---- the wavefront build --------------------------------------------------------
 // The regenerating build's samples with the walks taken out of the shading.
 // A path slot a pixel of the 16 x 16 tiles keeps its sample's state in HBM;
@@ -1949,15 +2209,25 @@ bool regen_allowed(const PhrCamera& cam) {
     return cam.adapt[3] == 0.0f && (cam.count != 0u || lit);
 }
 
-// regen: the path-regenerating build (k_path_rg), for one tree, no list and
-// no Adaptive Sampling or studio rig (the caller checks the camera).
+// regen: 0 the nested build; 3 the split build (k_path_sw); else the
+// regenerating build (k_path_rg); the last two for one tree, no list and no
+// Adaptive Sampling or studio rig (the caller checks the camera).
 // short_stack: the forest's 3-wave build with FOREST_SHORT_STACK mesh-tree
 // entries (the caller checked the mesh trees' depth).
-void launch_path(uint32_t waves, bool forest, bool regen, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp,
+void launch_path(uint32_t waves, bool forest, uint32_t regen, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp,
                  uint32_t tiles_x, uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed,
                  uint32_t tile0, bool short_stack = false) {
     const dim3 block(TRACE_THREADS);
-    if (regen && !forest && !listed) {
+    if (regen == 3u && !forest && !listed) {
+        if (waves == 5)
+            hipLaunchKernelGGL(k_path_sw<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 4)
+            hipLaunchKernelGGL(k_path_sw<4>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 3)
+            hipLaunchKernelGGL(k_path_sw<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else
+            hipLaunchKernelGGL(k_path_sw<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+    } else if (regen && !forest && !listed) {
         if (waves == 5)
             hipLaunchKernelGGL(k_path_rg<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
         else if (waves == 4)
@@ -2203,7 +2473,7 @@ struct phr_tracer {
     std::vector<uint32_t> order;          // slot -> input face
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
-    uint32_t path_regen = 0;  // 1: the path-regenerating build, 2: the wavefront build, where the camera allows it
+    uint32_t path_regen = 0;  // 1: the path-regenerating build, 2: the wavefront build, 3: the split build, where the camera allows it
     uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
     bool short_stack = false;         // the forest's 3-wave build with the 24-entry mesh-tree stack
     // The binding split phiray's WGSL needs (items a binding for nodes, faces,
@@ -2976,7 +3246,9 @@ int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, ui
         {"k_path_forest<4>", (const void*)k_path<true, 4>},
         {"k_path_forest<3, short stack>", (const void*)k_path<true, 3, FOREST_SHORT_STACK>}, {"k_path_rg<2>", (const void*)k_path_rg<2>},
         {"k_path_rg<3>", (const void*)k_path_rg<3>}, {"k_path_rg<4>", (const void*)k_path_rg<4>},
-        {"k_path_rg<5>", (const void*)k_path_rg<5>}, {"k_path_batch<2>", (const void*)k_path_batch<2>},
+        {"k_path_rg<5>", (const void*)k_path_rg<5>}, {"k_path_sw<2>", (const void*)k_path_sw<2>},
+        {"k_path_sw<3>", (const void*)k_path_sw<3>}, {"k_path_sw<4>", (const void*)k_path_sw<4>},
+        {"k_path_sw<5>", (const void*)k_path_sw<5>}, {"k_path_batch<2>", (const void*)k_path_batch<2>},
         {"k_path_batch<3>", (const void*)k_path_batch<3>}, {"k_path_batch<4>", (const void*)k_path_batch<4>},
         {"k_path_batch<5>", (const void*)k_path_batch<5>},
     };
@@ -3137,7 +3409,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
     slices = std::min(std::max<uint32_t>(slices, 1u), tiles);
     bool abandoned = false;
     // One launch of `count` samples from f0, in `slices` pieces of the tiles.
-    const bool regen = t->path_regen && regen_allowed(cam);
+    const uint32_t regen = regen_allowed(cam) ? t->path_regen : 0u;
     auto path = [&](uint32_t f0, uint32_t count, uint32_t listed) -> int {
         for (uint32_t k = 0; k < slices && !abandoned; ++k) {
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
@@ -3216,7 +3488,7 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t) { return t ? t->path_waves :
 // several slices) and a recorded frame take the regenerating build for 2
 // (no counterpart in phiray).
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on) {
-    PHR_CHECK(t && on <= 2u, PHR_ERR_ARG, "phr_tracer_set_path_regen: null, or not 0, 1 or 2");
+    PHR_CHECK(t && on <= 3u, PHR_ERR_ARG, "phr_tracer_set_path_regen: null, or not 0, 1, 2 or 3");
     t->path_regen = on;
     return PHR_OK;
 }
@@ -3251,7 +3523,7 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     double best = 1e30;
     PhrCamera probe;
     std::memcpy(&probe, cam, sizeof probe);
-    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 3u : 1u;
+    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 4u : 1u;
     const bool fits_short = t->forest && t->mesh_depth <= FOREST_SHORT_STACK;
     bool best_short = false;
     for (uint32_t regen = 0; regen < regens; ++regen)
@@ -3454,7 +3726,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     uint32_t* pxp = (uint32_t*)t->px->ptr;
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    launch_path(path_build_waves(t), t->forest, t->path_regen && regen_allowed(cam), dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
+    launch_path(path_build_waves(t), t->forest, regen_allowed(cam) ? t->path_regen : 0u, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
                 t->rays, 0u, 0u, path_short_stack(t));
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
@@ -3581,7 +3853,7 @@ int phr_tracer_record_pose_frame(phr_tracer* t, phr_array* verts, const float* w
     if (rc == PHR_OK) {
         hipLaunchKernelGGL(k_stamp_pose, dim3(1), dim3(64), 0, s, (const float4*)t->nodes->ptr, (const uint32_t*)t->lights->ptr, 1u,
                            cam.ao <= 0.f ? 1u : 0u, cp);
-        launch_path(path_build_waves(t), t->forest, false, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u,
+        launch_path(path_build_waves(t), t->forest, 0u, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp, t->rays, 0u, 0u,
                     path_short_stack(t));
         if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
         if (denoise == 2) {

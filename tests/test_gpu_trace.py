@@ -122,7 +122,7 @@ def test_path_kernel_variants_agree(gpu, orc):
         sc.sample(cam, w, h, px, f, nthreads=8)
     tr = P.Tracer.new(gpu, tris, 4)
     assert tr.path_waves == 2 and not tr.path_regen
-    for regen in (False, True, P.Tracer.PATH_WAVEFRONT):
+    for regen in (False, True, P.Tracer.PATH_WAVEFRONT, P.Tracer.PATH_SPLIT):
         tr.set_path_regen(regen)
         assert tr.path_regen == int(regen)
         for waves in (2, 3, 4, 5):
@@ -145,12 +145,14 @@ def soup(n, seed):
     return S.tris_from_corners(corners, color=(0.7, 0.6, 0.5, 1.0))
 
 
-@pytest.mark.parametrize("build", [1, 2], ids=["regen", "wavefront"])
+@pytest.mark.parametrize("build", [1, 2, 3], ids=["regen", "wavefront", "split"])
 @pytest.mark.parametrize("case", ["lamps", "world", "occlusion"])
 def test_path_regeneration_matches_the_twin(gpu, orc, case, build):
     """The path-regenerating build (a lane that ends a sample starts its
-    pixel's next one) and the wavefront build (walks, shading and shadow
-    walks as kernels of their own over queues of slots): every plane and the
+    pixel's next one), the wavefront build (walks, shading and shadow walks
+    as kernels of their own over queues of slots) and the split build (walks
+    that span turns, a lane shading as soon as half its wave waits): every
+    plane and the
     ray counts equal the oracle's, samples spread over three calls, for a
     scene lit by lamps, a soup under the world colour, and a soup in a black
     world (the ambient-occlusion bounce)."""
@@ -284,7 +286,7 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     """The full path workload: the 1,310,724-triangle spheres scene at
     1920x1080 (2 samples of the 64 the bench takes). Three bands of rows (top,
     middle, bottom) traced by the oracle equal the device frame's planes, for
-    the nested, regenerating and wavefront builds."""
+    the nested, regenerating, wavefront and split builds."""
     print("[trace] full frame bands, 1.31M tris, 1920x1080", flush=True)
     tris = S.spheres_scene(spheres=64, level=5, seed=1)
     assert len(tris) == 1_310_724
@@ -303,6 +305,10 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     tr.sample(S.path_camera(64, 64), 64, 64, 1)
     tr.sample(cam, w, h, spp)
     wave = P.planes(tr.px(), w, h)
+    tr.set_path_regen(P.Tracer.PATH_SPLIT)
+    tr.sample(S.path_camera(64, 64), 64, 64, 1)
+    tr.sample(cam, w, h, spp)
+    split = P.planes(tr.px(), w, h)
     sc = orc.Scene(tris, 4, P.NODE, 8)
     px = np.zeros(P.PX_LEN * w * h + 1, dtype=np.uint32)
     bands = [(0, 6), (h // 2 - 3, 6), (h - 6, 6)]
@@ -314,7 +320,7 @@ def test_full_frame_bands_match_the_twin(gpu, orc):
     for row, rows in bands:
         sl = slice(row * w, (row + rows) * w)
         for k in ["radiance", "albedo", "normal", "depth", "cov", "film", "count", "missed", "outline", "display", "pixel"]:
-            for build, planes in (("plain", got), ("regen", regen), ("wavefront", wave)):
+            for build, planes in (("plain", got), ("regen", regen), ("wavefront", wave), ("split", split)):
                 a, b = planes[k][sl], want[k][sl]
                 if k == "pixel":
                     assert (np.abs(a.astype(np.int32) - b.astype(np.int32)) <= 1).all(), (build, row, k)
