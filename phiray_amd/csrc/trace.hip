@@ -174,11 +174,21 @@ __device__ __forceinline__ bool visit1(const float4* __restrict__ r, V3 o, V3 in
     if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); return true; }
     return false;
 }
+// visit2_rows: the same visit from rows already loaded (the record's four,
+// then its eight kids rows).
+__device__ __forceinline__ bool visit2_rows(float4 alo, float4 ahi, float4 blo, float4 bhi, float4 a0lo, float4 a0hi, float4 a1lo,
+                                            float4 a1hi, float4 b0lo, float4 b0hi, float4 b1lo, float4 b1hi, V3 o, V3 inv, float best_t,
+                                            uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp,
+                                            uint32_t& steps);
 __device__ __forceinline__ bool visit2(const float4* __restrict__ r, const float4* __restrict__ k, V3 o, V3 inv, float best_t,
                                        uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp, uint32_t& steps) {
-    const float4 alo = r[0], ahi = r[1], blo = r[2], bhi = r[3];
-    const float4 a0lo = k[0], a0hi = k[1], a1lo = k[2], a1hi = k[3];
-    const float4 b0lo = k[4], b0hi = k[5], b1lo = k[6], b1hi = k[7];
+    return visit2_rows(r[0], r[1], r[2], r[3], k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], o, inv, best_t, cur, end, stk, stride, sp,
+                       steps);
+}
+__device__ __forceinline__ bool visit2_rows(float4 alo, float4 ahi, float4 blo, float4 bhi, float4 a0lo, float4 a0hi, float4 a1lo,
+                                            float4 a1hi, float4 b0lo, float4 b0hi, float4 b1lo, float4 b1hi, V3 o, V3 inv, float best_t,
+                                            uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp,
+                                            uint32_t& steps) {
     ++steps;
     const float ta = box_t(alo, ahi, o, inv, best_t);
     const float tb = box_t(blo, bhi, o, inv, best_t);
@@ -312,13 +322,42 @@ __device__ __forceinline__ void walker_begin(Walker& w, const SceneDev& s, V3 o,
     w.end = __float_as_uint(rhi.w);
     if (!(w.cur & PHR_LEAF)) w.cur = s.root;
 }
+// TWO: an inner node's unit is visit2's (two levels from the record and its
+// kids rows), else the binary step.
+template <bool TWO = false>
 __device__ __forceinline__ void walker_step(Walker& w, const SceneDev& s, uint32_t* stk, uint32_t stride, uint32_t& steps,
                                             uint32_t& tests) {
     const bool leaf = (w.cur & PHR_LEAF) != 0u;
     const float4* q = leaf ? s.tris + 3 * (size_t)(w.cur & ~PHR_LEAF) : s.recs + 4 * (size_t)w.cur;
     const float4 a = q[0], b = q[1], c = q[2];
-    float4 d = a;
-    if (!leaf) d = q[3];
+    float4 d = a, k0 = a, k1 = a, k2 = a, k3 = a, k4 = a, k5 = a, k6 = a, k7 = a;
+    if (!leaf) {
+        d = q[3];
+        if constexpr (TWO) {
+            const float4* kk = s.kids + 8 * (size_t)w.cur;
+            k0 = kk[0];
+            k1 = kk[1];
+            k2 = kk[2];
+            k3 = kk[3];
+            k4 = kk[4];
+            k5 = kk[5];
+            k6 = kk[6];
+            k7 = kk[7];
+        }
+    }
+    if constexpr (TWO) {
+        if (!leaf) {
+            if (visit2_rows(a, b, c, d, k0, k1, k2, k3, k4, k5, k6, k7, w.o, w.inv, w.t, w.cur, w.end, stk, stride, w.sp, steps)) return;
+            if (w.sp == 0) {
+                w.on = false;
+                return;
+            }
+            --w.sp;
+            w.cur = stk[w.sp * stride];
+            w.end = PHR_NONE;
+            return;
+        }
+    }
     if (leaf) {
         const uint32_t id = w.cur & ~PHR_LEAF;
         ++tests;
@@ -1623,7 +1662,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
             const uint32_t waiting = (uint32_t)__popcll(__ballot(!done && !w.on));
             const uint32_t working = (uint32_t)__popcll(__ballot(!done));
             if (waiting == working || 2u * waiting >= working) break;
-            if (w.on) walker_step(w, sc, stk, TRACE_THREADS, steps, tests);
+            if (w.on) walker_step<true>(w, sc, stk, TRACE_THREADS, steps, tests);
         }
         if (done || w.on) continue;
         bool next = false;
