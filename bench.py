@@ -687,7 +687,8 @@ def main() -> int:
         nbytes = (rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES) / args.steps
         achieved = nbytes / kernel_s / 1e9
         build = PATH_BUILDS[tracer.path_regen]
-        traffic, source = profiled_traffic("k_path_spheres_1080p_64spp_traffic.txt", spp, w, h, tracer.path_waves, build)
+        profile = "k_path_sw_spheres_1080p_64spp_traffic.txt" if build == "split" else "k_path_spheres_1080p_64spp_traffic.txt"
+        traffic, source = profiled_traffic(profile, spp, w, h, tracer.path_waves, build)
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
                             "algorithmic_bytes": nbytes,
@@ -700,7 +701,9 @@ def main() -> int:
             # L2 and the MALL, so the kernel waits on latency, not bandwidth.
             line["roofline"]["dram_achieved"] = traffic / kernel_s / 1e9
             line["roofline"]["dram_frac"] = traffic / kernel_s / 1e9 / HBM_PEAK_GBS
-            line["roofline"]["limiter"] = "memory latency (L2/MALL hits): frac counts the walk's logical bytes, dram_frac the counted DRAM bytes"
+            line["roofline"]["limiter"] = ("VALU issue of divergent waves (lanes a VALU instruction: 14.3 of 64 split, 11.0 nested; "
+                                           "VALU busy 0.77 / 0.73) and L2/MALL latency: frac counts the walk's logical bytes, "
+                                           "dram_frac the counted DRAM bytes")
         line["s_per_frame"] = per
         line["path_waves_per_simd"] = tracer.path_waves
         line["path_regen"] = tracer.path_regen
