@@ -150,7 +150,7 @@ def batch_render_rates(gpu: P.Gpu, scenes: int = 256, w: int = 256, h: int = 256
     cam = S.path_camera(w, h, eye=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), rr_from=3)
     cam["env"] = (1.0, 1.0, 1.0, 0.0)
     waves, grouped, _ = b.autotune(cam, w, h, 4, reps=1)
-    log(f"batch render: kernel at {waves} waves a SIMD, grouped order {grouped}")
+    log(f"batch render: kernel at {waves} waves a SIMD, grouped order {grouped}, {PATH_BUILDS[b.path_build]} build")
     build_s = sorted(b.rebuild() for _ in range(5))[2]
     b.render(cam, w, h, spp)
     c0 = b.counters()
@@ -165,7 +165,7 @@ def batch_render_rates(gpu: P.Gpu, scenes: int = 256, w: int = 256, h: int = 256
     nbytes = rays * (RAY_BYTES + HIT_BYTES + ROOT_BYTES) + steps * STEP_BYTES + tests * TEST_BYTES
     n = int(seg_lo[-1])
     return {"scenes": scenes, "tris": n, "image": f"{w}x{h}", "spp": spp, "bounces": 4, "light": "world colour 1.0",
-            "setup_s": setup_s, "path_waves_per_simd": waves, "grouped": grouped,
+            "setup_s": setup_s, "path_waves_per_simd": waves, "grouped": grouped, "path_build": PATH_BUILDS[b.path_build],
             "build_ms": build_s * 1e3, "build_mtri_s": n / build_s / 1e6,
             "render_ms": t_render * 1e3, "rays_per_pass": rays, "mrays_s": rays / t_render / 1e6,
             "render_gbs": nbytes / t_render / 1e9, "render_hbm_frac": nbytes / t_render / 1e9 / HBM_PEAK_GBS,

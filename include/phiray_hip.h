@@ -438,7 +438,13 @@ int phr_batch_render(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint
  * blocks of one XCD cover whole scenes; needs a multiple of 8 scenes). */
 int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped);
 int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped);
-/* Time every (waves, order) pair on this render and keep the fastest. */
+/* The render kernel's samples: 0 the nested build, 3 the split build (walks
+ * that span turns; used where the camera has lamps or a lit world and no
+ * Adaptive Sampling, else the nested build runs). No reference counterpart:
+ * phiray renders one scene a call. */
+int phr_batch_set_path_build(phr_batch* b, uint32_t build);
+uint32_t phr_batch_path_build(const phr_batch* b);
+/* Time every (build, waves, order) triple on this render and keep the fastest. */
 int phr_batch_autotune(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps, uint32_t* waves,
                        uint32_t* grouped, double* best_s);
 /* The whole pass (rebuild 1: build, then render) as one hipGraph; kernels

@@ -319,6 +319,8 @@ _SIGS = {
     "phr_batch_render": (_I, [_P, _P, _U32, _U32, _U32, C.POINTER(C.c_double)]),
     "phr_batch_set_path": (_I, [_P, _U32, _U32]),
     "phr_batch_path": (_I, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "phr_batch_set_path_build": (_I, [_P, _U32]),
+    "phr_batch_path_build": (_U32, [_P]),
     "phr_batch_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_batch_record": (_I, [_P, _P, _U32, _U32, _U32, _I]),
     "phr_batch_replay": (_I, [_P, C.POINTER(C.c_double)]),
@@ -1253,8 +1255,19 @@ class Batch:
         _check(lib().phr_batch_path(self._h, C.byref(wv), C.byref(g)), "Batch path")
         return int(wv.value), int(g.value)
 
+    def set_path_build(self, build: int) -> None:
+        """The render kernel's samples: Tracer.PATH_NESTED (0) or
+        Tracer.PATH_SPLIT (3; taken where the camera has lamps or a lit world
+        and no Adaptive Sampling, else the nested build runs)."""
+        _check(lib().phr_batch_set_path_build(self._h, int(build)), "Batch set_path_build")
+
+    @property
+    def path_build(self) -> int:
+        return int(lib().phr_batch_path_build(self._h))
+
     def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 1) -> tuple[int, int, float]:
-        """(waves a SIMD, grouped order, seconds) of the fastest render kernel."""
+        """(waves a SIMD, grouped order, seconds) of the fastest render kernel,
+        over the nested and split builds where the camera allows the split."""
         cam = np.ascontiguousarray(np.asarray(cam, dtype=CAMERA).reshape(1))
         wv, g, t = C.c_uint32(0), C.c_uint32(0), C.c_double(0)
         _check(lib().phr_batch_autotune(self._h, _ptr(cam), w, h, spp, reps, C.byref(wv), C.byref(g), C.byref(t)),

@@ -1612,14 +1612,12 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 // before the walk: a sample that ended waits with its radiance both ways; a
 // sample that goes on is advanced as lit and, if the ray was blocked, gets
 // back the two sums that differ (the camera hit's light, the bounces' sum).
-template <int WPE>
-__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw(
-    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-    unsigned long long* __restrict__ rays, uint32_t tile0) {
-    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+// path_split: tile `blk` of one image; s_stack: the block's stack columns.
+__device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* __restrict__ camp, uint32_t blk, uint32_t tiles_x,
+                                           uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+                                           unsigned long long* __restrict__ rays, uint32_t* s_stack) {
     const PhrCamera* p = camp;
     const size_t n = (size_t)p->w * p->h;
-    const uint32_t blk = blockIdx.x + tile0;
     const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
     const uint32_t pxl = tx * 16 + (threadIdx.x & 15u), pyl = ty * 16 + (threadIdx.x >> 4);
     const bool inside = pxl < p->w && pyl < p->h;
@@ -1759,6 +1757,13 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         atomicAdd(&rays[2], st_);
         atomicAdd(&rays[3], tt);
     }
+}
+template <int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    path_split(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack);
 }
 
 // ---- the wavefront build --------------------------------------------------------
@@ -2236,6 +2241,25 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     const SceneDev sc = scenes[scene];
     path_block<false>(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, 0u, s_stack, s_tstack);
 }
+// The same batch with the split build's samples (path_split): for cameras
+// with lamps or a lit world and no Adaptive Sampling.
+template <int WPE>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_batch_sw(
+    const SceneDev* __restrict__ scenes, const PhrCamera* __restrict__ cams, uint32_t tiles, uint32_t tiles_x, uint32_t spp,
+    uint32_t* __restrict__ px, uint64_t px_stride, unsigned long long* __restrict__ rays, uint32_t grouped) {
+    __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
+    uint32_t scene, tile;
+    if (grouped) {
+        const uint32_t x = blockIdx.x % 8u, q = blockIdx.x / 8u;
+        scene = x + 8u * (q / tiles);
+        tile = q % tiles;
+    } else {
+        scene = blockIdx.x / tiles;
+        tile = blockIdx.x % tiles;
+    }
+    const SceneDev sc = scenes[scene];
+    path_split(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, s_stack);
+}
 
 // The builds of the path kernels the tracer and the batch choose among.
 constexpr uint32_t PATH_WAVES_MIN = 2, PATH_WAVES_MAX = 5;
@@ -2293,10 +2317,21 @@ void launch_path(uint32_t waves, bool forest, uint32_t regen, dim3 grid, hipStre
         hipLaunchKernelGGL((k_path<false, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
 }
 
-void launch_path_batch(uint32_t waves, dim3 grid, hipStream_t s, const SceneDev* table, const PhrCamera* cams, uint32_t tiles,
-                       uint32_t tiles_x, uint32_t spp, uint32_t* px, uint64_t stride, unsigned long long* rays, uint32_t grouped) {
+// split: the split build (k_path_batch_sw), else the nested one.
+void launch_path_batch(uint32_t waves, bool split, dim3 grid, hipStream_t s, const SceneDev* table, const PhrCamera* cams,
+                       uint32_t tiles, uint32_t tiles_x, uint32_t spp, uint32_t* px, uint64_t stride, unsigned long long* rays,
+                       uint32_t grouped) {
     const dim3 block(TRACE_THREADS);
-    if (waves == 5)
+    if (split) {
+        if (waves == 5)
+            hipLaunchKernelGGL(k_path_batch_sw<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+        else if (waves == 4)
+            hipLaunchKernelGGL(k_path_batch_sw<4>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+        else if (waves == 3)
+            hipLaunchKernelGGL(k_path_batch_sw<3>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+        else
+            hipLaunchKernelGGL(k_path_batch_sw<2>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+    } else if (waves == 5)
         hipLaunchKernelGGL(k_path_batch<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
     else if (waves == 4)
         hipLaunchKernelGGL(k_path_batch<4>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
@@ -3289,7 +3324,9 @@ int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, ui
         {"k_path_sw<3>", (const void*)k_path_sw<3>}, {"k_path_sw<4>", (const void*)k_path_sw<4>},
         {"k_path_sw<5>", (const void*)k_path_sw<5>}, {"k_path_batch<2>", (const void*)k_path_batch<2>},
         {"k_path_batch<3>", (const void*)k_path_batch<3>}, {"k_path_batch<4>", (const void*)k_path_batch<4>},
-        {"k_path_batch<5>", (const void*)k_path_batch<5>},
+        {"k_path_batch<5>", (const void*)k_path_batch<5>}, {"k_path_batch_sw<2>", (const void*)k_path_batch_sw<2>},
+        {"k_path_batch_sw<3>", (const void*)k_path_batch_sw<3>}, {"k_path_batch_sw<4>", (const void*)k_path_batch_sw<4>},
+        {"k_path_batch_sw<5>", (const void*)k_path_batch_sw<5>},
     };
     uint32_t n = 0;
     for (const K& k : ks) {
@@ -4342,6 +4379,8 @@ struct phr_batch {
     unsigned long long* rays = nullptr;  // closest, shadow, inner steps, triangle tests
     uint32_t w = 0, h = 0;
     uint32_t path_waves = 2, grouped = 0;
+    uint32_t path_build = 0;      // 0 nested, 3 split (where the camera allows it)
+    bool split_ok = false;        // the last camera allows the split build
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_gpu_s = 0.0;
     hipGraph_t graph = nullptr;
@@ -4384,6 +4423,7 @@ int batch_enqueue_build(phr_batch* b) {
 
 // Each scene's camera as a tracer of that scene alone stamps it.
 int batch_cameras(phr_batch* b, const PhrCamera& cam, uint32_t w, uint32_t h) {
+    b->split_ok = regen_allowed(cam);
     std::vector<PhrCamera> c(b->scenes, cam);
     for (uint32_t k = 0; k < b->scenes; ++k) {
         c[k].w = w;
@@ -4417,11 +4457,12 @@ int batch_image(phr_batch* b, uint32_t w, uint32_t h) {
     return PHR_OK;
 }
 
-void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t waves, uint32_t grouped) {
+void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t waves, uint32_t grouped, uint32_t build) {
     const uint32_t tiles_x = (b->w + 15) / 16, tiles = tiles_x * ((b->h + 15) / 16);
     const dim3 grid(tiles * b->scenes);
     const uint64_t stride = PX_LEN * (uint64_t)b->w * b->h + 1;
-    launch_path_batch(waves, grid, b->gpu->stream, b->table, cams, tiles, tiles_x, spp, (uint32_t*)b->px->ptr, stride, b->rays, grouped);
+    launch_path_batch(waves, build == 3u && b->split_ok, grid, b->gpu->stream, b->table, cams, tiles, tiles_x, spp, (uint32_t*)b->px->ptr,
+                      stride, b->rays, grouped);
 }
 
 int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
@@ -4624,7 +4665,7 @@ int phr_batch_render(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, ui
         return rc;
     hipStream_t s = b->gpu->stream;
     PHR_HIP(hipEventRecord(b->ev0, s));
-    batch_launch(b, b->cams, spp, b->path_waves, b->grouped);
+    batch_launch(b, b->cams, spp, b->path_waves, b->grouped, b->path_build);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(b->ev1, s));
     PHR_HIP(hipEventSynchronize(b->ev1));
@@ -4645,6 +4686,15 @@ int phr_batch_set_path(phr_batch* b, uint32_t waves, uint32_t grouped) {
     b->grouped = grouped;
     return PHR_OK;
 }
+// The render kernel's samples: 0 the nested build, 3 the split build (taken
+// only where the camera allows it: lamps or a lit world, no Adaptive
+// Sampling; other renders take the nested build).
+int phr_batch_set_path_build(phr_batch* b, uint32_t build) {
+    PHR_CHECK(b && (build == 0u || build == 3u), PHR_ERR_ARG, "phr_batch_set_path_build: null, or not 0 or 3");
+    b->path_build = build;
+    return PHR_OK;
+}
+uint32_t phr_batch_path_build(const phr_batch* b) { return b ? b->path_build : 0u; }
 int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped) {
     PHR_CHECK(b, PHR_ERR_ARG, "phr_batch_path: null");
     if (waves) *waves = b->path_waves;
@@ -4668,22 +4718,26 @@ int phr_batch_autotune(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, 
         return rc;
     hipStream_t s = b->gpu->stream;
     double best = 1e30;
-    uint32_t bw = b->path_waves, bg = b->grouped;
-    for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
-        for (uint32_t wv = PATH_WAVES_MIN; wv <= PATH_WAVES_MAX; ++wv) {
-            batch_launch(b, b->cams, spp, wv, g);
-            PHR_HIP(hipEventRecord(b->ev0, s));
-            for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g);
-            PHR_HIP(hipGetLastError());
-            PHR_HIP(hipEventRecord(b->ev1, s));
-            PHR_HIP(hipEventSynchronize(b->ev1));
-            float ms = 0.f;
-            PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            const double t = ms * 1e-3 / reps;
-            if (t < best) best = t, bw = wv, bg = g;
-        }
+    uint32_t bw = b->path_waves, bg = b->grouped, bb = b->path_build;
+    for (uint32_t build : {0u, 3u}) {
+        if (build == 3u && !b->split_ok) continue;
+        for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
+            for (uint32_t wv = PATH_WAVES_MIN; wv <= PATH_WAVES_MAX; ++wv) {
+                batch_launch(b, b->cams, spp, wv, g, build);
+                PHR_HIP(hipEventRecord(b->ev0, s));
+                for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g, build);
+                PHR_HIP(hipGetLastError());
+                PHR_HIP(hipEventRecord(b->ev1, s));
+                PHR_HIP(hipEventSynchronize(b->ev1));
+                float ms = 0.f;
+                PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                const double t = ms * 1e-3 / reps;
+                if (t < best) best = t, bw = wv, bg = g, bb = build;
+            }
+    }
     b->path_waves = bw;
     b->grouped = bg;
+    b->path_build = bb;
     if (waves) *waves = bw;
     if (grouped) *grouped = bg;
     if (best_s) *best_s = best;
@@ -4711,7 +4765,7 @@ int phr_batch_record(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, ui
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     if (rebuild) rc = batch_enqueue_build(b);
-    if (rc == PHR_OK) batch_launch(b, b->gcams, spp, b->path_waves, b->grouped);
+    if (rc == PHR_OK) batch_launch(b, b->gcams, spp, b->path_waves, b->grouped, b->path_build);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (rc != PHR_OK) {

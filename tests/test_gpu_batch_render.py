@@ -73,13 +73,19 @@ def test_every_scene_renders_as_its_own_tracer(batch):
     print("[batch render] scenes against tracers", flush=True)
     parts, b, cam, want = batch
     assert b.scenes == 8 and len(b) == sum(len(p) for p in parts)
-    for waves, grouped in [(2, 0), (2, 1), (3, 1), (4, 0)]:
-        b.set_path(waves, grouped)
-        c0 = b.counters()
-        assert b.render(cam, W, H, SPP) > 0
-        for k in range(b.scenes):
-            assert np.array_equal(b.px(k), want[k]), (waves, grouped, k)
-        assert b.counters()[0] > c0[0]
+    for build in (P.Tracer.PATH_NESTED, P.Tracer.PATH_SPLIT):
+        b.set_path_build(build)
+        assert b.path_build == build
+        for waves, grouped in [(2, 0), (2, 1), (3, 1), (4, 0)]:
+            b.set_path(waves, grouped)
+            c0 = b.counters()
+            assert b.render(cam, W, H, SPP) > 0
+            for k in range(b.scenes):
+                assert np.array_equal(b.px(k), want[k]), (build, waves, grouped, k)
+            assert b.counters()[0] > c0[0]
+    b.set_path_build(P.Tracer.PATH_NESTED)
+    with pytest.raises(P.PhrError):
+        b.set_path_build(1)
 
 
 def test_a_scene_matches_the_oracle(batch, orc):
@@ -102,6 +108,7 @@ def test_recorded_pass_replays_equal_eager(batch):
     print("[batch render] recorded pass", flush=True)
     _, b, cam, want = batch
     b.set_path(3, 0)
+    b.set_path_build(P.Tracer.PATH_SPLIT)
     b.record(cam, W, H, SPP, rebuild=True)
     other = S.path_camera(W, H, rr_from=3, eye=(1.5, -5.0, 3.0))
     for _ in range(3):
@@ -112,6 +119,7 @@ def test_recorded_pass_replays_equal_eager(batch):
     b.render(cam, W + 16, H, 1)  # new image size: the graph is refused
     with pytest.raises(P.PhrError):
         b.replay()
+    b.set_path_build(P.Tracer.PATH_NESTED)
 
 
 def test_rebuild_and_autotune_keep_the_images(batch):
@@ -120,7 +128,7 @@ def test_rebuild_and_autotune_keep_the_images(batch):
     assert b.rebuild() > 0
     waves, grouped, best = b.autotune(cam, W, H, SPP, reps=1)
     assert waves in (2, 3, 4, 5) and grouped in (0, 1) and best > 0
-    assert b.path() == (waves, grouped)
+    assert b.path() == (waves, grouped) and b.path_build in (P.Tracer.PATH_NESTED, P.Tracer.PATH_SPLIT)
     b.render(cam, W, H, SPP)
     for k in range(b.scenes):
         assert np.array_equal(b.px(k), want[k]), k
@@ -146,8 +154,9 @@ def test_bad_batches_are_refused(gpu):
 
 def test_full_batch_scenes_match_their_tracers(gpu):
     """The full batch workload: 256 scenes, 9,402,507 triangles in one build
-    and one launch (2 of the 16 samples, under the world colour). Four
-    scenes across the batch, the last included, equal their own tracers."""
+    and one launch (2 of the 16 samples, under the world colour), with the
+    nested and the split build. Four scenes across the batch, the last
+    included, equal their own tracers."""
     print("[batch render] 256-scene workload", flush=True)
     tris, lo = S.batch_tris(256)
     assert int(lo[-1]) == 9_402_507
@@ -157,12 +166,15 @@ def test_full_batch_scenes_match_their_tracers(gpu):
     cam["env"] = (1.0, 1.0, 1.0, 0.0)
     b.set_path(4, 1)
     b.render(cam, w, h, 2)
+    nested = {k: b.px(k).copy() for k in (0, 97, 200, 255)}
+    b.set_path_build(P.Tracer.PATH_SPLIT)
+    b.render(cam, w, h, 2)
     for k in (0, 97, 200, 255):
         tr = P.Tracer.new(gpu, tris[lo[k]:lo[k + 1]], 4)
         tr.sample(cam, w, h, 2)
         want = tr.px()
-        got = b.px(k)
-        assert np.array_equal(got, want), k
+        assert np.array_equal(nested[k], want), ("nested", k)
+        assert np.array_equal(b.px(k), want), ("split", k)
         assert P.planes(want, w, h)["radiance"].max() > 0
         tr.free()
     b.free()
