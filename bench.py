@@ -213,10 +213,19 @@ def forest_rates(gpu: P.Gpu, substeps: int = 8, w: int = 1280, h: int = 720, spp
     t_frame, _ = clock.stop()
     c1 = tr.counters()
     rays = int(c1[0] - c0[0] + c1[1] - c0[1])
+    chosen = tr.path_regen
+    by_build = {}
+    for build in (0, 3):  # each build the forest allows, at the chosen register fit
+        tr.set_path_regen(build)
+        tr.sample(S.path_camera(64, 64), 64, 64, 1)  # another view: the next call starts over
+        tr.sample(cam, w, h, spp)
+        by_build[PATH_BUILDS[build]] = tr.time()[1] * 1e3
+    tr.set_path_regen(chosen)
     return {"bodies": len(worlds), "meshes": len(me), "tris": len(f), "build_s": build_s,
             "substep_ms": wall_s / substeps * 1e3, "substep_gpu_span_ms": g_s / substeps * 1e3,
             "frame": f"{w}x{h}, {spp} spp", "frame_ms": t_frame * 1e3, "mrays_s": rays / t_frame / 1e6,
-            "rays_per_frame": rays, "path_waves_per_simd": waves, "mesh_stack_entries": tr.path_stack}
+            "rays_per_frame": rays, "path_waves_per_simd": waves, "path_build": PATH_BUILDS[chosen],
+            "frame_ms_by_build": by_build, "mesh_stack_entries": tr.path_stack}
 
 
 def xpbd_rates(gpu: P.Gpu, frames: int = 120, substeps: int = 8, w: int = 1280, h: int = 720, spp: int = 4) -> dict:
@@ -270,7 +279,7 @@ def xpbd_rates(gpu: P.Gpu, frames: int = 120, substeps: int = 8, w: int = 1280, 
             "physics_ms_per_substep": phys / n * 1e3, "scene_ms_per_substep": scene / n * 1e3,
             "trace_ms_per_frame": trace / frames * 1e3, "mrays_s": rays / trace / 1e6,
             "pairs_per_substep": pairs / n, "contacts_last": world.contact_count(),
-            "mean_height_m": [z0, z1]}
+            "mean_height_m": [z0, z1], "path_waves_per_simd": tr.path_waves, "path_build": PATH_BUILDS[tr.path_regen]}
 
 
 def rel_mse(x: np.ndarray, y: np.ndarray) -> float:

@@ -805,8 +805,8 @@ template <uint32_t CW>
 __device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t n, uint64_t s0, uint64_t s1,
                                       const uint4* __restrict__ up, float4* nodes, Arrivals<CW>& arr, uint32_t* s_fcount,
                                       uint32_t* front_out) {
+    uint4 u = up[x];
     for (;;) {
-        const uint4 u = up[x];
         const uint32_t p = u.x;
         if (p == PHR_NONE) return;  // the root is done
         if (u.y < s0 || u.z >= s1) {
@@ -814,6 +814,7 @@ __device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t 
             front_out[at] = x;
             return;
         }
+        const uint4 un = up[p];  // the parent's record, on its way while this step finishes
         if (!arr.second(p)) return;  // the sibling's climb finishes p
         const uint32_t sib = u.w & ~UP_LEFT;
         const uint32_t so = out_of(sib, n, g), xo = out_of(x, n, g);
@@ -822,6 +823,7 @@ __device__ __forceinline__ void climb(uint32_t x, Box b, const Seg& g, uint32_t 
         const bool x_left = (u.w & UP_LEFT) != 0u;
         store_node(nodes, out_of(p, n, g), b, x_left ? xo : so, x_left ? so : xo);
         x = p;
+        u = un;
     }
 }
 

@@ -573,6 +573,146 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint16_
     return best;
 }
 
+// walk_forest as a walk that can stop after any unit and go on later (the
+// split build over a forest): a unit is one pass of walk_forest's loop (a
+// node of either tree, or a top-tree leaf's instances entered in turn), with
+// a mesh leaf's triangles one a unit, so the visits, decisions and pushes
+// are walk_forest's in its order. `any`: a shadow ray's walk.
+struct ForestWalker {
+    V3 o, d, winv;  // the world ray
+    V3 ro, rinv;    // the ray the current level walks with
+    RaySpace rs;
+    float t, u, v;  // closest: the nearest hit so far; shadow: t is the ray's length
+    uint32_t tri, hinst;  // the hit's face and instance
+    uint32_t li, ri, tsp, bsp, inst, k, kend;
+    bool on, any, in_mesh;
+};
+__device__ __forceinline__ void forest_begin(ForestWalker& w, const SceneDev& s, V3 o, V3 d, float tmax, bool any) {
+    w.o = o;
+    w.d = d;
+    w.winv = safe_inv(d);
+    w.ro = o;
+    w.rinv = w.winv;
+    w.rs = RaySpace{0u, false, 0.f, 0.f, 0.f};
+    w.t = tmax;
+    w.u = w.v = 0.f;
+    w.tri = PHR_NONE;
+    w.hinst = 0u;
+    w.tsp = w.bsp = w.inst = w.k = w.kend = 0u;
+    w.any = any;
+    w.in_mesh = false;
+    const float4 lo = s.nodes[0], hi = s.nodes[1];
+    w.on = box_t(lo, hi, o, w.winv, tmax) < PHR_INF;
+    w.li = __float_as_uint(lo.w);
+    w.ri = __float_as_uint(hi.w);
+}
+__device__ __forceinline__ void forest_step(ForestWalker& w, const SceneDev& s, uint16_t* tstk, uint32_t* bstk, uint32_t stride,
+                                            uint32_t& steps, uint32_t& tests) {
+    bool pop = false;
+    if (!(w.li & PHR_LEAF)) {
+        ++steps;
+        const float4 llo = s.nodes[2 * w.li], lhi = s.nodes[2 * w.li + 1];
+        const float4 rlo = s.nodes[2 * w.ri], rhi = s.nodes[2 * w.ri + 1];
+        const float tl = box_t(llo, lhi, w.ro, w.rinv, w.t);
+        const float tr = box_t(rlo, rhi, w.ro, w.rinv, w.t);
+        if (tl < PHR_INF && tr < PHR_INF) {
+            const uint32_t far = tl <= tr ? w.ri : w.li;
+            if (w.in_mesh) bstk[w.bsp++ * stride] = far;
+            else tstk[w.tsp++ * stride] = (uint16_t)far;
+            const float4 nl = tl <= tr ? llo : rlo, nh = tl <= tr ? lhi : rhi;
+            w.li = __float_as_uint(nl.w);
+            w.ri = __float_as_uint(nh.w);
+            return;
+        }
+        if (tl < PHR_INF) { w.li = __float_as_uint(llo.w); w.ri = __float_as_uint(lhi.w); return; }
+        if (tr < PHR_INF) { w.li = __float_as_uint(rlo.w); w.ri = __float_as_uint(rhi.w); return; }
+        pop = true;
+    } else if (w.in_mesh) {  // one triangle of the mesh leaf [li, ri)
+        const uint32_t id = w.li & ~PHR_LEAF;
+        if (id < w.ri) {
+            ++tests;
+            const float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
+            float u, v;
+            const float t = tri_t(t0, t1, t2, w.ro, w.rs, u, v);
+            if (t < w.t) {
+                if (w.any) {
+                    const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
+                    if (!(m[6] > 0.5f || m[3] < 0.5f)) {
+                        w.tri = id;
+                        w.hinst = w.inst;
+                        w.on = false;
+                        return;
+                    }
+                } else {
+                    w.t = t;
+                    w.tri = id;
+                    w.u = u;
+                    w.v = v;
+                    w.hinst = w.inst;
+                }
+            }
+            if (id + 1u < w.ri) {
+                w.li = (id + 1u) | PHR_LEAF;
+                return;
+            }
+        }
+        pop = true;
+    } else {  // a top-tree leaf: its instances, in slot order
+        w.k = w.li & ~(PHR_LEAF | PHR_INST);
+        w.kend = w.ri;
+    }
+    if (pop && w.in_mesh && w.bsp > 0) {
+        const uint32_t x = bstk[--w.bsp * stride];
+        w.li = __float_as_uint(s.nodes[2 * x].w);
+        w.ri = __float_as_uint(s.nodes[2 * x + 1].w);
+        return;
+    }
+    if (pop && w.in_mesh) {
+        w.in_mesh = false;
+        ++w.k;
+    }
+    const uint32_t* inst_order = s.insts + INSTANCE_LEN * (size_t)s.ninsts;
+    while (!w.in_mesh && w.k < w.kend) {
+        const uint32_t i = inst_order[w.k];
+        const uint32_t* iw = s.insts + INSTANCE_LEN * (size_t)i;
+        const uint4 info = *(const uint4*)(iw + 24);
+        const uint32_t root = info.x, flags = info.w;
+        if (root != PHR_NONE && !(w.any && (flags & 2u))) {
+            V3 mo = w.o, md = w.d;
+            if ((flags & 1u) == 0u) {
+                const float4 r0 = *(const float4*)(iw + 12), r1 = *(const float4*)(iw + 16), r2 = *(const float4*)(iw + 20);
+                mo = v3(dot4(r0, w.o.x, w.o.y, w.o.z, 1.0f), dot4(r1, w.o.x, w.o.y, w.o.z, 1.0f), dot4(r2, w.o.x, w.o.y, w.o.z, 1.0f));
+                md = v3((r0.x * w.d.x + r0.y * w.d.y) + r0.z * w.d.z, (r1.x * w.d.x + r1.y * w.d.y) + r1.z * w.d.z,
+                        (r2.x * w.d.x + r2.y * w.d.y) + r2.z * w.d.z);
+            }
+            const V3 minv = safe_inv(md);
+            const float4 rlo = s.nodes[2 * root], rhi = s.nodes[2 * root + 1];
+            if (box_t(rlo, rhi, mo, minv, w.t) < PHR_INF) {
+                w.in_mesh = true;
+                w.inst = i;
+                w.ro = mo;
+                w.rinv = minv;
+                w.rs = ray_space(md);
+                w.bsp = 0;
+                w.li = __float_as_uint(rlo.w);
+                w.ri = __float_as_uint(rhi.w);
+                break;
+            }
+        }
+        ++w.k;
+    }
+    if (w.in_mesh) return;
+    w.ro = w.o;
+    w.rinv = w.winv;
+    if (w.tsp == 0) {
+        w.on = false;
+        return;
+    }
+    const uint32_t x = tstk[--w.tsp * stride];
+    w.li = __float_as_uint(s.nodes[2 * x].w);
+    w.ri = __float_as_uint(s.nodes[2 * x + 1].w);
+}
+
 constexpr uint32_t TRACE_THREADS = 256;
 // LDS stacks a thread: PHR_STACK words for one tree or a mesh tree, and for a
 // forest PHR_STACK 16-bit entries for the top tree.
@@ -1285,7 +1425,10 @@ __device__ __forceinline__ void rg_start(RgSample& st, RgGuide& g, const PhrCame
 }
 
 // The turn's closest hit h: shading up to the shadow ray (shade.wgsl:96-302
-// for a camera segment, integrate.wgsl:124-228 for a bounce).
+// for a camera segment, integrate.wgsl:124-228 for a bounce). F: the scene
+// is a forest (faces placed by their instance, a mirroring instance's
+// camera-segment normal turned back, as shade<true> does).
+template <bool F = false>
 __device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, const SceneDev& sc, const PhrCamera* p, const RgConst& k,
                                        const PhrHit& h) {
     const V3 one = v3(1, 1, 1);  // shade's throughput, never changed
@@ -1314,10 +1457,12 @@ __device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, cons
         }
         if (st.pass == 0u) g.outline = h.inst + 1u;
         const uint32_t m = mat_at(sc, h.tri);
-        const Tri t = tri_of<false>(sc, h.tri, h.inst, m);
+        const Tri t = tri_of<F>(sc, h.tri, h.inst, m);
         const float w = 1.0f - h.u - h.v;
         const V3 hit = st.o + st.d * h.t;
-        const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
+        if constexpr (F)
+            if (sc.insts[INSTANCE_LEN * (size_t)h.inst + 27] & 4u) ng0 = -ng0;
         const bool front = dot(ng0, st.d) < 0.f;
         const V3 ng = front ? ng0 : -ng0;
         V3 sn = t.na * w + t.nb * h.u + t.nc * h.v;
@@ -1354,7 +1499,7 @@ __device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, cons
         // direct (integrate.wgsl:66-85) without the rig: the lamp pick's
         // number, then one lamp.
         (void)rnd(&st.seed);
-        const Shadowed sh = glow_query<false>(sc, sf, &st.seed);
+        const Shadowed sh = glow_query<F>(sc, sf, &st.seed);
         if (maxc(sh.c) > 0.f) {
             q.qon = q.qdirect = true;
             q.qpos = sf.pos;
@@ -1401,7 +1546,7 @@ __device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, cons
         q.end = 4u;
         return;
     }
-    const Tri t = tri_of<false>(sc, h.tri, h.inst, mat_at(sc, h.tri));
+    const Tri t = tri_of<F>(sc, h.tri, h.inst, mat_at(sc, h.tri));
     const V3 ng0 = normalize(cross(t.b - t.a, t.c - t.a));
     const V3 ng = dot(ng0, st.d) < 0.f ? ng0 : -ng0;
     const V3 hit = st.o + st.d * h.t + ng * (p->eps + h.t * 1e-5f);
@@ -1417,7 +1562,7 @@ __device__ __forceinline__ void rg_hit(RgSample& st, RgGuide& g, RgTurn& q, cons
     q.near = (!k.lit && h.t <= p->ao) ? 0.25f : 1.0f;
     const Surf sf{hit, ng, -st.d, albedo, mixv(fd, base, metal), rough, 0.0f, 0.0f};
     (void)rnd(&st.seed);
-    const Shadowed sh = glow_query<false>(sc, sf, &st.seed);
+    const Shadowed sh = glow_query<F>(sc, sf, &st.seed);
     if (maxc(sh.c) > 0.f) {
         q.qon = q.qdirect = true;
         q.qpos = sf.pos;
@@ -1612,10 +1757,32 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 // before the walk: a sample that ended waits with its radiance both ways; a
 // sample that goes on is advanced as lit and, if the ray was blocked, gets
 // back the two sums that differ (the camera hit's light, the bounces' sum).
-// path_split: tile `blk` of one image; s_stack: the block's stack columns.
+// The split build's walk of one tree, or of a forest (F).
+template <bool F>
+struct SplitWalk {
+    Walker w;
+    __device__ __forceinline__ void begin(const SceneDev& s, V3 o, V3 d, float tmax, bool any) { walker_begin(w, s, o, d, tmax, any); }
+    __device__ __forceinline__ void step(const SceneDev& s, uint32_t* stk, uint16_t*, uint32_t& steps, uint32_t& tests) {
+        walker_step<true>(w, s, stk, TRACE_THREADS, steps, tests);
+    }
+    __device__ __forceinline__ PhrHit hit() const { return PhrHit{w.t, w.tri, w.u, w.v, 0u}; }
+};
+template <>
+struct SplitWalk<true> {
+    ForestWalker w;
+    __device__ __forceinline__ void begin(const SceneDev& s, V3 o, V3 d, float tmax, bool any) { forest_begin(w, s, o, d, tmax, any); }
+    __device__ __forceinline__ void step(const SceneDev& s, uint32_t* stk, uint16_t* tstk, uint32_t& steps, uint32_t& tests) {
+        forest_step(w, s, tstk, stk, TRACE_THREADS, steps, tests);
+    }
+    __device__ __forceinline__ PhrHit hit() const { return PhrHit{w.t, w.tri, w.u, w.v, w.hinst}; }
+};
+
+// path_split: tile `blk` of one image. s_stack: the block's stack columns
+// (a forest's mesh trees'); s_tstack: a forest's top-tree columns.
+template <bool F = false>
 __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* __restrict__ camp, uint32_t blk, uint32_t tiles_x,
                                            uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                           unsigned long long* __restrict__ rays, uint32_t* s_stack) {
+                                           unsigned long long* __restrict__ rays, uint32_t* s_stack, uint16_t* s_tstack) {
     const PhrCamera* p = camp;
     const size_t n = (size_t)p->w * p->h;
     const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
@@ -1623,6 +1790,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
     const bool inside = pxl < p->w && pyl < p->h;
     const uint32_t i = pyl * p->w + pxl;
     uint32_t* stk = s_stack + threadIdx.x;
+    uint16_t* tstk = F ? s_tstack + threadIdx.x : nullptr;
     uint32_t closest = 0, shadow = 0, steps = 0, tests = 0;
     const bool keep = frame0 > 0u;
     V3 l = v3(0, 0, 0), ga = l, gn = l;
@@ -1645,14 +1813,15 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
     st.p_diffuse = 1.0f;
     st.min_pdf = 3.0e38f;
     RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
-    Walker w;
+    SplitWalk<F> sw;
+    auto& w = sw.w;
     w.on = w.any = false;
     bool done = !inside || spp == 0u, pfinal = false;
     V3 pa = v3(0, 0, 0), pb = pa;  // what a shadow walk's answer picks between
     uint32_t s = 0;
     if (!done) {
         rg_start(st, g, p, i, frame0);
-        walker_begin(w, sc, st.o, st.d, PHR_INF, false);
+        sw.begin(sc, st.o, st.d, PHR_INF, false);
         ++closest;
     }
     while (__ballot(!done) != 0ull) {
@@ -1660,7 +1829,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
             const uint32_t waiting = (uint32_t)__popcll(__ballot(!done && !w.on));
             const uint32_t working = (uint32_t)__popcll(__ballot(!done));
             if (waiting == working || 2u * waiting >= working) break;
-            if (w.on) walker_step<true>(w, sc, stk, TRACE_THREADS, steps, tests);
+            if (w.on) sw.step(sc, stk, tstk, steps, tests);
         }
         if (done || w.on) continue;
         bool next = false;
@@ -1674,9 +1843,9 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
             }
             next = true;
         } else {  // the closest hit: the segment's shading
-            const PhrHit h{w.t, w.tri, w.u, w.v, 0u};
+            const PhrHit h = sw.hit();
             RgTurn q;
-            rg_hit(st, g, q, sc, p, k, h);
+            rg_hit<F>(st, g, q, sc, p, k, h);
             bool ended;
             if (q.qon) {
                 RgPost lit{st.col, st.cc, st.acc, st.phase}, dark = lit;
@@ -1694,7 +1863,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
                     pa = dark.cc;
                     pb = dark.acc;
                 }
-                walker_begin(w, sc, q.qpos, q.qdir, q.qdist, true);
+                sw.begin(sc, q.qpos, q.qdir, q.qdist, true);
                 ++shadow;
             } else {
                 ended = rg_post(st, q, p, 1.0f);
@@ -1715,7 +1884,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
         if (next) {
             if (s < spp) {
                 if (st.phase == 0u) rg_start(st, g, p, i, frame0 + s);
-                walker_begin(w, sc, st.o, st.d, PHR_INF, false);
+                sw.begin(sc, st.o, st.d, PHR_INF, false);
                 ++closest;
             } else {
                 done = true;
@@ -1763,7 +1932,17 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
     unsigned long long* __restrict__ rays, uint32_t tile0) {
     __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
-    path_split(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack);
+    path_split(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, nullptr);
+}
+// The split build over a forest: MSTK mesh-tree entries a thread (the
+// caller checked the mesh trees' depth when it is below PHR_STACK).
+template <int WPE, uint32_t MSTK>
+__global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw_forest(
+    SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
+    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    __shared__ uint32_t s_stack[MSTK * TRACE_THREADS];
+    __shared__ uint16_t s_tstack[PHR_STACK * TRACE_THREADS];
+    path_split<true>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, s_tstack);
 }
 
 // ---- the wavefront build --------------------------------------------------------
@@ -2258,7 +2437,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         tile = blockIdx.x % tiles;
     }
     const SceneDev sc = scenes[scene];
-    path_split(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, s_stack);
+    path_split(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, s_stack, nullptr);
 }
 
 // The builds of the path kernels the tracer and the batch choose among.
@@ -2290,6 +2469,15 @@ void launch_path(uint32_t waves, bool forest, uint32_t regen, dim3 grid, hipStre
             hipLaunchKernelGGL(k_path_sw<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
         else
             hipLaunchKernelGGL(k_path_sw<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+    } else if (regen == 3u && forest && !listed) {  // the caller checked the mesh trees' depth for the short stacks
+        if (waves >= 4)
+            hipLaunchKernelGGL((k_path_sw_forest<4, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 3 && short_stack)
+            hipLaunchKernelGGL((k_path_sw_forest<3, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else if (waves == 3)
+            hipLaunchKernelGGL((k_path_sw_forest<3, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+        else
+            hipLaunchKernelGGL((k_path_sw_forest<2, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
     } else if (regen && !forest && !listed) {
         if (waves == 5)
             hipLaunchKernelGGL(k_path_rg<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
@@ -3326,7 +3514,10 @@ int phr_tracer_warm(phr_gpu* g, uint32_t cap, const char** names, double* ms, ui
         {"k_path_batch<3>", (const void*)k_path_batch<3>}, {"k_path_batch<4>", (const void*)k_path_batch<4>},
         {"k_path_batch<5>", (const void*)k_path_batch<5>}, {"k_path_batch_sw<2>", (const void*)k_path_batch_sw<2>},
         {"k_path_batch_sw<3>", (const void*)k_path_batch_sw<3>}, {"k_path_batch_sw<4>", (const void*)k_path_batch_sw<4>},
-        {"k_path_batch_sw<5>", (const void*)k_path_batch_sw<5>},
+        {"k_path_batch_sw<5>", (const void*)k_path_batch_sw<5>}, {"k_path_sw_forest<2>", (const void*)k_path_sw_forest<2, PHR_STACK>},
+        {"k_path_sw_forest<3>", (const void*)k_path_sw_forest<3, PHR_STACK>},
+        {"k_path_sw_forest<3, short stack>", (const void*)k_path_sw_forest<3, FOREST_SHORT_STACK>},
+        {"k_path_sw_forest<4>", (const void*)k_path_sw_forest<4, FOREST_SHORT_STACK>},
     };
     uint32_t n = 0;
     for (const K& k : ks) {
@@ -3599,13 +3790,23 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     double best = 1e30;
     PhrCamera probe;
     std::memcpy(&probe, cam, sizeof probe);
-    const uint32_t regens = (!t->forest && regen_allowed(probe)) ? 4u : 1u;
+    // The builds this camera and scene allow: nested always; the split build
+    // where the camera allows the segment shading; regenerating and
+    // wavefront for one tree only.
+    uint32_t builds[4] = {0u, 0u, 0u, 0u}, nbuilds = 1;
+    if (regen_allowed(probe)) {
+        if (!t->forest) {
+            builds[nbuilds++] = 1u;
+            builds[nbuilds++] = 2u;
+        }
+        builds[nbuilds++] = 3u;
+    }
     const bool fits_short = t->forest && t->mesh_depth <= FOREST_SHORT_STACK;
     bool best_short = false;
-    for (uint32_t regen = 0; regen < regens; ++regen)
+    for (uint32_t bi = 0; bi < nbuilds; ++bi)
         // The wavefront build's kernels have one register fit: timed once,
         // keeping the waves the others chose for its recorded-frame stand-in.
-        for (uint32_t waves = regen == 2u ? best_w : PATH_WAVES_MIN;
+        for (uint32_t regen = builds[bi], waves = regen == 2u ? best_w : PATH_WAVES_MIN;
              waves <= (regen == 2u ? best_w : (t->forest ? (fits_short ? 4u : 3u) : PATH_WAVES_MAX)); ++waves)
             for (uint32_t stk = 0; stk < ((fits_short && waves == 3) ? 2u : 1u); ++stk) {
                 t->path_waves = waves;

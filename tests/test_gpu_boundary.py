@@ -93,7 +93,7 @@ def test_warm_loads_every_path_build(gpu):
     print("[boundary] warm", flush=True)
     first = P.Tracer.warm(gpu)
     names = [n for n, _ in first]
-    assert len(names) == len(set(names)) == 24 and "k_path<4>" in names and "k_path_sw<4>" in names and "k_path_batch_sw<5>" in names
+    assert len(names) == len(set(names)) == 28 and "k_path<4>" in names and "k_path_sw<4>" in names and "k_path_batch_sw<5>" in names
     assert all(ms >= 0.0 for _, ms in first)
     again = P.Tracer.warm(gpu)
     assert [n for n, _ in again] == names

@@ -44,12 +44,18 @@ def test_the_forest_matches_the_oracle(bodies):
 
 
 def test_a_frame_matches_the_twin_on_three_bands(bodies):
-    print("[config4] 1280x720 frame against the oracle's bands", flush=True)
+    print("[config4] 1280x720 frame against the oracle's bands, nested and split builds", flush=True)
     arrs, _, t, fo = bodies
     cam = S.path_camera(W, H, eye=(0.0, -16.0, 9.0), target=(0.0, 0.0, 1.5), rr_from=3)
     t.set_path_waves(4)
     t.sample(cam, W, H, 1)
     got = P.planes(t.px(), W, H)
+    nested_px = t.px().copy()
+    t.set_path_regen(P.Tracer.PATH_SPLIT)  # the split build: the same bits
+    t.sample(S.path_camera(64, 64), 64, 64, 1)
+    t.sample(cam, W, H, 1)
+    assert np.array_equal(t.px(), nested_px)
+    t.set_path_regen(P.Tracer.PATH_NESTED)
     px = np.zeros(P.PX_LEN * W * H + 1, dtype=np.uint32)
     bands = [(0, 6), (H // 2 - 3, 6), (H - 6, 6)]
     for row, rows in bands:

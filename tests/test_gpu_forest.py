@@ -158,18 +158,27 @@ def test_forest_path_matches_the_twin(gpu, scene, spp):
     for f in range(spp):
         fo.sample(cam, w, h, want, f, 8)
     n = w * h
-    for waves, stack in ((2, 32), (3, 32), (3, 24), (4, 24)):  # register fits and mesh-tree stacks
-        t.set_path_waves(waves)
-        t.set_path_stack(stack)
-        assert t.path_stack == stack
-        t.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next call starts over
-        t.sample(cam, w, h, spp)
-        got = t.px()
-        for lo, hi in [(1 * n, 4 * n), (7 * n, 14 * n)]:
-            a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)
-            bad = ~np.isclose(a, b, rtol=1e-5, atol=1e-7)
-            assert not bad.any(), f"waves {waves} words {lo}..{hi}: {bad.sum()} beyond 1e-5, first {np.nonzero(bad)[0][:5] + lo}"
-        assert (got[31 * n:32 * n] == want[31 * n:32 * n]).all()
+    nested = {}
+    for build in (P.Tracer.PATH_NESTED, P.Tracer.PATH_SPLIT):  # the split build: walks spanning turns
+        t.set_path_regen(build)
+        for waves, stack in ((2, 32), (3, 32), (3, 24), (4, 24)):  # register fits and mesh-tree stacks
+            t.set_path_waves(waves)
+            t.set_path_stack(stack)
+            assert t.path_stack == stack
+            t.sample(S.path_camera(w + 1, h), w + 1, h, 1)  # another view: the next call starts over
+            t.sample(cam, w, h, spp)
+            got = t.px()
+            for lo, hi in [(1 * n, 4 * n), (7 * n, 14 * n)]:
+                a, b = got[lo:hi].view(np.float32), want[lo:hi].view(np.float32)
+                bad = ~np.isclose(a, b, rtol=1e-5, atol=1e-7)
+                assert not bad.any(), (f"build {build} waves {waves} words {lo}..{hi}: {bad.sum()} beyond 1e-5, "
+                                       f"first {np.nonzero(bad)[0][:5] + lo}")
+            assert (got[31 * n:32 * n] == want[31 * n:32 * n]).all()
+            if build == P.Tracer.PATH_NESTED:
+                nested[(waves, stack)] = got.copy()
+            else:  # the same bits as the nested build
+                assert np.array_equal(got, nested[(waves, stack)]), (waves, stack)
+    t.set_path_regen(P.Tracer.PATH_NESTED)
 
 
 def test_reinstance_matches_the_oracle(gpu, orc):
