@@ -399,10 +399,14 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t);
  * the wave's longest path; 2 is the wavefront build: each turn walks every
  * pixel's ray in a persistent walker (a lane whose walk ends takes the next
  * queued ray), shades in a kernel of its own, and appends the next rays and
- * the shadow rays to queues by wave ballots. Both give the same sums, bit
- * for bit. They serve one tree and a camera with lamps or a lit world and no
- * Adaptive Sampling; other frames take the nested build, and a sliced sample
- * or a recorded frame takes 1 for 2. The autotune tries all three. */
+ * the shadow rays to queues by wave ballots; 3 is the split build: each lane
+ * keeps one resumable walk (closest or shadow ray) and a wave walks only
+ * until half its working lanes have finished, those shade and begin their
+ * next walk while the others walk on. All give the same sums, bit for bit.
+ * They serve a camera with lamps or a lit world and no Adaptive Sampling, and
+ * 1 and 2 one tree only (3 serves a forest too); other frames take the
+ * nested build, and a sliced sample or a recorded frame takes 1 for 2. The
+ * autotune tries every build the scene and camera allow. */
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on);
 uint32_t phr_tracer_path_regen(const phr_tracer* t);
 /* No counterpart in the reference: the mesh-tree stack entries a thread of
