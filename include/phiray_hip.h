@@ -507,6 +507,10 @@ int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, ui
  * (D^2 + 0.01)). Blocking. */
 int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_array* frame_b, uint64_t n, double* err,
                       double* noise);
+/* No counterpart in the reference: the spread of two denoised images a and b
+ * (3n floats each) made from independent halves of one frame, mean((a - b)^2 /
+ * (a^2 + 0.01)): twice the noise-driven part of either image's relMSE. Blocking. */
+int phr_kpcn_spread(phr_gpu* g, phr_array* a, phr_array* b, uint64_t n, double* out);
 /* No counterpart in the reference: the frame of both halves (equal sample
  * counts) as one: each channel's mean, depth over the halves that met
  * something; channels 10 (a frame) or 3 (the radiance alone). Enqueued. */

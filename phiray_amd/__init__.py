@@ -276,6 +276,7 @@ _SIGS = {
     "phr_kpcn_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P]),
     "phr_kpcn_held_out": (_I, [_P, _P, _P, _P, _U64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "phr_kpcn_frame_mean": (_I, [_P, _P, _P, _U64, _U32, _P]),
+    "phr_kpcn_spread": (_I, [_P, _P, _P, _U64, C.POINTER(C.c_double)]),
     "phr_kpcn_train_indexed": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, C.c_float, C.POINTER(C.c_float)]),
     "phr_tracer_frame": (_I, [_P, _P, _U64]),
     "phr_tracer_set_path_waves": (_I, [_P, _U32]),
@@ -1360,6 +1361,14 @@ class Kpcn:
         _check(lib().phr_kpcn_held_out(gpu._h, denoised._h, half_a._h, half_b._h, n, C.byref(err), C.byref(noise)),
                "Kpcn held-out check")
         return float(err.value), float(noise.value)
+
+    @staticmethod
+    def spread(gpu: Gpu, a: GpuArray, b: GpuArray, n: int) -> float:
+        """mean((a - b)^2 / (a^2 + 0.01)) of two denoised images from
+        independent halves: twice the noise-driven part of either one's relMSE."""
+        v = C.c_double(0)
+        _check(lib().phr_kpcn_spread(gpu._h, a._h, b._h, n, C.byref(v)), "Kpcn spread")
+        return float(v.value)
 
     @staticmethod
     def frame_mean(gpu: Gpu, half_a: GpuArray, half_b: GpuArray, n: int, out: GpuArray, channels: int = 10) -> None:

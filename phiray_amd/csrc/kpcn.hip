@@ -1042,6 +1042,26 @@ __global__ __launch_bounds__(256) void k_held_out(const float* __restrict__ d, c
     }
 }
 
+// The spread of two denoised images of the same view from independent
+// halves: mean over values of (a - b)^2 / (a^2 + 0.01), which is twice the
+// noise-driven part of either image's relMSE (its bias is common to both).
+__global__ __launch_bounds__(256) void k_spread(const float* __restrict__ a, const float* __restrict__ b, uint64_t n,
+                                                double* __restrict__ part) {
+    __shared__ double s_e[256];
+    double e = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < 3 * n; i += (uint64_t)gridDim.x * 256) {
+        const float x = a[i], y = b[i];
+        e += (double)((x - y) * (x - y) / (x * x + 0.01f));
+    }
+    s_e[threadIdx.x] = e;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s_e[threadIdx.x] += s_e[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s_e[0];
+}
+
 // The frame of both halves together (equal sample counts): the mean of each
 // channel, depth over the halves that met something (-1 when neither did).
 // With channels = 3 only the radiance, 3 floats a pixel.
@@ -1234,6 +1254,28 @@ int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_a
     for (uint32_t i = 0; i < blocks; ++i) se += h[2 * i], sv += h[2 * i + 1];
     *err = se / (3.0 * (double)n);
     if (noise) *noise = sv / (3.0 * (double)n);
+    return PHR_OK;
+}
+
+int phr_kpcn_spread(phr_gpu* g, phr_array* a, phr_array* b, uint64_t n, double* out) {
+    PHR_CHECK(g && a && b && out && n, PHR_ERR_ARG, "phr_kpcn_spread: null");
+    PHR_CHECK(a->len * a->elem >= n * 12 && b->len * b->elem >= n * 12, PHR_ERR_SIZE, "phr_kpcn_spread: arrays smaller than n pixels");
+    double* part = nullptr;
+    PHR_HIP(hipMalloc((void**)&part, 8ull * HO_BLOCKS));
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(HO_BLOCKS, (3 * n + 255) / 256);
+    hipLaunchKernelGGL(k_spread, dim3(blocks), dim3(256), 0, g->stream, (const float*)a->ptr, (const float*)b->ptr, n, part);
+    std::vector<double> h(blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 8ull * blocks, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) {
+        phr_set_error("phr_kpcn_spread", e);
+        return PHR_ERR_HIP;
+    }
+    double se = 0.0;
+    for (uint32_t i = 0; i < blocks; ++i) se += h[i];
+    *out = se / (3.0 * (double)n);
     return PHR_OK;
 }
 

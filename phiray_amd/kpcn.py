@@ -226,13 +226,16 @@ def render_multi(gpu: Gpu, seeds, crop: int, spps, spp_ref: int, variants: int, 
 class LearnedFrame:
     """Frames of one scene made by the learned denoiser under its held-out
     check. Two tracers hold the two halves of every frame (the same samples
-    a pixel on other streams); `render` denoises half A, estimates that
-    image's relMSE from half B, and takes twice the samples while the
-    estimate is over `bound`. The answer is the network's output from both
-    halves together, or, once `max_spp` samples would be passed, the plain
-    render of every sample taken. A failed check takes as many more samples
-    as its estimate asks for, the error falling as 1 / samples, at least one
-    and at most as many as were taken."""
+    a pixel on other streams); `render` denoises each half (D_A, D_B),
+    estimates D_A's relMSE E from half B (Kpcn.held_out) and the answer's
+    from E and the halves' spread S (Kpcn.spread): the answer is the
+    network's output from both halves together, whose noise-driven error is
+    half of D_A's (S / 2 of it) and whose bias is D_A's, so its relMSE is
+    taken as E - S / 4. More samples are taken while that is over `bound`;
+    once `max_spp` samples would be passed the plain render of every sample
+    taken is the answer. A failed check takes as many more samples as its
+    estimate asks for, the error falling as 1 / samples, at least one and at
+    most as many as were taken."""
 
     def __init__(self, gpu: Gpu, tris: np.ndarray, net: Kpcn, leaf: int = 4):
         self.gpu, self.net = gpu, net
@@ -249,15 +252,17 @@ class LearnedFrame:
         self.fb = GpuArray.zeroed(self.gpu, np.float32, n * Kpcn.FRAME)
         self.fm = GpuArray.zeroed(self.gpu, np.float32, n * Kpcn.FRAME)
         self.da = GpuArray.zeroed(self.gpu, np.float32, n * 3)
+        self.db = GpuArray.zeroed(self.gpu, np.float32, n * 3)
         self.out = GpuArray.zeroed(self.gpu, np.float32, n * 3)
         self._size = (w, h)
 
     def render(self, cam: np.ndarray, w: int, h: int, spp: int = 4, bound: float = 0.01, max_spp: int = 256) -> dict:
         """Render, check and denoise; the image is left in `self.out`
         (3 floats a pixel). Returns what was done: samples taken, each
-        check's estimate, whether the plain render was the answer. Past the
-        first check each half goes to min(2 t, max(t + 1, ceil(t err /
-        bound))) samples from t."""
+        check's estimates (half A's, and the answer's), whether the plain
+        render was the answer. Past the first check each half goes to
+        min(2 t, max(t + 1, ceil(t est / bound))) samples from t, est the
+        answer's estimate."""
         self._buffers(w, h)
         n = w * h
         half = max(spp // 2, 1)
@@ -271,12 +276,15 @@ class LearnedFrame:
             self.b.frame_into(self.fb)
             self.net.infer(self.fa, 1, w, h, self.da)
             err, noise = Kpcn.held_out(self.gpu, self.da, self.fa, self.fb, n)
-            checks.append({"spp": 2 * taken, "estimate": err, "half_noise": noise})
-            if err <= bound:
+            self.net.infer(self.fb, 1, w, h, self.db)
+            spread = Kpcn.spread(self.gpu, self.da, self.db, n)
+            est = err - spread / 4.0  # the answer's: D_A's bias, half of its noise-driven error
+            checks.append({"spp": 2 * taken, "estimate": err, "half_noise": noise, "spread": spread, "answer_estimate": est})
+            if est <= bound:
                 Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.fm)
                 self.net.infer(self.fm, 1, w, h, self.out)
                 return {"spp": 2 * taken, "checks": checks, "plain": False}
-            nxt = min(2 * taken, max(taken + 1, math.ceil(taken * err / bound)))
+            nxt = min(2 * taken, max(taken + 1, math.ceil(taken * est / bound)))
             if 2 * nxt > max_spp:
                 break
             self.a.sample(ca, w, h, nxt - taken)  # the same view: the halves go on accumulating

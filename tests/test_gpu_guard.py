@@ -5,7 +5,11 @@ Kpcn.held_out) on the device.
   denoiser's scenes, the estimated relMSE of the network's output from half A
   equals its measured relMSE against a 2,048-sample render (same D^2 + 0.01
   scale) within 20% + 1e-3: E[(D - B)^2] - E[(A - B)^2] / 2 = E[(D - T)^2]
-  for independent halves of equal sample counts.
+  for independent halves of equal sample counts. The answer's estimate (the
+  output from both halves: half A's error common to both halves, half of
+  its noise-driven part, E - spread / 4) lies between the answer's measured
+  relMSE and half A's estimate: it does not understate the answer's error
+  (the common part shrinks with more samples too, so it overstates it).
 * A weight set that shifts the image by two pixels (every filter on the
   corner tap) is caught: the check asks for more samples until the budget
   ends, and the answer is then the plain render of every sample, exactly.
@@ -48,9 +52,16 @@ def test_the_estimate_matches_the_measured_error(gpu):
     res = lf.render(cam, W, H, spp=4, bound=1e9)  # one check, whatever it says
     est = res["checks"][0]["estimate"]
     d = lf.da.read().reshape(-1, 3)
-    meas, _ = _measured(gpu, tris, cam, d)
+    meas, t = _measured(gpu, tris, cam, d)
     assert abs(est - meas) <= 0.2 * meas + 1e-3, (est, meas)
     assert res["checks"][0]["half_noise"] > 0.0
+    out = lf.out.read().reshape(-1, 3).astype(np.float64)
+    meas_out = float(np.mean((out - t) ** 2 / (out * out + 0.01)))
+    est_out = res["checks"][0]["answer_estimate"]
+    print(f"[guard] half A: estimate {est:.5f} measured {meas:.5f}; answer: estimate {est_out:.5f} measured {meas_out:.5f}",
+          flush=True)
+    assert res["checks"][0]["spread"] > 0.0
+    assert meas_out - 1e-3 <= est_out < est, (est_out, meas_out, est)
 
 
 def _shifting_weights():
@@ -71,7 +82,7 @@ def test_a_bad_network_falls_back_to_the_plain_render(gpu):
     res = lf.render(cam, W, H, spp=4, bound=0.01, max_spp=32)
     assert res["plain"], res
     assert res["spp"] == 32 and len(res["checks"]) == 4, res
-    assert all(c["estimate"] > 0.01 for c in res["checks"])
+    assert all(c["estimate"] > 0.01 and c["answer_estimate"] > 0.01 for c in res["checks"])
     plain = 0.5 * (lf.a.radiance() + lf.b.radiance())
     assert np.array_equal(lf.out.read().reshape(-1, 3), plain.astype(np.float32))
 
