@@ -500,18 +500,34 @@ __device__ PhrHit walk_forest(const SceneDev& s, V3 o, V3 d, float tmax, uint16_
             if (tr < PHR_INF) { li = __float_as_uint(rlo.w); ri = __float_as_uint(rhi.w); continue; }
             pop = true;
         } else if (in_mesh) {
-            for (uint32_t id = li & ~PHR_LEAF; id < ri; ++id) {
-                ++tests;
-                const float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
-                float u, v;
-                const float t = tri_t(t0, t1, t2, ro, rs, u, v);
-                if (t < best.t) {
-                    if (ANY) {
-                        const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
-                        if (m[6] > 0.5f || m[3] < 0.5f) continue;
-                        return PhrHit{t, id, u, v, inst};
+            uint32_t id = li & ~PHR_LEAF;
+            if (id < ri) {
+                float4 t0 = s.tris[3 * id], t1 = s.tris[3 * id + 1], t2 = s.tris[3 * id + 2];
+                for (;;) {
+                    ++tests;
+                    // The leaf's next triangle is loaded while this one is tested.
+                    const bool next = id + 1u < ri;
+                    float4 n0 = t0, n1 = t1, n2 = t2;
+                    if (next) {
+                        n0 = s.tris[3 * (id + 1u)];
+                        n1 = s.tris[3 * (id + 1u) + 1];
+                        n2 = s.tris[3 * (id + 1u) + 2];
                     }
-                    best = PhrHit{t, id, u, v, inst};
+                    float u, v;
+                    const float t = tri_t(t0, t1, t2, ro, rs, u, v);
+                    if (t < best.t) {
+                        if (ANY) {
+                            const float* m = s.mats + MAT_WORDS * (size_t)s.faces[id].w;
+                            if (!(m[6] > 0.5f || m[3] < 0.5f)) return PhrHit{t, id, u, v, inst};
+                        } else {
+                            best = PhrHit{t, id, u, v, inst};
+                        }
+                    }
+                    if (!next) break;
+                    ++id;
+                    t0 = n0;
+                    t1 = n1;
+                    t2 = n2;
                 }
             }
             pop = true;
