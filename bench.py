@@ -710,8 +710,8 @@ def main() -> int:
             # L2 and the MALL, so the kernel waits on latency, not bandwidth.
             line["roofline"]["dram_achieved"] = traffic / kernel_s / 1e9
             line["roofline"]["dram_frac"] = traffic / kernel_s / 1e9 / HBM_PEAK_GBS
-            line["roofline"]["limiter"] = ("VALU issue of divergent waves (lanes a VALU instruction: 14.3 of 64 split, 11.0 nested; "
-                                           "VALU busy 0.77 / 0.73) and L2/MALL latency: frac counts the walk's logical bytes, "
+            line["roofline"]["limiter"] = ("VALU issue of divergent waves (lanes a VALU instruction: 15.1 of 64 split, 11.0 nested; "
+                                           "VALU busy 0.76 / 0.73) and L2/MALL latency: frac counts the walk's logical bytes, "
                                            "dram_frac the counted DRAM bytes")
         line["s_per_frame"] = per
         line["path_waves_per_simd"] = tracer.path_waves

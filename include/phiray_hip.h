@@ -401,7 +401,7 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t);
  * queued ray), shades in a kernel of its own, and appends the next rays and
  * the shadow rays to queues by wave ballots; 3 is the split build: each lane
  * keeps one resumable walk (closest or shadow ray) and a wave walks only
- * until half its working lanes have finished, those shade and begin their
+ * until three quarters of its working lanes have finished, those shade and begin their
  * next walk while the others walk on. All give the same sums, bit for bit.
  * They serve a camera with lamps or a lit world and no Adaptive Sampling, and
  * 1 and 2 one tree only (3 serves a forest too); other frames take the

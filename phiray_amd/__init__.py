@@ -990,7 +990,7 @@ class Tracer:
     def set_path_regen(self, mode: "bool | int") -> None:
         """Which build the path frames take: the nested one (False, 0), the
         path-regenerating one (True, 1), the wavefront one (2) or the split
-        one (3: a wave walks until half its lanes have finished their walk,
+        one (3: a wave walks until three quarters of its lanes have finished their walk,
         those shade and start their next walk while the others walk on). The
         last three serve one tree and a camera with lamps or a lit world, no
         Adaptive Sampling; other frames take the nested build."""

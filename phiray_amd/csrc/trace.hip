@@ -1763,8 +1763,8 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 
 // The same samples as k_path_rg with walks that span turns. Each lane keeps
 // one walk (a Walker: the sample's closest ray, or its shadow ray) and the
-// wave walks only until half of its lanes still at work have finished
-// theirs. Those lanes then shade (rg_hit after a closest walk, the turn's end
+// wave walks only until three quarters of its lanes still at work have
+// finished theirs. Those lanes then shade (rg_hit after a closest walk, the turn's end
 // after a shadow walk) and begin their next walk, while the others keep
 // their walk where it stopped for the next turn. A wave no longer waits on
 // its longest walk, and closest and shadow walks share one loop. A lane's
@@ -1841,10 +1841,10 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
         ++closest;
     }
     while (__ballot(!done) != 0ull) {
-        for (;;) {  // walk until half the lanes at work wait to shade, or all do
+        for (;;) {  // walk until three quarters of the lanes at work wait to shade, or all do
             const uint32_t waiting = (uint32_t)__popcll(__ballot(!done && !w.on));
             const uint32_t working = (uint32_t)__popcll(__ballot(!done));
-            if (waiting == working || 2u * waiting >= working) break;
+            if (waiting == working || 4u * waiting >= 3u * working) break;
             if (w.on) sw.step(sc, stk, tstk, steps, tests);
         }
         if (done || w.on) continue;
