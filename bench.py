@@ -166,6 +166,7 @@ def batch_render_rates(gpu: P.Gpu, scenes: int = 256, w: int = 256, h: int = 256
     n = int(seg_lo[-1])
     return {"scenes": scenes, "tris": n, "image": f"{w}x{h}", "spp": spp, "bounces": 4, "light": "world colour 1.0",
             "setup_s": setup_s, "path_waves_per_simd": waves, "grouped": grouped, "path_build": PATH_BUILDS[b.path_build],
+            "split_shade_eighths": b.split_shade,
             "build_ms": build_s * 1e3, "build_mtri_s": n / build_s / 1e6,
             "render_ms": t_render * 1e3, "rays_per_pass": rays, "mrays_s": rays / t_render / 1e6,
             "render_gbs": nbytes / t_render / 1e9, "render_hbm_frac": nbytes / t_render / 1e9 / HBM_PEAK_GBS,
@@ -717,6 +718,7 @@ def main() -> int:
         line["path_waves_per_simd"] = tracer.path_waves
         line["path_regen"] = tracer.path_regen
         line["path_build"] = PATH_BUILDS[tracer.path_regen]
+        line["split_shade_eighths"] = tracer.split_shade
         line["rays_per_frame"] = rays / args.steps
         line["closest_per_frame"] = closest / args.steps
         line["shadow_per_frame"] = shadow / args.steps

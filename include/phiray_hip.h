@@ -409,6 +409,11 @@ uint32_t phr_tracer_path_waves(const phr_tracer* t);
  * autotune tries every build the scene and camera allow. */
 int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on);
 uint32_t phr_tracer_path_regen(const phr_tracer* t);
+/* No counterpart in the reference: the split build's shading point. A wave
+ * walks until `eighths` (1 to 8) eighths of its working lanes wait to shade
+ * (default 6); the autotune picks 4 or 6. The sums do not depend on it. */
+int phr_tracer_set_split_shade(phr_tracer* t, uint32_t eighths);
+uint32_t phr_tracer_split_shade(const phr_tracer* t);
 /* No counterpart in the reference: the mesh-tree stack entries a thread of
  * the path launches holds (24 or 32; the autotune picks the 24-entry stack
  * at 3 waves too when every mesh tree of a forest is at most 24 deep). */
@@ -448,7 +453,10 @@ int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped);
  * phiray renders one scene a call. */
 int phr_batch_set_path_build(phr_batch* b, uint32_t build);
 uint32_t phr_batch_path_build(const phr_batch* b);
-/* Time every (build, waves, order) triple on this render and keep the fastest. */
+/* The split build's shading point, as phr_tracer_set_split_shade (default 4). */
+int phr_batch_set_split_shade(phr_batch* b, uint32_t eighths);
+uint32_t phr_batch_split_shade(const phr_batch* b);
+/* Time every (build, shading point, waves, order) on this render and keep the fastest. */
 int phr_batch_autotune(phr_batch* b, const void* cam, uint32_t w, uint32_t h, uint32_t spp, uint32_t reps, uint32_t* waves,
                        uint32_t* grouped, double* best_s);
 /* The whole pass (rebuild 1: build, then render) as one hipGraph; kernels

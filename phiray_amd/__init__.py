@@ -305,6 +305,8 @@ _SIGS = {
     "phr_tracer_path_waves": (_U32, [_P]),
     "phr_tracer_set_path_regen": (_I, [_P, _U32]),
     "phr_tracer_path_regen": (_U32, [_P]),
+    "phr_tracer_set_split_shade": (_I, [_P, _U32]),
+    "phr_tracer_split_shade": (_U32, [_P]),
     "phr_tracer_path_stack": (_U32, [_P]),
     "phr_tracer_set_path_stack": (_I, [_P, _U32]),
     "phr_tracer_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(C.c_double)]),
@@ -322,6 +324,8 @@ _SIGS = {
     "phr_batch_path": (_I, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
     "phr_batch_set_path_build": (_I, [_P, _U32]),
     "phr_batch_path_build": (_U32, [_P]),
+    "phr_batch_set_split_shade": (_I, [_P, _U32]),
+    "phr_batch_split_shade": (_U32, [_P]),
     "phr_batch_autotune": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(C.c_double)]),
     "phr_batch_record": (_I, [_P, _P, _U32, _U32, _U32, _I]),
     "phr_batch_replay": (_I, [_P, C.POINTER(C.c_double)]),
@@ -1001,6 +1005,16 @@ class Tracer:
     def path_regen(self) -> int:
         return int(lib().phr_tracer_path_regen(self._h))
 
+    def set_split_shade(self, eighths: int) -> None:
+        """The split build's shading point: a wave walks until `eighths`
+        (1 to 8) eighths of its working lanes wait to shade. The sums do not
+        depend on it; the autotune picks 4 or 6."""
+        _check(lib().phr_tracer_set_split_shade(self._h, int(eighths)), "Tracer set_split_shade")
+
+    @property
+    def split_shade(self) -> int:
+        return int(lib().phr_tracer_split_shade(self._h))
+
     @property
     def path_stack(self) -> int:
         """Mesh-tree stack entries a thread of the path launches holds."""
@@ -1265,6 +1279,14 @@ class Batch:
     @property
     def path_build(self) -> int:
         return int(lib().phr_batch_path_build(self._h))
+
+    def set_split_shade(self, eighths: int) -> None:
+        """The split build's shading point, as Tracer.set_split_shade."""
+        _check(lib().phr_batch_set_split_shade(self._h, int(eighths)), "Batch set_split_shade")
+
+    @property
+    def split_shade(self) -> int:
+        return int(lib().phr_batch_split_shade(self._h))
 
     def autotune(self, cam: np.ndarray, w: int, h: int, spp: int = 1, reps: int = 1) -> tuple[int, int, float]:
         """(waves a SIMD, grouped order, seconds) of the fastest render kernel,

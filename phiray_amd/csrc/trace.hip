@@ -1763,8 +1763,8 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 
 // The same samples as k_path_rg with walks that span turns. Each lane keeps
 // one walk (a Walker: the sample's closest ray, or its shadow ray) and the
-// wave walks only until three quarters of its lanes still at work have
-// finished theirs. Those lanes then shade (rg_hit after a closest walk, the turn's end
+// wave walks only until shade8 eighths (1 to 8) of its lanes still at work
+// have finished theirs. Those lanes then shade (rg_hit after a closest walk, the turn's end
 // after a shadow walk) and begin their next walk, while the others keep
 // their walk where it stopped for the next turn. A wave no longer waits on
 // its longest walk, and closest and shadow walks share one loop. A lane's
@@ -1798,7 +1798,8 @@ struct SplitWalk<true> {
 template <bool F = false>
 __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* __restrict__ camp, uint32_t blk, uint32_t tiles_x,
                                            uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-                                           unsigned long long* __restrict__ rays, uint32_t* s_stack, uint16_t* s_tstack) {
+                                           unsigned long long* __restrict__ rays, uint32_t* s_stack, uint16_t* s_tstack,
+                                           uint32_t shade8) {
     const PhrCamera* p = camp;
     const size_t n = (size_t)p->w * p->h;
     const uint32_t tx = blk % tiles_x, ty = blk / tiles_x;
@@ -1841,10 +1842,10 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
         ++closest;
     }
     while (__ballot(!done) != 0ull) {
-        for (;;) {  // walk until three quarters of the lanes at work wait to shade, or all do
+        for (;;) {  // walk until shade8 / 8 of the lanes at work wait to shade, or all do
             const uint32_t waiting = (uint32_t)__popcll(__ballot(!done && !w.on));
             const uint32_t working = (uint32_t)__popcll(__ballot(!done));
-            if (waiting == working || 4u * waiting >= 3u * working) break;
+            if (waiting == working || 8u * waiting >= shade8 * working) break;
             if (w.on) sw.step(sc, stk, tstk, steps, tests);
         }
         if (done || w.on) continue;
@@ -1946,19 +1947,19 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
 template <int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    unsigned long long* __restrict__ rays, uint32_t tile0, uint32_t shade8) {
     __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
-    path_split(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, nullptr);
+    path_split(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, nullptr, shade8);
 }
 // The split build over a forest: MSTK mesh-tree entries a thread (the
 // caller checked the mesh trees' depth when it is below PHR_STACK).
 template <int WPE, uint32_t MSTK>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_sw_forest(
     SceneDev sc, const PhrCamera* __restrict__ camp, uint32_t tiles_x, uint32_t frame0, uint32_t spp, uint32_t* __restrict__ px,
-    unsigned long long* __restrict__ rays, uint32_t tile0) {
+    unsigned long long* __restrict__ rays, uint32_t tile0, uint32_t shade8) {
     __shared__ uint32_t s_stack[MSTK * TRACE_THREADS];
     __shared__ uint16_t s_tstack[PHR_STACK * TRACE_THREADS];
-    path_split<true>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, s_tstack);
+    path_split<true>(sc, camp, blockIdx.x + tile0, tiles_x, frame0, spp, px, rays, s_stack, s_tstack, shade8);
 }
 
 // ---- the wavefront build --------------------------------------------------------
@@ -2441,7 +2442,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
 template <int WPE>
 __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_path_batch_sw(
     const SceneDev* __restrict__ scenes, const PhrCamera* __restrict__ cams, uint32_t tiles, uint32_t tiles_x, uint32_t spp,
-    uint32_t* __restrict__ px, uint64_t px_stride, unsigned long long* __restrict__ rays, uint32_t grouped) {
+    uint32_t* __restrict__ px, uint64_t px_stride, unsigned long long* __restrict__ rays, uint32_t grouped, uint32_t shade8) {
     __shared__ uint32_t s_stack[PHR_STACK * TRACE_THREADS];
     uint32_t scene, tile;
     if (grouped) {
@@ -2453,7 +2454,7 @@ __global__ __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(W
         tile = blockIdx.x % tiles;
     }
     const SceneDev sc = scenes[scene];
-    path_split(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, s_stack, nullptr);
+    path_split(sc, cams + scene, tile, tiles_x, 0u, spp, px + px_stride * scene, rays, s_stack, nullptr, shade8);
 }
 
 // The builds of the path kernels the tracer and the batch choose among.
@@ -2467,33 +2468,34 @@ bool regen_allowed(const PhrCamera& cam) {
     return cam.adapt[3] == 0.0f && (cam.count != 0u || lit);
 }
 
-// regen: 0 the nested build; 3 the split build (k_path_sw); else the
-// regenerating build (k_path_rg); the last two for one tree, no list and no
-// Adaptive Sampling or studio rig (the caller checks the camera).
+// regen: 0 the nested build; 3 the split build (k_path_sw, shading at shade8
+// eighths of a wave); else the regenerating build (k_path_rg); the last two
+// for one tree, no list and no Adaptive Sampling or studio rig (the caller
+// checks the camera).
 // short_stack: the forest's 3-wave build with FOREST_SHORT_STACK mesh-tree
 // entries (the caller checked the mesh trees' depth).
 void launch_path(uint32_t waves, bool forest, uint32_t regen, dim3 grid, hipStream_t s, const SceneDev& sc, const PhrCamera* cp,
                  uint32_t tiles_x, uint32_t f0, uint32_t spp, uint32_t* px, unsigned long long* rays, uint32_t listed,
-                 uint32_t tile0, bool short_stack = false) {
+                 uint32_t tile0, bool short_stack = false, uint32_t shade8 = 4) {
     const dim3 block(TRACE_THREADS);
     if (regen == 3u && !forest && !listed) {
         if (waves == 5)
-            hipLaunchKernelGGL(k_path_sw<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL(k_path_sw<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else if (waves == 4)
-            hipLaunchKernelGGL(k_path_sw<4>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL(k_path_sw<4>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else if (waves == 3)
-            hipLaunchKernelGGL(k_path_sw<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL(k_path_sw<3>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else
-            hipLaunchKernelGGL(k_path_sw<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL(k_path_sw<2>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
     } else if (regen == 3u && forest && !listed) {  // the caller checked the mesh trees' depth for the short stacks
         if (waves >= 4)
-            hipLaunchKernelGGL((k_path_sw_forest<4, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL((k_path_sw_forest<4, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else if (waves == 3 && short_stack)
-            hipLaunchKernelGGL((k_path_sw_forest<3, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL((k_path_sw_forest<3, FOREST_SHORT_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else if (waves == 3)
-            hipLaunchKernelGGL((k_path_sw_forest<3, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL((k_path_sw_forest<3, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
         else
-            hipLaunchKernelGGL((k_path_sw_forest<2, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
+            hipLaunchKernelGGL((k_path_sw_forest<2, PHR_STACK>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0, shade8);
     } else if (regen && !forest && !listed) {
         if (waves == 5)
             hipLaunchKernelGGL(k_path_rg<5>, grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, tile0);
@@ -2521,20 +2523,21 @@ void launch_path(uint32_t waves, bool forest, uint32_t regen, dim3 grid, hipStre
         hipLaunchKernelGGL((k_path<false, 2>), grid, block, 0, s, sc, cp, tiles_x, f0, spp, px, rays, listed, tile0);
 }
 
-// split: the split build (k_path_batch_sw), else the nested one.
+// split: the split build (k_path_batch_sw, shading at shade8 eighths), else
+// the nested one.
 void launch_path_batch(uint32_t waves, bool split, dim3 grid, hipStream_t s, const SceneDev* table, const PhrCamera* cams,
                        uint32_t tiles, uint32_t tiles_x, uint32_t spp, uint32_t* px, uint64_t stride, unsigned long long* rays,
-                       uint32_t grouped) {
+                       uint32_t grouped, uint32_t shade8) {
     const dim3 block(TRACE_THREADS);
     if (split) {
         if (waves == 5)
-            hipLaunchKernelGGL(k_path_batch_sw<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+            hipLaunchKernelGGL(k_path_batch_sw<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped, shade8);
         else if (waves == 4)
-            hipLaunchKernelGGL(k_path_batch_sw<4>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+            hipLaunchKernelGGL(k_path_batch_sw<4>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped, shade8);
         else if (waves == 3)
-            hipLaunchKernelGGL(k_path_batch_sw<3>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+            hipLaunchKernelGGL(k_path_batch_sw<3>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped, shade8);
         else
-            hipLaunchKernelGGL(k_path_batch_sw<2>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
+            hipLaunchKernelGGL(k_path_batch_sw<2>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped, shade8);
     } else if (waves == 5)
         hipLaunchKernelGGL(k_path_batch<5>, grid, block, 0, s, table, cams, tiles, tiles_x, spp, px, stride, rays, grouped);
     else if (waves == 4)
@@ -2752,6 +2755,7 @@ struct phr_tracer {
     uint32_t w = 0, h = 0, frame = 0, samples = 32;
     uint32_t path_waves = 2;  // which k_path variant runs (2 to 5 waves a SIMD)
     uint32_t path_regen = 0;  // 1: the path-regenerating build, 2: the wavefront build, 3: the split build, where the camera allows it
+    uint32_t split_shade = 6;  // the split build shades once this many eighths of a wave's working lanes wait
     uint32_t mesh_depth = PHR_STACK;  // a forest: its deepest mesh tree (the 4-wave build needs <= 24)
     bool short_stack = false;         // the forest's 3-wave build with the 24-entry mesh-tree stack
     // The binding split phiray's WGSL needs (items a binding for nodes, faces,
@@ -3698,7 +3702,7 @@ int phr_tracer_sample_in(phr_tracer* t, const void* cam_v, uint32_t w, uint32_t 
             const uint32_t a = (uint32_t)((uint64_t)tiles * k / slices), b = (uint32_t)((uint64_t)tiles * (k + 1) / slices);
             if (b == a) continue;
             launch_path(path_build_waves(t), t->forest, regen, dim3(b - a), s, sc, cp, tiles_x, f0, count, pxp, t->rays, listed, a,
-                        path_short_stack(t));
+                        path_short_stack(t), t->split_shade);
             PHR_HIP(hipGetLastError());
             if (slices > 1) {
                 PHR_HIP(hipStreamSynchronize(s));
@@ -3776,6 +3780,14 @@ int phr_tracer_set_path_regen(phr_tracer* t, uint32_t on) {
     return PHR_OK;
 }
 uint32_t phr_tracer_path_regen(const phr_tracer* t) { return t ? t->path_regen : 0; }
+// The split build's shading point: it shades once `eighths` (1 to 8) eighths
+// of a wave's working lanes wait. The images do not depend on it.
+int phr_tracer_set_split_shade(phr_tracer* t, uint32_t eighths) {
+    PHR_CHECK(t && eighths >= 1u && eighths <= 8u, PHR_ERR_ARG, "phr_tracer_set_split_shade: null, or not 1 to 8");
+    t->split_shade = eighths;
+    return PHR_OK;
+}
+uint32_t phr_tracer_split_shade(const phr_tracer* t) { return t ? t->split_shade : 0u; }
 
 // The mesh-tree stack entries the path launches take: the 4-wave forest
 // build's 24, the 3-wave build's 24 or 32 as the autotune chose, 32 otherwise.
@@ -3819,27 +3831,35 @@ int phr_tracer_autotune(phr_tracer* t, const void* cam, uint32_t w, uint32_t h, 
     }
     const bool fits_short = t->forest && t->mesh_depth <= FOREST_SHORT_STACK;
     bool best_short = false;
+    uint32_t best_shade = t->split_shade;
     for (uint32_t bi = 0; bi < nbuilds; ++bi)
         // The wavefront build's kernels have one register fit: timed once,
         // keeping the waves the others chose for its recorded-frame stand-in.
         for (uint32_t regen = builds[bi], waves = regen == 2u ? best_w : PATH_WAVES_MIN;
              waves <= (regen == 2u ? best_w : (t->forest ? (fits_short ? 4u : 3u) : PATH_WAVES_MAX)); ++waves)
-            for (uint32_t stk = 0; stk < ((fits_short && waves == 3) ? 2u : 1u); ++stk) {
-                t->path_waves = waves;
-                t->path_regen = regen;
-                t->short_stack = stk == 1;
-                double mine = 1e30;
-                for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
-                    t->have_last = false;  // start the image over every launch
-                    int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
-                    if (rc != PHR_OK) return rc;
-                    if (r > 0) mine = std::min(mine, t->last_gpu_s);
+            for (uint32_t stk = 0; stk < ((fits_short && waves == 3) ? 2u : 1u); ++stk)
+                for (uint32_t shade : {4u, 6u}) {  // the split build's shading point; one pass for the others
+                    if (regen != 3u && shade != 4u) continue;
+                    t->path_waves = waves;
+                    t->path_regen = regen;
+                    t->short_stack = stk == 1;
+                    if (regen == 3u) t->split_shade = shade;
+                    double mine = 1e30;
+                    for (uint32_t r = 0; r <= reps; ++r) {  // the first launch warms the variant
+                        t->have_last = false;  // start the image over every launch
+                        int rc = phr_tracer_sample(t, cam, w, h, spp, nullptr, nullptr);
+                        if (rc != PHR_OK) return rc;
+                        if (r > 0) mine = std::min(mine, t->last_gpu_s);
+                    }
+                    if (mine < best) {
+                        best = mine, best_w = waves, best_r = regen, best_short = stk == 1;
+                        if (regen == 3u) best_shade = shade;
+                    }
                 }
-                if (mine < best) best = mine, best_w = waves, best_r = regen, best_short = stk == 1;
-            }
     t->path_regen = best_r;
     t->path_waves = best_w;
     t->short_stack = best_short;
+    t->split_shade = best_shade;
     t->have_last = false;
     if (chosen) *chosen = best_w;
     if (best_s) *best_s = best;
@@ -4020,7 +4040,7 @@ int phr_tracer_record_frame(phr_tracer* t, const void* cam_v, uint32_t w, uint32
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     launch_path(path_build_waves(t), t->forest, regen_allowed(cam) ? t->path_regen : 0u, dim3(tiles), s, sc, cp, tiles_x, 0u, spp, pxp,
-                t->rays, 0u, 0u, path_short_stack(t));
+                t->rays, 0u, 0u, path_short_stack(t), t->split_shade);
     int rc = PHR_OK;
     if (denoise == 1) rc = phr_denoise_enqueue(t->gpu, pxp, t->w, t->h, 0, t->dn_scratch, t->last.view);
     if (denoise == 2) {
@@ -4597,6 +4617,7 @@ struct phr_batch {
     uint32_t w = 0, h = 0;
     uint32_t path_waves = 2, grouped = 0;
     uint32_t path_build = 0;      // 0 nested, 3 split (where the camera allows it)
+    uint32_t split_shade = 4;     // the split build shades once this many eighths of a wave's working lanes wait
     bool split_ok = false;        // the last camera allows the split build
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_gpu_s = 0.0;
@@ -4674,12 +4695,13 @@ int batch_image(phr_batch* b, uint32_t w, uint32_t h) {
     return PHR_OK;
 }
 
-void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t waves, uint32_t grouped, uint32_t build) {
+void batch_launch(phr_batch* b, const PhrCamera* cams, uint32_t spp, uint32_t waves, uint32_t grouped, uint32_t build,
+                  uint32_t shade8) {
     const uint32_t tiles_x = (b->w + 15) / 16, tiles = tiles_x * ((b->h + 15) / 16);
     const dim3 grid(tiles * b->scenes);
     const uint64_t stride = PX_LEN * (uint64_t)b->w * b->h + 1;
     launch_path_batch(waves, build == 3u && b->split_ok, grid, b->gpu->stream, b->table, cams, tiles, tiles_x, spp, (uint32_t*)b->px->ptr,
-                      stride, b->rays, grouped);
+                      stride, b->rays, grouped, shade8);
 }
 
 int batch_check_camera(const PhrCamera& cam, uint32_t spp) {
@@ -4882,7 +4904,7 @@ int phr_batch_render(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, ui
         return rc;
     hipStream_t s = b->gpu->stream;
     PHR_HIP(hipEventRecord(b->ev0, s));
-    batch_launch(b, b->cams, spp, b->path_waves, b->grouped, b->path_build);
+    batch_launch(b, b->cams, spp, b->path_waves, b->grouped, b->path_build, b->split_shade);
     PHR_HIP(hipGetLastError());
     PHR_HIP(hipEventRecord(b->ev1, s));
     PHR_HIP(hipEventSynchronize(b->ev1));
@@ -4912,6 +4934,14 @@ int phr_batch_set_path_build(phr_batch* b, uint32_t build) {
     return PHR_OK;
 }
 uint32_t phr_batch_path_build(const phr_batch* b) { return b ? b->path_build : 0u; }
+// The split build's shading point: it shades once `eighths` (1 to 8) eighths
+// of a wave's working lanes wait. The images do not depend on it.
+int phr_batch_set_split_shade(phr_batch* b, uint32_t eighths) {
+    PHR_CHECK(b && eighths >= 1u && eighths <= 8u, PHR_ERR_ARG, "phr_batch_set_split_shade: null, or not 1 to 8");
+    b->split_shade = eighths;
+    return PHR_OK;
+}
+uint32_t phr_batch_split_shade(const phr_batch* b) { return b ? b->split_shade : 0u; }
 int phr_batch_path(const phr_batch* b, uint32_t* waves, uint32_t* grouped) {
     PHR_CHECK(b, PHR_ERR_ARG, "phr_batch_path: null");
     if (waves) *waves = b->path_waves;
@@ -4935,26 +4965,30 @@ int phr_batch_autotune(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, 
         return rc;
     hipStream_t s = b->gpu->stream;
     double best = 1e30;
-    uint32_t bw = b->path_waves, bg = b->grouped, bb = b->path_build;
+    uint32_t bw = b->path_waves, bg = b->grouped, bb = b->path_build, bs = b->split_shade;
     for (uint32_t build : {0u, 3u}) {
         if (build == 3u && !b->split_ok) continue;
-        for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
-            for (uint32_t wv = PATH_WAVES_MIN; wv <= PATH_WAVES_MAX; ++wv) {
-                batch_launch(b, b->cams, spp, wv, g, build);
-                PHR_HIP(hipEventRecord(b->ev0, s));
-                for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g, build);
-                PHR_HIP(hipGetLastError());
-                PHR_HIP(hipEventRecord(b->ev1, s));
-                PHR_HIP(hipEventSynchronize(b->ev1));
-                float ms = 0.f;
-                PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-                const double t = ms * 1e-3 / reps;
-                if (t < best) best = t, bw = wv, bg = g, bb = build;
-            }
+        for (uint32_t shade : {4u, 6u}) {
+            if (build == 0u && shade != 4u) continue;  // the nested build has no shading point
+            for (uint32_t g = 0; g <= (b->scenes % 8 == 0 ? 1u : 0u); ++g)
+                for (uint32_t wv = PATH_WAVES_MIN; wv <= PATH_WAVES_MAX; ++wv) {
+                    batch_launch(b, b->cams, spp, wv, g, build, shade);
+                    PHR_HIP(hipEventRecord(b->ev0, s));
+                    for (uint32_t r = 0; r < reps; ++r) batch_launch(b, b->cams, spp, wv, g, build, shade);
+                    PHR_HIP(hipGetLastError());
+                    PHR_HIP(hipEventRecord(b->ev1, s));
+                    PHR_HIP(hipEventSynchronize(b->ev1));
+                    float ms = 0.f;
+                    PHR_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                    const double t = ms * 1e-3 / reps;
+                    if (t < best) best = t, bw = wv, bg = g, bb = build, bs = build == 3u ? shade : b->split_shade;
+                }
+        }
     }
     b->path_waves = bw;
     b->grouped = bg;
     b->path_build = bb;
+    b->split_shade = bs;
     if (waves) *waves = bw;
     if (grouped) *grouped = bg;
     if (best_s) *best_s = best;
@@ -4982,7 +5016,7 @@ int phr_batch_record(phr_batch* b, const void* cam_v, uint32_t w, uint32_t h, ui
     PHR_HIP(hipStreamSynchronize(s));
     PHR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     if (rebuild) rc = batch_enqueue_build(b);
-    if (rc == PHR_OK) batch_launch(b, b->gcams, spp, b->path_waves, b->grouped, b->path_build);
+    if (rc == PHR_OK) batch_launch(b, b->gcams, spp, b->path_waves, b->grouped, b->path_build, b->split_shade);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (rc != PHR_OK) {

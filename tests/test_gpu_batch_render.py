@@ -83,6 +83,15 @@ def test_every_scene_renders_as_its_own_tracer(batch):
             for k in range(b.scenes):
                 assert np.array_equal(b.px(k), want[k]), (build, waves, grouped, k)
             assert b.counters()[0] > c0[0]
+    b.set_path_build(P.Tracer.PATH_SPLIT)
+    b.set_path(4, 1)
+    for eighths in (2, 6, 8):  # the split build's shading point
+        b.set_split_shade(eighths)
+        assert b.split_shade == eighths
+        b.render(cam, W, H, SPP)
+        for k in range(b.scenes):
+            assert np.array_equal(b.px(k), want[k]), ("shade", eighths, k)
+    b.set_split_shade(4)
     b.set_path_build(P.Tracer.PATH_NESTED)
     with pytest.raises(P.PhrError):
         b.set_path_build(1)

@@ -132,6 +132,18 @@ def test_path_kernel_variants_agree(gpu, orc):
             tr.sample(fresh, w, h, 2)
             compare_planes(tr.px(), px, w, h)
             tr.sample(S.path_camera(w + 1, h, rr_from=3), w + 1, h, 1)  # another view: the next call starts over
+    tr.set_path_regen(P.Tracer.PATH_SPLIT)
+    tr.set_path_waves(4)
+    for eighths in (1, 2, 4, 6, 8):  # the split build's shading point: the same sums at every one
+        tr.set_split_shade(eighths)
+        assert tr.split_shade == eighths
+        fresh = cam.copy()
+        fresh["stream"] = 0
+        tr.sample(fresh, w, h, 2)
+        compare_planes(tr.px(), px, w, h)
+        tr.sample(S.path_camera(w + 1, h, rr_from=3), w + 1, h, 1)
+    with pytest.raises(P.PhrError):
+        tr.set_split_shade(9)
     with pytest.raises(P.PhrError):
         tr.set_path_waves(6)
     chosen, best = tr.autotune(cam, w, h, 1, reps=1)
