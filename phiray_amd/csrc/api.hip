@@ -110,6 +110,7 @@ void phr_gpu_close(phr_gpu* g) {
         (void)hipStreamSynchronize(g->stream);
         (void)hipStreamDestroy(g->stream);
     }
+    if (g->sums) (void)hipFree(g->sums);
     delete g;
 }
 

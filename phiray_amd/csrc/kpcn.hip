@@ -1230,6 +1230,13 @@ int phr_kpcn_adam(phr_kpcn* k, float lr) {
 // timing both on b frames of w x h: the best of `reps` launches each, with HIP
 // events on the network's stream. chosen (optional) gets the rows a wave owns
 // per layer. The outputs do not depend on the tile.
+// The GPU's partial-sum buffer (HO_BLOCKS pairs of doubles), made on first use.
+static int gpu_sums(phr_gpu* g, double** out) {
+    if (!g->sums) PHR_HIP(hipMalloc((void**)&g->sums, 16ull * HO_BLOCKS));
+    *out = g->sums;
+    return PHR_OK;
+}
+
 int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_array* frame_b, uint64_t n, double* err,
                       double* noise) {
     PHR_CHECK(g && denoised && frame_a && frame_b && err && n, PHR_ERR_ARG, "phr_kpcn_held_out: null");
@@ -1237,7 +1244,7 @@ int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_a
                   frame_b->len * frame_b->elem >= n * FRAME * 4,
               PHR_ERR_SIZE, "phr_kpcn_held_out: arrays smaller than n pixels");
     double* part = nullptr;
-    PHR_HIP(hipMalloc((void**)&part, 16ull * HO_BLOCKS));
+    if (int rc = gpu_sums(g, &part); rc != PHR_OK) return rc;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(HO_BLOCKS, (n + 255) / 256);
     hipLaunchKernelGGL(k_held_out, dim3(blocks), dim3(256), 0, g->stream, (const float*)denoised->ptr, (const float*)frame_a->ptr,
                        (const float*)frame_b->ptr, n, part);
@@ -1245,7 +1252,6 @@ int phr_kpcn_held_out(phr_gpu* g, phr_array* denoised, phr_array* frame_a, phr_a
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 16ull * blocks, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
-    (void)hipFree(part);
     if (e != hipSuccess) {
         phr_set_error("phr_kpcn_held_out", e);
         return PHR_ERR_HIP;
@@ -1261,14 +1267,13 @@ int phr_kpcn_spread(phr_gpu* g, phr_array* a, phr_array* b, uint64_t n, double* 
     PHR_CHECK(g && a && b && out && n, PHR_ERR_ARG, "phr_kpcn_spread: null");
     PHR_CHECK(a->len * a->elem >= n * 12 && b->len * b->elem >= n * 12, PHR_ERR_SIZE, "phr_kpcn_spread: arrays smaller than n pixels");
     double* part = nullptr;
-    PHR_HIP(hipMalloc((void**)&part, 8ull * HO_BLOCKS));
+    if (int rc = gpu_sums(g, &part); rc != PHR_OK) return rc;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(HO_BLOCKS, (3 * n + 255) / 256);
     hipLaunchKernelGGL(k_spread, dim3(blocks), dim3(256), 0, g->stream, (const float*)a->ptr, (const float*)b->ptr, n, part);
     std::vector<double> h(blocks);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, 8ull * blocks, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
-    (void)hipFree(part);
     if (e != hipSuccess) {
         phr_set_error("phr_kpcn_spread", e);
         return PHR_ERR_HIP;

@@ -13,6 +13,7 @@ struct phr_gpu {
     uint64_t max_alloc = 0;
     char name[256] = {0};
     bool shared = false;  // the process's device (Gpu::shared): close leaves it open
+    double* sums = nullptr;  // per-block partial sums of the reductions read back to the host (made on first use)
 };
 
 // A device buffer grown by doubling (src/gpu/array.rs:7-56): `len` items in
