@@ -234,8 +234,9 @@ class LearnedFrame:
     taken as E - S / 4. More samples are taken while that is over `bound`;
     once `max_spp` samples would be passed the plain render of every sample
     taken is the answer. A failed check takes as many more samples as its
-    estimate asks for, the error falling as 1 / samples, at least one and at
-    most as many as were taken."""
+    estimate asks for, the error falling as 1 / samples: at least one, at
+    most eight times as many as were taken, and never past the budget (the
+    last check before a fallback is at `max_spp`)."""
 
     def __init__(self, gpu: Gpu, tris: np.ndarray, net: Kpcn, leaf: int = 4):
         self.gpu, self.net = gpu, net
@@ -260,9 +261,10 @@ class LearnedFrame:
         """Render, check and denoise; the image is left in `self.out`
         (3 floats a pixel). Returns what was done: samples taken, each
         check's estimates (half A's, and the answer's), whether the plain
-        render was the answer. Past the first check each half goes to
-        min(2 t, max(t + 1, ceil(t est / bound))) samples from t, est the
-        answer's estimate."""
+        render was the answer. Past a failed check each half goes from t to
+        min(8 t, max(t + 1, ceil(t est / bound)), max_spp / 2) samples, est
+        the answer's estimate; with no room left the plain render is the
+        answer."""
         self._buffers(w, h)
         n = w * h
         half = max(spp // 2, 1)
@@ -284,8 +286,8 @@ class LearnedFrame:
                 Kpcn.frame_mean(self.gpu, self.fa, self.fb, n, self.fm)
                 self.net.infer(self.fm, 1, w, h, self.out)
                 return {"spp": 2 * taken, "checks": checks, "plain": False}
-            nxt = min(2 * taken, max(taken + 1, math.ceil(taken * est / bound)))
-            if 2 * nxt > max_spp:
+            nxt = min(8 * taken, max(taken + 1, math.ceil(taken * est / bound)), max_spp // 2)
+            if nxt <= taken:  # the budget is spent
                 break
             self.a.sample(ca, w, h, nxt - taken)  # the same view: the halves go on accumulating
             self.b.sample(cb, w, h, nxt - taken)
