@@ -174,17 +174,8 @@ __device__ __forceinline__ bool visit1(const float4* __restrict__ r, V3 o, V3 in
     if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); return true; }
     return false;
 }
-// visit2_rows: the same visit from rows already loaded (the record's four,
-// then its eight kids rows).
-__device__ __forceinline__ bool visit2_rows(float4 alo, float4 ahi, float4 blo, float4 bhi, float4 a0lo, float4 a0hi, float4 a1lo,
-                                            float4 a1hi, float4 b0lo, float4 b0hi, float4 b1lo, float4 b1hi, V3 o, V3 inv, float best_t,
-                                            uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp,
-                                            uint32_t& steps);
-__device__ __forceinline__ bool visit2(const float4* __restrict__ r, const float4* __restrict__ k, V3 o, V3 inv, float best_t,
-                                       uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp, uint32_t& steps) {
-    return visit2_rows(r[0], r[1], r[2], r[3], k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], o, inv, best_t, cur, end, stk, stride, sp,
-                       steps);
-}
+// visit2_rows: the visit from rows already loaded (the record's four, then
+// its eight kids rows); visit2 loads them from the record.
 __device__ __forceinline__ bool visit2_rows(float4 alo, float4 ahi, float4 blo, float4 bhi, float4 a0lo, float4 a0hi, float4 a1lo,
                                             float4 a1hi, float4 b0lo, float4 b0hi, float4 b1lo, float4 b1hi, V3 o, V3 inv, float best_t,
                                             uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp,
@@ -231,6 +222,11 @@ __device__ __forceinline__ bool visit2_rows(float4 alo, float4 ahi, float4 blo, 
     if (tl < PHR_INF) { cur = __float_as_uint(llo.w); end = __float_as_uint(lhi.w); return true; }
     if (tr < PHR_INF) { cur = __float_as_uint(qlo.w); end = __float_as_uint(qhi.w); return true; }
     return false;
+}
+__device__ __forceinline__ bool visit2(const float4* __restrict__ r, const float4* __restrict__ k, V3 o, V3 inv, float best_t,
+                                       uint32_t& cur, uint32_t& end, uint32_t* stk, uint32_t stride, uint32_t& sp, uint32_t& steps) {
+    return visit2_rows(r[0], r[1], r[2], r[3], k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], o, inv, best_t, cur, end, stk, stride, sp,
+                       steps);
 }
 
 // TWO: a visit takes two levels from the record (visit2), else one (visit1);
