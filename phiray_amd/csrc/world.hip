@@ -53,6 +53,7 @@ namespace {
 
 constexpr uint32_t MAX_BODIES = 2048;    // the bodies' state in LDS: 128 KB
 constexpr uint32_t LPUSH_BODIES = 1280;  // bodies whose state and pushed-out offsets fit LDS together (120 KB)
+constexpr uint32_t DATAFLOW_ROUNDS = 1u << 24;  // a bound far past any wait of the dataflow solve
 constexpr uint32_t PAIR_SLOTS = 32;      // pairs a body may own
 constexpr uint32_t MAX_COLOURS = 1024;
 constexpr uint32_t SOLVE_THREADS = 512;
@@ -1209,15 +1210,17 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
                                                          const PairRec* __restrict__ pairs, PointRec* __restrict__ points,
                                                          uint32_t* __restrict__ live, uint32_t* __restrict__ order,
                                                          PairRec* __restrict__ spairs, PointRec* __restrict__ spoints,
-                                                         float4* __restrict__ prev, float4* __restrict__ gpush,
+                                                         uint32_t* __restrict__ rank, float4* __restrict__ prev, float4* __restrict__ gpush,
                                                          float* __restrict__ poses, uint32_t* __restrict__ stats, Params P) {
     __shared__ float4 s_body[4 * NB];
     __shared__ uint32_t s_flags[NB];
     __shared__ uint32_t s_claim[NB];
     __shared__ float4 s_push[LPUSH ? 2 * NB : 1];
     __shared__ uint32_t s_cstart[MAX_COLOURS + 1];
-    constexpr uint32_t LIVE = LPUSH ? 5632u : 1792u;  // live pairs whose bodies the LDS left over holds
+    constexpr uint32_t LIVE = LPUSH ? 5120u : 1792u;  // live pairs whose bodies the LDS left over holds
     __shared__ uint32_t s_ab[LIVE];
+    __shared__ uint32_t s_deg[LPUSH ? NB : 1];  // live pairs at each moving body (the dataflow order)
+    __shared__ uint32_t s_err;
     float4* push = LPUSH ? s_push : gpush;
     __shared__ uint32_t s_nlive, s_cursor, s_colours, s_contacts, s_pairs;
     const uint32_t tid = threadIdx.x, T = SOLVE_THREADS, n = P.n;
@@ -1300,6 +1303,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
         b = pr.b;
         return true;
     };
+    if constexpr (LPUSH) {
+        for (uint32_t i = tid; i < n; i += T) s_deg[i] = 0u;
+        if (tid == 0) s_err = 0u;
+    }
     for (uint32_t round = 0; round < nlive; ++round) {
         if (s_cursor == nlive || s_colours == MAX_COLOURS) break;
         for (uint32_t i = tid; i < n; i += T) s_claim[i] = NONE;
@@ -1318,9 +1325,18 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
             const uint32_t key = (a << 11) | b;
             const bool ok = (!dynamic_f(s_flags[a]) || s_claim[a] == key) && (!dynamic_f(s_flags[b]) || s_claim[b] == key);
             if (ok) {
-                order[atomicAdd(&s_cursor, 1u)] = live[e];
+                const uint32_t at = atomicAdd(&s_cursor, 1u);
+                order[at] = live[e];
                 if (cached) s_ab[e] = NONE;
                 else live[e] = NONE;
+                if constexpr (LPUSH) {
+                    // The pair's place among its moving bodies' pairs (one
+                    // pair a body a round: the colour order is their order).
+                    uint32_t ra = 0, rb = 0;
+                    if (dynamic_f(s_flags[a])) { ra = s_deg[a]; s_deg[a] = ra + 1u; }
+                    if (dynamic_f(s_flags[b])) { rb = s_deg[b]; s_deg[b] = rb + 1u; }
+                    rank[at] = ra | (rb << 16);
+                }
             }
         }
         __syncthreads();
@@ -1376,7 +1392,62 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(Body* __restrict__ bodi
             L.set_q(i, qstep(q, L.w(i), h));
         }
         __syncthreads();
-        if constexpr (QUAD) {
+        if constexpr (QUAD && LPUSH) {
+            // Dataflow order, a wave at a time: a wave takes its groups'
+            // entries of each colour (g, g + T/4, ... for its 16 groups of
+            // four lanes), iteration after iteration, and projects them
+            // together once every earlier projection at their moving bodies
+            // is done (counted a body in LDS: iteration it, rank r at a body
+            // of d live pairs waits for it d + r). Each body sees its pairs
+            // in the colour order's sequence, as with a barrier between
+            // colours, but a wave goes on as soon as its own pairs are
+            // ready. The earliest pending entry is always at its wave's head
+            // and ready, so the loop ends; the round bound only turns a
+            // fault in that reasoning into an error.
+            for (uint32_t i = tid; i < n; i += T) s_claim[i] = 0u;
+            __syncthreads();
+            const uint32_t g = tid / 4, lane = tid & 3u, wv = tid >> 6;
+            bool failed = false;
+            for (uint32_t it = 0; it < P.iterations && !failed; ++it)
+                for (uint32_t c = 0; c < ncol && !failed; ++c)
+                    for (uint32_t base = s_cstart[c]; base + 16u * wv < s_cstart[c + 1]; base += T / 4) {
+                        const uint32_t e = base + g;
+                        const bool has = e < s_cstart[c + 1];
+                        PairPre cur;
+                        uint32_t rk = 0;
+                        if (has) {
+                            pair_pre(cur, spairs, spoints, prev, kc, e, lane);
+                            rk = rank[e];
+                        }
+                        const uint32_t a = has ? cur.pr.a : 0u, b = has ? cur.pr.b : 0u;
+                        const bool da = has && dynamic_f(s_flags[a]), db = has && dynamic_f(s_flags[b]);
+                        const uint32_t wa = it * (da ? s_deg[a] : 0u) + (rk & 0xFFFFu), wb = it * (db ? s_deg[b] : 0u) + (rk >> 16);
+                        for (uint32_t spins = 0;; ++spins) {
+                            const bool ok = (!da || __hip_atomic_load(&s_claim[a], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == wa) &&
+                                            (!db || __hip_atomic_load(&s_claim[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == wb);
+                            if (__ballot(!ok) == 0ull) break;
+                            if (spins >= DATAFLOW_ROUNDS) {
+                                failed = true;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        if (failed) {
+                            s_err = 1u;
+                            break;
+                        }
+                        if (has) project_quad(L, cur, s_flags, push, spoints + 4 * (size_t)e, P.cap, lane);
+                        if (has && lane == 0u) {
+                            if (da) __hip_atomic_fetch_add(&s_claim[a], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (db) __hip_atomic_fetch_add(&s_claim[b], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+            __syncthreads();
+            if (s_err) {  // report, change nothing more
+                if (tid == 0) { stats[0] = s_pairs; stats[1] = s_contacts; stats[2] = ncol; stats[3] = 3u; }
+                return;
+            }
+        } else if constexpr (QUAD) {
             // A group of four lanes takes the colour's entries g, g + T/4, ...;
             // its first entry of the next colour is read while this one is
             // solved (the next iteration's colour 0 after the last colour).
@@ -1560,6 +1631,7 @@ struct phr_world {
     PairRec* d_pairs = nullptr;
     PointRec* d_points = nullptr;
     PairRec* d_spairs = nullptr;    // the pairs in colour order
+    uint32_t* d_rank = nullptr;     // their places among their moving bodies' pairs
     PointRec* d_spoints = nullptr;  // their points
     uint32_t* d_live = nullptr;
     uint32_t* d_order = nullptr;
@@ -1578,12 +1650,13 @@ namespace {
 void world_release_device(phr_world* w) {
     for (void* p : {(void*)w->d_bodies, (void*)w->d_konst, (void*)w->d_aux, (void*)w->d_shapes, (void*)w->d_count,
                     (void*)w->d_other, (void*)w->d_pairs, (void*)w->d_points, (void*)w->d_spairs, (void*)w->d_spoints,
+                    (void*)w->d_rank,
                     (void*)w->d_live, (void*)w->d_order,
                     (void*)w->d_prev, (void*)w->d_push, (void*)w->d_poses, (void*)w->d_stats})
         if (p) (void)hipFree(p);
     w->d_bodies = nullptr; w->d_konst = nullptr; w->d_aux = nullptr; w->d_shapes = nullptr; w->d_count = nullptr;
     w->d_other = nullptr; w->d_pairs = nullptr; w->d_points = nullptr; w->d_live = nullptr; w->d_order = nullptr;
-    w->d_spairs = nullptr; w->d_spoints = nullptr;
+    w->d_spairs = nullptr; w->d_spoints = nullptr; w->d_rank = nullptr;
     w->d_prev = nullptr; w->d_push = nullptr; w->d_poses = nullptr; w->d_stats = nullptr;
     phr_array_release(w->d_boxes);
     w->d_boxes = nullptr;
@@ -1619,6 +1692,7 @@ int world_push(phr_world* w) {
                         dev((void**)&w->d_count, 4ull * cap) && dev((void**)&w->d_other, 4 * slots) &&
                         dev((void**)&w->d_pairs, sizeof(PairRec) * slots) && dev((void**)&w->d_points, sizeof(PointRec) * 4 * slots) &&
                         dev((void**)&w->d_spairs, sizeof(PairRec) * slots) && dev((void**)&w->d_spoints, sizeof(PointRec) * 4 * slots) &&
+                        dev((void**)&w->d_rank, 4 * slots) &&
                         dev((void**)&w->d_live, 4 * slots) && dev((void**)&w->d_order, 4 * slots) &&
                         dev((void**)&w->d_prev, 32ull * cap) && dev((void**)&w->d_push, 32ull * cap) &&
                         dev((void**)&w->d_poses, 64ull * cap) && dev((void**)&w->d_stats, 32);
@@ -1799,8 +1873,8 @@ int phr_world_step(phr_world* w, float frame_dt) {
     // (status 1); more colours than it can hold, too (status 2).
     auto solve = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(1), dim3(SOLVE_THREADS), 0, s, w->d_bodies, w->d_konst, w->d_aux, w->d_count, w->d_pairs,
-                           w->d_points, w->d_live, w->d_order, w->d_spairs, w->d_spoints, w->d_prev, w->d_push, w->d_poses,
-                           w->d_stats, P);
+                           w->d_points, w->d_live, w->d_order, w->d_spairs, w->d_spoints, w->d_rank, w->d_prev, w->d_push,
+                           w->d_poses, w->d_stats, P);
     };
     if (w->quad && n <= LPUSH_BODIES)
         solve(k_solve<true, LPUSH_BODIES, true>);
@@ -1813,7 +1887,8 @@ int phr_world_step(phr_world* w, float frame_dt) {
     PHR_HIP(hipStreamSynchronize(s));
     w->host_stale = true;
     PHR_CHECK(w->stats[3] != 1u, PHR_ERR_SIZE, "phr_world_step: a body meets more than 32 others");
-    PHR_CHECK(w->stats[3] == 0u, PHR_ERR_SIZE, "phr_world_step: the pairs need more than 1,024 colours");
+    PHR_CHECK(w->stats[3] != 2u, PHR_ERR_SIZE, "phr_world_step: the pairs need more than 1,024 colours");
+    PHR_CHECK(w->stats[3] == 0u, PHR_ERR_SIZE, "phr_world_step: the solve's dataflow order did not finish");
     return PHR_OK;
 }
 
