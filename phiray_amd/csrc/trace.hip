@@ -1825,7 +1825,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
     st.thr = v3(1, 1, 1);
     st.p_diffuse = 1.0f;
     st.min_pdf = 3.0e38f;
-    RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
+    RgGuide g0;  // what rg_start clears of a sample's guide (the first segment's rg_hit sets it anew)
     SplitWalk<F> sw;
     auto& w = sw.w;
     w.on = w.any = false;
@@ -1833,7 +1833,7 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
     V3 pa = v3(0, 0, 0), pb = pa;  // what a shadow walk's answer picks between
     uint32_t s = 0;
     if (!done) {
-        rg_start(st, g, p, i, frame0);
+        rg_start(st, g0, p, i, frame0);
         sw.begin(sc, st.o, st.d, PHR_INF, false);
         ++closest;
     }
@@ -1858,7 +1858,20 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
         } else {  // the closest hit: the segment's shading
             const PhrHit h = sw.hit();
             RgTurn q;
+            // A sample's guide is whole once its first camera segment is
+            // shaded; it joins the sums then, in sample order as at the
+            // sample's end, and is not carried through the walks.
+            const bool first = st.phase == 1u && st.pass == 0u;
+            RgGuide g{v3(0, 0, 0), v3(0, 0, 0), 0.f, 0.f, 0.f, 0u};
             rg_hit<F>(st, g, q, sc, p, k, h);
+            if (first) {  // shade's `out`, summed as path_block sums it
+                ga = ga + g.alb;
+                gn = gn + g.nrm;
+                gd = gd + g.dep;
+                gone = gone + g.missed;
+                cov = cov + g.cov;
+                outline = g.outline;
+            }
             bool ended;
             if (q.qon) {
                 RgPost lit{st.col, st.cc, st.acc, st.phase}, dark = lit;
@@ -1883,20 +1896,14 @@ __device__ __forceinline__ void path_split(const SceneDev& sc, const PhrCamera* 
                 if (ended) l = l + st.col;
                 next = true;
             }
-            if (ended) {  // shade's `out`, summed as path_block sums it
-                ga = ga + g.alb;
-                gn = gn + g.nrm;
-                gd = gd + g.dep;
-                gone = gone + g.missed;
-                cov = cov + g.cov;
-                outline = g.outline;
+            if (ended) {
                 ++s;
                 st.phase = 0u;
             }
         }
         if (next) {
             if (s < spp) {
-                if (st.phase == 0u) rg_start(st, g, p, i, frame0 + s);
+                if (st.phase == 0u) rg_start(st, g0, p, i, frame0 + s);
                 sw.begin(sc, st.o, st.d, PHR_INF, false);
                 ++closest;
             } else {
