@@ -813,6 +813,12 @@ struct Ctx {
     uint32_t* stk;
     uint16_t* tstk;  // a forest's top-tree stack
     uint32_t closest, shadow, steps, tests;
+    // The pixel's guide sums (albedo, normal, depth, missed, coverage) and the
+    // last sample's outline word: a sample adds its guide at its first
+    // camera segment, where the guide is whole.
+    V3 ga, gn;
+    float gd, gone, cov;
+    uint32_t outline;
 };
 
 // F: the scene is a forest (top tree + instances' trees); else one tree under
@@ -1133,21 +1139,17 @@ __device__ V3 indirect(Ctx& c, V3 pos, V3 n, uint32_t* seed, bool bounce) {  // 
     return acc;
 }
 
-// One camera sample of pixel i (shade.wgsl:41-373). Returns what `guide` and
-// `emit` add to the pixel's sums.
+// One camera sample of pixel i (shade.wgsl:41-373). Returns what `emit` adds
+// to the pixel's radiance; what `guide` adds goes into c's guide sums at the
+// first camera segment, in sample order as at the sample's end.
 struct Sample {
-    V3 lin, albedo, normal;
-    float depth, missed, cov;
-    uint32_t outline;
+    V3 lin;
 };
 
 template <bool F>
 __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
     const PhrCamera* p = c.p;
     Sample out;
-    out.missed = 0.f;
-    out.cov = 0.f;
-    out.outline = 0u;
     float fw = (float)p->w, fh = (float)p->h;
     float su = ((float)(i % p->w) + 0.5f) / fw - 0.5f;
     float sv = ((float)(i / p->w) + 0.5f) / fh - 0.5f;
@@ -1178,18 +1180,18 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
         PhrHit h = trace<F>(c, o, d, PHR_INF, false);
         if (h.tri == PHR_NONE) {
             if (bounce == 0u) {
-                out.albedo = v3(0, 0, 0);
-                out.normal = -d;
-                out.depth = -1.0f;
+                c.ga = c.ga + v3(0, 0, 0);
+                c.gn = c.gn + (-d);
+                c.gd = c.gd + -1.0f;
+                c.gone = c.gone + 1.0f;
+                c.cov = c.cov + 1.0f;
+                c.outline = 0u;
                 out.lin = v3(0, 0, 0);
-                out.missed = 1.0f;
-                out.cov = 1.0f;
                 return out;
             }
             col = col + thr * env_rgb(p);
             break;
         }
-        if (bounce == 0u) out.outline = h.inst + 1u;
         uint32_t m = mat_at(c.s, h.tri);
         Tri t = tri_of<F>(c.s, h.tri, h.inst, m);
         float w = 1.0f - h.u - h.v;
@@ -1210,9 +1212,12 @@ __device__ Sample shade(Ctx& c, uint32_t i, uint32_t frame) {
         V3 base = vmax(v3(t.color[0], t.color[1], t.color[2]), 0.0f);
         float alpha = t.color[3];
         if (bounce == 0u) {
-            out.albedo = base;
-            out.normal = sn;
-            out.depth = h.t;
+            c.ga = c.ga + base;
+            c.gn = c.gn + sn;
+            c.gd = c.gd + h.t;
+            c.gone = c.gone + 0.f;
+            c.cov = c.cov + 0.f;
+            c.outline = h.inst + 1u;
         }
         if (rnd(&seed) >= alpha) {
             n_see++;
@@ -1298,27 +1303,24 @@ __device__ __forceinline__ void path_block(const SceneDev& sc, const PhrCamera* 
     if (inside) {
         bool keep = frame0 > 0u;
         V3 l = keep ? v3(ldf(px, n + 3 * i), ldf(px, n + 3 * i + 1), ldf(px, n + 3 * i + 2)) : v3(0, 0, 0);
-        V3 ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
-        V3 gn = keep ? v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2)) : v3(0, 0, 0);
-        float gd = keep ? ldf(px, 13 * n + i) : 0.f;
-        float cov = keep ? ldf(px, 23 * n + i) : 0.f;
-        float gone = keep ? ldf(px, 33 * n + i) : 0.f;
+        c.ga = keep ? v3(ldf(px, 7 * n + 3 * i), ldf(px, 7 * n + 3 * i + 1), ldf(px, 7 * n + 3 * i + 2)) : v3(0, 0, 0);
+        c.gn = keep ? v3(ldf(px, 10 * n + 3 * i), ldf(px, 10 * n + 3 * i + 1), ldf(px, 10 * n + 3 * i + 2)) : v3(0, 0, 0);
+        c.gd = keep ? ldf(px, 13 * n + i) : 0.f;
+        c.cov = keep ? ldf(px, 23 * n + i) : 0.f;
+        c.gone = keep ? ldf(px, 33 * n + i) : 0.f;
         // The half image (util.wgsl:126-132): odd samples at double weight; a
         // new image starts it at zero.
         V3 ax = (adapt && keep) ? v3(ldf(px, 26 * n + 3 * i), ldf(px, 26 * n + 3 * i + 1), ldf(px, 26 * n + 3 * i + 2))
                                 : v3(0, 0, 0);
-        uint32_t outline = 0;
+        c.outline = 0;
         for (uint32_t s = 0; s < spp; ++s) {
             Sample sm = shade<F>(c, i, frame0 + s);
             if (adapt && ((frame0 + s) & 1u) == 1u) ax = ax + sm.lin * 2.0f;
-            ga = ga + sm.albedo;
-            gn = gn + sm.normal;
-            gd = gd + sm.depth;
-            gone = gone + sm.missed;
             l = l + sm.lin;
-            cov = cov + sm.cov;
-            outline = sm.outline;
         }
+        const V3 ga = c.ga, gn = c.gn;
+        const float gd = c.gd, cov = c.cov, gone = c.gone;
+        const uint32_t outline = c.outline;
         // The grid plane, the film weight and the count take the same
         // per-sample additions as the sums above (+0, +1.0, +1), done after
         // the samples so they hold no registers across them.
