@@ -578,7 +578,10 @@ int phr_world_configure(phr_world* w, uint32_t substeps, uint32_t iterations, fl
                         float sleep_after, const float gravity[3]);
 int phr_world_add_shape(phr_world* w, const phr_shape* s, uint32_t* id);
 int phr_world_add_body(phr_world* w, const phr_body_def* d, uint32_t* id);
-/* World::step (world.rs:104-129): one frame of frame_dt seconds. */
+/* World::step (world.rs:104-129): one frame of frame_dt seconds. PHR_ERR_SIZE
+ * when the frame's contact pairs need more than 1,024 colours, or when the
+ * solve's dataflow order (up to 1,280 bodies) does not finish within its bound
+ * of waits; the bodies then keep the state of the frame's start. */
 int phr_world_step(phr_world* w, float frame_dt);
 uint32_t phr_world_body_count(const phr_world* w);
 uint32_t phr_world_contact_count(const phr_world* w);
