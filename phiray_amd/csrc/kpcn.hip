@@ -1226,10 +1226,6 @@ int phr_kpcn_adam(phr_kpcn* k, float lr) {
     return pack_weights(k);
 }
 
-// Pick each layer's forward conv tile (4 or 8 rows of 32 pixels a block) by
-// timing both on b frames of w x h: the best of `reps` launches each, with HIP
-// events on the network's stream. chosen (optional) gets the rows a wave owns
-// per layer. The outputs do not depend on the tile.
 // The GPU's partial-sum buffer (HO_BLOCKS pairs of doubles), made on first use.
 static int gpu_sums(phr_gpu* g, double** out) {
     if (!g->sums) PHR_HIP(hipMalloc((void**)&g->sums, 16ull * HO_BLOCKS));
@@ -1296,6 +1292,10 @@ int phr_kpcn_frame_mean(phr_gpu* g, phr_array* frame_a, phr_array* frame_b, uint
     return PHR_OK;
 }
 
+// Pick each layer's forward conv tile (4 or 8 rows of 32 pixels a block) by
+// timing both on b frames of w x h: the best of `reps` launches each, with HIP
+// events on the network's stream. chosen (optional) gets the rows a wave owns
+// per layer. The outputs do not depend on the tile.
 int phr_kpcn_autotune(phr_kpcn* k, phr_array* frames, uint32_t b, uint32_t w, uint32_t h, uint32_t reps, uint8_t* chosen) {
     PHR_CHECK(k, PHR_ERR_ARG, "phr_kpcn_autotune: null");
     KP_TRY(check_shape(frames, b, w, h, nullptr, "phr_kpcn_autotune"));
