@@ -260,7 +260,8 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ nodes,
 // body with its stack and its pair run in LDS too. Every visit is an LDS
 // read instead of an L2 round trip. For worlds of at most LDS_PAIR_BODIES.
 constexpr uint32_t LDS_PAIR_BODIES = 1280;
-__global__ __launch_bounds__(64) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
+constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree; its first 64 walk
+__global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
                                                   const float4* __restrict__ boxes, const Body* __restrict__ bodies,
                                                   const Konst* __restrict__ konst, uint32_t n, uint32_t* __restrict__ count,
                                                   uint32_t* __restrict__ other, uint32_t* __restrict__ overflow) {
@@ -271,26 +272,28 @@ __global__ __launch_bounds__(64) void k_pairs_lds(const float4* __restrict__ nod
     __shared__ uint32_t s_mine[PAIR_SLOTS * 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t nn = 2 * n - 1;
-    // Eight loads in flight a lane: one wave copies ~110 KB.
+    // Eight loads in flight a lane: four waves copy ~110 KB; the first walks.
+    constexpr uint32_t T = PAIR_STAGE_THREADS;
     auto stage = [&](float4* dst, const float4* src, uint32_t len) {
-        for (uint32_t k = tid; k < len; k += 64 * 8) {
+        for (uint32_t k = tid; k < len; k += T * 8) {
             float4 v[8];
 #pragma unroll
             for (uint32_t j = 0; j < 8; ++j)
-                if (k + 64 * j < len) v[j] = src[k + 64 * j];
+                if (k + T * j < len) v[j] = src[k + T * j];
 #pragma unroll
             for (uint32_t j = 0; j < 8; ++j)
-                if (k + 64 * j < len) dst[k + 64 * j] = v[j];
+                if (k + T * j < len) dst[k + T * j] = v[j];
         }
     };
     stage(s_nodes, nodes, 2 * nn);
     stage(s_boxes, boxes, 2 * n);
-    for (uint32_t k = tid; k < n; k += 64) {
+    for (uint32_t k = tid; k < n; k += T) {
         s_order[k] = order[k];
         s_flags[k] = bodies[k].flags;
         s_groups[k] = konst[k].groups;
     }
     __syncthreads();
+    if (tid >= 64) return;
     const uint32_t i = blockIdx.x * 64 + tid;
     if (i >= n) return;
     const uint32_t fi = s_flags[i];
@@ -1847,7 +1850,7 @@ int phr_world_step(phr_world* w, float frame_dt) {
     PHR_HIP(hipMemsetAsync(overflow, 0, 4, s));
     PHR_CHECK(phr_bvh_internal_nodes(w->tree)->len >= 2ull * n - 1, PHR_ERR_SIZE, "phr_world_step: the broadphase tree is short");
     if (n <= LDS_PAIR_BODIES)
-        hipLaunchKernelGGL(k_pairs_lds, dim3((n + 63) / 64), dim3(64), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
+        hipLaunchKernelGGL(k_pairs_lds, dim3((n + 63) / 64), dim3(PAIR_STAGE_THREADS), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
                            (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->d_boxes->ptr, w->d_bodies,
                            w->d_konst, n, w->d_count, w->d_other, overflow);
     else
