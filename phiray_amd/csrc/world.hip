@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ nodes,
 // body with its stack and its pair run in LDS too. Every visit is an LDS
 // read instead of an L2 round trip. For worlds of at most LDS_PAIR_BODIES.
 constexpr uint32_t LDS_PAIR_BODIES = 1280;
-constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree; its first 64 walk
+constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree; 16 lanes a wave walk
 __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
                                                   const float4* __restrict__ boxes, const Body* __restrict__ bodies,
                                                   const Konst* __restrict__ konst, uint32_t n, uint32_t* __restrict__ count,
@@ -272,7 +272,8 @@ __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* 
     __shared__ uint32_t s_mine[PAIR_SLOTS * 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t nn = 2 * n - 1;
-    // Eight loads in flight a lane: four waves copy ~110 KB; the first walks.
+    // Eight loads in flight a lane: four waves copy ~110 KB, then walk 16
+    // bodies each (a wave walks as long as its longest walk).
     constexpr uint32_t T = PAIR_STAGE_THREADS;
     auto stage = [&](float4* dst, const float4* src, uint32_t len) {
         for (uint32_t k = tid; k < len; k += T * 8) {
@@ -293,12 +294,13 @@ __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* 
         s_groups[k] = konst[k].groups;
     }
     __syncthreads();
-    if (tid >= 64) return;
-    const uint32_t i = blockIdx.x * 64 + tid;
+    if ((tid & 63u) >= 16u) return;
+    const uint32_t slot = (tid >> 6) * 16u + (tid & 63u);  // the body's column in s_stack and s_mine
+    const uint32_t i = blockIdx.x * 64 + slot;
     if (i >= n) return;
     const uint32_t fi = s_flags[i];
     uint32_t found = 0;
-    uint32_t* mine = s_mine + tid;  // slot k at mine[64 k]
+    uint32_t* mine = s_mine + slot;  // slot k at mine[64 k]
     if (moves_f(fi)) {
         const float4 blo = s_boxes[2 * i], bhi = s_boxes[2 * i + 1];
         const uint32_t gi = s_groups[i];
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* 
         auto meets = [&](float4 lo, float4 hi) {
             return lo.x <= bhi.x && hi.x >= blo.x && lo.y <= bhi.y && hi.y >= blo.y && lo.z <= bhi.z && hi.z >= blo.z;
         };
-        uint32_t* stack = s_stack + tid;  // entry k at stack[64 k]
+        uint32_t* stack = s_stack + slot;  // entry k at stack[64 k]
         int sp = 0;
         stack[0] = 0;
         sp = 1;
