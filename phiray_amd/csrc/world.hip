@@ -260,7 +260,8 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ nodes,
 // body with its stack and its pair run in LDS too. Every visit is an LDS
 // read instead of an L2 round trip. For worlds of at most LDS_PAIR_BODIES.
 constexpr uint32_t LDS_PAIR_BODIES = 1280;
-constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree; 16 lanes a wave walk
+constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree
+constexpr uint32_t PAIR_BLOCK_BODIES = 32;    // bodies a block walks, PAIR_BLOCK_BODIES / 4 lanes of each wave
 __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
                                                   const float4* __restrict__ boxes, const Body* __restrict__ bodies,
                                                   const Konst* __restrict__ konst, uint32_t n, uint32_t* __restrict__ count,
@@ -272,8 +273,9 @@ __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* 
     __shared__ uint32_t s_mine[PAIR_SLOTS * 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t nn = 2 * n - 1;
-    // Eight loads in flight a lane: four waves copy ~110 KB, then walk 16
-    // bodies each (a wave walks as long as its longest walk).
+    // Eight loads in flight a lane: four waves copy ~110 KB, then walk a
+    // quarter of the block's bodies each (a wave walks as long as its
+    // longest walk).
     constexpr uint32_t T = PAIR_STAGE_THREADS;
     auto stage = [&](float4* dst, const float4* src, uint32_t len) {
         for (uint32_t k = tid; k < len; k += T * 8) {
@@ -294,9 +296,10 @@ __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* 
         s_groups[k] = konst[k].groups;
     }
     __syncthreads();
-    if ((tid & 63u) >= 16u) return;
-    const uint32_t slot = (tid >> 6) * 16u + (tid & 63u);  // the body's column in s_stack and s_mine
-    const uint32_t i = blockIdx.x * 64 + slot;
+    constexpr uint32_t WL = PAIR_BLOCK_BODIES / 4;  // walking lanes a wave
+    if ((tid & 63u) >= WL) return;
+    const uint32_t slot = (tid >> 6) * WL + (tid & 63u);  // the body's column in s_stack and s_mine
+    const uint32_t i = blockIdx.x * PAIR_BLOCK_BODIES + slot;
     if (i >= n) return;
     const uint32_t fi = s_flags[i];
     uint32_t found = 0;
@@ -1852,7 +1855,7 @@ int phr_world_step(phr_world* w, float frame_dt) {
     PHR_HIP(hipMemsetAsync(overflow, 0, 4, s));
     PHR_CHECK(phr_bvh_internal_nodes(w->tree)->len >= 2ull * n - 1, PHR_ERR_SIZE, "phr_world_step: the broadphase tree is short");
     if (n <= LDS_PAIR_BODIES)
-        hipLaunchKernelGGL(k_pairs_lds, dim3((n + 63) / 64), dim3(PAIR_STAGE_THREADS), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
+        hipLaunchKernelGGL(k_pairs_lds, dim3((n + PAIR_BLOCK_BODIES - 1) / PAIR_BLOCK_BODIES), dim3(PAIR_STAGE_THREADS), 0, s, (const float4*)phr_bvh_internal_nodes(w->tree)->ptr,
                            (const uint32_t*)phr_bvh_internal_order(w->tree)->ptr, (const float4*)w->d_boxes->ptr, w->d_bodies,
                            w->d_konst, n, w->d_count, w->d_other, overflow);
     else
