@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ nodes,
 // read instead of an L2 round trip. For worlds of at most LDS_PAIR_BODIES.
 constexpr uint32_t LDS_PAIR_BODIES = 1280;
 constexpr uint32_t PAIR_STAGE_THREADS = 256;  // k_pairs_lds: threads that stage the tree
-constexpr uint32_t PAIR_BLOCK_BODIES = 16;    // bodies a block walks, PAIR_BLOCK_BODIES / 4 lanes of each wave
+constexpr uint32_t PAIR_BLOCK_BODIES = 8;     // bodies a block walks, PAIR_BLOCK_BODIES / 4 lanes of each wave
 __global__ __launch_bounds__(PAIR_STAGE_THREADS) void k_pairs_lds(const float4* __restrict__ nodes, const uint32_t* __restrict__ order,
                                                   const float4* __restrict__ boxes, const Body* __restrict__ bodies,
                                                   const Konst* __restrict__ konst, uint32_t n, uint32_t* __restrict__ count,
